@@ -1,0 +1,201 @@
+/*
+ * omni3d.h -- C-ABI of the MI355X (gfx950) omni-directional 3D pressure
+ * reconstruction library libomni3d.so.
+ *
+ * Plain C linkage: no C++ types, no exceptions, plain pointers and sizes.
+ * Every entry point returns an int status (OMNI3D_OK = 0); the message of the
+ * last failure on a context is omni3d_last_error(ctx) (or
+ * omni3d_last_global_error() for calls without a context).
+ *
+ * Reference interfaces replaced (paths relative to the reference root,
+ * Omni3D_8.0/Omni3D_8.0/):
+ *   kernel.cu:8467-8983  main()                      -> omni3d_run_parameter_file
+ *   kernel.cu:8622-8648  direction-file reader       -> omni3d_load_directions_file
+ *   kernel.cu:8805-8861  iteration driver            -> omni3d_reconstruct
+ *   kernel.cu:6151       omni3dparallellinesESInner  -> omni3d_es_inner
+ *   kernel.cu:6281       ...ESInnerWeighted          -> omni3d_es_inner_weighted
+ *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select
+ *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner
+ *   kernel.cu:8352       calCurlofMaterialAccCPU     -> omni3d_curl
+ *   sor3D.h:5            sor3D                       -> sor3D (same signature)
+ *   calRHS.h:4           calRHS                      -> calRHS (same signature; the
+ *                        result is written to the caller's RHS buffer, which
+ *                        the reference loses, calRHS.cpp:10)
+ * Additions without a reference counterpart: the adjoint pair
+ * (omni3d_forward / omni3d_backward), the GPU red-black SOR
+ * (omni3d_sor_solve) and the snapshot batch (omni3d_batch_*).
+ *
+ * Threading: a context is not thread-safe; use one per host thread.  All GPU
+ * work of a context is ordered on the stream given to it (NULL = the null
+ * stream).  Layout: x fastest, index = i + j*nx + k*nx*ny.
+ */
+#ifndef OMNI3D_H
+#define OMNI3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMNI3D_OK 0
+#define OMNI3D_ERR_ARG 1       /* invalid argument */
+#define OMNI3D_ERR_STATE 2     /* call out of order (no field / no directions) */
+#define OMNI3D_ERR_HIP 3       /* HIP runtime error (message has the HIP string) */
+#define OMNI3D_ERR_IO 4        /* file missing or malformed */
+#define OMNI3D_ERR_NONFINITE 5 /* NaN / Inf in the input field */
+#define OMNI3D_ERR_NODEVICE 6  /* no GPU visible */
+
+/* Integration method (which reference integrator a pass restates). */
+#define OMNI3D_METHOD_PLAIN 0      /* bodyIntegralInner, kernel.cu:1743 */
+#define OMNI3D_METHOD_PLAIN_MASK 1 /* Select walk with mask, unit weights, threshold +inf */
+#define OMNI3D_METHOD_SELECT 2     /* bodyIntegralInnerSelect, kernel.cu:2280 */
+#define OMNI3D_METHOD_WEIGHTED 3   /* bodyIntegralInnerWeighted, kernel.cu:1993 */
+
+/* Arithmetic of the ray sums. */
+#define OMNI3D_FP64 0        /* gradient and ray sums in FP64 */
+#define OMNI3D_FP32_KAHAN 1  /* gradient in FP32, Kahan-compensated FP32 ray sums;
+                                averaging, boundary update, residual in FP64 */
+
+/* Iteration schedule. */
+#define OMNI3D_SOLVER_AFFINE 0 /* iterate only the voxels the rays read, then one full pass */
+#define OMNI3D_SOLVER_NAIVE 1  /* a full pass per iteration (kernel.cu:8840-8861) */
+
+/* Buffer dtypes and memory kinds. */
+#define OMNI3D_F32 0
+#define OMNI3D_F64 1
+#define OMNI3D_MEM_HOST 0
+#define OMNI3D_MEM_DEVICE 1
+
+typedef struct omni3d_params {
+  int nx, ny, nz;          /* grid (Parameter_Omni3D.dat lines 1-3, after the ROI crop) */
+  double dx, dy, dz;       /* spacing (lines 4-6) */
+  double density;          /* rho (line 7) */
+  double line_spacing;     /* parallel-line spacing in grid units (line 9) */
+  int method;              /* OMNI3D_METHOD_* */
+  double threshold;        /* curl threshold of the Select method (line 15) */
+  int faithful_select;     /* 1 = keep the Select per-axis asymmetries (kernel.cu:2394 vs 2442) */
+  int precision;           /* OMNI3D_FP64 / OMNI3D_FP32_KAHAN */
+  int solver;              /* OMNI3D_SOLVER_AFFINE / OMNI3D_SOLVER_NAIVE */
+  int max_iterations;      /* NoItr (line 14) */
+  double tolerance;        /* stop when ||dp_S|| / ||p_S|| < tolerance over the voxels the
+                              rays read; 0 = always run max_iterations (reference) */
+  int seed_pn_with_p;      /* 1 = the first pass's sums start from p (kernel.cu:8813) */
+} omni3d_params;
+
+typedef struct omni3d_report {
+  int iterations;          /* passes whose result the output holds */
+  double rel_change;       /* last relative change over the source voxels */
+  int64_t rays;            /* accepted rays (all directions) */
+  int64_t steps;           /* staircase steps per full pass */
+  int64_t sources;         /* voxels whose p the rays read */
+  int status;              /* OMNI3D_OK or OMNI3D_ERR_NONFINITE */
+  double ms_plan;          /* GPU time of the geometry plan build (0 if reused) */
+  double ms_solve;         /* GPU time of the reconstruction */
+} omni3d_report;
+
+typedef struct omni3d_ctx omni3d_ctx;
+
+/* Defaults: rho 1, line spacing 1, plain FP64 affine, 10 iterations (the
+ * shipped Parameter_Omni3D.dat line 14), tolerance 0, seed_pn_with_p 1. */
+void omni3d_default_params(omni3d_params* p);
+
+int omni3d_create(const omni3d_params* params, omni3d_ctx** out);
+void omni3d_destroy(omni3d_ctx* ctx);
+const char* omni3d_last_error(const omni3d_ctx* ctx);
+const char* omni3d_last_global_error(void);
+int omni3d_set_stream(omni3d_ctx* ctx, void* hip_stream);
+
+/* Direction set (kernel.cu:8622-8672: float k1,k2,k3[NoAngles]); vectors are
+ * normalised in FP64 on the host. */
+int omni3d_set_directions(omni3d_ctx* ctx, const float* k1, const float* k2, const float* k3,
+                          int count);
+/* Reads the first `count` rows "k1 k2 k3" of a GridIcosahedronDivideN.dat file
+ * (count <= 0: the count line at the end of the file). */
+int omni3d_load_directions_file(omni3d_ctx* ctx, const char* path, int count);
+/* Pole-aligned icosahedral directions, `divide` = N of GridIcosahedronDivideN
+ * (N-1 midpoint subdivisions): 10*4^(N-1)+2 unit vectors in exact antipodal
+ * pairs.  Writes min(count, capacity) rows; returns the count (<0 on error). */
+int omni3d_generate_directions(int divide, float* k1, float* k2, float* k3, int capacity);
+
+/* Input field: material acceleration Du/Dt, Dv/Dt, Dw/Dt (grad p = -rho a,
+ * kernel.cu:1816), dtype OMNI3D_F32/F64, host or device memory.  mask may
+ * be NULL (all valid); for SELECT/WEIGHTED the curl*mask field is derived
+ * from it (kernel.cu:8824, 8352-8379).  The context keeps its own copy in
+ * the precision it computes in. */
+int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const void* dwdt,
+                     const void* mask, int dtype, int mem_kind);
+
+/* Iterate to convergence on the GPU (kernel.cu:8805-8861).  p: in = initial
+ * pressure (NULL = zeros), out = result, dtype/mem_kind as given.
+ * weight_out (may be NULL): the per-voxel weight sum of the last pass
+ * ("Count" column, kernel.cu:8929), same dtype/mem_kind.  rep may be NULL.
+ * Returns after the work is queued and, for host buffers, copied back. */
+int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, int mem_kind,
+                       omni3d_report* rep);
+
+/* Kernel-level drop-ins (device pointers, float, reference semantics with
+ * full line x direction coverage and deterministic accumulation):
+ * pn += sum over visits, pcountinner += weights (kernel.cu:6151/6281/6414). */
+int omni3d_es_inner(int X, int Y, int Z, int NoAngles, float linespacing, const float* k1_d,
+                    const float* k2_d, const float* k3_d, float deltx, float delty,
+                    float deltz, float density, const float* DuDt, const float* DvDt,
+                    const float* DwDt, const float* p, float* pn, float* pcountinner,
+                    void* hip_stream);
+int omni3d_es_inner_weighted(int X, int Y, int Z, int NoAngles, float linespacing,
+                             const float* k1_d, const float* k2_d, const float* k3_d,
+                             float deltx, float delty, float deltz, float density,
+                             const float* DuDt, const float* DvDt, const float* DwDt,
+                             const float* curl, const float* p, float* pn,
+                             float* pcountinner, void* hip_stream);
+int omni3d_es_inner_select(int X, int Y, int Z, int NoAngles, float linespacing,
+                           const float* k1_d, const float* k2_d, const float* k3_d,
+                           float deltx, float delty, float deltz, float density,
+                           const float* DuDt, const float* DvDt, const float* DwDt,
+                           const float* curl, const float* p, float* pn, float* pcountinner,
+                           float threshold, void* hip_stream);
+/* devidecountInner, kernel.cu:6746-6759: where pcountinner > 0,
+ * p = pn / pcountinner and pn = 0. */
+int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const float* pcountinner,
+                              void* hip_stream);
+
+/* |curl a| * mask with the reference's one-sided wall differences
+ * (kernel.cu:8352-8379).  Device pointers, float; mask may be NULL. */
+int omni3d_curl(int X, int Y, int Z, float deltx, float delty, float deltz, const float* DuDt,
+                const float* DvDt, const float* DwDt, const float* mask, float* curl,
+                void* hip_stream);
+
+/* Adjoint pair of the linear operator g -> p (plain or plain-mask method,
+ * zero initial pressure, the context's iteration count, no seed term).
+ * forward: grad p field (3 components, device F32 or F64 by the context's
+ * precision) -> p; backward: dL/dp -> dL/d(grad p).  Component layout:
+ * g[c*V + v], c = x,y,z. */
+int omni3d_forward(omni3d_ctx* ctx, const void* grad_d, void* p_d);
+int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d);
+
+/* GPU red-black SOR on the 7-point Neumann operator of sor3D (same 26
+ * boundary stencils), omega e.g. 1.8; iterates until the mean |residual|
+ * (sor3D.cpp:741-781) is below eps or max_iter red+black sweeps.  Host
+ * buffers, double.  *iterations_out / *residual_out may be NULL. */
+int omni3d_sor_solve(int X, int Y, int Z, double omega, int max_iter, double eps,
+                     const double* rhs, double* p, int* iterations_out, double* residual_out);
+
+/* The reference's CPU entry points, same signatures (sor3D.h:5, calRHS.h:4),
+ * executed on the GPU. */
+void sor3D(int Xsize, int Ysize, int Zsize, int NoItr, int* BCScheme, double* x, double* y,
+           double* z, double* p, double* pn, double* RHS, double eps);
+void calRHS(int Xsize, int Ysize, int Zsize, double deltx, double delty, double deltz,
+            double density, double* DuDt, double* DvDt, double* DwDt, double* RHS);
+
+/* Process-level drop-in for Omni3D_8.0.exe (kernel.cu:8467-8983): reads the
+ * 22-line Parameter_Omni3D.dat at `path`, reconstructs every file of the
+ * acceleration folder and writes PressureOmni3D_%05d.dat (+ the curl file
+ * for file 0) and log.dat.  Returns the number of files processed (<0 on
+ * error). */
+int omni3d_run_parameter_file(const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,487 @@
+// C-ABI of libomni3d: context, plan build and the reconstruction driver
+// (kernel.cu:8805-8861 restated as: plan -> pass A -> source iteration ->
+// pass B -> averaging, all queued on the context's stream).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "launch.hpp"
+
+using namespace omni3d;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+#define OMNI_TRY_HIP(ctx, expr)                                                   \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);             \
+      return OMNI3D_ERR_HIP;                                                      \
+    }                                                                             \
+  } while (0)
+
+int fail(omni3d_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  g_last_error = msg;
+  return code;
+}
+
+int64_t voxels(const omni3d_params& p) { return (int64_t)p.nx * p.ny * p.nz; }
+
+bool params_ok(const omni3d_params& p, std::string* why) {
+  if (p.nx < 2 || p.ny < 2 || p.nz < 2 || p.nx > kMaxDim || p.ny > kMaxDim || p.nz > kMaxDim) {
+    *why = "grid dimensions must be in [2, 1023]";
+    return false;
+  }
+  if (!(p.dx > 0 && p.dy > 0 && p.dz > 0)) {
+    *why = "grid spacing must be positive";
+    return false;
+  }
+  if (!(p.line_spacing > 0)) {
+    *why = "line spacing must be positive";
+    return false;
+  }
+  if (p.method < OMNI3D_METHOD_PLAIN || p.method > OMNI3D_METHOD_WEIGHTED) {
+    *why = "unknown method";
+    return false;
+  }
+  if (p.precision != OMNI3D_FP64 && p.precision != OMNI3D_FP32_KAHAN) {
+    *why = "unknown precision";
+    return false;
+  }
+  if (p.solver != OMNI3D_SOLVER_AFFINE && p.solver != OMNI3D_SOLVER_NAIVE) {
+    *why = "unknown solver";
+    return false;
+  }
+  if (p.max_iterations < 1) {
+    *why = "max_iterations must be >= 1";
+    return false;
+  }
+  return true;
+}
+
+size_t elem_size(int dtype) { return dtype == OMNI3D_F64 ? 8 : 4; }
+
+// Builds the geometry plan: ray list, visit counts, source voxels and the CSR
+// of visits that land on source voxels.
+int build_plan(omni3d_ctx* ctx) {
+  const omni3d_params& P = ctx->prm;
+  Plan& pl = ctx->plan;
+  const int D = (int)(ctx->k.size() / 3);
+  if (pl.valid && !ctx->dirs_dirty && pl.X == P.nx && pl.Y == P.ny && pl.Z == P.nz &&
+      pl.ls == P.line_spacing && pl.D == D)
+    return OMNI3D_OK;
+  pl.valid = false;
+  hipStream_t st = ctx->stream;
+  const int X = P.nx, Y = P.ny, Z = P.nz;
+  const int64_t V = (int64_t)X * Y * Z;
+  std::vector<DirConst> dc(D);
+  for (int i = 0; i < D; i++) dc[i] = make_dir_const(ctx->k[3 * i], ctx->k[3 * i + 1], ctx->k[3 * i + 2]);
+  OMNI_TRY_HIP(ctx, pl.dirs.alloc(sizeof(DirConst) * D));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.dirs.as<void>(), dc.data(), sizeof(DirConst) * D,
+                                   hipMemcpyHostToDevice, st));
+  GridDesc g;
+  g.X = X;
+  g.Y = Y;
+  g.Z = Z;
+  g.ls = P.line_spacing;
+  g.ng = lattice_size(X, Y, Z, P.line_spacing);
+  g.cc[0] = (X - 1) / 2.0;
+  g.cc[1] = (Y - 1) / 2.0;
+  g.cc[2] = (Z - 1) / 2.0;
+  const int64_t npts = (int64_t)g.ng * g.ng;
+  const int64_t ncand = npts * D;
+  const int64_t nblk = (ncand + kEnumItemsPerBlock - 1) / kEnumItemsPerBlock;
+  if (nblk > (int64_t)1 << 31) return fail(ctx, OMNI3D_ERR_ARG, "too many candidate rays");
+  DevBuf bcnt, boff;
+  OMNI_TRY_HIP(ctx, bcnt.alloc(sizeof(unsigned int) * nblk));
+  OMNI_TRY_HIP(ctx, boff.alloc(sizeof(unsigned long long) * (nblk + 1)));
+  OMNI_TRY_HIP(ctx, launch_enum_count(pl.dirs.as<DirConst>(), g, npts, ncand,
+                                      bcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(bcnt.as<unsigned int>(), (int)nblk,
+                                       boff.as<unsigned long long>(), st));
+  unsigned long long nrays = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nrays, boff.as<unsigned long long>() + nblk, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (nrays == 0) return fail(ctx, OMNI3D_ERR_ARG, "no ray crosses the grid");
+  OMNI_TRY_HIP(ctx, pl.rays.alloc(sizeof(RayRec) * nrays));
+  OMNI_TRY_HIP(ctx, launch_enum_write(pl.dirs.as<DirConst>(), g, npts, ncand,
+                                      boff.as<unsigned long long>(), pl.rays.as<RayRec>(), st));
+  // visit counts and source flags
+  DevBuf flag, steps, mx;
+  OMNI_TRY_HIP(ctx, pl.cnt.alloc(sizeof(unsigned int) * V));
+  OMNI_TRY_HIP(ctx, flag.alloc(V));
+  OMNI_TRY_HIP(ctx, steps.alloc(16));
+  OMNI_TRY_HIP(ctx, mx.alloc(16));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.cnt.as<void>(), 0, sizeof(unsigned int) * V, st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(flag.as<void>(), 0, V, st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(steps.as<void>(), 0, 16, st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(mx.as<void>(), 0, 16, st));
+  OMNI_TRY_HIP(ctx, launch_plan_counts(pl.rays.as<RayRec>(), (int64_t)nrays, pl.dirs.as<DirConst>(),
+                                       g, pl.cnt.as<unsigned int>(), flag.as<uint8_t>(),
+                                       steps.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, pl.sid.alloc(sizeof(int) * V));
+  OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * V));
+  DevBuf ns;
+  OMNI_TRY_HIP(ctx, ns.alloc(sizeof(int)));
+  OMNI_TRY_HIP(ctx, launch_source_ids(flag.as<uint8_t>(), V, pl.sid.as<int>(), pl.slist.as<int>(),
+                                      ns.as<int>(), st));
+  OMNI_TRY_HIP(ctx, launch_max_u32(pl.cnt.as<unsigned int>(), V, mx.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, launch_max_len(pl.rays.as<RayRec>(), (int64_t)nrays, mx.as<unsigned int>() + 1, st));
+  int nS = 0;
+  unsigned int mxh[2] = {0, 0};
+  unsigned long long nsteps = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, ns.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mxh, mx.as<unsigned int>(), 8, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsteps, steps.as<void>(), 8, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  // CSR of visits landing on source voxels
+  DevBuf rowcnt;
+  OMNI_TRY_HIP(ctx, rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  OMNI_TRY_HIP(ctx, launch_plan_rowcount(pl.rays.as<RayRec>(), (int64_t)nrays,
+                                         pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
+                                         rowcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, pl.rowptr.alloc(sizeof(unsigned long long) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(rowcnt.as<unsigned int>(), nS,
+                                       pl.rowptr.as<unsigned long long>(), st));
+  unsigned long long nnz = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, pl.rowptr.as<unsigned long long>() + nS, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  OMNI_TRY_HIP(ctx, pl.col.alloc(sizeof(int) * (nnz + 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  OMNI_TRY_HIP(ctx, launch_plan_rowfill(pl.rays.as<RayRec>(), (int64_t)nrays,
+                                        pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
+                                        pl.rowptr.as<unsigned long long>(),
+                                        rowcnt.as<unsigned int>(), pl.col.as<int>(), st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  pl.X = X;
+  pl.Y = Y;
+  pl.Z = Z;
+  pl.D = D;
+  pl.ls = P.line_spacing;
+  pl.g = g;
+  pl.nrays = (int64_t)nrays;
+  pl.steps = (int64_t)nsteps;
+  pl.nS = nS;
+  pl.nnz = (int64_t)nnz;
+  pl.cmax = mxh[0];
+  pl.lmax = mxh[1];
+  pl.valid = true;
+  ctx->dirs_dirty = false;
+  return OMNI3D_OK;
+}
+
+WalkParams walk_params(const omni3d_params& P) {
+  WalkParams wp;
+  wp.rho = P.density;
+  const double del[3] = {P.dx, P.dy, P.dz};
+  for (int a = 0; a < 3; a++) {
+    // ((-rho * s) * delta) * 0.5, the factor of -rho*(next-last)*delta*0.5*(a_n+a_l)
+    // (kernel.cu:1816) for s = -1 / +1
+    wp.hs[a][0] = -P.density * -1.0 * del[a] * 0.5;
+    wp.hs[a][1] = -P.density * 1.0 * del[a] * 0.5;
+  }
+  wp.thr = P.threshold;
+  wp.method = P.method;
+  wp.faithful = P.faithful_select;
+  return wp;
+}
+
+template <typename T>
+int run_affine(omni3d_ctx* ctx, int seed) {
+  const omni3d_params& P = ctx->prm;
+  Plan& pl = ctx->plan;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  DevScalars* sc = ctx->scal.as<DevScalars>();
+  const T* ax = ctx->ax.as<T>();
+  const T* ay = ctx->ay.as<T>();
+  const T* az = ctx->az.as<T>();
+  const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
+  const WalkParams wp = walk_params(P);
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(sc, 0, sizeof(DevScalars), st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ax, V, &sc->amax_bits, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ay, V, &sc->amax_bits, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(az, V, &sc->amax_bits, st));
+  if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
+  const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
+  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
+                                         (double)P.max_iterations, st));
+  // pass A: trapezoid sums landing on source voxels
+  OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * std::max(pl.nS, 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->bs.as<void>(), 0, sizeof(long long) * pl.nS, st));
+  OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
+                                         pl.dirs.as<DirConst>(), pl.g, wp, ax, ay, az,
+                                         pl.sid.as<int>(), nullptr, nullptr, sc,
+                                         ctx->bs.as<long long>(), st));
+  // source iteration
+  OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * std::max(pl.nS, 1)));
+  OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * std::max(pl.nS, 1)));
+  OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * iterate_grid(pl.nS)));
+  OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
+  double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
+  for (int k = 1; k <= P.max_iterations; k++) {
+    OMNI_TRY_HIP(ctx, launch_iterate(k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.col.as<int>(),
+                                     pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
+                                     ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0,
+                                     pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+                                     ctx->partials.as<double>(), P.max_iterations, P.tolerance, st));
+  }
+  // pass B: every voxel from p_S^{k*-1}
+  OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->sum.as<void>(), 0, sizeof(long long) * V, st));
+  OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
+                                         pl.dirs.as<DirConst>(), pl.g, wp, ax, ay, az,
+                                         pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
+                                         ctx->sum.as<long long>(), st));
+  return OMNI3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void omni3d_default_params(omni3d_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->dx = p->dy = p->dz = 1.0;
+  p->density = 1.0;
+  p->line_spacing = 1.0;
+  p->method = OMNI3D_METHOD_PLAIN;
+  p->threshold = 0.0;
+  p->faithful_select = 1;
+  p->precision = OMNI3D_FP64;
+  p->solver = OMNI3D_SOLVER_AFFINE;
+  p->max_iterations = 10;
+  p->tolerance = 0.0;
+  p->seed_pn_with_p = 1;
+}
+
+const char* omni3d_last_error(const omni3d_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+const char* omni3d_last_global_error(void) { return g_last_error.c_str(); }
+
+int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
+  if (!params || !out) return fail(nullptr, OMNI3D_ERR_ARG, "null argument");
+  std::string why;
+  if (!params_ok(*params, &why)) return fail(nullptr, OMNI3D_ERR_ARG, why);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(nullptr, OMNI3D_ERR_NODEVICE, "no HIP device visible");
+  omni3d_ctx* ctx = new omni3d_ctx();
+  ctx->prm = *params;
+  for (auto& e : ctx->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete ctx;
+      return fail(nullptr, OMNI3D_ERR_HIP, "hipEventCreate failed");
+    }
+  }
+  if (ctx->scal.alloc(sizeof(DevScalars)) != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, OMNI3D_ERR_HIP, "hipMalloc failed");
+  }
+  *out = ctx;
+  return OMNI3D_OK;
+}
+
+void omni3d_destroy(omni3d_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete ctx;
+}
+
+int omni3d_set_stream(omni3d_ctx* ctx, void* s) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  ctx->stream = static_cast<hipStream_t>(s);
+  return OMNI3D_OK;
+}
+
+int omni3d_set_directions(omni3d_ctx* ctx, const float* k1, const float* k2, const float* k3,
+                          int count) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  if (!k1 || !k2 || !k3 || count < 1 || count > 65535)
+    return fail(ctx, OMNI3D_ERR_ARG, "need 1..65535 direction vectors");
+  std::vector<float> k(3 * (size_t)count);
+  for (int i = 0; i < count; i++) {
+    const double n = std::sqrt((double)k1[i] * k1[i] + (double)k2[i] * k2[i] + (double)k3[i] * k3[i]);
+    if (!(n > 0) || !std::isfinite(n)) return fail(ctx, OMNI3D_ERR_ARG, "zero or non-finite direction");
+    k[3 * i] = k1[i];
+    k[3 * i + 1] = k2[i];
+    k[3 * i + 2] = k3[i];
+  }
+  ctx->k.swap(k);
+  ctx->dirs_dirty = true;
+  return OMNI3D_OK;
+}
+
+int omni3d_load_directions_file(omni3d_ctx* ctx, const char* path, int count) {
+  if (!ctx || !path) return fail(ctx, OMNI3D_ERR_ARG, "null argument");
+  std::vector<float> k;
+  std::string err;
+  if (!read_direction_file(path, count, &k, &err)) return fail(ctx, OMNI3D_ERR_IO, err);
+  const int n = (int)(k.size() / 3);
+  std::vector<float> a(n), b(n), c(n);
+  for (int i = 0; i < n; i++) {
+    a[i] = k[3 * i];
+    b[i] = k[3 * i + 1];
+    c[i] = k[3 * i + 2];
+  }
+  return omni3d_set_directions(ctx, a.data(), b.data(), c.data(), n);
+}
+
+int omni3d_generate_directions(int divide, float* k1, float* k2, float* k3, int capacity) {
+  if (divide < 1 || divide > 8) return -fail(nullptr, OMNI3D_ERR_ARG, "divide must be 1..8");
+  const std::vector<double> v = icosahedral_directions(divide);
+  const int n = (int)(v.size() / 3);
+  for (int i = 0; i < n && i < capacity; i++) {
+    if (k1) k1[i] = (float)v[3 * i];
+    if (k2) k2[i] = (float)v[3 * i + 1];
+    if (k3) k3[i] = (float)v[3 * i + 2];
+  }
+  return n;
+}
+
+int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const void* dwdt,
+                     const void* mask, int dtype, int mem_kind) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  if (!dudt || !dvdt || !dwdt) return fail(ctx, OMNI3D_ERR_ARG, "null field component");
+  if (dtype != OMNI3D_F32 && dtype != OMNI3D_F64) return fail(ctx, OMNI3D_ERR_ARG, "bad dtype");
+  if (mem_kind != OMNI3D_MEM_HOST && mem_kind != OMNI3D_MEM_DEVICE)
+    return fail(ctx, OMNI3D_ERR_ARG, "bad mem_kind");
+  const int64_t V = voxels(ctx->prm);
+  const bool f32 = ctx->prm.precision == OMNI3D_FP32_KAHAN;
+  const size_t esz = f32 ? 4 : 8;
+  hipStream_t st = ctx->stream;
+  DevBuf stage;
+  const void* src[4] = {dudt, dvdt, dwdt, mask};
+  DevBuf* dst[4] = {&ctx->ax, &ctx->ay, &ctx->az, &ctx->mask};
+  for (int c = 0; c < 4; c++) {
+    if (!src[c]) continue;
+    OMNI_TRY_HIP(ctx, dst[c]->alloc(esz * V));
+    const void* dsrc = src[c];
+    if (mem_kind == OMNI3D_MEM_HOST) {
+      OMNI_TRY_HIP(ctx, stage.alloc(elem_size(dtype) * V));
+      OMNI_TRY_HIP(ctx, hipMemcpyAsync(stage.as<void>(), src[c], elem_size(dtype) * V,
+                                       hipMemcpyHostToDevice, st));
+      dsrc = stage.as<void>();
+    }
+    hipError_t e;
+    if (dtype == OMNI3D_F64)
+      e = f32 ? launch_convert<double, float>((const double*)dsrc, dst[c]->as<float>(), V, st)
+              : launch_convert<double, double>((const double*)dsrc, dst[c]->as<double>(), V, st);
+    else
+      e = f32 ? launch_convert<float, float>((const float*)dsrc, dst[c]->as<float>(), V, st)
+              : launch_convert<float, double>((const float*)dsrc, dst[c]->as<double>(), V, st);
+    OMNI_TRY_HIP(ctx, e);
+    if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  }
+  ctx->have_field = true;
+  ctx->have_mask = mask != nullptr;
+  return OMNI3D_OK;
+}
+
+int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, int mem_kind,
+                       omni3d_report* rep) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  if (dtype != OMNI3D_F32 && dtype != OMNI3D_F64) return fail(ctx, OMNI3D_ERR_ARG, "bad dtype");
+  if (mem_kind != OMNI3D_MEM_HOST && mem_kind != OMNI3D_MEM_DEVICE)
+    return fail(ctx, OMNI3D_ERR_ARG, "bad mem_kind");
+  if (ctx->k.empty()) return fail(ctx, OMNI3D_ERR_STATE, "no directions set");
+  if (!ctx->have_field) return fail(ctx, OMNI3D_ERR_STATE, "no field set");
+  const omni3d_params& P = ctx->prm;
+  if (P.method != OMNI3D_METHOD_PLAIN || P.solver != OMNI3D_SOLVER_AFFINE)
+    return fail(ctx, OMNI3D_ERR_ARG, "method/solver combination not built yet");
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  const bool had_plan = ctx->plan.valid && !ctx->dirs_dirty;
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+  int rc = build_plan(ctx);
+  if (rc) return rc;
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+  // initial pressure
+  int seed = -1;
+  if (p) {
+    OMNI_TRY_HIP(ctx, ctx->pinit.alloc(sizeof(double) * V));
+    DevBuf stage;
+    const void* src = p;
+    if (mem_kind == OMNI3D_MEM_HOST) {
+      OMNI_TRY_HIP(ctx, stage.alloc(elem_size(dtype) * V));
+      OMNI_TRY_HIP(ctx, hipMemcpyAsync(stage.as<void>(), p, elem_size(dtype) * V,
+                                       hipMemcpyHostToDevice, st));
+      src = stage.as<void>();
+    }
+    hipError_t e = dtype == OMNI3D_F64
+                       ? launch_convert<double, double>((const double*)src, ctx->pinit.as<double>(), V, st)
+                       : launch_convert<float, double>((const float*)src, ctx->pinit.as<double>(), V, st);
+    OMNI_TRY_HIP(ctx, e);
+    if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    seed = P.seed_pn_with_p ? 1 : 0;
+  }
+  rc = P.precision == OMNI3D_FP32_KAHAN ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
+  if (rc) return rc;
+  // averaging into the caller's buffers (device) or a staging buffer (host)
+  const size_t esz = elem_size(dtype);
+  void* pdst = p;
+  void* wdst = weight_out;
+  if (mem_kind == OMNI3D_MEM_HOST) {
+    OMNI_TRY_HIP(ctx, ctx->pout.alloc(esz * V));
+    pdst = ctx->pout.as<void>();
+    if (weight_out) {
+      OMNI_TRY_HIP(ctx, ctx->wout.alloc(esz * V));
+      wdst = ctx->wout.as<void>();
+    }
+  }
+  DevBuf ptmp;
+  if (!pdst) {
+    OMNI_TRY_HIP(ctx, ptmp.alloc(esz * V));
+    pdst = ptmp.as<void>();
+  }
+  const bool f64 = dtype == OMNI3D_F64;
+  OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->plan.cnt.as<unsigned int>(),
+                                    seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
+                                    ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
+                                    f64 ? nullptr : (float*)pdst,
+                                    (wdst && f64) ? (double*)wdst : nullptr,
+                                    (wdst && !f64) ? (float*)wdst : nullptr, st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  if (mem_kind == OMNI3D_MEM_HOST) {
+    if (p) OMNI_TRY_HIP(ctx, hipMemcpyAsync(p, pdst, esz * V, hipMemcpyDeviceToHost, st));
+    if (weight_out)
+      OMNI_TRY_HIP(ctx, hipMemcpyAsync(weight_out, wdst, esz * V, hipMemcpyDeviceToHost, st));
+  }
+  DevScalars hs;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&hs, ctx->scal.as<void>(), sizeof(hs), hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (rep) {
+    rep->iterations = hs.kstar;
+    rep->rel_change = hs.rel;
+    rep->rays = ctx->plan.nrays;
+    rep->steps = ctx->plan.steps;
+    rep->sources = ctx->plan.nS;
+    rep->status = hs.status;
+    float a = 0, b = 0;
+    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
+    (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
+    rep->ms_plan = had_plan ? 0.0 : a;
+    rep->ms_solve = b;
+  }
+  if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");
+  return OMNI3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Device data structures and kernel launch wrappers of libomni3d.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "geometry.hpp"
+
+namespace omni3d {
+
+// One accepted ray (line x direction pair), 16 bytes.
+struct RayRec {
+  uint32_t in;       // packed entry voxel (10 bits per axis)
+  uint32_t out;      // packed exit voxel
+  uint32_t point;    // lattice index
+  uint32_t dir_len;  // direction (low 16 bits) | L1 length (high 16 bits)
+};
+
+struct GridDesc {
+  int X, Y, Z;
+  int ng;        // lattice size NoGrid
+  double ls;     // line spacing
+  double cc[3];  // (dims - 1) / 2
+};
+
+// Scalars shared between the kernels of one reconstruction, kept on the device
+// so that the iteration needs no host round trip.
+struct DevScalars {
+  unsigned long long amax_bits;  // max |a| (bit pattern of a non-negative double)
+  unsigned long long pmax_bits;  // max |p_init|
+  double scale;                  // fixed-point scale, a power of two
+  double inv_scale;
+  double rel;                    // last relative change
+  double dnum, dden;             // its numerator / denominator
+  int status;                    // 0 ok, OMNI3D_ERR_NONFINITE
+  int iter_done;                 // iterations whose p_S is computed
+  int converged;                 // 1 once the stop rule fired
+  int kstar;                     // iterations the output represents
+  unsigned int ticket;           // last-block counter of the reductions
+  unsigned int pad;
+  unsigned long long steps;      // staircase steps of the last full pass
+};
+
+// Method-specific walk parameters.
+struct WalkParams {
+  double rho;
+  double hs[3][2];   // ((-rho * s) * delta_a) * 0.5 for s = -1 (index 0) / +1 (index 1)
+  double thr;        // Select threshold (+inf for plain-mask)
+  int method;
+  int faithful;
+};
+
+}  // namespace omni3d

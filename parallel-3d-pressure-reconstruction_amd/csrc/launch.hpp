@@ -1,0 +1,67 @@
+// Host-side launchers of the kernels (each defined next to its kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace omni3d {
+
+// Grid size for a grid-stride loop over n items with 256-thread blocks.
+inline unsigned int grid_for(int64_t n, unsigned int cap = 65536) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned int)b;
+}
+
+// walk.hip -----------------------------------------------------------------
+constexpr int kEnumItemsPerBlock = 1024;  // candidates per enumeration block
+hipError_t launch_enum_count(const DirConst* dirs, GridDesc g, int64_t npts, int64_t ncand,
+                             unsigned int* block_counts, hipStream_t st);
+hipError_t launch_enum_write(const DirConst* dirs, GridDesc g, int64_t npts, int64_t ncand,
+                             const unsigned long long* block_offsets, RayRec* rays,
+                             hipStream_t st);
+hipError_t launch_plan_counts(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                              GridDesc g, unsigned int* cnt, uint8_t* src,
+                              unsigned long long* steps, hipStream_t st);
+hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                GridDesc g, const int* sid, unsigned int* rowcnt,
+                                hipStream_t st);
+hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                               GridDesc g, const int* sid, const unsigned long long* rowptr,
+                               unsigned int* cursor, int* col, hipStream_t st);
+template <typename T>
+hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                             GridDesc g, const WalkParams& wp, const T* ax, const T* ay,
+                             const T* az, const int* sid, const double* p0, const double* p1,
+                             const DevScalars* sc, long long* acc, hipStream_t st);
+
+// solve.hip ----------------------------------------------------------------
+template <typename T>
+hipError_t launch_absmax(const T* x, int64_t n, unsigned long long* bits, hipStream_t st);
+hipError_t launch_compute_scale(DevScalars* sc, double gfac, double lmax, double cmax,
+                                double iters, hipStream_t st);
+hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, double* ps,
+                                 hipStream_t st);
+unsigned int iterate_grid(int nS);
+hipError_t launch_iterate(int k, int nS, const unsigned long long* rowptr, const int* col,
+                          const int* slist, const unsigned int* cnt, const long long* bs,
+                          const double* pinit, int seed, const double* pold, double* pnew,
+                          DevScalars* sc, double* partials, int max_iter, double tol,
+                          hipStream_t st);
+hipError_t launch_finalize(int64_t V, const long long* sum, const unsigned int* cnt,
+                           const double* pinit, int seed, const DevScalars* sc, double* pd,
+                           float* pf, double* wd, float* wf, hipStream_t st);
+template <typename S, typename D>
+hipError_t launch_convert(const S* s, D* d, int64_t n, hipStream_t st);
+hipError_t launch_source_ids(const uint8_t* flag, int64_t V, int* sid, int* slist, int* nS,
+                             hipStream_t st);
+hipError_t launch_scan_counts(const unsigned int* cnt, int n, unsigned long long* ptr,
+                              hipStream_t st);
+hipError_t launch_max_u32(const unsigned int* x, int64_t n, unsigned int* out, hipStream_t st);
+hipError_t launch_max_len(const RayRec* rays, int64_t n, unsigned int* out, hipStream_t st);
+
+}  // namespace omni3d

@@ -1,0 +1,206 @@
+"""Python harness binding of libomni3d.so (ctypes over include/omni3d.h).
+
+The library is C++/HIP; this module only loads it and wraps the C-ABI for
+tests and the benchmark.  It never falls back to anything else: if the
+shared library is missing or no GPU is visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libomni3d.so")
+
+OK = 0
+ERR_ARG, ERR_STATE, ERR_HIP, ERR_IO, ERR_NONFINITE, ERR_NODEVICE = 1, 2, 3, 4, 5, 6
+PLAIN, PLAIN_MASK, SELECT, WEIGHTED = 0, 1, 2, 3
+FP64, FP32_KAHAN = 0, 1
+AFFINE, NAIVE = 0, 1
+F32, F64 = 0, 1
+HOST, DEVICE = 0, 1
+
+EXPORTS = [
+    "omni3d_default_params", "omni3d_create", "omni3d_destroy", "omni3d_last_error",
+    "omni3d_last_global_error", "omni3d_set_stream", "omni3d_set_directions",
+    "omni3d_load_directions_file", "omni3d_generate_directions", "omni3d_set_field",
+    "omni3d_reconstruct", "omni3d_es_inner", "omni3d_es_inner_weighted",
+    "omni3d_es_inner_select", "omni3d_devide_count_inner", "omni3d_curl", "omni3d_forward",
+    "omni3d_backward", "omni3d_sor_solve", "sor3D", "calRHS", "omni3d_run_parameter_file",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("density", C.c_double), ("line_spacing", C.c_double),
+        ("method", C.c_int), ("threshold", C.c_double), ("faithful_select", C.c_int),
+        ("precision", C.c_int), ("solver", C.c_int), ("max_iterations", C.c_int),
+        ("tolerance", C.c_double), ("seed_pn_with_p", C.c_int),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int), ("rel_change", C.c_double), ("rays", C.c_int64),
+        ("steps", C.c_int64), ("sources", C.c_int64), ("status", C.c_int),
+        ("ms_plan", C.c_double), ("ms_solve", C.c_double),
+    ]
+
+
+_lib = None
+
+
+class Omni3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"omni3d error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    vp, i, d, f = C.c_void_p, C.c_int, C.c_double, C.c_float
+    fp = C.POINTER(C.c_float)
+    dp = C.POINTER(C.c_double)
+    L.omni3d_default_params.argtypes = [C.POINTER(Params)]
+    L.omni3d_default_params.restype = None
+    L.omni3d_create.argtypes = [C.POINTER(Params), C.POINTER(vp)]
+    L.omni3d_destroy.argtypes = [vp]
+    L.omni3d_destroy.restype = None
+    L.omni3d_last_error.argtypes = [vp]
+    L.omni3d_last_error.restype = C.c_char_p
+    L.omni3d_last_global_error.argtypes = []
+    L.omni3d_last_global_error.restype = C.c_char_p
+    L.omni3d_set_stream.argtypes = [vp, vp]
+    L.omni3d_set_directions.argtypes = [vp, fp, fp, fp, i]
+    L.omni3d_load_directions_file.argtypes = [vp, C.c_char_p, i]
+    L.omni3d_generate_directions.argtypes = [i, fp, fp, fp, i]
+    L.omni3d_set_field.argtypes = [vp, vp, vp, vp, vp, i, i]
+    L.omni3d_reconstruct.argtypes = [vp, vp, vp, i, i, C.POINTER(Report)]
+    es = [i, i, i, i, f, vp, vp, vp, f, f, f, f, vp, vp, vp]
+    L.omni3d_es_inner.argtypes = es + [vp, vp, vp, vp]
+    L.omni3d_es_inner_weighted.argtypes = es + [vp, vp, vp, vp, vp]
+    L.omni3d_es_inner_select.argtypes = es + [vp, vp, vp, vp, f, vp]
+    L.omni3d_devide_count_inner.argtypes = [i, i, i, vp, vp, vp, vp]
+    L.omni3d_curl.argtypes = [i, i, i, f, f, f, vp, vp, vp, vp, vp, vp]
+    L.omni3d_forward.argtypes = [vp, vp, vp]
+    L.omni3d_backward.argtypes = [vp, vp, vp]
+    L.omni3d_sor_solve.argtypes = [i, i, i, d, i, d, dp, dp, C.POINTER(C.c_int), dp]
+    L.sor3D.argtypes = [i, i, i, i, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, d]
+    L.sor3D.restype = None
+    L.calRHS.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
+    L.calRHS.restype = None
+    L.omni3d_run_parameter_file.argtypes = [C.c_char_p]
+
+
+def check(rc, ctx=None):
+    if rc != OK:
+        L = lib()
+        msg = (L.omni3d_last_error(ctx) if ctx else L.omni3d_last_global_error()) or b""
+        raise Omni3DError(rc, msg.decode(errors="replace"))
+
+
+def generate_directions(divide):
+    """Icosahedral unit vectors, shape (D, 3) float32."""
+    L = lib()
+    n = L.omni3d_generate_directions(divide, None, None, None, 0)
+    if n < 0:
+        check(-n)
+    k = np.zeros((3, n), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    L.omni3d_generate_directions(divide, k[0].ctypes.data_as(fp), k[1].ctypes.data_as(fp),
+                                 k[2].ctypes.data_as(fp), n)
+    return np.ascontiguousarray(k.T)
+
+
+def default_params(**kw):
+    p = Params()
+    lib().omni3d_default_params(C.byref(p))
+    for key, val in kw.items():
+        setattr(p, key, val)
+    return p
+
+
+class Context:
+    """One omni3d_ctx: directions + field -> reconstruct."""
+
+    def __init__(self, shape, delta=(1.0, 1.0, 1.0), **kw):
+        Z, Y, X = shape
+        self.shape = tuple(shape)
+        self.params = default_params(nx=X, ny=Y, nz=Z, dx=delta[0], dy=delta[1], dz=delta[2],
+                                     **kw)
+        self._h = C.c_void_p()
+        check(lib().omni3d_create(C.byref(self.params), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().omni3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        check(rc, self._h)
+
+    def set_directions(self, k):
+        k = np.ascontiguousarray(np.asarray(k, dtype=np.float32).reshape(-1, 3).T)
+        fp = C.POINTER(C.c_float)
+        self._chk(lib().omni3d_set_directions(self._h, k[0].ctypes.data_as(fp),
+                                              k[1].ctypes.data_as(fp), k[2].ctypes.data_as(fp),
+                                              k.shape[1]))
+
+    def load_directions_file(self, path, count=0):
+        self._chk(lib().omni3d_load_directions_file(self._h, path.encode(), count))
+
+    def set_field(self, acc, mask=None, dtype=np.float64):
+        self._keep = [np.ascontiguousarray(a, dtype=dtype) for a in acc]
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=dtype)
+        self._keep.append(m)
+        dt = F64 if dtype == np.float64 else F32
+        self._chk(lib().omni3d_set_field(self._h, self._keep[0].ctypes.data,
+                                         self._keep[1].ctypes.data, self._keep[2].ctypes.data,
+                                         None if m is None else m.ctypes.data, dt, HOST))
+
+    def set_field_device(self, ptrs, mask_ptr=None, dtype=F32):
+        self._chk(lib().omni3d_set_field(self._h, ptrs[0], ptrs[1], ptrs[2], mask_ptr, dtype,
+                                         DEVICE))
+
+    def reconstruct(self, p0=None, dtype=np.float64, weights=True):
+        """Host buffers; p0 = initial pressure (zeros if None).  Returns (p, w, report)."""
+        p = (np.zeros(self.shape, dtype=dtype) if p0 is None
+             else np.ascontiguousarray(p0, dtype=dtype).copy())
+        w = np.zeros(self.shape, dtype=dtype) if weights else None
+        rep = Report()
+        dt = F64 if dtype == np.float64 else F32
+        self._chk(lib().omni3d_reconstruct(self._h, p.ctypes.data,
+                                           w.ctypes.data if w is not None else None, dt, HOST,
+                                           C.byref(rep)))
+        return p, w, rep
+
+    def reconstruct_device(self, p_ptr, w_ptr=None, dtype=F64, report=True):
+        rep = Report()
+        self._chk(lib().omni3d_reconstruct(self._h, p_ptr, w_ptr, dtype, DEVICE,
+                                           C.byref(rep) if report else None))
+        return rep

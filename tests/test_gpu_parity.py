@@ -1,0 +1,91 @@
+"""GPU parity of the plain-method reconstruction against the FP64 oracle.
+
+Every test calls libomni3d.so through its C-ABI on the GPU and compares with
+the oracle (oracle/omni3d_oracle.c) on the same seeded input.  Tolerances:
+FP64 path rel-L2 <= 1e-12 (the only difference is the fixed-point vs FP64
+summation order); FP32-Kahan path rel-L2 <= 1e-5 against the FP64 oracle;
+visit counts bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import omni3d
+from omni3d import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(f, dirs, shape, precision=omni3d.FP64, max_iter=5, tol=0.0, p0=None, seed_pn=1,
+            ls=1.0, dtype=np.float64):
+    with omni3d.Context(shape, f["delta"], density=f["rho"], line_spacing=ls,
+                        precision=precision, max_iterations=max_iter, tolerance=tol,
+                        seed_pn_with_p=seed_pn) as ctx:
+        ctx.set_directions(dirs)
+        ctx.set_field(f["acc"], dtype=dtype)
+        return ctx.reconstruct(p0)
+
+
+def run_oracle(f, dirs, shape, max_iter=5, tol=0.0, p0=None, seed_pn=1, ls=1.0):
+    return O.reconstruct(shape, dirs, ls, f["delta"], f["rho"], f["acc"], p0=p0,
+                         max_iter=max_iter, tol=tol, seed_pn=seed_pn)
+
+
+@pytest.mark.parametrize("shape", [(9, 10, 11), (16, 12, 7)])
+def test_quadratic_exact_seed(gpu_available, ref_dirs2, shape):
+    print(f"case quadratic {shape}", flush=True)
+    f = F.quadratic(shape, seed=3)
+    p, w, rep = run_gpu(f, ref_dirs2, shape, max_iter=3, p0=f["p"], seed_pn=0)
+    po, co, it, rel = run_oracle(f, ref_dirs2, shape, max_iter=3, p0=f["p"], seed_pn=0)
+    assert rep.iterations == it == 3
+    np.testing.assert_array_equal(w, co)
+    assert F.rel_l2(p, po) < 1e-12
+    # trapezoid integrates a linear gradient exactly: p is reproduced
+    assert np.abs(p - f["p"]).max() < 1e-9 * max(1.0, np.abs(f["p"]).max())
+
+
+@pytest.mark.parametrize("dirs_name", ["ref2", "gen2", "gen3"])
+def test_taylor_green_fp64_matches_oracle(gpu_available, ref_dirs2, dirs_name):
+    shape = (20, 20, 20)
+    print(f"case taylor-green fp64 {dirs_name}", flush=True)
+    dirs = ref_dirs2 if dirs_name == "ref2" else omni3d.generate_directions(int(dirs_name[-1]))
+    f = F.taylor_green(20, noise=0.05, seed=1)
+    p, w, rep = run_gpu(f, dirs, shape, max_iter=12)
+    po, co, it, rel = run_oracle(f, dirs, shape, max_iter=12)
+    assert rep.iterations == it
+    np.testing.assert_array_equal(w, co)
+    assert F.rel_l2(p, po) < 1e-12
+    assert abs(rep.rel_change - rel) <= 1e-9 * max(rel, 1e-30)
+
+
+def test_seeded_pn_and_tolerance_stop(gpu_available, ref_dirs2):
+    shape = (14, 15, 16)
+    print("case seed+tolerance", flush=True)
+    f = F.taylor_green(16, noise=0.02, seed=2, shape=shape)
+    rng = np.random.default_rng(5)
+    p0 = rng.normal(size=shape)
+    p, w, rep = run_gpu(f, ref_dirs2, shape, max_iter=60, tol=1e-4, p0=p0, seed_pn=1)
+    po, co, it, rel = run_oracle(f, ref_dirs2, shape, max_iter=60, tol=1e-4, p0=p0, seed_pn=1)
+    assert rep.iterations == it
+    assert F.rel_l2(p, po) < 1e-11
+
+
+def test_fp32_kahan_close_to_fp64_oracle(gpu_available, ref_dirs3):
+    shape = (24, 24, 24)
+    print("case fp32 kahan", flush=True)
+    f = F.taylor_green(24, noise=0.05, seed=4)
+    p, w, rep = run_gpu(f, ref_dirs3, shape, precision=omni3d.FP32_KAHAN, max_iter=10)
+    po, co, it, rel = run_oracle(f, ref_dirs3, shape, max_iter=10)
+    np.testing.assert_array_equal(w, co)
+    err = F.rel_l2(p, po)
+    print("fp32-kahan rel-L2 vs FP64 oracle", err)
+    assert err < 1e-5
+
+
+def test_repeatable_bitwise(gpu_available, ref_dirs3):
+    shape = (18, 17, 19)
+    print("case determinism", flush=True)
+    f = F.taylor_green(18, noise=0.05, seed=6, shape=shape)
+    a = run_gpu(f, ref_dirs3, shape, max_iter=8)[0]
+    b = run_gpu(f, ref_dirs3, shape, max_iter=8)[0]
+    np.testing.assert_array_equal(a, b)
