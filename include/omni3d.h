@@ -92,7 +92,11 @@ typedef struct omni3d_report {
   int64_t sources;         /* voxels whose p the rays read */
   int status;              /* OMNI3D_OK or OMNI3D_ERR_NONFINITE */
   double ms_plan;          /* GPU time of the geometry plan build (0 if reused) */
-  double ms_solve;         /* GPU time of the reconstruction */
+  double ms_solve;         /* GPU time of the reconstruction (everything after the plan) */
+  double ms_pass_a;        /* first traversal: ray sums landing on source voxels */
+  double ms_iterate;       /* source-voxel iterations (or the naive passes) */
+  double ms_pass_b;        /* final traversal into every voxel */
+  double ms_finalize;      /* averaging into the output */
 } omni3d_report;
 
 typedef struct omni3d_ctx omni3d_ctx;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -70,6 +71,72 @@ bool params_ok(const omni3d_params& p, std::string* why) {
 
 size_t elem_size(int dtype) { return dtype == OMNI3D_F64 ? 8 : 4; }
 
+// Tile engine part of the plan: compiled step codes and per-tile segments.
+int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
+  Plan& pl = ctx->plan;
+  hipStream_t st = ctx->stream;
+  TileGrid tg;
+  tg.ntx = (g.X + kTile - 1) / kTile;
+  tg.nty = (g.Y + kTile - 1) / kTile;
+  tg.ntz = (g.Z + kTile - 1) / kTile;
+  const int nt = tg.ntx * tg.nty * tg.ntz;
+  DevBuf nsteps, nseg, tcnt, tmp;
+  OMNI_TRY_HIP(ctx, nsteps.alloc(sizeof(unsigned int) * nrays));
+  OMNI_TRY_HIP(ctx, nseg.alloc(sizeof(unsigned int) * nrays));
+  OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * nt));
+  OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(nrays, nt))));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nt, st));
+  OMNI_TRY_HIP(ctx, launch_tile_count(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
+                                      nsteps.as<unsigned int>(), nseg.as<unsigned int>(),
+                                      tcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, pl.word_off.alloc(sizeof(unsigned long long) * (nrays + 1)));
+  OMNI_TRY_HIP(ctx, pl.seg_base.alloc(sizeof(unsigned long long) * (nrays + 1)));
+  OMNI_TRY_HIP(ctx, pl.tile_ptr.alloc(sizeof(unsigned long long) * (nt + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(nsteps.as<unsigned int>(), nrays,
+                                    pl.word_off.as<unsigned long long>(), 1, tmp.as<void>(),
+                                    tmp.size(), st));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(nseg.as<unsigned int>(), nrays,
+                                    pl.seg_base.as<unsigned long long>(), 0, tmp.as<void>(),
+                                    tmp.size(), st));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(tcnt.as<unsigned int>(), nt,
+                                    pl.tile_ptr.as<unsigned long long>(), 0, tmp.as<void>(),
+                                    tmp.size(), st));
+  unsigned long long tot[2] = {0, 0};
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&tot[0], pl.word_off.as<unsigned long long>() + nrays, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&tot[1], pl.seg_base.as<unsigned long long>() + nrays, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (tot[1] >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
+  OMNI_TRY_HIP(ctx, pl.words.alloc(sizeof(uint32_t) * (tot[0] + 1)));
+  OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(tot[1], 1)));
+  OMNI_TRY_HIP(ctx, pl.segval.alloc(sizeof(double) * std::max<unsigned long long>(tot[1], 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.words.as<void>(), 0, sizeof(uint32_t) * (tot[0] + 1), st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nt, st));
+  OMNI_TRY_HIP(ctx, launch_tile_fill(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
+                                     pl.word_off.as<unsigned long long>(),
+                                     pl.seg_base.as<unsigned long long>(),
+                                     pl.tile_ptr.as<unsigned long long>(), tcnt.as<unsigned int>(),
+                                     pl.words.as<uint32_t>(), pl.segs.as<SegRec>(), st));
+  // tiles that touch the box surface hold every source voxel of the plain method
+  std::vector<int> surf;
+  for (int t = 0; t < nt; t++) {
+    const int tx = t % tg.ntx, ty = (t / tg.ntx) % tg.nty, tz = t / (tg.ntx * tg.nty);
+    if (tx == 0 || ty == 0 || tz == 0 || tx == tg.ntx - 1 || ty == tg.nty - 1 || tz == tg.ntz - 1)
+      surf.push_back(t);
+  }
+  OMNI_TRY_HIP(ctx, pl.surf.alloc(sizeof(int) * std::max<size_t>(surf.size(), 1)));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.surf.as<void>(), surf.data(), sizeof(int) * surf.size(),
+                                   hipMemcpyHostToDevice, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  pl.tg = tg;
+  pl.ntiles = nt;
+  pl.nsurf = (int)surf.size();
+  pl.nwords = (int64_t)tot[0];
+  pl.nsegs = (int64_t)tot[1];
+  return OMNI3D_OK;
+}
+
 // Builds the geometry plan: ray list, visit counts, source voxels and the CSR
 // of visits that land on source voxels.
 int build_plan(omni3d_ctx* ctx) {
@@ -97,7 +164,9 @@ int build_plan(omni3d_ctx* ctx) {
   g.cc[0] = (X - 1) / 2.0;
   g.cc[1] = (Y - 1) / 2.0;
   g.cc[2] = (Z - 1) / 2.0;
-  const int64_t npts = (int64_t)g.ng * g.ng;
+  // candidates per direction: the lattice padded to whole 16x16 patches
+  const int64_t side = (g.ng + 15) / 16;
+  const int64_t npts = side * side * 256;
   const int64_t ncand = npts * D;
   const int64_t nblk = (ncand + kEnumItemsPerBlock - 1) / kEnumItemsPerBlock;
   if (nblk > (int64_t)1 << 31) return fail(ctx, OMNI3D_ERR_ARG, "too many candidate rays");
@@ -147,10 +216,12 @@ int build_plan(omni3d_ctx* ctx) {
   // CSR of visits landing on source voxels
   DevBuf rowcnt;
   OMNI_TRY_HIP(ctx, rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, pl.diag.alloc(sizeof(unsigned int) * (nS + 1)));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.diag.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
   OMNI_TRY_HIP(ctx, launch_plan_rowcount(pl.rays.as<RayRec>(), (int64_t)nrays,
                                          pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
-                                         rowcnt.as<unsigned int>(), st));
+                                         rowcnt.as<unsigned int>(), pl.diag.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, pl.rowptr.alloc(sizeof(unsigned long long) * (nS + 1)));
   OMNI_TRY_HIP(ctx, launch_scan_counts(rowcnt.as<unsigned int>(), nS,
                                        pl.rowptr.as<unsigned long long>(), st));
@@ -164,7 +235,10 @@ int build_plan(omni3d_ctx* ctx) {
                                         pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
                                         pl.rowptr.as<unsigned long long>(),
                                         rowcnt.as<unsigned int>(), pl.col.as<int>(), st));
+  OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.rowptr.as<unsigned long long>(), pl.col.as<int>(), st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  int rc = build_tiles(ctx, g, (int64_t)nrays);
+  if (rc) return rc;
   pl.X = X;
   pl.Y = Y;
   pl.Z = Z;
@@ -205,46 +279,77 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   DevScalars* sc = ctx->scal.as<DevScalars>();
-  const T* ax = ctx->ax.as<T>();
-  const T* ay = ctx->ay.as<T>();
-  const T* az = ctx->az.as<T>();
+  const void* field = ctx->field.as<void>();
   const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
   const WalkParams wp = walk_params(P);
   OMNI_TRY_HIP(ctx, hipMemsetAsync(sc, 0, sizeof(DevScalars), st));
-  OMNI_TRY_HIP(ctx, launch_absmax<T>(ax, V, &sc->amax_bits, st));
-  OMNI_TRY_HIP(ctx, launch_absmax<T>(ay, V, &sc->amax_bits, st));
-  OMNI_TRY_HIP(ctx, launch_absmax<T>(az, V, &sc->amax_bits, st));
+  // max |a| over the packed field (the fourth component is 0)
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
   if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
   const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
   OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
                                          (double)P.max_iterations, st));
   // pass A: trapezoid sums landing on source voxels
-  OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * std::max(pl.nS, 1)));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->bs.as<void>(), 0, sizeof(long long) * pl.nS, st));
-  OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
-                                         pl.dirs.as<DirConst>(), pl.g, wp, ax, ay, az,
-                                         pl.sid.as<int>(), nullptr, nullptr, sc,
-                                         ctx->bs.as<long long>(), st));
+  const size_t ns = (size_t)std::max(pl.nS, 1);
+  OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->bs.as<void>(), 0, sizeof(long long) * ns, st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  if (ctx->engine == 0) {
+    // segment sums over all tiles, per-ray carries, then the source sums on
+    // the surface tiles
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.rays.as<RayRec>(),
+                                          pl.word_off.as<unsigned long long>(), pl.words.as<uint32_t>(),
+                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), nullptr, nullptr,
+                                          sc, pl.segval.as<double>(), nullptr, st));
+    OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.seg_base.as<unsigned long long>(),
+                                        pl.segval.as<double>(), st));
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, pl.nsurf, pl.surf.as<int>(),
+                                          pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(),
+                                          pl.rays.as<RayRec>(), pl.word_off.as<unsigned long long>(),
+                                          pl.words.as<uint32_t>(), pl.g, pl.tg, wp, field,
+                                          pl.sid.as<int>(), nullptr, nullptr, sc,
+                                          pl.segval.as<double>(), ctx->bs.as<long long>(), st));
+  } else {
+    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
+                                           pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
+                                           nullptr, nullptr, sc, ctx->bs.as<long long>(), 0, 1, st));
+  }
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
   // source iteration
-  OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * std::max(pl.nS, 1)));
-  OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * std::max(pl.nS, 1)));
+  OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * ns));
+  OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns));
   OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * iterate_grid(pl.nS)));
   OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
   for (int k = 1; k <= P.max_iterations; k++) {
     OMNI_TRY_HIP(ctx, launch_iterate(k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.col.as<int>(),
-                                     pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
-                                     ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0,
-                                     pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+                                     pl.diag.as<unsigned int>(), pl.slist.as<int>(),
+                                     pl.cnt.as<unsigned int>(), ctx->bs.as<long long>(), pinit,
+                                     seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
                                      ctx->partials.as<double>(), P.max_iterations, P.tolerance, st));
   }
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[4], st));
   // pass B: every voxel from p_S^{k*-1}
-  OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->sum.as<void>(), 0, sizeof(long long) * V, st));
-  OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
-                                         pl.dirs.as<DirConst>(), pl.g, wp, ax, ay, az,
-                                         pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
-                                         ctx->sum.as<long long>(), st));
+  const int nc = ctx->engine == 0 ? 1 : ctx->acc_copies;
+  ctx->used_copies = nc;
+  OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V * nc));
+  if (ctx->engine == 0) {
+    // every voxel belongs to exactly one tile, which stores its sum
+    OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.rays.as<RayRec>(),
+                                          pl.word_off.as<unsigned long long>(), pl.words.as<uint32_t>(),
+                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), pbuf[0], pbuf[1],
+                                          sc, pl.segval.as<double>(), ctx->sum.as<long long>(), st));
+  } else {
+    OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->sum.as<void>(), 0, sizeof(long long) * V * nc, st));
+    OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
+                                           pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
+                                           pbuf[0], pbuf[1], sc, ctx->sum.as<long long>(), V, nc, st));
+  }
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], st));
   return OMNI3D_OK;
 }
 
@@ -283,6 +388,11 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
     return fail(nullptr, OMNI3D_ERR_NODEVICE, "no HIP device visible");
   omni3d_ctx* ctx = new omni3d_ctx();
   ctx->prm = *params;
+  if (const char* e = std::getenv("OMNI3D_ACC_COPIES")) {
+    const int n = std::atoi(e);
+    if (n >= 1 && n <= 64) ctx->acc_copies = n;
+  }
+  if (const char* e = std::getenv("OMNI3D_ENGINE")) ctx->engine = std::strcmp(e, "rays") == 0 ? 1 : 0;
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;
@@ -365,31 +475,41 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
     return fail(ctx, OMNI3D_ERR_ARG, "bad mem_kind");
   const int64_t V = voxels(ctx->prm);
   const bool f32 = ctx->prm.precision == OMNI3D_FP32_KAHAN;
-  const size_t esz = f32 ? 4 : 8;
+  const size_t esz = elem_size(dtype);
   hipStream_t st = ctx->stream;
   DevBuf stage;
-  const void* src[4] = {dudt, dvdt, dwdt, mask};
-  DevBuf* dst[4] = {&ctx->ax, &ctx->ay, &ctx->az, &ctx->mask};
-  for (int c = 0; c < 4; c++) {
-    if (!src[c]) continue;
-    OMNI_TRY_HIP(ctx, dst[c]->alloc(esz * V));
-    const void* dsrc = src[c];
-    if (mem_kind == OMNI3D_MEM_HOST) {
-      OMNI_TRY_HIP(ctx, stage.alloc(elem_size(dtype) * V));
-      OMNI_TRY_HIP(ctx, hipMemcpyAsync(stage.as<void>(), src[c], elem_size(dtype) * V,
-                                       hipMemcpyHostToDevice, st));
-      dsrc = stage.as<void>();
+  const void* comp[4] = {dudt, dvdt, dwdt, mask};
+  const void* dsrc[4] = {dudt, dvdt, dwdt, mask};
+  if (mem_kind == OMNI3D_MEM_HOST) {
+    OMNI_TRY_HIP(ctx, stage.alloc(esz * V * 4));
+    for (int c = 0; c < 4; c++) {
+      if (!comp[c]) continue;
+      void* d = static_cast<char*>(stage.as<void>()) + esz * V * c;
+      OMNI_TRY_HIP(ctx, hipMemcpyAsync(d, comp[c], esz * V, hipMemcpyHostToDevice, st));
+      dsrc[c] = d;
     }
-    hipError_t e;
-    if (dtype == OMNI3D_F64)
-      e = f32 ? launch_convert<double, float>((const double*)dsrc, dst[c]->as<float>(), V, st)
-              : launch_convert<double, double>((const double*)dsrc, dst[c]->as<double>(), V, st);
-    else
-      e = f32 ? launch_convert<float, float>((const float*)dsrc, dst[c]->as<float>(), V, st)
-              : launch_convert<float, double>((const float*)dsrc, dst[c]->as<double>(), V, st);
-    OMNI_TRY_HIP(ctx, e);
-    if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   }
+  OMNI_TRY_HIP(ctx, ctx->field.alloc((f32 ? 4 : 8) * 4 * V));
+  hipError_t e;
+  if (dtype == OMNI3D_F64)
+    e = f32 ? launch_pack_field<double, float>((const double*)dsrc[0], (const double*)dsrc[1],
+                                               (const double*)dsrc[2], ctx->field.as<void>(), V, st)
+            : launch_pack_field<double, double>((const double*)dsrc[0], (const double*)dsrc[1],
+                                                (const double*)dsrc[2], ctx->field.as<void>(), V, st);
+  else
+    e = f32 ? launch_pack_field<float, float>((const float*)dsrc[0], (const float*)dsrc[1],
+                                              (const float*)dsrc[2], ctx->field.as<void>(), V, st)
+            : launch_pack_field<float, double>((const float*)dsrc[0], (const float*)dsrc[1],
+                                               (const float*)dsrc[2], ctx->field.as<void>(), V, st);
+  OMNI_TRY_HIP(ctx, e);
+  if (mask) {
+    OMNI_TRY_HIP(ctx, ctx->mask.alloc(sizeof(double) * V));
+    e = dtype == OMNI3D_F64
+            ? launch_convert<double, double>((const double*)dsrc[3], ctx->mask.as<double>(), V, st)
+            : launch_convert<float, double>((const float*)dsrc[3], ctx->mask.as<double>(), V, st);
+    OMNI_TRY_HIP(ctx, e);
+  }
+  if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   ctx->have_field = true;
   ctx->have_mask = mask != nullptr;
   return OMNI3D_OK;
@@ -452,13 +572,14 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     pdst = ptmp.as<void>();
   }
   const bool f64 = dtype == OMNI3D_F64;
-  OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->plan.cnt.as<unsigned int>(),
+  OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
+                                    ctx->plan.cnt.as<unsigned int>(),
                                     seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
                                     ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
                                     f64 ? nullptr : (float*)pdst,
                                     (wdst && f64) ? (double*)wdst : nullptr,
                                     (wdst && !f64) ? (float*)wdst : nullptr, st));
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], st));
   if (mem_kind == OMNI3D_MEM_HOST) {
     if (p) OMNI_TRY_HIP(ctx, hipMemcpyAsync(p, pdst, esz * V, hipMemcpyDeviceToHost, st));
     if (weight_out)
@@ -474,11 +595,17 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->steps = ctx->plan.steps;
     rep->sources = ctx->plan.nS;
     rep->status = hs.status;
-    float a = 0, b = 0;
-    (void)hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]);
-    (void)hipEventElapsedTime(&b, ctx->ev[1], ctx->ev[2]);
-    rep->ms_plan = had_plan ? 0.0 : a;
-    rep->ms_solve = b;
+    auto ms = [&](int a, int b) {
+      float t = 0;
+      (void)hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]);
+      return (double)t;
+    };
+    rep->ms_plan = had_plan ? 0.0 : ms(0, 1);
+    rep->ms_solve = ms(1, 7);
+    rep->ms_pass_a = ms(2, 3);
+    rep->ms_iterate = ms(3, 4);
+    rep->ms_pass_b = ms(5, 6);
+    rep->ms_finalize = ms(6, 7);
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");
   return OMNI3D_OK;

@@ -12,6 +12,7 @@
 #include "../../include/omni3d.h"
 #include "geometry.hpp"
 #include "kernels.hpp"
+#include "tiles.hpp"
 
 namespace omni3d {
 
@@ -70,6 +71,20 @@ struct Plan {
   DevBuf slist;    // int32[nS] voxel of each source
   DevBuf rowptr;   // uint64[nS+1]
   DevBuf col;      // int32[nnz] source id of each visit landing on a source
+  DevBuf diag;     // uint32[nS] rays entering at each source (their self visit)
+  // tile engine: compiled paths and per-tile segment lists
+  TileGrid tg{};
+  int ntiles = 0;
+  int nsurf = 0;
+  int64_t nsegs = 0;
+  int64_t nwords = 0;
+  DevBuf word_off;  // uint64[nrays+1]
+  DevBuf words;     // uint32[nwords+1] 2-bit step codes
+  DevBuf seg_base;  // uint64[nrays+1]
+  DevBuf tile_ptr;  // uint64[ntiles+1]
+  DevBuf segs;      // SegRec[nsegs], grouped by tile
+  DevBuf segval;    // double[nsegs] segment sums, then carries
+  DevBuf surf;      // int32[nsurf] tiles that hold source voxels
 };
 
 }  // namespace omni3d
@@ -81,8 +96,8 @@ struct omni3d_ctx {
   std::vector<float> k;  // 3*D, row-major (k1,k2,k3)
   bool dirs_dirty = true;
   omni3d::Plan plan;
-  // field in the compute precision
-  omni3d::DevBuf ax, ay, az, mask;
+  // field in the compute precision, one 4-vector (a_x, a_y, a_z, 0) per voxel
+  omni3d::DevBuf field, mask;
   bool have_field = false;
   bool have_mask = false;
   // per-reconstruction work buffers
@@ -91,7 +106,12 @@ struct omni3d_ctx {
   omni3d::DevBuf bs;        // int64[nS]
   omni3d::DevBuf ps0, ps1;  // double[nS]
   omni3d::DevBuf partials;  // double[2*grid]
-  omni3d::DevBuf sum;       // int64[V]
+  omni3d::DevBuf sum;       // int64[ncopies*V]
+  int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
+  int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
+  int used_copies = 1;
   omni3d::DevBuf pout, wout;  // staging for host outputs
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
+  // 5 pass B start, 6 pass B done, 7 finalize done
+  hipEvent_t ev[8] = {};
 };

@@ -29,15 +29,17 @@ hipError_t launch_plan_counts(const RayRec* rays, int64_t nrays, const DirConst*
                               unsigned long long* steps, hipStream_t st);
 hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                 GridDesc g, const int* sid, unsigned int* rowcnt,
-                                hipStream_t st);
+                                unsigned int* diag, hipStream_t st);
 hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                GridDesc g, const int* sid, const unsigned long long* rowptr,
                                unsigned int* cursor, int* col, hipStream_t st);
+// field: float4 / double4 per voxel (a_x, a_y, a_z, 0); acc: ncopies accumulator
+// copies copy_stride apart (block b adds into copy b % ncopies).
 template <typename T>
 hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                             GridDesc g, const WalkParams& wp, const T* ax, const T* ay,
-                             const T* az, const int* sid, const double* p0, const double* p1,
-                             const DevScalars* sc, long long* acc, hipStream_t st);
+                             GridDesc g, const WalkParams& wp, const void* field, const int* sid,
+                             const double* p0, const double* p1, const DevScalars* sc,
+                             long long* acc, int64_t copy_stride, int ncopies, hipStream_t st);
 
 // solve.hip ----------------------------------------------------------------
 template <typename T>
@@ -48,11 +50,15 @@ hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, 
                                  hipStream_t st);
 unsigned int iterate_grid(int nS);
 hipError_t launch_iterate(int k, int nS, const unsigned long long* rowptr, const int* col,
-                          const int* slist, const unsigned int* cnt, const long long* bs,
-                          const double* pinit, int seed, const double* pold, double* pnew,
-                          DevScalars* sc, double* partials, int max_iter, double tol,
-                          hipStream_t st);
-hipError_t launch_finalize(int64_t V, const long long* sum, const unsigned int* cnt,
+                          const unsigned int* diag, const int* slist, const unsigned int* cnt,
+                          const long long* bs, const double* pinit, int seed, const double* pold,
+                          double* pnew, DevScalars* sc, double* partials, int max_iter,
+                          double tol, hipStream_t st);
+hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, int* col, hipStream_t st);
+template <typename S, typename T>
+hipError_t launch_pack_field(const S* ax, const S* ay, const S* az, void* out, int64_t n,
+                             hipStream_t st);
+hipError_t launch_finalize(int64_t V, const long long* sum, int ncopies, const unsigned int* cnt,
                            const double* pinit, int seed, const DevScalars* sc, double* pd,
                            float* pf, double* wd, float* wf, hipStream_t st);
 template <typename S, typename D>

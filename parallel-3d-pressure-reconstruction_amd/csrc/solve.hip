@@ -64,44 +64,60 @@ k_gather_sources(const int* __restrict__ slist, int nS, const double* __restrict
 }
 
 // One iteration on the source voxels S (affine form of kernel.cu:8840-8861):
-//   p_S^k[v] = (sum_{visits of v} fix(p_S^{k-1}[src]) + bs[v] + seed) / C[v]
-// where C[v] > 0, else the old value.  One wavefront per row of the visit
-// CSR.  The last block reduces the relative change and sets the stop flag;
-// once it is set the remaining launches return at once.
+//   p_S^k[v] = (bs[v] + diag[v] p_S^{k-1}[v] + sum_{visits of v} p_S^{k-1}[src]
+//               + seed) / C[v]
+// where C[v] > 0, else the old value.  One lane per row; the rows are sorted
+// at plan time, so the FP64 sum has a fixed order.  The last block
+// reduces the relative change and sets the stop flag; once it is set the
+// remaining launches return at once.
 constexpr int kIterBlock = 256;
 
 __global__ void __launch_bounds__(kIterBlock)
 k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
-          const int* __restrict__ col, const int* __restrict__ slist,
-          const unsigned int* __restrict__ cnt, const long long* __restrict__ bs,
-          const double* __restrict__ pinit, int seed, const double* __restrict__ pold,
-          double* __restrict__ pnew, DevScalars* __restrict__ sc, double* __restrict__ partials,
-          int max_iter, double tol) {
+          const int* __restrict__ col, const unsigned int* __restrict__ diag,
+          const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
+          const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
+          const double* __restrict__ pold, double* __restrict__ pnew, DevScalars* __restrict__ sc,
+          double* __restrict__ partials, int max_iter, double tol) {
   if (sc->converged) return;
   constexpr int kWaves = kIterBlock / 64;
   __shared__ double s_d2[kWaves], s_n2[kWaves];
   __shared__ int s_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gw = blockIdx.x * kWaves + wave, nw = gridDim.x * kWaves;
-  const double scale = sc->scale, inv = sc->inv_scale;
+  const double inv = sc->inv_scale;
   double d2 = 0.0, n2 = 0.0;
-  for (int row = gw; row < nS; row += nw) {
-    long long acc = 0;
+  // One lane per row: every lane keeps four gathers in flight and sums its
+  // row's sorted entries in order.
+  for (int row = blockIdx.x * kIterBlock + threadIdx.x; row < nS; row += gridDim.x * kIterBlock) {
     const unsigned long long b = rowptr[row], e = rowptr[row + 1];
-    for (unsigned long long j = b + lane; j < e; j += 64) acc += to_fixed(pold[col[j]], scale);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) {
-      const int v = slist[row];
-      const unsigned int c = cnt[v];
-      long long tot = acc + bs[row];
-      if (k == 1 && seed && pinit) tot += to_fixed(pinit[v], scale);
-      const double po = pold[row];
-      const double pn = c > 0 ? (double)tot * inv / (double)c : po;
-      pnew[row] = pn;
-      const double df = pn - po;
-      d2 += df * df;
-      n2 += pn * pn;
+    const int v = slist[row];
+    const unsigned int c = cnt[v];
+    const double po = pold[row];
+    const double bsv = (double)bs[row] * inv;
+    const double dg = (double)diag[row];
+    const double pi0 = (k == 1 && seed && pinit) ? pinit[v] : 0.0;
+    double acc = 0.0;
+    unsigned long long j = b;
+    for (; j + 4 <= e; j += 4) {
+      const int c0 = col[j], c1 = col[j + 1], c2 = col[j + 2], c3 = col[j + 3];
+      const double x0 = pold[c0], x1 = pold[c1], x2 = pold[c2], x3 = pold[c3];
+      acc += x0;
+      acc += x1;
+      acc += x2;
+      acc += x3;
     }
+    for (; j < e; j++) acc += pold[col[j]];
+    double tot = bsv + dg * po + acc;
+    tot += pi0;
+    const double pn = c > 0 ? tot / (double)c : po;
+    pnew[row] = pn;
+    const double df = pn - po;
+    d2 += df * df;
+    n2 += pn * pn;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
   }
   if (lane == 0) {
     s_d2[wave] = d2;
@@ -122,12 +138,31 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
     s_last = (t == gridDim.x - 1);
   }
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    double a = 0.0, b = 0.0;
-    for (unsigned int i = 0; i < gridDim.x; i++) {
-      a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (!s_last) return;
+  // Last block: every thread sums a fixed strided subset of the partials,
+  // then a fixed-order tree over the waves (deterministic).
+  __threadfence();
+  double a = 0.0, b = 0.0;
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += kIterBlock) {
+    a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_d2[wave] = a;
+    s_n2[wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0.0;
+    b = 0.0;
+    for (int w = 0; w < kWaves; w++) {
+      a += s_d2[w];
+      b += s_n2[w];
     }
     const double num = sqrt(a), den = sqrt(b);
     const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
@@ -143,13 +178,82 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
   }
 }
 
+// Sorts the columns of every CSR row (plan build), so that the iteration's
+// FP64 row sums do not depend on the order the fill kernel's atomics gave.
+// One block per row: bitonic sort in LDS for rows up to 4096 entries, a
+// serial insertion sort beyond.
+constexpr int kSortCap = 4096;
+
+__global__ void __launch_bounds__(256)
+k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr, int* __restrict__ col) {
+  __shared__ int s[kSortCap];
+  for (int row = blockIdx.x; row < nS; row += gridDim.x) {
+    const unsigned long long b = rowptr[row];
+    const int n = (int)(rowptr[row + 1] - b);
+    if (n <= 1) continue;
+    if (n > kSortCap) {
+      if (threadIdx.x == 0) {
+        for (int i = 1; i < n; i++) {
+          const int x = col[b + i];
+          int j = i - 1;
+          while (j >= 0 && col[b + j] > x) {
+            col[b + j + 1] = col[b + j];
+            j--;
+          }
+          col[b + j + 1] = x;
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    int p = 1;
+    while (p < n) p <<= 1;
+    for (int i = threadIdx.x; i < p; i += blockDim.x) s[i] = i < n ? col[b + i] : 0x7fffffff;
+    __syncthreads();
+    for (int kk = 2; kk <= p; kk <<= 1) {
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int i = threadIdx.x; i < p; i += blockDim.x) {
+          const int l = i ^ jj;
+          if (l > i) {
+            const int x = s[i], y = s[l];
+            const bool up = (i & kk) == 0;
+            if ((x > y) == up) {
+              s[i] = y;
+              s[l] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) col[b + i] = s[i];
+    __syncthreads();
+  }
+}
+
+// Stages three component arrays as one 4-vector per voxel.
+template <typename S, typename V4>
+__global__ void __launch_bounds__(256)
+k_pack_field(const S* __restrict__ ax, const S* __restrict__ ay, const S* __restrict__ az,
+             V4* __restrict__ out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    V4 v;
+    v.x = ax[i];
+    v.y = ay[i];
+    v.z = az[i];
+    v.w = 0;
+    out[i] = v;
+  }
+}
+
 // Averaging of the final pass (devidecountInner, kernel.cu:6746-6759): where
 // C > 0, p = sum / C; elsewhere p keeps its initial value.
 __global__ void __launch_bounds__(256)
-k_finalize(int64_t V, const long long* __restrict__ sum, const unsigned int* __restrict__ cnt,
-           const double* __restrict__ pinit, int seed, const DevScalars* __restrict__ sc,
-           double* __restrict__ pd, float* __restrict__ pf, double* __restrict__ wd,
-           float* __restrict__ wf) {
+k_finalize(int64_t V, const long long* __restrict__ sum, int ncopies,
+           const unsigned int* __restrict__ cnt, const double* __restrict__ pinit, int seed,
+           const DevScalars* __restrict__ sc, double* __restrict__ pd, float* __restrict__ pf,
+           double* __restrict__ wd, float* __restrict__ wf) {
   const double inv = sc->inv_scale, scale = sc->scale;
   const bool add_seed = seed && pinit && sc->kstar == 1;
   for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
@@ -158,7 +262,8 @@ k_finalize(int64_t V, const long long* __restrict__ sum, const unsigned int* __r
     const double p0 = pinit ? pinit[v] : 0.0;
     double p = p0;
     if (c > 0) {
-      long long tot = sum[v];
+      long long tot = 0;
+      for (int q = 0; q < ncopies; q++) tot += sum[(int64_t)q * V + v];
       if (add_seed) tot += to_fixed(p0, scale);
       p = (double)tot * inv / (double)c;
     }
@@ -307,29 +412,47 @@ hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, 
   return hipGetLastError();
 }
 
-unsigned int iterate_grid(int nS) {
-  // one wavefront per row, 4 rows in flight per block
-  int64_t b = ((int64_t)nS + 3) / 4;
-  if (b > 8192) b = 8192;
-  if (b < 1) b = 1;
-  return (unsigned int)b;
-}
+unsigned int iterate_grid(int nS) { return grid_for(nS, 4096); }
 
 hipError_t launch_iterate(int k, int nS, const unsigned long long* rowptr, const int* col,
-                          const int* slist, const unsigned int* cnt, const long long* bs,
-                          const double* pinit, int seed, const double* pold, double* pnew,
-                          DevScalars* sc, double* partials, int max_iter, double tol,
-                          hipStream_t st) {
+                          const unsigned int* diag, const int* slist, const unsigned int* cnt,
+                          const long long* bs, const double* pinit, int seed, const double* pold,
+                          double* pnew, DevScalars* sc, double* partials, int max_iter,
+                          double tol, hipStream_t st) {
   hipLaunchKernelGGL(k_iterate, dim3(iterate_grid(nS)), dim3(kIterBlock), 0, st, k, nS, rowptr,
-                     col, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials, max_iter, tol);
+                     col, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials, max_iter,
+                     tol);
   return hipGetLastError();
 }
 
-hipError_t launch_finalize(int64_t V, const long long* sum, const unsigned int* cnt,
+hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, int* col, hipStream_t st) {
+  hipLaunchKernelGGL(k_sort_rows, dim3(grid_for((int64_t)nS * 256, 16384)), dim3(256), 0, st, nS,
+                     rowptr, col);
+  return hipGetLastError();
+}
+
+template <typename S, typename T>
+hipError_t launch_pack_field(const S* ax, const S* ay, const S* az, void* out, int64_t n,
+                             hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
+  hipLaunchKernelGGL((k_pack_field<S, V4>), dim3(grid_for(n)), dim3(256), 0, st, ax, ay, az,
+                     static_cast<V4*>(out), n);
+  return hipGetLastError();
+}
+template hipError_t launch_pack_field<float, float>(const float*, const float*, const float*,
+                                                    void*, int64_t, hipStream_t);
+template hipError_t launch_pack_field<double, float>(const double*, const double*, const double*,
+                                                     void*, int64_t, hipStream_t);
+template hipError_t launch_pack_field<float, double>(const float*, const float*, const float*,
+                                                     void*, int64_t, hipStream_t);
+template hipError_t launch_pack_field<double, double>(const double*, const double*,
+                                                      const double*, void*, int64_t, hipStream_t);
+
+hipError_t launch_finalize(int64_t V, const long long* sum, int ncopies, const unsigned int* cnt,
                            const double* pinit, int seed, const DevScalars* sc, double* pd,
                            float* pf, double* wd, float* wf, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(V)), dim3(256), 0, st, V, sum, cnt, pinit, seed,
-                     sc, pd, pf, wd, wf);
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(V)), dim3(256), 0, st, V, sum, ncopies, cnt, pinit,
+                     seed, sc, pd, pf, wd, wf);
   return hipGetLastError();
 }
 

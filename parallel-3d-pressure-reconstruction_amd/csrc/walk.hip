@@ -22,11 +22,32 @@ namespace omni3d {
 constexpr int kEnumBlock = 256;
 constexpr int kEnumPerThread = 4;
 
+// Candidate order: direction, then 16x16 patches of the lattice, then 8x8
+// sub-patches (one wavefront each), row-major inside.  Neighbouring lanes then
+// walk neighbouring parallel lines, which share cache lines of the field.
+__device__ __forceinline__ bool candidate_point(int64_t cand, int ng, int64_t per_dir, int* dir,
+                                                int64_t* pt) {
+  const int side = (ng + 15) >> 4;
+  *dir = (int)(cand / per_dir);
+  const int64_t r = cand - (int64_t)(*dir) * per_dir;
+  const int64_t patch = r >> 8;
+  const int l = (int)(r & 255);
+  const int pm = (int)(patch / side), pn = (int)(patch % side);
+  const int sub = l >> 6, q = l & 63;
+  const int m = pm * 16 + (sub >> 1) * 8 + (q >> 3);
+  const int n = pn * 16 + (sub & 1) * 8 + (q & 7);
+  if (m >= ng || n >= ng) return false;
+  *pt = (int64_t)m * ng + n;
+  return true;
+}
+
 __device__ __forceinline__ bool candidate_ray(const DirConst* __restrict__ dirs, GridDesc g,
-                                              int64_t cand, int64_t npts, uint32_t* in_p,
-                                              uint32_t* out_p, uint32_t* len) {
-  const int dir = (int)(cand / npts);
-  const int64_t pt = cand - (int64_t)dir * npts;
+                                              int64_t cand, int64_t per_dir, uint32_t* in_p,
+                                              uint32_t* out_p, uint32_t* len, int* dir_out,
+                                              int64_t* pt_out) {
+  int dir;
+  int64_t pt;
+  if (!candidate_point(cand, g.ng, per_dir, &dir, &pt)) return false;
   const DirConst d = dirs[dir];
   double o[3];
   lattice_origin(d, g.ng, g.ls, pt, o);
@@ -35,6 +56,8 @@ __device__ __forceinline__ bool candidate_ray(const DirConst* __restrict__ dirs,
   *in_p = pack_voxel(in[0], in[1], in[2]);
   *out_p = pack_voxel(out[0], out[1], out[2]);
   *len = (uint32_t)(abs(out[0] - in[0]) + abs(out[1] - in[1]) + abs(out[2] - in[2]));
+  *dir_out = dir;
+  *pt_out = pt;
   return true;
 }
 
@@ -49,7 +72,9 @@ k_enum_count(const DirConst* __restrict__ dirs, GridDesc g, int64_t npts, int64_
   for (int r = 0; r < kEnumPerThread; r++) {
     const int64_t c = base + (int64_t)r * kEnumBlock + threadIdx.x;
     uint32_t a, b, l;
-    if (c < ncand && candidate_ray(dirs, g, c, npts, &a, &b, &l)) mine++;
+    int dir;
+    int64_t pt;
+    if (c < ncand && candidate_ray(dirs, g, c, npts, &a, &b, &l, &dir, &pt)) mine++;
   }
   atomicAdd(&s_cnt, mine);
   __syncthreads();
@@ -69,7 +94,9 @@ k_enum_write(const DirConst* __restrict__ dirs, GridDesc g, int64_t npts, int64_
   for (int r = 0; r < kEnumPerThread; r++) {
     const int64_t c = base + (int64_t)r * kEnumBlock + threadIdx.x;
     uint32_t a = 0, b = 0, l = 0;
-    const bool ok = c < ncand && candidate_ray(dirs, g, c, npts, &a, &b, &l);
+    int dir = 0;
+    int64_t pt = 0;
+    const bool ok = c < ncand && candidate_ray(dirs, g, c, npts, &a, &b, &l, &dir, &pt);
     const unsigned long long m = __ballot(ok);
     const unsigned int below = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) s_wave[wave] = (unsigned int)__popcll(m);
@@ -80,11 +107,10 @@ k_enum_write(const DirConst* __restrict__ dirs, GridDesc g, int64_t npts, int64_
       total += s_wave[w];
     }
     if (ok) {
-      const int dir = (int)(c / npts);
       RayRec rr;
       rr.in = a;
       rr.out = b;
-      rr.point = (uint32_t)(c - (int64_t)dir * npts);
+      rr.point = (uint32_t)pt;
       rr.dir_len = (uint32_t)dir | (l << 16);
       rays[offset + wbase + below] = rr;
     }
@@ -122,10 +148,13 @@ k_plan_counts(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __
   if ((threadIdx.x & 63) == 0) atomicAdd(steps_total, steps);
 }
 
+// The entry visit of a plain ray reads its own source voxel: it is counted in
+// the per-row diagonal instead of the CSR.
 struct RowCountVisitor {
   const int* sid;
   unsigned int* rowcnt;
-  __device__ __forceinline__ void entry(int64_t v) { atomicAdd(&rowcnt[sid[v]], 1u); }
+  unsigned int* diag;
+  __device__ __forceinline__ void entry(int64_t v) { atomicAdd(&diag[sid[v]], 1u); }
   __device__ __forceinline__ void visit(int64_t v, bool surface) {
     if (!surface) return;
     const int s = sid[v];
@@ -135,10 +164,11 @@ struct RowCountVisitor {
 
 __global__ void __launch_bounds__(256)
 k_plan_rowcount(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-                GridDesc g, const int* __restrict__ sid, unsigned int* __restrict__ rowcnt) {
+                GridDesc g, const int* __restrict__ sid, unsigned int* __restrict__ rowcnt,
+                unsigned int* __restrict__ diag) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
        r += (int64_t)gridDim.x * blockDim.x) {
-    RowCountVisitor vis{sid, rowcnt};
+    RowCountVisitor vis{sid, rowcnt, diag};
     walk_geometry(rays[r], dirs, g, vis);
   }
 }
@@ -153,7 +183,7 @@ struct RowFillVisitor {
     const unsigned int pos = atomicAdd(&cursor[s], 1u);
     col[rowptr[s] + pos] = src_id;
   }
-  __device__ __forceinline__ void entry(int64_t v) { put(sid[v]); }
+  __device__ __forceinline__ void entry(int64_t) {}
   __device__ __forceinline__ void visit(int64_t v, bool surface) {
     if (!surface) return;
     const int s = sid[v];
@@ -185,23 +215,24 @@ k_plan_rowfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* _
 template <typename T, bool kFull>
 __global__ void __launch_bounds__(256)
 k_walk_plain(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-             GridDesc g, WalkParams wp, const T* __restrict__ ax, const T* __restrict__ ay,
-             const T* __restrict__ az, const int* __restrict__ sid,
-             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
-             const DevScalars* __restrict__ sc, long long* __restrict__ acc) {
+             GridDesc g, WalkParams wp, const typename Vec4<T>::type* __restrict__ field,
+             const int* __restrict__ sid, const double* __restrict__ pbuf0,
+             const double* __restrict__ pbuf1, const DevScalars* __restrict__ sc,
+             long long* __restrict__ acc, int64_t copy_stride, int ncopies) {
   const double scale = sc->scale;
   // Pass B reads p^{k*-1} from the buffer the iteration left it in.
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    const RayRec rr = rays[r];
-    int c[3];
-    unpack_voxel(rr.in, c);
-    const int64_t vin = (int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y;
-    const double pin = kFull ? ps[sid[vin]] : 0.0;
-    PlainAccum<T, kFull> vis{acc, sid, scale, pin};
-    walk_gradient<T>(rr, dirs, g, wp, ax, ay, az, vis);
-  }
+  // private accumulator copy of this block's XCD (summed by the averaging kernel)
+  long long* __restrict__ mine = acc + (int64_t)(blockIdx.x % (unsigned)ncopies) * copy_stride;
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const RayRec rr = rays[r];
+  int c[3];
+  unpack_voxel(rr.in, c);
+  const int64_t vin = (int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y;
+  const double pin = kFull ? ps[sid[vin]] : 0.0;
+  PlainAccum<kFull> vis{mine, sid, scale, pin};
+  walk_gradient<T>(rr, dirs, g, wp, field, vis);
 }
 
 // ---------------------------------------------------------------------------
@@ -238,9 +269,9 @@ hipError_t launch_plan_counts(const RayRec* rays, int64_t nrays, const DirConst*
 
 hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                 GridDesc g, const int* sid, unsigned int* rowcnt,
-                                hipStream_t st) {
+                                unsigned int* diag, hipStream_t st) {
   hipLaunchKernelGGL(k_plan_rowcount, dim3(grid_for(nrays)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, sid, rowcnt);
+                     dirs, g, sid, rowcnt, diag);
   return hipGetLastError();
 }
 
@@ -254,28 +285,30 @@ hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst
 
 template <typename T>
 hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                             GridDesc g, const WalkParams& wp, const T* ax, const T* ay,
-                             const T* az, const int* sid, const double* p0, const double* p1,
-                             const DevScalars* sc, long long* acc, hipStream_t st) {
+                             GridDesc g, const WalkParams& wp, const void* field, const int* sid,
+                             const double* p0, const double* p1, const DevScalars* sc,
+                             long long* acc, int64_t copy_stride, int ncopies, hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
   const unsigned int grid = grid_for(nrays, 1u << 30);
+  const V4* f = static_cast<const V4*>(field);
   if (full)
     hipLaunchKernelGGL((k_walk_plain<T, true>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs,
-                       g, wp, ax, ay, az, sid, p0, p1, sc, acc);
+                       g, wp, f, sid, p0, p1, sc, acc, copy_stride, ncopies);
   else
     hipLaunchKernelGGL((k_walk_plain<T, false>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs,
-                       g, wp, ax, ay, az, sid, p0, p1, sc, acc);
+                       g, wp, f, sid, p0, p1, sc, acc, copy_stride, ncopies);
   return hipGetLastError();
 }
 
 template hipError_t launch_walk_plain<float>(bool, const RayRec*, int64_t, const DirConst*,
-                                             GridDesc, const WalkParams&, const float*,
-                                             const float*, const float*, const int*,
-                                             const double*, const double*, const DevScalars*,
-                                             long long*, hipStream_t);
+                                             GridDesc, const WalkParams&, const void*,
+                                             const int*, const double*, const double*,
+                                             const DevScalars*, long long*, int64_t, int,
+                                             hipStream_t);
 template hipError_t launch_walk_plain<double>(bool, const RayRec*, int64_t, const DirConst*,
-                                              GridDesc, const WalkParams&, const double*,
-                                              const double*, const double*, const int*,
-                                              const double*, const double*, const DevScalars*,
-                                              long long*, hipStream_t);
+                                              GridDesc, const WalkParams&, const void*,
+                                              const int*, const double*, const double*,
+                                              const DevScalars*, long long*, int64_t, int,
+                                              hipStream_t);
 
 }  // namespace omni3d

@@ -155,36 +155,65 @@ __device__ __forceinline__ float trapezoid<float>(const WalkParams& wp, int a, i
   return (float)hs * (an + al);
 }
 
+// Field staged as one 4-vector per voxel (a_x, a_y, a_z, 0): a step loads the
+// next voxel once and the trapezoid's "last" value comes from registers.
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+
+template <typename V4>
+__device__ __forceinline__ auto component(const V4& v, int a) -> decltype(v.x) {
+  return a == 0 ? v.x : (a == 1 ? v.y : v.z);
+}
+
 // Gradient walk of the plain method (kernel.cu:1743-1863).  vis.entry(v) for
 // the entry self-add, vis.visit(v, surface, pint) after every step with the
 // running trapezoid sum (kernel.cu:1816-1838).
 template <typename T, typename Visitor>
 __device__ __forceinline__ int walk_gradient(const RayRec& rr, const DirConst* __restrict__ dirs,
                                              const GridDesc& g, const WalkParams& wp,
-                                             const T* __restrict__ ax, const T* __restrict__ ay,
-                                             const T* __restrict__ az, Visitor& vis) {
+                                             const typename Vec4<T>::type* __restrict__ field,
+                                             Visitor& vis) {
+  using V4 = typename Vec4<T>::type;
   Walker w;
   w.init(rr, dirs, g);
   vis.entry(w.v);
+  V4 cur = field[w.v];
   RaySum<T> pint;
   int steps = 0;
   while (!w.done()) {
-    const int64_t vl = w.v;
     int s;
     const int a = w.advance(&s);
     if (a < 0) break;
-    const T* __restrict__ A = a == 0 ? ax : (a == 1 ? ay : az);
-    pint.add(trapezoid<T>(wp, a, s, A[w.v], A[vl]));
+    const V4 nxt = field[w.v];
+    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
     vis.visit(w.v, w.on_surface(), pint.value());
+    cur = nxt;
     steps++;
   }
   return steps;
 }
 
+// Logical block index that gives each XCD a contiguous range of blocks
+// (hardware dispatches workgroups round-robin over the 8 XCDs), so that
+// neighbouring ray patches share an L2.
+__device__ __forceinline__ unsigned int xcd_block() {
+  const unsigned int nb = gridDim.x, b = blockIdx.x;
+  const unsigned int x = b & 7u, q = nb >> 3, r = nb & 7u;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // Visitor of the plain reconstruction walks.  kFull: every visit adds
 // fix(p_src + pint) into acc[v] (and fix(p_src) at the entry).  !kFull: visits
 // to source voxels add fix(pint) into acc[sid[v]].
-template <typename T, bool kFull>
+template <bool kFull>
 struct PlainAccum {
   long long* acc;
   const int* sid;

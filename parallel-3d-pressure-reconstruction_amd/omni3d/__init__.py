@@ -46,7 +46,8 @@ class Report(C.Structure):
     _fields_ = [
         ("iterations", C.c_int), ("rel_change", C.c_double), ("rays", C.c_int64),
         ("steps", C.c_int64), ("sources", C.c_int64), ("status", C.c_int),
-        ("ms_plan", C.c_double), ("ms_solve", C.c_double),
+        ("ms_plan", C.c_double), ("ms_solve", C.c_double), ("ms_pass_a", C.c_double),
+        ("ms_iterate", C.c_double), ("ms_pass_b", C.c_double), ("ms_finalize", C.c_double),
     ]
 
 
