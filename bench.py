@@ -171,6 +171,8 @@ def main():
                    "grid": [n, n, n], "directions": D, "iterations": reps[-1].iterations,
                    "rel_change": reps[-1].rel_change, "rays": reps[-1].rays,
                    "steps_per_pass": reps[-1].steps, "sources": reps[-1].sources,
+                   "source_visits": reps[-1].source_visits, "segments": reps[-1].segments,
+                   "sell_cells": reps[-1].sell_cells,
                    "plan_s": round(rep_cold.ms_plan * 1e-3, 4), "cold_s": round(cold_s, 4),
                    "ms_pass_a": round(float(np.mean([x.ms_pass_a for x in reps])), 4),
                    "ms_iterate": round(float(np.mean([x.ms_iterate for x in reps])), 4),

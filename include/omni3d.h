@@ -97,6 +97,9 @@ typedef struct omni3d_report {
   double ms_iterate;       /* source-voxel iterations (or the naive passes) */
   double ms_pass_b;        /* final traversal into every voxel */
   double ms_finalize;      /* averaging into the output */
+  int64_t source_visits;   /* visits landing on source voxels (iteration matrix entries) */
+  int64_t segments;        /* ray segments of the tile engine (one per ray and tile) */
+  int64_t sell_cells;      /* stored cells of the iteration matrix incl. padding */
 } omni3d_report;
 
 typedef struct omni3d_ctx omni3d_ctx;

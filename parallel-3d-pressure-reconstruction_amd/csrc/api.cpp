@@ -71,6 +71,45 @@ bool params_ok(const omni3d_params& p, std::string* why) {
 
 size_t elem_size(int dtype) { return dtype == OMNI3D_F64 ? 8 : 4; }
 
+// Source numbering for the iteration's gathers: the voxels flagged as ray
+// sources, grouped by box face (the first face a voxel lies on, in the order
+// z=0, z=max, y=0, y=max, x=0, x=max), then by 8x8 block of the face, then
+// row-major inside the block; sources off the surface (none for the plain
+// method) follow in voxel order.
+void number_sources(int X, int Y, int Z, const std::vector<uint8_t>& flag, std::vector<int>* sid,
+                    std::vector<int>* slist) {
+  const int64_t V = (int64_t)X * Y * Z;
+  std::vector<std::pair<unsigned long long, int>> keys;
+  for (int64_t v = 0; v < V; v++) {
+    if (!flag[v]) continue;
+    const int i = (int)(v % X), j = (int)((v / X) % Y), k = (int)(v / ((int64_t)X * Y));
+    int face, u, w, nu;
+    if (k == 0) { face = 0; u = i; w = j; nu = X; }
+    else if (k == Z - 1) { face = 1; u = i; w = j; nu = X; }
+    else if (j == 0) { face = 2; u = i; w = k; nu = X; }
+    else if (j == Y - 1) { face = 3; u = i; w = k; nu = X; }
+    else if (i == 0) { face = 4; u = j; w = k; nu = Y; }
+    else if (i == X - 1) { face = 5; u = j; w = k; nu = Y; }
+    else { face = 6; u = 0; w = 0; nu = 1; }
+    unsigned long long key;
+    if (face < 6) {
+      const unsigned long long blk = (unsigned long long)(w >> 3) * (unsigned long long)((nu + 7) >> 3) +
+                                     (unsigned long long)(u >> 3);
+      key = ((unsigned long long)face << 56) | (blk << 6) | (unsigned long long)(((w & 7) << 3) | (u & 7));
+    } else {
+      key = (6ull << 56) | (unsigned long long)v;
+    }
+    keys.emplace_back(key, (int)v);
+  }
+  std::sort(keys.begin(), keys.end());
+  sid->assign(V, -1);
+  slist->resize(keys.size());
+  for (size_t s = 0; s < keys.size(); s++) {
+    (*slist)[s] = keys[s].second;
+    (*sid)[keys[s].second] = (int)s;
+  }
+}
+
 // Tile engine part of the plan: compiled step codes and per-tile segments.
 int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   Plan& pl = ctx->plan;
@@ -80,44 +119,34 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   tg.nty = (g.Y + kTile - 1) / kTile;
   tg.ntz = (g.Z + kTile - 1) / kTile;
   const int nt = tg.ntx * tg.nty * tg.ntz;
-  DevBuf nsteps, nseg, tcnt, tmp;
-  OMNI_TRY_HIP(ctx, nsteps.alloc(sizeof(unsigned int) * nrays));
+  const int nslots = nt * kLenBuckets;
+  DevBuf nseg, tcnt, tmp;
   OMNI_TRY_HIP(ctx, nseg.alloc(sizeof(unsigned int) * nrays));
-  OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * nt));
-  OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(nrays, nt))));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nt, st));
+  OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * nslots));
+  OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(nrays, nslots))));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
   OMNI_TRY_HIP(ctx, launch_tile_count(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
-                                      nsteps.as<unsigned int>(), nseg.as<unsigned int>(),
-                                      tcnt.as<unsigned int>(), st));
-  OMNI_TRY_HIP(ctx, pl.word_off.alloc(sizeof(unsigned long long) * (nrays + 1)));
+                                      nseg.as<unsigned int>(), tcnt.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, pl.seg_base.alloc(sizeof(unsigned long long) * (nrays + 1)));
-  OMNI_TRY_HIP(ctx, pl.tile_ptr.alloc(sizeof(unsigned long long) * (nt + 1)));
-  OMNI_TRY_HIP(ctx, launch_scan_u32(nsteps.as<unsigned int>(), nrays,
-                                    pl.word_off.as<unsigned long long>(), 1, tmp.as<void>(),
-                                    tmp.size(), st));
+  OMNI_TRY_HIP(ctx, pl.tile_ptr.alloc(sizeof(unsigned long long) * (nslots + 1)));
   OMNI_TRY_HIP(ctx, launch_scan_u32(nseg.as<unsigned int>(), nrays,
                                     pl.seg_base.as<unsigned long long>(), 0, tmp.as<void>(),
                                     tmp.size(), st));
-  OMNI_TRY_HIP(ctx, launch_scan_u32(tcnt.as<unsigned int>(), nt,
+  OMNI_TRY_HIP(ctx, launch_scan_u32(tcnt.as<unsigned int>(), nslots,
                                     pl.tile_ptr.as<unsigned long long>(), 0, tmp.as<void>(),
                                     tmp.size(), st));
-  unsigned long long tot[2] = {0, 0};
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&tot[0], pl.word_off.as<unsigned long long>() + nrays, 8,
-                                   hipMemcpyDeviceToHost, st));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&tot[1], pl.seg_base.as<unsigned long long>() + nrays, 8,
+  unsigned long long nsegs = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsegs, pl.seg_base.as<unsigned long long>() + nrays, 8,
                                    hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  if (tot[1] >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
-  OMNI_TRY_HIP(ctx, pl.words.alloc(sizeof(uint32_t) * (tot[0] + 1)));
-  OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(tot[1], 1)));
-  OMNI_TRY_HIP(ctx, pl.segval.alloc(sizeof(double) * std::max<unsigned long long>(tot[1], 1)));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.words.as<void>(), 0, sizeof(uint32_t) * (tot[0] + 1), st));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nt, st));
+  if (nsegs >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
+  OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
+  OMNI_TRY_HIP(ctx, pl.segval.alloc(sizeof(double) * std::max<unsigned long long>(nsegs, 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
   OMNI_TRY_HIP(ctx, launch_tile_fill(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
-                                     pl.word_off.as<unsigned long long>(),
-                                     pl.seg_base.as<unsigned long long>(),
+                                     pl.sid.as<int>(), pl.seg_base.as<unsigned long long>(),
                                      pl.tile_ptr.as<unsigned long long>(), tcnt.as<unsigned int>(),
-                                     pl.words.as<uint32_t>(), pl.segs.as<SegRec>(), st));
+                                     pl.segs.as<SegRec>(), st));
   // tiles that touch the box surface hold every source voxel of the plain method
   std::vector<int> surf;
   for (int t = 0; t < nt; t++) {
@@ -132,8 +161,7 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   pl.tg = tg;
   pl.ntiles = nt;
   pl.nsurf = (int)surf.size();
-  pl.nwords = (int64_t)tot[0];
-  pl.nsegs = (int64_t)tot[1];
+  pl.nsegs = (int64_t)nsegs;
   return OMNI3D_OK;
 }
 
@@ -198,21 +226,26 @@ int build_plan(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, launch_plan_counts(pl.rays.as<RayRec>(), (int64_t)nrays, pl.dirs.as<DirConst>(),
                                        g, pl.cnt.as<unsigned int>(), flag.as<uint8_t>(),
                                        steps.as<unsigned long long>(), st));
-  OMNI_TRY_HIP(ctx, pl.sid.alloc(sizeof(int) * V));
-  OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * V));
-  DevBuf ns;
-  OMNI_TRY_HIP(ctx, ns.alloc(sizeof(int)));
-  OMNI_TRY_HIP(ctx, launch_source_ids(flag.as<uint8_t>(), V, pl.sid.as<int>(), pl.slist.as<int>(),
-                                      ns.as<int>(), st));
   OMNI_TRY_HIP(ctx, launch_max_u32(pl.cnt.as<unsigned int>(), V, mx.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, launch_max_len(pl.rays.as<RayRec>(), (int64_t)nrays, mx.as<unsigned int>() + 1, st));
-  int nS = 0;
   unsigned int mxh[2] = {0, 0};
   unsigned long long nsteps = 0;
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, ns.as<int>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  std::vector<uint8_t> hflag(V);
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hflag.data(), flag.as<void>(), V, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(mxh, mx.as<unsigned int>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsteps, steps.as<void>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  // source numbering: 8x8 blocks of each box face, so that consecutive
+  // source ids are neighbouring voxels in both face directions
+  std::vector<int> hsid, hslist;
+  number_sources(X, Y, Z, hflag, &hsid, &hslist);
+  const int nS = (int)hslist.size();
+  OMNI_TRY_HIP(ctx, pl.sid.alloc(sizeof(int) * V));
+  OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * std::max(nS, 1)));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.sid.as<void>(), hsid.data(), sizeof(int) * V,
+                                   hipMemcpyHostToDevice, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.slist.as<void>(), hslist.data(), sizeof(int) * nS,
+                                   hipMemcpyHostToDevice, st));
   // CSR of visits landing on source voxels
   DevBuf rowcnt;
   OMNI_TRY_HIP(ctx, rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
@@ -229,13 +262,38 @@ int build_plan(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, pl.rowptr.as<unsigned long long>() + nS, 8,
                                    hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  OMNI_TRY_HIP(ctx, pl.col.alloc(sizeof(int) * (nnz + 1)));
+  DevBuf key;
+  OMNI_TRY_HIP(ctx, key.alloc(sizeof(unsigned long long) * (nnz + 1)));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
   OMNI_TRY_HIP(ctx, launch_plan_rowfill(pl.rays.as<RayRec>(), (int64_t)nrays,
                                         pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
                                         pl.rowptr.as<unsigned long long>(),
-                                        rowcnt.as<unsigned int>(), pl.col.as<int>(), st));
-  OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.rowptr.as<unsigned long long>(), pl.col.as<int>(), st));
+                                        rowcnt.as<unsigned int>(), key.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.rowptr.as<unsigned long long>(),
+                                     key.as<unsigned long long>(), st));
+  {
+    // masked sliced-ELL copy of the sorted rows for the iteration
+    if (D > 4096) return fail(ctx, OMNI3D_ERR_ARG, "at most 4096 directions");
+    const int nch = (nS + 63) / 64;
+    DevBuf nslot, tmp;
+    OMNI_TRY_HIP(ctx, nslot.alloc(sizeof(unsigned int) * (nch + 1)));
+    OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(nch)));
+    OMNI_TRY_HIP(ctx, pl.moff.alloc(sizeof(unsigned long long) * (nch + 1)));
+    OMNI_TRY_HIP(ctx, launch_sell_layout(nS, D, pl.rowptr.as<unsigned long long>(),
+                                         key.as<unsigned long long>(), nslot.as<unsigned int>(), st));
+    OMNI_TRY_HIP(ctx, launch_scan_u32(nslot.as<unsigned int>(), nch, pl.moff.as<unsigned long long>(),
+                                      0, tmp.as<void>(), tmp.size(), st));
+    unsigned long long nm = 0;
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nm, pl.moff.as<unsigned long long>() + nch, 8,
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    OMNI_TRY_HIP(ctx, pl.masks.alloc(sizeof(unsigned long long) * (nm + 1)));
+    OMNI_TRY_HIP(ctx, pl.ent.alloc(sizeof(int) * (nnz + 1)));
+    OMNI_TRY_HIP(ctx, launch_sell_fill(nS, D, pl.rowptr.as<unsigned long long>(),
+                                       key.as<unsigned long long>(), pl.moff.as<unsigned long long>(),
+                                       pl.masks.as<unsigned long long>(), pl.ent.as<int>(), st));
+    pl.ncell = (int64_t)nm;
+  }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   int rc = build_tiles(ctx, g, (int64_t)nrays);
   if (rc) return rc;
@@ -298,18 +356,15 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     // segment sums over all tiles, per-ray carries, then the source sums on
     // the surface tiles
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
-                                          pl.segs.as<SegRec>(), pl.rays.as<RayRec>(),
-                                          pl.word_off.as<unsigned long long>(), pl.words.as<uint32_t>(),
-                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), nullptr, nullptr,
-                                          sc, pl.segval.as<double>(), nullptr, st));
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
+                                          pl.sid.as<int>(), nullptr, nullptr, sc,
+                                          pl.segval.as<double>(), nullptr, st));
     OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.seg_base.as<unsigned long long>(),
                                         pl.segval.as<double>(), st));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, pl.nsurf, pl.surf.as<int>(),
                                           pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(),
-                                          pl.rays.as<RayRec>(), pl.word_off.as<unsigned long long>(),
-                                          pl.words.as<uint32_t>(), pl.g, pl.tg, wp, field,
-                                          pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          pl.segval.as<double>(), ctx->bs.as<long long>(), st));
+                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), nullptr, nullptr,
+                                          sc, pl.segval.as<double>(), ctx->bs.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
@@ -323,7 +378,9 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
   for (int k = 1; k <= P.max_iterations; k++) {
-    OMNI_TRY_HIP(ctx, launch_iterate(k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.col.as<int>(),
+    OMNI_TRY_HIP(ctx, launch_iterate(k, pl.nS, pl.moff.as<unsigned long long>(),
+                                     pl.masks.as<unsigned long long>(),
+                                     pl.rowptr.as<unsigned long long>(), pl.ent.as<int>(),
                                      pl.diag.as<unsigned int>(), pl.slist.as<int>(),
                                      pl.cnt.as<unsigned int>(), ctx->bs.as<long long>(), pinit,
                                      seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
@@ -338,10 +395,9 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     // every voxel belongs to exactly one tile, which stores its sum
     OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
-                                          pl.segs.as<SegRec>(), pl.rays.as<RayRec>(),
-                                          pl.word_off.as<unsigned long long>(), pl.words.as<uint32_t>(),
-                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), pbuf[0], pbuf[1],
-                                          sc, pl.segval.as<double>(), ctx->sum.as<long long>(), st));
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
+                                          pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
+                                          pl.segval.as<double>(), ctx->sum.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->sum.as<void>(), 0, sizeof(long long) * V * nc, st));
     OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
@@ -606,6 +662,9 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->ms_iterate = ms(3, 4);
     rep->ms_pass_b = ms(5, 6);
     rep->ms_finalize = ms(6, 7);
+    rep->source_visits = ctx->plan.nnz;
+    rep->segments = ctx->plan.nsegs;
+    rep->sell_cells = ctx->plan.ncell;
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");
   return OMNI3D_OK;

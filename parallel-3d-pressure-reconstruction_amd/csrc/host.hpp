@@ -69,17 +69,17 @@ struct Plan {
   DevBuf cnt;      // uint32[V] visits per voxel
   DevBuf sid;      // int32[V] compact source id or -1
   DevBuf slist;    // int32[nS] voxel of each source
-  DevBuf rowptr;   // uint64[nS+1]
-  DevBuf col;      // int32[nnz] source id of each visit landing on a source
+  DevBuf rowptr;   // uint64[nS+1] row offsets of the visits landing on sources
   DevBuf diag;     // uint32[nS] rays entering at each source (their self visit)
+  DevBuf moff;     // uint64[nchunks+1] first slot of each 64-row chunk
+  DevBuf masks;    // uint64[ncell] lane mask of each (direction, k) slot
+  DevBuf ent;      // int32[nnz] source ids packed in slot order
+  int64_t ncell = 0;
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
   int ntiles = 0;
   int nsurf = 0;
   int64_t nsegs = 0;
-  int64_t nwords = 0;
-  DevBuf word_off;  // uint64[nrays+1]
-  DevBuf words;     // uint32[nwords+1] 2-bit step codes
   DevBuf seg_base;  // uint64[nrays+1]
   DevBuf tile_ptr;  // uint64[ntiles+1]
   DevBuf segs;      // SegRec[nsegs], grouped by tile

@@ -32,7 +32,7 @@ hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirCons
                                 unsigned int* diag, hipStream_t st);
 hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                GridDesc g, const int* sid, const unsigned long long* rowptr,
-                               unsigned int* cursor, int* col, hipStream_t st);
+                               unsigned int* cursor, unsigned long long* key, hipStream_t st);
 // field: float4 / double4 per voxel (a_x, a_y, a_z, 0); acc: ncopies accumulator
 // copies copy_stride apart (block b adds into copy b % ncopies).
 template <typename T>
@@ -49,12 +49,14 @@ hipError_t launch_compute_scale(DevScalars* sc, double gfac, double lmax, double
 hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, double* ps,
                                  hipStream_t st);
 unsigned int iterate_grid(int nS);
-hipError_t launch_iterate(int k, int nS, const unsigned long long* rowptr, const int* col,
-                          const unsigned int* diag, const int* slist, const unsigned int* cnt,
-                          const long long* bs, const double* pinit, int seed, const double* pold,
-                          double* pnew, DevScalars* sc, double* partials, int max_iter,
-                          double tol, hipStream_t st);
-hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, int* col, hipStream_t st);
+hipError_t launch_iterate(int k, int nS, const unsigned long long* moff,
+                          const unsigned long long* masks, const unsigned long long* rowptr,
+                          const int* ent, const unsigned int* diag, const int* slist,
+                          const unsigned int* cnt, const long long* bs, const double* pinit,
+                          int seed, const double* pold, double* pnew, DevScalars* sc,
+                          double* partials, int max_iter, double tol, hipStream_t st);
+hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, unsigned long long* key,
+                            hipStream_t st);
 template <typename S, typename T>
 hipError_t launch_pack_field(const S* ax, const S* ay, const S* az, void* out, int64_t n,
                              hipStream_t st);

@@ -73,8 +73,9 @@ k_gather_sources(const int* __restrict__ slist, int nS, const double* __restrict
 constexpr int kIterBlock = 256;
 
 __global__ void __launch_bounds__(kIterBlock)
-k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
-          const int* __restrict__ col, const unsigned int* __restrict__ diag,
+k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
+          const unsigned long long* __restrict__ masks, const unsigned long long* __restrict__ rowptr,
+          const int* __restrict__ ent, const unsigned int* __restrict__ diag,
           const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
           const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
           const double* __restrict__ pold, double* __restrict__ pnew, DevScalars* __restrict__ sc,
@@ -85,30 +86,54 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
   __shared__ int s_last;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const double inv = sc->inv_scale;
+  const unsigned long long below = (1ull << lane) - 1ull;
   double d2 = 0.0, n2 = 0.0;
-  // One lane per row: every lane keeps four gathers in flight and sums its
-  // row's sorted entries in order.
-  for (int row = blockIdx.x * kIterBlock + threadIdx.x; row < nS; row += gridDim.x * kIterBlock) {
-    const unsigned long long b = rowptr[row], e = rowptr[row + 1];
+  // One lane per row, one wavefront per 64-row chunk of the masked
+  // sliced-ELL layout; sixteen slots per round keep sixteen gathers in flight.
+  const int nrows = (nS + 63) & ~63;
+  for (int row = blockIdx.x * kIterBlock + threadIdx.x; row < nrows; row += gridDim.x * kIterBlock) {
+    // the chunk is uniform across the wavefront: its slot masks are scalar loads
+    const int chunk = __builtin_amdgcn_readfirstlane(row >> 6);
+    const unsigned long long m0 = moff[chunk], m1 = moff[chunk + 1];
+    unsigned long long eb = rowptr[chunk << 6];
+    double acc = 0.0;
+    unsigned long long s = m0;
+    constexpr int kU = 16;
+    for (; s + kU <= m1; s += kU) {
+      // masks first, then every entry load, then every gather: three
+      // dependent rounds per kU slots.  A lane without an entry in a slot
+      // reads the slot's first entry and adds nothing.
+      unsigned long long m[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) m[u] = masks[s + u];
+      bool has[kU];
+      unsigned long long pos[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        has[u] = (m[u] >> lane) & 1ull;
+        pos[u] = eb + (has[u] ? (unsigned long long)__popcll(m[u] & below) : 0ull);
+        eb += __popcll(m[u]);
+      }
+      int idx[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) idx[u] = ent[pos[u]];
+      double x[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) x[u] = pold[idx[u]];
+#pragma unroll
+      for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
+    }
+    for (; s < m1; s++) {
+      const unsigned long long m = masks[s];
+      if ((m >> lane) & 1ull) acc += pold[ent[eb + __popcll(m & below)]];
+      eb += __popcll(m);
+    }
+    if (row >= nS) continue;
     const int v = slist[row];
     const unsigned int c = cnt[v];
     const double po = pold[row];
-    const double bsv = (double)bs[row] * inv;
-    const double dg = (double)diag[row];
-    const double pi0 = (k == 1 && seed && pinit) ? pinit[v] : 0.0;
-    double acc = 0.0;
-    unsigned long long j = b;
-    for (; j + 4 <= e; j += 4) {
-      const int c0 = col[j], c1 = col[j + 1], c2 = col[j + 2], c3 = col[j + 3];
-      const double x0 = pold[c0], x1 = pold[c1], x2 = pold[c2], x3 = pold[c3];
-      acc += x0;
-      acc += x1;
-      acc += x2;
-      acc += x3;
-    }
-    for (; j < e; j++) acc += pold[col[j]];
-    double tot = bsv + dg * po + acc;
-    tot += pi0;
+    double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
+    if (k == 1 && seed && pinit) tot += pinit[v];
     const double pn = c > 0 ? tot / (double)c : po;
     pnew[row] = pn;
     const double df = pn - po;
@@ -178,15 +203,16 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
   }
 }
 
-// Sorts the columns of every CSR row (plan build), so that the iteration's
-// FP64 row sums do not depend on the order the fill kernel's atomics gave.
-// One block per row: bitonic sort in LDS for rows up to 4096 entries, a
-// serial insertion sort beyond.
-constexpr int kSortCap = 4096;
+// Sorts the (direction, source) keys of every CSR row (plan build), so that
+// the iteration's FP64 row sums do not depend on the order the fill kernel's
+// atomics gave.  One block per row: bitonic sort in LDS for rows up to 2048
+// entries, a serial insertion sort beyond.
+constexpr int kSortCap = 2048;
 
 __global__ void __launch_bounds__(256)
-k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr, int* __restrict__ col) {
-  __shared__ int s[kSortCap];
+k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr,
+            unsigned long long* __restrict__ key) {
+  __shared__ unsigned long long s[kSortCap];
   for (int row = blockIdx.x; row < nS; row += gridDim.x) {
     const unsigned long long b = rowptr[row];
     const int n = (int)(rowptr[row + 1] - b);
@@ -194,13 +220,13 @@ k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr, int* __restri
     if (n > kSortCap) {
       if (threadIdx.x == 0) {
         for (int i = 1; i < n; i++) {
-          const int x = col[b + i];
+          const unsigned long long x = key[b + i];
           int j = i - 1;
-          while (j >= 0 && col[b + j] > x) {
-            col[b + j + 1] = col[b + j];
+          while (j >= 0 && key[b + j] > x) {
+            key[b + j + 1] = key[b + j];
             j--;
           }
-          col[b + j + 1] = x;
+          key[b + j + 1] = x;
         }
       }
       __syncthreads();
@@ -208,14 +234,14 @@ k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr, int* __restri
     }
     int p = 1;
     while (p < n) p <<= 1;
-    for (int i = threadIdx.x; i < p; i += blockDim.x) s[i] = i < n ? col[b + i] : 0x7fffffff;
+    for (int i = threadIdx.x; i < p; i += blockDim.x) s[i] = i < n ? key[b + i] : ~0ull;
     __syncthreads();
     for (int kk = 2; kk <= p; kk <<= 1) {
       for (int jj = kk >> 1; jj > 0; jj >>= 1) {
         for (int i = threadIdx.x; i < p; i += blockDim.x) {
           const int l = i ^ jj;
           if (l > i) {
-            const int x = s[i], y = s[l];
+            const unsigned long long x = s[i], y = s[l];
             const bool up = (i & kk) == 0;
             if ((x > y) == up) {
               s[i] = y;
@@ -226,7 +252,7 @@ k_sort_rows(int nS, const unsigned long long* __restrict__ rowptr, int* __restri
         __syncthreads();
       }
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) col[b + i] = s[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) key[b + i] = s[i];
     __syncthreads();
   }
 }
@@ -412,22 +438,24 @@ hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, 
   return hipGetLastError();
 }
 
-unsigned int iterate_grid(int nS) { return grid_for(nS, 4096); }
+unsigned int iterate_grid(int nS) { return grid_for((int64_t)nS + 63, 4096); }
 
-hipError_t launch_iterate(int k, int nS, const unsigned long long* rowptr, const int* col,
-                          const unsigned int* diag, const int* slist, const unsigned int* cnt,
-                          const long long* bs, const double* pinit, int seed, const double* pold,
-                          double* pnew, DevScalars* sc, double* partials, int max_iter,
-                          double tol, hipStream_t st) {
-  hipLaunchKernelGGL(k_iterate, dim3(iterate_grid(nS)), dim3(kIterBlock), 0, st, k, nS, rowptr,
-                     col, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials, max_iter,
-                     tol);
+hipError_t launch_iterate(int k, int nS, const unsigned long long* moff,
+                          const unsigned long long* masks, const unsigned long long* rowptr,
+                          const int* ent, const unsigned int* diag, const int* slist,
+                          const unsigned int* cnt, const long long* bs, const double* pinit,
+                          int seed, const double* pold, double* pnew, DevScalars* sc,
+                          double* partials, int max_iter, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_iterate, dim3(iterate_grid(nS)), dim3(kIterBlock), 0, st, k, nS, moff, masks,
+                     rowptr, ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials,
+                     max_iter, tol);
   return hipGetLastError();
 }
 
-hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, int* col, hipStream_t st) {
+hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, unsigned long long* key,
+                            hipStream_t st) {
   hipLaunchKernelGGL(k_sort_rows, dim3(grid_for((int64_t)nS * 256, 16384)), dim3(256), 0, st, nS,
-                     rowptr, col);
+                     rowptr, key);
   return hipGetLastError();
 }
 

@@ -29,79 +29,102 @@ __device__ __forceinline__ int tile_of(int i, int j, int k, const TileGrid& tg) 
 // Plan kernels.  Visitor-style geometry walks (the same step rule as every
 // other walk) that count steps and segments, then write codes and records.
 
+// Segments of a tile are grouped by length, longest first, so that the lanes
+// of a wavefront walk segments of similar length: slot list index
+// tile * kLenBuckets + bucket.
+__device__ __forceinline__ int len_slot(int tile, unsigned int nvis) {
+  const unsigned int b = nvis >> 2;
+  return tile * kLenBuckets + (kLenBuckets - 1) - (int)(b < kLenBuckets ? b : kLenBuckets - 1);
+}
+
 __global__ void __launch_bounds__(256)
 k_tile_count(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-             GridDesc g, TileGrid tg, unsigned int* __restrict__ nsteps,
-             unsigned int* __restrict__ nseg, unsigned int* __restrict__ tile_cnt) {
+             GridDesc g, TileGrid tg, unsigned int* __restrict__ nseg,
+             unsigned int* __restrict__ tile_cnt) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   Walker w;
   w.init(rays[r], dirs, g);
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
-  unsigned int segs = 1, steps = 0;
-  atomicAdd(&tile_cnt[tile], 1u);
+  unsigned int segs = 1, nvis = 1;
   while (!w.done()) {
     int s;
     if (w.advance(&s) < 0) break;
-    steps++;
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
     if (t != tile) {
+      atomicAdd(&tile_cnt[len_slot(tile, nvis)], 1u);
       tile = t;
       segs++;
-      atomicAdd(&tile_cnt[tile], 1u);
+      nvis = 0;
     }
+    nvis++;
   }
-  nsteps[r] = steps;
+  atomicAdd(&tile_cnt[len_slot(tile, nvis)], 1u);
   nseg[r] = segs;
 }
 
 __global__ void __launch_bounds__(256)
 k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-            GridDesc g, TileGrid tg, const unsigned long long* __restrict__ word_off,
+            GridDesc g, TileGrid tg, const int* __restrict__ sid,
             const unsigned long long* __restrict__ seg_base,
             const unsigned long long* __restrict__ tile_ptr, unsigned int* __restrict__ cursor,
-            uint32_t* __restrict__ words, SegRec* __restrict__ segs) {
+            SegRec* __restrict__ segs) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
+  const RayRec rr = rays[r];
   Walker w;
-  w.init(rays[r], dirs, g);
-  uint32_t* wp = words + word_off[r];
+  w.init(rr, dirs, g);
+  int ci[3], co[3];
+  unpack_voxel(rr.in, ci);
+  unpack_voxel(rr.out, co);
+  const uint32_t signs = (co[0] > ci[0] ? 1u : 0u) | (co[1] > ci[1] ? 2u : 0u) |
+                         (co[2] > ci[2] ? 4u : 0u);
+  const uint32_t src = (uint32_t)sid[w.v];
   unsigned long long gseg = seg_base[r];
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
-  uint32_t cur = 0;
-  unsigned int step = 0, seg_start = 0;
-  uint32_t seg_vox = pack_voxel(w.c0, w.c1, w.c2);
-  auto emit = [&](unsigned int end_visit) {
-    // visits seg_start .. end_visit-1 lie in `tile`
-    const unsigned int slot = atomicAdd(&cursor[tile], 1u);
-    SegRec sr;
-    sr.ray = (uint32_t)r;
+  SegRec sr;
+  uint32_t nvis = 1, ncode = 0, first = 1;
+  auto start = [&](int i, int j, int k) {
     sr.g = (uint32_t)gseg;
-    sr.s0n = seg_start | ((end_visit - seg_start) << 16);
-    sr.vox = seg_vox;
-    segs[tile_ptr[tile] + slot] = sr;
+    sr.src = src;
+    const uint32_t lv = (uint32_t)((i & kTileMask) | ((j & kTileMask) << kTileBits) |
+                                   ((k & kTileMask) << (2 * kTileBits)));
+    sr.info = lv | (signs << 18) | (first << 21);
+    sr.code[0] = sr.code[1] = sr.code[2] = 0;
+  };
+  auto emit = [&]() {
+    sr.info |= nvis << kSegLvBits;
+    const int ls = len_slot(tile, nvis);
+    const unsigned int slot = atomicAdd(&cursor[ls], 1u);
+    segs[tile_ptr[ls] + slot] = sr;
     gseg++;
   };
+  auto put_code = [&](int a) {
+    const uint32_t sh = 2 * (ncode & 15);
+    const uint32_t word = ncode >> 4;
+    if (word == 0) sr.code[0] |= (uint32_t)a << sh;
+    else if (word == 1) sr.code[1] |= (uint32_t)a << sh;
+    else sr.code[2] |= (uint32_t)a << sh;
+    ncode++;
+  };
+  start(w.c0, w.c1, w.c2);
   while (!w.done()) {
     int s;
     const int a = w.advance(&s);
     if (a < 0) break;
-    cur |= (uint32_t)a << (2 * (step & 15));
-    step++;
-    if ((step & 15) == 0) {
-      wp[(step >> 4) - 1] = cur;
-      cur = 0;
-    }
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
     if (t != tile) {
-      emit(step);
+      emit();
       tile = t;
-      seg_start = step;
-      seg_vox = pack_voxel(w.c0, w.c1, w.c2);
+      first = 0;
+      nvis = 0;
+      ncode = 0;
+      start(w.c0, w.c1, w.c2);
     }
+    put_code(a);
+    nvis++;
   }
-  if (step & 15) wp[step >> 4] = cur;
-  emit(step + 1);
+  emit();
 }
 
 // ---------------------------------------------------------------------------
@@ -113,30 +136,12 @@ struct TileLds {
   long long acc[kTileVox];
 };
 
-// Step code of step s (1-based) of a ray whose codes start at `words`.
-struct CodeReader {
-  const uint32_t* __restrict__ words;
-  uint32_t cur;
-  unsigned int idx;  // index of the next code
-  __device__ __forceinline__ void seek(const uint32_t* w, unsigned int first) {
-    words = w;
-    idx = first;
-    cur = w[first >> 4] >> (2 * (first & 15));
-  }
-  __device__ __forceinline__ int next() {
-    const int a = (int)(cur & 3u);
-    idx++;
-    cur = (idx & 15) ? (cur >> 2) : words[idx >> 4];
-    return a;
-  }
-};
+constexpr int kTileThreads = 1024;
 
 template <typename T, int kMode>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kTileThreads)
 k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restrict__ tile_ptr,
-            const SegRec* __restrict__ segs, const RayRec* __restrict__ rays,
-            const unsigned long long* __restrict__ word_off, const uint32_t* __restrict__ words,
-            GridDesc g, TileGrid tg, WalkParams wp,
+            const SegRec* __restrict__ segs, GridDesc g, TileGrid tg, WalkParams wp,
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
             const DevScalars* __restrict__ sc, double* __restrict__ segval,
@@ -162,69 +167,61 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   __syncthreads();
   const double scale = sc->scale;
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
-  const unsigned long long b = tile_ptr[tile], e = tile_ptr[tile + 1];
+  const unsigned long long b = tile_ptr[tile * kLenBuckets],
+                           e = tile_ptr[(tile + 1) * kLenBuckets];
   for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x) {
     const SegRec sr = segs[j];
-    const RayRec rr = rays[sr.ray];
-    int ci[3], co[3], cv[3];
-    unpack_voxel(rr.in, ci);
-    unpack_voxel(rr.out, co);
-    unpack_voxel(sr.vox, cv);
-    const int sg0 = co[0] > ci[0] ? 1 : -1, sg1 = co[1] > ci[1] ? 1 : -1,
-              sg2 = co[2] > ci[2] ? 1 : -1;
-    const unsigned int s0 = sr.s0n & 0xffffu, n = sr.s0n >> 16;
-    int lv = (cv[0] - ox) + ((cv[1] - oy) << kTileBits) + ((cv[2] - oz) << (2 * kTileBits));
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const bool first = (sr.info >> 21) & 1u;
     double pin = 0.0, carry = 0.0;
-    if (kMode == kFull) {
-      const int64_t vin = (int64_t)ci[0] + ci[1] * sy + ci[2] * sz;
-      pin = ps[sid[vin]];
-    }
+    if (kMode == kFull) pin = ps[sr.src];
     if (kMode != kLocalSum) carry = segval[sr.g];
     RaySum<T> loc;
-    CodeReader cr;
-    const uint32_t* w = words + word_off[sr.ray];
-    unsigned int vis = 0;
-    if (s0 == 0) {
+    uint32_t c0 = sr.code[0], c1 = sr.code[1], c2 = sr.code[2];
+    auto next_code = [&]() {
+      const int a = (int)(c0 & 3u);
+      c0 = (c0 >> 2) | (c1 << 30);
+      c1 = (c1 >> 2) | (c2 << 30);
+      c2 >>= 2;
+      return a;
+    };
+    auto sign_of = [&](int a) { return ((sg >> a) & 1u) ? 1 : -1; };
+    auto add_lds = [&](int l, double x) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]),
+                (unsigned long long)to_fixed(x, scale));
+    };
+    auto on_source = [&](int l) {
+      const int i = ox + (l & kTileMask), jj = oy + ((l >> kTileBits) & kTileMask),
+                k = oz + (l >> (2 * kTileBits));
+      const bool surface =
+          i == 0 || i == g.X - 1 || jj == 0 || jj == g.Y - 1 || k == 0 || k == g.Z - 1;
+      return surface && sid[i + jj * sy + k * sz] >= 0;
+    };
+    if (first) {
       // the entry visit: self-add p_src (pint = 0)
-      if (kMode == kFull) atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[lv]),
-                                    (unsigned long long)to_fixed(pin, scale));
-      cr.seek(w, 0);
-      vis = 1;
+      if (kMode == kFull) add_lds(lv, pin);
     } else {
-      // crossing step into the tile: the previous voxel lies outside it
-      cr.seek(w, s0 - 1);
-      const int a = cr.next();
-      const int sgn = a == 0 ? sg0 : (a == 1 ? sg1 : sg2);
-      const int64_t vg = (int64_t)cv[0] + cv[1] * sy + cv[2] * sz;
-      const int64_t vp = vg - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
+      // step into the tile: the previous voxel lies outside it
+      const int a = next_code();
+      const int sgn = sign_of(a);
+      const int i = ox + (lv & kTileMask), jj = oy + ((lv >> kTileBits) & kTileMask),
+                k = oz + (lv >> (2 * kTileBits));
+      const int64_t vp = (int64_t)i + jj * sy + k * sz - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
       const T prev = component(field[vp], a);
       loc.add(trapezoid<T>(wp, a, sgn, L.g[a][lv], prev));
-      if (kMode == kFull)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[lv]),
-                  (unsigned long long)to_fixed(pin + (carry + loc.value()), scale));
-      if (kMode == kSourceSums && sid[vg] >= 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[lv]),
-                  (unsigned long long)to_fixed(carry + loc.value(), scale));
-      vis = 1;
+      if (kMode == kFull) add_lds(lv, pin + (carry + loc.value()));
+      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry + loc.value());
     }
-    for (; vis < n; vis++) {
-      const int a = cr.next();
-      const int sgn = a == 0 ? sg0 : (a == 1 ? sg1 : sg2);
+    for (unsigned int vis = 1; vis < n; vis++) {
+      const int a = next_code();
+      const int sgn = sign_of(a);
       const int ln = lv + sgn * (a == 0 ? 1 : (a == 1 ? kTile : kTile * kTile));
       loc.add(trapezoid<T>(wp, a, sgn, L.g[a][ln], L.g[a][lv]));
       lv = ln;
-      if (kMode == kFull)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[lv]),
-                  (unsigned long long)to_fixed(pin + (carry + loc.value()), scale));
-      if (kMode == kSourceSums) {
-        const int i = ox + (lv & kTileMask), jj = oy + ((lv >> kTileBits) & kTileMask),
-                  k = oz + (lv >> (2 * kTileBits));
-        const bool surface =
-            i == 0 || i == g.X - 1 || jj == 0 || jj == g.Y - 1 || k == 0 || k == g.Z - 1;
-        if (surface && sid[i + jj * sy + k * sz] >= 0)
-          atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[lv]),
-                    (unsigned long long)to_fixed(carry + loc.value(), scale));
-      }
+      if (kMode == kFull) add_lds(lv, pin + (carry + loc.value()));
+      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry + loc.value());
     }
     if (kMode == kLocalSum) segval[sr.g] = loc.value();
   }
@@ -243,6 +240,84 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   }
 }
 
+// Iteration matrix layout (plan build).  A chunk is 64 consecutive rows
+// (neighbouring surface voxels under the blocked source numbering).  Its
+// entries are visited in slots (d, k): the k-th entry of direction d of each
+// row, for k below the largest count of d among the chunk's rows.  A slot
+// stores a 64-bit mask of the rows that have such an entry, and the entries
+// themselves are packed without padding in slot order, each row's at its
+// rank among the set bits.  The lanes of one slot then gather sources of one
+// direction for neighbouring targets -- neighbouring sources -- and a row's
+// entries are summed in (direction, source) order.
+constexpr int kSellDirCap = 4096;
+
+__device__ __forceinline__ void sell_count_dirs(const unsigned long long* __restrict__ rowptr,
+                                                const unsigned long long* __restrict__ key,
+                                                int nS, int row, unsigned int* K) {
+  if (row >= nS) return;
+  const unsigned long long b = rowptr[row], e = rowptr[row + 1];
+  unsigned int run = 0, d = 0xffffffffu;
+  for (unsigned long long j = b; j < e; j++) {
+    const unsigned int dj = (unsigned int)(key[j] >> 32);
+    if (dj != d) {
+      if (run) atomicMax(&K[d], run);
+      d = dj;
+      run = 0;
+    }
+    run++;
+  }
+  if (run) atomicMax(&K[d], run);
+}
+
+__global__ void __launch_bounds__(64)
+k_sell_layout(int nS, int D, const unsigned long long* __restrict__ rowptr,
+              const unsigned long long* __restrict__ key, unsigned int* __restrict__ nslot) {
+  __shared__ unsigned int K[kSellDirCap];
+  const int chunk = blockIdx.x, lane = threadIdx.x, row = chunk * 64 + lane;
+  for (int d = lane; d < D; d += 64) K[d] = 0;
+  __syncthreads();
+  sell_count_dirs(rowptr, key, nS, row, K);
+  __syncthreads();
+  unsigned int tot = 0;
+  for (int d = lane; d < D; d += 64) tot += K[d];
+  for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+  if (lane == 0) nslot[chunk] = tot;
+}
+
+__global__ void __launch_bounds__(64)
+k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
+            const unsigned long long* __restrict__ key, const unsigned long long* __restrict__ moff,
+            unsigned long long* __restrict__ masks, int* __restrict__ ent) {
+  const int chunk = blockIdx.x, lane = threadIdx.x, row = chunk * 64 + lane;
+  const unsigned long long rb = row < nS ? rowptr[row] : 0ull;
+  const unsigned long long re = row < nS ? rowptr[row + 1] : 0ull;
+  unsigned long long mp = moff[chunk];
+  unsigned long long eb = rowptr[chunk * 64];  // the chunk's entries start where its rows do
+  unsigned long long j = rb;
+  for (int d = 0; d < D; d++) {
+    // entries of direction d in this lane's row (rows are sorted by (d, source))
+    unsigned int c = 0;
+    while (j + c < re && (unsigned int)(key[j + c] >> 32) == (unsigned int)d) c++;
+    unsigned int kmax = c;
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned int o = __shfl_xor(kmax, off);
+      kmax = o > kmax ? o : kmax;
+    }
+    for (unsigned int k = 0; k < kmax; k++) {
+      const bool has = c > k;
+      const unsigned long long m = __ballot(has);
+      if (lane == 0) masks[mp] = m;
+      mp++;
+      if (has) {
+        const unsigned int rank = __popcll(m & ((1ull << lane) - 1ull));
+        ent[eb + rank] = (int)(unsigned int)(key[j + k] & 0xffffffffull);
+      }
+      eb += __popcll(m);
+    }
+    j += c;
+  }
+}
+
 // Per-ray exclusive scan of the segment sums into carries, in place.
 __global__ void __launch_bounds__(256)
 k_carry_scan(int64_t nrays, const unsigned long long* __restrict__ seg_base,
@@ -251,7 +326,19 @@ k_carry_scan(int64_t nrays, const unsigned long long* __restrict__ seg_base,
   if (r >= nrays) return;
   const unsigned long long b = seg_base[r], e = seg_base[r + 1];
   double run = 0.0;
-  for (unsigned long long j = b; j < e; j++) {
+  unsigned long long j = b;
+  // eight loads in flight per lane; the sum order is the segment order
+  for (; j + 8 <= e; j += 8) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) x[u] = segval[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      segval[j + u] = run;
+      run = run + x[u];
+    }
+  }
+  for (; j < e; j++) {
     const double x = segval[j];
     segval[j] = run;
     run = run + x;
@@ -341,58 +428,67 @@ k_scan_final(const unsigned int* __restrict__ cnt, int64_t n, int words,
 // Launchers.
 
 hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                             TileGrid tg, unsigned int* nsteps, unsigned int* nseg,
-                             unsigned int* tile_cnt, hipStream_t st) {
+                             TileGrid tg, unsigned int* nseg, unsigned int* tile_cnt,
+                             hipStream_t st) {
   hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, tg, nsteps, nseg, tile_cnt);
+                     dirs, g, tg, nseg, tile_cnt);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                            TileGrid tg, const unsigned long long* word_off,
-                            const unsigned long long* seg_base, const unsigned long long* tile_ptr,
-                            unsigned int* cursor, uint32_t* words, SegRec* segs, hipStream_t st) {
+                            TileGrid tg, const int* sid, const unsigned long long* seg_base,
+                            const unsigned long long* tile_ptr, unsigned int* cursor,
+                            SegRec* segs, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_fill, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, tg, word_off, seg_base, tile_ptr, cursor, words, segs);
+                     dirs, g, tg, sid, seg_base, tile_ptr, cursor, segs);
+  return hipGetLastError();
+}
+
+hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
+                              const unsigned long long* key, unsigned int* nslot, hipStream_t st) {
+  if (D > kSellDirCap) return hipErrorInvalidValue;
+  const unsigned int nch = (unsigned int)((nS + 63) / 64);
+  hipLaunchKernelGGL(k_sell_layout, dim3(nch), dim3(64), 0, st, nS, D, rowptr, key, nslot);
+  return hipGetLastError();
+}
+
+hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
+                            const unsigned long long* key, const unsigned long long* moff,
+                            unsigned long long* masks, int* ent, hipStream_t st) {
+  const unsigned int nch = (unsigned int)((nS + 63) / 64);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nch), dim3(64), 0, st, nS, D, rowptr, key, moff, masks,
+                     ent);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_tile_pass(int mode, int ntiles, const int* tile_list,
-                            const unsigned long long* tile_ptr, const SegRec* segs,
-                            const RayRec* rays, const unsigned long long* word_off,
-                            const uint32_t* words, GridDesc g, TileGrid tg, const WalkParams& wp,
-                            const void* field, const int* sid, const double* p0,
-                            const double* p1, const DevScalars* sc, double* segval,
-                            long long* out, hipStream_t st) {
+                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                            TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
+                            const double* p0, const double* p1, const DevScalars* sc,
+                            double* segval, long long* out, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   if (ntiles <= 0) return hipSuccess;
   if (mode == kLocalSum)
-    hipLaunchKernelGGL((k_tile_pass<T, kLocalSum>), dim3(ntiles), dim3(256), 0, st, tile_list,
-                       tile_ptr, segs, rays, word_off, words, g, tg, wp, f, sid, p0, p1, sc,
-                       segval, out);
+    hipLaunchKernelGGL((k_tile_pass<T, kLocalSum>), dim3(ntiles), dim3(kTileThreads), 0, st,
+                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
   else if (mode == kSourceSums)
-    hipLaunchKernelGGL((k_tile_pass<T, kSourceSums>), dim3(ntiles), dim3(256), 0, st, tile_list,
-                       tile_ptr, segs, rays, word_off, words, g, tg, wp, f, sid, p0, p1, sc,
-                       segval, out);
+    hipLaunchKernelGGL((k_tile_pass<T, kSourceSums>), dim3(ntiles), dim3(kTileThreads), 0, st,
+                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
   else
-    hipLaunchKernelGGL((k_tile_pass<T, kFull>), dim3(ntiles), dim3(256), 0, st, tile_list,
-                       tile_ptr, segs, rays, word_off, words, g, tg, wp, f, sid, p0, p1, sc,
-                       segval, out);
+    hipLaunchKernelGGL((k_tile_pass<T, kFull>), dim3(ntiles), dim3(kTileThreads), 0, st,
+                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
   return hipGetLastError();
 }
 
 template hipError_t launch_tile_pass<float>(int, int, const int*, const unsigned long long*,
-                                            const SegRec*, const RayRec*,
-                                            const unsigned long long*, const uint32_t*, GridDesc,
-                                            TileGrid, const WalkParams&, const void*, const int*,
-                                            const double*, const double*, const DevScalars*,
-                                            double*, long long*, hipStream_t);
+                                            const SegRec*, GridDesc, TileGrid, const WalkParams&,
+                                            const void*, const int*, const double*, const double*,
+                                            const DevScalars*, double*, long long*, hipStream_t);
 template hipError_t launch_tile_pass<double>(int, int, const int*, const unsigned long long*,
-                                             const SegRec*, const RayRec*,
-                                             const unsigned long long*, const uint32_t*, GridDesc,
-                                             TileGrid, const WalkParams&, const void*, const int*,
+                                             const SegRec*, GridDesc, TileGrid,
+                                             const WalkParams&, const void*, const int*,
                                              const double*, const double*, const DevScalars*,
                                              double*, long long*, hipStream_t);
 

@@ -15,34 +15,45 @@ constexpr int kTileMask = kTile - 1;
 constexpr int kTileVox = kTile * kTile * kTile;
 
 enum { kLocalSum = 0, kSourceSums = 1, kFull = 2 };
+constexpr int kLenBuckets = 12;  // segment-length groups per tile (4 visits each)
 
 struct TileGrid {
   int ntx, nty, ntz;
 };
 
-// One segment: the visits of one ray inside one tile.
+// One segment: the visits of one ray inside one tile, self-contained so that
+// a tile pass starts it with one coalesced load.  A monotone staircase visits
+// at most 16+15+15 = 46 voxels of a 16^3 tile, so its step codes fit in 96 bits.
 struct SegRec {
-  uint32_t ray;  // index into the ray list
-  uint32_t g;    // segment id in ray-major order (index of its carry)
-  uint32_t s0n;  // first visit index (low 16 bits) | number of visits (high 16 bits)
-  uint32_t vox;  // packed voxel of the first visit
+  uint32_t g;        // segment id in ray-major order (index of its carry)
+  uint32_t src;      // source id of the ray (index of p_src)
+  uint32_t info;     // local voxel of the first visit (12 bits) | visits (6) << 12 |
+                     // step signs x,y,z (3) << 18 | first segment of the ray (1) << 21
+  uint32_t code[3];  // 2-bit axis codes: the step into the tile (not for the first
+                     // segment) and the steps between its visits, low bits first
 };
+constexpr int kSegLvBits = 12;
 
 hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                             TileGrid tg, unsigned int* nsteps, unsigned int* nseg,
-                             unsigned int* tile_cnt, hipStream_t st);
+                             TileGrid tg, unsigned int* nseg, unsigned int* tile_cnt,
+                             hipStream_t st);
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                            TileGrid tg, const unsigned long long* word_off,
-                            const unsigned long long* seg_base, const unsigned long long* tile_ptr,
-                            unsigned int* cursor, uint32_t* words, SegRec* segs, hipStream_t st);
+                            TileGrid tg, const int* sid, const unsigned long long* seg_base,
+                            const unsigned long long* tile_ptr, unsigned int* cursor,
+                            SegRec* segs, hipStream_t st);
 template <typename T>
 hipError_t launch_tile_pass(int mode, int ntiles, const int* tile_list,
-                            const unsigned long long* tile_ptr, const SegRec* segs,
-                            const RayRec* rays, const unsigned long long* word_off,
-                            const uint32_t* words, GridDesc g, TileGrid tg, const WalkParams& wp,
-                            const void* field, const int* sid, const double* p0,
-                            const double* p1, const DevScalars* sc, double* segval,
-                            long long* out, hipStream_t st);
+                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                            TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
+                            const double* p0, const double* p1, const DevScalars* sc,
+                            double* segval, long long* out, hipStream_t st);
+// masked sliced-ELL copy of the source CSR (64-row chunks, per-direction
+// slots with lane masks, entries packed without padding); D <= 4096
+hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
+                              const unsigned long long* key, unsigned int* nslot, hipStream_t st);
+hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
+                            const unsigned long long* key, const unsigned long long* moff,
+                            unsigned long long* masks, int* ent, hipStream_t st);
 hipError_t launch_carry_scan(int64_t nrays, const unsigned long long* seg_base, double* segval,
                              hipStream_t st);
 // exclusive scan of n uint32 counts into n+1 uint64 offsets (multi-block);

@@ -173,15 +173,17 @@ k_plan_rowcount(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* 
   }
 }
 
+// Each entry is the key (direction << 32) | source id, so that sorting a row
+// groups its entries by direction (the sliced-ELL layout relies on it).
 struct RowFillVisitor {
   const int* sid;
   const unsigned long long* rowptr;
   unsigned int* cursor;
-  int* col;
-  int src_id;
+  unsigned long long* key;
+  unsigned long long src_key;
   __device__ __forceinline__ void put(int s) {
     const unsigned int pos = atomicAdd(&cursor[s], 1u);
-    col[rowptr[s] + pos] = src_id;
+    key[rowptr[s] + pos] = src_key;
   }
   __device__ __forceinline__ void entry(int64_t) {}
   __device__ __forceinline__ void visit(int64_t v, bool surface) {
@@ -194,13 +196,15 @@ struct RowFillVisitor {
 __global__ void __launch_bounds__(256)
 k_plan_rowfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
                GridDesc g, const int* __restrict__ sid, const unsigned long long* __restrict__ rowptr,
-               unsigned int* __restrict__ cursor, int* __restrict__ col) {
+               unsigned int* __restrict__ cursor, unsigned long long* __restrict__ key) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
        r += (int64_t)gridDim.x * blockDim.x) {
     int c[3];
     unpack_voxel(rays[r].in, c);
     const int64_t vin = (int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y;
-    RowFillVisitor vis{sid, rowptr, cursor, col, sid[vin]};
+    const unsigned long long k = ((unsigned long long)(rays[r].dir_len & 0xffffu) << 32) |
+                                 (unsigned long long)(unsigned int)sid[vin];
+    RowFillVisitor vis{sid, rowptr, cursor, key, k};
     walk_geometry(rays[r], dirs, g, vis);
   }
 }
@@ -277,9 +281,9 @@ hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirCons
 
 hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                GridDesc g, const int* sid, const unsigned long long* rowptr,
-                               unsigned int* cursor, int* col, hipStream_t st) {
+                               unsigned int* cursor, unsigned long long* key, hipStream_t st) {
   hipLaunchKernelGGL(k_plan_rowfill, dim3(grid_for(nrays)), dim3(256), 0, st, rays, nrays, dirs,
-                     g, sid, rowptr, cursor, col);
+                     g, sid, rowptr, cursor, key);
   return hipGetLastError();
 }
 

@@ -48,6 +48,7 @@ class Report(C.Structure):
         ("steps", C.c_int64), ("sources", C.c_int64), ("status", C.c_int),
         ("ms_plan", C.c_double), ("ms_solve", C.c_double), ("ms_pass_a", C.c_double),
         ("ms_iterate", C.c_double), ("ms_pass_b", C.c_double), ("ms_finalize", C.c_double),
+        ("source_visits", C.c_int64), ("segments", C.c_int64), ("sell_cells", C.c_int64),
     ]
 
 
