@@ -44,8 +44,19 @@ class Report(C.Structure):
     _fields_ = [("iterations", C.c_int), ("rel_change", C.c_double)]
 
 
+class Dir(C.Structure):
+    _fields_ = [("k", C.c_double * 3), ("e1", C.c_double * 3), ("e2", C.c_double * 3),
+                ("M", (C.c_double * 3) * 3)]
+
+
 def _declare(L):
     i, d, i64 = C.c_int, C.c_double, C.c_int64
+    L.orc_make_dir.argtypes = [C.c_float, C.c_float, C.c_float, C.POINTER(Dir)]
+    L.orc_make_dir.restype = None
+    L.orc_origin.argtypes = [C.POINTER(Dir), i, d, i64, _dp]
+    L.orc_origin.restype = None
+    L.orc_clip.argtypes = [i, i, i, _dp, _dp, _ip, _ip]
+    L.orc_clip.restype = i
     L.orc_nogrid.argtypes = [i, i, i, d]
     L.orc_nogrid.restype = i
     L.orc_enumerate_rays.argtypes = [i, i, i, i, _fp, _fp, _fp, d, _i32p, i64]
@@ -84,6 +95,34 @@ def _dirs(k):
     k = np.ascontiguousarray(k, dtype=np.float32).reshape(-1, 3)
     return (np.ascontiguousarray(k[:, 0]), np.ascontiguousarray(k[:, 1]),
             np.ascontiguousarray(k[:, 2]), k.shape[0])
+
+
+def make_dir(kvec):
+    """(k, e1, e2, M) of one direction as float64 arrays (kernel.cu:6441-6457)."""
+    d = Dir()
+    lib().orc_make_dir(float(kvec[0]), float(kvec[1]), float(kvec[2]), C.byref(d))
+    return (np.array(d.k[:]), np.array(d.e1[:]), np.array(d.e2[:]),
+            np.array([d.M[r][:] for r in range(3)]))
+
+
+def origin(kvec, ng, ls, point):
+    """Centred ray origin of lattice point `point` (kernel.cu:6172-6186)."""
+    d = Dir()
+    lib().orc_make_dir(float(kvec[0]), float(kvec[1]), float(kvec[2]), C.byref(d))
+    o = np.zeros(3)
+    lib().orc_origin(C.byref(d), ng, ls, int(point), _ptr(o, _dp))
+    return o
+
+
+def clip(shape, o, kvec):
+    """cross2point + launcher acceptance: (accepted, in_ijk, out_ijk)."""
+    Z, Y, X = shape
+    o = _f64(o)
+    k = _f64(kvec)
+    a = np.zeros(3, dtype=np.int32)
+    b = np.zeros(3, dtype=np.int32)
+    ok = lib().orc_clip(X, Y, Z, _ptr(o, _dp), _ptr(k, _dp), _ptr(a, _ip), _ptr(b, _ip))
+    return bool(ok), a.copy(), b.copy()
 
 
 def nogrid(shape, ls):

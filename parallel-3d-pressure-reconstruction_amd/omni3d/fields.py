@@ -109,8 +109,9 @@ def burgers_vortex(n=256, noise=0.02, seed=0, rc=0.1, gamma=1.0, rho=1.0, L=1.0)
     return _pack(p, g, (h, h, h), rho=rho)
 
 
-def quadratic(shape, coef=None, seed=0, h=(1.0, 1.0, 1.0)):
+def quadratic(shape, coef=None, seed=0, delta=(1.0, 1.0, 1.0), rho=1.0):
     """Quadratic p (linear gradient): the trapezoid rule integrates it exactly."""
+    h = delta
     rng = np.random.default_rng(seed)
     c = rng.normal(size=10) if coef is None else np.asarray(coef, dtype=np.float64)
     x, y, z = _grid(shape, h)
@@ -119,7 +120,7 @@ def quadratic(shape, coef=None, seed=0, h=(1.0, 1.0, 1.0)):
     gx = c[1] + 2 * c[4] * x + c[7] * y + c[9] * z
     gy = c[2] + 2 * c[5] * y + c[7] * x + c[8] * z
     gz = c[3] + 2 * c[6] * z + c[8] * y + c[9] * x
-    return _pack(p, [gx, gy, gz], h)
+    return _pack(p, [gx, gy, gz], h, rho=rho)
 
 
 def snapshot_series(count=64, seed=0):
