@@ -409,6 +409,104 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   return OMNI3D_OK;
 }
 
+
+// Reference schedule (kernel.cu:8840-8861): every iteration walks every ray
+// over the whole volume and averages.  Used by the NAIVE solver and by every
+// method other than PLAIN.  Writes the result of iteration k* to pdst/wdst.
+template <typename T>
+int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
+  const omni3d_params& P = ctx->prm;
+  Plan& pl = ctx->plan;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  const int m = P.method;
+  DevScalars* sc = ctx->scal.as<DevScalars>();
+  const void* field = ctx->field.as<void>();
+  const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
+  WalkParams wp = walk_params(P);
+  OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
+  if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
+  const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
+  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
+                                         (double)P.max_iterations, st));
+  // the field the path-break rules test
+  const double* aux = nullptr;
+  if (m != OMNI3D_METHOD_PLAIN) {
+    OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
+    const double* mask = ctx->have_mask ? ctx->mask.as<double>() : nullptr;
+    if (m == OMNI3D_METHOD_PLAIN_MASK) {
+      if (mask)
+        OMNI_TRY_HIP(ctx, (launch_convert<double, double>(mask, ctx->aux.as<double>(), V, st)));
+      else
+        OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->aux.as<double>(), V, 1.0, st));
+      wp.thr = INFINITY;
+      wp.faithful = 0;
+    } else {
+      OMNI_TRY_HIP(ctx, launch_curl_packed<T>(P.nx, P.ny, P.nz, P.dx, P.dy, P.dz, field, mask,
+                                              ctx->aux.as<double>(), st));
+    }
+    aux = ctx->aux.as<double>();
+  }
+  const bool fixed = m == OMNI3D_METHOD_PLAIN || m == OMNI3D_METHOD_PLAIN_MASK;
+  OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->pb.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
+  OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * naive_update_grid(V)));
+  if (fixed) {
+    OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
+    if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+  } else {
+    OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
+    OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
+  }
+  double* pa = ctx->pa.as<double>();
+  double* pb = ctx->pb.as<double>();
+  if (pinit)
+    OMNI_TRY_HIP(ctx, (launch_convert<double, double>(pinit, pa, V, st)));
+  else
+    OMNI_TRY_HIP(ctx, launch_fill_f64(pa, V, 0.0, st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
+  const unsigned int* cntw =
+      m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+  for (int k = 1; k <= P.max_iterations; k++) {
+    if (fixed) {
+      OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
+      if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+    } else {
+      OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
+      OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
+    }
+    OMNI_TRY_HIP(ctx, launch_walk_naive<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                           pl.g, wp, field, aux, pa, pb, k, sc,
+                                           fixed ? ctx->sum.as<long long>() : nullptr,
+                                           m == OMNI3D_METHOD_PLAIN_MASK ? ctx->wcnt.as<unsigned int>() : nullptr,
+                                           fixed ? nullptr : ctx->sumd.as<double>(),
+                                           fixed ? nullptr : ctx->wsum.as<double>(),
+                                           ctx->srcf.as<uint8_t>(), st));
+    OMNI_TRY_HIP(ctx, launch_naive_update(k, V, fixed ? 1 : 0, fixed ? ctx->sum.as<long long>() : nullptr,
+                                          cntw, fixed ? nullptr : ctx->sumd.as<double>(),
+                                          fixed ? nullptr : ctx->wsum.as<double>(), pinit,
+                                          seed > 0 ? 1 : 0,
+                                          m == OMNI3D_METHOD_PLAIN ? pl.sid.as<int>() : nullptr,
+                                          ctx->srcf.as<uint8_t>(), pa, pb, sc,
+                                          ctx->partials.as<double>(), P.max_iterations,
+                                          P.tolerance, st));
+  }
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], st));
+  const bool f64 = dtype == OMNI3D_F64;
+  OMNI_TRY_HIP(ctx, launch_naive_output(V, pa, pb, sc, fixed ? 1 : 0, cntw,
+                                        fixed ? nullptr : ctx->wsum.as<double>(),
+                                        f64 ? (double*)pdst : nullptr, f64 ? nullptr : (float*)pdst,
+                                        (wdst && f64) ? (double*)wdst : nullptr,
+                                        (wdst && !f64) ? (float*)wdst : nullptr, st));
+  return OMNI3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -580,8 +678,6 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
   if (ctx->k.empty()) return fail(ctx, OMNI3D_ERR_STATE, "no directions set");
   if (!ctx->have_field) return fail(ctx, OMNI3D_ERR_STATE, "no field set");
   const omni3d_params& P = ctx->prm;
-  if (P.method != OMNI3D_METHOD_PLAIN || P.solver != OMNI3D_SOLVER_AFFINE)
-    return fail(ctx, OMNI3D_ERR_ARG, "method/solver combination not built yet");
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   const bool had_plan = ctx->plan.valid && !ctx->dirs_dirty;
@@ -608,8 +704,6 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
     seed = P.seed_pn_with_p ? 1 : 0;
   }
-  rc = P.precision == OMNI3D_FP32_KAHAN ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
-  if (rc) return rc;
   // averaging into the caller's buffers (device) or a staging buffer (host)
   const size_t esz = elem_size(dtype);
   void* pdst = p;
@@ -628,13 +722,24 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     pdst = ptmp.as<void>();
   }
   const bool f64 = dtype == OMNI3D_F64;
-  OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
-                                    ctx->plan.cnt.as<unsigned int>(),
-                                    seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
-                                    ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
-                                    f64 ? nullptr : (float*)pdst,
-                                    (wdst && f64) ? (double*)wdst : nullptr,
-                                    (wdst && !f64) ? (float*)wdst : nullptr, st));
+  const bool fp32 = P.precision == OMNI3D_FP32_KAHAN;
+  // the source-voxel (affine) schedule exists for the plain method; every
+  // other method, and the NAIVE solver, walks the whole volume per iteration
+  if (P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN) {
+    rc = fp32 ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
+    if (rc) return rc;
+    OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
+                                      ctx->plan.cnt.as<unsigned int>(),
+                                      seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
+                                      ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
+                                      f64 ? nullptr : (float*)pdst,
+                                      (wdst && f64) ? (double*)wdst : nullptr,
+                                      (wdst && !f64) ? (float*)wdst : nullptr, st));
+  } else {
+    rc = fp32 ? run_naive<float>(ctx, seed, pdst, wdst, dtype)
+              : run_naive<double>(ctx, seed, pdst, wdst, dtype);
+    if (rc) return rc;
+  }
   OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], st));
   if (mem_kind == OMNI3D_MEM_HOST) {
     if (p) OMNI_TRY_HIP(ctx, hipMemcpyAsync(p, pdst, esz * V, hipMemcpyDeviceToHost, st));

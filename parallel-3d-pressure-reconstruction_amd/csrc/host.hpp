@@ -111,6 +111,9 @@ struct omni3d_ctx {
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
   int used_copies = 1;
   omni3d::DevBuf pout, wout;  // staging for host outputs
+  // naive schedule: pressure double buffer, aux field (mask or |curl a|*mask),
+  // curl-weighted sums, unit visit counts, flags of the voxels read
+  omni3d::DevBuf pa, pb, aux, sumd, wsum, wcnt, srcf;
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
   hipEvent_t ev[8] = {};

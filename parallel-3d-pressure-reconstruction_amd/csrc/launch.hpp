@@ -41,6 +41,35 @@ hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const
                              const double* p0, const double* p1, const DevScalars* sc,
                              long long* acc, int64_t copy_stride, int ncopies, hipStream_t st);
 
+// naive.hip ----------------------------------------------------------------
+// method: 0 plain, 1 plain-mask, 2 select, 3 weighted.  Iteration k reads
+// p0/p1[(k-1)&1]; fixed-point sums go to sumfx (+ unit counts to wcnt for
+// plain-mask), curl-weighted sums to sumd/wsum; src flags the voxels read.
+template <typename T>
+hipError_t launch_walk_naive(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                             GridDesc g, const WalkParams& wp, const void* field,
+                             const double* aux, const double* p0, const double* p1, int k,
+                             const DevScalars* sc, long long* sumfx, unsigned int* wcnt,
+                             double* sumd, double* wsum, uint8_t* src, hipStream_t st);
+unsigned int naive_update_grid(int64_t V);
+hipError_t launch_naive_update(int k, int64_t V, int fixed, const long long* sumfx,
+                               const unsigned int* cnt, const double* sumd, const double* wsum,
+                               const double* pinit, int seed, const int* sid, const uint8_t* src,
+                               double* p0, double* p1, DevScalars* sc, double* partials,
+                               int max_iter, double tol, hipStream_t st);
+hipError_t launch_naive_output(int64_t V, const double* p0, const double* p1,
+                               const DevScalars* sc, int fixed, const unsigned int* cnt,
+                               const double* wsum, double* pd, float* pf, double* wd, float* wf,
+                               hipStream_t st);
+template <typename T>
+hipError_t launch_curl_packed(int X, int Y, int Z, double dx, double dy, double dz,
+                              const void* field, const double* mask, double* out, hipStream_t st);
+hipError_t launch_curl_f32(int X, int Y, int Z, double dx, double dy, double dz, const float* ax,
+                           const float* ay, const float* az, const float* mask, float* out,
+                           hipStream_t st);
+hipError_t launch_fill_u32(void* x, int64_t words, unsigned int v, hipStream_t st);
+hipError_t launch_fill_f64(double* x, int64_t n, double v, hipStream_t st);
+
 // solve.hip ----------------------------------------------------------------
 template <typename T>
 hipError_t launch_absmax(const T* x, int64_t n, unsigned long long* bits, hipStream_t st);

@@ -1,0 +1,481 @@
+// Reference-schedule iteration: one full traversal of every ray per
+// iteration (kernel.cu:8840-8861) for all four integration methods, the
+// curl field of the Select / Weighted methods, and the per-iteration
+// averaging with the on-device stop rule.
+//
+// One lane walks one ray.  Sums whose weights are whole visit counts (plain,
+// plain-mask) are rounded to 64-bit fixed point and added with integer
+// atomics, so they do not depend on the order lanes arrive in; the
+// curl-weighted methods carry real-valued weights 1/curltmp * counttmp with
+// no useful bound, and add value and weight with FP64 atomics instead.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "kernels.hpp"
+#include "launch.hpp"
+#include "walk.hpp"
+
+namespace omni3d {
+
+__device__ __forceinline__ void add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// Plain walk (bodyIntegralInner, kernel.cu:1743-1863) reading p over the
+// whole volume: entry self-add, then p_in + pint at every visit.
+template <typename T>
+__device__ __forceinline__ void naive_plain(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                            const GridDesc& g, const WalkParams& wp,
+                                            const typename Vec4<T>::type* __restrict__ field,
+                                            const double* __restrict__ p, double scale,
+                                            long long* __restrict__ sum) {
+  using V4 = typename Vec4<T>::type;
+  Walker w;
+  w.init(rr, dirs, g);
+  const double pin = p[w.v];
+  fx_add(&sum[w.v], to_fixed(pin, scale));
+  V4 cur = field[w.v];
+  RaySum<T> pint;
+  while (!w.done()) {
+    int s;
+    const int a = w.advance(&s);
+    if (a < 0) break;
+    const V4 nxt = field[w.v];
+    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+    fx_add(&sum[w.v], to_fixed(pin + pint.value(), scale));
+    cur = nxt;
+  }
+}
+
+// Select walk (bodyIntegralInnerSelect, kernel.cu:2280-2537).  aux is the
+// |curl a| * mask field (Select) or the mask (plain-mask, thr = +inf, unit
+// weights, symmetric rule).  With faithful the per-axis asymmetries of the
+// reference are kept: y/z steps test only curl != 0 for the first valid
+// voxel and do not treat curl == 0 as leaving the valid zone
+// (kernel.cu:2422,2442,2468,2488), and a z-step restart keeps the weight
+// accumulators (kernel.cu:2497-2506).
+template <typename T, bool kUnit>
+__device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                             const GridDesc& g, const WalkParams& wp,
+                                             const typename Vec4<T>::type* __restrict__ field,
+                                             const double* __restrict__ aux,
+                                             const double* __restrict__ p, double scale,
+                                             long long* __restrict__ sumfx,
+                                             unsigned int* __restrict__ wcnt,
+                                             double* __restrict__ sumd, double* __restrict__ wsum,
+                                             uint8_t* __restrict__ src) {
+  using V4 = typename Vec4<T>::type;
+  const double thr = wp.thr;
+  const bool faithful = wp.faithful != 0;
+  Walker w;
+  w.init(rr, dirs, g);
+  double p0 = 0.0, curltmp = 0.0, counttmp = 0.0;
+  bool p0set = false, outthr = false;
+  RaySum<T> pint;
+  {
+    const double ci = aux[w.v];
+    const bool entry_valid = faithful ? !(ci == 0.0 || ci > thr) : (ci != 0.0 && ci < thr);
+    if (entry_valid) {
+      p0 = p[w.v];
+      src[w.v] = 1;
+      p0set = true;
+    }
+  }
+  V4 cur = field[w.v];
+  while (!w.done()) {
+    int s;
+    const int a = w.advance(&s);
+    if (a < 0) break;
+    const V4 nxt = field[w.v];
+    const double cn = aux[w.v];
+    const bool valid = cn < thr && cn != 0.0;
+    if (!p0set) {
+      const bool ok = (faithful && a != 0) ? (cn != 0.0) : valid;
+      if (ok) {
+        p0 = p[w.v];
+        src[w.v] = 1;
+        p0set = true;
+        pint.reset();
+        counttmp = 0.0;
+        curltmp = 0.0;
+      }
+    } else {
+      if (!outthr && valid) {
+        pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+        if (kUnit) {
+          fx_add(&sumfx[w.v], to_fixed(p0 + pint.value(), scale));
+          atomicAdd(&wcnt[w.v], 1u);
+        } else {
+          curltmp += cn / 10000;
+          counttmp += 1.0;
+          add_f64(&sumd[w.v], (p0 + pint.value()) * 1 / curltmp * counttmp);
+          add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+        }
+      }
+      const bool enter_high = (faithful && a != 0) ? (!outthr && cn >= thr)
+                                                   : ((!outthr && cn >= thr) || cn == 0.0);
+      if (enter_high) outthr = true;
+      if (outthr && valid) {
+        p0 = p[w.v];
+        src[w.v] = 1;
+        pint.reset();
+        outthr = false;
+        if (!(faithful && a == 2)) {
+          counttmp = 0.0;
+          curltmp = 0.0;
+        }
+      }
+    }
+    cur = nxt;
+  }
+}
+
+// Weighted walk (bodyIntegralInnerWeighted, kernel.cu:1993-2135): no entry
+// self-add, no path break; weight counttmp / curltmp with curltmp the running
+// sum of curl / 10^4.
+template <typename T>
+__device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                               const GridDesc& g, const WalkParams& wp,
+                                               const typename Vec4<T>::type* __restrict__ field,
+                                               const double* __restrict__ aux,
+                                               const double* __restrict__ p,
+                                               double* __restrict__ sumd,
+                                               double* __restrict__ wsum,
+                                               uint8_t* __restrict__ src) {
+  using V4 = typename Vec4<T>::type;
+  Walker w;
+  w.init(rr, dirs, g);
+  const double pin = p[w.v];
+  src[w.v] = 1;
+  double curltmp = 0.0, counttmp = 0.0;
+  RaySum<T> pint;
+  V4 cur = field[w.v];
+  while (!w.done()) {
+    int s;
+    const int a = w.advance(&s);
+    if (a < 0) break;
+    const V4 nxt = field[w.v];
+    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+    curltmp += aux[w.v] / 10000;
+    counttmp += 1.0;
+    if (curltmp != 0.0) {
+      add_f64(&sumd[w.v], (pin + pint.value()) * 1 / curltmp * counttmp);
+      add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+    }
+    cur = nxt;
+  }
+}
+
+template <typename T, int kMethod>
+__global__ void __launch_bounds__(256)
+k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+             GridDesc g, WalkParams wp, const typename Vec4<T>::type* __restrict__ field,
+             const double* __restrict__ aux, const double* __restrict__ p0,
+             const double* __restrict__ p1, int k, const DevScalars* __restrict__ sc,
+             long long* __restrict__ sumfx, unsigned int* __restrict__ wcnt,
+             double* __restrict__ sumd, double* __restrict__ wsum, uint8_t* __restrict__ src) {
+  if (sc->converged) return;
+  // iteration k reads the pressure of iteration k-1
+  const double* __restrict__ p = ((k - 1) & 1) ? p1 : p0;
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const RayRec rr = rays[r];
+  if (kMethod == 0)
+    naive_plain<T>(rr, dirs, g, wp, field, p, sc->scale, sumfx);
+  else if (kMethod == 1)
+    naive_select<T, true>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd, wsum, src);
+  else if (kMethod == 2)
+    naive_select<T, false>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd, wsum,
+                           src);
+  else
+    naive_weighted<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
+}
+
+// Averaging of one iteration (devidecountInner, kernel.cu:6746-6759: where
+// the weight is > 0, p = pn / weight, elsewhere p keeps its value) and the
+// relative change over the voxels whose p the rays read.  The first
+// iteration adds the initial pressure to the sums when pn starts as a copy
+// of p (kernel.cu:8813).  The last block reduces and sets the stop flag.
+constexpr int kUpdBlock = 256;
+
+__global__ void __launch_bounds__(kUpdBlock)
+k_naive_update(int k, int64_t V, int fixed, const long long* __restrict__ sumfx,
+               const unsigned int* __restrict__ cnt, const double* __restrict__ sumd,
+               const double* __restrict__ wsum, const double* __restrict__ pinit, int seed,
+               const int* __restrict__ sid, const uint8_t* __restrict__ src,
+               double* __restrict__ p0, double* __restrict__ p1, DevScalars* __restrict__ sc,
+               double* __restrict__ partials, int max_iter, double tol) {
+  if (sc->converged) return;
+  constexpr int kWaves = kUpdBlock / 64;
+  __shared__ double s_d2[kWaves], s_n2[kWaves];
+  __shared__ int s_last;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
+  double* __restrict__ pnew = (k & 1) ? p1 : p0;
+  const double inv = sc->inv_scale;
+  const bool add_seed = seed && pinit && k == 1;
+  double d2 = 0.0, n2 = 0.0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double po = pold[v];
+    double pn = po;
+    if (fixed) {
+      const unsigned int c = cnt[v];
+      if (c > 0) {
+        double tot = (double)sumfx[v] * inv;
+        if (add_seed) tot += pinit[v];
+        pn = tot / (double)c;
+      }
+    } else {
+      const double w = wsum[v];
+      if (w > 0.0) {
+        double tot = sumd[v];
+        if (add_seed) tot += pinit[v];
+        pn = tot / w;
+      }
+    }
+    pnew[v] = pn;
+    const bool is_src = sid ? sid[v] >= 0 : src[v] != 0;
+    if (is_src) {
+      const double df = pn - po;
+      d2 += df * df;
+      n2 += pn * pn;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+  }
+  if (lane == 0) {
+    s_d2[wave] = d2;
+    s_n2[wave] = n2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kWaves; w++) {
+      a += s_d2[w];
+      b += s_n2[w];
+    }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = b;
+    __threadfence();
+    const unsigned int t =
+        __hip_atomic_fetch_add(&sc->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double a = 0.0, b = 0.0;
+  for (unsigned int i = threadIdx.x; i < gridDim.x; i += kUpdBlock) {
+    a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_d2[wave] = a;
+    s_n2[wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0.0;
+    b = 0.0;
+    for (int w = 0; w < kWaves; w++) {
+      a += s_d2[w];
+      b += s_n2[w];
+    }
+    const double num = sqrt(a), den = sqrt(b);
+    const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
+    sc->rel = rel;
+    sc->dnum = num;
+    sc->dden = den;
+    sc->iter_done = k;
+    if ((tol > 0.0 && rel < tol) || k >= max_iter) {
+      sc->converged = 1;
+      sc->kstar = k;
+    }
+    sc->ticket = 0;
+  }
+}
+
+// Output of a naive reconstruction: the pressure of iteration k* and the
+// weights of its pass.
+__global__ void __launch_bounds__(256)
+k_naive_output(int64_t V, const double* __restrict__ p0, const double* __restrict__ p1,
+               const DevScalars* __restrict__ sc, int fixed, const unsigned int* __restrict__ cnt,
+               const double* __restrict__ wsum, double* __restrict__ pd, float* __restrict__ pf,
+               double* __restrict__ wd, float* __restrict__ wf) {
+  const double* __restrict__ p = (sc->kstar & 1) ? p1 : p0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double x = p[v];
+    const double w = fixed ? (double)cnt[v] : wsum[v];
+    if (pd) pd[v] = x;
+    if (pf) pf[v] = (float)x;
+    if (wd) wd[v] = w;
+    if (wf) wf[v] = (float)w;
+  }
+}
+
+// |curl a| * mask (calCurlofMaterialAccCPU, kernel.cu:8352-8379): central
+// differences inside, one-sided at the walls, evaluated in FP64 in the
+// reference's operation order.
+template <typename Load>
+__device__ __forceinline__ double curl_at(int X, int Y, int Z, int i, int j, int k, double dx,
+                                          double dy, double dz, const Load& ld) {
+  const int i0 = i - 1 >= 0 ? i - 1 : i, j0 = j - 1 >= 0 ? j - 1 : j, k0 = k - 1 >= 0 ? k - 1 : k;
+  const int ie = i + 1 <= X - 1 ? i + 1 : i, je = j + 1 <= Y - 1 ? j + 1 : j,
+            ke = k + 1 <= Z - 1 ? k + 1 : k;
+  auto at = [&](int c, int a, int b, int d) {
+    return ld(c, (int64_t)a + (int64_t)b * X + (int64_t)d * X * Y);
+  };
+  double cx = (at(2, i, je, k) - at(2, i, j0, k)) / (double)(je - j0) / dy;
+  cx += -(at(1, i, j, ke) - at(1, i, j, k0)) / (double)(ke - k0) / dz;
+  double cy = -(at(2, ie, j, k) - at(2, i0, j, k)) / (double)(ie - i0) / dx;
+  cy += (at(0, i, j, ke) - at(0, i, j, k0)) / (double)(ke - k0) / dz;
+  double cz = (at(1, ie, j, k) - at(1, i0, j, k)) / (double)(ie - i0) / dx;
+  cz += -(at(0, i, je, k) - at(0, i, j0, k)) / (double)(je - j0) / dy;
+  return sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+template <typename V4>
+__global__ void __launch_bounds__(256)
+k_curl_packed(int X, int Y, int Z, double dx, double dy, double dz, const V4* __restrict__ f,
+              const double* __restrict__ mask, double* __restrict__ out) {
+  const int64_t V = (int64_t)X * Y * Z;
+  auto ld = [&](int c, int64_t v) -> double { return (double)component(f[v], c); };
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(v % X), j = (int)((v / X) % Y), k = (int)(v / ((int64_t)X * Y));
+    const double c = curl_at(X, Y, Z, i, j, k, dx, dy, dz, ld);
+    out[v] = c * (mask ? mask[v] : 1.0);
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+k_curl_arrays(int X, int Y, int Z, double dx, double dy, double dz, const S* __restrict__ ax,
+              const S* __restrict__ ay, const S* __restrict__ az, const S* __restrict__ mask,
+              S* __restrict__ out) {
+  const int64_t V = (int64_t)X * Y * Z;
+  const S* comp[3] = {ax, ay, az};
+  auto ld = [&](int c, int64_t v) -> double { return (double)comp[c][v]; };
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(v % X), j = (int)((v / X) % Y), k = (int)(v / ((int64_t)X * Y));
+    const double c = curl_at(X, Y, Z, i, j, k, dx, dy, dz, ld);
+    out[v] = (S)(c * (mask ? (double)mask[v] : 1.0));
+  }
+}
+
+// Fills n 32-bit words (a kernel, not a memset: it may be captured in a graph).
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned int* __restrict__ x, int64_t n,
+                                                  unsigned int v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f64(double* __restrict__ x, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+
+template <typename T>
+hipError_t launch_walk_naive(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                             GridDesc g, const WalkParams& wp, const void* field,
+                             const double* aux, const double* p0, const double* p1, int k,
+                             const DevScalars* sc, long long* sumfx, unsigned int* wcnt,
+                             double* sumd, double* wsum, uint8_t* src, hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
+  const unsigned int grid = grid_for(nrays, 1u << 30);
+  const V4* f = static_cast<const V4*>(field);
+#define OMNI_NAIVE(M)                                                                          \
+  hipLaunchKernelGGL((k_walk_naive<T, M>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g, \
+                     wp, f, aux, p0, p1, k, sc, sumfx, wcnt, sumd, wsum, src)
+  switch (method) {
+    case 0: OMNI_NAIVE(0); break;
+    case 1: OMNI_NAIVE(1); break;
+    case 2: OMNI_NAIVE(2); break;
+    default: OMNI_NAIVE(3); break;
+  }
+#undef OMNI_NAIVE
+  return hipGetLastError();
+}
+template hipError_t launch_walk_naive<float>(int, const RayRec*, int64_t, const DirConst*,
+                                             GridDesc, const WalkParams&, const void*,
+                                             const double*, const double*, const double*, int,
+                                             const DevScalars*, long long*, unsigned int*,
+                                             double*, double*, uint8_t*, hipStream_t);
+template hipError_t launch_walk_naive<double>(int, const RayRec*, int64_t, const DirConst*,
+                                              GridDesc, const WalkParams&, const void*,
+                                              const double*, const double*, const double*, int,
+                                              const DevScalars*, long long*, unsigned int*,
+                                              double*, double*, uint8_t*, hipStream_t);
+
+unsigned int naive_update_grid(int64_t V) { return grid_for(V, 2048); }
+
+hipError_t launch_naive_update(int k, int64_t V, int fixed, const long long* sumfx,
+                               const unsigned int* cnt, const double* sumd, const double* wsum,
+                               const double* pinit, int seed, const int* sid, const uint8_t* src,
+                               double* p0, double* p1, DevScalars* sc, double* partials,
+                               int max_iter, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_naive_update, dim3(naive_update_grid(V)), dim3(kUpdBlock), 0, st, k, V,
+                     fixed, sumfx, cnt, sumd, wsum, pinit, seed, sid, src, p0, p1, sc, partials,
+                     max_iter, tol);
+  return hipGetLastError();
+}
+
+hipError_t launch_naive_output(int64_t V, const double* p0, const double* p1,
+                               const DevScalars* sc, int fixed, const unsigned int* cnt,
+                               const double* wsum, double* pd, float* pf, double* wd, float* wf,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_naive_output, dim3(grid_for(V)), dim3(256), 0, st, V, p0, p1, sc, fixed,
+                     cnt, wsum, pd, pf, wd, wf);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_curl_packed(int X, int Y, int Z, double dx, double dy, double dz,
+                              const void* field, const double* mask, double* out,
+                              hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
+  const int64_t V = (int64_t)X * Y * Z;
+  hipLaunchKernelGGL((k_curl_packed<V4>), dim3(grid_for(V)), dim3(256), 0, st, X, Y, Z, dx, dy,
+                     dz, static_cast<const V4*>(field), mask, out);
+  return hipGetLastError();
+}
+template hipError_t launch_curl_packed<float>(int, int, int, double, double, double, const void*,
+                                              const double*, double*, hipStream_t);
+template hipError_t launch_curl_packed<double>(int, int, int, double, double, double,
+                                               const void*, const double*, double*, hipStream_t);
+
+hipError_t launch_curl_f32(int X, int Y, int Z, double dx, double dy, double dz, const float* ax,
+                           const float* ay, const float* az, const float* mask, float* out,
+                           hipStream_t st) {
+  const int64_t V = (int64_t)X * Y * Z;
+  hipLaunchKernelGGL((k_curl_arrays<float>), dim3(grid_for(V)), dim3(256), 0, st, X, Y, Z, dx,
+                     dy, dz, ax, ay, az, mask, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(void* x, int64_t words, unsigned int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(words)), dim3(256), 0, st,
+                     static_cast<unsigned int*>(x), words, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* x, int64_t n, double v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n)), dim3(256), 0, st, x, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace omni3d

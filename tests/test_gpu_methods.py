@@ -1,0 +1,137 @@
+"""GPU parity of the reference-schedule iteration (a full traversal per
+iteration, kernel.cu:8840-8861) for every integration method, against the
+FP64 oracle on the same seeded inputs, through the C-ABI.
+
+Tolerances: FP64 rel-L2 <= 1e-12 for plain and plain-mask (fixed-point sums,
+exact unit weights, bit-exact counts), <= 1e-10 for select / weighted (FP64
+atomics: the summation order varies); FP32-Kahan rel-L2 <= 1e-5 against the
+FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import omni3d
+from omni3d import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(f, dirs, shape, mask=None, precision=omni3d.FP64, dtype=np.float64, p0=None, **kw):
+    with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision, **kw) as ctx:
+        ctx.set_directions(dirs)
+        ctx.set_field([np.asarray(a, dtype=dtype) for a in f["acc"]], mask=mask, dtype=dtype)
+        return ctx.reconstruct(p0)
+
+
+def sphere(shape, r):
+    Z, Y, X = shape
+    k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    c = [(X - 1) / 2, (Y - 1) / 2, (Z - 1) / 2]
+    return (np.sqrt((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2) > r).astype(np.float64)
+
+
+def test_naive_plain_matches_oracle_and_affine(gpu_available, ref_dirs2):
+    shape = (20, 18, 16)
+    print("case naive plain fp64", flush=True)
+    f = F.taylor_green(20, noise=0.05, seed=1, shape=shape)
+    p, w, rep = gpu(f, ref_dirs2, shape, solver=omni3d.NAIVE, max_iterations=10)
+    pa, wa, repa = gpu(f, ref_dirs2, shape, solver=omni3d.AFFINE, max_iterations=10)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], f["acc"],
+                                    max_iter=10)
+    assert rep.iterations == it == repa.iterations
+    np.testing.assert_array_equal(w, co)
+    assert F.rel_l2(p, po) < 1e-12
+    assert F.rel_l2(p, pa) < 1e-12
+    assert abs(rep.rel_change - rel) <= 1e-9 * rel
+
+
+def test_naive_tolerance_stop(gpu_available, ref_dirs2):
+    shape = (14, 15, 16)
+    print("case naive tolerance", flush=True)
+    f = F.taylor_green(16, noise=0.02, seed=2, shape=shape)
+    rng = np.random.default_rng(5)
+    p0 = rng.normal(size=shape)
+    p, w, rep = gpu(f, ref_dirs2, shape, solver=omni3d.NAIVE, max_iterations=60, tolerance=1e-4,
+                    p0=p0)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], f["acc"],
+                                    p0=p0, max_iter=60, tol=1e-4)
+    assert rep.iterations == it < 60
+    assert F.rel_l2(p, po) < 1e-11
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_plain_mask_sphere(gpu_available, ref_dirs3, precision):
+    shape = (22, 20, 24)
+    print(f"case plain-mask sphere precision={precision}", flush=True)
+    f = F.taylor_green(24, noise=0.05, seed=3, shape=shape)
+    m = sphere(shape, 5.5)
+    p, w, rep = gpu(f, ref_dirs3, shape, mask=m, precision=precision,
+                    method=omni3d.PLAIN_MASK, max_iterations=8)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs3, 1.0, f["delta"], f["rho"], f["acc"], curl=m,
+                                    method=O.PLAIN_MASK, max_iter=8)
+    assert rep.iterations == it
+    np.testing.assert_array_equal(w, co)
+    assert np.all(w[m == 0] == 0)
+    err = F.rel_l2(p, po)
+    print("plain-mask rel-L2 vs oracle", err)
+    assert err < (1e-12 if precision == omni3d.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("faithful", [1, 0])
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_select(gpu_available, ref_dirs2, faithful, precision):
+    shape = (18, 20, 19)
+    print(f"case select faithful={faithful} precision={precision}", flush=True)
+    f = F.taylor_green(20, noise=0.05, seed=4, shape=shape)
+    # the GPU computes the curl from the field it holds: give the oracle the
+    # same (FP32-rounded) field so the threshold tests see the same values
+    acc = [np.asarray(a, np.float32).astype(np.float64) if precision else a for a in f["acc"]]
+    f = dict(f, acc=acc)
+    m = sphere(shape, 4.0)
+    curl = O.curl(shape, f["delta"], acc, m)
+    thr = float(np.quantile(curl[m > 0], 0.8))
+    p, w, rep = gpu(f, ref_dirs2, shape, mask=m, precision=precision, method=omni3d.SELECT,
+                    threshold=thr, faithful_select=faithful, max_iterations=6)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc, curl=curl,
+                                    threshold=thr, method=O.SELECT, faithful=faithful,
+                                    max_iter=6)
+    assert rep.iterations == it
+    assert F.rel_l2(w, co) < 1e-12
+    err = F.rel_l2(p, po)
+    print("select rel-L2 vs oracle", err)
+    assert err < (1e-10 if precision == omni3d.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_weighted(gpu_available, ref_dirs2, precision):
+    shape = (16, 17, 18)
+    print(f"case weighted precision={precision}", flush=True)
+    f = F.taylor_green(18, noise=0.05, seed=5, shape=shape)
+    acc = [np.asarray(a, np.float32).astype(np.float64) if precision else a for a in f["acc"]]
+    f = dict(f, acc=acc)
+    curl = O.curl(shape, f["delta"], acc)
+    p, w, rep = gpu(f, ref_dirs2, shape, precision=precision, method=omni3d.WEIGHTED,
+                    max_iterations=6)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc, curl=curl,
+                                    method=O.WEIGHTED, max_iter=6)
+    assert F.rel_l2(w, co) < 1e-12
+    err = F.rel_l2(p, po)
+    print("weighted rel-L2 vs oracle", err)
+    assert err < (1e-10 if precision == omni3d.FP64 else 1e-5)
+
+
+def test_fourier_sphere_masked_fp32(gpu_available, ref_dirs3):
+    """The 128^3 masked Fourier-sphere workload at 32^3: plain-mask, FP32-Kahan."""
+    n = 32
+    print("case fourier sphere 32^3", flush=True)
+    f = F.fourier_sphere(n, seed=0)
+    shape = (n, n, n)
+    p, w, rep = gpu(f, ref_dirs3, shape, mask=f["mask"], precision=omni3d.FP32_KAHAN,
+                    method=omni3d.PLAIN_MASK, max_iterations=5)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs3, 1.0, f["delta"], f["rho"], f["acc"],
+                                    curl=f["mask"], method=O.PLAIN_MASK, max_iter=5)
+    np.testing.assert_array_equal(w, co)
+    err = F.rel_l2(p, po, mask=f["mask"])
+    print("fourier-sphere fp32 rel-L2 vs FP64 oracle", err)
+    assert err < 1e-5
