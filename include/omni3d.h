@@ -189,11 +189,22 @@ int omni3d_sor_solve(int X, int Y, int Z, double omega, int max_iter, double eps
                      const double* rhs, double* p, int* iterations_out, double* residual_out);
 
 /* The reference's CPU entry points, same signatures (sor3D.h:5, calRHS.h:4),
- * executed on the GPU. */
+ * executed on the GPU with host buffers.  sor3D: damped Jacobi (weight 0.8)
+ * over the BCScheme range, two sweeps per NoItr pair, stop when the mean
+ * |interior residual| < eps (then p = pn over the whole grid); x, y, z are
+ * unused, as in the reference.  calRHS writes RHS (the reference computes it
+ * into a local copy and loses it, calRHS.cpp:10).  Being void, they report
+ * failures only through omni3d_last_global_error(). */
 void sor3D(int Xsize, int Ysize, int Zsize, int NoItr, int* BCScheme, double* x, double* y,
            double* z, double* p, double* pn, double* RHS, double eps);
 void calRHS(int Xsize, int Ysize, int Zsize, double deltx, double delty, double deltz,
             double density, double* DuDt, double* DvDt, double* DwDt, double* RHS);
+/* The same two with a status return; omni3d_sor3d_ex also returns the
+ * number of sweeps run and the last residue (may be NULL). */
+int omni3d_sor3d_ex(int X, int Y, int Z, int NoItr, const int* BCScheme, double* p, double* pn,
+                    const double* rhs, double eps, int* sweeps_out, double* residue_out);
+int omni3d_calrhs(int X, int Y, int Z, double dx, double dy, double dz, double density,
+                  const double* DuDt, const double* DvDt, const double* DwDt, double* rhs);
 
 /* Process-level drop-in for Omni3D_8.0.exe (kernel.cu:8467-8983): reads the
  * 22-line Parameter_Omni3D.dat at `path`, reconstructs every file of the

@@ -11,23 +11,15 @@
 #include <string>
 #include <vector>
 
+#include "driver.hpp"
 #include "host.hpp"
 #include "launch.hpp"
 
 using namespace omni3d;
 
-namespace {
+namespace omni3d {
 
 thread_local std::string g_last_error;
-
-#define OMNI_TRY_HIP(ctx, expr)                                                   \
-  do {                                                                            \
-    hipError_t _e = (expr);                                                       \
-    if (_e != hipSuccess) {                                                       \
-      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(_e);             \
-      return OMNI3D_ERR_HIP;                                                      \
-    }                                                                             \
-  } while (0)
 
 int fail(omni3d_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -507,7 +499,10 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   return OMNI3D_OK;
 }
 
-}  // namespace
+template int run_naive<float>(omni3d_ctx*, int, void*, void*, int);
+template int run_naive<double>(omni3d_ctx*, int, void*, void*, int);
+
+}  // namespace omni3d
 
 extern "C" {
 

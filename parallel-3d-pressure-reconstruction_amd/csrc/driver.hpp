@@ -1,0 +1,39 @@
+// Internal driver functions shared by the C-ABI files (api.cpp defines them).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "host.hpp"
+
+namespace omni3d {
+
+extern thread_local std::string g_last_error;
+
+// Records msg on the context (if any) and as the global last error.
+int fail(omni3d_ctx* ctx, int code, const std::string& msg);
+int64_t voxels(const omni3d_params& p);
+size_t elem_size(int dtype);
+bool params_ok(const omni3d_params& p, std::string* why);
+// Geometry plan of the context's grid, direction set and line spacing
+// (rebuilt only when one of them changed).
+int build_plan(omni3d_ctx* ctx);
+WalkParams walk_params(const omni3d_params& P);
+// Reference-schedule iteration; the result of iteration k* goes to
+// pdst/wdst (device, dtype), seed: -1 no initial pressure, 0 p only,
+// 1 p and the seed term.
+template <typename T>
+int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
+
+}  // namespace omni3d
+
+#define OMNI_TRY_HIP(ctx, expr)                                                   \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      const std::string _m = std::string(#expr) + ": " + hipGetErrorString(_e);  \
+      return omni3d::fail((ctx), OMNI3D_ERR_HIP, _m);                             \
+    }                                                                             \
+  } while (0)

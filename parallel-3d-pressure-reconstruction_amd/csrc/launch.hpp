@@ -69,6 +69,10 @@ hipError_t launch_curl_f32(int X, int Y, int Z, double dx, double dy, double dz,
                            hipStream_t st);
 hipError_t launch_fill_u32(void* x, int64_t words, unsigned int v, hipStream_t st);
 hipError_t launch_fill_f64(double* x, int64_t n, double v, hipStream_t st);
+hipError_t launch_accumulate_out(int64_t V, int fixed, const long long* sumfx,
+                                 const unsigned int* cnt, const double* sumd, const double* wsum,
+                                 const DevScalars* sc, float* pn, float* pc, hipStream_t st);
+hipError_t launch_divide_count(int64_t V, float* p, float* pn, const float* pc, hipStream_t st);
 
 // solve.hip ----------------------------------------------------------------
 template <typename T>

@@ -380,6 +380,40 @@ __global__ void __launch_bounds__(256) k_fill_u32(unsigned int* __restrict__ x, 
     x[i] = v;
 }
 
+// Adds one pass's sums and weights into the caller's FP32 pn / pcountinner
+// (the kernel-level drop-ins accumulate like the reference kernels do).
+__global__ void __launch_bounds__(256)
+k_accumulate_out(int64_t V, int fixed, const long long* __restrict__ sumfx,
+                 const unsigned int* __restrict__ cnt, const double* __restrict__ sumd,
+                 const double* __restrict__ wsum, const DevScalars* __restrict__ sc,
+                 float* __restrict__ pn, float* __restrict__ pc) {
+  const double inv = sc->inv_scale;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double s = fixed ? (double)sumfx[v] * inv : sumd[v];
+    const double w = fixed ? (double)cnt[v] : wsum[v];
+    if (w != 0.0 || s != 0.0) {
+      pn[v] = (float)((double)pn[v] + s);
+      pc[v] = (float)((double)pc[v] + w);
+    }
+  }
+}
+
+// devidecountInner (kernel.cu:6746-6759): where pcountinner > 0,
+// p = pn / pcountinner and pn = 0.
+__global__ void __launch_bounds__(256)
+k_divide_count(int64_t V, float* __restrict__ p, float* __restrict__ pn,
+               const float* __restrict__ pc) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const float c = pc[v];
+    if (c > 0.0f) {
+      p[v] = pn[v] / c;
+      pn[v] = 0.0f;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_fill_f64(double* __restrict__ x, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -470,6 +504,19 @@ hipError_t launch_curl_f32(int X, int Y, int Z, double dx, double dy, double dz,
 hipError_t launch_fill_u32(void* x, int64_t words, unsigned int v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(words)), dim3(256), 0, st,
                      static_cast<unsigned int*>(x), words, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate_out(int64_t V, int fixed, const long long* sumfx,
+                                 const unsigned int* cnt, const double* sumd, const double* wsum,
+                                 const DevScalars* sc, float* pn, float* pc, hipStream_t st) {
+  hipLaunchKernelGGL(k_accumulate_out, dim3(grid_for(V)), dim3(256), 0, st, V, fixed, sumfx, cnt,
+                     sumd, wsum, sc, pn, pc);
+  return hipGetLastError();
+}
+
+hipError_t launch_divide_count(int64_t V, float* p, float* pn, const float* pc, hipStream_t st) {
+  hipLaunchKernelGGL(k_divide_count, dim3(grid_for(V)), dim3(256), 0, st, V, p, pn, pc);
   return hipGetLastError();
 }
 

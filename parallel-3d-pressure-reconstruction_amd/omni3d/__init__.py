@@ -27,7 +27,8 @@ EXPORTS = [
     "omni3d_load_directions_file", "omni3d_generate_directions", "omni3d_set_field",
     "omni3d_reconstruct", "omni3d_es_inner", "omni3d_es_inner_weighted",
     "omni3d_es_inner_select", "omni3d_devide_count_inner", "omni3d_curl", "omni3d_forward",
-    "omni3d_backward", "omni3d_sor_solve", "sor3D", "calRHS", "omni3d_run_parameter_file",
+    "omni3d_backward", "omni3d_sor_solve", "sor3D", "calRHS", "omni3d_sor3d_ex",
+    "omni3d_calrhs", "omni3d_run_parameter_file",
 ]
 
 
@@ -104,6 +105,9 @@ def _declare(L):
     L.sor3D.restype = None
     L.calRHS.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
     L.calRHS.restype = None
+    L.omni3d_sor3d_ex.argtypes = [i, i, i, i, C.POINTER(C.c_int), dp, dp, dp, d,
+                                  C.POINTER(C.c_int), dp]
+    L.omni3d_calrhs.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
     L.omni3d_run_parameter_file.argtypes = [C.c_char_p]
 
 
@@ -125,6 +129,46 @@ def generate_directions(divide):
     L.omni3d_generate_directions(divide, k[0].ctypes.data_as(fp), k[1].ctypes.data_as(fp),
                                  k[2].ctypes.data_as(fp), n)
     return np.ascontiguousarray(k.T)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def calrhs(shape, delta, density, acc):
+    """calRHS on the GPU (host arrays, FP64); returns the RHS volume."""
+    Z, Y, X = shape
+    a = [np.ascontiguousarray(c, dtype=np.float64) for c in acc]
+    out = np.zeros(shape)
+    check(lib().omni3d_calrhs(X, Y, Z, delta[0], delta[1], delta[2], density, _dptr(a[0]),
+                              _dptr(a[1]), _dptr(a[2]), _dptr(out)))
+    return out
+
+
+def sor3d(p, rhs, noitr, eps, bcscheme=(0, 0, 0, 0, 0, 0), pn=None):
+    """sor3D on the GPU; returns (p, pn, sweeps, residue)."""
+    Z, Y, X = p.shape
+    p = np.ascontiguousarray(p, dtype=np.float64).copy()
+    pn = np.zeros_like(p) if pn is None else np.ascontiguousarray(pn, dtype=np.float64).copy()
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    bc = (C.c_int * 6)(*bcscheme)
+    sw = C.c_int(0)
+    res = C.c_double(0)
+    check(lib().omni3d_sor3d_ex(X, Y, Z, noitr, bc, _dptr(p), _dptr(pn), _dptr(rhs), eps,
+                                C.byref(sw), C.byref(res)))
+    return p, pn, sw.value, res.value
+
+
+def sor_solve(rhs, p0=None, omega=1.8, max_iter=1000, eps=0.0):
+    """Red-black SOR on the GPU; returns (p, iterations, residual)."""
+    Z, Y, X = rhs.shape
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    p = np.zeros_like(rhs) if p0 is None else np.ascontiguousarray(p0, dtype=np.float64).copy()
+    it = C.c_int(0)
+    res = C.c_double(0)
+    check(lib().omni3d_sor_solve(X, Y, Z, omega, max_iter, eps, _dptr(rhs), _dptr(p),
+                                 C.byref(it), C.byref(res)))
+    return p, it.value, res.value
 
 
 def default_params(**kw):
