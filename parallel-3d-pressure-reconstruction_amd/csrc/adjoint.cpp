@@ -1,0 +1,132 @@
+// C-ABI of the adjoint pair: omni3d_forward (g = grad p -> p) and
+// omni3d_backward (dL/dp -> dL/dg) of the plain / plain-mask reconstruction
+// with zero initial pressure and the context's iteration count.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "driver.hpp"
+#include "host.hpp"
+#include "launch.hpp"
+
+using namespace omni3d;
+
+namespace {
+
+int adjoint_checks(omni3d_ctx* ctx) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  const omni3d_params& P = ctx->prm;
+  if (P.method != OMNI3D_METHOD_PLAIN && P.method != OMNI3D_METHOD_PLAIN_MASK)
+    return fail(ctx, OMNI3D_ERR_ARG,
+                "the adjoint exists for the linear methods (plain, plain-mask) only");
+  if (ctx->k.empty()) return fail(ctx, OMNI3D_ERR_STATE, "no directions set");
+  return OMNI3D_OK;
+}
+
+int io_dtype(const omni3d_ctx* ctx) {
+  return ctx->prm.precision == OMNI3D_FP64 ? OMNI3D_F64 : OMNI3D_F32;
+}
+
+}  // namespace
+
+extern "C" {
+
+int omni3d_forward(omni3d_ctx* ctx, const void* grad_d, void* p_d) {
+  int rc = adjoint_checks(ctx);
+  if (rc) return rc;
+  if (!grad_d || !p_d) return fail(ctx, OMNI3D_ERR_ARG, "null pointer");
+  const int64_t V = voxels(ctx->prm);
+  const int dt = io_dtype(ctx);
+  const size_t esz = elem_size(dt);
+  const char* g = static_cast<const char*>(grad_d);
+  // the gradient becomes the context's field, integrated with density -1
+  // (-rho * a = g); the mask set before stays
+  const bool had_mask = ctx->have_mask;
+  rc = omni3d_set_field(ctx, g, g + esz * V, g + 2 * esz * V, nullptr, dt, OMNI3D_MEM_DEVICE);
+  ctx->have_mask = had_mask;
+  if (rc) return rc;
+  const omni3d_params saved = ctx->prm;
+  ctx->prm.density = -1.0;
+  ctx->prm.tolerance = 0.0;
+  ctx->prm.seed_pn_with_p = 0;
+  // zero initial pressure, written into the output buffer
+  hipError_t e = dt == OMNI3D_F64 ? launch_fill_f64(static_cast<double*>(p_d), V, 0.0, ctx->stream)
+                                  : launch_fill_u32(p_d, V, 0u, ctx->stream);
+  if (e != hipSuccess) {
+    ctx->prm = saved;
+    return fail(ctx, OMNI3D_ERR_HIP, hipGetErrorString(e));
+  }
+  rc = omni3d_reconstruct(ctx, p_d, nullptr, dt, OMNI3D_MEM_DEVICE, nullptr);
+  ctx->prm = saved;
+  return rc;
+}
+
+int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
+  int rc = adjoint_checks(ctx);
+  if (rc) return rc;
+  if (!dldp_d || !dldgrad_d) return fail(ctx, OMNI3D_ERR_ARG, "null pointer");
+  rc = build_plan(ctx);
+  if (rc) return rc;
+  const omni3d_params& P = ctx->prm;
+  const Plan& pl = ctx->plan;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  const int dt = io_dtype(ctx);
+  const bool mask_rule = P.method == OMNI3D_METHOD_PLAIN_MASK;
+  const double* mask = nullptr;
+  if (mask_rule) {
+    OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
+    if (ctx->have_mask)
+      OMNI_TRY_HIP(ctx, (launch_convert<double, double>(ctx->mask.as<double>(), ctx->aux.as<double>(), V, st)));
+    else
+      OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->aux.as<double>(), V, 1.0, st));
+    mask = ctx->aux.as<double>();
+  }
+  // visit counts C of the method
+  const unsigned int* cnt = pl.cnt.as<unsigned int>();
+  if (mask_rule) {
+    OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+    OMNI_TRY_HIP(ctx, launch_adj_count(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(), pl.g,
+                                       mask, ctx->wcnt.as<unsigned int>(), st));
+    cnt = ctx->wcnt.as<unsigned int>();
+  }
+  for (DevBuf* b : {&ctx->adj_y, &ctx->adj_z, &ctx->adj_u, &ctx->adj_t})
+    OMNI_TRY_HIP(ctx, b->alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->adj_g.alloc(sizeof(double) * 3 * V));
+  double* y = ctx->adj_y.as<double>();
+  double* z = ctx->adj_z.as<double>();
+  double* u = ctx->adj_u.as<double>();
+  double* t = ctx->adj_t.as<double>();
+  double* dg = ctx->adj_g.as<double>();
+  if (dt == OMNI3D_F64)
+    OMNI_TRY_HIP(ctx, (launch_convert<double, double>(static_cast<const double*>(dldp_d), y, V, st)));
+  else
+    OMNI_TRY_HIP(ctx, (launch_convert<float, double>(static_cast<const float*>(dldp_d), y, V, st)));
+  OMNI_TRY_HIP(ctx, (launch_convert<double, double>(y, z, V, st)));
+  // z = sum_{j<K} (A^T)^j y
+  for (int m = 1; m < P.max_iterations; m++) {
+    OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
+    OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
+    OMNI_TRY_HIP(ctx, launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
+                                        pl.dirs.as<DirConst>(), pl.g, mask, u, t, st));
+    OMNI_TRY_HIP(ctx, launch_adj_horner(V, y, t, cnt, z, st));
+  }
+  // dL/dg = B^T (z / C)
+  OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
+  OMNI_TRY_HIP(ctx, launch_fill_f64(dg, 3 * V, 0.0, st));
+  omni3d_params pg = P;
+  pg.density = -1.0;
+  const WalkParams wp = walk_params(pg);
+  OMNI_TRY_HIP(ctx, launch_adj_scatter(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
+                                       pl.dirs.as<DirConst>(), pl.g, wp, mask, u, dg, V, st));
+  if (dt == OMNI3D_F64)
+    OMNI_TRY_HIP(ctx, (launch_convert<double, double>(dg, static_cast<double*>(dldgrad_d), 3 * V, st)));
+  else
+    OMNI_TRY_HIP(ctx, (launch_convert<double, float>(dg, static_cast<float*>(dldgrad_d), 3 * V, st)));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  return OMNI3D_OK;
+}
+
+}  // extern "C"

@@ -114,6 +114,8 @@ struct omni3d_ctx {
   // naive schedule: pressure double buffer, aux field (mask or |curl a|*mask),
   // curl-weighted sums, unit visit counts, flags of the voxels read
   omni3d::DevBuf pa, pb, aux, sumd, wsum, wcnt, srcf;
+  // adjoint: dL/dp, Horner sum, scaled sum, gathered sums, dL/dg (3 V)
+  omni3d::DevBuf adj_y, adj_z, adj_u, adj_t, adj_g;
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
   hipEvent_t ev[8] = {};

@@ -74,6 +74,21 @@ hipError_t launch_accumulate_out(int64_t V, int fixed, const long long* sumfx,
                                  const DevScalars* sc, float* pn, float* pc, hipStream_t st);
 hipError_t launch_divide_count(int64_t V, float* p, float* pn, const float* pc, hipStream_t st);
 
+// adjoint.hip --------------------------------------------------------------
+hipError_t launch_adj_gather(bool mask_rule, const RayRec* rays, int64_t nrays,
+                             const DirConst* dirs, GridDesc g, const double* mask,
+                             const double* u, double* t, hipStream_t st);
+hipError_t launch_adj_scatter(bool mask_rule, const RayRec* rays, int64_t nrays,
+                              const DirConst* dirs, GridDesc g, const WalkParams& wp,
+                              const double* mask, const double* u, double* dg, int64_t V,
+                              hipStream_t st);
+hipError_t launch_adj_count(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
+                            const double* mask, unsigned int* cnt, hipStream_t st);
+hipError_t launch_adj_scale(int64_t V, const double* z, const unsigned int* cnt, double* u,
+                            hipStream_t st);
+hipError_t launch_adj_horner(int64_t V, const double* y, const double* t,
+                             const unsigned int* cnt, double* z, hipStream_t st);
+
 // solve.hip ----------------------------------------------------------------
 template <typename T>
 hipError_t launch_absmax(const T* x, int64_t n, unsigned long long* bits, hipStream_t st);

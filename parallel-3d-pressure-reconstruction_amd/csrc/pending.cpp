@@ -7,7 +7,5 @@ static int pending(const char* what) {
   (void)what;
   return OMNI3D_ERR_STATE;
 }
-int omni3d_forward(omni3d_ctx*, const void*, void*) { return pending("fwd"); }
-int omni3d_backward(omni3d_ctx*, const void*, void*) { return pending("bwd"); }
 int omni3d_run_parameter_file(const char*) { return -1; }
 }

@@ -245,6 +245,15 @@ class Context:
                                            C.byref(rep)))
         return p, w, rep
 
+    def forward_device(self, grad_ptr, p_ptr):
+        """p = F(g) for g = grad p (3 components, c*V + v), device buffers in
+        the context's precision (double for FP64, float for FP32_KAHAN)."""
+        self._chk(lib().omni3d_forward(self._h, grad_ptr, p_ptr))
+
+    def backward_device(self, dldp_ptr, dldg_ptr):
+        """dL/dg = F^T dL/dp, device buffers as for forward_device."""
+        self._chk(lib().omni3d_backward(self._h, dldp_ptr, dldg_ptr))
+
     def reconstruct_device(self, p_ptr, w_ptr=None, dtype=F64, report=True):
         rep = Report()
         self._chk(lib().omni3d_reconstruct(self._h, p_ptr, w_ptr, dtype, DEVICE,
