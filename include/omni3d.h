@@ -114,6 +114,12 @@ const char* omni3d_last_error(const omni3d_ctx* ctx);
 const char* omni3d_last_global_error(void);
 int omni3d_set_stream(omni3d_ctx* ctx, void* hip_stream);
 
+/* Tile engine shape of the plain reconstruction: tile edge 2^tile_bits voxels
+ * (3: 8^3, 4: 16^3 -- a segment then holds at most 22 / 46 visits) and the
+ * workgroup size of its passes (256, 512, 1024).  Default 4 / 1024; the
+ * batch autotuner sets them per grid.  No reference counterpart. */
+int omni3d_set_tile_shape(omni3d_ctx* ctx, int tile_bits, int threads);
+
 /* Direction set (kernel.cu:8622-8672: float k1,k2,k3[NoAngles]); vectors are
  * normalised in FP64 on the host. */
 int omni3d_set_directions(omni3d_ctx* ctx, const float* k1, const float* k2, const float* k3,
@@ -205,6 +211,34 @@ int omni3d_sor3d_ex(int X, int Y, int Z, int NoItr, const int* BCScheme, double*
                     const double* rhs, double eps, int* sweeps_out, double* residue_out);
 int omni3d_calrhs(int X, int Y, int Z, double dx, double dy, double dz, double density,
                   const double* DuDt, const double* DvDt, const double* DwDt, double* rhs);
+
+/* Snapshot batches (no reference counterpart; the reference loops over the
+ * files of a folder, kernel.cu:8650-8960).  `count` snapshots of grids
+ * nx[i] x ny[i] x nz[i] sharing the other parameters of `base`; snapshots of
+ * one grid share one geometry plan.  omni3d_batch_run reconstructs every
+ * snapshot from a zero pressure: use_graph = 1 replays one hipGraph holding
+ * every snapshot's reconstruction (captured on the first run over four
+ * fork-join branches; convergence is per snapshot), use_graph = 0 queues
+ * them one by one with a synchronisation each.  *ms_out gets the GPU time.
+ * omni3d_batch_tune times the tile shapes (8^3 / 16^3 tiles x 256 / 512 /
+ * 1024 threads) on the first snapshot of each distinct grid and keeps the
+ * fastest; it returns the number of distinct grids (< 0: error) and writes
+ * the choices in the order of the grids' first snapshots (arrays may be
+ * NULL).  Results: omni3d_batch_result copies snapshot i's pressure
+ * (dtype/mem_kind) and fills its report (iterations, rel_change, sizes). */
+typedef struct omni3d_batch omni3d_batch;
+int omni3d_batch_create(const omni3d_params* base, int count, const int* nx, const int* ny,
+                        const int* nz, omni3d_batch** out);
+void omni3d_batch_destroy(omni3d_batch* b);
+const char* omni3d_batch_last_error(const omni3d_batch* b);
+int omni3d_batch_set_directions(omni3d_batch* b, const float* k1, const float* k2,
+                                const float* k3, int count);
+int omni3d_batch_set_field(omni3d_batch* b, int index, const void* dudt, const void* dvdt,
+                           const void* dwdt, const void* mask, int dtype, int mem_kind);
+int omni3d_batch_tune(omni3d_batch* b, int* tile_bits_out, int* threads_out);
+int omni3d_batch_run(omni3d_batch* b, int use_graph, double* ms_out);
+int omni3d_batch_result(omni3d_batch* b, int index, void* p, int dtype, int mem_kind,
+                        omni3d_report* rep);
 
 /* Process-level drop-in for Omni3D_8.0.exe (kernel.cu:8467-8983): reads the
  * 22-line Parameter_Omni3D.dat at `path`, reconstructs every file of the

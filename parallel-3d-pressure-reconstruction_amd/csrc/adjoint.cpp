@@ -69,7 +69,7 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   rc = build_plan(ctx);
   if (rc) return rc;
   const omni3d_params& P = ctx->prm;
-  const Plan& pl = ctx->plan;
+  const Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   const int dt = io_dtype(ctx);

@@ -104,9 +104,11 @@ void number_sources(int X, int Y, int Z, const std::vector<uint8_t>& flag, std::
 
 // Tile engine part of the plan: compiled step codes and per-tile segments.
 int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
-  Plan& pl = ctx->plan;
+  Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   TileGrid tg;
+  tg.bits = ctx->tile_bits;
+  const int kTile = 1 << tg.bits;
   tg.ntx = (g.X + kTile - 1) / kTile;
   tg.nty = (g.Y + kTile - 1) / kTile;
   tg.ntz = (g.Z + kTile - 1) / kTile;
@@ -133,7 +135,6 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   if (nsegs >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
   OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
-  OMNI_TRY_HIP(ctx, pl.segval.alloc(sizeof(double) * std::max<unsigned long long>(nsegs, 1)));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
   OMNI_TRY_HIP(ctx, launch_tile_fill(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
                                      pl.sid.as<int>(), pl.seg_base.as<unsigned long long>(),
@@ -161,11 +162,16 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
 // of visits that land on source voxels.
 int build_plan(omni3d_ctx* ctx) {
   const omni3d_params& P = ctx->prm;
-  Plan& pl = ctx->plan;
   const int D = (int)(ctx->k.size() / 3);
-  if (pl.valid && !ctx->dirs_dirty && pl.X == P.nx && pl.Y == P.ny && pl.Z == P.nz &&
-      pl.ls == P.line_spacing && pl.D == D)
-    return OMNI3D_OK;
+  {
+    const Plan& cur = *ctx->plan;
+    if (cur.valid && !ctx->dirs_dirty && cur.X == P.nx && cur.Y == P.ny && cur.Z == P.nz &&
+        cur.ls == P.line_spacing && cur.D == D && cur.tg.bits == ctx->tile_bits)
+      return OMNI3D_OK;
+  }
+  // a plan shared with other contexts stays theirs: build a new one
+  if (ctx->plan.use_count() > 1) ctx->plan = std::make_shared<Plan>();
+  Plan& pl = *ctx->plan;
   pl.valid = false;
   hipStream_t st = ctx->stream;
   const int X = P.nx, Y = P.ny, Z = P.nz;
@@ -306,6 +312,11 @@ int build_plan(omni3d_ctx* ctx) {
   return OMNI3D_OK;
 }
 
+// Timing events of the phases (not recorded while a graph is being captured).
+void record_event(omni3d_ctx* ctx, int i) {
+  if (!ctx->capturing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
+}
+
 WalkParams walk_params(const omni3d_params& P) {
   WalkParams wp;
   wp.rho = P.density;
@@ -325,14 +336,14 @@ WalkParams walk_params(const omni3d_params& P) {
 template <typename T>
 int run_affine(omni3d_ctx* ctx, int seed) {
   const omni3d_params& P = ctx->prm;
-  Plan& pl = ctx->plan;
+  Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   DevScalars* sc = ctx->scal.as<DevScalars>();
   const void* field = ctx->field.as<void>();
   const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
   const WalkParams wp = walk_params(P);
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(sc, 0, sizeof(DevScalars), st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
   // max |a| over the packed field (the fourth component is 0)
   OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
   if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
@@ -342,27 +353,30 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   // pass A: trapezoid sums landing on source voxels
   const size_t ns = (size_t)std::max(pl.nS, 1);
   OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->bs.as<void>(), 0, sizeof(long long) * ns, st));
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->bs.as<void>(), 2 * (int64_t)ns, 0u, st));
+  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nsegs, 1)));
+  record_event(ctx, 2);
   if (ctx->engine == 0) {
     // segment sums over all tiles, per-ray carries, then the source sums on
     // the surface tiles
-    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
+                                          pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          pl.segval.as<double>(), nullptr, st));
+                                          ctx->segval.as<double>(), nullptr, st));
     OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.seg_base.as<unsigned long long>(),
-                                        pl.segval.as<double>(), st));
-    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, pl.nsurf, pl.surf.as<int>(),
-                                          pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(),
-                                          pl.g, pl.tg, wp, field, pl.sid.as<int>(), nullptr, nullptr,
-                                          sc, pl.segval.as<double>(), ctx->bs.as<long long>(), st));
+                                        ctx->segval.as<double>(), st));
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, ctx->tile_threads, pl.nsurf,
+                                          pl.surf.as<int>(), pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
+                                          pl.sid.as<int>(), nullptr, nullptr, sc,
+                                          ctx->segval.as<double>(), ctx->bs.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
                                            nullptr, nullptr, sc, ctx->bs.as<long long>(), 0, 1, st));
   }
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+  record_event(ctx, 3);
   // source iteration
   OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * ns));
   OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns));
@@ -378,26 +392,27 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                      seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
                                      ctx->partials.as<double>(), P.max_iterations, P.tolerance, st));
   }
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+  record_event(ctx, 4);
   // pass B: every voxel from p_S^{k*-1}
   const int nc = ctx->engine == 0 ? 1 : ctx->acc_copies;
   ctx->used_copies = nc;
   OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V * nc));
   if (ctx->engine == 0) {
     // every voxel belongs to exactly one tile, which stores its sum
-    OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
-    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, pl.ntiles, nullptr, pl.tile_ptr.as<unsigned long long>(),
+    record_event(ctx, 5);
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, ctx->tile_threads, pl.ntiles, nullptr,
+                                          pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
-                                          pl.segval.as<double>(), ctx->sum.as<long long>(), st));
+                                          ctx->segval.as<double>(), ctx->sum.as<long long>(), st));
   } else {
-    OMNI_TRY_HIP(ctx, hipMemsetAsync(ctx->sum.as<void>(), 0, sizeof(long long) * V * nc, st));
-    OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V * nc, 0u, st));
+    record_event(ctx, 5);
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
                                            pbuf[0], pbuf[1], sc, ctx->sum.as<long long>(), V, nc, st));
   }
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], st));
+  record_event(ctx, 6);
   return OMNI3D_OK;
 }
 
@@ -408,7 +423,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
 template <typename T>
 int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const omni3d_params& P = ctx->prm;
-  Plan& pl = ctx->plan;
+  Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   const int m = P.method;
@@ -461,8 +476,8 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+  record_event(ctx, 2);
+  record_event(ctx, 3);
   for (int k = 1; k <= P.max_iterations; k++) {
     if (fixed) {
       OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
@@ -487,15 +502,45 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
                                           ctx->partials.as<double>(), P.max_iterations,
                                           P.tolerance, st));
   }
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[5], st));
-  OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[6], st));
+  record_event(ctx, 4);
+  record_event(ctx, 5);
+  record_event(ctx, 6);
   const bool f64 = dtype == OMNI3D_F64;
   OMNI_TRY_HIP(ctx, launch_naive_output(V, pa, pb, sc, fixed ? 1 : 0, cntw,
                                         fixed ? nullptr : ctx->wsum.as<double>(),
                                         f64 ? (double*)pdst : nullptr, f64 ? nullptr : (float*)pdst,
                                         (wdst && f64) ? (double*)wdst : nullptr,
                                         (wdst && !f64) ? (float*)wdst : nullptr, st));
+  return OMNI3D_OK;
+}
+
+// Queues a whole reconstruction of the context's field on its stream (no
+// host synchronisation; every buffer must already have its size when this
+// runs under graph capture).  The result goes to pdst / wdst (device, dtype).
+int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
+  const omni3d_params& P = ctx->prm;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  int rc;
+  const bool f64 = dtype == OMNI3D_F64;
+  const bool fp32 = P.precision == OMNI3D_FP32_KAHAN;
+  // the source-voxel (affine) schedule exists for the plain method; every
+  // other method, and the NAIVE solver, walks the whole volume per iteration
+  if (P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN) {
+    rc = fp32 ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
+    if (rc) return rc;
+    OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
+                                      ctx->plan->cnt.as<unsigned int>(),
+                                      seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
+                                      ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
+                                      f64 ? nullptr : (float*)pdst,
+                                      (wdst && f64) ? (double*)wdst : nullptr,
+                                      (wdst && !f64) ? (float*)wdst : nullptr, st));
+  } else {
+    rc = fp32 ? run_naive<float>(ctx, seed, pdst, wdst, dtype)
+              : run_naive<double>(ctx, seed, pdst, wdst, dtype);
+    if (rc) return rc;
+  }
   return OMNI3D_OK;
 }
 
@@ -567,6 +612,17 @@ void omni3d_destroy(omni3d_ctx* ctx) {
 int omni3d_set_stream(omni3d_ctx* ctx, void* s) {
   if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
   ctx->stream = static_cast<hipStream_t>(s);
+  return OMNI3D_OK;
+}
+
+int omni3d_set_tile_shape(omni3d_ctx* ctx, int tile_bits, int threads) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  if (tile_bits < kTileBitsMin || tile_bits > kTileBitsMax)
+    return fail(ctx, OMNI3D_ERR_ARG, "tile_bits must be 3 (8^3 tiles) or 4 (16^3 tiles)");
+  if (threads != 256 && threads != 512 && threads != 1024)
+    return fail(ctx, OMNI3D_ERR_ARG, "threads must be 256, 512 or 1024");
+  ctx->tile_bits = tile_bits;
+  ctx->tile_threads = threads;
   return OMNI3D_OK;
 }
 
@@ -675,7 +731,7 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
   const omni3d_params& P = ctx->prm;
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
-  const bool had_plan = ctx->plan.valid && !ctx->dirs_dirty;
+  const bool had_plan = ctx->plan->valid && !ctx->dirs_dirty;
   OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[0], st));
   int rc = build_plan(ctx);
   if (rc) return rc;
@@ -716,25 +772,8 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     OMNI_TRY_HIP(ctx, ptmp.alloc(esz * V));
     pdst = ptmp.as<void>();
   }
-  const bool f64 = dtype == OMNI3D_F64;
-  const bool fp32 = P.precision == OMNI3D_FP32_KAHAN;
-  // the source-voxel (affine) schedule exists for the plain method; every
-  // other method, and the NAIVE solver, walks the whole volume per iteration
-  if (P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN) {
-    rc = fp32 ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
-    if (rc) return rc;
-    OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
-                                      ctx->plan.cnt.as<unsigned int>(),
-                                      seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,
-                                      ctx->scal.as<DevScalars>(), f64 ? (double*)pdst : nullptr,
-                                      f64 ? nullptr : (float*)pdst,
-                                      (wdst && f64) ? (double*)wdst : nullptr,
-                                      (wdst && !f64) ? (float*)wdst : nullptr, st));
-  } else {
-    rc = fp32 ? run_naive<float>(ctx, seed, pdst, wdst, dtype)
-              : run_naive<double>(ctx, seed, pdst, wdst, dtype);
-    if (rc) return rc;
-  }
+  rc = enqueue_solve(ctx, seed, pdst, wdst, dtype);
+  if (rc) return rc;
   OMNI_TRY_HIP(ctx, hipEventRecord(ctx->ev[7], st));
   if (mem_kind == OMNI3D_MEM_HOST) {
     if (p) OMNI_TRY_HIP(ctx, hipMemcpyAsync(p, pdst, esz * V, hipMemcpyDeviceToHost, st));
@@ -747,9 +786,9 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
   if (rep) {
     rep->iterations = hs.kstar;
     rep->rel_change = hs.rel;
-    rep->rays = ctx->plan.nrays;
-    rep->steps = ctx->plan.steps;
-    rep->sources = ctx->plan.nS;
+    rep->rays = ctx->plan->nrays;
+    rep->steps = ctx->plan->steps;
+    rep->sources = ctx->plan->nS;
     rep->status = hs.status;
     auto ms = [&](int a, int b) {
       float t = 0;
@@ -762,9 +801,9 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->ms_iterate = ms(3, 4);
     rep->ms_pass_b = ms(5, 6);
     rep->ms_finalize = ms(6, 7);
-    rep->source_visits = ctx->plan.nnz;
-    rep->segments = ctx->plan.nsegs;
-    rep->sell_cells = ctx->plan.ncell;
+    rep->source_visits = ctx->plan->nnz;
+    rep->segments = ctx->plan->nsegs;
+    rep->sell_cells = ctx->plan->ncell;
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");
   return OMNI3D_OK;

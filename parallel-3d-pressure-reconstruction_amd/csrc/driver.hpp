@@ -26,6 +26,11 @@ WalkParams walk_params(const omni3d_params& P);
 // 1 p and the seed term.
 template <typename T>
 int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
+// A whole reconstruction queued on the context's stream without host
+// synchronisation (capturable once the buffers are sized); pdst / wdst device.
+int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
+// Phase timing event i of the context (skipped during graph capture).
+void record_event(omni3d_ctx* ctx, int i);
 
 }  // namespace omni3d
 

@@ -103,7 +103,7 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   if (rc) return fail(nullptr, rc, ctx->err);
   rc = build_plan(ctx);
   if (rc) return fail(nullptr, rc, ctx->err);
-  const Plan& pl = ctx->plan;
+  const Plan& pl = *ctx->plan;
   const int64_t V = (int64_t)X * Y * Z;
   DevScalars* sc = ctx->scal.as<DevScalars>();
   const bool fixed = method == OMNI3D_METHOD_PLAIN;

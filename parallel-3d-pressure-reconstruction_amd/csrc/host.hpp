@@ -6,6 +6,7 @@
 
 #include <array>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -83,7 +84,6 @@ struct Plan {
   DevBuf seg_base;  // uint64[nrays+1]
   DevBuf tile_ptr;  // uint64[ntiles+1]
   DevBuf segs;      // SegRec[nsegs], grouped by tile
-  DevBuf segval;    // double[nsegs] segment sums, then carries
   DevBuf surf;      // int32[nsurf] tiles that hold source voxels
 };
 
@@ -95,7 +95,11 @@ struct omni3d_ctx {
   std::string err;
   std::vector<float> k;  // 3*D, row-major (k1,k2,k3)
   bool dirs_dirty = true;
-  omni3d::Plan plan;
+  // geometry plan; contexts of one grid may share it (snapshot batches)
+  std::shared_ptr<omni3d::Plan> plan = std::make_shared<omni3d::Plan>();
+  int tile_bits = omni3d::kTileBitsDefault;  // tile edge 2^bits of the tile engine
+  int tile_threads = 1024;                   // workgroup size of the tile passes
+  bool capturing = false;                    // work is being captured into a graph
   // field in the compute precision, one 4-vector (a_x, a_y, a_z, 0) per voxel
   omni3d::DevBuf field, mask;
   bool have_field = false;
@@ -107,6 +111,7 @@ struct omni3d_ctx {
   omni3d::DevBuf ps0, ps1;  // double[nS]
   omni3d::DevBuf partials;  // double[2*grid]
   omni3d::DevBuf sum;       // int64[ncopies*V]
+  omni3d::DevBuf segval;    // double[nsegs] segment sums, then carries
   int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
   int used_copies = 1;

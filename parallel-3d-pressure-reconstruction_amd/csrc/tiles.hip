@@ -22,7 +22,8 @@
 namespace omni3d {
 
 __device__ __forceinline__ int tile_of(int i, int j, int k, const TileGrid& tg) {
-  return (i >> kTileBits) + (j >> kTileBits) * tg.ntx + (k >> kTileBits) * tg.ntx * tg.nty;
+  const int b = tg.bits;
+  return (i >> b) + (j >> b) * tg.ntx + (k >> b) * tg.ntx * tg.nty;
 }
 
 // ---------------------------------------------------------------------------
@@ -84,11 +85,11 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
   SegRec sr;
   uint32_t nvis = 1, ncode = 0, first = 1;
+  const int tb = tg.bits, tmask = (1 << tb) - 1;
   auto start = [&](int i, int j, int k) {
     sr.g = (uint32_t)gseg;
     sr.src = src;
-    const uint32_t lv = (uint32_t)((i & kTileMask) | ((j & kTileMask) << kTileBits) |
-                                   ((k & kTileMask) << (2 * kTileBits)));
+    const uint32_t lv = (uint32_t)((i & tmask) | ((j & tmask) << tb) | ((k & tmask) << (2 * tb)));
     sr.info = lv | (signs << 18) | (first << 21);
     sr.code[0] = sr.code[1] = sr.code[2] = 0;
   };
@@ -130,16 +131,14 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 // ---------------------------------------------------------------------------
 // Tile passes.
 
-template <typename T>
+template <typename T, int kVox>
 struct TileLds {
-  T g[3][kTileVox];
-  long long acc[kTileVox];
+  T g[3][kVox];
+  long long acc[kVox];
 };
 
-constexpr int kTileThreads = 1024;
-
-template <typename T, int kMode>
-__global__ void __launch_bounds__(kTileThreads)
+template <typename T, int kMode, int kBits, int kThreads>
+__global__ void __launch_bounds__(kThreads)
 k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restrict__ tile_ptr,
             const SegRec* __restrict__ segs, GridDesc g, TileGrid tg, WalkParams wp,
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
@@ -147,7 +146,11 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
             const DevScalars* __restrict__ sc, double* __restrict__ segval,
             long long* __restrict__ out) {
   using V4 = typename Vec4<T>::type;
-  __shared__ TileLds<T> L;
+  constexpr int kTileBits = kBits;
+  constexpr int kTile = 1 << kBits;
+  constexpr int kTileMask = kTile - 1;
+  constexpr int kTileVox = kTile * kTile * kTile;
+  __shared__ TileLds<T, kTileVox> L;
   const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
   const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
   const int ox = tx << kTileBits, oy = ty << kTileBits, oz = tz << kTileBits;
@@ -461,8 +464,38 @@ hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
   return hipGetLastError();
 }
 
+template <typename T, int kMode, int kBits>
+static void tile_pass_threads(int threads, int ntiles, const int* tile_list,
+                              const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                              TileGrid tg, const WalkParams& wp,
+                              const typename Vec4<T>::type* f, const int* sid, const double* p0,
+                              const double* p1, const DevScalars* sc, double* segval,
+                              long long* out, hipStream_t st) {
+#define OMNI_TILE(NT)                                                                        \
+  hipLaunchKernelGGL((k_tile_pass<T, kMode, kBits, NT>), dim3(ntiles), dim3(NT), 0, st,       \
+                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out)
+  if (threads <= 256) OMNI_TILE(256);
+  else if (threads <= 512) OMNI_TILE(512);
+  else OMNI_TILE(1024);
+#undef OMNI_TILE
+}
+
+template <typename T, int kMode>
+static void tile_pass_bits(int threads, int ntiles, const int* tile_list,
+                           const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                           TileGrid tg, const WalkParams& wp, const typename Vec4<T>::type* f,
+                           const int* sid, const double* p0, const double* p1,
+                           const DevScalars* sc, double* segval, long long* out, hipStream_t st) {
+  if (tg.bits == 3)
+    tile_pass_threads<T, kMode, 3>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
+                                   p0, p1, sc, segval, out, st);
+  else
+    tile_pass_threads<T, kMode, 4>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
+                                   p0, p1, sc, segval, out, st);
+}
+
 template <typename T>
-hipError_t launch_tile_pass(int mode, int ntiles, const int* tile_list,
+hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,
@@ -470,23 +503,24 @@ hipError_t launch_tile_pass(int mode, int ntiles, const int* tile_list,
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   if (ntiles <= 0) return hipSuccess;
+  if (tg.bits < kTileBitsMin || tg.bits > kTileBitsMax) return hipErrorInvalidValue;
   if (mode == kLocalSum)
-    hipLaunchKernelGGL((k_tile_pass<T, kLocalSum>), dim3(ntiles), dim3(kTileThreads), 0, st,
-                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
+    tile_pass_bits<T, kLocalSum>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
+                                 p0, p1, sc, segval, out, st);
   else if (mode == kSourceSums)
-    hipLaunchKernelGGL((k_tile_pass<T, kSourceSums>), dim3(ntiles), dim3(kTileThreads), 0, st,
-                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
+    tile_pass_bits<T, kSourceSums>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
+                                   p0, p1, sc, segval, out, st);
   else
-    hipLaunchKernelGGL((k_tile_pass<T, kFull>), dim3(ntiles), dim3(kTileThreads), 0, st,
-                       tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out);
+    tile_pass_bits<T, kFull>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0,
+                             p1, sc, segval, out, st);
   return hipGetLastError();
 }
 
-template hipError_t launch_tile_pass<float>(int, int, const int*, const unsigned long long*,
+template hipError_t launch_tile_pass<float>(int, int, int, const int*, const unsigned long long*,
                                             const SegRec*, GridDesc, TileGrid, const WalkParams&,
                                             const void*, const int*, const double*, const double*,
                                             const DevScalars*, double*, long long*, hipStream_t);
-template hipError_t launch_tile_pass<double>(int, int, const int*, const unsigned long long*,
+template hipError_t launch_tile_pass<double>(int, int, int, const int*, const unsigned long long*,
                                              const SegRec*, GridDesc, TileGrid,
                                              const WalkParams&, const void*, const int*,
                                              const double*, const double*, const DevScalars*,

@@ -9,16 +9,19 @@
 
 namespace omni3d {
 
-constexpr int kTileBits = 4;
-constexpr int kTile = 1 << kTileBits;  // 16^3 voxels per tile
-constexpr int kTileMask = kTile - 1;
-constexpr int kTileVox = kTile * kTile * kTile;
+// Tile edge 2^bits voxels: 8^3 (bits 3) or 16^3 (bits 4), chosen per grid
+// (the autotuner's "tile shape"); a segment then holds at most 3 * 2^bits - 2
+// visits (22 or 46: its "segment length").
+constexpr int kTileBitsMin = 3;
+constexpr int kTileBitsMax = 4;
+constexpr int kTileBitsDefault = 4;
 
 enum { kLocalSum = 0, kSourceSums = 1, kFull = 2 };
 constexpr int kLenBuckets = 12;  // segment-length groups per tile (4 visits each)
 
 struct TileGrid {
   int ntx, nty, ntz;
+  int bits;
 };
 
 // One segment: the visits of one ray inside one tile, self-contained so that
@@ -41,8 +44,9 @@ hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* d
                             TileGrid tg, const int* sid, const unsigned long long* seg_base,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
                             SegRec* segs, hipStream_t st);
+// threads: workgroup size of the tile passes, 256, 512 or 1024
 template <typename T>
-hipError_t launch_tile_pass(int mode, int ntiles, const int* tile_list,
+hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,

@@ -28,7 +28,10 @@ EXPORTS = [
     "omni3d_reconstruct", "omni3d_es_inner", "omni3d_es_inner_weighted",
     "omni3d_es_inner_select", "omni3d_devide_count_inner", "omni3d_curl", "omni3d_forward",
     "omni3d_backward", "omni3d_sor_solve", "sor3D", "calRHS", "omni3d_sor3d_ex",
-    "omni3d_calrhs", "omni3d_run_parameter_file",
+    "omni3d_calrhs", "omni3d_run_parameter_file", "omni3d_set_tile_shape",
+    "omni3d_batch_create", "omni3d_batch_destroy", "omni3d_batch_last_error",
+    "omni3d_batch_set_directions", "omni3d_batch_set_field", "omni3d_batch_tune",
+    "omni3d_batch_run", "omni3d_batch_result",
 ]
 
 
@@ -109,6 +112,18 @@ def _declare(L):
                                   C.POINTER(C.c_int), dp]
     L.omni3d_calrhs.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
     L.omni3d_run_parameter_file.argtypes = [C.c_char_p]
+    L.omni3d_set_tile_shape.argtypes = [vp, i, i]
+    ip = C.POINTER(C.c_int)
+    L.omni3d_batch_create.argtypes = [C.POINTER(Params), i, ip, ip, ip, C.POINTER(vp)]
+    L.omni3d_batch_destroy.argtypes = [vp]
+    L.omni3d_batch_destroy.restype = None
+    L.omni3d_batch_last_error.argtypes = [vp]
+    L.omni3d_batch_last_error.restype = C.c_char_p
+    L.omni3d_batch_set_directions.argtypes = [vp, fp, fp, fp, i]
+    L.omni3d_batch_set_field.argtypes = [vp, i, vp, vp, vp, vp, i, i]
+    L.omni3d_batch_tune.argtypes = [vp, ip, ip]
+    L.omni3d_batch_run.argtypes = [vp, i, dp]
+    L.omni3d_batch_result.argtypes = [vp, i, vp, i, i, C.POINTER(Report)]
 
 
 def check(rc, ctx=None):
@@ -220,6 +235,9 @@ class Context:
     def load_directions_file(self, path, count=0):
         self._chk(lib().omni3d_load_directions_file(self._h, path.encode(), count))
 
+    def set_tile_shape(self, tile_bits, threads):
+        self._chk(lib().omni3d_set_tile_shape(self._h, tile_bits, threads))
+
     def set_field(self, acc, mask=None, dtype=np.float64):
         self._keep = [np.ascontiguousarray(a, dtype=dtype) for a in acc]
         m = None if mask is None else np.ascontiguousarray(mask, dtype=dtype)
@@ -259,3 +277,76 @@ class Context:
         self._chk(lib().omni3d_reconstruct(self._h, p_ptr, w_ptr, dtype, DEVICE,
                                            C.byref(rep) if report else None))
         return rep
+
+
+class Batch:
+    """omni3d_batch: snapshots of uneven grids reconstructed in one hipGraph."""
+
+    def __init__(self, shapes, delta=(1.0, 1.0, 1.0), **kw):
+        self.shapes = [tuple(s) for s in shapes]
+        n = len(self.shapes)
+        nx = (C.c_int * n)(*[s[2] for s in self.shapes])
+        ny = (C.c_int * n)(*[s[1] for s in self.shapes])
+        nz = (C.c_int * n)(*[s[0] for s in self.shapes])
+        self.params = default_params(dx=delta[0], dy=delta[1], dz=delta[2], **kw)
+        self._h = C.c_void_p()
+        check(lib().omni3d_batch_create(C.byref(self.params), n, nx, ny, nz, C.byref(self._h)))
+        self._keep = {}
+
+    def _chk(self, rc):
+        if rc != OK:
+            msg = lib().omni3d_batch_last_error(self._h) or b""
+            raise Omni3DError(rc, msg.decode(errors="replace"))
+
+    def close(self):
+        if self._h:
+            lib().omni3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_directions(self, k):
+        k = np.ascontiguousarray(np.asarray(k, dtype=np.float32).reshape(-1, 3).T)
+        fp = C.POINTER(C.c_float)
+        self._chk(lib().omni3d_batch_set_directions(self._h, k[0].ctypes.data_as(fp),
+                                                    k[1].ctypes.data_as(fp),
+                                                    k[2].ctypes.data_as(fp), k.shape[1]))
+
+    def set_field(self, i, acc, mask=None, dtype=np.float64):
+        keep = [np.ascontiguousarray(a, dtype=dtype) for a in acc]
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=dtype)
+        self._keep[i] = keep + [m]
+        dt = F64 if dtype == np.float64 else F32
+        self._chk(lib().omni3d_batch_set_field(self._h, i, keep[0].ctypes.data,
+                                               keep[1].ctypes.data, keep[2].ctypes.data,
+                                               None if m is None else m.ctypes.data, dt, HOST))
+
+    def tune(self):
+        n = len(self.shapes)
+        bits = (C.c_int * n)()
+        thr = (C.c_int * n)()
+        r = lib().omni3d_batch_tune(self._h, bits, thr)
+        if r < 0:
+            self._chk(-r)
+        return [(bits[j], thr[j]) for j in range(r)]
+
+    def run(self, use_graph=True):
+        ms = C.c_double(0)
+        self._chk(lib().omni3d_batch_run(self._h, 1 if use_graph else 0, C.byref(ms)))
+        return ms.value
+
+    def result(self, i):
+        p = np.zeros(self.shapes[i])
+        rep = Report()
+        self._chk(lib().omni3d_batch_result(self._h, i, p.ctypes.data, F64, HOST, C.byref(rep)))
+        return p, rep
