@@ -240,12 +240,43 @@ int omni3d_batch_run(omni3d_batch* b, int use_graph, double* ms_out);
 int omni3d_batch_result(omni3d_batch* b, int index, void* p, int dtype, int mem_kind,
                         omni3d_report* rep);
 
+/* |curl a| * mask of the context's field (calCurlofMaterialAccCPU,
+ * kernel.cu:8352-8379, in FP64 from the stored field) into out. */
+int omni3d_field_curl(omni3d_ctx* ctx, void* out, int dtype, int mem_kind);
+
+/* Parameter_Omni3D.dat (kernel.cu:8540-8571): lines 1-15 as below, line 16
+ * pref and lines 17-22 the region of interest (blank = 0 / whole grid). */
+typedef struct omni3d_parameter_file {
+  int nx, ny, nz;            /* lines 1-3 (the grid of the files, before the crop) */
+  double dx, dy, dz;         /* lines 4-6 */
+  double density;            /* line 7 */
+  int scale;                 /* line 8 (read, unused, as in the reference) */
+  double line_spacing;       /* line 9 */
+  int n_angles;              /* line 10: rows of the direction file to read */
+  char grid_file[1024];      /* line 11: direction file */
+  char acc_dir[1024];        /* line 12: acceleration folder */
+  char pressure_dir[1024];   /* line 13: output folder (created if missing) */
+  int n_iterations;          /* line 14 */
+  double threshold;          /* line 15: curl threshold of the Select method */
+  double pref;               /* line 16: Bernoulli reference pressure */
+  int roi[6];                /* lines 17-22: i0, i1, j0, j1, k0, k1 (inclusive) */
+} omni3d_parameter_file;
+int omni3d_read_parameter_file(const char* path, omni3d_parameter_file* out);
+
 /* Process-level drop-in for Omni3D_8.0.exe (kernel.cu:8467-8983): reads the
- * 22-line Parameter_Omni3D.dat at `path`, reconstructs every file of the
- * acceleration folder and writes PressureOmni3D_%05d.dat (+ the curl file
- * for file 0) and log.dat.  Returns the number of files processed (<0 on
- * error). */
+ * Parameter_Omni3D.dat at `path` (relative paths in it are taken from the
+ * file's folder, '\' read as '/'), and for every file of the acceleration
+ * folder (sorted; Tecplot POINT, 3 header lines, columns x y z u v w DuDt
+ * DvDt DwDt mask, i fastest) crops the region of interest, seeds the j = 0
+ * face with pref - rho |u|^2 / 2, runs the Select method (faithful, curl
+ * threshold of line 15) for the line-14 iterations and writes
+ * PressureOmni3D_%05d.dat (X Y Z P Count, %15.9f); file 0 also writes
+ * CurlofMaterialAcc_Sample.dat; log.dat lists the files.  Outputs and
+ * log.dat go beside the parameter file.  Returns the number of files
+ * reconstructed, or -(error code).  _ex chooses the precision
+ * (OMNI3D_FP64 / OMNI3D_FP32_KAHAN); the plain call uses FP64. */
 int omni3d_run_parameter_file(const char* path);
+int omni3d_run_parameter_file_ex(const char* path, int precision);
 
 #ifdef __cplusplus
 }

@@ -720,6 +720,35 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
   return OMNI3D_OK;
 }
 
+int omni3d_field_curl(omni3d_ctx* ctx, void* out, int dtype, int mem_kind) {
+  if (!ctx || !out) return fail(ctx, OMNI3D_ERR_ARG, "null argument");
+  if (!ctx->have_field) return fail(ctx, OMNI3D_ERR_STATE, "no field set");
+  if (dtype != OMNI3D_F32 && dtype != OMNI3D_F64) return fail(ctx, OMNI3D_ERR_ARG, "bad dtype");
+  const omni3d_params& P = ctx->prm;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  DevBuf c, c32;
+  OMNI_TRY_HIP(ctx, c.alloc(sizeof(double) * V));
+  const double* mask = ctx->have_mask ? ctx->mask.as<double>() : nullptr;
+  if (P.precision == OMNI3D_FP32_KAHAN)
+    OMNI_TRY_HIP(ctx, launch_curl_packed<float>(P.nx, P.ny, P.nz, P.dx, P.dy, P.dz,
+                                                ctx->field.as<void>(), mask, c.as<double>(), st));
+  else
+    OMNI_TRY_HIP(ctx, launch_curl_packed<double>(P.nx, P.ny, P.nz, P.dx, P.dy, P.dz,
+                                                 ctx->field.as<void>(), mask, c.as<double>(), st));
+  const void* src = c.as<void>();
+  if (dtype == OMNI3D_F32) {
+    OMNI_TRY_HIP(ctx, c32.alloc(sizeof(float) * V));
+    OMNI_TRY_HIP(ctx, (launch_convert<double, float>(c.as<double>(), c32.as<float>(), V, st)));
+    src = c32.as<void>();
+  }
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(out, src, elem_size(dtype) * V,
+                                   mem_kind == OMNI3D_MEM_HOST ? hipMemcpyDeviceToHost
+                                                               : hipMemcpyDeviceToDevice, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  return OMNI3D_OK;
+}
+
 int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, int mem_kind,
                        omni3d_report* rep) {
   if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");

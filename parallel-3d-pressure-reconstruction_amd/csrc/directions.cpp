@@ -153,12 +153,15 @@ bool read_direction_file(const char* path, int count, std::vector<float>* k, std
     return false;
   }
   k->clear();
+  // the reference takes the last three numbers of a row (it parses from the
+  // right, kernel.cu:8626-8647)
   for (int i = 0; i < count; i++) {
-    if (rows[i].size() < 3) {
+    const size_t n = rows[i].size();
+    if (n < 3) {
       *err = "direction file row " + std::to_string(i) + " has fewer than 3 values";
       return false;
     }
-    for (int c = 0; c < 3; c++) k->push_back((float)rows[i][c]);
+    for (size_t c = n - 3; c < n; c++) k->push_back((float)rows[i][c]);
   }
   return true;
 }

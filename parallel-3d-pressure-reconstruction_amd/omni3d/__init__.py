@@ -31,7 +31,8 @@ EXPORTS = [
     "omni3d_calrhs", "omni3d_run_parameter_file", "omni3d_set_tile_shape",
     "omni3d_batch_create", "omni3d_batch_destroy", "omni3d_batch_last_error",
     "omni3d_batch_set_directions", "omni3d_batch_set_field", "omni3d_batch_tune",
-    "omni3d_batch_run", "omni3d_batch_result",
+    "omni3d_batch_run", "omni3d_batch_result", "omni3d_field_curl",
+    "omni3d_read_parameter_file", "omni3d_run_parameter_file_ex",
 ]
 
 
@@ -43,6 +44,17 @@ class Params(C.Structure):
         ("method", C.c_int), ("threshold", C.c_double), ("faithful_select", C.c_int),
         ("precision", C.c_int), ("solver", C.c_int), ("max_iterations", C.c_int),
         ("tolerance", C.c_double), ("seed_pn_with_p", C.c_int),
+    ]
+
+
+class ParameterFile(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("density", C.c_double), ("scale", C.c_int), ("line_spacing", C.c_double),
+        ("n_angles", C.c_int), ("grid_file", C.c_char * 1024), ("acc_dir", C.c_char * 1024),
+        ("pressure_dir", C.c_char * 1024), ("n_iterations", C.c_int), ("threshold", C.c_double),
+        ("pref", C.c_double), ("roi", C.c_int * 6),
     ]
 
 
@@ -113,6 +125,9 @@ def _declare(L):
     L.omni3d_calrhs.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
     L.omni3d_run_parameter_file.argtypes = [C.c_char_p]
     L.omni3d_set_tile_shape.argtypes = [vp, i, i]
+    L.omni3d_field_curl.argtypes = [vp, vp, i, i]
+    L.omni3d_read_parameter_file.argtypes = [C.c_char_p, C.POINTER(ParameterFile)]
+    L.omni3d_run_parameter_file_ex.argtypes = [C.c_char_p, i]
     ip = C.POINTER(C.c_int)
     L.omni3d_batch_create.argtypes = [C.POINTER(Params), i, ip, ip, ip, C.POINTER(vp)]
     L.omni3d_batch_destroy.argtypes = [vp]
@@ -186,6 +201,21 @@ def sor_solve(rhs, p0=None, omega=1.8, max_iter=1000, eps=0.0):
     return p, it.value, res.value
 
 
+def read_parameter_file(path):
+    """Parameter_Omni3D.dat as a ParameterFile structure (host only)."""
+    pf = ParameterFile()
+    check(lib().omni3d_read_parameter_file(str(path).encode(), C.byref(pf)))
+    return pf
+
+
+def run_parameter_file(path, precision=FP64):
+    """The Omni3D_8.0.exe drop-in; returns the number of files reconstructed."""
+    n = lib().omni3d_run_parameter_file_ex(str(path).encode(), precision)
+    if n < 0:
+        check(-n)
+    return n
+
+
 def default_params(**kw):
     p = Params()
     lib().omni3d_default_params(C.byref(p))
@@ -234,6 +264,12 @@ class Context:
 
     def load_directions_file(self, path, count=0):
         self._chk(lib().omni3d_load_directions_file(self._h, path.encode(), count))
+
+    def field_curl(self):
+        """|curl a| * mask of the context's field (FP64 host array)."""
+        out = np.zeros(self.shape)
+        self._chk(lib().omni3d_field_curl(self._h, out.ctypes.data, F64, HOST))
+        return out
 
     def set_tile_shape(self, tile_bits, threads):
         self._chk(lib().omni3d_set_tile_shape(self._h, tile_bits, threads))

@@ -1,0 +1,102 @@
+"""End-to-end check of the process-level drop-in (Omni3D_8.0.exe,
+kernel.cu:8467-8983) on the GPU: a Parameter_Omni3D.dat with a region of
+interest, Windows-style folder names and two Tecplot acceleration files;
+the written PressureOmni3D_%05d.dat (P and Count columns),
+CurlofMaterialAcc_Sample.dat and log.dat against the FP64 oracle run of the
+same Select reconstruction (Bernoulli seed on the j = 0 face, curl * mask,
+threshold, fixed iteration count), to the 9 decimals the files hold.  The
+same folder is then run through the omni3d command-line program."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import omni3d
+from conftest import GOLDEN, ROOT
+from omni3d import fields as F
+
+pytestmark = pytest.mark.gpu
+
+ORG = (10, 12, 14)          # (Z, Y, X) of the files
+ROI = (1, 12, 0, 11, 1, 8)  # i0, i1, j0, j1, k0, k1
+DELTA, RHO, PREF, NITR = 0.3, 1.2, 2.0, 4
+
+
+def write_case(tmp, seeds):
+    Z, Y, X = ORG
+    os.makedirs(tmp / "accdir")
+    shutil.copy(os.path.join(GOLDEN, "GridIcosahedronDivide2.dat"), tmp)
+    cases = []
+    for n, seed in enumerate(seeds):
+        rng = np.random.default_rng(seed)
+        f = F.taylor_green(14, noise=0.05, seed=seed, shape=ORG, phase=tuple(rng.uniform(0, 6, 3)))
+        vel = [rng.normal(size=ORG) for _ in range(3)]
+        k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+        mask = (np.sqrt((i - 6.5) ** 2 + (j - 5.5) ** 2 + (k - 4.5) ** 2) > 2.5).astype(float)
+        xyz = [i * DELTA, j * DELTA, k * DELTA]
+        cols = xyz + vel + list(f["acc"]) + [mask]
+        with open(tmp / "accdir" / f"acc_{n:03d}.dat", "w") as fh:
+            fh.write('TITLE = "acc"\nVARIABLES = "X","Y","Z","U","V","W","DuDt","DvDt","DwDt","M"\n')
+            fh.write(f"ZONE I={X}, J={Y}, K={Z},F=POINT\n")
+            data = np.stack([c.reshape(-1) for c in cols], axis=1)
+            for row in data:
+                fh.write(" ".join(f"{v:.17g}" for v in row) + "\r\n")
+        cases.append((cols, mask))
+    return cases
+
+
+def crop(a):
+    i0, i1, j0, j1, k0, k1 = ROI
+    return np.ascontiguousarray(a[k0:k1 + 1, j0:j1 + 1, i0:i1 + 1])
+
+
+def expected(cols, thr, dirs):
+    x, y, z, u, v, w, ax, ay, az, m = [crop(c) for c in cols]
+    shape = ax.shape
+    acc = [ax, ay, az]
+    curl = O.curl(shape, (DELTA,) * 3, acc, m)
+    p0 = np.zeros(shape)
+    p0[:, 0, :] = PREF - 0.5 * RHO * (u[:, 0, :] ** 2 + v[:, 0, :] ** 2 + w[:, 0, :] ** 2)
+    p, cnt, it, rel = O.reconstruct(shape, dirs, 1.0, (DELTA,) * 3, RHO, acc, p0=p0, curl=curl,
+                                    threshold=thr, method=O.SELECT, faithful=1, max_iter=NITR,
+                                    seed_pn=1)
+    return p, cnt, curl, (x, y, z)
+
+
+def test_parameter_file_run(gpu_available, ref_dirs2, tmp_path):
+    print("case parameter file", flush=True)
+    cases = write_case(tmp_path, [11, 12])
+    cols0, m0 = cases[0]
+    c0 = O.curl(crop(cols0[6]).shape, (DELTA,) * 3, [crop(c) for c in cols0[6:9]], crop(m0))
+    thr = float(np.quantile(c0[c0 > 0], 0.85))
+    Z, Y, X = ORG
+    lines = [X, Y, Z, DELTA, DELTA, DELTA, RHO, 1, 1.0, 42, "GridIcosahedronDivide2.dat",
+             ".\\accdir\\", ".\\out\\", NITR, repr(thr), PREF] + list(ROI)
+    par = tmp_path / "Parameter_Omni3D.dat"
+    par.write_text("\r\n".join(str(v) for v in lines) + "\r\n")
+    assert omni3d.run_parameter_file(par) == 2
+    assert (tmp_path / "log.dat").read_text() == "Processing Starts\nacc_000.datacc_001.dat"
+    for n, (cols, m) in enumerate(cases):
+        p, cnt, curl, xyz = expected(cols, thr, ref_dirs2)
+        out = np.loadtxt(tmp_path / "out" / f"PressureOmni3D_{n:05d}.dat", skiprows=3)
+        shape = p.shape
+        assert out.shape == (p.size, 5)
+        for c in range(3):
+            np.testing.assert_allclose(out[:, c], xyz[c].reshape(-1), atol=1e-9)
+        np.testing.assert_allclose(out[:, 3], p.reshape(-1), atol=2e-9 * max(1, np.abs(p).max()))
+        np.testing.assert_allclose(out[:, 4], cnt.reshape(-1), atol=2e-9 * max(1, cnt.max()))
+        head = open(tmp_path / "out" / f"PressureOmni3D_{n:05d}.dat").read().split("\n")[:3]
+        assert head[2] == f"ZONE I={shape[2]}, J={shape[1]}, K={shape[0]},F=POINT"
+    cf = np.loadtxt(tmp_path / "CurlofMaterialAcc_Sample.dat", skiprows=3)
+    p0, cnt0, curl0, _ = expected(cases[0][0], thr, ref_dirs2)
+    np.testing.assert_allclose(cf[:, 3], curl0.reshape(-1), atol=2e-9 * max(1, curl0.max()))
+    # the command-line program gives the same files
+    shutil.rmtree(tmp_path / "out")
+    exe = os.path.join(ROOT, "parallel-3d-pressure-reconstruction_amd", "bin", "omni3d")
+    r = subprocess.run([exe, str(par)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = np.loadtxt(tmp_path / "out" / "PressureOmni3D_00000.dat", skiprows=3)
+    np.testing.assert_allclose(out[:, 3], p0.reshape(-1), atol=2e-9 * max(1, np.abs(p0).max()))

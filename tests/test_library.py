@@ -98,3 +98,34 @@ def test_fixture_direction_sets(ref_dirs2, ref_dirs3):
         np.testing.assert_allclose(np.linalg.norm(k, axis=1), 1.0, atol=2e-6)
         d = np.abs(k[:, None, :] + k[None, :, :]).max(axis=2)
         assert d.min(axis=1).max() < 0.02
+
+
+def test_parameter_file_reader():
+    """Parameter_Omni3D.dat (kernel.cu:8540-8571): the shipped file's first
+    22 lines, then a file with pref and a region of interest on lines 16-22."""
+    import os
+    from conftest import GOLDEN
+    pf = omni3d.read_parameter_file(os.path.join(GOLDEN, "Parameter_Omni3D.dat"))
+    assert (pf.nx, pf.ny, pf.nz) == (64, 64, 64)
+    assert (pf.dx, pf.dy, pf.dz, pf.density, pf.scale) == (0.00024, 0.00024, 0.00024, 1000.0, 1)
+    assert pf.line_spacing == 0.7 and pf.n_angles == 10242 and pf.n_iterations == 10
+    assert pf.grid_file == b"GridIcosahedronDivide6.dat"
+    assert pf.acc_dir == b".\\AccelerationFilesSample\\"
+    assert pf.threshold == 52921780.0 and pf.pref == 0.0
+    assert list(pf.roi) == [0, 63, 0, 63, 0, 63]
+
+
+def test_parameter_file_roi(tmp_path):
+    lines = ["12", "10", "8", "0.5", "0.5", "0.5", "1.2", "1", "1", "42",
+             "GridIcosahedronDivide2.dat", "acc", "out", "3", "7.5", "101325", "1", "10", "0", "8",
+             "2", "6"]
+    p = tmp_path / "Parameter_Omni3D.dat"
+    p.write_text("\r\n".join(lines) + "\r\n")
+    pf = omni3d.read_parameter_file(p)
+    assert (pf.nx, pf.ny, pf.nz, pf.n_iterations) == (12, 10, 8, 3)
+    assert pf.pref == 101325.0 and pf.threshold == 7.5
+    assert list(pf.roi) == [1, 10, 0, 8, 2, 6]
+    bad = tmp_path / "bad.dat"
+    bad.write_text("\n".join(lines[:16] + ["0", "12"] + lines[18:]) + "\n")
+    with pytest.raises(omni3d.Omni3DError):
+        omni3d.read_parameter_file(bad)
