@@ -7,14 +7,22 @@ on the gradient; 162 icosahedral directions (twice subdivided), line spacing
 source update and residual in FP64; stop at a relative change < 1e-6 of the
 source voxels or 200 iterations.  The field is resident on the GPU before
 the timed region; a step is one reconstruction of that snapshot with the
-geometry plan (rays, visit counts, source CSR) of the grid already built,
-as in the reference's batch loop over snapshots of one grid.  The plan build
-time and the cold total are reported beside it.
+geometry plan (rays, visit counts, source CSR, tile segments) of the grid
+already built, as in the reference's loop over the snapshots of one grid.
+The plan build time and the cold total are reported beside it.
 
-Also measured in the same run: the FP64 GPU path (rel-L2 of the FP32 result
-against it, speed ratio), the per-phase times from the library's HIP events
-on its stream, the roofline of the dominant kernel (the final traversal), and
-the oracle's single-thread CPU time on a bounded sample (cpu_baseline).
+Also measured in the same run: the FP64 GPU path on the same input (rel-L2
+of the FP32 result against it, speed ratio), the cost of one reference-
+schedule FP64 pass (a full traversal per iteration, as the reference
+iterates), the per-phase times from the library's HIP events on its stream,
+the roofline of the dominant kernel (the final full traversal), and the
+oracle's single-thread CPU time on a bounded sample (cpu_baseline).
+
+Other modes (not the headline): --copy-calibration runs a plain 1 GiB device
+copy for scaling the FETCH_SIZE / WRITE_SIZE counters; --batch times the
+64-snapshot uneven-grid series (one hipGraph vs one launch sequence per
+snapshot); --adjoint times forward + backward of the 128^3 masked
+Fourier-sphere reconstruction.
 """
 import argparse
 import json
@@ -28,10 +36,100 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parallel-3d-pressure-reconstruction_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tile_pass_full_burgers256.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calibration():
+    """A plain device copy of known size (1 GiB read + 1 GiB written) whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the kernels."""
+    import torch
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+
+
+def batch_mode(args):
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    series = F.snapshot_series(64, seed=0)
+    shapes = [s["shape"] for s in series]
+    dirs = omni3d.generate_directions(3)
+    h = 2 * np.pi / 64
+    with omni3d.Batch(shapes, (h, h, h), precision=omni3d.FP32_KAHAN,
+                      max_iterations=args.max_iter, tolerance=args.tol) as b:
+        b.set_directions(dirs)
+        for i, s in enumerate(series):
+            f = F.taylor_green(s["shape"][2], noise=s["noise"], seed=s["seed"], shape=s["shape"],
+                               phase=s["phase"])
+            b.set_field(i, f["acc"], dtype=np.float32)
+        t0 = time.perf_counter()
+        picks = b.tune()
+        tune_s = time.perf_counter() - t0
+        log(f"batch: tuned {picks} in {tune_s:.2f}s")
+        b.run(use_graph=False)
+        seq = [b.run(use_graph=False) for _ in range(args.steps)]
+        b.run(use_graph=True)
+        gr = [b.run(use_graph=True) for _ in range(args.steps)]
+        its = [b.result(i)[1].iterations for i in range(len(shapes))]
+        vox = int(sum(np.prod(s) for s in shapes))
+    ms_seq, ms_graph = float(np.mean(seq)), float(np.mean(gr))
+    print(json.dumps({
+        "metric": "snapshots per second, 64 uneven snapshots (one hipGraph)",
+        "value": round(64 / (ms_graph * 1e-3), 3), "unit": "snapshots/s", "n_gpus": 1,
+        "steps": args.steps, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
+        "dtype": "fp32-kahan ray sums, fp64 averaging", "data": "synthetic",
+        "config": {"workload": "taylor-green series, 64 snapshots from {64^3, 96^3, 128x96x64, "
+                               "160x128x96}, 0-5% noise, 162 directions, plain affine",
+                   "voxels": vox, "tile_choices": picks, "tune_s": round(tune_s, 3),
+                   "ms_sequential": round(ms_seq, 3), "speedup_graph_vs_sequential":
+                   round(ms_seq / ms_graph, 3), "iterations_min_max": [min(its), max(its)]}}))
+
+
+def adjoint_mode(args):
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    n = 128
+    f = F.fourier_sphere(n, seed=0)
+    V = n ** 3
+    dirs = omni3d.generate_directions(3)
+    ctx = omni3d.Context((n, n, n), f["delta"], method=omni3d.PLAIN_MASK,
+                         precision=omni3d.FP32_KAHAN, max_iterations=args.max_iter)
+    ctx.set_directions(dirs)
+    ctx.set_field([np.zeros((n, n, n))] * 3, mask=f["mask"])
+    g = torch.from_numpy(np.concatenate([c.reshape(-1) for c in f["grad"]]).astype(np.float32)).cuda()
+    p = torch.empty(V, dtype=torch.float32, device="cuda")
+    y = torch.randn(V, dtype=torch.float32, device="cuda")
+    dg = torch.empty(3 * V, dtype=torch.float32, device="cuda")
+    ctx.forward_device(g.data_ptr(), p.data_ptr())
+    ctx.backward_device(y.data_ptr(), dg.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.forward_device(g.data_ptr(), p.data_ptr())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.backward_device(y.data_ptr(), dg.data_ptr())
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fw, bw = (t1 - t0) / args.steps, (t2 - t1) / args.steps
+    print(json.dumps({
+        "metric": "s per forward+backward, 128^3 masked Fourier sphere",
+        "value": round(fw + bw, 5), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": False, "dtype": "fp32-kahan forward, fp64 adjoint", "data": "synthetic",
+        "config": {"workload": f"fourier-sphere 128^3, plain-mask, 162 directions, "
+                               f"{args.max_iter} iterations", "forward_s": round(fw, 5),
+                   "backward_s": round(bw, 5)}}))
 
 
 def main():
@@ -46,6 +144,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-dirs", type=int, default=12)
+    ap.add_argument("--naive-passes", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--batch", action="store_true")
+    ap.add_argument("--adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -54,10 +156,17 @@ def main():
         sys.exit(2)
 
     import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.batch:
+        return batch_mode(args)
+    if args.adjoint:
+        return adjoint_mode(args)
+
     import omni3d
     from omni3d import fields as F
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     n = args.n
     shape = (n, n, n)
     V = n ** 3
@@ -66,10 +175,10 @@ def main():
     dirs = omni3d.generate_directions(args.divide)
     D = dirs.shape[0]
 
-    def make_ctx(precision):
+    def make_ctx(precision, **kw):
         ctx = omni3d.Context(shape, f["delta"], density=f["rho"], line_spacing=1.0,
                              precision=precision, max_iterations=args.max_iter,
-                             tolerance=args.tol, seed_pn_with_p=1)
+                             tolerance=args.tol, seed_pn_with_p=1, **kw)
         ctx.set_directions(dirs)
         ctx.set_field(f["acc"])
         return ctx
@@ -104,8 +213,9 @@ def main():
     r = reps[-1]
     log(f"bench: FP32-Kahan {s_per * 1e3:.2f} ms/recon (A {r.ms_pass_a:.2f} it {r.ms_iterate:.2f} "
         f"B {r.ms_pass_b:.2f} fin {r.ms_finalize:.2f}) iters {r.iterations} rel {r.rel_change:.3e}")
+    ctx32.close()
 
-    # FP64 GPU path on the same input
+    # FP64 GPU paths on the same input
     fp64 = {}
     if not args.no_fp64:
         log("bench: FP64 context")
@@ -121,25 +231,49 @@ def main():
         p64 = p_dev.clone()
         rel = float(torch.linalg.norm(p32 - p64) / torch.linalg.norm(p64))
         maxerr = float((p32 - p64).abs().max() / p64.abs().max())
+        ctx64.close()
+        # the reference's schedule in FP64: a full traversal per iteration;
+        # a few passes are timed and the per-pass cost is reported
+        ctxn = omni3d.Context(shape, f["delta"], density=f["rho"], precision=omni3d.FP64,
+                              solver=omni3d.NAIVE, max_iterations=args.naive_passes)
+        ctxn.set_directions(dirs)
+        ctxn.set_field(f["acc"])
+        step(ctxn)
+        rn = step(ctxn)
+        torch.cuda.synchronize()
+        ms_naive = rn.ms_iterate / args.naive_passes
+        ctxn.close()
         fp64 = {"s_per_recon_fp64": round(s64, 6), "rel_l2_vs_fp64": rel,
                 "max_err_rel_vs_fp64": maxerr, "speedup_vs_fp64": round(s64 / s_per, 3),
-                "fp64_iterations": r64.iterations, "ms_pass_b_fp64": round(r64.ms_pass_b, 4)}
-        log(f"bench: FP64 {s64 * 1e3:.2f} ms/recon, FP32 rel-L2 vs FP64 {rel:.3e}")
+                "fp64_iterations": r64.iterations, "ms_pass_b_fp64": round(r64.ms_pass_b, 4),
+                "fp64_reference_schedule_ms_per_pass": round(ms_naive, 3),
+                "fp64_reference_schedule_s_for_same_iterations":
+                    round(ms_naive * 1e-3 * r.iterations, 3),
+                "speedup_vs_fp64_reference_schedule":
+                    round(ms_naive * 1e-3 * r.iterations / s_per, 2)}
+        log(f"bench: FP64 {s64 * 1e3:.2f} ms/recon, FP32 rel-L2 vs FP64 {rel:.3e}; "
+            f"FP64 full-traversal pass {ms_naive:.2f} ms")
         assert rel <= 1e-3, f"FP32-Kahan result misses rel-L2 <= 1e-3 vs FP64 ({rel})"
-        ctx64.close()
 
     pa = np.asarray(f["p"]).reshape(-1)
     p32h = p32.cpu().numpy()
     err_analytic = F.rel_l2(F.demean(p32h), F.demean(pa))
 
-    # roofline of the final traversal: D*V*3*4 B of FP32 gradient + V*20 B
-    # (p read+write in FP64, weight read)
+    # roofline of the final full traversal (k_tile_pass, full mode): it must
+    # stream the gradient once per direction, D*V*3*4 B in FP32, and read and
+    # write the FP64 pressure and FP32 weight once, V*20 B
     ms_b = float(np.mean([x.ms_pass_b for x in reps]))
     bytes_pass = D * V * 3 * 4 + V * 20
     achieved = bytes_pass / (ms_b * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))
+        if t.get("workload_n") == n and t.get("directions") == D:
+            traffic = t.get("traffic_bytes")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_walk_plain<float,true> (final traversal)", "ms": round(ms_b, 4)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_tile_pass<float, full, 16^3 tiles, 1024 threads> (final traversal)",
+            "algorithmic_bytes": bytes_pass, "ms": round(ms_b, 4)}
 
     cpu = None
     if not args.no_cpu:

@@ -114,30 +114,37 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   tg.ntz = (g.Z + kTile - 1) / kTile;
   const int nt = tg.ntx * tg.nty * tg.ntz;
   const int nslots = nt * kLenBuckets;
-  DevBuf nseg, tcnt, tmp;
-  OMNI_TRY_HIP(ctx, nseg.alloc(sizeof(unsigned int) * nrays));
+  const int64_t ngroups = (nrays + 63) / 64;
+  DevBuf tcnt, gcnt, tmp;
+  OMNI_TRY_HIP(ctx, pl.nseg.alloc(sizeof(unsigned int) * nrays));
   OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * nslots));
-  OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(nrays, nslots))));
+  OMNI_TRY_HIP(ctx, gcnt.alloc(sizeof(unsigned int) * ngroups));
+  OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(ngroups, nslots))));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
   OMNI_TRY_HIP(ctx, launch_tile_count(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
-                                      nseg.as<unsigned int>(), tcnt.as<unsigned int>(), st));
-  OMNI_TRY_HIP(ctx, pl.seg_base.alloc(sizeof(unsigned long long) * (nrays + 1)));
-  OMNI_TRY_HIP(ctx, pl.tile_ptr.alloc(sizeof(unsigned long long) * (nslots + 1)));
-  OMNI_TRY_HIP(ctx, launch_scan_u32(nseg.as<unsigned int>(), nrays,
-                                    pl.seg_base.as<unsigned long long>(), 0, tmp.as<void>(),
+                                      pl.nseg.as<unsigned int>(), tcnt.as<unsigned int>(), st));
+  // value slots: groups of 64 rays, segment-ordinal major inside a group
+  OMNI_TRY_HIP(ctx, launch_seg_groups(nrays, pl.nseg.as<unsigned int>(), gcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, pl.val_off.alloc(sizeof(unsigned long long) * (ngroups + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(gcnt.as<unsigned int>(), ngroups,
+                                    pl.val_off.as<unsigned long long>(), 0, tmp.as<void>(),
                                     tmp.size(), st));
+  OMNI_TRY_HIP(ctx, pl.tile_ptr.alloc(sizeof(unsigned long long) * (nslots + 1)));
   OMNI_TRY_HIP(ctx, launch_scan_u32(tcnt.as<unsigned int>(), nslots,
                                     pl.tile_ptr.as<unsigned long long>(), 0, tmp.as<void>(),
                                     tmp.size(), st));
-  unsigned long long nsegs = 0;
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsegs, pl.seg_base.as<unsigned long long>() + nrays, 8,
+  unsigned long long nsegs = 0, nval = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsegs, pl.tile_ptr.as<unsigned long long>() + nslots, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nval, pl.val_off.as<unsigned long long>() + ngroups, 8,
                                    hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  if (nsegs >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
+  if (nsegs >= (1ull << 32) || nval >= (1ull << 32))
+    return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
   OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
   OMNI_TRY_HIP(ctx, launch_tile_fill(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
-                                     pl.sid.as<int>(), pl.seg_base.as<unsigned long long>(),
+                                     pl.sid.as<int>(), pl.val_off.as<unsigned long long>(),
                                      pl.tile_ptr.as<unsigned long long>(), tcnt.as<unsigned int>(),
                                      pl.segs.as<SegRec>(), st));
   // tiles that touch the box surface hold every source voxel of the plain method
@@ -155,6 +162,7 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   pl.ntiles = nt;
   pl.nsurf = (int)surf.size();
   pl.nsegs = (int64_t)nsegs;
+  pl.nval = (int64_t)nval;
   return OMNI3D_OK;
 }
 
@@ -285,11 +293,14 @@ int build_plan(omni3d_ctx* ctx) {
     OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nm, pl.moff.as<unsigned long long>() + nch, 8,
                                      hipMemcpyDeviceToHost, st));
     OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (nnz >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many source visits");
     OMNI_TRY_HIP(ctx, pl.masks.alloc(sizeof(unsigned long long) * (nm + 1)));
+    OMNI_TRY_HIP(ctx, pl.cell_eb.alloc(sizeof(unsigned int) * (nm + 1)));
     OMNI_TRY_HIP(ctx, pl.ent.alloc(sizeof(int) * (nnz + 1)));
     OMNI_TRY_HIP(ctx, launch_sell_fill(nS, D, pl.rowptr.as<unsigned long long>(),
                                        key.as<unsigned long long>(), pl.moff.as<unsigned long long>(),
-                                       pl.masks.as<unsigned long long>(), pl.ent.as<int>(), st));
+                                       pl.masks.as<unsigned long long>(),
+                                       pl.cell_eb.as<unsigned int>(), pl.ent.as<int>(), st));
     pl.ncell = (int64_t)nm;
   }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
@@ -310,6 +321,14 @@ int build_plan(omni3d_ctx* ctx) {
   pl.valid = true;
   ctx->dirs_dirty = false;
   return OMNI3D_OK;
+}
+
+void drop_iteration_graph(omni3d_ctx* ctx) {
+  if (ctx->it_exec) (void)hipGraphExecDestroy(ctx->it_exec);
+  if (ctx->it_graph) (void)hipGraphDestroy(ctx->it_graph);
+  ctx->it_exec = nullptr;
+  ctx->it_graph = nullptr;
+  ctx->it_key.clear();
 }
 
 // Timing events of the phases (not recorded while a graph is being captured).
@@ -354,7 +373,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   const size_t ns = (size_t)std::max(pl.nS, 1);
   OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->bs.as<void>(), 2 * (int64_t)ns, 0u, st));
-  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nsegs, 1)));
+  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
+  OMNI_TRY_HIP(ctx, ctx->eterm.alloc(sizeof(T) * std::max<int64_t>(pl.nsegs, 1)));
   record_event(ctx, 2);
   if (ctx->engine == 0) {
     // segment sums over all tiles, per-ray carries, then the source sums on
@@ -363,14 +383,17 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(), nullptr, st));
-    OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.seg_base.as<unsigned long long>(),
+                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          nullptr, st));
+    OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.nseg.as<unsigned int>(),
+                                        pl.val_off.as<unsigned long long>(),
                                         ctx->segval.as<double>(), st));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, ctx->tile_threads, pl.nsurf,
                                           pl.surf.as<int>(), pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(), ctx->bs.as<long long>(), st));
+                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          ctx->bs.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
@@ -383,14 +406,53 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * iterate_grid(pl.nS)));
   OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
-  for (int k = 1; k <= P.max_iterations; k++) {
-    OMNI_TRY_HIP(ctx, launch_iterate(k, pl.nS, pl.moff.as<unsigned long long>(),
-                                     pl.masks.as<unsigned long long>(),
-                                     pl.rowptr.as<unsigned long long>(), pl.ent.as<int>(),
-                                     pl.diag.as<unsigned int>(), pl.slist.as<int>(),
-                                     pl.cnt.as<unsigned int>(), ctx->bs.as<long long>(), pinit,
-                                     seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
-                                     ctx->partials.as<double>(), P.max_iterations, P.tolerance, st));
+  auto iterations = [&](hipStream_t s) -> hipError_t {
+    for (int k = 1; k <= P.max_iterations; k++) {
+      const hipError_t e = launch_iterate(ctx->iter_waves, k, pl.nS, pl.moff.as<unsigned long long>(),
+                                          pl.masks.as<unsigned long long>(),
+                                          pl.cell_eb.as<unsigned int>(), pl.ent.as<int>(),
+                                          pl.diag.as<unsigned int>(), pl.slist.as<int>(),
+                                          pl.cnt.as<unsigned int>(), ctx->bs.as<long long>(), pinit,
+                                          seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+                                          ctx->partials.as<double>(), P.max_iterations,
+                                          P.tolerance, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
+  if (ctx->capturing) {
+    OMNI_TRY_HIP(ctx, iterations(st));
+  } else {
+    // the iteration launches are replayed from a graph captured once per set
+    // of arguments (no launch gaps between the short iterations)
+    double tol = P.tolerance;
+    uint64_t tolbits;
+    std::memcpy(&tolbits, &tol, 8);
+    const std::vector<uint64_t> key = {
+        (uint64_t)P.max_iterations, (uint64_t)pl.nS, (uint64_t)(uintptr_t)pbuf[0],
+        (uint64_t)(uintptr_t)pbuf[1], (uint64_t)(uintptr_t)sc,
+        (uint64_t)(uintptr_t)ctx->partials.as<void>(), (uint64_t)(uintptr_t)ctx->bs.as<void>(),
+        (uint64_t)(uintptr_t)pinit, (uint64_t)seed, tolbits,
+        (uint64_t)(uintptr_t)pl.moff.as<void>(), (uint64_t)(uintptr_t)pl.masks.as<void>(),
+        (uint64_t)(uintptr_t)pl.cell_eb.as<void>(), (uint64_t)(uintptr_t)pl.ent.as<void>(),
+        (uint64_t)(uintptr_t)pl.diag.as<void>(), (uint64_t)(uintptr_t)pl.slist.as<void>(),
+        (uint64_t)(uintptr_t)pl.cnt.as<void>(), (uint64_t)ctx->iter_waves};
+    if (!ctx->it_exec || key != ctx->it_key) {
+      drop_iteration_graph(ctx);
+      if (!ctx->cap_stream) OMNI_TRY_HIP(ctx, hipStreamCreate(&ctx->cap_stream));
+      OMNI_TRY_HIP(ctx, hipStreamBeginCapture(ctx->cap_stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t e1 = iterations(ctx->cap_stream);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(ctx->cap_stream, &g);
+      if (e1 != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        OMNI_TRY_HIP(ctx, e1 != hipSuccess ? e1 : e2);
+      }
+      ctx->it_graph = g;
+      OMNI_TRY_HIP(ctx, hipGraphInstantiate(&ctx->it_exec, g, nullptr, nullptr, 0));
+      ctx->it_key = key;
+    }
+    OMNI_TRY_HIP(ctx, hipGraphLaunch(ctx->it_exec, st));
   }
   record_event(ctx, 4);
   // pass B: every voxel from p_S^{k*-1}
@@ -404,7 +466,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
-                                          ctx->segval.as<double>(), ctx->sum.as<long long>(), st));
+                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          ctx->sum.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V * nc, 0u, st));
     record_event(ctx, 5);
@@ -587,6 +650,10 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
     if (n >= 1 && n <= 64) ctx->acc_copies = n;
   }
   if (const char* e = std::getenv("OMNI3D_ENGINE")) ctx->engine = std::strcmp(e, "rays") == 0 ? 1 : 0;
+  if (const char* e = std::getenv("OMNI3D_ITER_WAVES")) {
+    const int w = std::atoi(e);
+    if (w == 1 || w == 2 || w == 4) ctx->iter_waves = w;
+  }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;
@@ -604,6 +671,8 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
 void omni3d_destroy(omni3d_ctx* ctx) {
   if (!ctx) return;
   (void)hipStreamSynchronize(ctx->stream);
+  drop_iteration_graph(ctx);
+  if (ctx->cap_stream) (void)hipStreamDestroy(ctx->cap_stream);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   delete ctx;

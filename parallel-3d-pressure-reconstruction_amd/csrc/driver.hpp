@@ -29,6 +29,8 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
 // A whole reconstruction queued on the context's stream without host
 // synchronisation (capturable once the buffers are sized); pdst / wdst device.
 int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
+// Releases the context's cached graph of the iteration launches.
+void drop_iteration_graph(omni3d_ctx* ctx);
 // Phase timing event i of the context (skipped during graph capture).
 void record_event(omni3d_ctx* ctx, int i);
 

@@ -74,6 +74,7 @@ struct Plan {
   DevBuf diag;     // uint32[nS] rays entering at each source (their self visit)
   DevBuf moff;     // uint64[nchunks+1] first slot of each 64-row chunk
   DevBuf masks;    // uint64[ncell] lane mask of each (direction, k) slot
+  DevBuf cell_eb;  // uint32[ncell] first packed entry of each slot
   DevBuf ent;      // int32[nnz] source ids packed in slot order
   int64_t ncell = 0;
   // tile engine: compiled paths and per-tile segment lists
@@ -81,7 +82,9 @@ struct Plan {
   int ntiles = 0;
   int nsurf = 0;
   int64_t nsegs = 0;
-  DevBuf seg_base;  // uint64[nrays+1]
+  int64_t nval = 0;  // value slots (segments padded per group of 64 rays)
+  DevBuf nseg;       // uint32[nrays] segments of each ray
+  DevBuf val_off;    // uint64[ngroups+1] first value slot of each group of 64 rays
   DevBuf tile_ptr;  // uint64[ntiles+1]
   DevBuf segs;      // SegRec[nsegs], grouped by tile
   DevBuf surf;      // int32[nsurf] tiles that hold source voxels
@@ -99,7 +102,13 @@ struct omni3d_ctx {
   std::shared_ptr<omni3d::Plan> plan = std::make_shared<omni3d::Plan>();
   int tile_bits = omni3d::kTileBitsDefault;  // tile edge 2^bits of the tile engine
   int tile_threads = 1024;                   // workgroup size of the tile passes
+  int iter_waves = 2;                        // wavefronts per 64-row chunk of the iteration
   bool capturing = false;                    // work is being captured into a graph
+  // graph of the source-iteration launches, reused while its arguments stay
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t it_graph = nullptr;
+  hipGraphExec_t it_exec = nullptr;
+  std::vector<uint64_t> it_key;
   // field in the compute precision, one 4-vector (a_x, a_y, a_z, 0) per voxel
   omni3d::DevBuf field, mask;
   bool have_field = false;
@@ -111,7 +120,8 @@ struct omni3d_ctx {
   omni3d::DevBuf ps0, ps1;  // double[nS]
   omni3d::DevBuf partials;  // double[2*grid]
   omni3d::DevBuf sum;       // int64[ncopies*V]
-  omni3d::DevBuf segval;    // double[nsegs] segment sums, then carries
+  omni3d::DevBuf segval;    // double[nval] segment sums, then carries
+  omni3d::DevBuf eterm;     // T[nsegs] trapezoid of the step into each segment's tile
   int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
   int used_copies = 1;

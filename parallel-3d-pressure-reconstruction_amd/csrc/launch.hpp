@@ -97,8 +97,9 @@ hipError_t launch_compute_scale(DevScalars* sc, double gfac, double lmax, double
 hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, double* ps,
                                  hipStream_t st);
 unsigned int iterate_grid(int nS);
-hipError_t launch_iterate(int k, int nS, const unsigned long long* moff,
-                          const unsigned long long* masks, const unsigned long long* rowptr,
+// waves: wavefronts per 64-row chunk (1, 2 or 4)
+hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* moff,
+                          const unsigned long long* masks, const unsigned int* cell_eb,
                           const int* ent, const unsigned int* diag, const int* slist,
                           const unsigned int* cnt, const long long* bs, const double* pinit,
                           int seed, const double* pold, double* pnew, DevScalars* sc,

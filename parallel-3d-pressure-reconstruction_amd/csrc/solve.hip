@@ -3,6 +3,7 @@
 // kernel.cu:8840-8861).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "kernels.hpp"
@@ -66,41 +67,42 @@ k_gather_sources(const int* __restrict__ slist, int nS, const double* __restrict
 // One iteration on the source voxels S (affine form of kernel.cu:8840-8861):
 //   p_S^k[v] = (bs[v] + diag[v] p_S^{k-1}[v] + sum_{visits of v} p_S^{k-1}[src]
 //               + seed) / C[v]
-// where C[v] > 0, else the old value.  One lane per row; the rows are sorted
-// at plan time, so the FP64 sum has a fixed order.  The last block
-// reduces the relative change and sets the stop flag; once it is set the
-// remaining launches return at once.
-constexpr int kIterBlock = 256;
-
-__global__ void __launch_bounds__(kIterBlock)
+// where C[v] > 0, else the old value.  One workgroup per 64-row chunk of the
+// masked sliced-ELL layout: its kWaves wavefronts take consecutive shares of
+// the chunk's slots (one lane per row, sixteen gathers in flight), and the
+// partial row sums are added in wavefront order, so the FP64 sum has a fixed
+// order.  Each workgroup leaves its share of the relative-change sums in
+// partials; k_iter_finish reduces them and sets the stop flag, after which
+// the remaining launches return at once.
+template <int kWaves>
+__global__ void __launch_bounds__(64 * kWaves)
 k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
-          const unsigned long long* __restrict__ masks, const unsigned long long* __restrict__ rowptr,
+          const unsigned long long* __restrict__ masks, const unsigned int* __restrict__ cell_eb,
           const int* __restrict__ ent, const unsigned int* __restrict__ diag,
           const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
           const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
-          const double* __restrict__ pold, double* __restrict__ pnew, DevScalars* __restrict__ sc,
-          double* __restrict__ partials, int max_iter, double tol) {
+          const double* __restrict__ pold, double* __restrict__ pnew,
+          const DevScalars* __restrict__ sc, double* __restrict__ partials) {
   if (sc->converged) return;
-  constexpr int kWaves = kIterBlock / 64;
-  __shared__ double s_d2[kWaves], s_n2[kWaves];
-  __shared__ int s_last;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double s_acc[kWaves][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const double inv = sc->inv_scale;
   const unsigned long long below = (1ull << lane) - 1ull;
   double d2 = 0.0, n2 = 0.0;
-  // One lane per row, one wavefront per 64-row chunk of the masked
-  // sliced-ELL layout; sixteen slots per round keep sixteen gathers in flight.
-  const int nrows = (nS + 63) & ~63;
-  for (int row = blockIdx.x * kIterBlock + threadIdx.x; row < nrows; row += gridDim.x * kIterBlock) {
-    // the chunk is uniform across the wavefront: its slot masks are scalar loads
-    const int chunk = __builtin_amdgcn_readfirstlane(row >> 6);
+  const int nchunks = (nS + 63) >> 6;
+  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const unsigned long long m0 = moff[chunk], m1 = moff[chunk + 1];
-    unsigned long long eb = rowptr[chunk << 6];
+    const unsigned long long q = (m1 - m0 + kWaves - 1) / kWaves;
+    const unsigned long long ms = m0 + (unsigned long long)wave * q;
+    const unsigned long long me = ms + q < m1 ? ms + q : m1;
+    unsigned long long eb = ms < me ? (unsigned long long)cell_eb[ms] : 0ull;
     double acc = 0.0;
-    unsigned long long s = m0;
+    unsigned long long s = ms;
     constexpr int kU = 16;
-    for (; s + kU <= m1; s += kU) {
-      // masks first, then every entry load, then every gather: three
+    for (; s + kU <= me; s += kU) {
+      // masks first (scalar loads: the slot is uniform across the
+      // wavefront), then every entry load, then every gather: three
       // dependent rounds per kU slots.  A lane without an entry in a slot
       // reads the slot's first entry and adds nothing.
       unsigned long long m[kU];
@@ -123,83 +125,85 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
 #pragma unroll
       for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
     }
-    for (; s < m1; s++) {
-      const unsigned long long m = masks[s];
-      if ((m >> lane) & 1ull) acc += pold[ent[eb + __popcll(m & below)]];
-      eb += __popcll(m);
+    for (; s < me; s++) {
+      const unsigned long long mm = masks[s];
+      if ((mm >> lane) & 1ull) acc += pold[ent[eb + __popcll(mm & below)]];
+      eb += __popcll(mm);
     }
-    if (row >= nS) continue;
-    const int v = slist[row];
-    const unsigned int c = cnt[v];
-    const double po = pold[row];
-    double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
-    if (k == 1 && seed && pinit) tot += pinit[v];
-    const double pn = c > 0 ? tot / (double)c : po;
-    pnew[row] = pn;
-    const double df = pn - po;
-    d2 += df * df;
-    n2 += pn * pn;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    d2 += __shfl_xor(d2, off);
-    n2 += __shfl_xor(n2, off);
-  }
-  if (lane == 0) {
-    s_d2[wave] = d2;
-    s_n2[wave] = n2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, b = 0.0;
-    for (int w = 0; w < kWaves; w++) {
-      a += s_d2[w];
-      b += s_n2[w];
+    if (kWaves > 1) {
+      s_acc[wave][lane] = acc;
+      __syncthreads();
     }
-    partials[2 * blockIdx.x] = a;
-    partials[2 * blockIdx.x + 1] = b;
-    __threadfence();
-    const unsigned int t =
-        __hip_atomic_fetch_add(&sc->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == gridDim.x - 1);
+    const int row = chunk * 64 + lane;
+    if (wave == 0 && row < nS) {
+      double tacc = acc;
+#pragma unroll
+      for (int w = 1; w < kWaves; w++) tacc += s_acc[w][lane];
+      const int v = slist[row];
+      const unsigned int c = cnt[v];
+      const double po = pold[row];
+      double tot = (double)bs[row] * inv + (double)diag[row] * po + tacc;
+      if (k == 1 && seed && pinit) tot += pinit[v];
+      const double pn = c > 0 ? tot / (double)c : po;
+      pnew[row] = pn;
+      const double df = pn - po;
+      d2 += df * df;
+      n2 += pn * pn;
+    }
+    if (kWaves > 1) __syncthreads();
   }
-  __syncthreads();
-  if (!s_last) return;
-  // Last block: every thread sums a fixed strided subset of the partials,
-  // then a fixed-order tree over the waves (deterministic).
-  __threadfence();
+  if (wave == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      d2 += __shfl_xor(d2, off);
+      n2 += __shfl_xor(n2, off);
+    }
+    if (lane == 0) {
+      partials[2 * blockIdx.x] = d2;
+      partials[2 * blockIdx.x + 1] = n2;
+    }
+  }
+}
+
+// Relative change of iteration k from the workgroups' partial sums (fixed
+// order: a strided share per thread, then a tree), and the stop rule.
+constexpr int kFinishBlock = 1024;
+
+__global__ void __launch_bounds__(kFinishBlock)
+k_iter_finish(int k, int nparts, const double* __restrict__ partials, DevScalars* __restrict__ sc,
+              int max_iter, double tol) {
+  if (sc->converged) return;
+  __shared__ double s_a[kFinishBlock / 64], s_b[kFinishBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double a = 0.0, b = 0.0;
-  for (unsigned int i = threadIdx.x; i < gridDim.x; i += kIterBlock) {
-    a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    b += __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int i = threadIdx.x; i < nparts; i += kFinishBlock) {
+    a += partials[2 * i];
+    b += partials[2 * i + 1];
   }
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
   }
-  __syncthreads();
   if (lane == 0) {
-    s_d2[wave] = a;
-    s_n2[wave] = b;
+    s_a[wave] = a;
+    s_b[wave] = b;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    a = 0.0;
-    b = 0.0;
-    for (int w = 0; w < kWaves; w++) {
-      a += s_d2[w];
-      b += s_n2[w];
-    }
-    const double num = sqrt(a), den = sqrt(b);
-    const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
-    sc->rel = rel;
-    sc->dnum = num;
-    sc->dden = den;
-    sc->iter_done = k;
-    if ((tol > 0.0 && rel < tol) || k >= max_iter) {
-      sc->converged = 1;
-      sc->kstar = k;
-    }
-    sc->ticket = 0;
+  if (threadIdx.x != 0) return;
+  a = 0.0;
+  b = 0.0;
+  for (int w = 0; w < kFinishBlock / 64; w++) {
+    a += s_a[w];
+    b += s_b[w];
+  }
+  const double num = sqrt(a), den = sqrt(b);
+  const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
+  sc->rel = rel;
+  sc->dnum = num;
+  sc->dden = den;
+  sc->iter_done = k;
+  if ((tol > 0.0 && rel < tol) || k >= max_iter) {
+    sc->converged = 1;
+    sc->kstar = k;
   }
 }
 
@@ -438,16 +442,26 @@ hipError_t launch_gather_sources(const int* slist, int nS, const double* pfull, 
   return hipGetLastError();
 }
 
-unsigned int iterate_grid(int nS) { return grid_for((int64_t)nS + 63, 4096); }
+unsigned int iterate_grid(int nS) {
+  const int64_t nchunks = ((int64_t)nS + 63) / 64;
+  return (unsigned int)std::max<int64_t>(1, std::min<int64_t>(nchunks, 65536));
+}
 
-hipError_t launch_iterate(int k, int nS, const unsigned long long* moff,
-                          const unsigned long long* masks, const unsigned long long* rowptr,
+hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* moff,
+                          const unsigned long long* masks, const unsigned int* cell_eb,
                           const int* ent, const unsigned int* diag, const int* slist,
                           const unsigned int* cnt, const long long* bs, const double* pinit,
                           int seed, const double* pold, double* pnew, DevScalars* sc,
                           double* partials, int max_iter, double tol, hipStream_t st) {
-  hipLaunchKernelGGL(k_iterate, dim3(iterate_grid(nS)), dim3(kIterBlock), 0, st, k, nS, moff, masks,
-                     rowptr, ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials,
+  const unsigned int grid = iterate_grid(nS);
+#define OMNI_ITER(W)                                                                           \
+  hipLaunchKernelGGL(k_iterate<W>, dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks, cell_eb, \
+                     ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
+  if (waves <= 1) OMNI_ITER(1);
+  else if (waves == 2) OMNI_ITER(2);
+  else OMNI_ITER(4);
+#undef OMNI_ITER
+  hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol);
   return hipGetLastError();
 }

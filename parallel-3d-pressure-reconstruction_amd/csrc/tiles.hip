@@ -67,7 +67,7 @@ k_tile_count(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
 __global__ void __launch_bounds__(256)
 k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
             GridDesc g, TileGrid tg, const int* __restrict__ sid,
-            const unsigned long long* __restrict__ seg_base,
+            const unsigned long long* __restrict__ val_off,
             const unsigned long long* __restrict__ tile_ptr, unsigned int* __restrict__ cursor,
             SegRec* __restrict__ segs) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,7 +81,11 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   const uint32_t signs = (co[0] > ci[0] ? 1u : 0u) | (co[1] > ci[1] ? 2u : 0u) |
                          (co[2] > ci[2] ? 4u : 0u);
   const uint32_t src = (uint32_t)sid[w.v];
-  unsigned long long gseg = seg_base[r];
+  // value slot of the k-th segment: blocked transposed layout (k * 64 + lane
+  // within the ray's group of 64), so that a wavefront's rays touch
+  // consecutive slots in the carry scan and, within a tile, neighbouring
+  // parallel rays at the same ordinal read neighbouring carries
+  unsigned long long gseg = val_off[r >> 6] + (unsigned long long)(r & 63);
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
   SegRec sr;
   uint32_t nvis = 1, ncode = 0, first = 1;
@@ -98,7 +102,7 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     const int ls = len_slot(tile, nvis);
     const unsigned int slot = atomicAdd(&cursor[ls], 1u);
     segs[tile_ptr[ls] + slot] = sr;
-    gseg++;
+    gseg += 64;
   };
   auto put_code = [&](int a) {
     const uint32_t sh = 2 * (ncode & 15);
@@ -144,14 +148,16 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
             const DevScalars* __restrict__ sc, double* __restrict__ segval,
-            long long* __restrict__ out) {
+            T* __restrict__ eterm, long long* __restrict__ out) {
   using V4 = typename Vec4<T>::type;
   constexpr int kTileBits = kBits;
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
   constexpr int kTileVox = kTile * kTile * kTile;
   __shared__ TileLds<T, kTileVox> L;
-  const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
+  // contiguous tile ranges per XCD: neighbouring tiles share an L2
+  const int tb = (int)xcd_block();
+  const int tile = tile_list ? tile_list[tb] : tb;
   const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
   const int ox = tx << kTileBits, oy = ty << kTileBits, oz = tz << kTileBits;
   const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
@@ -174,6 +180,9 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
                            e = tile_ptr[(tile + 1) * kLenBuckets];
   for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x) {
     const SegRec sr = segs[j];
+    // carry: the ray's sum before this segment plus the step into the tile
+    // (kLocalSum computes that step, the only read outside the tile, and
+    // stores it per segment in tile order)
     int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
     const unsigned int n = (sr.info >> kSegLvBits) & 63u;
     const unsigned int sg = (sr.info >> 18) & 7u;
@@ -205,7 +214,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     if (first) {
       // the entry visit: self-add p_src (pint = 0)
       if (kMode == kFull) add_lds(lv, pin);
-    } else {
+    } else if (kMode == kLocalSum) {
       // step into the tile: the previous voxel lies outside it
       const int a = next_code();
       const int sgn = sign_of(a);
@@ -213,9 +222,14 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
                 k = oz + (lv >> (2 * kTileBits));
       const int64_t vp = (int64_t)i + jj * sy + k * sz - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
       const T prev = component(field[vp], a);
-      loc.add(trapezoid<T>(wp, a, sgn, L.g[a][lv], prev));
-      if (kMode == kFull) add_lds(lv, pin + (carry + loc.value()));
-      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry + loc.value());
+      const T et = trapezoid<T>(wp, a, sgn, L.g[a][lv], prev);
+      eterm[j] = et;
+      loc.add(et);
+    } else {
+      (void)next_code();
+      carry += (double)eterm[j];
+      if (kMode == kFull) add_lds(lv, pin + carry);
+      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry);
     }
     for (unsigned int vis = 1; vis < n; vis++) {
       const int a = next_code();
@@ -290,7 +304,8 @@ k_sell_layout(int nS, int D, const unsigned long long* __restrict__ rowptr,
 __global__ void __launch_bounds__(64)
 k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
             const unsigned long long* __restrict__ key, const unsigned long long* __restrict__ moff,
-            unsigned long long* __restrict__ masks, int* __restrict__ ent) {
+            unsigned long long* __restrict__ masks, unsigned int* __restrict__ cell_eb,
+            int* __restrict__ ent) {
   const int chunk = blockIdx.x, lane = threadIdx.x, row = chunk * 64 + lane;
   const unsigned long long rb = row < nS ? rowptr[row] : 0ull;
   const unsigned long long re = row < nS ? rowptr[row + 1] : 0ull;
@@ -309,7 +324,10 @@ k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
     for (unsigned int k = 0; k < kmax; k++) {
       const bool has = c > k;
       const unsigned long long m = __ballot(has);
-      if (lane == 0) masks[mp] = m;
+      if (lane == 0) {
+        masks[mp] = m;
+        cell_eb[mp] = (unsigned int)eb;
+      }
       mp++;
       if (has) {
         const unsigned int rank = __popcll(m & ((1ull << lane) - 1ull));
@@ -321,29 +339,47 @@ k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
   }
 }
 
-// Per-ray exclusive scan of the segment sums into carries, in place.
+// Value slots of the segments: rays in groups of 64 (one wavefront of the
+// carry scan); group q holds 64 * (its largest segment count) slots, the k-th
+// segment of its ray l at val_off[q] + 64 k + l.
+__global__ void __launch_bounds__(64)
+k_seg_groups(int64_t nrays, const unsigned int* __restrict__ nseg,
+             unsigned int* __restrict__ gcnt) {
+  const int64_t r = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  unsigned int m = r < nrays ? nseg[r] : 0u;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned int o = __shfl_xor(m, off);
+    m = o > m ? o : m;
+  }
+  if (threadIdx.x == 0) gcnt[blockIdx.x] = 64u * m;
+}
+
+// Per-ray exclusive scan of the segment sums into carries, in place; the
+// lanes of a wavefront are the 64 rays of one group, so every step reads 64
+// consecutive slots.
 __global__ void __launch_bounds__(256)
-k_carry_scan(int64_t nrays, const unsigned long long* __restrict__ seg_base,
-             double* __restrict__ segval) {
+k_carry_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
+             const unsigned long long* __restrict__ val_off, double* __restrict__ segval) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
-  const unsigned long long b = seg_base[r], e = seg_base[r + 1];
+  const unsigned long long b = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int n = nseg[r];
   double run = 0.0;
-  unsigned long long j = b;
+  unsigned int k = 0;
   // eight loads in flight per lane; the sum order is the segment order
-  for (; j + 8 <= e; j += 8) {
+  for (; k + 8 <= n; k += 8) {
     double x[8];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = segval[j + u];
+    for (int u = 0; u < 8; u++) x[u] = segval[b + 64ull * (k + u)];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      segval[j + u] = run;
+      segval[b + 64ull * (k + u)] = run;
       run = run + x[u];
     }
   }
-  for (; j < e; j++) {
-    const double x = segval[j];
-    segval[j] = run;
+  for (; k < n; k++) {
+    const double x = segval[b + 64ull * k];
+    segval[b + 64ull * k] = run;
     run = run + x;
   }
 }
@@ -439,11 +475,18 @@ hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* 
 }
 
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                            TileGrid tg, const int* sid, const unsigned long long* seg_base,
+                            TileGrid tg, const int* sid, const unsigned long long* val_off,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
                             SegRec* segs, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_fill, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, tg, sid, seg_base, tile_ptr, cursor, segs);
+                     dirs, g, tg, sid, val_off, tile_ptr, cursor, segs);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_groups(int64_t nrays, const unsigned int* nseg, unsigned int* gcnt,
+                             hipStream_t st) {
+  const unsigned int ng = (unsigned int)((nrays + 63) / 64);
+  hipLaunchKernelGGL(k_seg_groups, dim3(ng), dim3(64), 0, st, nrays, nseg, gcnt);
   return hipGetLastError();
 }
 
@@ -457,10 +500,11 @@ hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
 
 hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
                             const unsigned long long* key, const unsigned long long* moff,
-                            unsigned long long* masks, int* ent, hipStream_t st) {
+                            unsigned long long* masks, unsigned int* cell_eb, int* ent,
+                            hipStream_t st) {
   const unsigned int nch = (unsigned int)((nS + 63) / 64);
   hipLaunchKernelGGL(k_sell_fill, dim3(nch), dim3(64), 0, st, nS, D, rowptr, key, moff, masks,
-                     ent);
+                     cell_eb, ent);
   return hipGetLastError();
 }
 
@@ -469,11 +513,11 @@ static void tile_pass_threads(int threads, int ntiles, const int* tile_list,
                               const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                               TileGrid tg, const WalkParams& wp,
                               const typename Vec4<T>::type* f, const int* sid, const double* p0,
-                              const double* p1, const DevScalars* sc, double* segval,
+                              const double* p1, const DevScalars* sc, double* segval, T* eterm,
                               long long* out, hipStream_t st) {
 #define OMNI_TILE(NT)                                                                        \
   hipLaunchKernelGGL((k_tile_pass<T, kMode, kBits, NT>), dim3(ntiles), dim3(NT), 0, st,       \
-                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out)
+                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, eterm, out)
   if (threads <= 256) OMNI_TILE(256);
   else if (threads <= 512) OMNI_TILE(512);
   else OMNI_TILE(1024);
@@ -485,13 +529,14 @@ static void tile_pass_bits(int threads, int ntiles, const int* tile_list,
                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                            TileGrid tg, const WalkParams& wp, const typename Vec4<T>::type* f,
                            const int* sid, const double* p0, const double* p1,
-                           const DevScalars* sc, double* segval, long long* out, hipStream_t st) {
+                           const DevScalars* sc, double* segval, T* eterm, long long* out,
+                           hipStream_t st) {
   if (tg.bits == 3)
     tile_pass_threads<T, kMode, 3>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   p0, p1, sc, segval, eterm, out, st);
   else
     tile_pass_threads<T, kMode, 4>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   p0, p1, sc, segval, eterm, out, st);
 }
 
 template <typename T>
@@ -499,32 +544,34 @@ hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_l
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, long long* out, hipStream_t st) {
+                            double* segval, void* eterm, long long* out, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
+  T* et = static_cast<T*>(eterm);
   if (ntiles <= 0) return hipSuccess;
   if (tg.bits < kTileBitsMin || tg.bits > kTileBitsMax) return hipErrorInvalidValue;
   if (mode == kLocalSum)
     tile_pass_bits<T, kLocalSum>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                 p0, p1, sc, segval, out, st);
+                                 p0, p1, sc, segval, et, out, st);
   else if (mode == kSourceSums)
     tile_pass_bits<T, kSourceSums>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   p0, p1, sc, segval, et, out, st);
   else
     tile_pass_bits<T, kFull>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0,
-                             p1, sc, segval, out, st);
+                             p1, sc, segval, et, out, st);
   return hipGetLastError();
 }
 
 template hipError_t launch_tile_pass<float>(int, int, int, const int*, const unsigned long long*,
                                             const SegRec*, GridDesc, TileGrid, const WalkParams&,
                                             const void*, const int*, const double*, const double*,
-                                            const DevScalars*, double*, long long*, hipStream_t);
+                                            const DevScalars*, double*, void*, long long*,
+                                            hipStream_t);
 template hipError_t launch_tile_pass<double>(int, int, int, const int*, const unsigned long long*,
                                              const SegRec*, GridDesc, TileGrid,
                                              const WalkParams&, const void*, const int*,
                                              const double*, const double*, const DevScalars*,
-                                             double*, long long*, hipStream_t);
+                                             double*, void*, long long*, hipStream_t);
 
 size_t scan_u32_tmp_bytes(int64_t n) {
   return sizeof(unsigned long long) * (size_t)((n + kScanBlock - 1) / kScanBlock + 1);
@@ -541,10 +588,10 @@ hipError_t launch_scan_u32(const unsigned int* cnt, int64_t n, unsigned long lon
   return hipGetLastError();
 }
 
-hipError_t launch_carry_scan(int64_t nrays, const unsigned long long* seg_base, double* segval,
-                             hipStream_t st) {
+hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, double* segval, hipStream_t st) {
   hipLaunchKernelGGL(k_carry_scan, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, nrays,
-                     seg_base, segval);
+                     nseg, val_off, segval);
   return hipGetLastError();
 }
 

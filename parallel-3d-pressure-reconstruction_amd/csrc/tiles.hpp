@@ -28,7 +28,7 @@ struct TileGrid {
 // a tile pass starts it with one coalesced load.  A monotone staircase visits
 // at most 16+15+15 = 46 voxels of a 16^3 tile, so its step codes fit in 96 bits.
 struct SegRec {
-  uint32_t g;        // segment id in ray-major order (index of its carry)
+  uint32_t g;        // value slot of the segment (its sum, then its carry)
   uint32_t src;      // source id of the ray (index of p_src)
   uint32_t info;     // local voxel of the first visit (12 bits) | visits (6) << 12 |
                      // step signs x,y,z (3) << 18 | first segment of the ray (1) << 21
@@ -41,25 +41,29 @@ hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* 
                              TileGrid tg, unsigned int* nseg, unsigned int* tile_cnt,
                              hipStream_t st);
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
-                            TileGrid tg, const int* sid, const unsigned long long* seg_base,
+                            TileGrid tg, const int* sid, const unsigned long long* val_off,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
                             SegRec* segs, hipStream_t st);
+// slots per group of 64 rays: 64 * the group's largest segment count
+hipError_t launch_seg_groups(int64_t nrays, const unsigned int* nseg, unsigned int* gcnt,
+                             hipStream_t st);
 // threads: workgroup size of the tile passes, 256, 512 or 1024
 template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, long long* out, hipStream_t st);
+                            double* segval, void* eterm, long long* out, hipStream_t st);
 // masked sliced-ELL copy of the source CSR (64-row chunks, per-direction
 // slots with lane masks, entries packed without padding); D <= 4096
 hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
                               const unsigned long long* key, unsigned int* nslot, hipStream_t st);
 hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
                             const unsigned long long* key, const unsigned long long* moff,
-                            unsigned long long* masks, int* ent, hipStream_t st);
-hipError_t launch_carry_scan(int64_t nrays, const unsigned long long* seg_base, double* segval,
-                             hipStream_t st);
+                            unsigned long long* masks, unsigned int* cell_eb, int* ent,
+                            hipStream_t st);
+hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, double* segval, hipStream_t st);
 // exclusive scan of n uint32 counts into n+1 uint64 offsets (multi-block);
 // `words` != 0 scans ceil(count/16) instead of count
 hipError_t launch_scan_u32(const unsigned int* cnt, int64_t n, unsigned long long* out,
