@@ -345,6 +345,8 @@ WalkParams walk_params(const omni3d_params& P) {
     // (kernel.cu:1816) for s = -1 / +1
     wp.hs[a][0] = -P.density * -1.0 * del[a] * 0.5;
     wp.hs[a][1] = -P.density * 1.0 * del[a] * 0.5;
+    wp.hsf[a][0] = (float)wp.hs[a][0];
+    wp.hsf[a][1] = (float)wp.hs[a][1];
   }
   wp.thr = P.threshold;
   wp.method = P.method;

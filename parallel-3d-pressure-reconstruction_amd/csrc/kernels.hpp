@@ -46,6 +46,7 @@ struct DevScalars {
 struct WalkParams {
   double rho;
   double hs[3][2];   // ((-rho * s) * delta_a) * 0.5 for s = -1 (index 0) / +1 (index 1)
+  float hsf[3][2];   // the same factors rounded to FP32
   double thr;        // Select threshold (+inf for plain-mask)
   int method;
   int faithful;

@@ -34,12 +34,14 @@ k_absmax(const T* __restrict__ x, int64_t n, unsigned long long* __restrict__ bi
 
 // Fixed-point scale: every accumulator holds at most cmax contributions of
 // magnitude <= pmax + (iters + 1) * gstep * lmax (plus one seed term), so the
-// scale 2^e keeps the sums below 2^62.
+// scale 2^e keeps every term below 2^51 (the range of the rounding in
+// to_fixed) and the sums below 2^62.
 __global__ void k_compute_scale(DevScalars* sc, double gfac, double lmax, double cmax,
                                 double iters) {
   const double amax = __longlong_as_double((long long)sc->amax_bits);
   const double pmax = __longlong_as_double((long long)sc->pmax_bits);
-  const double bound = (cmax + 1.0) * (pmax + (iters + 1.0) * gfac * amax * lmax);
+  const double term = pmax + (iters + 1.0) * gfac * amax * lmax;
+  const double bound = (cmax + 1.0) * term;
   if (!(bound <= DBL_MAX)) {
     sc->status = 5;  // OMNI3D_ERR_NONFINITE
     sc->scale = 1.0;
@@ -48,9 +50,10 @@ __global__ void k_compute_scale(DevScalars* sc, double gfac, double lmax, double
   }
   int e = 40;
   if (bound > 0.0) {
-    int ex;
+    int ex, et;
     frexp(bound, &ex);  // bound < 2^ex
-    e = 62 - ex;
+    frexp(term, &et);   // term < 2^et
+    e = 62 - ex < 51 - et ? 62 - ex : 51 - et;
   }
   if (e > 1000) e = 1000;
   sc->scale = ldexp(1.0, e);

@@ -231,14 +231,16 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
       if (kMode == kFull) add_lds(lv, pin + carry);
       if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry);
     }
+    // every later visit adds base + (in-tile sum so far)
+    const double base = kMode == kFull ? pin + carry : carry;
     for (unsigned int vis = 1; vis < n; vis++) {
       const int a = next_code();
       const int sgn = sign_of(a);
       const int ln = lv + sgn * (a == 0 ? 1 : (a == 1 ? kTile : kTile * kTile));
       loc.add(trapezoid<T>(wp, a, sgn, L.g[a][ln], L.g[a][lv]));
       lv = ln;
-      if (kMode == kFull) add_lds(lv, pin + (carry + loc.value()));
-      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry + loc.value());
+      if (kMode == kFull) add_lds(lv, base + loc.value());
+      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, base + loc.value());
     }
     if (kMode == kLocalSum) segval[sr.g] = loc.value();
   }

@@ -102,8 +102,11 @@ __device__ __forceinline__ int walk_geometry(const RayRec& rr, const DirConst* _
   return steps;
 }
 
+// round(x * scale) to an integer for |x * scale| < 2^51 (the scale is chosen
+// so): adding 1.5 * 2^52 leaves the rounded integer in the low mantissa bits.
 __device__ __forceinline__ long long to_fixed(double x, double scale) {
-  return __double2ll_rn(x * scale);
+  const double y = x * scale + 6755399441055744.0;
+  return __double_as_longlong(y) - 0x4338000000000000LL;
 }
 
 __device__ __forceinline__ void fx_add(long long* p, long long v) {
@@ -139,20 +142,26 @@ struct RaySum<float> {
 template <typename T>
 __device__ __forceinline__ T trapezoid(const WalkParams& wp, int a, int s, T an, T al);
 
+// The step factor is picked with constant indices (selects, no indexed
+// load), so the six factors stay in scalar registers across the walk.
 template <>
 __device__ __forceinline__ double trapezoid<double>(const WalkParams& wp, int a, int s, double an,
                                                     double al) {
-  const int si = s > 0;
-  const double hs = a == 0 ? wp.hs[0][si] : (a == 1 ? wp.hs[1][si] : wp.hs[2][si]);
-  return hs * (an + al);
+  const bool up = s > 0;
+  const double h0 = up ? wp.hs[0][1] : wp.hs[0][0];
+  const double h1 = up ? wp.hs[1][1] : wp.hs[1][0];
+  const double h2 = up ? wp.hs[2][1] : wp.hs[2][0];
+  return (a == 0 ? h0 : (a == 1 ? h1 : h2)) * (an + al);
 }
 
 template <>
 __device__ __forceinline__ float trapezoid<float>(const WalkParams& wp, int a, int s, float an,
                                                   float al) {
-  const int si = s > 0;
-  const double hs = a == 0 ? wp.hs[0][si] : (a == 1 ? wp.hs[1][si] : wp.hs[2][si]);
-  return (float)hs * (an + al);
+  const bool up = s > 0;
+  const float h0 = up ? wp.hsf[0][1] : wp.hsf[0][0];
+  const float h1 = up ? wp.hsf[1][1] : wp.hsf[1][0];
+  const float h2 = up ? wp.hsf[2][1] : wp.hsf[2][0];
+  return (a == 0 ? h0 : (a == 1 ? h1 : h2)) * (an + al);
 }
 
 // Field staged as one 4-vector per voxel (a_x, a_y, a_z, 0): a step loads the
