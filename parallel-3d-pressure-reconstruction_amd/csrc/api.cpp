@@ -296,7 +296,8 @@ int build_plan(omni3d_ctx* ctx) {
     if (nnz >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many source visits");
     OMNI_TRY_HIP(ctx, pl.masks.alloc(sizeof(unsigned long long) * (nm + 1)));
     OMNI_TRY_HIP(ctx, pl.cell_eb.alloc(sizeof(unsigned int) * (nm + 1)));
-    OMNI_TRY_HIP(ctx, pl.ent.alloc(sizeof(int) * (nnz + 1)));
+    // slack after the entries: the iteration stages them with 16-byte loads
+    OMNI_TRY_HIP(ctx, pl.ent.alloc(sizeof(int) * (nnz + 8)));
     OMNI_TRY_HIP(ctx, launch_sell_fill(nS, D, pl.rowptr.as<unsigned long long>(),
                                        key.as<unsigned long long>(), pl.moff.as<unsigned long long>(),
                                        pl.masks.as<unsigned long long>(),

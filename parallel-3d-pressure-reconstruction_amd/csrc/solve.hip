@@ -87,9 +87,15 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
           const double* __restrict__ pold, double* __restrict__ pnew,
           const DevScalars* __restrict__ sc, double* __restrict__ partials) {
   if (sc->converged) return;
+  constexpr int kU = 16;
+  // per wavefront: the packed entries of one round of kU slots (at most
+  // 64 each) plus the alignment slack of the wide loads
+  constexpr int kStage = kU * 64 + 8;
   __shared__ double s_acc[kWaves][64];
+  __shared__ __attribute__((aligned(16))) int s_ent[kWaves][kStage];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int* __restrict__ se = s_ent[wave];
   const double inv = sc->inv_scale;
   const unsigned long long below = (1ull << lane) - 1ull;
   double d2 = 0.0, n2 = 0.0;
@@ -102,31 +108,46 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
     unsigned long long eb = ms < me ? (unsigned long long)cell_eb[ms] : 0ull;
     double acc = 0.0;
     unsigned long long s = ms;
-    constexpr int kU = 16;
     for (; s + kU <= me; s += kU) {
-      // masks first (scalar loads: the slot is uniform across the
-      // wavefront), then every entry load, then every gather: three
-      // dependent rounds per kU slots.  A lane without an entry in a slot
-      // reads the slot's first entry and adds nothing.
+      // the slot masks (scalar loads: the slot is uniform across the
+      // wavefront) give the round's entry range [eb, e_end)
       unsigned long long m[kU];
 #pragma unroll
       for (int u = 0; u < kU; u++) m[u] = masks[s + u];
+      unsigned int off[kU];
+      unsigned int n = 0;
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        off[u] = n;
+        n += (unsigned int)__popcll(m[u]);
+      }
+      // stage the range with 16-byte loads from the aligned-down start (a
+      // few address batches instead of one per slot), then read each
+      // lane's entries from LDS
+      const unsigned long long a0 = eb & ~3ull;
+      const unsigned int sh = (unsigned int)(eb - a0);
+      const unsigned int n4 = (sh + n + 3) >> 2;
+      const int4* __restrict__ src4 = reinterpret_cast<const int4*>(ent + a0);
+      for (unsigned int t = lane; t < n4; t += 64)
+        reinterpret_cast<int4*>(se)[t] = src4[t];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       bool has[kU];
-      unsigned long long pos[kU];
+      int idx[kU];
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         has[u] = (m[u] >> lane) & 1ull;
-        pos[u] = eb + (has[u] ? (unsigned long long)__popcll(m[u] & below) : 0ull);
-        eb += __popcll(m[u]);
+        const unsigned int r = has[u] ? (unsigned int)__popcll(m[u] & below) : 0u;
+        idx[u] = se[sh + off[u] + r];
       }
-      int idx[kU];
-#pragma unroll
-      for (int u = 0; u < kU; u++) idx[u] = ent[pos[u]];
       double x[kU];
 #pragma unroll
       for (int u = 0; u < kU; u++) x[u] = pold[idx[u]];
 #pragma unroll
       for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
+      eb += n;
+      __builtin_amdgcn_wave_barrier();
     }
     for (; s < me; s++) {
       const unsigned long long mm = masks[s];
@@ -458,7 +479,7 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
                           double* partials, int max_iter, double tol, hipStream_t st) {
   const unsigned int grid = iterate_grid(nS);
 #define OMNI_ITER(W)                                                                           \
-  hipLaunchKernelGGL(k_iterate<W>, dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks, cell_eb, \
+  hipLaunchKernelGGL((k_iterate<W>), dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks, cell_eb, \
                      ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
   if (waves <= 1) OMNI_ITER(1);
   else if (waves == 2) OMNI_ITER(2);

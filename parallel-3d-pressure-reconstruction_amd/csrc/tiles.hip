@@ -135,10 +135,24 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 // ---------------------------------------------------------------------------
 // Tile passes.
 
-template <typename T, int kVox>
+// LDS of a tile pass: the tile's gradient; the fixed-point counters (full
+// and source-sum passes); the source ids of the tile's voxels (source-sum
+// pass).  Exact sizes matter: two FP32 16^3 full-pass workgroups fill the
+// 160 KB of a CU.
+template <typename T, int kVox, int kMode>
 struct TileLds {
   T g[3][kVox];
   long long acc[kVox];
+};
+template <typename T, int kVox>
+struct TileLds<T, kVox, kLocalSum> {
+  T g[3][kVox];
+};
+template <typename T, int kVox>
+struct TileLds<T, kVox, kSourceSums> {
+  T g[3][kVox];
+  long long acc[kVox];
+  int sidl[kVox];
 };
 
 template <typename T, int kMode, int kBits, int kThreads>
@@ -154,7 +168,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
   constexpr int kTileVox = kTile * kTile * kTile;
-  __shared__ TileLds<T, kTileVox> L;
+  __shared__ TileLds<T, kTileVox, kMode> L;
   // contiguous tile ranges per XCD: neighbouring tiles share an L2
   const int tb = (int)xcd_block();
   const int tile = tile_list ? tile_list[tb] : tb;
@@ -166,12 +180,15 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const int i = ox + (l & kTileMask), j = oy + ((l >> kTileBits) & kTileMask),
               k = oz + (l >> (2 * kTileBits));
     V4 v;
-    if (i < g.X && j < g.Y && k < g.Z) v = field[i + j * sy + k * sz];
+    const bool in = i < g.X && j < g.Y && k < g.Z;
+    if (in) v = field[i + j * sy + k * sz];
     else v.x = v.y = v.z = v.w = 0;
     L.g[0][l] = v.x;
     L.g[1][l] = v.y;
     L.g[2][l] = v.z;
-    if (kMode != kLocalSum) L.acc[l] = 0;
+    if constexpr (kMode != kLocalSum) L.acc[l] = 0;
+    // source voxels (all on the box surface) by their compact id
+    if constexpr (kMode == kSourceSums) L.sidl[l] = in ? sid[i + j * sy + k * sz] : -1;
   }
   __syncthreads();
   const double scale = sc->scale;
@@ -201,15 +218,13 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     };
     auto sign_of = [&](int a) { return ((sg >> a) & 1u) ? 1 : -1; };
     auto add_lds = [&](int l, double x) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]),
-                (unsigned long long)to_fixed(x, scale));
+      if constexpr (kMode != kLocalSum)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]),
+                  (unsigned long long)to_fixed(x, scale));
     };
     auto on_source = [&](int l) {
-      const int i = ox + (l & kTileMask), jj = oy + ((l >> kTileBits) & kTileMask),
-                k = oz + (l >> (2 * kTileBits));
-      const bool surface =
-          i == 0 || i == g.X - 1 || jj == 0 || jj == g.Y - 1 || k == 0 || k == g.Z - 1;
-      return surface && sid[i + jj * sy + k * sz] >= 0;
+      if constexpr (kMode == kSourceSums) return L.sidl[l] >= 0;
+      else return false;
     };
     if (first) {
       // the entry visit: self-add p_src (pint = 0)
@@ -251,9 +266,9 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
               k = oz + (l >> (2 * kTileBits));
     if (i >= g.X || j >= g.Y || k >= g.Z) continue;
     const int64_t v = i + j * sy + k * sz;
-    if (kMode == kFull) out[v] = L.acc[l];
-    if (kMode == kSourceSums) {
-      const int s = sid[v];
+    if constexpr (kMode == kFull) out[v] = L.acc[l];
+    if constexpr (kMode == kSourceSums) {
+      const int s = L.sidl[l];
       if (s >= 0) out[s] = L.acc[l];
     }
   }
