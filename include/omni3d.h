@@ -267,8 +267,9 @@ int omni3d_read_parameter_file(const char* path, omni3d_parameter_file* out);
  * Parameter_Omni3D.dat at `path` (relative paths in it are taken from the
  * file's folder, '\' read as '/'), and for every file of the acceleration
  * folder (sorted; Tecplot POINT, 3 header lines, columns x y z u v w DuDt
- * DvDt DwDt mask, i fastest) crops the region of interest, seeds the j = 0
- * face with pref - rho |u|^2 / 2, runs the Select method (faithful, curl
+ * DvDt DwDt mask, i fastest) crops the region of interest, starts from the
+ * previous file's result (zero before the first; kernel.cu:8863) with the j = 0
+ * face set to pref - rho |u|^2 / 2, runs the Select method (faithful, curl
  * threshold of line 15) for the line-14 iterations and writes
  * PressureOmni3D_%05d.dat (X Y Z P Count, %15.9f); file 0 also writes
  * CurlofMaterialAcc_Sample.dat; log.dat lists the files.  Outputs and

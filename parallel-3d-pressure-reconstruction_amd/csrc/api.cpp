@@ -147,6 +147,20 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
                                      pl.sid.as<int>(), pl.val_off.as<unsigned long long>(),
                                      pl.tile_ptr.as<unsigned long long>(), tcnt.as<unsigned int>(),
                                      pl.segs.as<SegRec>(), st));
+  {
+    // every (tile, length bucket) list in value-slot order
+    int key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) < nval) key_bits++;
+    DevBuf sorted, stmp;
+    OMNI_TRY_HIP(ctx, sorted.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
+    const size_t tb = sort_segments_tmp_bytes((int64_t)nsegs, nslots);
+    OMNI_TRY_HIP(ctx, stmp.alloc(tb));
+    OMNI_TRY_HIP(ctx, launch_sort_segments(pl.segs.as<SegRec>(), sorted.as<SegRec>(), (int64_t)nsegs,
+                                           nslots, pl.tile_ptr.as<unsigned long long>(), key_bits,
+                                           stmp.as<void>(), tb, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    pl.segs.swap(sorted);
+  }
   // tiles that touch the box surface hold every source voxel of the plain method
   std::vector<int> surf;
   for (int t = 0; t < nt; t++) {
@@ -653,9 +667,17 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
     if (n >= 1 && n <= 64) ctx->acc_copies = n;
   }
   if (const char* e = std::getenv("OMNI3D_ENGINE")) ctx->engine = std::strcmp(e, "rays") == 0 ? 1 : 0;
+  if (const char* e = std::getenv("OMNI3D_TILE_BITS")) {
+    const int b = std::atoi(e);
+    if (b >= kTileBitsMin && b <= kTileBitsMax) ctx->tile_bits = b;
+  }
+  if (const char* e = std::getenv("OMNI3D_TILE_THREADS")) {
+    const int t = std::atoi(e);
+    if (t == 256 || t == 512 || t == 1024) ctx->tile_threads = t;
+  }
   if (const char* e = std::getenv("OMNI3D_ITER_WAVES")) {
     const int w = std::atoi(e);
-    if (w == 1 || w == 2 || w == 4) ctx->iter_waves = w;
+    if (w == 1 || w == 2 || w == 4 || w == -1 || w == -2 || w == -4) ctx->iter_waves = w;
   }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {

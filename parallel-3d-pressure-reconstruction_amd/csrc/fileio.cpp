@@ -240,6 +240,7 @@ int omni3d_run_parameter_file_ex(const char* path, int precision) {
   log << "Processing Starts" << std::endl;
   const int X = P.nx, Y = P.ny, Z = P.nz;
   const size_t V = (size_t)X * Y * Z;
+  std::vector<double> p(V, 0.0);
   int done = 0;
   for (size_t fi = 0; fi < files.size(); fi++) {
     const std::string& fpath = files[fi];
@@ -254,8 +255,10 @@ int omni3d_run_parameter_file_ex(const char* path, int precision) {
     rc = omni3d_set_field(ctx, s.ax.data(), s.ay.data(), s.az.data(), s.mask.data(), OMNI3D_F64,
                           OMNI3D_MEM_HOST);
     if (rc) return -fail(nullptr, rc, ctx->err);
-    // Bernoulli pressure on the j = 0 face, zero elsewhere (kernel.cu:8806-8811)
-    std::vector<double> p(V, 0.0), cnt(V, 0.0);
+    // the previous file's result stays the initial pressure (the host array
+    // carries over, kernel.cu:8863 -> 8812), with the Bernoulli pressure
+    // written on the j = 0 face (kernel.cu:8806-8811); zero before file 0
+    std::vector<double> cnt(V, 0.0);
     for (int i = 0; i < X; i++)
       for (int k = 0; k < Z; k++) {
         const size_t ind = (size_t)i + (size_t)k * X * Y;

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/omni3d.h"
@@ -40,6 +41,10 @@ class DevBuf {
     if (ptr_) (void)hipFree(ptr_);
     ptr_ = nullptr;
     size_ = 0;
+  }
+  void swap(DevBuf& o) {
+    std::swap(ptr_, o.ptr_);
+    std::swap(size_, o.size_);
   }
   template <typename T>
   T* as() const {

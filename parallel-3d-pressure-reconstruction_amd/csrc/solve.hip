@@ -77,7 +77,7 @@ k_gather_sources(const int* __restrict__ slist, int nS, const double* __restrict
 // order.  Each workgroup leaves its share of the relative-change sums in
 // partials; k_iter_finish reduces them and sets the stop flag, after which
 // the remaining launches return at once.
-template <int kWaves>
+template <int kWaves, bool kMasked>
 __global__ void __launch_bounds__(64 * kWaves)
 k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
           const unsigned long long* __restrict__ masks, const unsigned int* __restrict__ cell_eb,
@@ -142,10 +142,21 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
         idx[u] = se[sh + off[u] + r];
       }
       double x[kU];
+      if (kMasked) {
+        // only the lanes with an entry issue their gather
 #pragma unroll
-      for (int u = 0; u < kU; u++) x[u] = pold[idx[u]];
+        for (int u = 0; u < kU; u++) {
+          x[u] = 0.0;
+          if (has[u]) x[u] = pold[idx[u]];
+        }
 #pragma unroll
-      for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
+        for (int u = 0; u < kU; u++) acc += x[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < kU; u++) x[u] = pold[idx[u]];
+#pragma unroll
+        for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
+      }
       eb += n;
       __builtin_amdgcn_wave_barrier();
     }
@@ -478,12 +489,20 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
                           int seed, const double* pold, double* pnew, DevScalars* sc,
                           double* partials, int max_iter, double tol, hipStream_t st) {
   const unsigned int grid = iterate_grid(nS);
-#define OMNI_ITER(W)                                                                           \
-  hipLaunchKernelGGL((k_iterate<W>), dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks, cell_eb, \
-                     ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
-  if (waves <= 1) OMNI_ITER(1);
-  else if (waves == 2) OMNI_ITER(2);
-  else OMNI_ITER(4);
+#define OMNI_ITER(W, M)                                                                        \
+  hipLaunchKernelGGL((k_iterate<W, M>), dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks,     \
+                     cell_eb, ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
+  const bool masked = waves < 0;
+  if (waves < 0) waves = -waves;
+  if (masked) {
+    if (waves <= 1) OMNI_ITER(1, true);
+    else if (waves == 2) OMNI_ITER(2, true);
+    else OMNI_ITER(4, true);
+  } else {
+    if (waves <= 1) OMNI_ITER(1, false);
+    else if (waves == 2) OMNI_ITER(2, false);
+    else OMNI_ITER(4, false);
+  }
 #undef OMNI_ITER
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol);

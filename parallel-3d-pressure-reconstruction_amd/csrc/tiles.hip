@@ -14,6 +14,8 @@
 //           k_tile_pass<kFull> -> averaging
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "tiles.hpp"
@@ -195,8 +197,13 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
   const unsigned long long b = tile_ptr[tile * kLenBuckets],
                            e = tile_ptr[(tile + 1) * kLenBuckets];
-  for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x) {
-    const SegRec sr = segs[j];
+  // the record of the next segment is loaded while this one is walked
+  SegRec nxt;
+  unsigned long long j = b + threadIdx.x;
+  if (j < e) nxt = segs[j];
+  for (; j < e; j += blockDim.x) {
+    const SegRec sr = nxt;
+    if (j + blockDim.x < e) nxt = segs[j + blockDim.x];
     // carry: the ray's sum before this segment plus the step into the tile
     // (kLocalSum computes that step, the only read outside the tile, and
     // stores it per segment in tile order)
@@ -371,6 +378,34 @@ k_seg_groups(int64_t nrays, const unsigned int* __restrict__ nseg,
   if (threadIdx.x == 0) gcnt[blockIdx.x] = 64u * m;
 }
 
+// Slot order inside every (tile, length bucket) list: the passes then read
+// the carries of neighbouring lanes from neighbouring value slots, and the
+// plan no longer depends on the order the fill kernel's atomics gave.
+__global__ void __launch_bounds__(256)
+k_seg_keys(const SegRec* __restrict__ segs, int64_t n, unsigned int* __restrict__ key,
+           unsigned int* __restrict__ idx) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    key[i] = segs[i].g;
+    idx[i] = (unsigned int)i;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_seg_gather(const SegRec* __restrict__ in, const unsigned int* __restrict__ idx, int64_t n,
+             SegRec* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[idx[i]];
+}
+
+__global__ void __launch_bounds__(256)
+k_u64_to_u32(const unsigned long long* __restrict__ in, int64_t n, unsigned int* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (unsigned int)in[i];
+}
+
 // Per-ray exclusive scan of the segment sums into carries, in place; the
 // lanes of a wavefront are the 64 rays of one group, so every step reads 64
 // consecutive slots.
@@ -497,6 +532,51 @@ hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* d
                             SegRec* segs, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_fill, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
                      dirs, g, tg, sid, val_off, tile_ptr, cursor, segs);
+  return hipGetLastError();
+}
+
+size_t sort_segments_tmp_bytes(int64_t nsegs, int nslots) {
+  size_t cub = 0;
+  (void)hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, cub, (const unsigned int*)nullptr,
+                                              (unsigned int*)nullptr, (const unsigned int*)nullptr,
+                                              (unsigned int*)nullptr, (int)nsegs, nslots,
+                                              (const unsigned int*)nullptr,
+                                              (const unsigned int*)nullptr, 0, 32, nullptr);
+  const size_t arrays = 4 * sizeof(unsigned int) * (size_t)nsegs + sizeof(unsigned int) * (nslots + 1);
+  return ((cub + 255) & ~(size_t)255) + ((arrays + 255) & ~(size_t)255) + 256 * 8;
+}
+
+hipError_t launch_sort_segments(SegRec* segs, SegRec* out, int64_t nsegs, int nslots,
+                                const unsigned long long* slot_ptr, int key_bits, void* tmp,
+                                size_t tmp_bytes, hipStream_t st) {
+  if (nsegs <= 0) return hipSuccess;
+  if (nsegs >= (1ll << 31) || tmp_bytes < sort_segments_tmp_bytes(nsegs, nslots))
+    return hipErrorInvalidValue;
+  char* const tmp_end = static_cast<char*>(tmp) + tmp_bytes;
+  auto carve = [&](size_t bytes) {
+    void* p = tmp;
+    tmp = static_cast<char*>(tmp) + ((bytes + 255) & ~(size_t)255);
+    return p;
+  };
+  unsigned int* kin = static_cast<unsigned int*>(carve(sizeof(unsigned int) * nsegs));
+  unsigned int* kout = static_cast<unsigned int*>(carve(sizeof(unsigned int) * nsegs));
+  unsigned int* vin = static_cast<unsigned int*>(carve(sizeof(unsigned int) * nsegs));
+  unsigned int* vout = static_cast<unsigned int*>(carve(sizeof(unsigned int) * nsegs));
+  unsigned int* off = static_cast<unsigned int*>(carve(sizeof(unsigned int) * (nslots + 1)));
+  size_t cub = 0;
+  hipError_t e = hipcub::DeviceSegmentedRadixSort::SortPairs(
+      nullptr, cub, kin, kout, vin, vout, (int)nsegs, nslots, off, off + 1, 0, key_bits, st);
+  if (e != hipSuccess) return e;
+  void* cub_tmp = carve(cub);
+  if (static_cast<char*>(tmp) > tmp_end) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_seg_keys, dim3(grid_for(nsegs)), dim3(256), 0, st, segs, nsegs, kin, vin);
+  hipLaunchKernelGGL(k_u64_to_u32, dim3(grid_for(nslots + 1)), dim3(256), 0, st, slot_ptr,
+                     (int64_t)nslots + 1, off);
+  e = hipcub::DeviceSegmentedRadixSort::SortPairs(cub_tmp, cub, kin, kout, vin, vout, (int)nsegs,
+                                                  nslots, off, off + 1, 0, key_bits, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_seg_gather, dim3(grid_for(nsegs)), dim3(256), 0, st, segs, vout, nsegs,
+                     out);
   return hipGetLastError();
 }
 

@@ -44,6 +44,12 @@ hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* d
                             TileGrid tg, const int* sid, const unsigned long long* val_off,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
                             SegRec* segs, hipStream_t st);
+// sorts every (tile, length bucket) list of segments by value slot into out
+// (slot_ptr: the nslots + 1 list offsets; key_bits: bits of the largest slot)
+size_t sort_segments_tmp_bytes(int64_t nsegs, int nslots);
+hipError_t launch_sort_segments(SegRec* segs, SegRec* out, int64_t nsegs, int nslots,
+                                const unsigned long long* slot_ptr, int key_bits, void* tmp,
+                                size_t tmp_bytes, hipStream_t st);
 // slots per group of 64 rays: 64 * the group's largest segment count
 hipError_t launch_seg_groups(int64_t nrays, const unsigned int* nseg, unsigned int* gcnt,
                              hipStream_t st);

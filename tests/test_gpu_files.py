@@ -1,6 +1,8 @@
 """End-to-end check of the process-level drop-in (Omni3D_8.0.exe,
 kernel.cu:8467-8983) on the GPU: a Parameter_Omni3D.dat with a region of
-interest, Windows-style folder names and two Tecplot acceleration files;
+interest, Windows-style folder names and two Tecplot acceleration files
+(the second starting from the first's result, as the reference's host
+pressure array carries over);
 the written PressureOmni3D_%05d.dat (P and Count columns),
 CurlofMaterialAcc_Sample.dat and log.dat against the FP64 oracle run of the
 same Select reconstruction (Bernoulli seed on the j = 0 face, curl * mask,
@@ -53,12 +55,14 @@ def crop(a):
     return np.ascontiguousarray(a[k0:k1 + 1, j0:j1 + 1, i0:i1 + 1])
 
 
-def expected(cols, thr, dirs):
+def expected(cols, thr, dirs, prev=None):
+    """prev: the previous file's result, the initial pressure of the next one
+    (kernel.cu:8863 -> 8812), with the Bernoulli face written over it."""
     x, y, z, u, v, w, ax, ay, az, m = [crop(c) for c in cols]
     shape = ax.shape
     acc = [ax, ay, az]
     curl = O.curl(shape, (DELTA,) * 3, acc, m)
-    p0 = np.zeros(shape)
+    p0 = np.zeros(shape) if prev is None else prev.copy()
     p0[:, 0, :] = PREF - 0.5 * RHO * (u[:, 0, :] ** 2 + v[:, 0, :] ** 2 + w[:, 0, :] ** 2)
     p, cnt, it, rel = O.reconstruct(shape, dirs, 1.0, (DELTA,) * 3, RHO, acc, p0=p0, curl=curl,
                                     threshold=thr, method=O.SELECT, faithful=1, max_iter=NITR,
@@ -79,8 +83,10 @@ def test_parameter_file_run(gpu_available, ref_dirs2, tmp_path):
     par.write_text("\r\n".join(str(v) for v in lines) + "\r\n")
     assert omni3d.run_parameter_file(par) == 2
     assert (tmp_path / "log.dat").read_text() == "Processing Starts\nacc_000.datacc_001.dat"
+    prev = None
     for n, (cols, m) in enumerate(cases):
-        p, cnt, curl, xyz = expected(cols, thr, ref_dirs2)
+        p, cnt, curl, xyz = expected(cols, thr, ref_dirs2, prev)
+        prev = p
         out = np.loadtxt(tmp_path / "out" / f"PressureOmni3D_{n:05d}.dat", skiprows=3)
         shape = p.shape
         assert out.shape == (p.size, 5)

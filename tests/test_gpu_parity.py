@@ -89,3 +89,16 @@ def test_repeatable_bitwise(gpu_available, ref_dirs3):
     a = run_gpu(f, ref_dirs3, shape, max_iter=8)[0]
     b = run_gpu(f, ref_dirs3, shape, max_iter=8)[0]
     np.testing.assert_array_equal(a, b)
+
+
+def test_ray_engine_matches_oracle(gpu_available, ref_dirs2, monkeypatch):
+    """OMNI3D_ENGINE=rays: one lane per ray with global fixed-point atomics
+    instead of the tile engine, same results."""
+    shape = (15, 14, 13)
+    print("case ray engine", flush=True)
+    monkeypatch.setenv("OMNI3D_ENGINE", "rays")
+    f = F.taylor_green(15, noise=0.05, seed=9, shape=shape)
+    p, w, rep = run_gpu(f, ref_dirs2, shape, max_iter=7)
+    po, co, it, rel = run_oracle(f, ref_dirs2, shape, max_iter=7)
+    np.testing.assert_array_equal(w, co)
+    assert F.rel_l2(p, po) < 1e-12
