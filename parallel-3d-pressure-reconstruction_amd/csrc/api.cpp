@@ -14,6 +14,7 @@
 #include "driver.hpp"
 #include "host.hpp"
 #include "launch.hpp"
+#include "stage.hpp"
 
 using namespace omni3d;
 
@@ -100,6 +101,81 @@ void number_sources(int X, int Y, int Z, const std::vector<uint8_t>& flag, std::
     (*slist)[s] = keys[s].second;
     (*sid)[keys[s].second] = (int)s;
   }
+}
+
+// LDS-staged layout of the iteration (stage.hip).  Left off (pl.staged
+// false, the slot-ordered gathers are used) when a (group, direction) needs
+// more lines than a stage holds or the sizes exceed the 32-bit indices.
+int build_staged(omni3d_ctx* ctx, int nS, int D, int64_t nnz) {
+  Plan& pl = *ctx->plan;
+  hipStream_t st = ctx->stream;
+  pl.staged = false;
+  const int nch = (nS + 63) / 64;
+  const int ngroups = (nch + kStageGroup - 1) / kStageGroup;
+  const int64_t ngd = (int64_t)ngroups * D;
+  if (nnz <= 0 || nnz >= (1ll << 31) || pl.ncell >= (1ll << 32) || D > 65535 ||
+      stage_key_bits(nS, D) > 64)
+    return OMNI3D_OK;
+  DevBuf ka, kb, va, vb, flag, incl, tmp, gdcnt, gdptr, st_dend, sbase, ovf;
+  OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
+  OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * nnz));
+  OMNI_TRY_HIP(ctx, va.alloc(sizeof(unsigned int) * nnz));
+  OMNI_TRY_HIP(ctx, vb.alloc(sizeof(unsigned int) * nnz));
+  OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * nnz));
+  OMNI_TRY_HIP(ctx, incl.alloc(sizeof(unsigned int) * nnz));
+  OMNI_TRY_HIP(ctx, tmp.alloc(stage_tmp_bytes(nnz, ngd)));
+  StageBuild sb;
+  sb.nS = nS;
+  sb.D = D;
+  sb.nch = nch;
+  sb.nnz = nnz;
+  sb.moff = pl.moff.as<unsigned long long>();
+  sb.masks = pl.masks.as<unsigned long long>();
+  sb.sdir = pl.sdir.as<unsigned short>();
+  sb.cell_eb = pl.cell_eb.as<unsigned int>();
+  sb.ent = pl.ent.as<int>();
+  sb.keys_a = ka.as<unsigned long long>();
+  sb.keys_b = kb.as<unsigned long long>();
+  sb.vals_a = va.as<unsigned int>();
+  sb.vals_b = vb.as<unsigned int>();
+  sb.flag = flag.as<unsigned int>();
+  sb.incl = incl.as<unsigned int>();
+  sb.tmp = tmp.as<void>();
+  sb.tmp_bytes = tmp.size();
+  OMNI_TRY_HIP(ctx, stage_keys_sorted(sb, st));
+  unsigned int nl = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nl, sb.incl + (nnz - 1), 4, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  OMNI_TRY_HIP(ctx, pl.ulines.alloc(sizeof(unsigned int) * ((size_t)nl + 1)));
+  OMNI_TRY_HIP(ctx, gdcnt.alloc(sizeof(unsigned int) * (ngd + 1)));
+  OMNI_TRY_HIP(ctx, gdptr.alloc(sizeof(unsigned int) * (ngd + 1)));
+  OMNI_TRY_HIP(ctx, pl.st_n.alloc(sizeof(unsigned int) * ngroups));
+  OMNI_TRY_HIP(ctx, pl.st_lo.alloc(sizeof(unsigned int) * ngd));
+  OMNI_TRY_HIP(ctx, pl.st_hi.alloc(sizeof(unsigned int) * ngd));
+  OMNI_TRY_HIP(ctx, st_dend.alloc(sizeof(unsigned short) * ngd));
+  OMNI_TRY_HIP(ctx, sbase.alloc(sizeof(unsigned int) * ngd));
+  OMNI_TRY_HIP(ctx, pl.st_cse.alloc(sizeof(unsigned int) * ngd * kStageGroup));
+  // slack after the positions: the iteration stages them with 16-byte loads
+  OMNI_TRY_HIP(ctx, pl.soff.alloc(sizeof(unsigned short) * (nnz + 16)));
+  OMNI_TRY_HIP(ctx, ovf.alloc(sizeof(unsigned int)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(ovf.as<void>(), 0, sizeof(unsigned int), st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.soff.as<void>(), 0, sizeof(unsigned short) * (nnz + 16), st));
+  OMNI_TRY_HIP(ctx, stage_finish(sb, ngroups, pl.ulines.as<unsigned int>(), gdcnt.as<unsigned int>(),
+                                 gdptr.as<unsigned int>(), pl.st_n.as<unsigned int>(),
+                                 pl.st_lo.as<unsigned int>(), pl.st_hi.as<unsigned int>(),
+                                 st_dend.as<unsigned short>(), sbase.as<unsigned int>(),
+                                 pl.st_cse.as<unsigned int>(), pl.soff.as<unsigned short>(),
+                                 ovf.as<unsigned int>(), st));
+  unsigned int overflow = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&overflow, ovf.as<void>(), 4, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  pl.nlines = nl;
+  pl.staged = overflow == 0;
+  if (std::getenv("OMNI3D_VERBOSE"))
+    std::fprintf(stderr, "omni3d plan: %d sources, %lld entries, %lld slots, %u staged lines%s\n",
+                 nS, (long long)nnz, (long long)pl.ncell, nl,
+                 pl.staged ? "" : " (stage overflow: slot-ordered gathers)");
+  return OMNI3D_OK;
 }
 
 // Tile engine part of the plan: compiled step codes and per-tile segments.
@@ -310,13 +386,44 @@ int build_plan(omni3d_ctx* ctx) {
     if (nnz >= (1ull << 32)) return fail(ctx, OMNI3D_ERR_ARG, "too many source visits");
     OMNI_TRY_HIP(ctx, pl.masks.alloc(sizeof(unsigned long long) * (nm + 1)));
     OMNI_TRY_HIP(ctx, pl.cell_eb.alloc(sizeof(unsigned int) * (nm + 1)));
+    OMNI_TRY_HIP(ctx, pl.sdir.alloc(sizeof(unsigned short) * (nm + 1)));
     // slack after the entries: the iteration stages them with 16-byte loads
     OMNI_TRY_HIP(ctx, pl.ent.alloc(sizeof(int) * (nnz + 8)));
     OMNI_TRY_HIP(ctx, launch_sell_fill(nS, D, pl.rowptr.as<unsigned long long>(),
                                        key.as<unsigned long long>(), pl.moff.as<unsigned long long>(),
                                        pl.masks.as<unsigned long long>(),
-                                       pl.cell_eb.as<unsigned int>(), pl.ent.as<int>(), st));
+                                       pl.cell_eb.as<unsigned int>(),
+                                       pl.sdir.as<unsigned short>(), pl.ent.as<int>(), st));
     pl.ncell = (int64_t)nm;
+  }
+  {
+    // dense layout (default iteration) unless sizes exceed its word format:
+    // 24-bit source ids, rows shorter than 4096 entries (int64 row sums of
+    // terms below 2^51)
+    const char* env = std::getenv("OMNI3D_ITER");
+    pl.dense = false;
+    if (!(env && std::strcmp(env, "slots") == 0) && nS < (1 << 24) && mxh[0] < 4096u &&
+        nnz > 0 && nnz < (1ull << 31)) {
+      DevBuf ka, kb, tmp;
+      OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
+      OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * nnz));
+      OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)nnz)));
+      // slack after the words: the iteration stages them with 16-byte loads
+      OMNI_TRY_HIP(ctx, pl.dent.alloc(sizeof(unsigned int) * (nnz + 8)));
+      OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.dent.as<void>(), 0, sizeof(unsigned int) * (nnz + 8), st));
+      OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, pl.rowptr.as<unsigned long long>(),
+                                            key.as<unsigned long long>(),
+                                            ka.as<unsigned long long>(), kb.as<unsigned long long>(),
+                                            tmp.as<void>(), tmp.size(), pl.dent.as<unsigned int>(),
+                                            st));
+      OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+      pl.dense = true;
+    }
+    pl.staged = false;
+    if (env && std::strcmp(env, "staged") == 0) {
+      const int rc = build_staged(ctx, nS, D, (int64_t)nnz);
+      if (rc) return rc;
+    }
   }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   int rc = build_tiles(ctx, g, (int64_t)nrays);
@@ -418,13 +525,48 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   }
   record_event(ctx, 3);
   // source iteration
-  OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * ns));
-  OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns));
-  OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * iterate_grid(pl.nS)));
+  // p_S padded to whole 16-source lines (the staged iteration copies lines)
+  const size_t ns16 = (ns + 15) & ~(size_t)15;
+  OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * ns16));
+  OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns16));
+  OMNI_TRY_HIP(ctx, ctx->partials.alloc(
+                        sizeof(double) * 2 *
+                        std::max(iterate_grid(pl.nS), std::max(staged_grid(pl.nS), dense_grid(pl.nS)))));
   OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
+  // max |p_S| of the starting values: the fixed-point scale of iteration 1
+  OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->ps0.as<double>(), pl.nS, &sc->itmax_bits[0], st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
   auto iterations = [&](hipStream_t s) -> hipError_t {
+    StagedIter si;
+    si.nS = pl.nS;
+    si.D = pl.D;
+    si.nst = pl.st_n.as<unsigned int>();
+    si.st_lo = pl.st_lo.as<unsigned int>();
+    si.st_hi = pl.st_hi.as<unsigned int>();
+    si.cse = pl.st_cse.as<unsigned int>();
+    si.ulines = pl.ulines.as<unsigned int>();
+    si.moff = pl.moff.as<unsigned long long>();
+    si.masks = pl.masks.as<unsigned long long>();
+    si.cell_eb = pl.cell_eb.as<unsigned int>();
+    si.off = pl.soff.as<unsigned short>();
     for (int k = 1; k <= P.max_iterations; k++) {
+      if (pl.dense && !pl.staged) {
+        const hipError_t e = launch_iterate_dense(
+            ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.dent.as<unsigned int>(),
+            pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
+            ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+            ctx->partials.as<double>(), P.max_iterations, P.tolerance, s);
+        if (e != hipSuccess) return e;
+        continue;
+      }
+      if (pl.staged) {
+        const hipError_t e = launch_iterate_staged(
+            si, k, pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
+            ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+            ctx->partials.as<double>(), P.max_iterations, P.tolerance, s);
+        if (e != hipSuccess) return e;
+        continue;
+      }
       const hipError_t e = launch_iterate(ctx->iter_waves, k, pl.nS, pl.moff.as<unsigned long long>(),
                                           pl.masks.as<unsigned long long>(),
                                           pl.cell_eb.as<unsigned int>(), pl.ent.as<int>(),
@@ -453,7 +595,10 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         (uint64_t)(uintptr_t)pl.moff.as<void>(), (uint64_t)(uintptr_t)pl.masks.as<void>(),
         (uint64_t)(uintptr_t)pl.cell_eb.as<void>(), (uint64_t)(uintptr_t)pl.ent.as<void>(),
         (uint64_t)(uintptr_t)pl.diag.as<void>(), (uint64_t)(uintptr_t)pl.slist.as<void>(),
-        (uint64_t)(uintptr_t)pl.cnt.as<void>(), (uint64_t)ctx->iter_waves};
+        (uint64_t)(uintptr_t)pl.cnt.as<void>(), (uint64_t)ctx->iter_waves, (uint64_t)pl.staged,
+        (uint64_t)(uintptr_t)pl.soff.as<void>(), (uint64_t)(uintptr_t)pl.ulines.as<void>(),
+        (uint64_t)(uintptr_t)pl.st_cse.as<void>(), (uint64_t)pl.dense,
+        (uint64_t)(uintptr_t)pl.dent.as<void>(), (uint64_t)(uintptr_t)pl.rowptr.as<void>(), (uint64_t)ctx->dense_variant};
     if (!ctx->it_exec || key != ctx->it_key) {
       drop_iteration_graph(ctx);
       if (!ctx->cap_stream) OMNI_TRY_HIP(ctx, hipStreamCreate(&ctx->cap_stream));
@@ -678,6 +823,10 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
   if (const char* e = std::getenv("OMNI3D_ITER_WAVES")) {
     const int w = std::atoi(e);
     if (w == 1 || w == 2 || w == 4 || w == -1 || w == -2 || w == -4) ctx->iter_waves = w;
+  }
+  if (const char* e = std::getenv("OMNI3D_DENSE_VARIANT")) {
+    const int v = std::atoi(e);
+    if (v >= 0 && v <= 4) ctx->dense_variant = v;
   }
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {

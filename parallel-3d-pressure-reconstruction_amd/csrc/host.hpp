@@ -81,7 +81,21 @@ struct Plan {
   DevBuf masks;    // uint64[ncell] lane mask of each (direction, k) slot
   DevBuf cell_eb;  // uint32[ncell] first packed entry of each slot
   DevBuf ent;      // int32[nnz] source ids packed in slot order
+  DevBuf sdir;     // uint16[ncell] direction of each slot
   int64_t ncell = 0;
+  // LDS-staged iteration (valid when staged): per (group, stage) line ranges,
+  // distinct source lines, per-chunk stage ends and 16-bit LDS positions
+  bool staged = false;
+  int64_t nlines = 0;  // distinct (group, direction, line) triples
+  DevBuf st_n;         // uint32[ngroups] stages of each group
+  DevBuf st_lo, st_hi; // uint32[ngroups*D] distinct-line range of each stage
+  DevBuf st_cse;       // uint32[ngroups*D*kStageGroup] end slot of each (stage, chunk)
+  DevBuf ulines;       // uint32[nlines] source line (16 sources) of each distinct triple
+  DevBuf soff;         // uint16[nnz+16] LDS position of each packed entry
+  // dense iteration layout (valid when dense): per chunk, entries sorted by
+  // source, word (row in chunk << 24 | source), in the CSR range of its rows
+  bool dense = false;
+  DevBuf dent;         // uint32[nnz+8]
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
   int ntiles = 0;
@@ -108,6 +122,7 @@ struct omni3d_ctx {
   int tile_bits = omni3d::kTileBitsDefault;  // tile edge 2^bits of the tile engine
   int tile_threads = 1024;                   // workgroup size of the tile passes
   int iter_waves = 2;                        // wavefronts per 64-row chunk of the iteration
+  int dense_variant = 0;                     // dense iteration: wavefronts x round size
   bool capturing = false;                    // work is being captured into a graph
   // graph of the source-iteration launches, reused while its arguments stay
   hipStream_t cap_stream = nullptr;

@@ -40,6 +40,7 @@ struct DevScalars {
   unsigned int ticket;           // last-block counter of the reductions
   unsigned int pad;
   unsigned long long steps;      // staircase steps of the last full pass
+  unsigned long long itmax_bits[2];  // max |p_S| after iteration k, in slot k & 1
 };
 
 // Method-specific walk parameters.

@@ -8,6 +8,7 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "stage.hpp"
 #include "walk.hpp"
 
 namespace omni3d {
@@ -199,6 +200,285 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
   }
 }
 
+// The same iteration over the LDS-staged layout: a workgroup of kG
+// wavefronts owns kG consecutive chunks.  Per stage, the workgroup copies the
+// stage's distinct source lines (16 doubles each) into LDS with 16-byte
+// loads, the next stage's lines already in flight in registers, and every
+// wavefront adds its chunk's entries of the stage's directions from LDS, in
+// slot order (the same fixed order on every run).
+template <int kG, int kCap>
+__global__ void __launch_bounds__(64 * kG)
+k_iterate_staged(StagedIter si, int k, const unsigned int* __restrict__ diag,
+                 const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
+                 const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
+                 const double* __restrict__ pold, double* __restrict__ pnew,
+                 const DevScalars* __restrict__ sc, double* __restrict__ partials) {
+  if (sc->converged) return;
+  constexpr int kU = 16;
+  constexpr int kOffStage = kU * 64 + 16;
+  constexpr int kThreads = 64 * kG;
+  constexpr int kPer = (kCap * 8 + kThreads - 1) / kThreads;
+  static_assert((kCap & (kCap - 1)) == 0, "stage capacity must be a power of two");
+  __shared__ __attribute__((aligned(16))) double s_line[kCap * 16];
+  __shared__ __attribute__((aligned(16))) unsigned short s_off[kG][kOffStage];
+  __shared__ double s_red[2][kG];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  unsigned short* __restrict__ so = s_off[wave];
+  int4* __restrict__ sl4 = reinterpret_cast<int4*>(s_line);
+  const int4* __restrict__ p4 = reinterpret_cast<const int4*>(pold);
+  const double inv = sc->inv_scale;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  double d2 = 0.0, n2 = 0.0;
+  const int nch = (si.nS + 63) >> 6;
+  const int ngroups = (nch + kG - 1) / kG;
+  for (int g = blockIdx.x; g < ngroups; g += gridDim.x) {
+    const int chunk = g * kG + wave;
+    const bool valid = chunk < nch;
+    unsigned long long s = valid ? si.moff[chunk] : 0ull;
+    const unsigned long long send = valid ? si.moff[chunk + 1] : 0ull;
+    unsigned long long eb = s < send ? (unsigned long long)si.cell_eb[s] : 0ull;
+    double acc = 0.0;
+    const size_t b = (size_t)g * si.D;
+    const unsigned int ns = si.nst[g];
+    int4 r[kPer];
+    unsigned int n4 = (si.st_hi[b] - si.st_lo[b]) * 8u;
+    {
+      const unsigned int lo = si.st_lo[b];
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const unsigned int i = tid + q * kThreads;
+        r[q] = i < n4 ? p4[(size_t)si.ulines[lo + (i >> 3)] * 8 + (i & 7)] : int4{0, 0, 0, 0};
+      }
+    }
+    for (unsigned int t = 0; t < ns; t++) {
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const unsigned int i = tid + q * kThreads;
+        if (i < n4) sl4[i] = r[q];
+      }
+      __syncthreads();
+      if (t + 1 < ns) {
+        // the next stage's lines travel while this stage is summed
+        const unsigned int lo = si.st_lo[b + t + 1];
+        n4 = (si.st_hi[b + t + 1] - lo) * 8u;
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+          const unsigned int i = tid + q * kThreads;
+          if (i < n4) r[q] = p4[(size_t)si.ulines[lo + (i >> 3)] * 8 + (i & 7)];
+        }
+      }
+      const unsigned long long se = valid ? (unsigned long long)si.cse[(b + t) * kG + wave] : 0ull;
+      for (; s < se; s += kU) {
+        unsigned long long m[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) m[u] = s + u < se ? si.masks[s + u] : 0ull;
+        unsigned int off[kU];
+        unsigned int n = 0;
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          off[u] = n;
+          n += (unsigned int)__popcll(m[u]);
+        }
+        // the round's LDS positions, staged with 16-byte loads
+        const unsigned long long a0 = eb & ~7ull;
+        const unsigned int sh = (unsigned int)(eb - a0);
+        const unsigned int n8 = (sh + n + 7) >> 3;
+        const int4* __restrict__ src4 = reinterpret_cast<const int4*>(si.off + a0);
+        for (unsigned int q = lane; q < n8; q += 64) reinterpret_cast<int4*>(so)[q] = src4[q];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double x[kU];
+        bool has[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+          has[u] = (m[u] >> lane) & 1ull;
+          const unsigned int rk = has[u] ? (unsigned int)__popcll(m[u] & below) : 0u;
+          // (lanes without an entry may read a stale position: kept in range)
+          x[u] = s_line[so[sh + off[u] + rk] & (kCap * 16 - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) acc += has[u] ? x[u] : 0.0;
+        eb += n;
+        __builtin_amdgcn_wave_barrier();
+      }
+      s = se;
+      __syncthreads();
+    }
+    const int row = chunk * 64 + lane;
+    if (valid && row < si.nS) {
+      const int v = slist[row];
+      const unsigned int c = cnt[v];
+      const double po = pold[row];
+      double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
+      if (k == 1 && seed && pinit) tot += pinit[v];
+      const double pn = c > 0 ? tot / (double)c : po;
+      pnew[row] = pn;
+      const double df = pn - po;
+      d2 += df * df;
+      n2 += pn * pn;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+  }
+  if (lane == 0) {
+    s_red[0][wave] = d2;
+    s_red[1][wave] = n2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, c = 0.0;
+    for (int w = 0; w < kG; w++) {
+      a += s_red[0][w];
+      c += s_red[1][w];
+    }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = c;
+  }
+}
+
+// The same iteration over the dense layout: each 64-row chunk's entries
+// sorted by source, one word (row in the chunk << 24 | source) each.  A
+// wavefront owns a chunk (kWaves chunks per workgroup) and its lanes take
+// rounds of kU * 64 consecutive entries, so one gather touches a few 128-byte
+// lines.  Row sums are int64 fixed-point sums in the wavefront's LDS with the
+// scale 2^(51 - e), 2^(e-1) <= max|p_S| < 2^e (every term below 2^51 in
+// magnitude, exact sums in any order: the result is the same on every run).
+// The iteration's own max |p_S| feeds the scale of the next one.
+template <int kWaves, int kU>
+__global__ void __launch_bounds__(64 * kWaves)
+k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
+                const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
+                const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
+                const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
+                const double* __restrict__ pold, double* __restrict__ pnew,
+                DevScalars* __restrict__ sc, double* __restrict__ partials) {
+  if (sc->converged) return;
+  static_assert(kU % 4 == 0, "rounds are whole 16-byte pieces per lane");
+  __shared__ unsigned long long s_acc[kWaves][64];
+  __shared__ __attribute__((aligned(16))) unsigned int s_ent[kWaves][kU * 64];
+  __shared__ double s_red[2][kWaves];
+  __shared__ unsigned long long s_max[kWaves];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned int* __restrict__ se = s_ent[wave];
+  unsigned long long* __restrict__ sa = s_acc[wave];
+  const double inv = sc->inv_scale;
+  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]);
+  const bool finite = B <= DBL_MAX;
+  int ex = 0;
+  (void)frexp(finite ? B : 1.0, &ex);
+  ex = ex < -960 ? -960 : ex;  // (a tiny bound: keep the scale finite)
+  const double scale = ldexp(1.0, 51 - ex), iscale = ldexp(1.0, ex - 51);
+  double d2 = 0.0, n2 = 0.0;
+  unsigned long long pm = 0ull;
+  const int nch = (nS + 63) >> 6;
+  for (int chunk = blockIdx.x * kWaves + wave; chunk < nch; chunk += gridDim.x * kWaves) {
+    sa[lane] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int r1 = chunk * 64 + 64 < nS ? chunk * 64 + 64 : nS;
+    const unsigned long long e0 = rowptr[chunk * 64], e1 = rowptr[r1];
+    // rounds of kU * 64 words from 4-aligned starts; each lane holds kU / 4
+    // 16-byte pieces of the next round in registers while the current round
+    // gathers
+    const int4* __restrict__ d4 = reinterpret_cast<const int4*>(dent);
+    unsigned long long base = e0 & ~3ull;
+    int4 pre[kU / 4];
+    {
+      const unsigned long long q0 = base >> 2, q1 = (e1 + 3) >> 2;
+#pragma unroll
+      for (int j = 0; j < kU / 4; j++) {
+        const unsigned long long q = q0 + (unsigned long long)(j * 64 + lane);
+        pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
+      }
+    }
+    for (; base < e1; base += kU * 64) {
+#pragma unroll
+      for (int j = 0; j < kU / 4; j++) reinterpret_cast<int4*>(se)[j * 64 + lane] = pre[j];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const unsigned long long nb = base + kU * 64;
+      if (nb < e1) {
+        const unsigned long long q0 = nb >> 2, q1 = (e1 + 3) >> 2;
+#pragma unroll
+        for (int j = 0; j < kU / 4; j++) {
+          const unsigned long long q = q0 + (unsigned long long)(j * 64 + lane);
+          if (q < q1) pre[j] = d4[q];
+        }
+      }
+      // words base + i with e0 <= base + i < e1 belong to this chunk
+      const unsigned long long lo = e0 > base ? e0 - base : 0ull;
+      const unsigned long long hi = e1 - base;
+      unsigned int w[kU];
+      double x[kU];
+      bool ok[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const unsigned int i = u * 64 + lane;
+        ok[u] = i >= lo && i < hi;
+        w[u] = ok[u] ? se[i] : se[lo];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) x[u] = pold[w[u] & 0xffffffu];
+#pragma unroll
+      for (int u = 0; u < kU; u++)
+        if (ok[u]) atomicAdd(&sa[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale));
+      __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int row = chunk * 64 + lane;
+    if (row < nS) {
+      double acc = (double)(long long)sa[lane] * iscale;
+      if (!finite) acc = B - B;
+      const int v = slist[row];
+      const unsigned int c = cnt[v];
+      const double po = pold[row];
+      double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
+      if (k == 1 && seed && pinit) tot += pinit[v];
+      const double pn = c > 0 ? tot / (double)c : po;
+      pnew[row] = pn;
+      const double df = pn - po;
+      d2 += df * df;
+      n2 += pn * pn;
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(pn));
+      pm = b > pm ? b : pm;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+    const unsigned long long o = __shfl_xor(pm, off);
+    pm = o > pm ? o : pm;
+  }
+  if (lane == 0) {
+    s_red[0][wave] = d2;
+    s_red[1][wave] = n2;
+    s_max[wave] = pm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    unsigned long long m = 0ull;
+    for (int q = 0; q < kWaves; q++) {
+      a += s_red[0][q];
+      c += s_red[1][q];
+      m = s_max[q] > m ? s_max[q] : m;
+    }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = c;
+    atomicMax(&sc->itmax_bits[k & 1], m);
+  }
+}
+
 // Relative change of iteration k from the workgroups' partial sums (fixed
 // order: a strided share per thread, then a tree), and the stop rule.
 constexpr int kFinishBlock = 1024;
@@ -236,6 +516,8 @@ k_iter_finish(int k, int nparts, const double* __restrict__ partials, DevScalars
   sc->dnum = num;
   sc->dden = den;
   sc->iter_done = k;
+  // the slot iteration k + 1 fills with its max |p_S| (iteration k read it)
+  sc->itmax_bits[(k + 1) & 1] = 0ull;
   if ((tol > 0.0 && rel < tol) || k >= max_iter) {
     sc->converged = 1;
     sc->kstar = k;
@@ -504,6 +786,57 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
     else OMNI_ITER(4, false);
   }
 #undef OMNI_ITER
+  hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
+                     max_iter, tol);
+  return hipGetLastError();
+}
+
+unsigned int dense_grid(int nS) {
+  // one chunk per wavefront, up to 4 wavefronts per workgroup
+  const int nch = (nS + 63) / 64;
+  return (unsigned int)std::max(1, std::min(nch, 65536));
+}
+
+hipError_t launch_iterate_dense(int variant, int k, int nS, const unsigned long long* rowptr,
+                                const unsigned int* dent, const unsigned int* diag,
+                                const int* slist, const unsigned int* cnt, const long long* bs,
+                                const double* pinit, int seed, const double* pold, double* pnew,
+                                DevScalars* sc, double* partials, int max_iter, double tol,
+                                hipStream_t st) {
+  unsigned int grid = dense_grid(nS);
+#define OMNI_DENSE(W, U)                                                                       \
+  hipLaunchKernelGGL((k_iterate_dense<W, U>), dim3(grid), dim3(64 * W), 0, st, k, nS, rowptr,    \
+                     dent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
+  const int nch = (nS + 63) / 64;
+  const int w = variant == 1 || variant == 2 ? 8 : 4;
+  grid = (unsigned int)std::max(1, std::min((nch + w - 1) / w, 65536));
+  switch (variant) {
+    case 1: OMNI_DENSE(8, 8); break;
+    case 2: OMNI_DENSE(8, 4); break;
+    case 3: OMNI_DENSE(4, 4); break;
+    case 4: OMNI_DENSE(4, 16); break;
+    default: OMNI_DENSE(4, 8); break;
+  }
+#undef OMNI_DENSE
+  hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
+                     max_iter, tol);
+  return hipGetLastError();
+}
+
+unsigned int staged_grid(int nS) {
+  const int nch = (nS + 63) / 64;
+  const int ng = (nch + kStageGroup - 1) / kStageGroup;
+  return (unsigned int)std::max(1, std::min(ng, 65536));
+}
+
+hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
+                                 const int* slist, const unsigned int* cnt, const long long* bs,
+                                 const double* pinit, int seed, const double* pold, double* pnew,
+                                 DevScalars* sc, double* partials, int max_iter, double tol,
+                                 hipStream_t st) {
+  const unsigned int grid = staged_grid(si.nS);
+  hipLaunchKernelGGL((k_iterate_staged<kStageGroup, kStageCap>), dim3(grid), dim3(64 * kStageGroup),
+                     0, st, si, k, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials);
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol);
   return hipGetLastError();

@@ -1,0 +1,85 @@
+// LDS-staged source iteration: plan construction (stage.hip) and launcher
+// (solve.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace omni3d {
+
+constexpr int kStageGroup = 4;   // 64-row chunks per workgroup (one wavefront each)
+constexpr int kStageCap = 256;   // 128-byte source lines per stage (32 KB of LDS)
+
+// Inputs and temporaries of the stage construction (device pointers).
+struct StageBuild {
+  int nS = 0, D = 0, nch = 0;
+  int64_t nnz = 0;
+  const unsigned long long* moff = nullptr;
+  const unsigned long long* masks = nullptr;
+  const unsigned short* sdir = nullptr;
+  const unsigned int* cell_eb = nullptr;
+  const int* ent = nullptr;
+  unsigned long long* keys_a = nullptr;
+  unsigned long long* keys_b = nullptr;
+  unsigned int* vals_a = nullptr;
+  unsigned int* vals_b = nullptr;
+  unsigned int* flag = nullptr;
+  unsigned int* incl = nullptr;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+int stage_key_bits(int nS, int D);
+size_t stage_tmp_bytes(int64_t nnz, int64_t ngd);
+// sorted (group, direction, line) keys and the inclusive count of distinct keys
+hipError_t stage_keys_sorted(const StageBuild& sb, hipStream_t st);
+hipError_t stage_finish(const StageBuild& sb, int ngroups, unsigned int* ulines, unsigned int* gdcnt,
+                        unsigned int* gdptr, unsigned int* nst, unsigned int* st_lo,
+                        unsigned int* st_hi, unsigned short* st_dend, unsigned int* sbase,
+                        unsigned int* cse, unsigned short* off, unsigned int* overflow,
+                        hipStream_t st);
+
+// One source iteration k over the staged layout, then the relative change
+// and stop rule (as launch_iterate).
+struct StagedIter {
+  int nS = 0, D = 0;
+  const unsigned int* nst = nullptr;
+  const unsigned int* st_lo = nullptr;
+  const unsigned int* st_hi = nullptr;
+  const unsigned int* cse = nullptr;
+  const unsigned int* ulines = nullptr;
+  const unsigned long long* moff = nullptr;
+  const unsigned long long* masks = nullptr;
+  const unsigned int* cell_eb = nullptr;
+  const unsigned short* off = nullptr;
+};
+unsigned int staged_grid(int nS);
+
+// Dense layout: per 64-row chunk, entries sorted by source, one word
+// (row in chunk << 24 | source) each; the chunk's words occupy the CSR range
+// of its rows.
+unsigned int dense_grid(int nS);
+hipError_t build_dense_entries(int nS, int64_t nnz, const unsigned long long* rowptr,
+                               const unsigned long long* csr_key, unsigned long long* keys_a,
+                               unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
+                               unsigned int* dent, hipStream_t st);
+size_t dense_tmp_bytes(int64_t nnz);
+// variant: chunks (one per wavefront) per workgroup x entries per round
+// (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256, 4: 4 x 1024)
+hipError_t launch_iterate_dense(int variant, int k, int nS, const unsigned long long* rowptr,
+                                const unsigned int* dent, const unsigned int* diag,
+                                const int* slist, const unsigned int* cnt, const long long* bs,
+                                const double* pinit, int seed, const double* pold, double* pnew,
+                                DevScalars* sc, double* partials, int max_iter, double tol,
+                                hipStream_t st);
+hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
+                                 const int* slist, const unsigned int* cnt, const long long* bs,
+                                 const double* pinit, int seed, const double* pold, double* pnew,
+                                 DevScalars* sc, double* partials, int max_iter, double tol,
+                                 hipStream_t st);
+
+}  // namespace omni3d

@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(64)
 k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
             const unsigned long long* __restrict__ key, const unsigned long long* __restrict__ moff,
             unsigned long long* __restrict__ masks, unsigned int* __restrict__ cell_eb,
-            int* __restrict__ ent) {
+            unsigned short* __restrict__ sdir, int* __restrict__ ent) {
   const int chunk = blockIdx.x, lane = threadIdx.x, row = chunk * 64 + lane;
   const unsigned long long rb = row < nS ? rowptr[row] : 0ull;
   const unsigned long long re = row < nS ? rowptr[row + 1] : 0ull;
@@ -351,6 +351,7 @@ k_sell_fill(int nS, int D, const unsigned long long* __restrict__ rowptr,
       if (lane == 0) {
         masks[mp] = m;
         cell_eb[mp] = (unsigned int)eb;
+        sdir[mp] = (unsigned short)d;
       }
       mp++;
       if (has) {
@@ -597,11 +598,11 @@ hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
 
 hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
                             const unsigned long long* key, const unsigned long long* moff,
-                            unsigned long long* masks, unsigned int* cell_eb, int* ent,
-                            hipStream_t st) {
+                            unsigned long long* masks, unsigned int* cell_eb,
+                            unsigned short* sdir, int* ent, hipStream_t st) {
   const unsigned int nch = (unsigned int)((nS + 63) / 64);
   hipLaunchKernelGGL(k_sell_fill, dim3(nch), dim3(64), 0, st, nS, D, rowptr, key, moff, masks,
-                     cell_eb, ent);
+                     cell_eb, sdir, ent);
   return hipGetLastError();
 }
 

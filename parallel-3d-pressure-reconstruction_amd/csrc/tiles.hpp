@@ -66,8 +66,8 @@ hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
                               const unsigned long long* key, unsigned int* nslot, hipStream_t st);
 hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
                             const unsigned long long* key, const unsigned long long* moff,
-                            unsigned long long* masks, unsigned int* cell_eb, int* ent,
-                            hipStream_t st);
+                            unsigned long long* masks, unsigned int* cell_eb,
+                            unsigned short* sdir, int* ent, hipStream_t st);
 hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, double* segval, hipStream_t st);
 // exclusive scan of n uint32 counts into n+1 uint64 offsets (multi-block);
