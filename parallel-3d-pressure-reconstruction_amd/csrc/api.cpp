@@ -402,8 +402,9 @@ int build_plan(omni3d_ctx* ctx) {
     // terms below 2^51)
     const char* env = std::getenv("OMNI3D_ITER");
     pl.dense = false;
-    if (!(env && std::strcmp(env, "slots") == 0) && nS < (1 << 24) && mxh[0] < 4096u &&
-        nnz > 0 && nnz < (1ull << 31)) {
+    const int grp = ctx->dense_group;
+    if (!(env && std::strcmp(env, "slots") == 0) && nS < (1 << (32 - dense_row_bits(grp))) &&
+        mxh[0] < 4096u && nnz > 0 && nnz < (1ull << 31)) {
       DevBuf ka, kb, tmp;
       OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
       OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * nnz));
@@ -411,13 +412,14 @@ int build_plan(omni3d_ctx* ctx) {
       // slack after the words: the iteration stages them with 16-byte loads
       OMNI_TRY_HIP(ctx, pl.dent.alloc(sizeof(unsigned int) * (nnz + 8)));
       OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.dent.as<void>(), 0, sizeof(unsigned int) * (nnz + 8), st));
-      OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, pl.rowptr.as<unsigned long long>(),
+      OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, grp, pl.rowptr.as<unsigned long long>(),
                                             key.as<unsigned long long>(),
                                             ka.as<unsigned long long>(), kb.as<unsigned long long>(),
                                             tmp.as<void>(), tmp.size(), pl.dent.as<unsigned int>(),
                                             st));
       OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
       pl.dense = true;
+      pl.dense_group = grp;
     }
     pl.staged = false;
     if (env && std::strcmp(env, "staged") == 0) {
@@ -552,7 +554,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     for (int k = 1; k <= P.max_iterations; k++) {
       if (pl.dense && !pl.staged) {
         const hipError_t e = launch_iterate_dense(
-            ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.dent.as<unsigned int>(),
+            pl.dense_group, ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.dent.as<unsigned int>(),
             pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
             ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
             ctx->partials.as<double>(), P.max_iterations, P.tolerance, s);
@@ -598,7 +600,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         (uint64_t)(uintptr_t)pl.cnt.as<void>(), (uint64_t)ctx->iter_waves, (uint64_t)pl.staged,
         (uint64_t)(uintptr_t)pl.soff.as<void>(), (uint64_t)(uintptr_t)pl.ulines.as<void>(),
         (uint64_t)(uintptr_t)pl.st_cse.as<void>(), (uint64_t)pl.dense,
-        (uint64_t)(uintptr_t)pl.dent.as<void>(), (uint64_t)(uintptr_t)pl.rowptr.as<void>(), (uint64_t)ctx->dense_variant};
+        (uint64_t)(uintptr_t)pl.dent.as<void>(), (uint64_t)(uintptr_t)pl.rowptr.as<void>(), (uint64_t)ctx->dense_variant,
+        (uint64_t)pl.dense_group};
     if (!ctx->it_exec || key != ctx->it_key) {
       drop_iteration_graph(ctx);
       if (!ctx->cap_stream) OMNI_TRY_HIP(ctx, hipStreamCreate(&ctx->cap_stream));
@@ -823,6 +826,10 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
   if (const char* e = std::getenv("OMNI3D_ITER_WAVES")) {
     const int w = std::atoi(e);
     if (w == 1 || w == 2 || w == 4 || w == -1 || w == -2 || w == -4) ctx->iter_waves = w;
+  }
+  if (const char* e = std::getenv("OMNI3D_DENSE_GROUP")) {
+    const int v = std::atoi(e);
+    if (v == 1 || v == 4 || v == 8 || v == 16) ctx->dense_group = v;
   }
   if (const char* e = std::getenv("OMNI3D_DENSE_VARIANT")) {
     const int v = std::atoi(e);

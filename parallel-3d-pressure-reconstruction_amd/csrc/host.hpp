@@ -95,6 +95,7 @@ struct Plan {
   // dense iteration layout (valid when dense): per chunk, entries sorted by
   // source, word (row in chunk << 24 | source), in the CSR range of its rows
   bool dense = false;
+  int dense_group = 1; // chunks per group of the words (1, 4, 8, 16)
   DevBuf dent;         // uint32[nnz+8]
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
@@ -123,6 +124,7 @@ struct omni3d_ctx {
   int tile_threads = 1024;                   // workgroup size of the tile passes
   int iter_waves = 2;                        // wavefronts per 64-row chunk of the iteration
   int dense_variant = 0;                     // dense iteration: wavefronts x round size
+  int dense_group = 4;                       // dense iteration: chunks per row group
   bool capturing = false;                    // work is being captured into a graph
   // graph of the source-iteration launches, reused while its arguments stay
   hipStream_t cap_stream = nullptr;

@@ -479,6 +479,137 @@ k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
   }
 }
 
+// The dense iteration over groups of kG chunks: a workgroup of kG
+// wavefronts owns a group (kG * 64 rows, the group's words sorted by source,
+// word (row in group << (32 - bits)) | source), its wavefronts take the
+// group's rounds of 512 words in turn and add into the group's LDS row sums;
+// each wavefront then finishes the 64 rows of its chunk.  Sources shared by
+// neighbouring chunks touch the same 128-byte lines in one gather.
+template <int kG>
+__global__ void __launch_bounds__(64 * kG)
+k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
+              const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
+              const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
+              const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
+              const double* __restrict__ pold, double* __restrict__ pnew,
+              DevScalars* __restrict__ sc, double* __restrict__ partials) {
+  if (sc->converged) return;
+  constexpr int kU = 8;
+  constexpr int kRows = 64 * kG;
+  constexpr int kRB = dense_row_bits(kG);
+  constexpr unsigned int kSrcMask = (1u << (32 - kRB)) - 1u;
+  static_assert((1 << kRB) >= kRows, "row field too narrow");
+  __shared__ unsigned long long s_acc[kRows];
+  __shared__ __attribute__((aligned(16))) unsigned int s_ent[kG][kU * 64];
+  __shared__ double s_red[2][kG];
+  __shared__ unsigned long long s_max[kG];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned int* __restrict__ se = s_ent[wave];
+  const int4* __restrict__ d4 = reinterpret_cast<const int4*>(dent);
+  const double inv = sc->inv_scale;
+  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]);
+  const bool finite = B <= DBL_MAX;
+  int ex = 0;
+  (void)frexp(finite ? B : 1.0, &ex);
+  ex = ex < -960 ? -960 : ex;  // (a tiny bound: keep the scale finite)
+  const double scale = ldexp(1.0, 51 - ex), iscale = ldexp(1.0, ex - 51);
+  double d2 = 0.0, n2 = 0.0;
+  unsigned long long pm = 0ull;
+  const int ngr = (nS + kRows - 1) / kRows;
+  for (int g = blockIdx.x; g < ngr; g += gridDim.x) {
+    s_acc[threadIdx.x] = 0ull;
+    __syncthreads();
+    const int rbeg = g * kRows;
+    const int rend = rbeg + kRows < nS ? rbeg + kRows : nS;
+    const unsigned long long e0 = rowptr[rbeg], e1 = rowptr[rend];
+    const unsigned long long q1 = (e1 + 3) >> 2;
+    unsigned long long base = (e0 & ~3ull) + (unsigned long long)wave * (kU * 64);
+    int4 pre[kU / 4];
+#pragma unroll
+    for (int j = 0; j < kU / 4; j++) {
+      const unsigned long long q = (base >> 2) + (unsigned long long)(j * 64 + lane);
+      pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
+    }
+    for (; base < e1; base += (unsigned long long)kG * (kU * 64)) {
+#pragma unroll
+      for (int j = 0; j < kU / 4; j++) reinterpret_cast<int4*>(se)[j * 64 + lane] = pre[j];
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const unsigned long long nb = base + (unsigned long long)kG * (kU * 64);
+      if (nb < e1) {
+#pragma unroll
+        for (int j = 0; j < kU / 4; j++) {
+          const unsigned long long q = (nb >> 2) + (unsigned long long)(j * 64 + lane);
+          if (q < q1) pre[j] = d4[q];
+        }
+      }
+      const unsigned long long lo = e0 > base ? e0 - base : 0ull;
+      const unsigned long long hi = e1 - base;
+      unsigned int w[kU];
+      double x[kU];
+      bool ok[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const unsigned int i = u * 64 + lane;
+        ok[u] = i >= lo && i < hi;
+        w[u] = ok[u] ? se[i] : se[lo];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) x[u] = pold[w[u] & kSrcMask];
+#pragma unroll
+      for (int u = 0; u < kU; u++)
+        if (ok[u])
+          atomicAdd(&s_acc[w[u] >> (32 - kRB)], (unsigned long long)to_fixed(x[u], scale));
+      __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    const int row = rbeg + wave * 64 + lane;
+    if (row < nS) {
+      double acc = (double)(long long)s_acc[wave * 64 + lane] * iscale;
+      if (!finite) acc = B - B;
+      const int v = slist[row];
+      const unsigned int c = cnt[v];
+      const double po = pold[row];
+      double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
+      if (k == 1 && seed && pinit) tot += pinit[v];
+      const double pn = c > 0 ? tot / (double)c : po;
+      pnew[row] = pn;
+      const double df = pn - po;
+      d2 += df * df;
+      n2 += pn * pn;
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(pn));
+      pm = b > pm ? b : pm;
+    }
+    __syncthreads();
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+    const unsigned long long o = __shfl_xor(pm, off);
+    pm = o > pm ? o : pm;
+  }
+  if (lane == 0) {
+    s_red[0][wave] = d2;
+    s_red[1][wave] = n2;
+    s_max[wave] = pm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    unsigned long long m = 0ull;
+    for (int q = 0; q < kG; q++) {
+      a += s_red[0][q];
+      c += s_red[1][q];
+      m = s_max[q] > m ? s_max[q] : m;
+    }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = c;
+    atomicMax(&sc->itmax_bits[k & 1], m);
+  }
+}
+
 // Relative change of iteration k from the workgroups' partial sums (fixed
 // order: a strided share per thread, then a tree), and the stop rule.
 constexpr int kFinishBlock = 1024;
@@ -797,13 +928,28 @@ unsigned int dense_grid(int nS) {
   return (unsigned int)std::max(1, std::min(nch, 65536));
 }
 
-hipError_t launch_iterate_dense(int variant, int k, int nS, const unsigned long long* rowptr,
+hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
+                                const unsigned long long* rowptr,
                                 const unsigned int* dent, const unsigned int* diag,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
                                 hipStream_t st) {
   unsigned int grid = dense_grid(nS);
+  if (gchunks > 1) {
+    const int rows = 64 * gchunks;
+    grid = (unsigned int)std::max(1, std::min((nS + rows - 1) / rows, 65536));
+#define OMNI_GRP(G)                                                                            \
+  hipLaunchKernelGGL((k_iterate_grp<G>), dim3(grid), dim3(64 * G), 0, st, k, nS, rowptr, dent,   \
+                     diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
+    if (gchunks == 4) OMNI_GRP(4);
+    else if (gchunks == 8) OMNI_GRP(8);
+    else OMNI_GRP(16);
+#undef OMNI_GRP
+    hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials,
+                       sc, max_iter, tol);
+    return hipGetLastError();
+  }
 #define OMNI_DENSE(W, U)                                                                       \
   hipLaunchKernelGGL((k_iterate_dense<W, U>), dim3(grid), dim3(64 * W), 0, st, k, nS, rowptr,    \
                      dent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)

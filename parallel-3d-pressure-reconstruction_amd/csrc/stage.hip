@@ -136,24 +136,28 @@ __global__ void k_stage_offsets(int64_t n, int lb, int db, int D,
 
 unsigned int grid1(int64_t n) { return (unsigned int)((n + 255) / 256); }
 
-// One lane per row: key (chunk, source, row in chunk) of each CSR entry
-// (CSR keys are (direction << 32) | source).
-__global__ void k_dense_keys(int nS, const unsigned long long* __restrict__ rowptr,
+// One lane per row: key (group, source, row in group) of each CSR entry
+// (CSR keys are (direction << 32) | source); a group is 2^gshift rows.
+__global__ void k_dense_keys(int nS, int gshift, int rb, const unsigned long long* __restrict__ rowptr,
                              const unsigned long long* __restrict__ ckey,
                              unsigned long long* __restrict__ keys) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nS) return;
-  const unsigned long long hi = ((unsigned long long)(r >> 6) << 32) | (unsigned long long)(r & 63);
+  const unsigned long long hi = ((unsigned long long)(r >> gshift) << 32) |
+                                (unsigned long long)(r & ((1 << gshift) - 1));
+  const unsigned long long smask = (1ull << (32 - rb)) - 1ull;
   for (unsigned long long j = rowptr[r]; j < rowptr[r + 1]; j++)
-    keys[j] = hi | ((ckey[j] & 0xffffffull) << 8);
+    keys[j] = hi | ((ckey[j] & smask) << rb);
 }
 
-__global__ void k_dense_words(int64_t n, const unsigned long long* __restrict__ keys,
+__global__ void k_dense_words(int64_t n, int rb, const unsigned long long* __restrict__ keys,
                               unsigned int* __restrict__ dent) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const unsigned long long k = keys[i];
-  dent[i] = ((unsigned int)(k & 63ull) << 24) | (unsigned int)((k >> 8) & 0xffffffull);
+  const unsigned int row = (unsigned int)(k & ((1ull << rb) - 1ull));
+  const unsigned int src = (unsigned int)((k >> rb) & ((1ull << (32 - rb)) - 1ull));
+  dent[i] = (row << (32 - rb)) | src;
 }
 
 }  // namespace
@@ -165,19 +169,22 @@ size_t dense_tmp_bytes(int64_t nnz) {
   return a + 256;
 }
 
-hipError_t build_dense_entries(int nS, int64_t nnz, const unsigned long long* rowptr,
+hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* csr_key, unsigned long long* keys_a,
                                unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
                                unsigned int* dent, hipStream_t st) {
-  hipLaunchKernelGGL(k_dense_keys, dim3(grid1(nS)), dim3(256), 0, st, nS, rowptr, csr_key, keys_a);
+  const int gshift = 6 + bits_for((unsigned long long)gchunks);
+  const int rb = dense_row_bits(gchunks);
+  hipLaunchKernelGGL(k_dense_keys, dim3(grid1(nS)), dim3(256), 0, st, nS, gshift, rb, rowptr,
+                     csr_key, keys_a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int nch = (nS + 63) / 64;
-  const int end_bit = 32 + bits_for((unsigned long long)nch);
+  const int ng = (nS + (1 << gshift) - 1) >> gshift;
+  const int end_bit = 32 + bits_for((unsigned long long)ng);
   size_t bytes = tmp_bytes;
   e = hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys_a, keys_b, (int)nnz, 0, end_bit, st);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_dense_words, dim3(grid1(nnz)), dim3(256), 0, st, nnz, keys_b, dent);
+  hipLaunchKernelGGL(k_dense_words, dim3(grid1(nnz)), dim3(256), 0, st, nnz, rb, keys_b, dent);
   return hipGetLastError();
 }
 

@@ -63,14 +63,23 @@ unsigned int staged_grid(int nS);
 // (row in chunk << 24 | source) each; the chunk's words occupy the CSR range
 // of its rows.
 unsigned int dense_grid(int nS);
-hipError_t build_dense_entries(int nS, int64_t nnz, const unsigned long long* rowptr,
+// Row bits of a dense word for groups of gchunks chunks: (row << (32 - bits)) | source.
+constexpr int dense_row_bits(int gchunks) {
+  return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10);
+}
+// gchunks: chunks per group sharing the LDS row sums (1, 4, 8 or 16); the
+// group's words are sorted by source.
+hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* csr_key, unsigned long long* keys_a,
                                unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
                                unsigned int* dent, hipStream_t st);
 size_t dense_tmp_bytes(int64_t nnz);
-// variant: chunks (one per wavefront) per workgroup x entries per round
-// (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256, 4: 4 x 1024)
-hipError_t launch_iterate_dense(int variant, int k, int nS, const unsigned long long* rowptr,
+// gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
+// entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
+// 4: 4 x 1024); gchunks 4, 8, 16: one group per workgroup of gchunks
+// wavefronts sharing the group's LDS row sums.
+hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
+                                const unsigned long long* rowptr,
                                 const unsigned int* dent, const unsigned int* diag,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
