@@ -137,6 +137,28 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 // ---------------------------------------------------------------------------
 // Tile passes.
 
+// LDS slot of in-tile voxel l = x + (y << b) + (z << 2b): x is XORed with
+// bits of y and z, so that voxels one step apart along y or z -- the
+// positions of neighbouring parallel rays at the same step -- fall in
+// different LDS banks (a bijection within every (y, z) row).
+template <int kBits>
+__device__ __forceinline__ int lds_slot(int l) {
+  return l ^ (((l >> (kBits + 1)) ^ (l >> (2 * kBits))) & ((1 << kBits) - 1));
+}
+
+// Trapezoid factor ((-rho * s) * delta_a) * 0.5 of a step along axis a with
+// sign s (up: s = +1) in the arithmetic of T.
+template <typename T>
+__device__ __forceinline__ T step_factor(const WalkParams& wp, int a, bool up);
+template <>
+__device__ __forceinline__ float step_factor<float>(const WalkParams& wp, int a, bool up) {
+  return up ? wp.hsf[a][1] : wp.hsf[a][0];
+}
+template <>
+__device__ __forceinline__ double step_factor<double>(const WalkParams& wp, int a, bool up) {
+  return up ? wp.hs[a][1] : wp.hs[a][0];
+}
+
 // LDS of a tile pass: the tile's gradient; the fixed-point counters (full
 // and source-sum passes); the source ids of the tile's voxels (source-sum
 // pass).  Exact sizes matter: two FP32 16^3 full-pass workgroups fill the
@@ -185,25 +207,63 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const bool in = i < g.X && j < g.Y && k < g.Z;
     if (in) v = field[i + j * sy + k * sz];
     else v.x = v.y = v.z = v.w = 0;
-    L.g[0][l] = v.x;
-    L.g[1][l] = v.y;
-    L.g[2][l] = v.z;
-    if constexpr (kMode != kLocalSum) L.acc[l] = 0;
+    const int sl = lds_slot<kBits>(l);
+    L.g[0][sl] = v.x;
+    L.g[1][sl] = v.y;
+    L.g[2][sl] = v.z;
+    if constexpr (kMode != kLocalSum) L.acc[sl] = 0;
     // source voxels (all on the box surface) by their compact id
-    if constexpr (kMode == kSourceSums) L.sidl[l] = in ? sid[i + j * sy + k * sz] : -1;
+    if constexpr (kMode == kSourceSums) L.sidl[sl] = in ? sid[i + j * sy + k * sz] : -1;
   }
   __syncthreads();
   const double scale = sc->scale;
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
   const unsigned long long b = tile_ptr[tile * kLenBuckets],
                            e = tile_ptr[(tile + 1) * kLenBuckets];
-  // the record of the next segment is loaded while this one is walked
-  SegRec nxt;
+  // software pipeline over a thread's segments: the record two segments
+  // ahead and the values the next segment starts from (its carry, entry
+  // term and source pressure; for kLocalSum the gradient just outside the
+  // tile) are in flight while this segment is walked
+  struct Pre {
+    double pin = 0.0, cv = 0.0;
+    T ev = T(0), prev = T(0);
+  };
+  const unsigned int bd = blockDim.x;
+  auto fetch = [&](const SegRec& r, unsigned long long jj) {
+    Pre q;
+    const bool fs = (r.info >> 21) & 1u;
+    if constexpr (kMode == kFull) q.pin = ps[r.src];
+    if constexpr (kMode != kLocalSum) {
+      if (!fs) {
+        q.cv = segval[r.g];
+        q.ev = eterm[jj];
+      }
+    } else {
+      if (!fs) {
+        const int lv0 = (int)(r.info & ((1u << kSegLvBits) - 1u));
+        const int a = (int)(r.code[0] & 3u);
+        const int sgn = ((r.info >> (18 + a)) & 1u) ? 1 : -1;
+        const int i = ox + (lv0 & kTileMask), jy = oy + ((lv0 >> kTileBits) & kTileMask),
+                  k = oz + (lv0 >> (2 * kTileBits));
+        const int64_t vp =
+            (int64_t)i + jy * sy + k * sz - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
+        q.prev = component(field[vp], a);
+      }
+    }
+    return q;
+  };
   unsigned long long j = b + threadIdx.x;
-  if (j < e) nxt = segs[j];
-  for (; j < e; j += blockDim.x) {
-    const SegRec sr = nxt;
-    if (j + blockDim.x < e) nxt = segs[j + blockDim.x];
+  SegRec r1, r2;
+  Pre p1;
+  if (j < e) r1 = segs[j];
+  if (j + bd < e) r2 = segs[j + bd];
+  if (j < e) p1 = fetch(r1, j);
+  for (; j < e; j += bd) {
+    const SegRec sr = r1;
+    const Pre pr = p1;
+    r1 = r2;
+    if (j + 2ull * bd < e) r2 = segs[j + 2ull * bd];
+    if (j + bd < e) p1 = fetch(r1, j + bd);
     // carry: the ray's sum before this segment plus the step into the tile
     // (kLocalSum computes that step, the only read outside the tile, and
     // stores it per segment in tile order)
@@ -211,60 +271,85 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const unsigned int n = (sr.info >> kSegLvBits) & 63u;
     const unsigned int sg = (sr.info >> 18) & 7u;
     const bool first = (sr.info >> 21) & 1u;
-    double pin = 0.0, carry = 0.0;
-    if (kMode == kFull) pin = ps[sr.src];
-    if (kMode != kLocalSum) carry = segval[sr.g];
-    RaySum<T> loc;
-    uint32_t c0 = sr.code[0], c1 = sr.code[1], c2 = sr.code[2];
-    auto next_code = [&]() {
-      const int a = (int)(c0 & 3u);
-      c0 = (c0 >> 2) | (c1 << 30);
-      c1 = (c1 >> 2) | (c2 << 30);
-      c2 >>= 2;
-      return a;
-    };
-    auto sign_of = [&](int a) { return ((sg >> a) & 1u) ? 1 : -1; };
-    auto add_lds = [&](int l, double x) {
-      if constexpr (kMode != kLocalSum)
-        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]),
-                  (unsigned long long)to_fixed(x, scale));
+    // the segment's step signs are fixed: per axis, the LDS step and the
+    // trapezoid factor of a step along it
+    const bool up0 = sg & 1u, up1 = (sg >> 1) & 1u, up2 = (sg >> 2) & 1u;
+    const int d0 = up0 ? 1 : -1;
+    const int d1 = up1 ? kTile : -kTile;
+    const int d2 = up2 ? kTile * kTile : -kTile * kTile;
+    const T h0 = step_factor<T>(wp, 0, up0);
+    const T h1 = step_factor<T>(wp, 1, up1);
+    const T h2 = step_factor<T>(wp, 2, up2);
+    // in-tile sums accumulate in FP64: kLocalSum the segment's sum; the
+    // other passes (base + sum so far) * scale, the fixed-point value up to
+    // the final rounding (the scale is a power of two: every product by it
+    // is exact)
+    double loc = 0.0, accs = 0.0;
+    auto add_fixed = [&](int l) {
+      if constexpr (kMode != kLocalSum) {
+        const long long f =
+            __double_as_longlong(accs + 6755399441055744.0) - 0x4338000000000000LL;
+        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]), (unsigned long long)f);
+      }
     };
     auto on_source = [&](int l) {
       if constexpr (kMode == kSourceSums) return L.sidl[l] >= 0;
       else return false;
     };
+    uint32_t cw = sr.code[0];
+    int inw = 16;  // codes left in the current word
     if (first) {
       // the entry visit: self-add p_src (pint = 0)
-      if (kMode == kFull) add_lds(lv, pin);
-    } else if (kMode == kLocalSum) {
-      // step into the tile: the previous voxel lies outside it
-      const int a = next_code();
-      const int sgn = sign_of(a);
-      const int i = ox + (lv & kTileMask), jj = oy + ((lv >> kTileBits) & kTileMask),
-                k = oz + (lv >> (2 * kTileBits));
-      const int64_t vp = (int64_t)i + jj * sy + k * sz - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
-      const T prev = component(field[vp], a);
-      const T et = trapezoid<T>(wp, a, sgn, L.g[a][lv], prev);
-      eterm[j] = et;
-      loc.add(et);
+      if constexpr (kMode == kFull) {
+        accs = pr.pin * scale;
+        add_fixed(lds_slot<kBits>(lv));
+      }
     } else {
-      (void)next_code();
-      carry += (double)eterm[j];
-      if (kMode == kFull) add_lds(lv, pin + carry);
-      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, carry);
+      const int a = (int)(cw & 3u);
+      cw >>= 2;
+      inw = 15;
+      if constexpr (kMode == kLocalSum) {
+        // step into the tile: the previous voxel (pr.prev) lies outside it
+        const int sgn = ((sg >> a) & 1u) ? 1 : -1;
+        const T et = trapezoid<T>(wp, a, sgn, L.g[a][lds_slot<kBits>(lv)], pr.prev);
+        eterm[j] = et;
+        loc = (double)et;
+      } else {
+        const double carry = pr.cv + (double)pr.ev;
+        accs = (kMode == kFull ? pr.pin + carry : carry) * scale;
+        const int slv = lds_slot<kBits>(lv);
+        if (kMode == kFull || on_source(slv)) add_fixed(slv);
+      }
     }
     // every later visit adds base + (in-tile sum so far)
-    const double base = kMode == kFull ? pin + carry : carry;
-    for (unsigned int vis = 1; vis < n; vis++) {
-      const int a = next_code();
-      const int sgn = sign_of(a);
-      const int ln = lv + sgn * (a == 0 ? 1 : (a == 1 ? kTile : kTile * kTile));
-      loc.add(trapezoid<T>(wp, a, sgn, L.g[a][ln], L.g[a][lv]));
-      lv = ln;
-      if (kMode == kFull) add_lds(lv, base + loc.value());
-      if (kMode == kSourceSums && on_source(lv)) add_lds(lv, base + loc.value());
+    int rem = (int)n - 1;
+    int slv = lds_slot<kBits>(lv);
+#pragma unroll 1
+    for (int wi = 0; wi < 3 && rem > 0; wi++) {
+      const int m = rem < inw ? rem : inw;
+      for (int q = 0; q < m; q++) {
+        const int a = (int)(cw & 3u);
+        cw >>= 2;
+        const bool a0 = a == 0, a1 = a == 1;
+        const int ln = lv + (a0 ? d0 : (a1 ? d1 : d2));
+        const T h = a0 ? h0 : (a1 ? h1 : h2);
+        const T* __restrict__ gp = &L.g[0][0] + a * kTileVox;
+        const int sln = lds_slot<kBits>(ln);
+        const T t = h * (gp[sln] + gp[slv]);
+        lv = ln;
+        slv = sln;
+        if constexpr (kMode == kLocalSum) {
+          loc += (double)t;
+        } else {
+          accs = fma((double)t, scale, accs);
+          if (kMode == kFull || on_source(slv)) add_fixed(slv);
+        }
+      }
+      rem -= m;
+      cw = wi == 0 ? sr.code[1] : sr.code[2];
+      inw = 16;
     }
-    if (kMode == kLocalSum) segval[sr.g] = loc.value();
+    if (kMode == kLocalSum) segval[sr.g] = loc;
   }
   if (kMode == kLocalSum) return;
   __syncthreads();
@@ -273,10 +358,11 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
               k = oz + (l >> (2 * kTileBits));
     if (i >= g.X || j >= g.Y || k >= g.Z) continue;
     const int64_t v = i + j * sy + k * sz;
-    if constexpr (kMode == kFull) out[v] = L.acc[l];
+    const int sl = lds_slot<kBits>(l);
+    if constexpr (kMode == kFull) out[v] = L.acc[sl];
     if constexpr (kMode == kSourceSums) {
-      const int s = L.sidl[l];
-      if (s >= 0) out[s] = L.acc[l];
+      const int s = L.sidl[sl];
+      if (s >= 0) out[s] = L.acc[sl];
     }
   }
 }
