@@ -184,6 +184,8 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   hipStream_t st = ctx->stream;
   TileGrid tg;
   tg.bits = ctx->tile_bits;
+  tg.lshift = 3;  // segment-length groups of 8 visits
+  if (const char* e = std::getenv("OMNI3D_LEN_SHIFT")) tg.lshift = std::max(0, std::min(6, std::atoi(e)));
   const int kTile = 1 << tg.bits;
   tg.ntx = (g.X + kTile - 1) / kTile;
   tg.nty = (g.Y + kTile - 1) / kTile;

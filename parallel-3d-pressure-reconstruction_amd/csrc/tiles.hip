@@ -35,8 +35,8 @@ __device__ __forceinline__ int tile_of(int i, int j, int k, const TileGrid& tg) 
 // Segments of a tile are grouped by length, longest first, so that the lanes
 // of a wavefront walk segments of similar length: slot list index
 // tile * kLenBuckets + bucket.
-__device__ __forceinline__ int len_slot(int tile, unsigned int nvis) {
-  const unsigned int b = nvis >> 2;
+__device__ __forceinline__ int len_slot(int tile, unsigned int nvis, int lshift) {
+  const unsigned int b = nvis >> lshift;
   return tile * kLenBuckets + (kLenBuckets - 1) - (int)(b < kLenBuckets ? b : kLenBuckets - 1);
 }
 
@@ -55,14 +55,14 @@ k_tile_count(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
     if (w.advance(&s) < 0) break;
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
     if (t != tile) {
-      atomicAdd(&tile_cnt[len_slot(tile, nvis)], 1u);
+      atomicAdd(&tile_cnt[len_slot(tile, nvis, tg.lshift)], 1u);
       tile = t;
       segs++;
       nvis = 0;
     }
     nvis++;
   }
-  atomicAdd(&tile_cnt[len_slot(tile, nvis)], 1u);
+  atomicAdd(&tile_cnt[len_slot(tile, nvis, tg.lshift)], 1u);
   nseg[r] = segs;
 }
 
@@ -101,7 +101,7 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   };
   auto emit = [&]() {
     sr.info |= nvis << kSegLvBits;
-    const int ls = len_slot(tile, nvis);
+    const int ls = len_slot(tile, nvis, tg.lshift);
     const unsigned int slot = atomicAdd(&cursor[ls], 1u);
     segs[tile_ptr[ls] + slot] = sr;
     gseg += 64;
@@ -163,17 +163,20 @@ __device__ __forceinline__ double step_factor<double>(const WalkParams& wp, int 
 // and source-sum passes); the source ids of the tile's voxels (source-sum
 // pass).  Exact sizes matter: two FP32 16^3 full-pass workgroups fill the
 // 160 KB of a CU.
-template <typename T, int kVox, int kMode>
+// The segment-sum pass keeps a one-voxel halo ((edge + 2)^3 voxels, x
+// fastest) instead: the step into the tile then reads the voxel outside it
+// from LDS.
+template <typename T, int kVox, int kHaloVox, int kMode>
 struct TileLds {
   T g[3][kVox];
   long long acc[kVox];
 };
-template <typename T, int kVox>
-struct TileLds<T, kVox, kLocalSum> {
-  T g[3][kVox];
+template <typename T, int kVox, int kHaloVox>
+struct TileLds<T, kVox, kHaloVox, kLocalSum> {
+  T g[3][kHaloVox];
 };
-template <typename T, int kVox>
-struct TileLds<T, kVox, kSourceSums> {
+template <typename T, int kVox, int kHaloVox>
+struct TileLds<T, kVox, kHaloVox, kSourceSums> {
   T g[3][kVox];
   long long acc[kVox];
   int sidl[kVox];
@@ -192,7 +195,11 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
   constexpr int kTileVox = kTile * kTile * kTile;
-  __shared__ TileLds<T, kTileVox, kMode> L;
+  constexpr bool kHalo = kMode == kLocalSum;
+  constexpr int kW = kTile + 2;  // halo edge
+  constexpr int kHaloVox = kW * kW * kW;
+  constexpr int kPlane = kHalo ? kHaloVox : kTileVox;
+  __shared__ TileLds<T, kTileVox, kHaloVox, kMode> L;
   // contiguous tile ranges per XCD: neighbouring tiles share an L2
   const int tb = (int)xcd_block();
   const int tile = tile_list ? tile_list[tb] : tb;
@@ -200,20 +207,34 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   const int ox = tx << kTileBits, oy = ty << kTileBits, oz = tz << kTileBits;
   const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
   // stage the tile's gradient (zero outside the grid) and clear the counters
-  for (int l = threadIdx.x; l < kTileVox; l += blockDim.x) {
+  if constexpr (kHalo) {
+    for (int l = threadIdx.x; l < kHaloVox; l += blockDim.x) {
+      const int i = ox + l % kW - 1, j = oy + (l / kW) % kW - 1, k = oz + l / (kW * kW) - 1;
+      V4 v;
+      const bool in = i >= 0 && j >= 0 && k >= 0 && i < g.X && j < g.Y && k < g.Z;
+      if (in) v = field[i + j * sy + k * sz];
+      else v.x = v.y = v.z = v.w = 0;
+      L.g[0][l] = v.x;
+      L.g[1][l] = v.y;
+      L.g[2][l] = v.z;
+    }
+  }
+  for (int l = threadIdx.x; l < (kHalo ? 0 : kTileVox); l += blockDim.x) {
     const int i = ox + (l & kTileMask), j = oy + ((l >> kTileBits) & kTileMask),
               k = oz + (l >> (2 * kTileBits));
     V4 v;
     const bool in = i < g.X && j < g.Y && k < g.Z;
     if (in) v = field[i + j * sy + k * sz];
     else v.x = v.y = v.z = v.w = 0;
-    const int sl = lds_slot<kBits>(l);
-    L.g[0][sl] = v.x;
-    L.g[1][sl] = v.y;
-    L.g[2][sl] = v.z;
-    if constexpr (kMode != kLocalSum) L.acc[sl] = 0;
-    // source voxels (all on the box surface) by their compact id
-    if constexpr (kMode == kSourceSums) L.sidl[sl] = in ? sid[i + j * sy + k * sz] : -1;
+    if constexpr (!kHalo) {
+      const int sl = lds_slot<kBits>(l);
+      L.g[0][sl] = v.x;
+      L.g[1][sl] = v.y;
+      L.g[2][sl] = v.z;
+      L.acc[sl] = 0;
+      // source voxels (all on the box surface) by their compact id
+      if constexpr (kMode == kSourceSums) L.sidl[sl] = in ? sid[i + j * sy + k * sz] : -1;
+    }
   }
   __syncthreads();
   const double scale = sc->scale;
@@ -222,11 +243,10 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
                            e = tile_ptr[(tile + 1) * kLenBuckets];
   // software pipeline over a thread's segments: the record two segments
   // ahead and the values the next segment starts from (its carry, entry
-  // term and source pressure; for kLocalSum the gradient just outside the
-  // tile) are in flight while this segment is walked
+  // term and source pressure) are in flight while this segment is walked
   struct Pre {
     double pin = 0.0, cv = 0.0;
-    T ev = T(0), prev = T(0);
+    T ev = T(0);
   };
   const unsigned int bd = blockDim.x;
   auto fetch = [&](const SegRec& r, unsigned long long jj) {
@@ -237,17 +257,6 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
       if (!fs) {
         q.cv = segval[r.g];
         q.ev = eterm[jj];
-      }
-    } else {
-      if (!fs) {
-        const int lv0 = (int)(r.info & ((1u << kSegLvBits) - 1u));
-        const int a = (int)(r.code[0] & 3u);
-        const int sgn = ((r.info >> (18 + a)) & 1u) ? 1 : -1;
-        const int i = ox + (lv0 & kTileMask), jy = oy + ((lv0 >> kTileBits) & kTileMask),
-                  k = oz + (lv0 >> (2 * kTileBits));
-        const int64_t vp =
-            (int64_t)i + jy * sy + k * sz - sgn * (a == 0 ? 1 : (a == 1 ? sy : sz));
-        q.prev = component(field[vp], a);
       }
     }
     return q;
@@ -274,9 +283,19 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     // the segment's step signs are fixed: per axis, the LDS step and the
     // trapezoid factor of a step along it
     const bool up0 = sg & 1u, up1 = (sg >> 1) & 1u, up2 = (sg >> 2) & 1u;
+    constexpr int kSy = kHalo ? kW : kTile;  // LDS strides of the walked index
     const int d0 = up0 ? 1 : -1;
-    const int d1 = up1 ? kTile : -kTile;
-    const int d2 = up2 ? kTile * kTile : -kTile * kTile;
+    const int d1 = up1 ? kSy : -kSy;
+    const int d2 = up2 ? kSy * kSy : -kSy * kSy;
+    // the walked index: the halo index (kLocalSum) or the in-tile index,
+    // whose LDS slot is swizzled
+    if constexpr (kHalo)
+      lv = (lv & kTileMask) + 1 + (((lv >> kTileBits) & kTileMask) + 1) * kW +
+           ((lv >> (2 * kTileBits)) + 1) * kW * kW;
+    auto slot = [&](int l) {
+      if constexpr (kHalo) return l;
+      else return lds_slot<kBits>(l);
+    };
     const T h0 = step_factor<T>(wp, 0, up0);
     const T h1 = step_factor<T>(wp, 1, up1);
     const T h2 = step_factor<T>(wp, 2, up2);
@@ -302,28 +321,29 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
       // the entry visit: self-add p_src (pint = 0)
       if constexpr (kMode == kFull) {
         accs = pr.pin * scale;
-        add_fixed(lds_slot<kBits>(lv));
+        add_fixed(slot(lv));
       }
     } else {
       const int a = (int)(cw & 3u);
       cw >>= 2;
       inw = 15;
       if constexpr (kMode == kLocalSum) {
-        // step into the tile: the previous voxel (pr.prev) lies outside it
+        // step into the tile: the previous voxel lies in the halo
         const int sgn = ((sg >> a) & 1u) ? 1 : -1;
-        const T et = trapezoid<T>(wp, a, sgn, L.g[a][lds_slot<kBits>(lv)], pr.prev);
+        const int dprev = a == 0 ? d0 : (a == 1 ? d1 : d2);
+        const T et = trapezoid<T>(wp, a, sgn, L.g[a][lv], L.g[a][lv - dprev]);
         eterm[j] = et;
         loc = (double)et;
       } else {
         const double carry = pr.cv + (double)pr.ev;
         accs = (kMode == kFull ? pr.pin + carry : carry) * scale;
-        const int slv = lds_slot<kBits>(lv);
+        const int slv = slot(lv);
         if (kMode == kFull || on_source(slv)) add_fixed(slv);
       }
     }
     // every later visit adds base + (in-tile sum so far)
     int rem = (int)n - 1;
-    int slv = lds_slot<kBits>(lv);
+    int slv = slot(lv);
 #pragma unroll 1
     for (int wi = 0; wi < 3 && rem > 0; wi++) {
       const int m = rem < inw ? rem : inw;
@@ -333,8 +353,8 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
         const bool a0 = a == 0, a1 = a == 1;
         const int ln = lv + (a0 ? d0 : (a1 ? d1 : d2));
         const T h = a0 ? h0 : (a1 ? h1 : h2);
-        const T* __restrict__ gp = &L.g[0][0] + a * kTileVox;
-        const int sln = lds_slot<kBits>(ln);
+        const T* __restrict__ gp = &L.g[0][0] + a * kPlane;
+        const int sln = slot(ln);
         const T t = h * (gp[sln] + gp[slv]);
         lv = ln;
         slv = sln;

@@ -17,11 +17,12 @@ constexpr int kTileBitsMax = 4;
 constexpr int kTileBitsDefault = 4;
 
 enum { kLocalSum = 0, kSourceSums = 1, kFull = 2 };
-constexpr int kLenBuckets = 12;  // segment-length groups per tile (4 visits each)
+constexpr int kLenBuckets = 12;  // segment-length groups per tile (2^lshift visits each)
 
 struct TileGrid {
   int ntx, nty, ntz;
   int bits;
+  int lshift = 2;  // segment-length group of a segment: visits >> lshift
 };
 
 // One segment: the visits of one ray inside one tile, self-contained so that
