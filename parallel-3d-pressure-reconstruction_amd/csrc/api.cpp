@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "driver.hpp"
@@ -540,6 +541,15 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   // max |p_S| of the starting values: the fixed-point scale of iteration 1
   OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->ps0.as<double>(), pl.nS, &sc->itmax_bits[0], st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
+  // FP32 ray sums: the grouped dense iteration runs in increment form
+  const bool inc = std::is_same<T, float>::value && pl.dense && pl.dense_group > 1 &&
+                   !std::getenv("OMNI3D_NO_INCREMENTS");
+  if (inc) {
+    OMNI_TRY_HIP(ctx, ctx->srow.alloc(sizeof(double) * ns));
+    OMNI_TRY_HIP(ctx, ctx->dinc0.alloc(sizeof(float) * ns16));
+    OMNI_TRY_HIP(ctx, ctx->dinc1.alloc(sizeof(float) * ns16));
+  }
+  float* dbuf[2] = {ctx->dinc0.as<float>(), ctx->dinc1.as<float>()};
   auto iterations = [&](hipStream_t s) -> hipError_t {
     StagedIter si;
     si.nS = pl.nS;
@@ -559,7 +569,9 @@ int run_affine(omni3d_ctx* ctx, int seed) {
             pl.dense_group, ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.dent.as<unsigned int>(),
             pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
             ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
-            ctx->partials.as<double>(), P.max_iterations, P.tolerance, s);
+            ctx->partials.as<double>(), P.max_iterations, P.tolerance, s,
+            inc ? ctx->srow.as<double>() : nullptr, inc ? dbuf[(k - 1) & 1] : nullptr,
+            inc ? dbuf[k & 1] : nullptr);
         if (e != hipSuccess) return e;
         continue;
       }
@@ -603,7 +615,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         (uint64_t)(uintptr_t)pl.soff.as<void>(), (uint64_t)(uintptr_t)pl.ulines.as<void>(),
         (uint64_t)(uintptr_t)pl.st_cse.as<void>(), (uint64_t)pl.dense,
         (uint64_t)(uintptr_t)pl.dent.as<void>(), (uint64_t)(uintptr_t)pl.rowptr.as<void>(), (uint64_t)ctx->dense_variant,
-        (uint64_t)pl.dense_group};
+        (uint64_t)pl.dense_group, (uint64_t)inc, (uint64_t)(uintptr_t)ctx->srow.as<void>(),
+        (uint64_t)(uintptr_t)dbuf[0], (uint64_t)(uintptr_t)dbuf[1]};
     if (!ctx->it_exec || key != ctx->it_key) {
       drop_iteration_graph(ctx);
       if (!ctx->cap_stream) OMNI_TRY_HIP(ctx, hipStreamCreate(&ctx->cap_stream));

@@ -140,6 +140,9 @@ struct omni3d_ctx {
   omni3d::DevBuf pinit;     // double[V]
   omni3d::DevBuf bs;        // int64[nS]
   omni3d::DevBuf ps0, ps1;  // double[nS]
+  // increment form of the iteration (FP32 ray sums): row sums of p_S and the
+  // FP32 increments of two consecutive iterations
+  omni3d::DevBuf srow, dinc0, dinc1;
   omni3d::DevBuf partials;  // double[2*grid]
   omni3d::DevBuf sum;       // int64[ncopies*V]
   omni3d::DevBuf segval;    // double[nval] segment sums, then carries

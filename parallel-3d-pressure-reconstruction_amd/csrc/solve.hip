@@ -485,13 +485,21 @@ k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
 // group's rounds of 512 words in turn and add into the group's LDS row sums;
 // each wavefront then finishes the 64 rows of its chunk.  Sources shared by
 // neighbouring chunks touch the same 128-byte lines in one gather.
-template <int kG>
+//
+// kInc selects the increment form (FP32 ray-sum precision): a row keeps
+// S = sum of its entries' p_S in FP64 across iterations.  kInc == 1 (the
+// first iteration) gathers p_S, stores S and the FP32 increment dnew = p_new
+// - p_old; kInc == 2 gathers the previous increments din (4 bytes per entry)
+// and adds their sum to S.  The fixed-point scale then follows max |increment|
+// instead of max |p_S|, so the FP32 rounding applies to increments only.
+template <int kG, int kInc>
 __global__ void __launch_bounds__(64 * kG)
 k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
               const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
               const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
               const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
               const double* __restrict__ pold, double* __restrict__ pnew,
+              double* __restrict__ srow, const float* __restrict__ din, float* __restrict__ dnew,
               DevScalars* __restrict__ sc, double* __restrict__ partials) {
   if (sc->converged) return;
   constexpr int kU = 8;
@@ -557,7 +565,10 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
         w[u] = ok[u] ? se[i] : se[lo];
       }
 #pragma unroll
-      for (int u = 0; u < kU; u++) x[u] = pold[w[u] & kSrcMask];
+      for (int u = 0; u < kU; u++) {
+        if constexpr (kInc == 2) x[u] = (double)din[w[u] & kSrcMask];
+        else x[u] = pold[w[u] & kSrcMask];
+      }
 #pragma unroll
       for (int u = 0; u < kU; u++)
         if (ok[u])
@@ -569,6 +580,8 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
     if (row < nS) {
       double acc = (double)(long long)s_acc[wave * 64 + lane] * iscale;
       if (!finite) acc = B - B;
+      if constexpr (kInc == 2) acc += srow[row];
+      if constexpr (kInc != 0) srow[row] = acc;
       const int v = slist[row];
       const unsigned int c = cnt[v];
       const double po = pold[row];
@@ -579,7 +592,13 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
       const double df = pn - po;
       d2 += df * df;
       n2 += pn * pn;
-      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(pn));
+      double mv = pn;
+      if constexpr (kInc != 0) {
+        const float dq = (float)df;
+        dnew[row] = dq;
+        mv = (double)dq;
+      }
+      const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(mv));
       pm = b > pm ? b : pm;
     }
     __syncthreads();
@@ -934,17 +953,28 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
-                                hipStream_t st) {
+                                hipStream_t st, double* srow, const float* din, float* dnew) {
   unsigned int grid = dense_grid(nS);
   if (gchunks > 1) {
     const int rows = 64 * gchunks;
     grid = (unsigned int)std::max(1, std::min((nS + rows - 1) / rows, 65536));
-#define OMNI_GRP(G)                                                                            \
-  hipLaunchKernelGGL((k_iterate_grp<G>), dim3(grid), dim3(64 * G), 0, st, k, nS, rowptr, dent,   \
-                     diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
-    if (gchunks == 4) OMNI_GRP(4);
-    else if (gchunks == 8) OMNI_GRP(8);
-    else OMNI_GRP(16);
+    const int inc = srow ? (k == 1 ? 1 : 2) : 0;
+#define OMNI_GRP(G, I)                                                                         \
+  hipLaunchKernelGGL((k_iterate_grp<G, I>), dim3(grid), dim3(64 * G), 0, st, k, nS, rowptr,      \
+                     dent, diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc,  \
+                     partials)
+#define OMNI_GRPS(G)                 \
+  if (inc == 1) OMNI_GRP(G, 1);      \
+  else if (inc == 2) OMNI_GRP(G, 2); \
+  else OMNI_GRP(G, 0);
+    if (gchunks == 4) {
+      OMNI_GRPS(4)
+    } else if (gchunks == 8) {
+      OMNI_GRPS(8)
+    } else {
+      OMNI_GRPS(16)
+    }
+#undef OMNI_GRPS
 #undef OMNI_GRP
     hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials,
                        sc, max_iter, tol);

@@ -77,14 +77,17 @@ size_t dense_tmp_bytes(int64_t nnz);
 // gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
 // entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
 // 4: 4 x 1024); gchunks 4, 8, 16: one group per workgroup of gchunks
-// wavefronts sharing the group's LDS row sums.
+// wavefronts sharing the group's LDS row sums.  With srow (groups only) the
+// increment form: srow[nS] FP64 row sums of p_S kept across iterations,
+// din / dnew the FP32 increments of the previous / this iteration.
 hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 const unsigned long long* rowptr,
                                 const unsigned int* dent, const unsigned int* diag,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
-                                hipStream_t st);
+                                hipStream_t st, double* srow = nullptr,
+                                const float* din = nullptr, float* dnew = nullptr);
 hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
                                  const int* slist, const unsigned int* cnt, const long long* bs,
                                  const double* pinit, int seed, const double* pold, double* pnew,
