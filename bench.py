@@ -15,7 +15,8 @@ Also measured in the same run: the FP64 GPU path on the same input (rel-L2
 of the FP32 result against it, speed ratio), the cost of one reference-
 schedule FP64 pass (a full traversal per iteration, as the reference
 iterates), the per-phase times from the library's HIP events on its stream,
-the roofline of the dominant kernel (the final full traversal), and the
+the roofline of the dominant traversal kernel (the slower of pass A's
+segment-sum traversal and the final full traversal), and the
 oracle's single-thread CPU time on a bounded sample (cpu_baseline).
 
 Other modes (not the headline): --copy-calibration runs a plain 1 GiB device
@@ -36,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "parallel-3d-pressure-reconstruction_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tile_pass_full_burgers256.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tile_passes_burgers256.json")
 
 
 def log(msg):
@@ -259,21 +260,33 @@ def main():
     p32h = p32.cpu().numpy()
     err_analytic = F.rel_l2(F.demean(p32h), F.demean(pa))
 
-    # roofline of the final full traversal (k_tile_pass, full mode): it must
+    # roofline of the dominant traversal kernel: the slower of the segment-sum
+    # traversal of pass A (k_tile_pass, local-sum mode) and the final full
+    # traversal (k_tile_pass, full mode), each one ray pass.  A ray pass must
     # stream the gradient once per direction, D*V*3*4 B in FP32, and read and
-    # write the FP64 pressure and FP32 weight once, V*20 B
+    # write the FP64 pressure and FP32 weight once, V*20 B.  The kernels'
+    # durations are the library's HIP events around them on its stream.
     ms_b = float(np.mean([x.ms_pass_b for x in reps]))
+    ms_s = float(np.mean([x.ms_segment_sums for x in reps]))
     bytes_pass = D * V * 3 * 4 + V * 20
-    achieved = bytes_pass / (ms_b * 1e-3) / 1e9
-    traffic = None
+    traffic = {}
     if os.path.exists(TRAFFIC_FILE):
         t = json.load(open(TRAFFIC_FILE))
         if t.get("workload_n") == n and t.get("directions") == D:
-            traffic = t.get("traffic_bytes")
+            traffic = t.get("traffic_bytes", {})
+    kern = {"local_sum": ("k_tile_pass<float, local-sum, 16^3 tiles + halo, 1024 threads> "
+                          "(segment sums of pass A)", ms_s),
+            "full": ("k_tile_pass<float, full, 16^3 tiles, 1024 threads> (final traversal)", ms_b)}
+    dom = max(kern, key=lambda k: kern[k][1])
+    name, ms_dom = kern[dom]
+    achieved = bytes_pass / (ms_dom * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_tile_pass<float, full, 16^3 tiles, 1024 threads> (final traversal)",
-            "algorithmic_bytes": bytes_pass, "ms": round(ms_b, 4)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
+            "kernel": name, "algorithmic_bytes": bytes_pass, "ms": round(ms_dom, 4),
+            "other_traversal": {
+                k: {"kernel": kern[k][0], "ms": round(kern[k][1], 4),
+                    "frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get(k)} for k in kern if k != dom}}
 
     cpu = None
     if not args.no_cpu:
@@ -309,6 +322,7 @@ def main():
                    "sell_cells": reps[-1].sell_cells,
                    "plan_s": round(rep_cold.ms_plan * 1e-3, 4), "cold_s": round(cold_s, 4),
                    "ms_pass_a": round(float(np.mean([x.ms_pass_a for x in reps])), 4),
+                   "ms_segment_sums": round(ms_s, 4),
                    "ms_iterate": round(float(np.mean([x.ms_iterate for x in reps])), 4),
                    "ms_pass_b": round(ms_b, 4),
                    "ms_finalize": round(float(np.mean([x.ms_finalize for x in reps])), 4),

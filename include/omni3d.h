@@ -100,6 +100,7 @@ typedef struct omni3d_report {
   int64_t source_visits;   /* visits landing on source voxels (iteration matrix entries) */
   int64_t segments;        /* ray segments of the tile engine (one per ray and tile) */
   int64_t sell_cells;      /* stored cells of the iteration matrix incl. padding */
+  double ms_segment_sums;  /* the segment-sum traversal of pass A alone (tile engine) */
 } omni3d_report;
 
 typedef struct omni3d_ctx omni3d_ctx;

@@ -514,6 +514,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
                                           ctx->segval.as<double>(), ctx->eterm.as<void>(),
                                           nullptr, st));
+    record_event(ctx, 8);
     OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.nseg.as<unsigned int>(),
                                         pl.val_off.as<unsigned long long>(),
                                         ctx->segval.as<double>(), st));
@@ -527,6 +528,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
                                            nullptr, nullptr, sc, ctx->bs.as<long long>(), 0, 1, st));
+    record_event(ctx, 8);
   }
   record_event(ctx, 3);
   // source iteration
@@ -720,6 +722,7 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
   record_event(ctx, 2);
+  record_event(ctx, 8);
   record_event(ctx, 3);
   for (int k = 1; k <= P.max_iterations; k++) {
     if (fixed) {
@@ -1098,6 +1101,7 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->source_visits = ctx->plan->nnz;
     rep->segments = ctx->plan->nsegs;
     rep->sell_cells = ctx->plan->ncell;
+    rep->ms_segment_sums = ms(2, 8);
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");
   return OMNI3D_OK;

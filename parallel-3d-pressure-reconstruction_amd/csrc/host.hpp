@@ -158,5 +158,5 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_y, adj_z, adj_u, adj_t, adj_g;
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[9] = {};
 };
