@@ -405,7 +405,8 @@ int build_plan(omni3d_ctx* ctx) {
     // terms below 2^51)
     const char* env = std::getenv("OMNI3D_ITER");
     pl.dense = false;
-    const int grp = ctx->dense_group;
+    // groups of 4 chunks use merged words with 22-bit sources
+    const int grp = ctx->dense_group == 4 && nS < (1 << 22) ? 4 : 1;
     if (!(env && std::strcmp(env, "slots") == 0) && nS < (1 << (32 - dense_row_bits(grp))) &&
         mxh[0] < 4096u && nnz > 0 && nnz < (1ull << 31)) {
       DevBuf ka, kb, tmp;
@@ -420,9 +421,28 @@ int build_plan(omni3d_ctx* ctx) {
                                             ka.as<unsigned long long>(), kb.as<unsigned long long>(),
                                             tmp.as<void>(), tmp.size(), pl.dent.as<unsigned int>(),
                                             st));
+      pl.nwords = (int64_t)nnz;
+      if (grp > 1) {
+        DevBuf flag, excl;
+        OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * nnz));
+        OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * nnz));
+        const int ng = (nS + 64 * grp - 1) / (64 * grp);
+        OMNI_TRY_HIP(ctx, pl.dense_wptr.alloc(sizeof(unsigned int) * (ng + 1)));
+        unsigned int nw = 0;
+        OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)nnz, grp,
+                                              pl.rowptr.as<unsigned long long>(),
+                                              kb.as<unsigned long long>(), flag.as<unsigned int>(),
+                                              excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
+                                              pl.dent.as<unsigned int>(),
+                                              pl.dense_wptr.as<unsigned int>(), &nw, st));
+        pl.nwords = nw;
+      }
       OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
       pl.dense = true;
       pl.dense_group = grp;
+      if (std::getenv("OMNI3D_VERBOSE"))
+        std::fprintf(stderr, "omni3d plan: %lld entries in %lld dense words (groups of %d chunks)\n",
+                     (long long)nnz, (long long)pl.nwords, grp);
     }
     pl.staged = false;
     if (env && std::strcmp(env, "staged") == 0) {
@@ -568,7 +588,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     for (int k = 1; k <= P.max_iterations; k++) {
       if (pl.dense && !pl.staged) {
         const hipError_t e = launch_iterate_dense(
-            pl.dense_group, ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(), pl.dent.as<unsigned int>(),
+            pl.dense_group, ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(),
+            pl.dense_wptr.as<unsigned int>(), pl.dent.as<unsigned int>(),
             pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
             ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
             ctx->partials.as<double>(), P.max_iterations, P.tolerance, s,
@@ -847,7 +868,7 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
   }
   if (const char* e = std::getenv("OMNI3D_DENSE_GROUP")) {
     const int v = std::atoi(e);
-    if (v == 1 || v == 4 || v == 8 || v == 16) ctx->dense_group = v;
+    if (v == 1 || v == 4) ctx->dense_group = v;
   }
   if (const char* e = std::getenv("OMNI3D_DENSE_VARIANT")) {
     const int v = std::atoi(e);

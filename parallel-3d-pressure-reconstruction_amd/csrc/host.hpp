@@ -95,8 +95,10 @@ struct Plan {
   // dense iteration layout (valid when dense): per chunk, entries sorted by
   // source, word (row in chunk << 24 | source), in the CSR range of its rows
   bool dense = false;
-  int dense_group = 1; // chunks per group of the words (1, 4, 8, 16)
+  int dense_group = 1; // chunks per group of the words (1 or 4)
+  int64_t nwords = 0;  // dense words (merged equal entries for groups of 4)
   DevBuf dent;         // uint32[nnz+8]
+  DevBuf dense_wptr;   // uint32[ngroups+1] first word of each group (groups of 4)
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
   int ntiles = 0;

@@ -480,8 +480,9 @@ k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
 }
 
 // The dense iteration over groups of kG chunks: a workgroup of kG
-// wavefronts owns a group (kG * 64 rows, the group's words sorted by source,
-// word (row in group << (32 - bits)) | source), its wavefronts take the
+// wavefronts owns a group (kG * 64 = 256 rows, the group's words sorted by
+// source, word (row in group << 24) | (multiplicity - 1) << 22 | source, the
+// group's words from wptr[g]), its wavefronts take the
 // group's rounds of 512 words in turn and add into the group's LDS row sums;
 // each wavefront then finishes the 64 rows of its chunk.  Sources shared by
 // neighbouring chunks touch the same 128-byte lines in one gather.
@@ -494,7 +495,7 @@ k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
 // instead of max |p_S|, so the FP32 rounding applies to increments only.
 template <int kG, int kInc>
 __global__ void __launch_bounds__(64 * kG)
-k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
+k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
               const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
               const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
               const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
@@ -504,9 +505,8 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
   if (sc->converged) return;
   constexpr int kU = 8;
   constexpr int kRows = 64 * kG;
-  constexpr int kRB = dense_row_bits(kG);
-  constexpr unsigned int kSrcMask = (1u << (32 - kRB)) - 1u;
-  static_assert((1 << kRB) >= kRows, "row field too narrow");
+  static_assert(kRows == 256, "merged words hold 8-bit rows");
+  constexpr unsigned int kSrcMask = (1u << 22) - 1u;
   __shared__ unsigned long long s_acc[kRows];
   __shared__ __attribute__((aligned(16))) unsigned int s_ent[kG][kU * 64];
   __shared__ double s_red[2][kG];
@@ -529,8 +529,7 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
     s_acc[threadIdx.x] = 0ull;
     __syncthreads();
     const int rbeg = g * kRows;
-    const int rend = rbeg + kRows < nS ? rbeg + kRows : nS;
-    const unsigned long long e0 = rowptr[rbeg], e1 = rowptr[rend];
+    const unsigned long long e0 = wptr[g], e1 = wptr[g + 1];
     const unsigned long long q1 = (e1 + 3) >> 2;
     unsigned long long base = (e0 & ~3ull) + (unsigned long long)wave * (kU * 64);
     int4 pre[kU / 4];
@@ -571,8 +570,11 @@ k_iterate_grp(int k, int nS, const unsigned long long* __restrict__ rowptr,
       }
 #pragma unroll
       for (int u = 0; u < kU; u++)
-        if (ok[u])
-          atomicAdd(&s_acc[w[u] >> (32 - kRB)], (unsigned long long)to_fixed(x[u], scale));
+        if (ok[u]) {
+          // a word stands for 1..4 equal entries (the same source in one row)
+          const unsigned long long m = ((w[u] >> 22) & 3u) + 1u;
+          atomicAdd(&s_acc[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale) * m);
+        }
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
@@ -948,7 +950,7 @@ unsigned int dense_grid(int nS) {
 }
 
 hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
-                                const unsigned long long* rowptr,
+                                const unsigned long long* rowptr, const unsigned int* wptr,
                                 const unsigned int* dent, const unsigned int* diag,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
@@ -959,22 +961,12 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
     const int rows = 64 * gchunks;
     grid = (unsigned int)std::max(1, std::min((nS + rows - 1) / rows, 65536));
     const int inc = srow ? (k == 1 ? 1 : 2) : 0;
-#define OMNI_GRP(G, I)                                                                         \
-  hipLaunchKernelGGL((k_iterate_grp<G, I>), dim3(grid), dim3(64 * G), 0, st, k, nS, rowptr,      \
-                     dent, diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc,  \
-                     partials)
-#define OMNI_GRPS(G)                 \
-  if (inc == 1) OMNI_GRP(G, 1);      \
-  else if (inc == 2) OMNI_GRP(G, 2); \
-  else OMNI_GRP(G, 0);
-    if (gchunks == 4) {
-      OMNI_GRPS(4)
-    } else if (gchunks == 8) {
-      OMNI_GRPS(8)
-    } else {
-      OMNI_GRPS(16)
-    }
-#undef OMNI_GRPS
+#define OMNI_GRP(I)                                                                            \
+  hipLaunchKernelGGL((k_iterate_grp<4, I>), dim3(grid), dim3(256), 0, st, k, nS, wptr, dent,     \
+                     diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc, partials)
+    if (inc == 1) OMNI_GRP(1);
+    else if (inc == 2) OMNI_GRP(2);
+    else OMNI_GRP(0);
 #undef OMNI_GRP
     hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials,
                        sc, max_iter, tol);

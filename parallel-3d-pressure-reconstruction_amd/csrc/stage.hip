@@ -160,13 +160,82 @@ __global__ void k_dense_words(int64_t n, int rb, const unsigned long long* __res
   dent[i] = (row << (32 - rb)) | src;
 }
 
+// Merge of equal (group, source, row) keys into one word with a
+// multiplicity of 1..4: flag the first key of every run of up to 4.
+__global__ void k_merge_flags(int64_t n, const unsigned long long* __restrict__ k,
+                              unsigned int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t c = 0;
+  while (i - c - 1 >= 0 && k[i - c - 1] == k[i]) c++;
+  flag[i] = (c & 3) == 0 ? 1u : 0u;
+}
+
+// word (row << 24) | (multiplicity - 1) << 22 | source at the run's slot
+__global__ void k_merge_write(int64_t n, int rb, const unsigned long long* __restrict__ k,
+                              const unsigned int* __restrict__ flag,
+                              const unsigned int* __restrict__ excl, unsigned int* __restrict__ dent) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !flag[i]) return;
+  unsigned int m = 1;
+  while (m < 4 && i + m < n && k[i + m] == k[i]) m++;
+  const unsigned long long key = k[i];
+  const unsigned int row = (unsigned int)(key & ((1ull << rb) - 1ull));
+  const unsigned int src = (unsigned int)((key >> rb) & 0x3fffffull);
+  dent[excl[i]] = (row << 24) | ((m - 1u) << 22) | src;
+}
+
+// first word of every group: the merged index of the group's first entry
+__global__ void k_merge_gptr(int ng, int rows, int nS, int64_t n,
+                             const unsigned long long* __restrict__ rowptr,
+                             const unsigned int* __restrict__ excl, unsigned int nwords,
+                             unsigned int* __restrict__ wptr) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > ng) return;
+  const long long r = (long long)g * rows;
+  const unsigned long long pos = rowptr[r < nS ? r : nS];
+  wptr[g] = pos < (unsigned long long)n ? excl[pos] : nwords;
+}
+
 }  // namespace
 
 size_t dense_tmp_bytes(int64_t nnz) {
-  size_t a = 0;
+  size_t a = 0, b = 0;
   (void)hipcub::DeviceRadixSort::SortKeys(nullptr, a, (const unsigned long long*)nullptr,
                                           (unsigned long long*)nullptr, (int)nnz, 0, 64);
-  return a + 256;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b, (const unsigned int*)nullptr,
+                                         (unsigned int*)nullptr, (int)nnz);
+  return std::max(a, b) + 256;
+}
+
+hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
+                               const unsigned long long* keys, unsigned int* flag,
+                               unsigned int* excl, void* tmp, size_t tmp_bytes, unsigned int* dent,
+                               unsigned int* wptr, unsigned int* nwords, hipStream_t st) {
+  const int rb = dense_row_bits(gchunks);
+  hipLaunchKernelGGL(k_merge_flags, dim3(grid1(nnz)), dim3(256), 0, st, nnz, keys, flag);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = tmp_bytes;
+  e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flag, excl, (int)nnz, st);
+  if (e != hipSuccess) return e;
+  unsigned int last[2] = {0, 0};
+  e = hipMemcpyAsync(&last[0], excl + (nnz - 1), 4, hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(&last[1], flag + (nnz - 1), 4, hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  *nwords = last[0] + last[1];
+  hipLaunchKernelGGL(k_merge_write, dim3(grid1(nnz)), dim3(256), 0, st, nnz, rb, keys, flag, excl,
+                     dent);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int rows = 64 * gchunks;
+  const int ng = (nS + rows - 1) / rows;
+  hipLaunchKernelGGL(k_merge_gptr, dim3(grid1(ng + 1)), dim3(256), 0, st, ng, rows, nS, nnz, rowptr,
+                     excl, *nwords, wptr);
+  return hipGetLastError();
 }
 
 hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
@@ -184,6 +253,7 @@ hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
   size_t bytes = tmp_bytes;
   e = hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys_a, keys_b, (int)nnz, 0, end_bit, st);
   if (e != hipSuccess) return e;
+  if (gchunks > 1) return hipSuccess;  // merged words: merge_dense_entries
   hipLaunchKernelGGL(k_dense_words, dim3(grid1(nnz)), dim3(256), 0, st, nnz, rb, keys_b, dent);
   return hipGetLastError();
 }

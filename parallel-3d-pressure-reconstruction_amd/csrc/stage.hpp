@@ -64,24 +64,30 @@ unsigned int staged_grid(int nS);
 // of its rows.
 unsigned int dense_grid(int nS);
 // Row bits of a dense word for groups of gchunks chunks: (row << (32 - bits)) | source.
-constexpr int dense_row_bits(int gchunks) {
-  return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10);
-}
+constexpr int dense_row_bits(int gchunks) { return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10); }
 // gchunks: chunks per group sharing the LDS row sums (1, 4, 8 or 16); the
-// group's words are sorted by source.
+// group's words are sorted by source.  For gchunks > 1 this leaves the sorted
+// keys in keys_b for merge_dense_entries.
 hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* csr_key, unsigned long long* keys_a,
                                unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
                                unsigned int* dent, hipStream_t st);
 size_t dense_tmp_bytes(int64_t nnz);
+// Groups (gchunks > 1): equal (group, source, row) keys of the sorted keys
+// merged into words (row << 24) | (multiplicity - 1) << 22 | source (sources
+// below 2^22), wptr[ngroups+1] the first word of every group.
+hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
+                               const unsigned long long* keys, unsigned int* flag,
+                               unsigned int* excl, void* tmp, size_t tmp_bytes, unsigned int* dent,
+                               unsigned int* wptr, unsigned int* nwords, hipStream_t st);
 // gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
 // entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
-// 4: 4 x 1024); gchunks 4, 8, 16: one group per workgroup of gchunks
-// wavefronts sharing the group's LDS row sums.  With srow (groups only) the
+// 4: 4 x 1024); gchunks 4: one group per workgroup of 4 wavefronts sharing
+// the group's LDS row sums, merged words from wptr.  With srow (groups only) the
 // increment form: srow[nS] FP64 row sums of p_S kept across iterations,
 // din / dnew the FP32 increments of the previous / this iteration.
 hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
-                                const unsigned long long* rowptr,
+                                const unsigned long long* rowptr, const unsigned int* wptr,
                                 const unsigned int* dent, const unsigned int* diag,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
