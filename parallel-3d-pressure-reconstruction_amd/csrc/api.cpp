@@ -250,6 +250,34 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   OMNI_TRY_HIP(ctx, pl.surf.alloc(sizeof(int) * std::max<size_t>(surf.size(), 1)));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.surf.as<void>(), surf.data(), sizeof(int) * surf.size(),
                                    hipMemcpyHostToDevice, st));
+  {
+    // compact lists of the source-sum pass
+    DevBuf scnt, stmp2;
+    OMNI_TRY_HIP(ctx, scnt.alloc(sizeof(unsigned int) * nt));
+    OMNI_TRY_HIP(ctx, hipMemsetAsync(scnt.as<void>(), 0, sizeof(unsigned int) * nt, st));
+    OMNI_TRY_HIP(ctx, pl.sseg_ptr.alloc(sizeof(unsigned long long) * (nt + 1)));
+    OMNI_TRY_HIP(ctx, stmp2.alloc(scan_u32_tmp_bytes(nt)));
+    OMNI_TRY_HIP(ctx, launch_src_segs(0, (int)surf.size(), pl.surf.as<int>(),
+                                      pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(), g,
+                                      tg, pl.sid.as<int>(), scnt.as<unsigned int>(), nullptr,
+                                      nullptr, st));
+    OMNI_TRY_HIP(ctx, launch_scan_u32(scnt.as<unsigned int>(), nt,
+                                      pl.sseg_ptr.as<unsigned long long>(), 0, stmp2.as<void>(),
+                                      stmp2.size(), st));
+    unsigned long long nss = 0;
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nss, pl.sseg_ptr.as<unsigned long long>() + nt, 8,
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    OMNI_TRY_HIP(ctx, pl.ssegs.alloc(sizeof(SegRec) * std::max<unsigned long long>(nss, 1)));
+    OMNI_TRY_HIP(ctx, launch_src_segs(1, (int)surf.size(), pl.surf.as<int>(),
+                                      pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(), g,
+                                      tg, pl.sid.as<int>(), scnt.as<unsigned int>(),
+                                      pl.sseg_ptr.as<unsigned long long>(), pl.ssegs.as<SegRec>(),
+                                      st));
+    pl.nssegs = (int64_t)nss;
+    if (std::getenv("OMNI3D_VERBOSE"))
+      std::fprintf(stderr, "omni3d plan: %llu segments, %llu reach a source voxel\n", nsegs, nss);
+  }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   pl.tg = tg;
   pl.ntiles = nt;
@@ -539,8 +567,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                         pl.val_off.as<unsigned long long>(),
                                         ctx->segval.as<double>(), st));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, ctx->tile_threads, pl.nsurf,
-                                          pl.surf.as<int>(), pl.tile_ptr.as<unsigned long long>(),
-                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
+                                          pl.surf.as<int>(), pl.sseg_ptr.as<unsigned long long>(),
+                                          pl.ssegs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
                                           ctx->segval.as<double>(), ctx->eterm.as<void>(),
                                           ctx->bs.as<long long>(), st));

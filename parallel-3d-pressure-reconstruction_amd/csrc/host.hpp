@@ -110,6 +110,9 @@ struct Plan {
   DevBuf tile_ptr;  // uint64[ntiles+1]
   DevBuf segs;      // SegRec[nsegs], grouped by tile
   DevBuf surf;      // int32[nsurf] tiles that hold source voxels
+  DevBuf ssegs;     // SegRec[...] segments reaching a source voxel, per surface tile
+  DevBuf sseg_ptr;  // uint64[ntiles+1] their range per tile
+  int64_t nssegs = 0;
 };
 
 }  // namespace omni3d

@@ -239,8 +239,10 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   __syncthreads();
   const double scale = sc->scale;
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
-  const unsigned long long b = tile_ptr[tile * kLenBuckets],
-                           e = tile_ptr[(tile + 1) * kLenBuckets];
+  // kSourceSums walks the compact list of the tile's segments that reach a
+  // source voxel (one range per tile), the other passes every segment
+  constexpr int kStride = kMode == kSourceSums ? 1 : kLenBuckets;
+  const unsigned long long b = tile_ptr[tile * kStride], e = tile_ptr[(tile + 1) * kStride];
   // software pipeline over a thread's segments: the record two segments
   // ahead and the values the next segment starts from (its carry, entry
   // term and source pressure) are in flight while this segment is walked
@@ -256,7 +258,9 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     if constexpr (kMode != kLocalSum) {
       if (!fs) {
         q.cv = segval[r.g];
-        q.ev = eterm[jj];
+        // (the compact source-sum list keeps each segment's position in its
+        // tile list, where its step term is stored, in the source field)
+        q.ev = eterm[kMode == kSourceSums ? (unsigned long long)r.src : jj];
       }
     }
     return q;
@@ -385,6 +389,89 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
       if (s >= 0) out[s] = L.acc[sl];
     }
   }
+}
+
+// Compact list of the segments of the source-sum pass (plan build).  One
+// workgroup per surface tile replays the step codes of the tile's segments
+// and keeps those whose visits reach a source voxel after the ray's entry
+// (for a later segment the first visit counts too), with the visit count
+// cut after the last such visit and the position in the full list (where
+// the pass reads the segment's step term) in the source field.  pass 0
+// counts per tile, pass 1 writes at sptr[tile] (order inside a tile
+// immaterial: the pass sums exactly).
+template <int kBits>
+__global__ void __launch_bounds__(256)
+k_src_segs(int pass, const int* __restrict__ tile_list, const unsigned long long* __restrict__ tile_ptr,
+           const SegRec* __restrict__ segs, GridDesc g, TileGrid tg, const int* __restrict__ sid,
+           unsigned int* __restrict__ cnt, const unsigned long long* __restrict__ sptr,
+           SegRec* __restrict__ out) {
+  constexpr int kTile = 1 << kBits;
+  constexpr int kTileMask = kTile - 1;
+  constexpr int kTileVox = kTile * kTile * kTile;
+  __shared__ unsigned char s_src[kTileVox];
+  __shared__ unsigned int s_n;
+  const int tile = tile_list[blockIdx.x];
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  for (int l = threadIdx.x; l < kTileVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    s_src[l] = (i < g.X && j < g.Y && k < g.Z && sid[i + j * sy + k * sz] >= 0) ? 1 : 0;
+  }
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long j = b + threadIdx.x; j < e; j += blockDim.x) {
+    const SegRec sr = segs[j];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const bool first = (sr.info >> 21) & 1u;
+    uint32_t c0 = sr.code[0], c1 = sr.code[1], c2 = sr.code[2];
+    auto next_code = [&]() {
+      const int a = (int)(c0 & 3u);
+      c0 = (c0 >> 2) | (c1 << 30);
+      c1 = (c1 >> 2) | (c2 << 30);
+      c2 >>= 2;
+      return a;
+    };
+    int last = -1;
+    if (!first) {
+      (void)next_code();  // the step into the tile
+      if (s_src[lv]) last = 0;
+    }
+    for (unsigned int vis = 1; vis < n; vis++) {
+      const int a = next_code();
+      const int up = (sg >> a) & 1u;
+      const int d = a == 0 ? 1 : (a == 1 ? kTile : kTile * kTile);
+      lv += up ? d : -d;
+      if (s_src[lv]) last = (int)vis;
+    }
+    if (last < 0) continue;
+    const unsigned int pos = atomicAdd(&s_n, 1u);
+    if (pass == 1) {
+      SegRec r = sr;
+      r.info = (sr.info & ~(63u << kSegLvBits)) | ((unsigned int)(last + 1) << kSegLvBits);
+      r.src = (uint32_t)j;
+      out[sptr[tile] + pos] = r;
+    }
+  }
+  __syncthreads();
+  if (pass == 0 && threadIdx.x == 0) cnt[tile] = s_n;
+}
+
+hipError_t launch_src_segs(int pass, int nsurf, const int* tile_list,
+                           const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                           TileGrid tg, const int* sid, unsigned int* cnt,
+                           const unsigned long long* sptr, SegRec* out, hipStream_t st) {
+  if (nsurf <= 0) return hipSuccess;
+  if (tg.bits == 3)
+    hipLaunchKernelGGL(k_src_segs<3>, dim3(nsurf), dim3(256), 0, st, pass, tile_list, tile_ptr, segs,
+                       g, tg, sid, cnt, sptr, out);
+  else
+    hipLaunchKernelGGL(k_src_segs<4>, dim3(nsurf), dim3(256), 0, st, pass, tile_list, tile_ptr, segs,
+                       g, tg, sid, cnt, sptr, out);
+  return hipGetLastError();
 }
 
 // Iteration matrix layout (plan build).  A chunk is 64 consecutive rows

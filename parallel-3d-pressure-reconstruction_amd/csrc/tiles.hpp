@@ -55,6 +55,13 @@ hipError_t launch_sort_segments(SegRec* segs, SegRec* out, int64_t nsegs, int ns
 hipError_t launch_seg_groups(int64_t nrays, const unsigned int* nseg, unsigned int* gcnt,
                              hipStream_t st);
 // threads: workgroup size of the tile passes, 256, 512 or 1024
+// Compact per-tile lists of the segments that reach a source voxel (source-
+// sum pass): pass 0 counts into cnt[tile], pass 1 writes at sptr[tile].
+hipError_t launch_src_segs(int pass, int nsurf, const int* tile_list,
+                           const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                           TileGrid tg, const int* sid, unsigned int* cnt,
+                           const unsigned long long* sptr, SegRec* out, hipStream_t st);
+// kSourceSums: tile_ptr / segs are the compact lists (one range per tile)
 template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
