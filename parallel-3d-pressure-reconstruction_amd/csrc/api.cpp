@@ -551,7 +551,6 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->bs.as<void>(), 2 * (int64_t)ns, 0u, st));
   OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
-  OMNI_TRY_HIP(ctx, ctx->eterm.alloc(sizeof(T) * std::max<int64_t>(pl.nsegs, 1)));
   record_event(ctx, 2);
   if (ctx->engine == 0) {
     // segment sums over all tiles, per-ray carries, then the source sums on
@@ -560,7 +559,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          ctx->segval.as<double>(),
                                           nullptr, st));
     record_event(ctx, 8);
     OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.nseg.as<unsigned int>(),
@@ -570,7 +569,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.surf.as<int>(), pl.sseg_ptr.as<unsigned long long>(),
                                           pl.ssegs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          ctx->segval.as<double>(),
                                           ctx->bs.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
@@ -697,7 +696,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
-                                          ctx->segval.as<double>(), ctx->eterm.as<void>(),
+                                          ctx->segval.as<double>(),
                                           ctx->sum.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V * nc, 0u, st));

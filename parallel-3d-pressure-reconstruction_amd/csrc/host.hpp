@@ -151,7 +151,6 @@ struct omni3d_ctx {
   omni3d::DevBuf partials;  // double[2*grid]
   omni3d::DevBuf sum;       // int64[ncopies*V]
   omni3d::DevBuf segval;    // double[nval] segment sums, then carries
-  omni3d::DevBuf eterm;     // T[nsegs] trapezoid of the step into each segment's tile
   int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
   int used_copies = 1;

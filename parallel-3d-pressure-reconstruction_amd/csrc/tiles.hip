@@ -99,8 +99,8 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     sr.info = lv | (signs << 18) | (first << 21);
     sr.code[0] = sr.code[1] = sr.code[2] = 0;
   };
-  auto emit = [&]() {
-    sr.info |= nvis << kSegLvBits;
+  auto emit = [&](bool last) {
+    sr.info |= (nvis << kSegLvBits) | ((last ? 1u : 0u) << 22);
     const int ls = len_slot(tile, nvis, tg.lshift);
     const unsigned int slot = atomicAdd(&cursor[ls], 1u);
     segs[tile_ptr[ls] + slot] = sr;
@@ -120,18 +120,20 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     const int a = w.advance(&s);
     if (a < 0) break;
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
+    // the step out of a tile ends the code list of the segment it leaves
+    put_code(a);
     if (t != tile) {
-      emit();
+      emit(false);
       tile = t;
       first = 0;
-      nvis = 0;
+      nvis = 1;
       ncode = 0;
       start(w.c0, w.c1, w.c2);
+    } else {
+      nvis++;
     }
-    put_code(a);
-    nvis++;
   }
-  emit();
+  emit(true);
 }
 
 // ---------------------------------------------------------------------------
@@ -189,7 +191,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
             const DevScalars* __restrict__ sc, double* __restrict__ segval,
-            T* __restrict__ eterm, long long* __restrict__ out) {
+            long long* __restrict__ out) {
   using V4 = typename Vec4<T>::type;
   constexpr int kTileBits = kBits;
   constexpr int kTile = 1 << kBits;
@@ -248,7 +250,6 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   // term and source pressure) are in flight while this segment is walked
   struct Pre {
     double pin = 0.0, cv = 0.0;
-    T ev = T(0);
   };
   const unsigned int bd = blockDim.x;
   auto fetch = [&](const SegRec& r, unsigned long long jj) {
@@ -256,13 +257,9 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const bool fs = (r.info >> 21) & 1u;
     if constexpr (kMode == kFull) q.pin = ps[r.src];
     if constexpr (kMode != kLocalSum) {
-      if (!fs) {
-        q.cv = segval[r.g];
-        // (the compact source-sum list keeps each segment's position in its
-        // tile list, where its step term is stored, in the source field)
-        q.ev = eterm[kMode == kSourceSums ? (unsigned long long)r.src : jj];
-      }
+      if (!fs) q.cv = segval[r.g];
     }
+    (void)jj;
     return q;
   };
   unsigned long long j = b + threadIdx.x;
@@ -327,23 +324,12 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
         accs = pr.pin * scale;
         add_fixed(slot(lv));
       }
-    } else {
-      const int a = (int)(cw & 3u);
-      cw >>= 2;
-      inw = 15;
-      if constexpr (kMode == kLocalSum) {
-        // step into the tile: the previous voxel lies in the halo
-        const int sgn = ((sg >> a) & 1u) ? 1 : -1;
-        const int dprev = a == 0 ? d0 : (a == 1 ? d1 : d2);
-        const T et = trapezoid<T>(wp, a, sgn, L.g[a][lv], L.g[a][lv - dprev]);
-        eterm[j] = et;
-        loc = (double)et;
-      } else {
-        const double carry = pr.cv + (double)pr.ev;
-        accs = (kMode == kFull ? pr.pin + carry : carry) * scale;
-        const int slv = slot(lv);
-        if (kMode == kFull || on_source(slv)) add_fixed(slv);
-      }
+    } else if constexpr (kMode != kLocalSum) {
+      // the carry already holds the step into the tile (it ends the previous
+      // segment's sum)
+      accs = (kMode == kFull ? pr.pin + pr.cv : pr.cv) * scale;
+      const int slv = slot(lv);
+      if (kMode == kFull || on_source(slv)) add_fixed(slv);
     }
     // every later visit adds base + (in-tile sum so far)
     int rem = (int)n - 1;
@@ -373,7 +359,19 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
       cw = wi == 0 ? sr.code[1] : sr.code[2];
       inw = 16;
     }
-    if (kMode == kLocalSum) segval[sr.g] = loc;
+    if constexpr (kMode == kLocalSum) {
+      if (!((sr.info >> 22) & 1u)) {
+        // the step out of the tile (the next voxel lies in the halo) ends
+        // the segment: its code follows the n - 1 in-tile codes
+        const unsigned int q = n - 1;
+        const uint32_t wq = q < 16 ? sr.code[0] : (q < 32 ? sr.code[1] : sr.code[2]);
+        const int a = (int)((wq >> (2 * (q & 15))) & 3u);
+        const int sgn = ((sg >> a) & 1u) ? 1 : -1;
+        const int dn = a == 0 ? d0 : (a == 1 ? d1 : d2);
+        loc += (double)trapezoid<T>(wp, a, sgn, L.g[a][lv + dn], L.g[a][lv]);
+      }
+      segval[sr.g] = loc;
+    }
   }
   if (kMode == kLocalSum) return;
   __syncthreads();
@@ -436,10 +434,7 @@ k_src_segs(int pass, const int* __restrict__ tile_list, const unsigned long long
       return a;
     };
     int last = -1;
-    if (!first) {
-      (void)next_code();  // the step into the tile
-      if (s_src[lv]) last = 0;
-    }
+    if (!first && s_src[lv]) last = 0;
     for (unsigned int vis = 1; vis < n; vis++) {
       const int a = next_code();
       const int up = (sg >> a) & 1u;
@@ -804,11 +799,11 @@ static void tile_pass_threads(int threads, int ntiles, const int* tile_list,
                               const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                               TileGrid tg, const WalkParams& wp,
                               const typename Vec4<T>::type* f, const int* sid, const double* p0,
-                              const double* p1, const DevScalars* sc, double* segval, T* eterm,
+                              const double* p1, const DevScalars* sc, double* segval,
                               long long* out, hipStream_t st) {
 #define OMNI_TILE(NT)                                                                        \
   hipLaunchKernelGGL((k_tile_pass<T, kMode, kBits, NT>), dim3(ntiles), dim3(NT), 0, st,       \
-                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, eterm, out)
+                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out)
   if (threads <= 256) OMNI_TILE(256);
   else if (threads <= 512) OMNI_TILE(512);
   else OMNI_TILE(1024);
@@ -820,14 +815,14 @@ static void tile_pass_bits(int threads, int ntiles, const int* tile_list,
                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                            TileGrid tg, const WalkParams& wp, const typename Vec4<T>::type* f,
                            const int* sid, const double* p0, const double* p1,
-                           const DevScalars* sc, double* segval, T* eterm, long long* out,
+                           const DevScalars* sc, double* segval, long long* out,
                            hipStream_t st) {
   if (tg.bits == 3)
     tile_pass_threads<T, kMode, 3>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, eterm, out, st);
+                                   p0, p1, sc, segval, out, st);
   else
     tile_pass_threads<T, kMode, 4>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, eterm, out, st);
+                                   p0, p1, sc, segval, out, st);
 }
 
 template <typename T>
@@ -835,34 +830,33 @@ hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_l
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, void* eterm, long long* out, hipStream_t st) {
+                            double* segval, long long* out, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
-  T* et = static_cast<T*>(eterm);
   if (ntiles <= 0) return hipSuccess;
   if (tg.bits < kTileBitsMin || tg.bits > kTileBitsMax) return hipErrorInvalidValue;
   if (mode == kLocalSum)
     tile_pass_bits<T, kLocalSum>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                 p0, p1, sc, segval, et, out, st);
+                                 p0, p1, sc, segval, out, st);
   else if (mode == kSourceSums)
     tile_pass_bits<T, kSourceSums>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, et, out, st);
+                                   p0, p1, sc, segval, out, st);
   else
     tile_pass_bits<T, kFull>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0,
-                             p1, sc, segval, et, out, st);
+                             p1, sc, segval, out, st);
   return hipGetLastError();
 }
 
 template hipError_t launch_tile_pass<float>(int, int, int, const int*, const unsigned long long*,
                                             const SegRec*, GridDesc, TileGrid, const WalkParams&,
                                             const void*, const int*, const double*, const double*,
-                                            const DevScalars*, double*, void*, long long*,
+                                            const DevScalars*, double*, long long*,
                                             hipStream_t);
 template hipError_t launch_tile_pass<double>(int, int, int, const int*, const unsigned long long*,
                                              const SegRec*, GridDesc, TileGrid,
                                              const WalkParams&, const void*, const int*,
                                              const double*, const double*, const DevScalars*,
-                                             double*, void*, long long*, hipStream_t);
+                                             double*, long long*, hipStream_t);
 
 size_t scan_u32_tmp_bytes(int64_t n) {
   return sizeof(unsigned long long) * (size_t)((n + kScanBlock - 1) / kScanBlock + 1);

@@ -32,9 +32,10 @@ struct SegRec {
   uint32_t g;        // value slot of the segment (its sum, then its carry)
   uint32_t src;      // source id of the ray (index of p_src)
   uint32_t info;     // local voxel of the first visit (12 bits) | visits (6) << 12 |
-                     // step signs x,y,z (3) << 18 | first segment of the ray (1) << 21
-  uint32_t code[3];  // 2-bit axis codes: the step into the tile (not for the first
-                     // segment) and the steps between its visits, low bits first
+                     // step signs x,y,z (3) << 18 | first segment of the ray (1) << 21 |
+                     // last segment of the ray (1) << 22
+  uint32_t code[3];  // 2-bit axis codes: the steps between its visits, then the step
+                     // out of the tile (not for the last segment), low bits first
 };
 constexpr int kSegLvBits = 12;
 
@@ -67,7 +68,7 @@ hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_l
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, void* eterm, long long* out, hipStream_t st);
+                            double* segval, long long* out, hipStream_t st);
 // masked sliced-ELL copy of the source CSR (64-row chunks, per-direction
 // slots with lane masks, entries packed without padding); D <= 4096
 hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
