@@ -585,11 +585,22 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns16));
   OMNI_TRY_HIP(ctx, ctx->partials.alloc(
                         sizeof(double) * 2 *
-                        std::max(iterate_grid(pl.nS), std::max(staged_grid(pl.nS), dense_grid(pl.nS)))));
+                        std::max(iterate_grid(pl.nS),
+                                 std::max(staged_grid(pl.nS),
+                                          std::max(dense_grid(pl.nS), 2 * grouped_grid(pl.nS))))));
   OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
   // max |p_S| of the starting values: the fixed-point scale of iteration 1
   OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->ps0.as<double>(), pl.nS, &sc->itmax_bits[0], st));
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
+  // the grouped dense iteration runs one launch ahead of its stop rule, so
+  // p_S rotates over three buffers and p_S^{k*-1} is copied out after the loop
+  const bool rot3 = pl.dense && pl.dense_group > 1 && !pl.staged && P.max_iterations > 0;
+  double* pb3[3] = {pbuf[0], pbuf[1], nullptr};
+  if (rot3) {
+    OMNI_TRY_HIP(ctx, ctx->ps2.alloc(sizeof(double) * ns16));
+    OMNI_TRY_HIP(ctx, ctx->psel.alloc(sizeof(double) * ns16));
+    pb3[2] = ctx->ps2.as<double>();
+  }
   // FP32 ray sums: the grouped dense iteration runs in increment form
   const bool inc = std::is_same<T, float>::value && pl.dense && pl.dense_group > 1 &&
                    !std::getenv("OMNI3D_NO_INCREMENTS");
@@ -618,11 +629,23 @@ int run_affine(omni3d_ctx* ctx, int seed) {
             pl.dense_group, ctx->dense_variant, k, pl.nS, pl.rowptr.as<unsigned long long>(),
             pl.dense_wptr.as<unsigned int>(), pl.dent.as<unsigned int>(),
             pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
-            ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0, pbuf[(k - 1) & 1], pbuf[k & 1], sc,
+            ctx->bs.as<long long>(), pinit, seed > 0 ? 1 : 0,
+            rot3 ? pb3[(k - 1) % 3] : pbuf[(k - 1) & 1], rot3 ? pb3[k % 3] : pbuf[k & 1], sc,
             ctx->partials.as<double>(), P.max_iterations, P.tolerance, s,
             inc ? ctx->srow.as<double>() : nullptr, inc ? dbuf[(k - 1) & 1] : nullptr,
             inc ? dbuf[k & 1] : nullptr);
         if (e != hipSuccess) return e;
+        if (rot3 && k == P.max_iterations) {
+          // the stop rule of the last iteration (each grouped launch runs
+          // the previous one's), then p_S^{k*-1} for pass B
+          const unsigned int gg = grouped_grid(pl.nS);
+          hipError_t e2 = launch_iter_finish(
+              k, (int)gg, ctx->partials.as<double>() + (size_t)(k & 1) * 2 * gg, sc,
+              P.max_iterations, P.tolerance, s);
+          if (e2 == hipSuccess)
+            e2 = launch_select_ps(pl.nS, pb3[0], pb3[1], pb3[2], sc, ctx->psel.as<double>(), s);
+          if (e2 != hipSuccess) return e2;
+        }
         continue;
       }
       if (pl.staged) {
@@ -666,6 +689,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         (uint64_t)(uintptr_t)pl.st_cse.as<void>(), (uint64_t)pl.dense,
         (uint64_t)(uintptr_t)pl.dent.as<void>(), (uint64_t)(uintptr_t)pl.rowptr.as<void>(), (uint64_t)ctx->dense_variant,
         (uint64_t)pl.dense_group, (uint64_t)inc, (uint64_t)(uintptr_t)ctx->srow.as<void>(),
+        (uint64_t)rot3, (uint64_t)(uintptr_t)pb3[2], (uint64_t)(uintptr_t)ctx->psel.as<void>(),
         (uint64_t)(uintptr_t)dbuf[0], (uint64_t)(uintptr_t)dbuf[1]};
     if (!ctx->it_exec || key != ctx->it_key) {
       drop_iteration_graph(ctx);
@@ -695,7 +719,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, ctx->tile_threads, pl.ntiles, nullptr,
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
-                                          pl.sid.as<int>(), pbuf[0], pbuf[1], sc,
+                                          pl.sid.as<int>(), rot3 ? ctx->psel.as<double>() : pbuf[0],
+                                          rot3 ? ctx->psel.as<double>() : pbuf[1], sc,
                                           ctx->segval.as<double>(),
                                           ctx->sum.as<long long>(), st));
   } else {
@@ -703,7 +728,9 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     record_event(ctx, 5);
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
-                                           pbuf[0], pbuf[1], sc, ctx->sum.as<long long>(), V, nc, st));
+                                           rot3 ? ctx->psel.as<double>() : pbuf[0],
+                                           rot3 ? ctx->psel.as<double>() : pbuf[1], sc,
+                                           ctx->sum.as<long long>(), V, nc, st));
   }
   record_event(ctx, 6);
   return OMNI3D_OK;

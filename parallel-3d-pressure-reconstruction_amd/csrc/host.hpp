@@ -145,6 +145,7 @@ struct omni3d_ctx {
   omni3d::DevBuf pinit;     // double[V]
   omni3d::DevBuf bs;        // int64[nS]
   omni3d::DevBuf ps0, ps1;  // double[nS]
+  omni3d::DevBuf ps2, psel; // third p_S buffer of the grouped iteration, p_S^{k*-1}
   // increment form of the iteration (FP32 ray sums): row sums of p_S and the
   // FP32 increments of two consecutive iterations
   omni3d::DevBuf srow, dinc0, dinc1;

@@ -40,7 +40,8 @@ struct DevScalars {
   unsigned int ticket;           // last-block counter of the reductions
   unsigned int pad;
   unsigned long long steps;      // staircase steps of the last full pass
-  unsigned long long itmax_bits[2];  // max |p_S| after iteration k, in slot k & 1
+  unsigned long long itmax_bits[3];  // max |p_S| (or increment) after iteration k, slot k & 1
+                                     // (k % 3 with the finish folded into the next launch)
 };
 
 // Method-specific walk parameters.

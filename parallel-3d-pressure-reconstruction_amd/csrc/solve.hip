@@ -479,6 +479,49 @@ k_iterate_dense(int k, int nS, const unsigned long long* __restrict__ rowptr,
   }
 }
 
+constexpr int kFinishBlock = 1024;
+
+// Relative change of iteration k and the stop rule, by one workgroup of
+// kThreads (the caller has checked that no earlier iteration stopped).
+template <int kThreads>
+__device__ void finish_iteration(int k, int nparts, const double* __restrict__ partials,
+                                 DevScalars* __restrict__ sc, int max_iter, double tol) {
+  __shared__ double s_a[kThreads / 64], s_b[kThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kThreads) {
+    a += partials[2 * i];
+    b += partials[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  if (lane == 0) {
+    s_a[wave] = a;
+    s_b[wave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = 0.0;
+    b = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) {
+      a += s_a[w];
+      b += s_b[w];
+    }
+    const double num = sqrt(a), den = sqrt(b);
+    const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
+    sc->rel = rel;
+    sc->dnum = num;
+    sc->dden = den;
+    sc->iter_done = k;
+    if ((tol > 0.0 && rel < tol) || k >= max_iter) {
+      sc->converged = 1;
+      sc->kstar = k;
+    }
+  }
+}
+
 // The dense iteration over groups of kG chunks: a workgroup of kG
 // wavefronts owns a group (kG * 64 = 256 rows, the group's words sorted by
 // source, word (row in group << 24) | (multiplicity - 1) << 22 | source, the
@@ -501,8 +544,19 @@ k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
               const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
               const double* __restrict__ pold, double* __restrict__ pnew,
               double* __restrict__ srow, const float* __restrict__ din, float* __restrict__ dnew,
-              DevScalars* __restrict__ sc, double* __restrict__ partials) {
+              DevScalars* __restrict__ sc, double* __restrict__ partials, int max_iter,
+              double tol) {
+  // the stop rule of iteration k - 1 runs in workgroup 0 of this launch: the
+  // launch computes iteration k before knowing it (the buffers keep p_S of
+  // k - 1), and launch k + 1 returns at once if k - 1 stopped
   if (sc->converged) return;
+  const double* __restrict__ prev_part = partials + (size_t)((k - 1) & 1) * 2 * gridDim.x;
+  partials += (size_t)(k & 1) * 2 * gridDim.x;
+  if (blockIdx.x == 0) {
+    if (k > 1) finish_iteration<64 * kG>(k - 1, gridDim.x, prev_part, sc, max_iter, tol);
+    // the max slot launch k + 1 fills (launch k - 1 read it)
+    if (threadIdx.x == 0) sc->itmax_bits[(k + 1) % 3] = 0ull;
+  }
   constexpr int kU = 8;
   constexpr int kRows = 64 * kG;
   static_assert(kRows == 256, "merged words hold 8-bit rows");
@@ -516,7 +570,7 @@ k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
   unsigned int* __restrict__ se = s_ent[wave];
   const int4* __restrict__ d4 = reinterpret_cast<const int4*>(dent);
   const double inv = sc->inv_scale;
-  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]);
+  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) % 3]);
   const bool finite = B <= DBL_MAX;
   int ex = 0;
   (void)frexp(finite ? B : 1.0, &ex);
@@ -627,53 +681,19 @@ k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
     }
     partials[2 * blockIdx.x] = a;
     partials[2 * blockIdx.x + 1] = c;
-    atomicMax(&sc->itmax_bits[k & 1], m);
+    atomicMax(&sc->itmax_bits[k % 3], m);
   }
 }
 
 // Relative change of iteration k from the workgroups' partial sums (fixed
 // order: a strided share per thread, then a tree), and the stop rule.
-constexpr int kFinishBlock = 1024;
-
 __global__ void __launch_bounds__(kFinishBlock)
 k_iter_finish(int k, int nparts, const double* __restrict__ partials, DevScalars* __restrict__ sc,
-              int max_iter, double tol) {
+              int max_iter, double tol, int slots) {
   if (sc->converged) return;
-  __shared__ double s_a[kFinishBlock / 64], s_b[kFinishBlock / 64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += kFinishBlock) {
-    a += partials[2 * i];
-    b += partials[2 * i + 1];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_xor(a, off);
-    b += __shfl_xor(b, off);
-  }
-  if (lane == 0) {
-    s_a[wave] = a;
-    s_b[wave] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  a = 0.0;
-  b = 0.0;
-  for (int w = 0; w < kFinishBlock / 64; w++) {
-    a += s_a[w];
-    b += s_b[w];
-  }
-  const double num = sqrt(a), den = sqrt(b);
-  const double rel = den > 0.0 ? num / den : (num > 0.0 ? INFINITY : 0.0);
-  sc->rel = rel;
-  sc->dnum = num;
-  sc->dden = den;
-  sc->iter_done = k;
-  // the slot iteration k + 1 fills with its max |p_S| (iteration k read it)
-  sc->itmax_bits[(k + 1) & 1] = 0ull;
-  if ((tol > 0.0 && rel < tol) || k >= max_iter) {
-    sc->converged = 1;
-    sc->kstar = k;
-  }
+  // the max |p_S| slot iteration k + 1 fills (iteration k read it)
+  if (threadIdx.x == 0 && slots == 2) sc->itmax_bits[(k + 1) & 1] = 0ull;
+  finish_iteration<kFinishBlock>(k, nparts, partials, sc, max_iter, tol);
 }
 
 // Sorts the (direction, source) keys of every CSR row (plan build), so that
@@ -939,7 +959,31 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
   }
 #undef OMNI_ITER
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
-                     max_iter, tol);
+                     max_iter, tol, 2);
+  return hipGetLastError();
+}
+
+// p_S of iteration k* - 1 out of the three rotating buffers of the grouped
+// iteration (iteration k writes buffer k % 3).
+__global__ void k_select_ps(int nS, const double* __restrict__ b0, const double* __restrict__ b1,
+                            const double* __restrict__ b2, const DevScalars* __restrict__ sc,
+                            double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nS) return;
+  const int q = (sc->kstar - 1) % 3;
+  out[i] = q == 0 ? b0[i] : (q == 1 ? b1[i] : b2[i]);
+}
+
+hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const double* b2,
+                            const DevScalars* sc, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_select_ps, dim3(grid_for(nS)), dim3(256), 0, st, nS, b0, b1, b2, sc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_iter_finish(int k, int nparts, const double* partials, DevScalars* sc,
+                              int max_iter, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, nparts, partials, sc,
+                     max_iter, tol, 3);
   return hipGetLastError();
 }
 
@@ -963,13 +1007,12 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
     const int inc = srow ? (k == 1 ? 1 : 2) : 0;
 #define OMNI_GRP(I)                                                                            \
   hipLaunchKernelGGL((k_iterate_grp<4, I>), dim3(grid), dim3(256), 0, st, k, nS, wptr, dent,     \
-                     diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc, partials)
+                     diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc, partials, \
+                     max_iter, tol)
     if (inc == 1) OMNI_GRP(1);
     else if (inc == 2) OMNI_GRP(2);
     else OMNI_GRP(0);
 #undef OMNI_GRP
-    hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials,
-                       sc, max_iter, tol);
     return hipGetLastError();
   }
 #define OMNI_DENSE(W, U)                                                                       \
@@ -987,7 +1030,7 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
   }
 #undef OMNI_DENSE
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
-                     max_iter, tol);
+                     max_iter, tol, 2);
   return hipGetLastError();
 }
 
@@ -1006,7 +1049,7 @@ hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int
   hipLaunchKernelGGL((k_iterate_staged<kStageGroup, kStageCap>), dim3(grid), dim3(64 * kStageGroup),
                      0, st, si, k, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials);
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
-                     max_iter, tol);
+                     max_iter, tol, 2);
   return hipGetLastError();
 }
 

@@ -63,6 +63,19 @@ unsigned int staged_grid(int nS);
 // (row in chunk << 24 | source) each; the chunk's words occupy the CSR range
 // of its rows.
 unsigned int dense_grid(int nS);
+// Stop rule of iteration k alone (after the last grouped dense launch, whose
+// own stop rule the next launch would have run); partials of iteration k.
+hipError_t launch_iter_finish(int k, int nparts, const double* partials, DevScalars* sc,
+                              int max_iter, double tol, hipStream_t st);
+// The grouped iteration rotates p_S over three buffers (its launches run
+// one iteration ahead of the stop rule): out = p_S of iteration k* - 1.
+hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const double* b2,
+                            const DevScalars* sc, double* out, hipStream_t st);
+// grid of the grouped dense iteration (4 chunks per workgroup)
+inline unsigned int grouped_grid(int nS) {
+  const int g = (nS + 255) / 256;
+  return (unsigned int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
 // Row bits of a dense word for groups of gchunks chunks: (row << (32 - bits)) | source.
 constexpr int dense_row_bits(int gchunks) { return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10); }
 // gchunks: chunks per group sharing the LDS row sums (1, 4, 8 or 16); the
@@ -83,7 +96,9 @@ hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
 // gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
 // entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
 // 4: 4 x 1024); gchunks 4: one group per workgroup of 4 wavefronts sharing
-// the group's LDS row sums, merged words from wptr.  With srow (groups only) the
+// the group's LDS row sums, merged words from wptr; partials then hold two
+// halves (iterations alternate) and each launch runs the stop rule of the
+// previous iteration, max |.| slots k % 3 -- launch_iter_finish ends the loop.  With srow (groups only) the
 // increment form: srow[nS] FP64 row sums of p_S kept across iterations,
 // din / dnew the FP32 increments of the previous / this iteration.
 hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
