@@ -1,0 +1,120 @@
+"""Size-independent properties at the benchmark sizes (256^3 and 128^3).
+
+The oracle takes hours at 256^3, so parity there rests on properties the
+reconstruction must have whatever its size, all through the C-ABI on the GPU:
+
+* FP32 ray sums agree with the FP64 path (rel-L2 <= 1e-6) with identical
+  visit counts, on the 256^3 Burgers vortex;
+* linearity in the field with zero initial pressure (FP64, rel-L2 <= 1e-12);
+* bitwise repeatability of the FP32 path;
+* a quadratic pressure (linear gradient, integrated exactly by the
+  trapezoid rule) is reproduced from its exact seed (max error <= 1e-9);
+* the tile engine and the one-lane-per-ray engine agree (128^3, FP64,
+  rel-L2 <= 1e-12);
+* the FP32 source iteration in increment form agrees with gathering p_S
+  itself (128^3, rel-L2 <= 1e-7).
+"""
+import numpy as np
+import pytest
+
+import omni3d
+from omni3d import fields as F
+
+pytestmark = pytest.mark.gpu
+
+N = 256
+
+
+@pytest.fixture(scope="module")
+def dirs162():
+    return omni3d.generate_directions(3)
+
+
+@pytest.fixture(scope="module")
+def burgers():
+    return F.burgers_vortex(N, noise=0.02, seed=0)
+
+
+def run(f, dirs, shape, precision, max_iter, p0=None, dtype=np.float64):
+    with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision,
+                        max_iterations=max_iter, tolerance=0.0, seed_pn_with_p=0) as ctx:
+        ctx.set_directions(dirs)
+        ctx.set_field(f["acc"], dtype=dtype)
+        return ctx.reconstruct(p0)
+
+
+def test_fp32_matches_fp64_256(gpu_available, dirs162, burgers):
+    print("case burgers 256^3 fp32 vs fp64", flush=True)
+    shape = (N, N, N)
+    p32, w32, r32 = run(burgers, dirs162, shape, omni3d.FP32_KAHAN, 60, dtype=np.float32)
+    p64, w64, r64 = run(burgers, dirs162, shape, omni3d.FP64, 60)
+    assert r32.iterations == r64.iterations == 60
+    np.testing.assert_array_equal(w32, w64)
+    err = F.rel_l2(p32, p64)
+    print("fp32 vs fp64 rel-L2", err)
+    assert err < 1e-6
+
+
+def test_linearity_256(gpu_available, dirs162, burgers):
+    print("case linearity 256^3 fp64", flush=True)
+    shape = (N, N, N)
+    g = F.taylor_green(N, noise=0.02, seed=1)
+    comb = {"acc": [2.0 * a - 0.5 * b for a, b in zip(g["acc"], burgers["acc"])],
+            "delta": burgers["delta"], "rho": burgers["rho"]}
+    g = {"acc": g["acc"], "delta": burgers["delta"], "rho": burgers["rho"]}
+    p1 = run(g, dirs162, shape, omni3d.FP64, 20)[0]
+    p2 = run(burgers, dirs162, shape, omni3d.FP64, 20)[0]
+    pc = run(comb, dirs162, shape, omni3d.FP64, 20)[0]
+    err = F.rel_l2(pc, 2.0 * p1 - 0.5 * p2)
+    print("linearity rel-L2", err)
+    assert err < 1e-12
+
+
+def test_repeatable_bitwise_256(gpu_available, dirs162, burgers):
+    print("case repeatability 256^3 fp32", flush=True)
+    shape = (N, N, N)
+    with omni3d.Context(shape, burgers["delta"], density=burgers["rho"],
+                        precision=omni3d.FP32_KAHAN, max_iterations=40, tolerance=0.0,
+                        seed_pn_with_p=0) as ctx:
+        ctx.set_directions(dirs162)
+        ctx.set_field(burgers["acc"], dtype=np.float32)
+        a = ctx.reconstruct()[0]
+        b = ctx.reconstruct()[0]
+    np.testing.assert_array_equal(a, b)
+
+
+def test_quadratic_exact_seed_256(gpu_available, dirs162):
+    print("case quadratic 256^3 exact seed", flush=True)
+    shape = (N, N, N)
+    f = F.quadratic(shape, seed=3, delta=(1.0 / (N - 1),) * 3)
+    p = run(f, dirs162, shape, omni3d.FP64, 3, p0=f["p"])[0]
+    scale = max(1.0, np.abs(f["p"]).max())
+    err = np.abs(p - f["p"]).max() / scale
+    print("quadratic max error", err)
+    assert err < 1e-9
+
+
+def test_engines_agree_128(gpu_available, dirs162, monkeypatch):
+    print("case tile vs ray engine 128^3 fp64", flush=True)
+    n = 128
+    shape = (n, n, n)
+    f = F.taylor_green(n, noise=0.03, seed=4)
+    pt = run(f, dirs162, shape, omni3d.FP64, 10)[0]
+    monkeypatch.setenv("OMNI3D_ENGINE", "rays")
+    pr = run(f, dirs162, shape, omni3d.FP64, 10)[0]
+    err = F.rel_l2(pt, pr)
+    print("tile vs ray engine rel-L2", err)
+    assert err < 1e-12
+
+
+def test_increment_form_128(gpu_available, dirs162, monkeypatch):
+    print("case fp32 increment form 128^3", flush=True)
+    n = 128
+    shape = (n, n, n)
+    f = F.taylor_green(n, noise=0.03, seed=5)
+    pi = run(f, dirs162, shape, omni3d.FP32_KAHAN, 60, dtype=np.float32)[0]
+    monkeypatch.setenv("OMNI3D_NO_INCREMENTS", "1")
+    pp = run(f, dirs162, shape, omni3d.FP32_KAHAN, 60, dtype=np.float32)[0]
+    err = F.rel_l2(pi, pp)
+    print("increment form vs p_S gathers rel-L2", err)
+    assert err < 1e-7
