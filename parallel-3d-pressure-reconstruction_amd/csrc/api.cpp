@@ -240,6 +240,30 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
     OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
     pl.segs.swap(sorted);
   }
+  {
+    // each tile's segments in value-slot order (the segment-sum pass stores
+    // one slot per segment: coalesced stores)
+    std::vector<unsigned long long> hp((size_t)nslots + 1), h1((size_t)nt + 1);
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(hp.data(), pl.tile_ptr.as<void>(),
+                                     sizeof(unsigned long long) * (nslots + 1),
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    for (int t = 0; t <= nt; t++) h1[t] = hp[(size_t)t * kLenBuckets];
+    OMNI_TRY_HIP(ctx, pl.tile_ptr1.alloc(sizeof(unsigned long long) * (nt + 1)));
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.tile_ptr1.as<void>(), h1.data(),
+                                     sizeof(unsigned long long) * (nt + 1), hipMemcpyHostToDevice,
+                                     st));
+    int key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) < nval) key_bits++;
+    DevBuf stmp;
+    OMNI_TRY_HIP(ctx, pl.segs_slot.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
+    const size_t tb = sort_segments_tmp_bytes((int64_t)nsegs, nt);
+    OMNI_TRY_HIP(ctx, stmp.alloc(tb));
+    OMNI_TRY_HIP(ctx, launch_sort_segments(pl.segs.as<SegRec>(), pl.segs_slot.as<SegRec>(),
+                                           (int64_t)nsegs, nt, pl.tile_ptr1.as<unsigned long long>(),
+                                           key_bits, stmp.as<void>(), tb, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  }
   // tiles that touch the box surface hold every source voxel of the plain method
   std::vector<int> surf;
   for (int t = 0; t < nt; t++) {
@@ -556,8 +580,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     // segment sums over all tiles, per-ray carries, then the source sums on
     // the surface tiles
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
-                                          pl.tile_ptr.as<unsigned long long>(),
-                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
+                                          pl.tile_ptr1.as<unsigned long long>(),
+                                          pl.segs_slot.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), nullptr, nullptr, sc,
                                           ctx->segval.as<double>(),
                                           nullptr, st));

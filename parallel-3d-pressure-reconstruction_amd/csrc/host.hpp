@@ -110,6 +110,8 @@ struct Plan {
   DevBuf tile_ptr;  // uint64[ntiles+1]
   DevBuf segs;      // SegRec[nsegs], grouped by tile
   DevBuf surf;      // int32[nsurf] tiles that hold source voxels
+  DevBuf segs_slot; // SegRec[nsegs] each tile's segments in value-slot order
+  DevBuf tile_ptr1; // uint64[ntiles+1] their range per tile
   DevBuf ssegs;     // SegRec[...] segments reaching a source voxel, per surface tile
   DevBuf sseg_ptr;  // uint64[ntiles+1] their range per tile
   int64_t nssegs = 0;

@@ -145,7 +145,8 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 // different LDS banks (a bijection within every (y, z) row).
 template <int kBits>
 __device__ __forceinline__ int lds_slot(int l) {
-  return l ^ (((l >> (kBits + 1)) ^ (l >> (2 * kBits))) & ((1 << kBits) - 1));
+  constexpr int m = (1 << kBits) - 1;
+  return l ^ ((l >> (kBits + 1)) & m) ^ ((l >> (2 * kBits)) & m);
 }
 
 // Trapezoid factor ((-rho * s) * delta_a) * 0.5 of a step along axis a with
@@ -242,8 +243,9 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   const double scale = sc->scale;
   const double* __restrict__ ps = ((sc->kstar - 1) & 1) ? pbuf1 : pbuf0;
   // kSourceSums walks the compact list of the tile's segments that reach a
-  // source voxel (one range per tile), the other passes every segment
-  constexpr int kStride = kMode == kSourceSums ? 1 : kLenBuckets;
+  // source voxel and kLocalSum the tile's segments in value-slot order (one
+  // range per tile: coalesced slot stores); kFull the length groups
+  constexpr int kStride = kMode == kFull ? kLenBuckets : 1;
   const unsigned long long b = tile_ptr[tile * kStride], e = tile_ptr[(tile + 1) * kStride];
   // software pipeline over a thread's segments: the record two segments
   // ahead and the values the next segment starts from (its carry, entry
@@ -301,14 +303,13 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const T h1 = step_factor<T>(wp, 1, up1);
     const T h2 = step_factor<T>(wp, 2, up2);
     // in-tile sums accumulate in FP64: kLocalSum the segment's sum; the
-    // other passes (base + sum so far) * scale, the fixed-point value up to
-    // the final rounding (the scale is a power of two: every product by it
-    // is exact)
-    double loc = 0.0, accs = 0.0;
+    // other passes (base + sum so far) * scale + 1.5 * 2^52, whose low
+    // mantissa bits are the 64-bit fixed-point value (|value| < 2^51, the
+    // scale is a power of two: each step adds its term rounded to a unit)
+    double loc = 0.0, accs = 6755399441055744.0;
     auto add_fixed = [&](int l) {
       if constexpr (kMode != kLocalSum) {
-        const long long f =
-            __double_as_longlong(accs + 6755399441055744.0) - 0x4338000000000000LL;
+        const long long f = __double_as_longlong(accs) - 0x4338000000000000LL;
         atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]), (unsigned long long)f);
       }
     };
@@ -321,13 +322,13 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     if (first) {
       // the entry visit: self-add p_src (pint = 0)
       if constexpr (kMode == kFull) {
-        accs = pr.pin * scale;
+        accs = pr.pin * scale + 6755399441055744.0;
         add_fixed(slot(lv));
       }
     } else if constexpr (kMode != kLocalSum) {
       // the carry already holds the step into the tile (it ends the previous
       // segment's sum)
-      accs = (kMode == kFull ? pr.pin + pr.cv : pr.cv) * scale;
+      accs = (kMode == kFull ? pr.pin + pr.cv : pr.cv) * scale + 6755399441055744.0;
       const int slv = slot(lv);
       if (kMode == kFull || on_source(slv)) add_fixed(slv);
     }

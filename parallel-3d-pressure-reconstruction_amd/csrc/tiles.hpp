@@ -62,7 +62,8 @@ hipError_t launch_src_segs(int pass, int nsurf, const int* tile_list,
                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                            TileGrid tg, const int* sid, unsigned int* cnt,
                            const unsigned long long* sptr, SegRec* out, hipStream_t st);
-// kSourceSums: tile_ptr / segs are the compact lists (one range per tile)
+// kSourceSums: tile_ptr / segs are the compact lists, kLocalSum the slot-
+// ordered copy (one range per tile); kFull the length-grouped lists
 template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
