@@ -582,7 +582,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
                                           pl.tile_ptr1.as<unsigned long long>(),
                                           pl.segs_slot.as<SegRec>(), pl.g, pl.tg, wp, field,
-                                          pl.sid.as<int>(), nullptr, nullptr, sc,
+                                          pl.sid.as<int>(), pl.cnt.as<unsigned int>(), nullptr, nullptr, sc,
                                           ctx->segval.as<double>(),
                                           nullptr, st));
     record_event(ctx, 8);
@@ -592,7 +592,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, ctx->tile_threads, pl.nsurf,
                                           pl.surf.as<int>(), pl.sseg_ptr.as<unsigned long long>(),
                                           pl.ssegs.as<SegRec>(), pl.g, pl.tg, wp, field,
-                                          pl.sid.as<int>(), nullptr, nullptr, sc,
+                                          pl.sid.as<int>(), pl.cnt.as<unsigned int>(), nullptr, nullptr, sc,
                                           ctx->segval.as<double>(),
                                           ctx->bs.as<long long>(), st));
   } else {
@@ -743,7 +743,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, ctx->tile_threads, pl.ntiles, nullptr,
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field,
-                                          pl.sid.as<int>(), rot3 ? ctx->psel.as<double>() : pbuf[0],
+                                          pl.sid.as<int>(), pl.cnt.as<unsigned int>(),
+                                          rot3 ? ctx->psel.as<double>() : pbuf[0],
                                           rot3 ? ctx->psel.as<double>() : pbuf[1], sc,
                                           ctx->segval.as<double>(),
                                           ctx->sum.as<long long>(), st));

@@ -145,8 +145,9 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 // different LDS banks (a bijection within every (y, z) row).
 template <int kBits>
 __device__ __forceinline__ int lds_slot(int l) {
-  constexpr int m = (1 << kBits) - 1;
-  return l ^ ((l >> (kBits + 1)) & m) ^ ((l >> (2 * kBits)) & m);
+  const unsigned int u = (unsigned int)l;
+  return (int)(u ^ __builtin_amdgcn_ubfe(u, kBits + 1, kBits) ^
+               __builtin_amdgcn_ubfe(u, 2 * kBits, kBits));
 }
 
 // Trapezoid factor ((-rho * s) * delta_a) * 0.5 of a step along axis a with
@@ -190,6 +191,7 @@ __global__ void __launch_bounds__(kThreads)
 k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restrict__ tile_ptr,
             const SegRec* __restrict__ segs, GridDesc g, TileGrid tg, WalkParams wp,
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
+            const unsigned int* __restrict__ vcnt,
             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
             const DevScalars* __restrict__ sc, double* __restrict__ segval,
             long long* __restrict__ out) {
@@ -308,7 +310,11 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     // scale is a power of two: each step adds its term rounded to a unit)
     double loc = 0.0, accs = 6755399441055744.0;
     auto add_fixed = [&](int l) {
-      if constexpr (kMode != kLocalSum) {
+      if constexpr (kMode == kFull) {
+        // the biased bits; the flush removes visits x the bias
+        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]),
+                  (unsigned long long)__double_as_longlong(accs));
+      } else if constexpr (kMode == kSourceSums) {
         const long long f = __double_as_longlong(accs) - 0x4338000000000000LL;
         atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[l]), (unsigned long long)f);
       }
@@ -382,7 +388,10 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     if (i >= g.X || j >= g.Y || k >= g.Z) continue;
     const int64_t v = i + j * sy + k * sz;
     const int sl = lds_slot<kBits>(l);
-    if constexpr (kMode == kFull) out[v] = L.acc[sl];
+    // (every visit of v is walked by this tile: cnt[v] biased adds)
+    if constexpr (kMode == kFull)
+      out[v] = (long long)((unsigned long long)L.acc[sl] -
+                           (unsigned long long)vcnt[v] * 0x4338000000000000ull);
     if constexpr (kMode == kSourceSums) {
       const int s = L.sidl[sl];
       if (s >= 0) out[s] = L.acc[sl];
@@ -799,12 +808,13 @@ template <typename T, int kMode, int kBits>
 static void tile_pass_threads(int threads, int ntiles, const int* tile_list,
                               const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                               TileGrid tg, const WalkParams& wp,
-                              const typename Vec4<T>::type* f, const int* sid, const double* p0,
+                              const typename Vec4<T>::type* f, const int* sid,
+                              const unsigned int* vcnt, const double* p0,
                               const double* p1, const DevScalars* sc, double* segval,
                               long long* out, hipStream_t st) {
 #define OMNI_TILE(NT)                                                                        \
   hipLaunchKernelGGL((k_tile_pass<T, kMode, kBits, NT>), dim3(ntiles), dim3(NT), 0, st,       \
-                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0, p1, sc, segval, out)
+                     tile_list, tile_ptr, segs, g, tg, wp, f, sid, vcnt, p0, p1, sc, segval, out)
   if (threads <= 256) OMNI_TILE(256);
   else if (threads <= 512) OMNI_TILE(512);
   else OMNI_TILE(1024);
@@ -815,22 +825,22 @@ template <typename T, int kMode>
 static void tile_pass_bits(int threads, int ntiles, const int* tile_list,
                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                            TileGrid tg, const WalkParams& wp, const typename Vec4<T>::type* f,
-                           const int* sid, const double* p0, const double* p1,
-                           const DevScalars* sc, double* segval, long long* out,
-                           hipStream_t st) {
+                           const int* sid, const unsigned int* vcnt, const double* p0,
+                           const double* p1, const DevScalars* sc, double* segval,
+                           long long* out, hipStream_t st) {
   if (tg.bits == 3)
     tile_pass_threads<T, kMode, 3>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   vcnt, p0, p1, sc, segval, out, st);
   else
     tile_pass_threads<T, kMode, 4>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   vcnt, p0, p1, sc, segval, out, st);
 }
 
 template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
-                            const double* p0, const double* p1, const DevScalars* sc,
+                            const unsigned int* vcnt, const double* p0, const double* p1, const DevScalars* sc,
                             double* segval, long long* out, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
@@ -838,25 +848,27 @@ hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_l
   if (tg.bits < kTileBitsMin || tg.bits > kTileBitsMax) return hipErrorInvalidValue;
   if (mode == kLocalSum)
     tile_pass_bits<T, kLocalSum>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                 p0, p1, sc, segval, out, st);
+                                 vcnt, p0, p1, sc, segval, out, st);
   else if (mode == kSourceSums)
     tile_pass_bits<T, kSourceSums>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
-                                   p0, p1, sc, segval, out, st);
+                                   vcnt, p0, p1, sc, segval, out, st);
   else
-    tile_pass_bits<T, kFull>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid, p0,
-                             p1, sc, segval, out, st);
+    tile_pass_bits<T, kFull>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
+                             vcnt, p0, p1, sc, segval, out, st);
   return hipGetLastError();
 }
 
 template hipError_t launch_tile_pass<float>(int, int, int, const int*, const unsigned long long*,
                                             const SegRec*, GridDesc, TileGrid, const WalkParams&,
-                                            const void*, const int*, const double*, const double*,
+                                            const void*, const int*, const unsigned int*,
+                                            const double*, const double*,
                                             const DevScalars*, double*, long long*,
                                             hipStream_t);
 template hipError_t launch_tile_pass<double>(int, int, int, const int*, const unsigned long long*,
                                              const SegRec*, GridDesc, TileGrid,
                                              const WalkParams&, const void*, const int*,
-                                             const double*, const double*, const DevScalars*,
+                                             const unsigned int*, const double*, const double*,
+                                             const DevScalars*,
                                              double*, long long*, hipStream_t);
 
 size_t scan_u32_tmp_bytes(int64_t n) {

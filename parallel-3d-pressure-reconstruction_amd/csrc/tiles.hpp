@@ -68,6 +68,7 @@ template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
+                            const unsigned int* vcnt,
                             const double* p0, const double* p1, const DevScalars* sc,
                             double* segval, long long* out, hipStream_t st);
 // masked sliced-ELL copy of the source CSR (64-row chunks, per-direction
