@@ -492,9 +492,22 @@ int build_plan(omni3d_ctx* ctx) {
       OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
       pl.dense = true;
       pl.dense_group = grp;
-      if (std::getenv("OMNI3D_VERBOSE"))
+      if (std::getenv("OMNI3D_VERBOSE")) {
         std::fprintf(stderr, "omni3d plan: %lld entries in %lld dense words (groups of %d chunks)\n",
                      (long long)nnz, (long long)pl.nwords, grp);
+        if (grp > 1) {
+          const int ng = (nS + 64 * grp - 1) / (64 * grp);
+          std::vector<unsigned int> wp(ng + 1);
+          if (hipMemcpy(wp.data(), pl.dense_wptr.as<void>(), sizeof(unsigned int) * (ng + 1),
+                        hipMemcpyDeviceToHost) == hipSuccess) {
+            std::vector<unsigned int> n(ng);
+            for (int i = 0; i < ng; i++) n[i] = wp[i + 1] - wp[i];
+            std::sort(n.begin(), n.end());
+            std::fprintf(stderr, "omni3d plan: words per group min %u median %u p90 %u max %u\n",
+                         n[0], n[ng / 2], n[(ng * 9) / 10], n[ng - 1]);
+          }
+        }
+      }
     }
     pl.staged = false;
     if (env && std::strcmp(env, "staged") == 0) {
