@@ -28,6 +28,45 @@ int io_dtype(const omni3d_ctx* ctx) {
   return ctx->prm.precision == OMNI3D_FP64 ? OMNI3D_F64 : OMNI3D_F32;
 }
 
+// Transposed source rows of the plain method, kept with the plan: row s'
+// lists the source voxels that rays entering at s' visit after their entry
+// (the entries of the forward source rows, regrouped by their source).
+int build_transposed_rows(omni3d_ctx* ctx) {
+  Plan& pl = *ctx->plan;
+  if (pl.trows) return OMNI3D_OK;
+  hipStream_t st = ctx->stream;
+  const int nS = pl.nS;
+  DevBuf tcnt, key;
+  OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(tcnt.as<void>(), nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_adj_transpose_rows(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                              pl.g, pl.sid.as<int>(), tcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, pl.tptr.alloc(sizeof(unsigned long long) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(tcnt.as<unsigned int>(), nS,
+                                       pl.tptr.as<unsigned long long>(), st));
+  unsigned long long n = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&n, pl.tptr.as<unsigned long long>() + nS, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if ((int64_t)n != pl.nnz)
+    return fail(ctx, OMNI3D_ERR_STATE, "transposed source rows disagree with the source rows");
+  OMNI_TRY_HIP(ctx, key.alloc(sizeof(unsigned long long) * (n + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(tcnt.as<void>(), nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_adj_transpose_fill(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                              pl.g, pl.sid.as<int>(),
+                                              pl.tptr.as<unsigned long long>(),
+                                              tcnt.as<unsigned int>(),
+                                              key.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.tptr.as<unsigned long long>(),
+                                     key.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, pl.tent.alloc(sizeof(unsigned int) * (n + 1)));
+  OMNI_TRY_HIP(ctx, launch_adj_narrow(key.as<unsigned long long>(), (int64_t)n,
+                                      pl.tent.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  pl.trows = true;
+  return OMNI3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -106,12 +145,42 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
     OMNI_TRY_HIP(ctx, (launch_convert<float, double>(static_cast<const float*>(dldp_d), y, V, st)));
   OMNI_TRY_HIP(ctx, (launch_convert<double, double>(y, z, V, st)));
   // z = sum_{j<K} (A^T)^j y
-  for (int m = 1; m < P.max_iterations; m++) {
-    OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
-    OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
-    OMNI_TRY_HIP(ctx, launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
-                                        pl.dirs.as<DirConst>(), pl.g, mask, u, t, st));
-    OMNI_TRY_HIP(ctx, launch_adj_horner(V, y, t, cnt, z, st));
+  if (!mask_rule && P.solver == OMNI3D_SOLVER_AFFINE) {
+    // plain method: after the first gather only the source voxels carry
+    // values, so the Horner steps run over the nS sources with the
+    // transposed source rows
+    if (P.max_iterations > 1) {
+      rc = build_transposed_rows(ctx);
+      if (rc) return rc;
+      const int nS = pl.nS;
+      OMNI_TRY_HIP(ctx, ctx->adj_s.alloc(sizeof(double) * 6 * (size_t)nS));
+      double* ts = ctx->adj_s.as<double>();
+      double* h[2] = {ts + nS, ts + 2 * (size_t)nS};
+      double* w[2] = {ts + 3 * (size_t)nS, ts + 4 * (size_t)nS};
+      double* cs = ts + 5 * (size_t)nS;
+      OMNI_TRY_HIP(ctx, launch_adj_scale(V, y, cnt, u, st));
+      OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
+      OMNI_TRY_HIP(ctx, launch_adj_gather(false, pl.rays.as<RayRec>(), pl.nrays,
+                                          pl.dirs.as<DirConst>(), pl.g, nullptr, u, t, st));
+      OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, pl.slist.as<int>(), t, cnt, ts, h[0], w[0], cs, st));
+      int cur = 0;
+      for (int m = 2; m < P.max_iterations; m++) {
+        OMNI_TRY_HIP(ctx, launch_adj_src_iter(nS, pl.tptr.as<unsigned long long>(),
+                                              pl.tent.as<unsigned int>(),
+                                              pl.diag.as<unsigned int>(), cs, ts, w[cur],
+                                              h[cur ^ 1], w[cur ^ 1], st));
+        cur ^= 1;
+      }
+      OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, pl.slist.as<int>(), h[cur], z, st));
+    }
+  } else {
+    for (int m = 1; m < P.max_iterations; m++) {
+      OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
+      OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
+      OMNI_TRY_HIP(ctx, launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
+                                          pl.dirs.as<DirConst>(), pl.g, mask, u, t, st));
+      OMNI_TRY_HIP(ctx, launch_adj_horner(V, y, t, cnt, z, st));
+    }
   }
   // dL/dg = B^T (z / C)
   OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));

@@ -164,6 +164,171 @@ k_adj_horner(int64_t V, const double* __restrict__ y, const double* __restrict__
     z[v] = cnt[v] > 0 ? y[v] + t[v] : y[v];
 }
 
+// ---------------------------------------------------------------------------
+// Source-restricted backward of the plain method.  A^T w is nonzero only on
+// the source voxels S after its first application, so with
+//   t = P^T (y / C) restricted to S  (one gather pass),
+//   h_1 = t,  h_m = t + A_S^T h_{m-1}  (m = 2 .. K-1, vectors over S),
+// the Horner sum is z = y + E^T h_{K-1} (E^T places h at its voxels).  A_S^T
+// is the transpose of the source rows of the forward iteration: row s' lists,
+// for every ray entering at s', the source voxels its later visits land on,
+// and (A_S^T h)[s'] = diag[s'] w[s'] + sum_{s in row s'} w[s], w = h / C.
+
+// the ray's entry voxel, whose source id names its transposed row
+__device__ __forceinline__ int ray_source(const RayRec& rr, const GridDesc& g,
+                                          const int* __restrict__ sid) {
+  int c[3];
+  unpack_voxel(rr.in, c);
+  return sid[(int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y];
+}
+
+struct TCountVisitor {
+  const int* sid;
+  unsigned int n;
+  __device__ __forceinline__ void entry(int64_t) {}
+  __device__ __forceinline__ void visit(int64_t v, bool surface) {
+    if (surface && sid[v] >= 0) n++;
+  }
+};
+
+__global__ void __launch_bounds__(256)
+k_adj_tcount(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+             GridDesc g, const int* __restrict__ sid, unsigned int* __restrict__ tcnt) {
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  TCountVisitor vis{sid, 0u};
+  walk_geometry(rays[r], dirs, g, vis);
+  if (vis.n) atomicAdd(&tcnt[ray_source(rays[r], g, sid)], vis.n);
+}
+
+struct TFillVisitor {
+  const int* sid;
+  unsigned long long* key;
+  unsigned long long base;
+  unsigned int i;
+  __device__ __forceinline__ void entry(int64_t) {}
+  __device__ __forceinline__ void visit(int64_t v, bool surface) {
+    if (!surface) return;
+    const int s = sid[v];
+    if (s >= 0) key[base + i++] = (unsigned long long)(unsigned int)s;
+  }
+};
+
+// Each ray reserves its visits' run of its row (rows are sorted afterwards,
+// so the order the rays reserve in does not matter).
+__global__ void __launch_bounds__(256)
+k_adj_tfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+            GridDesc g, const int* __restrict__ sid, const unsigned long long* __restrict__ tptr,
+            unsigned int* __restrict__ cursor, unsigned long long* __restrict__ key) {
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  TCountVisitor cv{sid, 0u};
+  walk_geometry(rays[r], dirs, g, cv);
+  if (cv.n == 0) return;
+  const int row = ray_source(rays[r], g, sid);
+  const unsigned int pos = atomicAdd(&cursor[row], cv.n);
+  TFillVisitor fv{sid, key, tptr[row] + pos, 0u};
+  walk_geometry(rays[r], dirs, g, fv);
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_narrow(const unsigned long long* __restrict__ key, int64_t n, unsigned int* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (unsigned int)key[i];
+}
+
+// t_S = t at the source voxels; h_1 = t_S, w_1 = h_1 / C, and C per source.
+__global__ void __launch_bounds__(256)
+k_adj_src_init(int nS, const int* __restrict__ slist, const double* __restrict__ t,
+               const unsigned int* __restrict__ cnt, double* __restrict__ ts,
+               double* __restrict__ h, double* __restrict__ w, double* __restrict__ cs) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nS) return;
+  const int64_t v = slist[s];
+  const double c = (double)cnt[v];
+  const double x = t[v];
+  ts[s] = x;
+  h[s] = x;
+  w[s] = x / c;
+  cs[s] = c;
+}
+
+// One wavefront per transposed row: h' = t + diag w + sum of w over the row,
+// w' = h' / C.  The lane split and the reduction tree are fixed, so the sums
+// are bitwise repeatable.
+__global__ void __launch_bounds__(256)
+k_adj_src_iter(int nS, const unsigned long long* __restrict__ tptr,
+               const unsigned int* __restrict__ tent, const unsigned int* __restrict__ diag,
+               const double* __restrict__ cs, const double* __restrict__ ts,
+               const double* __restrict__ w, double* __restrict__ hn, double* __restrict__ wn) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (r >= nS) return;
+  const unsigned long long e1 = tptr[r + 1];
+  double acc = 0.0;
+  for (unsigned long long e = tptr[r] + lane; e < e1; e += 64) acc += w[tent[e]];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (lane == 0) {
+    const double h = ts[r] + (double)diag[r] * w[r] + acc;
+    hn[r] = h;
+    wn[r] = h / cs[r];
+  }
+}
+
+// z = y + E^T h (source voxels are distinct, so plain stores).
+__global__ void __launch_bounds__(256)
+k_adj_src_out(int nS, const int* __restrict__ slist, const double* __restrict__ h,
+              double* __restrict__ z) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nS) return;
+  z[slist[s]] += h[s];
+}
+
+hipError_t launch_adj_transpose_rows(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                     GridDesc g, const int* sid, unsigned int* tcnt,
+                                     hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_tcount, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
+                     dirs, g, sid, tcnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_transpose_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                     GridDesc g, const int* sid, const unsigned long long* tptr,
+                                     unsigned int* cursor, unsigned long long* key,
+                                     hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_tfill, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
+                     dirs, g, sid, tptr, cursor, key);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_narrow(const unsigned long long* key, int64_t n, unsigned int* out,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_narrow, dim3(grid_for(n)), dim3(256), 0, st, key, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_src_init(int nS, const int* slist, const double* t, const unsigned int* cnt,
+                               double* ts, double* h, double* w, double* cs, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_src_init, dim3((nS + 255) / 256), dim3(256), 0, st, nS, slist, t, cnt,
+                     ts, h, w, cs);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const unsigned int* tent,
+                               const unsigned int* diag, const double* cs, const double* ts,
+                               const double* w, double* hn, double* wn, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_src_iter, dim3((nS + 3) / 4), dim3(256), 0, st, nS, tptr, tent, diag,
+                     cs, ts, w, hn, wn);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, double* z,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_src_out, dim3((nS + 255) / 256), dim3(256), 0, st, nS, slist, h, z);
+  return hipGetLastError();
+}
+
 hipError_t launch_adj_gather(bool mask_rule, const RayRec* rays, int64_t nrays,
                              const DirConst* dirs, GridDesc g, const double* mask,
                              const double* u, double* t, hipStream_t st) {

@@ -115,6 +115,12 @@ struct Plan {
   DevBuf ssegs;     // SegRec[...] segments reaching a source voxel, per surface tile
   DevBuf sseg_ptr;  // uint64[ntiles+1] their range per tile
   int64_t nssegs = 0;
+  // transposed source rows of the plain method (built by the first
+  // source-restricted backward): row s' = the sources that rays entering at
+  // s' visit, sorted
+  bool trows = false;
+  DevBuf tptr;  // uint64[nS+1]
+  DevBuf tent;  // uint32[nnz+1]
 };
 
 }  // namespace omni3d
@@ -163,6 +169,8 @@ struct omni3d_ctx {
   omni3d::DevBuf pa, pb, aux, sumd, wsum, wcnt, srcf;
   // adjoint: dL/dp, Horner sum, scaled sum, gathered sums, dL/dg (3 V)
   omni3d::DevBuf adj_y, adj_z, adj_u, adj_t, adj_g;
+  // source-restricted backward: t_S, h (2), w (2), C_S over the nS sources
+  omni3d::DevBuf adj_s;
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
   hipEvent_t ev[9] = {};

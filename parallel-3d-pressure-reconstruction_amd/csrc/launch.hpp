@@ -88,6 +88,23 @@ hipError_t launch_adj_scale(int64_t V, const double* z, const unsigned int* cnt,
                             hipStream_t st);
 hipError_t launch_adj_horner(int64_t V, const double* y, const double* t,
                              const unsigned int* cnt, double* z, hipStream_t st);
+// source-restricted backward of the plain method
+hipError_t launch_adj_transpose_rows(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                     GridDesc g, const int* sid, unsigned int* tcnt,
+                                     hipStream_t st);
+hipError_t launch_adj_transpose_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                     GridDesc g, const int* sid, const unsigned long long* tptr,
+                                     unsigned int* cursor, unsigned long long* key,
+                                     hipStream_t st);
+hipError_t launch_adj_narrow(const unsigned long long* key, int64_t n, unsigned int* out,
+                             hipStream_t st);
+hipError_t launch_adj_src_init(int nS, const int* slist, const double* t, const unsigned int* cnt,
+                               double* ts, double* h, double* w, double* cs, hipStream_t st);
+hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const unsigned int* tent,
+                               const unsigned int* diag, const double* cs, const double* ts,
+                               const double* w, double* hn, double* wn, hipStream_t st);
+hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, double* z,
+                              hipStream_t st);
 
 // solve.hip ----------------------------------------------------------------
 template <typename T>

@@ -7,6 +7,9 @@ pressure, K iterations):
     column is one oracle reconstruction of a unit gradient), rel <= 1e-12;
   * central finite differences on 64 random unit directions in grad-p space
     at the masked Fourier-sphere workload (32^3), rel <= 1e-6;
+  * the source-restricted backward of the plain method (Horner steps over
+    the source voxels with the transposed source rows) against the
+    full-volume Horner sum, rel <= 1e-12, at 48^3 with 162 directions;
   * FP32-Kahan forward/backward within 1e-5 of FP64;
   * a torch.autograd.Function over the pair fits a gradient correction to
     512 sparse pressure probes with Tikhonov lambda = 1e-3 (50 steps).
@@ -22,12 +25,13 @@ pytestmark = pytest.mark.gpu
 
 
 class Adjoint:
-    def __init__(self, shape, dirs, delta, method, K, mask=None, precision=omni3d.FP64):
+    def __init__(self, shape, dirs, delta, method, K, mask=None, precision=omni3d.FP64,
+                 solver=omni3d.NAIVE):
         import torch
         self.shape, self.V = shape, int(np.prod(shape))
         self.dt = torch.float64 if precision == omni3d.FP64 else torch.float32
         self.ctx = omni3d.Context(shape, delta, method=method, max_iterations=K,
-                                  precision=precision, solver=omni3d.NAIVE)
+                                  precision=precision, solver=solver)
         self.ctx.set_directions(dirs)
         z = [np.zeros(shape)] * 3
         self.ctx.set_field(z, mask=mask)
@@ -56,13 +60,17 @@ def sphere(shape, r):
     return (np.sqrt((i - c[0]) ** 2 + (j - c[1]) ** 2 + (k - c[2]) ** 2) > r).astype(np.float64)
 
 
-@pytest.mark.parametrize("method", [omni3d.PLAIN, omni3d.PLAIN_MASK])
-def test_transpose_identity(gpu_available, ref_dirs2, method):
+SOLVERS = [(omni3d.PLAIN, omni3d.NAIVE), (omni3d.PLAIN, omni3d.AFFINE),
+           (omni3d.PLAIN_MASK, omni3d.NAIVE)]
+
+
+@pytest.mark.parametrize("method,solver", SOLVERS)
+def test_transpose_identity(gpu_available, ref_dirs2, method, solver):
     import torch
     shape = (16, 17, 18)
-    print(f"case adjoint transpose method={method}", flush=True)
+    print(f"case adjoint transpose method={method} solver={solver}", flush=True)
     m = sphere(shape, 4.0) if method == omni3d.PLAIN_MASK else None
-    A = Adjoint(shape, ref_dirs2, (0.3, 0.4, 0.5), method, 6, mask=m)
+    A = Adjoint(shape, ref_dirs2, (0.3, 0.4, 0.5), method, 6, mask=m, solver=solver)
     gen = torch.Generator(device="cuda").manual_seed(0)
     g = torch.randn(3 * A.V, dtype=torch.float64, device="cuda", generator=gen)
     y = torch.randn(A.V, dtype=torch.float64, device="cuda", generator=gen)
@@ -72,11 +80,11 @@ def test_transpose_identity(gpu_available, ref_dirs2, method):
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1e-300) * 10
 
 
-@pytest.mark.parametrize("method", [omni3d.PLAIN, omni3d.PLAIN_MASK])
-def test_against_oracle_jacobian(gpu_available, ref_dirs2, method):
+@pytest.mark.parametrize("method,solver", SOLVERS)
+def test_against_oracle_jacobian(gpu_available, ref_dirs2, method, solver):
     import torch
     shape = (5, 6, 7)
-    print(f"case adjoint vs oracle jacobian method={method}", flush=True)
+    print(f"case adjoint vs oracle jacobian method={method} solver={solver}", flush=True)
     V = int(np.prod(shape))
     delta, K = (0.5, 0.6, 0.7), 4
     m = None
@@ -94,7 +102,7 @@ def test_against_oracle_jacobian(gpu_available, ref_dirs2, method):
                                    method=O.PLAIN if m is None else O.PLAIN_MASK,
                                    max_iter=K, seed_pn=0)
         J[:, col] = p.reshape(-1)
-    A = Adjoint(shape, dirs, delta, method, K, mask=m)
+    A = Adjoint(shape, dirs, delta, method, K, mask=m, solver=solver)
     rng = np.random.default_rng(3)
     y = rng.normal(size=V)
     got = A.backward(torch.from_numpy(y).cuda()).cpu().numpy()
@@ -119,6 +127,50 @@ def test_finite_differences_fourier_sphere(gpu_available, ref_dirs3):
     # F is linear, so a central difference is exact up to rounding for any
     # step; the step is 1 % of |g0| so that the difference of two large
     # reconstructions does not cancel to the rounding level
+    eps = 1e-2 * float(torch.linalg.norm(g0))
+    worst = 0.0
+    for i in range(64):
+        d = torch.randn(3 * A.V, dtype=torch.float64, device="cuda", generator=gen)
+        d /= torch.linalg.norm(d)
+        fd = float(torch.dot(y, A.forward(g0 + eps * d) - A.forward(g0 - eps * d))) / (2 * eps)
+        an = float(torch.dot(dLdg, d))
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-12))
+    print("finite-difference worst rel error", worst)
+    assert worst < 1e-6
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 25])
+def test_source_restricted_backward(gpu_available, ref_dirs3, K):
+    import torch
+    n = 48
+    print(f"case adjoint source-restricted vs full-volume Horner K={K}", flush=True)
+    f = F.fourier_sphere(n, seed=3)
+    shape = (n, n, n)
+    full = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN, K)
+    src = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN, K, solver=omni3d.AFFINE)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    y = torch.randn(full.V, dtype=torch.float64, device="cuda", generator=gen)
+    a, b = src.backward(y), full.backward(y)
+    err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+    print("source-restricted vs full-volume rel-L2", err)
+    assert err < 1e-12
+    # a second call reuses the transposed rows kept with the plan (the final
+    # scatter adds in FP64 atomics, so the order of its last bits may vary)
+    again = float(torch.linalg.norm(src.backward(y) - a) / torch.linalg.norm(a))
+    assert again < 1e-13
+
+
+def test_finite_differences_plain_affine(gpu_available, ref_dirs3):
+    import torch
+    n = 32
+    print("case adjoint finite differences plain affine", flush=True)
+    f = F.fourier_sphere(n, seed=1)
+    shape = (n, n, n)
+    A = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN, 30, solver=omni3d.AFFINE)
+    g0 = torch.from_numpy(np.concatenate([c.reshape(-1) for c in f["grad"]])).cuda()
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    y = torch.randn(A.V, dtype=torch.float64, device="cuda", generator=gen)
+    dLdg = A.backward(y)
     eps = 1e-2 * float(torch.linalg.norm(g0))
     worst = 0.0
     for i in range(64):
