@@ -24,7 +24,9 @@ copy for scaling the FETCH_SIZE / WRITE_SIZE counters; --batch times the
 64-snapshot uneven-grid series (one hipGraph vs one launch sequence per
 snapshot); --adjoint times forward + backward of the 128^3 Fourier-sphere
 reconstruction with the plain method (source-restricted forward and
-backward), and once with the plain-mask method (a traversal per iteration).
+backward), and once with the plain-mask method (a traversal per iteration);
+--methods times one reference-schedule pass (a full traversal) of each
+integration method on the 128^3 masked Fourier-sphere field.
 """
 import argparse
 import json
@@ -141,6 +143,63 @@ def adjoint_mode(args):
                    "plain_mask_forward_s": round(mfw, 5), "plain_mask_backward_s": round(mbw, 5)}}))
 
 
+def methods_mode(args):
+    """One full traversal per iteration (the reference's schedule) for each
+    integration method, 128^3 masked Fourier sphere, 162 directions, FP32-Kahan."""
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    n = 128
+    shape = (n, n, n)
+    V = n ** 3
+    f = F.fourier_sphere(n, seed=0)
+    dirs = omni3d.generate_directions(3)
+    D = dirs.shape[0]
+    bytes_pass = D * V * 3 * 4 + V * 20
+    p_dev = torch.zeros(V, dtype=torch.float64, device="cuda")
+    passes = args.naive_passes
+    out = {}
+    thr = None
+    for name, method in (("plain", omni3d.PLAIN), ("plain-mask", omni3d.PLAIN_MASK),
+                         ("select", omni3d.SELECT), ("weighted", omni3d.WEIGHTED)):
+        kw = {}
+        if method == omni3d.SELECT:
+            kw["threshold"] = thr
+        ctx = omni3d.Context(shape, f["delta"], density=f["rho"], precision=omni3d.FP32_KAHAN,
+                             method=method, solver=omni3d.NAIVE, max_iterations=passes,
+                             tolerance=0.0, **kw)
+        ctx.set_directions(dirs)
+        ctx.set_field(f["acc"], mask=f["mask"])
+        if thr is None:
+            # Select threshold: the 80th percentile of |curl a| over the fluid
+            c = ctx.field_curl()
+            thr = float(np.quantile(c[f["mask"] > 0], 0.8))
+        ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
+        ms = []
+        for _ in range(max(1, args.steps)):
+            r = ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
+            ms.append(r.ms_iterate / passes)
+        torch.cuda.synchronize()
+        ctx.close()
+        m = float(np.mean(ms))
+        out[name] = {"ms_per_pass": round(m, 3),
+                     "frac": round(bytes_pass / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"bench: {name} {m:.2f} ms per pass")
+    first = out["select"]
+    print(json.dumps({
+        "metric": "ms per reference-schedule pass, 128^3 masked Fourier sphere",
+        "value": first["ms_per_pass"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": False, "dtype": "fp32-kahan ray sums, fp64 averaging",
+        "data": "synthetic",
+        "roofline": {"bound": "hbm", "achieved": round(bytes_pass / (first["ms_per_pass"] * 1e-3)
+                                                       / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"], "traffic": None,
+                     "kernel": "k_walk_naive<float, select>", "algorithmic_bytes": bytes_pass},
+        "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "
+                               f"one full traversal per iteration, {passes} iterations timed",
+                   "select_threshold": thr, "methods": out}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,6 +216,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--adjoint", action="store_true")
+    ap.add_argument("--methods", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -172,6 +232,8 @@ def main():
         return batch_mode(args)
     if args.adjoint:
         return adjoint_mode(args)
+    if args.methods:
+        return methods_mode(args)
 
     import omni3d
     from omni3d import fields as F

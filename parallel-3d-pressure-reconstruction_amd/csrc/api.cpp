@@ -832,26 +832,28 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   else
     OMNI_TRY_HIP(ctx, launch_fill_f64(pa, V, 0.0, st));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
+  // the weights depend on the field only: accumulated by the first iteration
+  if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+  if (!fixed) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
   record_event(ctx, 2);
   record_event(ctx, 8);
   record_event(ctx, 3);
   for (int k = 1; k <= P.max_iterations; k++) {
-    if (fixed) {
+    if (fixed)
       OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
-      if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
-    } else {
+    else
       OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
-      OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
-    }
+    const bool first = k == 1;
     OMNI_TRY_HIP(ctx, launch_walk_naive<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
                                            pl.g, wp, field, aux, pa, pb, k, sc,
                                            fixed ? ctx->sum.as<long long>() : nullptr,
-                                           m == OMNI3D_METHOD_PLAIN_MASK ? ctx->wcnt.as<unsigned int>() : nullptr,
+                                           (first && m == OMNI3D_METHOD_PLAIN_MASK)
+                                               ? ctx->wcnt.as<unsigned int>() : nullptr,
                                            fixed ? nullptr : ctx->sumd.as<double>(),
-                                           fixed ? nullptr : ctx->wsum.as<double>(),
-                                           ctx->srcf.as<uint8_t>(), st));
+                                           (first && !fixed) ? ctx->wsum.as<double>() : nullptr,
+                                           first ? ctx->srcf.as<uint8_t>() : nullptr, st));
     OMNI_TRY_HIP(ctx, launch_naive_update(k, V, fixed ? 1 : 0, fixed ? ctx->sum.as<long long>() : nullptr,
                                           cntw, fixed ? nullptr : ctx->sumd.as<double>(),
                                           fixed ? nullptr : ctx->wsum.as<double>(), pinit,

@@ -3,6 +3,10 @@
 // curl field of the Select / Weighted methods, and the per-iteration
 // averaging with the on-device stop rule.
 //
+// The weights (visit counts, curl weights) and the source flags depend on the
+// field only: the first iteration accumulates them and later ones pass null
+// pointers and add the pressure sums alone.
+//
 // One lane walks one ray.  Sums whose weights are whole visit counts (plain,
 // plain-mask) are rounded to 64-bit fixed point and added with integer
 // atomics, so they do not depend on the order lanes arrive in; the
@@ -76,7 +80,7 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
     const bool entry_valid = faithful ? !(ci == 0.0 || ci > thr) : (ci != 0.0 && ci < thr);
     if (entry_valid) {
       p0 = p[w.v];
-      src[w.v] = 1;
+      if (src) src[w.v] = 1;
       p0set = true;
     }
   }
@@ -92,7 +96,7 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
       const bool ok = (faithful && a != 0) ? (cn != 0.0) : valid;
       if (ok) {
         p0 = p[w.v];
-        src[w.v] = 1;
+        if (src) src[w.v] = 1;
         p0set = true;
         pint.reset();
         counttmp = 0.0;
@@ -103,12 +107,12 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
         pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
         if (kUnit) {
           fx_add(&sumfx[w.v], to_fixed(p0 + pint.value(), scale));
-          atomicAdd(&wcnt[w.v], 1u);
+          if (wcnt) atomicAdd(&wcnt[w.v], 1u);
         } else {
           curltmp += cn / 10000;
           counttmp += 1.0;
           add_f64(&sumd[w.v], (p0 + pint.value()) * 1 / curltmp * counttmp);
-          add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+          if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
         }
       }
       const bool enter_high = (faithful && a != 0) ? (!outthr && cn >= thr)
@@ -116,7 +120,7 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
       if (enter_high) outthr = true;
       if (outthr && valid) {
         p0 = p[w.v];
-        src[w.v] = 1;
+        if (src) src[w.v] = 1;
         pint.reset();
         outthr = false;
         if (!(faithful && a == 2)) {
@@ -145,7 +149,7 @@ __device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst*
   Walker w;
   w.init(rr, dirs, g);
   const double pin = p[w.v];
-  src[w.v] = 1;
+  if (src) src[w.v] = 1;
   double curltmp = 0.0, counttmp = 0.0;
   RaySum<T> pint;
   V4 cur = field[w.v];
@@ -159,7 +163,7 @@ __device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst*
     counttmp += 1.0;
     if (curltmp != 0.0) {
       add_f64(&sumd[w.v], (pin + pint.value()) * 1 / curltmp * counttmp);
-      add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+      if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
     }
     cur = nxt;
   }
