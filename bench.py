@@ -175,15 +175,20 @@ def methods_mode(args):
             c = ctx.field_curl()
             thr = float(np.quantile(c[f["mask"] > 0], 0.8))
         ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
-        ms = []
+        ms, setup = [], []
         for _ in range(max(1, args.steps)):
             r = ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
             ms.append(r.ms_iterate / passes)
+            setup.append(r.ms_segment_sums)
         torch.cuda.synchronize()
         ctx.close()
         m = float(np.mean(ms))
         out[name] = {"ms_per_pass": round(m, 3),
                      "frac": round(bytes_pass / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if method != omni3d.PLAIN:
+            # the per-field replay of the path-break rule (k_mask_walk), once
+            # per reconstruction
+            out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
         log(f"bench: {name} {m:.2f} ms per pass")
     first = out["select"]
     print(json.dumps({
@@ -194,9 +199,11 @@ def methods_mode(args):
         "roofline": {"bound": "hbm", "achieved": round(bytes_pass / (first["ms_per_pass"] * 1e-3)
                                                        / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"], "traffic": None,
-                     "kernel": "k_walk_naive<float, select>", "algorithmic_bytes": bytes_pass},
+                     "kernel": "k_mask_carry + k_tile_masked<float, select, 16^3, 1024> "
+                               "(one iteration)", "algorithmic_bytes": bytes_pass},
         "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "
-                               f"one full traversal per iteration, {passes} iterations timed",
+                               f"one full traversal per iteration, {passes} iterations timed; "
+                               f"plain: one lane per ray, masked methods: tile engine",
                    "select_threshold": thr, "methods": out}}))
 
 

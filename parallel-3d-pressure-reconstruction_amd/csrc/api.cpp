@@ -814,6 +814,21 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
     aux = ctx->aux.as<double>();
   }
   const bool fixed = m == OMNI3D_METHOD_PLAIN || m == OMNI3D_METHOD_PLAIN_MASK;
+  // the masked methods run on the tile engine (OMNI3D_MASKED=naive: one lane
+  // per ray with global atomics)
+  const char* menv = std::getenv("OMNI3D_MASKED");
+  const bool tiled = m != OMNI3D_METHOD_PLAIN && !(menv && std::strcmp(menv, "naive") == 0);
+  if (tiled) {
+    const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
+    const bool curlw = m != OMNI3D_METHOD_PLAIN_MASK;
+    OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_r.alloc(sizeof(uint32_t) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (curlw ? nv : 1)));
+    OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (curlw ? nv : 1)));
+  }
   OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
   OMNI_TRY_HIP(ctx, ctx->pb.alloc(sizeof(double) * V));
   OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
@@ -838,9 +853,42 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
   record_event(ctx, 2);
+  if (tiled)
+    OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                          pl.g, pl.tg, wp, field, aux,
+                                          pl.val_off.as<unsigned long long>(),
+                                          ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
+                                          ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
+                                          ctx->mk_wc.as<double>(), ctx->mk_wn.as<double>(),
+                                          m == OMNI3D_METHOD_PLAIN_MASK
+                                              ? ctx->wcnt.as<unsigned int>() : nullptr,
+                                          fixed ? nullptr : ctx->wsum.as<double>(),
+                                          ctx->srcf.as<uint8_t>(), st));
   record_event(ctx, 8);
   record_event(ctx, 3);
   for (int k = 1; k <= P.max_iterations; k++) {
+    if (tiled) {
+      OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
+                                          pl.val_off.as<unsigned long long>(),
+                                          ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
+                                          ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(), pa,
+                                          pb, k, sc, ctx->mk_c.as<double>(), st));
+      OMNI_TRY_HIP(ctx, launch_tile_masked<T>(m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                              pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field, aux,
+                                              ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                              ctx->mk_wn.as<double>(), ctx->mk_c.as<double>(), pa,
+                                              pb, k, sc,
+                                              fixed ? ctx->sum.as<long long>() : nullptr,
+                                              fixed ? nullptr : ctx->sumd.as<double>(), st));
+      OMNI_TRY_HIP(ctx, launch_naive_update(k, V, fixed ? 1 : 0,
+                                            fixed ? ctx->sum.as<long long>() : nullptr, cntw,
+                                            fixed ? nullptr : ctx->sumd.as<double>(),
+                                            fixed ? nullptr : ctx->wsum.as<double>(), pinit,
+                                            seed > 0 ? 1 : 0, nullptr, ctx->srcf.as<uint8_t>(),
+                                            pa, pb, sc, ctx->partials.as<double>(),
+                                            P.max_iterations, P.tolerance, st));
+      continue;
+    }
     if (fixed)
       OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
     else

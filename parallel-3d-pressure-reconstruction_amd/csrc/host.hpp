@@ -171,6 +171,9 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_y, adj_z, adj_u, adj_t, adj_g;
   // source-restricted backward: t_S, h (2), w (2), C_S over the nS sources
   omni3d::DevBuf adj_s;
+  // masked methods on the tile engine: per value slot the first-visit record
+  // (E, S, R, F, curl weights) of the field and the carries of an iteration
+  omni3d::DevBuf mk_e, mk_s, mk_r, mk_f, mk_wc, mk_wn, mk_c;
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
   hipEvent_t ev[9] = {};

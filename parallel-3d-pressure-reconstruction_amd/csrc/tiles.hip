@@ -23,11 +23,6 @@
 
 namespace omni3d {
 
-__device__ __forceinline__ int tile_of(int i, int j, int k, const TileGrid& tg) {
-  const int b = tg.bits;
-  return (i >> b) + (j >> b) * tg.ntx + (k >> b) * tg.ntx * tg.nty;
-}
-
 // ---------------------------------------------------------------------------
 // Plan kernels.  Visitor-style geometry walks (the same step rule as every
 // other walk) that count steps and segments, then write codes and records.
@@ -138,30 +133,6 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
 
 // ---------------------------------------------------------------------------
 // Tile passes.
-
-// LDS slot of in-tile voxel l = x + (y << b) + (z << 2b): x is XORed with
-// bits of y and z, so that voxels one step apart along y or z -- the
-// positions of neighbouring parallel rays at the same step -- fall in
-// different LDS banks (a bijection within every (y, z) row).
-template <int kBits>
-__device__ __forceinline__ int lds_slot(int l) {
-  const unsigned int u = (unsigned int)l;
-  return (int)(u ^ __builtin_amdgcn_ubfe(u, kBits + 1, kBits) ^
-               __builtin_amdgcn_ubfe(u, 2 * kBits, kBits));
-}
-
-// Trapezoid factor ((-rho * s) * delta_a) * 0.5 of a step along axis a with
-// sign s (up: s = +1) in the arithmetic of T.
-template <typename T>
-__device__ __forceinline__ T step_factor(const WalkParams& wp, int a, bool up);
-template <>
-__device__ __forceinline__ float step_factor<float>(const WalkParams& wp, int a, bool up) {
-  return up ? wp.hsf[a][1] : wp.hsf[a][0];
-}
-template <>
-__device__ __forceinline__ double step_factor<double>(const WalkParams& wp, int a, bool up) {
-  return up ? wp.hs[a][1] : wp.hs[a][0];
-}
 
 // LDS of a tile pass: the tile's gradient; the fixed-point counters (full
 // and source-sum passes); the source ids of the tile's voxels (source-sum

@@ -25,6 +25,36 @@ struct TileGrid {
   int lshift = 2;  // segment-length group of a segment: visits >> lshift
 };
 
+// tile of voxel (i, j, k)
+__device__ __forceinline__ int tile_of(int i, int j, int k, const TileGrid& tg) {
+  const int b = tg.bits;
+  return (i >> b) + (j >> b) * tg.ntx + (k >> b) * tg.ntx * tg.nty;
+}
+
+// LDS slot of in-tile voxel l = x + (y << b) + (z << 2b): x is XORed with
+// bits of y and z, so that voxels one step apart along y or z -- the
+// positions of neighbouring parallel rays at the same step -- fall in
+// different LDS banks (a bijection within every (y, z) row).
+template <int kBits>
+__device__ __forceinline__ int lds_slot(int l) {
+  const unsigned int u = (unsigned int)l;
+  return (int)(u ^ __builtin_amdgcn_ubfe(u, kBits + 1, kBits) ^
+               __builtin_amdgcn_ubfe(u, 2 * kBits, kBits));
+}
+
+// Trapezoid factor ((-rho * s) * delta_a) * 0.5 of a step along axis a with
+// sign s (up: s = +1) in the arithmetic of T.
+template <typename T>
+__device__ __forceinline__ T step_factor(const WalkParams& wp, int a, bool up);
+template <>
+__device__ __forceinline__ float step_factor<float>(const WalkParams& wp, int a, bool up) {
+  return up ? wp.hsf[a][1] : wp.hsf[a][0];
+}
+template <>
+__device__ __forceinline__ double step_factor<double>(const WalkParams& wp, int a, bool up) {
+  return up ? wp.hs[a][1] : wp.hs[a][0];
+}
+
 // One segment: the visits of one ray inside one tile, self-contained so that
 // a tile pass starts it with one coalesced load.  A monotone staircase visits
 // at most 16+15+15 = 46 voxels of a 16^3 tile, so its step codes fit in 96 bits.
@@ -86,5 +116,27 @@ hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
 hipError_t launch_scan_u32(const unsigned int* cnt, int64_t n, unsigned long long* out,
                            int words, void* tmp, size_t tmp_bytes, hipStream_t st);
 size_t scan_u32_tmp_bytes(int64_t n);
+
+
+// masked.hip: the masked methods' traversal per iteration (method 1
+// plain-mask, 2 Select, 3 Weighted as in the naive walk)
+template <typename T>
+hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                            GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
+                            const double* aux, const unsigned long long* val_off, double* E,
+                            double* S, uint32_t* R, uint32_t* F, double* wc, double* wn,
+                            unsigned int* wcnt, double* wsum, uint8_t* src, hipStream_t st);
+hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, const double* E, const double* S,
+                             const uint32_t* R, const uint32_t* F, const double* p0,
+                             const double* p1, int k, const DevScalars* sc, double* cval,
+                             hipStream_t st);
+template <typename T>
+hipError_t launch_tile_masked(int method, int ntiles, const unsigned long long* tile_ptr,
+                              const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                              const void* field, const double* aux, const uint32_t* F,
+                              const double* wc, const double* wn, const double* cval,
+                              const double* p0, const double* p1, int k, const DevScalars* sc,
+                              long long* sumfx, double* sumd, hipStream_t st);
 
 }  // namespace omni3d
