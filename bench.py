@@ -26,7 +26,9 @@ snapshot); --adjoint times forward + backward of the 128^3 Fourier-sphere
 reconstruction with the plain method (source-restricted forward and
 backward), and once with the plain-mask method (a traversal per iteration);
 --methods times one reference-schedule pass (a full traversal) of each
-integration method on the 128^3 masked Fourier-sphere field.
+integration method on the 128^3 masked Fourier-sphere field; --sor times the
+GPU red-black SOR (omega 1.8) on the 256^3 Neumann right-hand side of the
+Burgers-vortex field, beside the oracle's serial red-black SOR.
 """
 import argparse
 import json
@@ -207,6 +209,65 @@ def methods_mode(args):
                    "select_threshold": thr, "methods": out}}))
 
 
+SOR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sor256.json")
+
+
+def sor_mode(args):
+    """GPU red-black SOR on device buffers: 256^3, omega 1.8, fixed sweeps."""
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    n = 256
+    shape = (n, n, n)
+    V = n ** 3
+    f = F.burgers_vortex(n, noise=0.02, seed=0)
+    rhs = omni3d.calrhs(shape, f["delta"], f["rho"], f["acc"])
+    rhs -= rhs.mean()
+    rd = torch.from_numpy(rhs).cuda()
+    pd = torch.zeros(V, dtype=torch.float64, device="cuda")
+    iters = args.sor_iters
+    omni3d.sor_solve_device(shape, rd.data_ptr(), pd.data_ptr(), omega=1.8, max_iter=5)
+    pd.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it, res = omni3d.sor_solve_device(shape, rd.data_ptr(), pd.data_ptr(), omega=1.8,
+                                      max_iter=iters)
+    s_total = time.perf_counter() - t0
+    ms_it = s_total * 1e3 / iters
+    # one iteration (red then black half sweep) must read p and rhs and write
+    # p once: 3 x 8 B per voxel
+    bytes_it = 24 * V
+    achieved = bytes_it / (ms_it * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(SOR_TRAFFIC_FILE):
+        t = json.load(open(SOR_TRAFFIC_FILE))
+        if t.get("workload_n") == n:
+            traffic = t.get("traffic_bytes_per_iteration")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        k = 4
+        t0 = time.perf_counter()
+        O.sor_redblack(np.zeros(shape), rhs, k, 1.8)
+        ts = time.perf_counter() - t0
+        cpu = {"value": round(ts / k * 1e3, 2), "unit": "ms per iteration", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle serial red-black SOR, {k} iterations at {n}^3 took {ts:.2f}s"}
+    print(json.dumps({
+        "metric": "ms per red-black SOR iteration, 256^3", "value": round(ms_it, 4), "unit": "ms",
+        "n_gpus": 1, "steps": iters, "higher_is_better": False, "dtype": "f64",
+        "data": "synthetic",
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": "k_redblack_rows x 2 (red, black)",
+                     "algorithmic_bytes": bytes_it},
+        "cpu_baseline": cpu,
+        "config": {"workload": f"Neumann right-hand side of the burgers-vortex {n}^3 field, "
+                               f"omega 1.8, {iters} iterations from p = 0",
+                   "residual": res, "iterations": it}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +285,8 @@ def main():
     ap.add_argument("--batch", action="store_true")
     ap.add_argument("--adjoint", action="store_true")
     ap.add_argument("--methods", action="store_true")
+    ap.add_argument("--sor", action="store_true")
+    ap.add_argument("--sor-iters", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -241,6 +304,8 @@ def main():
         return adjoint_mode(args)
     if args.methods:
         return methods_mode(args)
+    if args.sor:
+        return sor_mode(args)
 
     import omni3d
     from omni3d import fields as F

@@ -196,6 +196,12 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d);
  * buffers, double.  *iterations_out / *residual_out may be NULL. */
 int omni3d_sor_solve(int X, int Y, int Z, double omega, int max_iter, double eps,
                      const double* rhs, double* p, int* iterations_out, double* residual_out);
+/* The same solver on device buffers (double, resident in HBM), queued on
+ * hip_stream (NULL = the null stream); returns after the last sweep.  No
+ * reference counterpart: for callers that keep the field on the GPU. */
+int omni3d_sor_solve_device(int X, int Y, int Z, double omega, int max_iter, double eps,
+                            const double* rhs_d, double* p_d, void* hip_stream,
+                            int* iterations_out, double* residual_out);
 
 /* The reference's CPU entry points, same signatures (sor3D.h:5, calRHS.h:4),
  * executed on the GPU with host buffers.  sor3D: damped Jacobi (weight 0.8)

@@ -118,6 +118,26 @@ k_redblack(int X, int Y, int Z, int color, double omega, double* __restrict__ p,
   }
 }
 
+// The same half sweep with one thread per voxel of the colour: thread
+// (h, j, k) updates i = 2h + ((j + k + color) & 1), so no lane idles and no
+// index is divided.  Blocks of 64 x 4 threads: a wavefront covers 128
+// consecutive voxels of one row.
+__global__ void __launch_bounds__(256)
+k_redblack_rows(int X, int Y, int Z, int color, double omega, double* __restrict__ p,
+                const double* __restrict__ rhs, const SorState* __restrict__ st) {
+  if (st->stop_at) return;
+  const int j = (int)(blockIdx.y * 4 + threadIdx.y);
+  const int k = (int)blockIdx.z;
+  const int i = 2 * (int)(blockIdx.x * 64 + threadIdx.x) + ((j + k + color) & 1);
+  if (i >= X || j >= Y) return;
+  const int64_t v = (int64_t)i + (int64_t)j * X + (int64_t)k * X * Y;
+  int cn;
+  double s = nb_sum(X, Y, Z, p, i, j, k, v, &cn);
+  s += rhs[v];
+  s = s / cn;
+  p[v] = p[v] + omega * (s - p[v]);
+}
+
 // Interior residual sum |sum_nb - 6 p + rhs| (sor3D.cpp:741-781) reduced
 // over the grid; the last block applies the stop rule of sweep / iteration
 // `it`: residue = sum / X / Y / Z, stop when residue < eps.  fp32 rounds the
@@ -325,32 +345,56 @@ int omni3d_sor_solve(int X, int Y, int Z, double omega, int max_iter, double eps
                      const double* rhs, double* p, int* iterations_out, double* residual_out) {
   if (!grid_ok(X, Y, Z)) return fail(nullptr, OMNI3D_ERR_ARG, "bad grid dimensions");
   if (!rhs || !p) return fail(nullptr, OMNI3D_ERR_ARG, "null pointer");
-  if (!(omega > 0.0 && omega < 2.0)) return fail(nullptr, OMNI3D_ERR_ARG, "omega must be in (0, 2)");
-  if (max_iter < 1) return fail(nullptr, OMNI3D_ERR_ARG, "max_iter must be >= 1");
   const int64_t V = (int64_t)X * Y * Z;
   const size_t b = sizeof(double) * V;
+  DevBuf dp, drhs;
+  POIS_TRY(dp.alloc(b));
+  POIS_TRY(drhs.alloc(b));
+  POIS_TRY(hipMemcpy(dp.as<void>(), p, b, hipMemcpyHostToDevice));
+  POIS_TRY(hipMemcpy(drhs.as<void>(), rhs, b, hipMemcpyHostToDevice));
+  const int rc = omni3d_sor_solve_device(X, Y, Z, omega, max_iter, eps, drhs.as<double>(),
+                                         dp.as<double>(), nullptr, iterations_out, residual_out);
+  if (rc) return rc;
+  POIS_TRY(hipMemcpy(p, dp.as<void>(), b, hipMemcpyDeviceToHost));
+  return OMNI3D_OK;
+}
+
+int omni3d_sor_solve_device(int X, int Y, int Z, double omega, int max_iter, double eps,
+                            const double* rhs_d, double* p_d, void* hip_stream,
+                            int* iterations_out, double* residual_out) {
+  if (!grid_ok(X, Y, Z)) return fail(nullptr, OMNI3D_ERR_ARG, "bad grid dimensions");
+  if (!rhs_d || !p_d) return fail(nullptr, OMNI3D_ERR_ARG, "null pointer");
+  if (!(omega > 0.0 && omega < 2.0)) return fail(nullptr, OMNI3D_ERR_ARG, "omega must be in (0, 2)");
+  if (max_iter < 1) return fail(nullptr, OMNI3D_ERR_ARG, "max_iter must be >= 1");
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  const int64_t V = (int64_t)X * Y * Z;
   PoissonBufs B;
-  for (int i = 0; i < 2; i++) POIS_TRY(B.a[i].alloc(b));
   POIS_TRY(B.state.alloc(sizeof(SorState)));
   POIS_TRY(B.partials.alloc(sizeof(double) * res_grid(X, Y, Z)));
-  double* dp = B.a[0].as<double>();
-  double* drhs = B.a[1].as<double>();
   SorState* st = B.state.as<SorState>();
-  POIS_TRY(hipMemcpy(dp, p, b, hipMemcpyHostToDevice));
-  POIS_TRY(hipMemcpy(drhs, rhs, b, hipMemcpyHostToDevice));
-  POIS_TRY(hipMemset(st, 0, sizeof(SorState)));
+  POIS_TRY(hipMemsetAsync(st, 0, sizeof(SorState), s));
+  // row-parallel half sweeps where the grid fits the launch limits
+  const unsigned int nh = (unsigned int)((X + 1) / 2);
+  const bool rows = (Y + 3) / 4 <= 65535 && Z <= 65535;
+  const dim3 rgrid((nh + 63) / 64, (unsigned int)((Y + 3) / 4), (unsigned int)Z), rblock(64, 4);
   const unsigned int g = grid_for(V);
   for (int it = 1; it <= max_iter; it++) {
-    hipLaunchKernelGGL(k_redblack, dim3(g), dim3(256), 0, nullptr, X, Y, Z, 0, omega, dp, drhs, st);
-    hipLaunchKernelGGL(k_redblack, dim3(g), dim3(256), 0, nullptr, X, Y, Z, 1, omega, dp, drhs, st);
+    for (int color = 0; color < 2; color++) {
+      if (rows)
+        hipLaunchKernelGGL(k_redblack_rows, rgrid, rblock, 0, s, X, Y, Z, color, omega, p_d, rhs_d,
+                           st);
+      else
+        hipLaunchKernelGGL(k_redblack, dim3(g), dim3(256), 0, s, X, Y, Z, color, omega, p_d,
+                           rhs_d, st);
+    }
     if (eps > 0.0 || it == max_iter)
-      hipLaunchKernelGGL(k_residual, dim3(res_grid(X, Y, Z)), dim3(kResBlock), 0, nullptr, X, Y, Z,
-                         dp, drhs, it, eps, 0, B.partials.as<double>(), st);
+      hipLaunchKernelGGL(k_residual, dim3(res_grid(X, Y, Z)), dim3(kResBlock), 0, s, X, Y, Z,
+                         p_d, rhs_d, it, eps, 0, B.partials.as<double>(), st);
     POIS_TRY(hipGetLastError());
   }
   SorState hs;
-  POIS_TRY(hipMemcpy(&hs, st, sizeof(hs), hipMemcpyDeviceToHost));
-  POIS_TRY(hipMemcpy(p, dp, b, hipMemcpyDeviceToHost));
+  POIS_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+  POIS_TRY(hipStreamSynchronize(s));
   if (iterations_out) *iterations_out = hs.stop_at ? hs.stop_at : max_iter;
   if (residual_out) *residual_out = hs.residual;
   return OMNI3D_OK;

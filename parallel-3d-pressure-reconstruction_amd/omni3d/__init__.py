@@ -27,7 +27,7 @@ EXPORTS = [
     "omni3d_load_directions_file", "omni3d_generate_directions", "omni3d_set_field",
     "omni3d_reconstruct", "omni3d_es_inner", "omni3d_es_inner_weighted",
     "omni3d_es_inner_select", "omni3d_devide_count_inner", "omni3d_curl", "omni3d_forward",
-    "omni3d_backward", "omni3d_sor_solve", "sor3D", "calRHS", "omni3d_sor3d_ex",
+    "omni3d_backward", "omni3d_sor_solve", "omni3d_sor_solve_device", "sor3D", "calRHS", "omni3d_sor3d_ex",
     "omni3d_calrhs", "omni3d_run_parameter_file", "omni3d_set_tile_shape",
     "omni3d_batch_create", "omni3d_batch_destroy", "omni3d_batch_last_error",
     "omni3d_batch_set_directions", "omni3d_batch_set_field", "omni3d_batch_tune",
@@ -117,6 +117,7 @@ def _declare(L):
     L.omni3d_forward.argtypes = [vp, vp, vp]
     L.omni3d_backward.argtypes = [vp, vp, vp]
     L.omni3d_sor_solve.argtypes = [i, i, i, d, i, d, dp, dp, C.POINTER(C.c_int), dp]
+    L.omni3d_sor_solve_device.argtypes = [i, i, i, d, i, d, vp, vp, vp, C.POINTER(C.c_int), dp]
     L.sor3D.argtypes = [i, i, i, i, C.POINTER(C.c_int), dp, dp, dp, dp, dp, dp, d]
     L.sor3D.restype = None
     L.calRHS.argtypes = [i, i, i, d, d, d, d, dp, dp, dp, dp]
@@ -200,6 +201,17 @@ def sor_solve(rhs, p0=None, omega=1.8, max_iter=1000, eps=0.0):
     check(lib().omni3d_sor_solve(X, Y, Z, omega, max_iter, eps, _dptr(rhs), _dptr(p),
                                  C.byref(it), C.byref(res)))
     return p, it.value, res.value
+
+
+def sor_solve_device(shape, rhs_ptr, p_ptr, omega=1.8, max_iter=1000, eps=0.0, stream=None):
+    """Red-black SOR on device buffers (double, z-major shape); returns
+    (iterations, residual)."""
+    Z, Y, X = shape
+    it = C.c_int(0)
+    res = C.c_double(0)
+    check(lib().omni3d_sor_solve_device(X, Y, Z, omega, max_iter, eps, rhs_ptr, p_ptr, stream,
+                                        C.byref(it), C.byref(res)))
+    return it.value, res.value
 
 
 def read_parameter_file(path):
