@@ -260,7 +260,7 @@ def sor_mode(args):
         "data": "synthetic",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_redblack_rows x 2 (red, black)",
+                     "traffic": traffic, "kernel": "k_redblack_fused<64, 8, 32, 256> (red and black in one pass)",
                      "algorithmic_bytes": bytes_it},
         "cpu_baseline": cpu,
         "config": {"workload": f"Neumann right-hand side of the burgers-vortex {n}^3 field, "

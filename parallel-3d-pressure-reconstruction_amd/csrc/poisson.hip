@@ -11,7 +11,9 @@
 // sum, which moves the eps stop only when the residual ties with eps).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -25,8 +27,6 @@ struct SorState {
   int stop_at;              // sweep (Jacobi) / iteration (red-black) that met eps, 0 = none
   int done;                 // sweeps / iterations executed
   double residual;          // last residual
-  unsigned int ticket;      // last-block counter
-  unsigned int pad;
 };
 
 __device__ __forceinline__ double nb_sum(int X, int Y, int Z, const double* __restrict__ p,
@@ -138,50 +138,202 @@ k_redblack_rows(int X, int Y, int Z, int color, double omega, double* __restrict
   p[v] = p[v] + omega * (s - p[v]);
 }
 
+// One whole red-black iteration in one pass: a workgroup owns a TX x TY
+// column of the grid over a chunk of z planes and marches up it with a ring
+// of planes of p (the column plus a two-voxel halo) in LDS.  At plane k it
+// takes the red half sweep of plane k (over the column plus a one-voxel
+// halo: red needs only old black values, which the halo of two holds), and
+// the black half sweep of plane k - 1 (its red neighbours on planes k - 2 ..
+// k are new by then); plane k - 1 is stored one step later.  The plane two
+// steps ahead and the next step's right-hand sides load into registers while
+// a step computes.  Reads come from p_in and writes go to p_out, so
+// workgroups never see each other's new values; each voxel's update is the
+// same expression on the same values as in the half-sweep kernels.
+template <int kRbTX, int kRbTY, int kRbZ, int kRbThreads>
+__global__ void __launch_bounds__(kRbThreads)
+k_redblack_fused(int X, int Y, int Z, double omega, const double* __restrict__ pin,
+                 const double* __restrict__ rhs, double* __restrict__ pout,
+                 const SorState* __restrict__ st) {
+  if (st->stop_at) return;
+  constexpr int kRbW = kRbTX + 4, kRbH = kRbTY + 4;
+  constexpr int kHalfX = kRbTX / 2 + 1;  // red voxels per row of the column plus halo
+  constexpr int kLoads = (kRbW * kRbH + kRbThreads - 1) / kRbThreads;  // plane values per thread
+  constexpr int kRed = (kHalfX * (kRbTY + 2) + kRbThreads - 1) / kRbThreads;
+  constexpr int kBlack = (kRbTX / 2 * kRbTY + kRbThreads - 1) / kRbThreads;
+  __shared__ double P[5][kRbH][kRbW];
+  const int x0 = (int)blockIdx.x * kRbTX, y0 = (int)blockIdx.y * kRbTY;
+  const int z0 = (int)blockIdx.z * kRbZ, z1 = min(Z, z0 + kRbZ);
+  const int64_t sy = X, sz = (int64_t)X * Y;
+  const int tid = (int)threadIdx.x;
+  auto slot = [](int k) { return (k + 5) % 5; };  // k >= -2
+  // the plane two steps ahead and the right-hand sides of the next step are
+  // loaded into registers while the current step computes
+  double pf[kLoads], rr[kRed], rb[kBlack];
+  auto issue = [&](int k) {
+    if (k < 0 || k >= Z) return;
+#pragma unroll
+    for (int m = 0; m < kLoads; m++) {
+      const int l = tid + m * kRbThreads;
+      const int i = x0 - 2 + l % kRbW, j = y0 - 2 + l / kRbW;
+      if (l < kRbW * kRbH && i >= 0 && i < X && j >= 0 && j < Y) pf[m] = pin[i + j * sy + k * sz];
+    }
+  };
+  auto commit = [&](int k) {
+    if (k < 0 || k >= Z) return;
+#pragma unroll
+    for (int m = 0; m < kLoads; m++) {
+      const int l = tid + m * kRbThreads;
+      const int i = x0 - 2 + l % kRbW, j = y0 - 2 + l / kRbW;
+      if (l < kRbW * kRbH && i >= 0 && i < X && j >= 0 && j < Y) P[slot(k)][l / kRbW][l % kRbW] = pf[m];
+    }
+  };
+  // red voxels ((i + j + k) even) of the column plus a one-voxel halo:
+  // TX / 2 + 1 per row of TX + 2; black voxels of the column: TX / 2 per row
+  auto red_at = [&](int l, int k, int* i, int* j) {
+    *j = y0 - 1 + l / kHalfX;
+    *i = x0 - 1 + ((x0 - 1 + *j + k) & 1) + 2 * (l % kHalfX);
+    return l < kHalfX * (kRbTY + 2) && *i >= 0 && *i < X && *j >= 0 && *j < Y;
+  };
+  auto black_at = [&](int l, int k, int* i, int* j) {
+    *j = y0 + l / (kRbTX / 2);
+    *i = x0 + ((x0 + *j + k + 1) & 1) + 2 * (l % (kRbTX / 2));
+    return l < kRbTX / 2 * kRbTY && *i < X && *j < Y;
+  };
+  auto issue_rhs = [&](int kr, int kb) {
+#pragma unroll
+    for (int m = 0; m < kRed; m++) {
+      int i, j;
+      if (kr >= 0 && kr < Z && red_at(tid + m * kRbThreads, kr, &i, &j)) rr[m] = rhs[i + j * sy + kr * sz];
+    }
+#pragma unroll
+    for (int m = 0; m < kBlack; m++) {
+      int i, j;
+      if (kb >= z0 && kb < z1 && black_at(tid + m * kRbThreads, kb, &i, &j))
+        rb[m] = rhs[i + j * sy + kb * sz];
+    }
+  };
+  // in-box neighbour sum in the reference's order (x-, x+, y-, y+, z-, z+)
+  auto update = [&](int i, int j, int k, double r) {
+    const int xx = i - x0 + 2, yy = j - y0 + 2;
+    const double(*Pc)[kRbW] = P[slot(k)];
+    double s = 0.0;
+    int n = 0;
+    if (i > 0) { s = n ? s + Pc[yy][xx - 1] : Pc[yy][xx - 1]; n++; }
+    if (i < X - 1) { s = n ? s + Pc[yy][xx + 1] : Pc[yy][xx + 1]; n++; }
+    if (j > 0) { s = n ? s + Pc[yy - 1][xx] : Pc[yy - 1][xx]; n++; }
+    if (j < Y - 1) { s = n ? s + Pc[yy + 1][xx] : Pc[yy + 1][xx]; n++; }
+    if (k > 0) { const double q = P[slot(k - 1)][yy][xx]; s = n ? s + q : q; n++; }
+    if (k < Z - 1) { const double q = P[slot(k + 1)][yy][xx]; s = n ? s + q : q; n++; }
+    s += r;
+    s = s / n;
+    const double old = Pc[yy][xx];
+    P[slot(k)][yy][xx] = old + omega * (s - old);
+  };
+  issue(z0 - 2);
+  commit(z0 - 2);
+  issue(z0 - 1);
+  commit(z0 - 1);
+  issue(z0);
+  issue_rhs(z0 - 1, z0 - 2);
+  // step k: red half sweep of plane k, store of plane k - 2 (finished by the
+  // previous step), black half sweep of plane k - 1; a ring of five planes
+  // keeps every plane a step reads or stores clear of the commit two steps on
+  for (int k = z0 - 1; k <= z1 + 1; k++) {
+    if (k + 1 <= z1 + 1) commit(k + 1);
+    if (k + 2 <= z1 + 1) issue(k + 2);
+    double r0[kRed], b0[kBlack];
+#pragma unroll
+    for (int m = 0; m < kRed; m++) r0[m] = rr[m];
+#pragma unroll
+    for (int m = 0; m < kBlack; m++) b0[m] = rb[m];
+    issue_rhs(k + 1, k);
+    __syncthreads();
+    if (k >= 0 && k < Z && k <= z1) {
+#pragma unroll
+      for (int m = 0; m < kRed; m++) {
+        int i, j;
+        if (red_at(tid + m * kRbThreads, k, &i, &j)) update(i, j, k, r0[m]);
+      }
+    }
+    const int ks = k - 2;
+    if (ks >= z0 && ks < z1) {
+      const double(*Pb)[kRbW] = P[slot(ks)];
+      for (int l = tid; l < kRbTX * kRbTY; l += kRbThreads) {
+        const int i = x0 + (l % kRbTX), j = y0 + l / kRbTX;
+        if (i < X && j < Y) pout[i + j * sy + ks * sz] = Pb[j - y0 + 2][i - x0 + 2];
+      }
+    }
+    __syncthreads();
+    const int kb = k - 1;
+    if (kb >= z0 && kb < z1) {
+#pragma unroll
+      for (int m = 0; m < kBlack; m++) {
+        int i, j;
+        if (black_at(tid + m * kRbThreads, kb, &i, &j)) update(i, j, kb, b0[m]);
+      }
+    }
+  }
+}
+
 // Interior residual sum |sum_nb - 6 p + rhs| (sor3D.cpp:741-781) reduced
 // over the grid; the last block applies the stop rule of sweep / iteration
 // `it`: residue = sum / X / Y / Z, stop when residue < eps.  fp32 rounds the
 // residue to float as the reference's accumulator does.
 constexpr int kResBlock = 256;
+constexpr int kResPlanes = 4;
 
+// Blocks of 64 x 4 threads over the interior (i, j), each thread over
+// kResPlanes consecutive planes; each block writes its partial sum.  k_residual_finish then
+// adds the partials in a fixed order (each thread a fixed strided subset,
+// then a fixed tree) and applies the stop rule.
 __global__ void __launch_bounds__(kResBlock)
 k_residual(int X, int Y, int Z, const double* __restrict__ p, const double* __restrict__ rhs,
-           int it, double eps, int fp32, double* __restrict__ partials, SorState* __restrict__ st) {
+           double* __restrict__ partials, const SorState* __restrict__ st) {
   if (st->stop_at) return;
   __shared__ double s_part[kResBlock / 64];
-  __shared__ int s_last;
-  const int nx = X - 2, ny = Y - 2, nz = Z - 2;
-  const int64_t n = (nx > 0 && ny > 0 && nz > 0) ? (int64_t)nx * ny * nz : 0;
+  const int i = 1 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int j = 1 + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int tid = (int)(threadIdx.y * 64 + threadIdx.x);
   double acc = 0.0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int i = 1 + (int)(t % nx), j = 1 + (int)((t / nx) % ny), k = 1 + (int)(t / ((int64_t)nx * ny));
-    const int64_t v = (int64_t)i + (int64_t)j * X + (int64_t)k * X * Y;
-    int cn;
-    double tmp = nb_sum(X, Y, Z, p, i, j, k, v, &cn);
-    tmp += -6.0 * p[v];
-    tmp += rhs[v];
-    acc += fabs(tmp);
+#pragma unroll
+  for (int q = 0; q < kResPlanes; q++) {
+    const int k = 1 + (int)blockIdx.z * kResPlanes + q;
+    if (i < X - 1 && j < Y - 1 && k < Z - 1) {
+      const int64_t v = (int64_t)i + (int64_t)j * X + (int64_t)k * X * Y;
+      int cn;
+      double tmp = nb_sum(X, Y, Z, p, i, j, k, v, &cn);
+      tmp += -6.0 * p[v];
+      tmp += rhs[v];
+      acc += fabs(tmp);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
   if (lane == 0) s_part[wave] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double a = 0.0;
     for (int w = 0; w < kResBlock / 64; w++) a += s_part[w];
-    partials[blockIdx.x] = a;
-    __threadfence();
-    const unsigned int t =
-        __hip_atomic_fetch_add(&st->ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == gridDim.x - 1);
+    partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = a;
   }
-  __syncthreads();
-  if (!s_last || threadIdx.x != 0) return;
-  __threadfence();
+}
+
+// One workgroup: residue = sum / X / Y / Z (fp32: rounded to float as the
+// reference's accumulator is), stop when residue < eps.
+__global__ void __launch_bounds__(kResBlock)
+k_residual_finish(int X, int Y, int Z, int64_t nblk, const double* __restrict__ partials, int it,
+                  double eps, int fp32, SorState* __restrict__ st) {
+  if (st->stop_at) return;
+  __shared__ double s_part[kResBlock / 64];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double a = 0.0;
-  for (unsigned int b = 0; b < gridDim.x; b++)
-    a += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int64_t q = tid; q < nblk; q += kResBlock) a += partials[q];
+  for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off);
+  if (lane == 0) s_part[wave] = a;
+  __syncthreads();
+  if (tid != 0) return;
+  a = 0.0;
+  for (int w = 0; w < kResBlock / 64; w++) a += s_part[w];
   double res;
   if (fp32) {
     float r = (float)a;
@@ -193,7 +345,6 @@ k_residual(int X, int Y, int Z, const double* __restrict__ p, const double* __re
   st->residual = res;
   st->done = it;
   if (res < eps) st->stop_at = it;
-  st->ticket = 0;
 }
 
 // After Jacobi sweep `it`: if it met eps, p = pn over the whole grid
@@ -235,7 +386,24 @@ struct PoissonBufs {
 
 bool grid_ok(int X, int Y, int Z) { return X >= 1 && Y >= 1 && Z >= 1 && (int64_t)X * Y * Z < (1ll << 40); }
 
-unsigned int res_grid(int X, int Y, int Z) { return grid_for((int64_t)X * Y * Z, 1024); }
+// residual launch: 64 x 4 blocks over the interior (i, j), kResPlanes planes per block
+dim3 res_grid(int X, int Y, int Z) {
+  return dim3((unsigned int)((std::max(X - 2, 1) + 63) / 64),
+              (unsigned int)((std::max(Y - 2, 1) + 3) / 4),
+              (unsigned int)((std::max(Z - 2, 1) + kResPlanes - 1) / kResPlanes));
+}
+size_t res_blocks(int X, int Y, int Z) {
+  const dim3 g = res_grid(X, Y, Z);
+  return (size_t)g.x * g.y * g.z;
+}
+
+void launch_residual(int X, int Y, int Z, const double* p, const double* rhs, int it, double eps,
+                     int fp32, double* partials, SorState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual, res_grid(X, Y, Z), dim3(64, 4), 0, s, X, Y, Z, p, rhs, partials,
+                     st);
+  hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(kResBlock), 0, s, X, Y, Z,
+                     (int64_t)res_blocks(X, Y, Z), partials, it, eps, fp32, st);
+}
 
 #define POIS_TRY(expr)                                                          \
   do {                                                                          \
@@ -276,7 +444,7 @@ int sor3d_impl(int X, int Y, int Z, int NoItr, const int* bc, double* p, double*
   PoissonBufs B;
   for (int i = 0; i < 3; i++) POIS_TRY(B.a[i].alloc(b));
   POIS_TRY(B.state.alloc(sizeof(SorState)));
-  POIS_TRY(B.partials.alloc(sizeof(double) * res_grid(X, Y, Z)));
+  POIS_TRY(B.partials.alloc(sizeof(double) * res_blocks(X, Y, Z)));
   double* dp = B.a[0].as<double>();
   double* dpn = B.a[1].as<double>();
   double* drhs = B.a[2].as<double>();
@@ -297,8 +465,7 @@ int sor3d_impl(int X, int Y, int Z, int NoItr, const int* bc, double* p, double*
       if (!empty)
         hipLaunchKernelGGL(k_jacobi, dim3(grid_for(nr)), dim3(256), 0, nullptr, X, Y, Z, i0, ie,
                            j0, je, k0, ke, dp, dpn, drhs, st);
-      hipLaunchKernelGGL(k_residual, dim3(res_grid(X, Y, Z)), dim3(kResBlock), 0, nullptr, X, Y,
-                         Z, dpn, drhs, sweep, eps, 1, B.partials.as<double>(), st);
+      launch_residual(X, Y, Z, dpn, drhs, sweep, eps, 1, B.partials.as<double>(), st, nullptr);
       hipLaunchKernelGGL(k_jacobi_commit, dim3(grid_for(std::max<int64_t>(nr, V))), dim3(256), 0,
                          nullptr, X, Y, Z, i0, empty ? i0 : ie, j0, je, k0, ke, sweep, dp, dpn, st);
       POIS_TRY(hipGetLastError());
@@ -370,32 +537,59 @@ int omni3d_sor_solve_device(int X, int Y, int Z, double omega, int max_iter, dou
   const int64_t V = (int64_t)X * Y * Z;
   PoissonBufs B;
   POIS_TRY(B.state.alloc(sizeof(SorState)));
-  POIS_TRY(B.partials.alloc(sizeof(double) * res_grid(X, Y, Z)));
+  POIS_TRY(B.partials.alloc(sizeof(double) * res_blocks(X, Y, Z)));
   SorState* st = B.state.as<SorState>();
   POIS_TRY(hipMemsetAsync(st, 0, sizeof(SorState), s));
-  // row-parallel half sweeps where the grid fits the launch limits
+  // default: one fused pass per iteration (p_d and a second buffer in
+  // turn); OMNI3D_SOR=halves: two row-parallel half sweeps in place
+  const char* env = std::getenv("OMNI3D_SOR");
+  // column 64 x 8 over 32 planes, 256 threads (fastest of 64 x {8, 16} x
+  // {16, 32} planes x {256, 512} threads at 256^3)
+  constexpr int tx = 64, ty = 8, tz = 32;
+  const bool fused = !(env && std::strcmp(env, "halves") == 0) &&
+                     (Z + tz - 1) / tz <= 65535 && (Y + ty - 1) / ty <= 65535;
+  double* buf[2] = {p_d, nullptr};
+  if (fused) {
+    POIS_TRY(B.a[0].alloc(sizeof(double) * V));
+    buf[1] = B.a[0].as<double>();
+  }
+  const dim3 fgrid((unsigned int)((X + tx - 1) / tx), (unsigned int)((Y + ty - 1) / ty),
+                   (unsigned int)((Z + tz - 1) / tz));
   const unsigned int nh = (unsigned int)((X + 1) / 2);
   const bool rows = (Y + 3) / 4 <= 65535 && Z <= 65535;
   const dim3 rgrid((nh + 63) / 64, (unsigned int)((Y + 3) / 4), (unsigned int)Z), rblock(64, 4);
   const unsigned int g = grid_for(V);
   for (int it = 1; it <= max_iter; it++) {
-    for (int color = 0; color < 2; color++) {
-      if (rows)
-        hipLaunchKernelGGL(k_redblack_rows, rgrid, rblock, 0, s, X, Y, Z, color, omega, p_d, rhs_d,
-                           st);
-      else
-        hipLaunchKernelGGL(k_redblack, dim3(g), dim3(256), 0, s, X, Y, Z, color, omega, p_d,
-                           rhs_d, st);
+    double* cur = p_d;
+    if (fused) {
+      cur = buf[it & 1];
+      const double* prev = buf[(it - 1) & 1];
+      hipLaunchKernelGGL((k_redblack_fused<tx, ty, tz, 256>), fgrid, dim3(256), 0, s, X, Y, Z,
+                         omega, prev, rhs_d, cur, st);
+    } else {
+      for (int color = 0; color < 2; color++) {
+        if (rows)
+          hipLaunchKernelGGL(k_redblack_rows, rgrid, rblock, 0, s, X, Y, Z, color, omega, p_d,
+                             rhs_d, st);
+        else
+          hipLaunchKernelGGL(k_redblack, dim3(g), dim3(256), 0, s, X, Y, Z, color, omega, p_d,
+                             rhs_d, st);
+      }
     }
     if (eps > 0.0 || it == max_iter)
-      hipLaunchKernelGGL(k_residual, dim3(res_grid(X, Y, Z)), dim3(kResBlock), 0, s, X, Y, Z,
-                         p_d, rhs_d, it, eps, 0, B.partials.as<double>(), st);
+      launch_residual(X, Y, Z, cur, rhs_d, it, eps, 0, B.partials.as<double>(), st, s);
     POIS_TRY(hipGetLastError());
   }
   SorState hs;
   POIS_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
   POIS_TRY(hipStreamSynchronize(s));
-  if (iterations_out) *iterations_out = hs.stop_at ? hs.stop_at : max_iter;
+  const int last = hs.stop_at ? hs.stop_at : max_iter;
+  if (fused && (last & 1)) {
+    // the last iteration wrote the second buffer
+    POIS_TRY(hipMemcpyAsync(p_d, buf[1], sizeof(double) * V, hipMemcpyDeviceToDevice, s));
+    POIS_TRY(hipStreamSynchronize(s));
+  }
+  if (iterations_out) *iterations_out = last;
   if (residual_out) *residual_out = hs.residual;
   return OMNI3D_OK;
 }
