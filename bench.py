@@ -193,6 +193,12 @@ def methods_mode(args):
             out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
         log(f"bench: {name} {m:.2f} ms per pass")
     first = out["select"]
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_masked_fourier128.json")
+    if os.path.exists(tf):
+        t = json.load(open(tf))
+        if t.get("workload_n") == n and t.get("directions") == D:
+            traffic = t.get("traffic_bytes", {}).get("select")
     print(json.dumps({
         "metric": "ms per reference-schedule pass, 128^3 masked Fourier sphere",
         "value": first["ms_per_pass"], "unit": "ms", "n_gpus": 1, "steps": args.steps,
@@ -200,7 +206,7 @@ def methods_mode(args):
         "data": "synthetic",
         "roofline": {"bound": "hbm", "achieved": round(bytes_pass / (first["ms_per_pass"] * 1e-3)
                                                        / 1e9, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"], "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"], "traffic": traffic,
                      "kernel": "k_mask_carry + k_tile_masked<float, select, 16^3, 1024> "
                                "(one iteration)", "algorithmic_bytes": bytes_pass},
         "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "

@@ -240,8 +240,10 @@ struct MaskLds {
   using Acc = typename std::conditional<kMethod == kMaskUnit, long long, double>::type;
   T g[3][kVox];
   Acc acc[kVox];
-  // curl / mask values in LDS where they fit (FP32 gradient)
+  // curl / mask values and the previous iteration's p (read where a run
+  // restarts) in LDS where they fit (FP32 gradient)
   double aux[sizeof(T) == 4 ? kVox : 1];
+  double pl[sizeof(T) == 4 ? kVox : 1];
 };
 
 template <typename T, int kMethod, int kBits, int kThreads>
@@ -270,11 +272,14 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), kk = oz + (l >> (2 * kBits));
     const bool in = i < g.X && j < g.Y && kk < g.Z;
     V4 v;
-    double cv = 0.0;
+    double cv = 0.0, pv = 0.0;
     if (in) {
       const int64_t gv = i + j * sy + kk * sz;
       v = field[gv];
-      if (kAuxLds) cv = aux[gv];
+      if (kAuxLds) {
+        cv = aux[gv];
+        pv = p[gv];
+      }
     } else {
       v.x = v.y = v.z = v.w = 0;
     }
@@ -283,7 +288,10 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     L.g[1][sl] = v.y;
     L.g[2][sl] = v.z;
     L.acc[sl] = 0;
-    if (kAuxLds) L.aux[sl] = cv;
+    if (kAuxLds) {
+      L.aux[sl] = cv;
+      L.pl[sl] = pv;
+    }
   }
   __syncthreads();
   const double scale = sc->scale;
@@ -315,7 +323,9 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     const T h2 = step_factor<T>(wp, 2, up2);
     bool p0set = (f & kFlagP0) != 0, outthr = (f & kFlagOut) != 0;
     const uint32_t mode = f & 3u;
-    double c = mode == kModeCont ? cval[sr.g] : (mode == kModeRestart ? p[gvox(lv)] : 0.0);
+    double c = 0.0;
+    if (mode == kModeCont) c = cval[sr.g];
+    else if (mode == kModeRestart) c = kAuxLds ? L.pl[lds_slot<kBits>(lv)] : p[gvox(lv)];
     double curl = 0.0, cnt = 0.0;
     if (kMethod != kMaskUnit) {
       curl = wc[sr.g];
@@ -350,7 +360,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
           if (added) add(sln, c, curl, cnt);
         }
         if (d.restart) {
-          c = p[gvox(ln)];
+          if constexpr (kAuxLds) c = L.pl[sln];
+          else c = p[gvox(ln)];
           if (!d.keep_w) {
             curl = 0.0;
             cnt = 0.0;
