@@ -105,3 +105,39 @@ def test_batch_tuner(gpu_available, ref_dirs2):
         b.run(use_graph=True)
         for i in range(len(SHAPES)):
             assert F.rel_l2(b.result(i)[0], before[i]) < 1e-12
+
+
+@pytest.mark.parametrize("method", [omni3d.PLAIN_MASK, omni3d.WEIGHTED])
+def test_batch_graph_masked_methods(gpu_available, ref_dirs2, method):
+    """The masked methods' tile engine (rule replay, carry chain, tile pass)
+    captured in the batch graph: replays give the sequential results (bit for
+    bit with unit weights; FP64 curl-weight sums within 1e-12) and match a
+    lone context."""
+    print(f"case batch graph masked method={method}", flush=True)
+    h = 2 * np.pi / 24
+    kw = dict(max_iterations=8, tolerance=0.0, method=method)
+    fs = fields_for(3)
+    masks = []
+    for shape in SHAPES:
+        Z, Y, X = shape
+        k, j, i = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+        masks.append(((i - X / 2) ** 2 + (j - Y / 2) ** 2 + (k - Z / 2) ** 2 > 16).astype(float))
+    with omni3d.Batch(SHAPES, (h, h, h), **kw) as b:
+        b.set_directions(ref_dirs2)
+        for i, f in enumerate(fs):
+            b.set_field(i, f["acc"], mask=masks[i])
+        b.run(use_graph=False)
+        seq = [b.result(i)[0] for i in range(len(SHAPES))]
+        b.run(use_graph=True)
+        for i in range(len(SHAPES)):
+            p = b.result(i)[0]
+            if method == omni3d.PLAIN_MASK:
+                np.testing.assert_array_equal(p, seq[i])
+            else:
+                assert F.rel_l2(p, seq[i]) < 1e-12
+        for i in (0, 2):
+            with omni3d.Context(SHAPES[i], (h, h, h), **kw) as ctx:
+                ctx.set_directions(ref_dirs2)
+                ctx.set_field(fs[i]["acc"], mask=masks[i])
+                ps = ctx.reconstruct()[0]
+            assert F.rel_l2(seq[i], ps) < 1e-12
