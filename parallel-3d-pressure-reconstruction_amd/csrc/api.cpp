@@ -856,7 +856,9 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   if (tiled)
     OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
                                           pl.g, pl.tg, wp, field, aux,
-                                          pl.val_off.as<unsigned long long>(),
+                                          pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                          pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), sc,
                                           ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
                                           ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
                                           ctx->mk_wc.as<double>(), ctx->mk_wn.as<double>(),

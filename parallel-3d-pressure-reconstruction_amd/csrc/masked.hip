@@ -8,12 +8,15 @@
 // start and stop, which visits add, and their curl weights are fixed for a
 // field.  Only the value a run carries depends on p, through the pressure at
 // the voxel the run (re)starts from.  So
-//   once per field:  k_mask_walk   one lane per ray replays the rule and
-//                    stores, per segment (ray x tile, in its value slot), how
-//                    its first visit is taken, the sum of its added terms
-//                    after its last restart and the voxel of that restart;
-//                    it also accumulates the weights (visit counts or curl
-//                    weights) and the source flags;
+//   once per field:  k_mask_walk   one lane per ray replays the rule on the
+//                    curl / mask alone and stores, per segment (ray x tile,
+//                    in its value slot), how its first visit is taken (with
+//                    the term of the step into the tile, the only gradient
+//                    it reads) and the voxel of its last restart, and sets
+//                    the source flags; then k_tile_masked<setup> replays the
+//                    rule per tile from those records with the gradient in
+//                    LDS: each segment's sum of added terms after its last
+//                    restart, and the weights (visit counts or curl weights);
 //   per iteration:   k_mask_carry  one lane per ray chains its segments: the
 //                    value a run enters each segment with;
 //                    k_tile_masked one workgroup per tile stages the tile's
@@ -102,10 +105,10 @@ __device__ __forceinline__ double curl_value(double c, double curl, double cnt) 
   return c * 1 / curl * cnt;
 }
 
-// Per-segment record of a field (value-slot indexed).
+// Per-segment record of a field (value-slot indexed); S comes from the
+// setup tile pass, the rest from k_mask_walk.
 struct MaskSegs {
   double* E;       // term of the step into the tile when it joins the run
-  double* S;       // terms added after the last restart (all when none)
   uint32_t* R;     // voxel of the last restart
   uint32_t* F;     // first-visit flags
   double* wc;      // curl weights after the first visit (curl methods):
@@ -117,9 +120,7 @@ __global__ void __launch_bounds__(256)
 k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
             GridDesc g, TileGrid tg, WalkParams wp, const typename Vec4<T>::type* __restrict__ field,
             const double* __restrict__ aux, const unsigned long long* __restrict__ val_off,
-            MaskSegs ms, unsigned int* __restrict__ wcnt, double* __restrict__ wsum,
-            uint8_t* __restrict__ src) {
-  using V4 = typename Vec4<T>::type;
+            MaskSegs ms, uint8_t* __restrict__ src) {
   const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   Walker w;
@@ -128,7 +129,7 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
   bool p0set = false, outthr = false;
   double curl = 0.0, cnt = 0.0;
-  double E = 0.0, S = 0.0;
+  double E = 0.0;
   uint32_t R = 0, F;
   bool hasR = false;
   {
@@ -147,7 +148,6 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   double c0 = curl, n0 = cnt;
   auto store = [&]() {
     ms.E[gs] = E;
-    ms.S[gs] = S;
     ms.R[gs] = R;
     ms.F[gs] = F | (hasR ? kFlagRestart : 0u);
     if (kMethod != kMaskUnit) {
@@ -156,12 +156,11 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     }
     gs += 64;
   };
-  V4 cur = field[w.v];
   while (!w.done()) {
+    const int64_t vprev = w.v;
     int s;
     const int a = w.advance(&s);
     if (a < 0) break;
-    const V4 nxt = field[w.v];
     const double cn = aux[w.v];
     const int t_now = tile_of(w.c0, w.c1, w.c2, tg);
     const bool enter = t_now != tile;
@@ -169,29 +168,24 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
       store();
       tile = t_now;
       E = 0.0;
-      S = 0.0;
       hasR = false;
     }
     const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
     bool added = false;
     if (d.add) {
-      const double t = (double)trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a));
       if (kMethod != kMaskUnit) {
         curl += cn / 10000;
         cnt += 1.0;
       }
       added = kMethod != kMaskWeighted || curl != 0.0;
-      if (enter) E = t;
-      else S += t;
-      if (added) {
-        if (kMethod == kMaskUnit) atomicAdd(&wcnt[w.v], 1u);
-        else unsafeAtomicAdd(&wsum[w.v], curl_weight(curl, cnt));
-      }
+      // the step into a tile: its term (the in-tile terms are summed by the
+      // setup tile pass from the gradient staged in LDS)
+      if (enter)
+        E = (double)trapezoid<T>(wp, a, s, component(field[w.v], a), component(field[vprev], a));
     }
     if (d.restart) {
       hasR = true;
       R = (uint32_t)w.v;
-      S = 0.0;
       src[w.v] = 1;
       if (!d.keep_w) {
         curl = 0.0;
@@ -204,7 +198,6 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
       c0 = curl;
       n0 = cnt;
     }
-    cur = nxt;
   }
   store();
 }
@@ -246,7 +239,10 @@ struct MaskLds {
   double pl[sizeof(T) == 4 ? kVox : 1];
 };
 
-template <typename T, int kMethod, int kBits, int kThreads>
+// kSetup (once per field): instead of the visits' values, the visits'
+// weights (visit counts or curl weights, into wcnt / wsum) and each
+// segment's sum of added terms after its last restart (into segS).
+template <typename T, int kMethod, int kBits, int kThreads, bool kSetup>
 __global__ void __launch_bounds__(kThreads)
 k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
               GridDesc g, TileGrid tg, WalkParams wp,
@@ -255,7 +251,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
               const double* __restrict__ wn, const double* __restrict__ cval,
               const double* __restrict__ p0, const double* __restrict__ p1, int k,
               const DevScalars* __restrict__ sc, long long* __restrict__ sumfx,
-              double* __restrict__ sumd) {
+              double* __restrict__ sumd, double* __restrict__ segS,
+              unsigned int* __restrict__ wcnt, double* __restrict__ wsum) {
   using V4 = typename Vec4<T>::type;
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
@@ -278,7 +275,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
       v = field[gv];
       if (kAuxLds) {
         cv = aux[gv];
-        pv = p[gv];
+        if (!kSetup) pv = p[gv];
       }
     } else {
       v.x = v.y = v.z = v.w = 0;
@@ -301,11 +298,17 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
            (int64_t)(oz + (l >> (2 * kBits))) * sz;
   };
   auto add = [&](int sl, double c, double curl, double cnt) {
-    if constexpr (kMethod == kMaskUnit)
+    if constexpr (kSetup) {
+      if constexpr (kMethod == kMaskUnit)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[sl]), 1ull);
+      else
+        atomicAdd(&L.acc[sl], curl_weight(curl, cnt));
+    } else if constexpr (kMethod == kMaskUnit) {
       atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[sl]),
                 (unsigned long long)to_fixed(c, scale));
-    else
+    } else {
       atomicAdd(&L.acc[sl], curl_value(c, curl, cnt));
+    }
   };
   const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
   for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
@@ -323,9 +326,11 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     const T h2 = step_factor<T>(wp, 2, up2);
     bool p0set = (f & kFlagP0) != 0, outthr = (f & kFlagOut) != 0;
     const uint32_t mode = f & 3u;
-    double c = 0.0;
-    if (mode == kModeCont) c = cval[sr.g];
-    else if (mode == kModeRestart) c = kAuxLds ? L.pl[lds_slot<kBits>(lv)] : p[gvox(lv)];
+    double c = 0.0, S = 0.0;
+    if constexpr (!kSetup) {
+      if (mode == kModeCont) c = cval[sr.g];
+      else if (mode == kModeRestart) c = kAuxLds ? L.pl[lds_slot<kBits>(lv)] : p[gvox(lv)];
+    }
     double curl = 0.0, cnt = 0.0;
     if (kMethod != kMaskUnit) {
       curl = wc[sr.g];
@@ -350,7 +355,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
         if (d.add) {
           const T h = a == 0 ? h0 : (a == 1 ? h1 : h2);
           const T t = h * (L.g[a][sln] + L.g[a][slv]);
-          c += (double)t;
+          if constexpr (kSetup) S += (double)t;
+          else c += (double)t;
           bool added = true;
           if (kMethod != kMaskUnit) {
             curl += cn / 10000;
@@ -360,7 +366,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
           if (added) add(sln, c, curl, cnt);
         }
         if (d.restart) {
-          if constexpr (kAuxLds) c = L.pl[sln];
+          if constexpr (kSetup) S = 0.0;
+          else if constexpr (kAuxLds) c = L.pl[sln];
           else c = p[gvox(ln)];
           if (!d.keep_w) {
             curl = 0.0;
@@ -373,6 +380,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
       rem -= m;
       cw = wi == 0 ? sr.code[1] : sr.code[2];
     }
+    if constexpr (kSetup) segS[sr.g] = S;
   }
   __syncthreads();
   for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
@@ -380,42 +388,87 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     if (i >= g.X || j >= g.Y || kk >= g.Z) continue;
     const int64_t gv = i + j * sy + kk * sz;
     const int sl = lds_slot<kBits>(l);
-    if constexpr (kMethod == kMaskUnit) sumfx[gv] = L.acc[sl];
-    else sumd[gv] = L.acc[sl];
+    if constexpr (kSetup) {
+      if constexpr (kMethod == kMaskUnit) wcnt[gv] = (unsigned int)L.acc[sl];
+      else wsum[gv] = L.acc[sl];
+    } else if constexpr (kMethod == kMaskUnit) {
+      sumfx[gv] = L.acc[sl];
+    } else {
+      sumd[gv] = L.acc[sl];
+    }
   }
+}
+
+template <typename T, bool kSetup>
+hipError_t launch_tile_pass_masked(int method, int ntiles, const unsigned long long* tile_ptr,
+                                   const SegRec* segs, GridDesc g, TileGrid tg,
+                                   const WalkParams& wp, const void* field, const double* aux,
+                                   const uint32_t* F, const double* wc, const double* wn,
+                                   const double* cval, const double* p0, const double* p1, int k,
+                                   const DevScalars* sc, long long* sumfx, double* sumd,
+                                   double* segS, unsigned int* wcnt, double* wsum,
+                                   hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
+  const V4* f = static_cast<const V4*>(field);
+  constexpr int kThreads = 1024;
+#define OMNI_TM(M, B)                                                                           \
+  hipLaunchKernelGGL((k_tile_masked<T, M, B, kThreads, kSetup>), dim3(ntiles), dim3(kThreads), 0, \
+                     st, tile_ptr, segs, g, tg, wp, f, aux, F, wc, wn, cval, p0, p1, k, sc, sumfx, \
+                     sumd, segS, wcnt, wsum)
+  if (tg.bits == 3) {
+    switch (method) {
+      case kMaskUnit: OMNI_TM(kMaskUnit, 3); break;
+      case kMaskSelect: OMNI_TM(kMaskSelect, 3); break;
+      default: OMNI_TM(kMaskWeighted, 3); break;
+    }
+  } else {
+    switch (method) {
+      case kMaskUnit: OMNI_TM(kMaskUnit, 4); break;
+      case kMaskSelect: OMNI_TM(kMaskSelect, 4); break;
+      default: OMNI_TM(kMaskWeighted, 4); break;
+    }
+  }
+#undef OMNI_TM
+  return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
-                            const double* aux, const unsigned long long* val_off, double* E,
-                            double* S, uint32_t* R, uint32_t* F, double* wc, double* wn,
-                            unsigned int* wcnt, double* wsum, uint8_t* src, hipStream_t st) {
+                            const double* aux, const unsigned long long* val_off, int ntiles,
+                            const unsigned long long* tile_ptr, const SegRec* segs,
+                            const DevScalars* sc, double* E, double* S, uint32_t* R, uint32_t* F,
+                            double* wc, double* wn, unsigned int* wcnt, double* wsum,
+                            uint8_t* src, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
-  const MaskSegs ms{E, S, R, F, wc, wn};
+  const MaskSegs ms{E, R, F, wc, wn};
   const unsigned int grid = grid_for(nrays, 1u << 30);
 #define OMNI_MW(M)                                                                              \
   hipLaunchKernelGGL((k_mask_walk<T, M>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g, tg, \
-                     wp, f, aux, val_off, ms, wcnt, wsum, src)
+                     wp, f, aux, val_off, ms, src)
   switch (method) {
     case kMaskUnit: OMNI_MW(kMaskUnit); break;
     case kMaskSelect: OMNI_MW(kMaskSelect); break;
     default: OMNI_MW(kMaskWeighted); break;
   }
 #undef OMNI_MW
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // setup tile pass: in-tile sums after the last restart, and the weights
+  return launch_tile_pass_masked<T, true>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux, F,
+                                          wc, wn, nullptr, nullptr, nullptr, 1, sc, nullptr,
+                                          nullptr, S, wcnt, wsum, st);
 }
-template hipError_t launch_mask_walk<float>(int, const RayRec*, int64_t, const DirConst*, GridDesc,
-                                            TileGrid, const WalkParams&, const void*,
-                                            const double*, const unsigned long long*, double*,
-                                            double*, uint32_t*, uint32_t*, double*, double*,
-                                            unsigned int*, double*, uint8_t*, hipStream_t);
-template hipError_t launch_mask_walk<double>(int, const RayRec*, int64_t, const DirConst*,
-                                             GridDesc, TileGrid, const WalkParams&, const void*,
-                                             const double*, const unsigned long long*, double*,
-                                             double*, uint32_t*, uint32_t*, double*, double*,
-                                             unsigned int*, double*, uint8_t*, hipStream_t);
+#define OMNI_MW_INST(T)                                                                        \
+  template hipError_t launch_mask_walk<T>(                                                      \
+      int, const RayRec*, int64_t, const DirConst*, GridDesc, TileGrid, const WalkParams&,       \
+      const void*, const double*, const unsigned long long*, int, const unsigned long long*,    \
+      const SegRec*, const DevScalars*, double*, double*, uint32_t*, uint32_t*, double*, double*, \
+      unsigned int*, double*, uint8_t*, hipStream_t);
+OMNI_MW_INST(float)
+OMNI_MW_INST(double)
+#undef OMNI_MW_INST
 
 hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, const double* E, const double* S,
@@ -434,28 +487,9 @@ hipError_t launch_tile_masked(int method, int ntiles, const unsigned long long* 
                               const double* wc, const double* wn, const double* cval,
                               const double* p0, const double* p1, int k, const DevScalars* sc,
                               long long* sumfx, double* sumd, hipStream_t st) {
-  using V4 = typename Vec4<T>::type;
-  const V4* f = static_cast<const V4*>(field);
-  constexpr int kThreads = 1024;
-#define OMNI_TM(M, B)                                                                         \
-  hipLaunchKernelGGL((k_tile_masked<T, M, B, kThreads>), dim3(ntiles), dim3(kThreads), 0, st, \
-                     tile_ptr, segs, g, tg, wp, f, aux, F, wc, wn, cval, p0, p1, k, sc, sumfx, \
-                     sumd)
-  if (tg.bits == 3) {
-    switch (method) {
-      case kMaskUnit: OMNI_TM(kMaskUnit, 3); break;
-      case kMaskSelect: OMNI_TM(kMaskSelect, 3); break;
-      default: OMNI_TM(kMaskWeighted, 3); break;
-    }
-  } else {
-    switch (method) {
-      case kMaskUnit: OMNI_TM(kMaskUnit, 4); break;
-      case kMaskSelect: OMNI_TM(kMaskSelect, 4); break;
-      default: OMNI_TM(kMaskWeighted, 4); break;
-    }
-  }
-#undef OMNI_TM
-  return hipGetLastError();
+  return launch_tile_pass_masked<T, false>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux,
+                                           F, wc, wn, cval, p0, p1, k, sc, sumfx, sumd, nullptr,
+                                           nullptr, nullptr, st);
 }
 template hipError_t launch_tile_masked<float>(int, int, const unsigned long long*, const SegRec*,
                                               GridDesc, TileGrid, const WalkParams&, const void*,

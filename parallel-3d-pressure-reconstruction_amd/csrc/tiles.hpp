@@ -120,12 +120,16 @@ size_t scan_u32_tmp_bytes(int64_t n);
 
 // masked.hip: the masked methods' traversal per iteration (method 1
 // plain-mask, 2 Select, 3 Weighted as in the naive walk)
+// the per-field rule replay: k_mask_walk, then the setup tile pass (segment
+// sums S and the weights wcnt (plain-mask) or wsum (curl methods))
 template <typename T>
 hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
-                            const double* aux, const unsigned long long* val_off, double* E,
-                            double* S, uint32_t* R, uint32_t* F, double* wc, double* wn,
-                            unsigned int* wcnt, double* wsum, uint8_t* src, hipStream_t st);
+                            const double* aux, const unsigned long long* val_off, int ntiles,
+                            const unsigned long long* tile_ptr, const SegRec* segs,
+                            const DevScalars* sc, double* E, double* S, uint32_t* R, uint32_t* F,
+                            double* wc, double* wn, unsigned int* wcnt, double* wsum,
+                            uint8_t* src, hipStream_t st);
 hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, const double* E, const double* S,
                              const uint32_t* R, const uint32_t* F, const double* p0,
