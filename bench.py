@@ -193,6 +193,24 @@ def methods_mode(args):
             out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
         log(f"bench: {name} {m:.2f} ms per pass")
     first = out["select"]
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's serial Select pass (the reference's shipped method) over
+        # a sample of the directions, scaled to one pass
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        with omni3d.Context(shape, f["delta"], density=f["rho"]) as cc:
+            cc.set_field(f["acc"], mask=f["mask"])
+            curl = cc.field_curl()
+        sub = dirs[:: max(1, D // args.cpu_dirs)][: args.cpu_dirs]
+        t0 = time.perf_counter()
+        O.one_pass(shape, sub, 1.0, f["delta"], f["rho"], f["acc"], np.zeros(shape), curl=curl,
+                   threshold=thr, method=O.SELECT, faithful=1)
+        ts = time.perf_counter() - t0
+        cpu = {"value": round(ts * D / len(sub) * 1e3, 1), "unit": "ms per pass", "cores": 1,
+               "kind": "port",
+               "sample": f"oracle FP64 serial Select pass over {len(sub)} of {D} directions at "
+                         f"{n}^3 took {ts:.2f}s, scaled x{D / len(sub):.1f} to one pass"}
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_masked_fourier128.json")
     if os.path.exists(tf):
@@ -209,6 +227,7 @@ def methods_mode(args):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": first["frac"], "traffic": traffic,
                      "kernel": "k_mask_carry + k_tile_masked<float, select, 16^3, 1024> "
                                "(one iteration)", "algorithmic_bytes": bytes_pass},
+        "cpu_baseline": cpu,
         "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "
                                f"one full traversal per iteration, {passes} iterations timed; "
                                f"plain: one lane per ray, masked methods: tile engine",

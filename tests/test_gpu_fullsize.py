@@ -12,7 +12,9 @@ reconstruction must have whatever its size, all through the C-ABI on the GPU:
 * the tile engine and the one-lane-per-ray engine agree (128^3, FP64,
   rel-L2 <= 1e-12);
 * the FP32 source iteration in increment form agrees with gathering p_S
-  itself (128^3, rel-L2 <= 1e-7).
+  itself (128^3, rel-L2 <= 1e-7);
+* the masked methods' tile engine agrees with the one-lane-per-ray walk on
+  the 128^3 masked Fourier sphere (FP64, rel-L2 <= 1e-12, equal counts).
 """
 import numpy as np
 import pytest
@@ -118,3 +120,37 @@ def test_increment_form_128(gpu_available, dirs162, monkeypatch):
     err = F.rel_l2(pi, pp)
     print("increment form vs p_S gathers rel-L2", err)
     assert err < 1e-7
+
+
+@pytest.mark.parametrize("method", [omni3d.PLAIN_MASK, omni3d.SELECT])
+def test_masked_tile_engine_128(gpu_available, dirs162, monkeypatch, method):
+    print(f"case masked tile engine vs lane walk 128^3 method={method}", flush=True)
+    n = 128
+    shape = (n, n, n)
+    f = F.fourier_sphere(n, seed=0)
+    thr = 0.0
+    if method == omni3d.SELECT:
+        with omni3d.Context(shape, f["delta"]) as ctx:
+            ctx.set_field(f["acc"], mask=f["mask"])
+            c = ctx.field_curl()
+        thr = float(np.quantile(c[f["mask"] > 0], 0.8))
+    p0 = np.random.default_rng(9).normal(size=shape)
+
+    def go():
+        with omni3d.Context(shape, f["delta"], density=f["rho"], precision=omni3d.FP64,
+                            method=method, solver=omni3d.NAIVE, max_iterations=3, threshold=thr,
+                            faithful_select=1) as ctx:
+            ctx.set_directions(dirs162)
+            ctx.set_field(f["acc"], mask=f["mask"])
+            return ctx.reconstruct(p0)
+
+    pt, wt, _ = go()
+    monkeypatch.setenv("OMNI3D_MASKED", "naive")
+    pn, wn, _ = go()
+    if method == omni3d.PLAIN_MASK:
+        np.testing.assert_array_equal(wt, wn)
+    else:
+        assert F.rel_l2(wt, wn) < 1e-12
+    err = F.rel_l2(pt, pn)
+    print("tile engine vs lane walk rel-L2", err)
+    assert err < 1e-12
