@@ -187,10 +187,9 @@ def methods_mode(args):
         m = float(np.mean(ms))
         out[name] = {"ms_per_pass": round(m, 3),
                      "frac": round(bytes_pass / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        if method != omni3d.PLAIN:
-            # the per-field replay of the path-break rule (k_mask_walk), once
-            # per reconstruction
-            out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
+        # the per-field replay of the path-break rule (k_mask_walk and the
+        # setup tile pass), once per reconstruction
+        out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
         log(f"bench: {name} {m:.2f} ms per pass")
     first = out["select"]
     cpu = None
@@ -230,7 +229,7 @@ def methods_mode(args):
         "cpu_baseline": cpu,
         "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "
                                f"one full traversal per iteration, {passes} iterations timed; "
-                               f"plain: one lane per ray, masked methods: tile engine",
+                               f"tile engine (rule replay once per field, carry chain + tile pass per iteration)",
                    "select_threshold": thr, "methods": out}}))
 
 
@@ -410,11 +409,19 @@ def main():
         rn = step(ctxn)
         torch.cuda.synchronize()
         ms_naive = rn.ms_iterate / args.naive_passes
+        # the same schedule walked one lane per ray with global atomics (the
+        # reference kernels' organisation)
+        os.environ["OMNI3D_MASKED"] = "naive"
+        rl = step(ctxn)
+        torch.cuda.synchronize()
+        del os.environ["OMNI3D_MASKED"]
+        ms_lanes = rl.ms_iterate / args.naive_passes
         ctxn.close()
         fp64 = {"s_per_recon_fp64": round(s64, 6), "rel_l2_vs_fp64": rel,
                 "max_err_rel_vs_fp64": maxerr, "speedup_vs_fp64": round(s64 / s_per, 3),
                 "fp64_iterations": r64.iterations, "ms_pass_b_fp64": round(r64.ms_pass_b, 4),
                 "fp64_reference_schedule_ms_per_pass": round(ms_naive, 3),
+                "fp64_reference_schedule_lane_walk_ms_per_pass": round(ms_lanes, 3),
                 "fp64_reference_schedule_s_for_same_iterations":
                     round(ms_naive * 1e-3 * r.iterations, 3),
                 "speedup_vs_fp64_reference_schedule":

@@ -814,13 +814,13 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
     aux = ctx->aux.as<double>();
   }
   const bool fixed = m == OMNI3D_METHOD_PLAIN || m == OMNI3D_METHOD_PLAIN_MASK;
-  // the masked methods run on the tile engine (OMNI3D_MASKED=naive: one lane
-  // per ray with global atomics)
+  // every method runs on the tile engine (OMNI3D_MASKED=naive: one lane per
+  // ray with global atomics)
   const char* menv = std::getenv("OMNI3D_MASKED");
-  const bool tiled = m != OMNI3D_METHOD_PLAIN && !(menv && std::strcmp(menv, "naive") == 0);
+  const bool tiled = !(menv && std::strcmp(menv, "naive") == 0);
   if (tiled) {
     const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
-    const bool curlw = m != OMNI3D_METHOD_PLAIN_MASK;
+    const bool curlw = !fixed;
     OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
     OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
     OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
@@ -886,9 +886,11 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
                                             fixed ? ctx->sum.as<long long>() : nullptr, cntw,
                                             fixed ? nullptr : ctx->sumd.as<double>(),
                                             fixed ? nullptr : ctx->wsum.as<double>(), pinit,
-                                            seed > 0 ? 1 : 0, nullptr, ctx->srcf.as<uint8_t>(),
-                                            pa, pb, sc, ctx->partials.as<double>(),
-                                            P.max_iterations, P.tolerance, st));
+                                            seed > 0 ? 1 : 0,
+                                            m == OMNI3D_METHOD_PLAIN ? pl.sid.as<int>() : nullptr,
+                                            ctx->srcf.as<uint8_t>(), pa, pb, sc,
+                                            ctx->partials.as<double>(), P.max_iterations,
+                                            P.tolerance, st));
       continue;
     }
     if (fixed)

@@ -1,6 +1,7 @@
-// Tile engine for the masked integration methods (plain-mask, Select,
-// Weighted) under the reference's schedule: a full traversal per iteration
-// (kernel.cu:8840-8861).
+// Tile engine for the reference's schedule -- a full traversal per
+// iteration (kernel.cu:8840-8861) -- of every integration method: the
+// masked ones (plain-mask, Select, Weighted) and plain (the rule that never
+// breaks a path).
 //
 // The path-break rule of these methods (bodyIntegralInnerSelect,
 // kernel.cu:2280-2537; bodyIntegralInnerWeighted, kernel.cu:1993-2135)
@@ -38,9 +39,13 @@
 
 namespace omni3d {
 
-// method ids as in k_walk_naive: 1 plain-mask (unit weights), 2 Select,
-// 3 Weighted
-enum { kMaskUnit = 1, kMaskSelect = 2, kMaskWeighted = 3 };
+// method ids as in k_walk_naive: 0 plain (bodyIntegralInner: every step
+// adds, the entry adds its own p, weights = the plan's visit counts),
+// 1 plain-mask (unit weights), 2 Select, 3 Weighted
+enum { kMaskPlain = 0, kMaskUnit = 1, kMaskSelect = 2, kMaskWeighted = 3 };
+// integer (fixed-point) sums for whole-count weights, FP64 for curl weights
+template <int kMethod>
+constexpr bool kCurlW = kMethod == kMaskSelect || kMethod == kMaskWeighted;
 
 // flags of a segment's first visit
 constexpr uint32_t kModeNone = 0u;     // no run: nothing carried
@@ -63,8 +68,8 @@ template <int kMethod>
 __device__ __forceinline__ StepRule rule_step(const WalkParams& wp, int a, double cn, bool& p0set,
                                               bool& outthr) {
   StepRule r{false, false, false};
-  if (kMethod == kMaskWeighted) {
-    r.add = true;  // every step adds when the curl sum is nonzero
+  if (kMethod == kMaskWeighted || kMethod == kMaskPlain) {
+    r.add = true;  // every step adds (Weighted: when the curl sum is nonzero)
     return r;
   }
   const double thr = wp.thr;
@@ -93,7 +98,7 @@ __device__ __forceinline__ StepRule rule_step(const WalkParams& wp, int a, doubl
 // The entry voxel of a ray: a run starts there (p0 = p, no add) when valid.
 template <int kMethod>
 __device__ __forceinline__ bool rule_entry(const WalkParams& wp, double ci) {
-  if (kMethod == kMaskWeighted) return true;
+  if (kMethod == kMaskWeighted || kMethod == kMaskPlain) return true;
   const double thr = wp.thr;
   return wp.faithful ? !(ci == 0.0 || ci > thr) : (ci != 0.0 && ci < thr);
 }
@@ -133,13 +138,14 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   uint32_t R = 0, F;
   bool hasR = false;
   {
-    const double ci = aux[w.v];
+    const double ci = kMethod == kMaskPlain ? 0.0 : aux[w.v];
     if (rule_entry<kMethod>(wp, ci)) {
       p0set = true;
       hasR = true;
       R = (uint32_t)w.v;
       src[w.v] = 1;
-      F = kModeRestart;
+      // the plain entry adds its own p (pint = 0)
+      F = kModeRestart | (kMethod == kMaskPlain ? kFlagAdd : 0u);
     } else {
       F = kModeNone;
     }
@@ -150,7 +156,7 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     ms.E[gs] = E;
     ms.R[gs] = R;
     ms.F[gs] = F | (hasR ? kFlagRestart : 0u);
-    if (kMethod != kMaskUnit) {
+    if (kCurlW<kMethod>) {
       ms.wc[gs] = c0;
       ms.wn[gs] = n0;
     }
@@ -161,7 +167,7 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     int s;
     const int a = w.advance(&s);
     if (a < 0) break;
-    const double cn = aux[w.v];
+    const double cn = kMethod == kMaskPlain ? 0.0 : aux[w.v];
     const int t_now = tile_of(w.c0, w.c1, w.c2, tg);
     const bool enter = t_now != tile;
     if (enter) {
@@ -173,7 +179,7 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
     bool added = false;
     if (d.add) {
-      if (kMethod != kMaskUnit) {
+      if (kCurlW<kMethod>) {
         curl += cn / 10000;
         cnt += 1.0;
       }
@@ -230,12 +236,12 @@ k_mask_carry(int64_t nrays, const unsigned int* __restrict__ nseg,
 
 template <typename T, int kMethod, int kVox>
 struct MaskLds {
-  using Acc = typename std::conditional<kMethod == kMaskUnit, long long, double>::type;
+  using Acc = typename std::conditional<kCurlW<kMethod>, double, long long>::type;
   T g[3][kVox];
   Acc acc[kVox];
   // curl / mask values and the previous iteration's p (read where a run
   // restarts) in LDS where they fit (FP32 gradient)
-  double aux[sizeof(T) == 4 ? kVox : 1];
+  double aux[sizeof(T) == 4 && kMethod != kMaskPlain ? kVox : 1];
   double pl[sizeof(T) == 4 ? kVox : 1];
 };
 
@@ -258,6 +264,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
   constexpr int kTileMask = kTile - 1;
   constexpr int kVox = kTile * kTile * kTile;
   constexpr bool kAuxLds = sizeof(T) == 4;
+  constexpr bool kAux = kMethod != kMaskPlain;  // the plain rule reads no curl / mask
   __shared__ MaskLds<T, kMethod, kVox> L;
   if (sc->converged) return;
   const double* __restrict__ p = ((k - 1) & 1) ? p1 : p0;
@@ -274,7 +281,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
       const int64_t gv = i + j * sy + kk * sz;
       v = field[gv];
       if (kAuxLds) {
-        cv = aux[gv];
+        if (kAux) cv = aux[gv];
         if (!kSetup) pv = p[gv];
       }
     } else {
@@ -286,7 +293,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     L.g[2][sl] = v.z;
     L.acc[sl] = 0;
     if (kAuxLds) {
-      L.aux[sl] = cv;
+      if (kAux) L.aux[sl] = cv;
       L.pl[sl] = pv;
     }
   }
@@ -299,11 +306,12 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
   };
   auto add = [&](int sl, double c, double curl, double cnt) {
     if constexpr (kSetup) {
+      // the plain method's weights are the plan's visit counts
       if constexpr (kMethod == kMaskUnit)
         atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[sl]), 1ull);
-      else
+      else if constexpr (kCurlW<kMethod>)
         atomicAdd(&L.acc[sl], curl_weight(curl, cnt));
-    } else if constexpr (kMethod == kMaskUnit) {
+    } else if constexpr (!kCurlW<kMethod>) {
       atomicAdd(reinterpret_cast<unsigned long long*>(&L.acc[sl]),
                 (unsigned long long)to_fixed(c, scale));
     } else {
@@ -332,7 +340,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
       else if (mode == kModeRestart) c = kAuxLds ? L.pl[lds_slot<kBits>(lv)] : p[gvox(lv)];
     }
     double curl = 0.0, cnt = 0.0;
-    if (kMethod != kMaskUnit) {
+    if (kCurlW<kMethod>) {
       curl = wc[sr.g];
       cnt = wn[sr.g];
     }
@@ -348,8 +356,9 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
         cw >>= 2;
         const int ln = lv + (a == 0 ? d0 : (a == 1 ? d1 : d2));
         const int sln = lds_slot<kBits>(ln);
-        double cn;
-        if constexpr (kAuxLds) cn = L.aux[sln];
+        double cn = 0.0;
+        if constexpr (!kAux) cn = 0.0;
+        else if constexpr (kAuxLds) cn = L.aux[sln];
         else cn = aux[gvox(ln)];
         const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
         if (d.add) {
@@ -358,7 +367,7 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
           if constexpr (kSetup) S += (double)t;
           else c += (double)t;
           bool added = true;
-          if (kMethod != kMaskUnit) {
+          if (kCurlW<kMethod>) {
             curl += cn / 10000;
             cnt += 1.0;
             added = kMethod != kMaskWeighted || curl != 0.0;
@@ -390,8 +399,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
     const int sl = lds_slot<kBits>(l);
     if constexpr (kSetup) {
       if constexpr (kMethod == kMaskUnit) wcnt[gv] = (unsigned int)L.acc[sl];
-      else wsum[gv] = L.acc[sl];
-    } else if constexpr (kMethod == kMaskUnit) {
+      else if constexpr (kCurlW<kMethod>) wsum[gv] = L.acc[sl];
+    } else if constexpr (!kCurlW<kMethod>) {
       sumfx[gv] = L.acc[sl];
     } else {
       sumd[gv] = L.acc[sl];
@@ -417,12 +426,14 @@ hipError_t launch_tile_pass_masked(int method, int ntiles, const unsigned long l
                      sumd, segS, wcnt, wsum)
   if (tg.bits == 3) {
     switch (method) {
+      case kMaskPlain: OMNI_TM(kMaskPlain, 3); break;
       case kMaskUnit: OMNI_TM(kMaskUnit, 3); break;
       case kMaskSelect: OMNI_TM(kMaskSelect, 3); break;
       default: OMNI_TM(kMaskWeighted, 3); break;
     }
   } else {
     switch (method) {
+      case kMaskPlain: OMNI_TM(kMaskPlain, 4); break;
       case kMaskUnit: OMNI_TM(kMaskUnit, 4); break;
       case kMaskSelect: OMNI_TM(kMaskSelect, 4); break;
       default: OMNI_TM(kMaskWeighted, 4); break;
@@ -448,6 +459,7 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
   hipLaunchKernelGGL((k_mask_walk<T, M>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g, tg, \
                      wp, f, aux, val_off, ms, src)
   switch (method) {
+    case kMaskPlain: OMNI_MW(kMaskPlain); break;
     case kMaskUnit: OMNI_MW(kMaskUnit); break;
     case kMaskSelect: OMNI_MW(kMaskSelect); break;
     default: OMNI_MW(kMaskWeighted); break;

@@ -138,6 +138,8 @@ def test_fourier_sphere_masked_fp32(gpu_available, ref_dirs3):
 
 
 @pytest.mark.parametrize("method,faithful,precision,bits", [
+    (omni3d.PLAIN, 0, omni3d.FP64, 4),
+    (omni3d.PLAIN, 0, omni3d.FP32_KAHAN, 3),
     (omni3d.PLAIN_MASK, 0, omni3d.FP64, 4),
     (omni3d.PLAIN_MASK, 0, omni3d.FP32_KAHAN, 4),
     (omni3d.PLAIN_MASK, 0, omni3d.FP64, 3),
@@ -150,9 +152,10 @@ def test_fourier_sphere_masked_fp32(gpu_available, ref_dirs3):
 ])
 def test_masked_tile_engine_matches_lane_walk(gpu_available, ref_dirs3, monkeypatch, method,
                                               faithful, precision, bits):
-    """The masked methods' tile-engine traversal (per-field rule replay,
-    per-iteration carry chain, LDS-tiled pass) against the one-lane-per-ray
-    walk on a 40^3 masked Fourier sphere from a random initial pressure."""
+    """The tile-engine traversal of the reference's schedule (per-field rule
+    replay, per-iteration carry chain, LDS-tiled pass) against the
+    one-lane-per-ray walk on a 40^3 masked Fourier sphere from a random
+    initial pressure, for every method."""
     n = 40
     shape = (n, n, n)
     print(f"case masked tile engine method={method} faithful={faithful} "
@@ -182,7 +185,7 @@ def test_masked_tile_engine_matches_lane_walk(gpu_available, ref_dirs3, monkeypa
     monkeypatch.setenv("OMNI3D_MASKED", "naive")
     pn, wn, rn = run()
     assert rt.iterations == rn.iterations == 6
-    if method == omni3d.PLAIN_MASK:
+    if method in (omni3d.PLAIN, omni3d.PLAIN_MASK):
         np.testing.assert_array_equal(wt, wn)
     else:
         assert F.rel_l2(wt, wn) < 1e-12
