@@ -5,17 +5,21 @@
 // accumulated into the caller's pn / pcountinner.
 //
 // Each host thread keeps one hidden context whose geometry plan (rays, visit
-// counts) is reused while the grid, line spacing and direction set stay the
-// same, as they do across the reference's iteration loop.
+// counts, tile segments) is reused while the grid, line spacing and
+// direction set stay the same, as they do across the reference's iteration
+// loop.  A pass runs on the tile engine of the reference's schedule.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "driver.hpp"
 #include "host.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 
 using namespace omni3d;
 
@@ -130,14 +134,54 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
   }
   WalkParams wp = walk_params(ctx->prm);
-  OMNI_TRY_HIP(ctx, launch_walk_naive<double>(method, pl.rays.as<RayRec>(), pl.nrays,
-                                              pl.dirs.as<DirConst>(), pl.g, wp,
-                                              ctx->field.as<void>(), aux, ctx->pa.as<double>(),
-                                              ctx->pa.as<double>(), 1, sc,
-                                              fixed ? ctx->sum.as<long long>() : nullptr, nullptr,
-                                              fixed ? nullptr : ctx->sumd.as<double>(),
-                                              fixed ? nullptr : ctx->wsum.as<double>(),
-                                              ctx->srcf.as<uint8_t>(), st));
+  // the pass runs on the tile engine (rule replay, carry chain, tile pass);
+  // OMNI3D_MASKED=naive walks one lane per ray with global atomics
+  const char* menv = std::getenv("OMNI3D_MASKED");
+  if (!(menv && std::strcmp(menv, "naive") == 0)) {
+    const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
+    OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_r.alloc(sizeof(uint32_t) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
+    OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (fixed ? 1 : nv)));
+    OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (fixed ? 1 : nv)));
+    const double* pa = ctx->pa.as<double>();
+    OMNI_TRY_HIP(ctx, launch_mask_walk<double>(method, pl.rays.as<RayRec>(), pl.nrays,
+                                               pl.dirs.as<DirConst>(), pl.g, pl.tg, wp,
+                                               ctx->field.as<void>(), aux,
+                                               pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                               pl.tile_ptr.as<unsigned long long>(),
+                                               pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
+                                               ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                               ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                               ctx->mk_wn.as<double>(), nullptr,
+                                               fixed ? nullptr : ctx->wsum.as<double>(),
+                                               ctx->srcf.as<uint8_t>(), st));
+    OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
+                                        pl.val_off.as<unsigned long long>(),
+                                        ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
+                                        ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(), pa, pa,
+                                        1, sc, ctx->mk_c.as<double>(), st));
+    OMNI_TRY_HIP(ctx, launch_tile_masked<double>(method, pl.ntiles,
+                                                 pl.tile_ptr.as<unsigned long long>(),
+                                                 pl.segs.as<SegRec>(), pl.g, pl.tg, wp,
+                                                 ctx->field.as<void>(), aux,
+                                                 ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                                 ctx->mk_wn.as<double>(), ctx->mk_c.as<double>(),
+                                                 pa, pa, 1, sc,
+                                                 fixed ? ctx->sum.as<long long>() : nullptr,
+                                                 fixed ? nullptr : ctx->sumd.as<double>(), st));
+  } else {
+    OMNI_TRY_HIP(ctx, launch_walk_naive<double>(method, pl.rays.as<RayRec>(), pl.nrays,
+                                                pl.dirs.as<DirConst>(), pl.g, wp,
+                                                ctx->field.as<void>(), aux, ctx->pa.as<double>(),
+                                                ctx->pa.as<double>(), 1, sc,
+                                                fixed ? ctx->sum.as<long long>() : nullptr,
+                                                nullptr, fixed ? nullptr : ctx->sumd.as<double>(),
+                                                fixed ? nullptr : ctx->wsum.as<double>(),
+                                                ctx->srcf.as<uint8_t>(), st));
+  }
   OMNI_TRY_HIP(ctx, launch_accumulate_out(V, fixed ? 1 : 0,
                                           fixed ? ctx->sum.as<long long>() : nullptr,
                                           pl.cnt.as<unsigned int>(),
