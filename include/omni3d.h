@@ -184,9 +184,10 @@ int omni3d_curl(int X, int Y, int Z, float deltx, float delty, float deltz, cons
  * zero initial pressure, the context's iteration count, no seed term).
  * forward: grad p field (3 components, device F32 or F64 by the context's
  * precision) -> p; backward: dL/dp -> dL/d(grad p).  Component layout:
- * g[c*V + v], c = x,y,z.  With the affine solver the plain method's backward
- * iterates over the source voxels only (its first call on a plan builds the
- * transposed source rows, kept with the plan). */
+ * g[c*V + v], c = x,y,z.  With the affine solver the backward iterates over
+ * the source voxels only: ray entries for plain (its first call on a plan
+ * builds the transposed source rows, kept with the plan), run starts for
+ * plain-mask (rebuilt per call from the mask). */
 int omni3d_forward(omni3d_ctx* ctx, const void* grad_d, void* p_d);
 int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d);
 

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "driver.hpp"
 #include "host.hpp"
@@ -64,6 +65,71 @@ int build_transposed_rows(omni3d_ctx* ctx) {
                                       pl.tent.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   pl.trows = true;
+  return OMNI3D_OK;
+}
+
+// Sources and transposed rows of the plain-mask rule for the current mask:
+// S = the run-start voxels (numbered in voxel order), row s' = the run-start
+// voxels that runs starting at s' visit (sorted).  No self visits: diag = 0.
+struct MaskRows {
+  int nS = 0;
+  DevBuf sid, slist, tptr, tent, diag;
+};
+
+int build_mask_rows(omni3d_ctx* ctx, const double* mask, MaskRows* mr) {
+  const Plan& pl = *ctx->plan;
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(ctx->prm);
+  DevBuf flag;
+  OMNI_TRY_HIP(ctx, flag.alloc((size_t)((V + 3) / 4) * 4));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(flag.as<void>(), (V + 3) / 4, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_adj_mask_starts(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                           pl.g, mask, flag.as<uint8_t>(), st));
+  std::vector<uint8_t> hf((size_t)V);
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hf.data(), flag.as<void>(), (size_t)V, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  std::vector<int> hsid((size_t)V, -1), hsl;
+  for (int64_t v = 0; v < V; v++)
+    if (hf[(size_t)v]) {
+      hsid[(size_t)v] = (int)hsl.size();
+      hsl.push_back((int)v);
+    }
+  const int nS = (int)hsl.size();
+  mr->nS = nS;
+  if (nS == 0) return OMNI3D_OK;
+  OMNI_TRY_HIP(ctx, mr->sid.alloc(sizeof(int) * V));
+  OMNI_TRY_HIP(ctx, mr->slist.alloc(sizeof(int) * nS));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mr->sid.as<void>(), hsid.data(), sizeof(int) * V,
+                                   hipMemcpyHostToDevice, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mr->slist.as<void>(), hsl.data(), sizeof(int) * nS,
+                                   hipMemcpyHostToDevice, st));
+  DevBuf cnt, key;
+  OMNI_TRY_HIP(ctx, cnt.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(cnt.as<void>(), nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_adj_mask_rows(0, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                         pl.g, mask, mr->sid.as<int>(), nullptr,
+                                         cnt.as<unsigned int>(), nullptr, st));
+  OMNI_TRY_HIP(ctx, mr->tptr.alloc(sizeof(unsigned long long) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(cnt.as<unsigned int>(), nS,
+                                       mr->tptr.as<unsigned long long>(), st));
+  unsigned long long n = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&n, mr->tptr.as<unsigned long long>() + nS, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  OMNI_TRY_HIP(ctx, key.alloc(sizeof(unsigned long long) * (n + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(cnt.as<void>(), nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_adj_mask_rows(1, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                         pl.g, mask, mr->sid.as<int>(),
+                                         mr->tptr.as<unsigned long long>(), cnt.as<unsigned int>(),
+                                         key.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, launch_sort_rows(nS, mr->tptr.as<unsigned long long>(),
+                                     key.as<unsigned long long>(), st));
+  OMNI_TRY_HIP(ctx, mr->tent.alloc(sizeof(unsigned int) * (n + 1)));
+  OMNI_TRY_HIP(ctx, launch_adj_narrow(key.as<unsigned long long>(), (int64_t)n,
+                                      mr->tent.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, mr->diag.alloc(sizeof(unsigned int) * nS));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(mr->diag.as<void>(), nS, 0u, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   return OMNI3D_OK;
 }
 
@@ -145,14 +211,33 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
     OMNI_TRY_HIP(ctx, (launch_convert<float, double>(static_cast<const float*>(dldp_d), y, V, st)));
   OMNI_TRY_HIP(ctx, (launch_convert<double, double>(y, z, V, st)));
   // z = sum_{j<K} (A^T)^j y
-  if (!mask_rule && P.solver == OMNI3D_SOLVER_AFFINE) {
-    // plain method: after the first gather only the source voxels carry
-    // values, so the Horner steps run over the nS sources with the
-    // transposed source rows
+  MaskRows mr;
+  if (P.solver == OMNI3D_SOLVER_AFFINE) {
+    // after the first gather only the source voxels (ray entries; for the
+    // plain-mask rule the run starts) carry values, so the Horner steps run
+    // over the nS sources with the transposed source rows
+    int nS = pl.nS;
+    const int* slist = pl.slist.as<int>();
+    const unsigned long long* tptr = pl.tptr.as<unsigned long long>();
+    const unsigned int* tent = pl.tent.as<unsigned int>();
+    const unsigned int* diagp = pl.diag.as<unsigned int>();
     if (P.max_iterations > 1) {
-      rc = build_transposed_rows(ctx);
-      if (rc) return rc;
-      const int nS = pl.nS;
+      if (mask_rule) {
+        rc = build_mask_rows(ctx, mask, &mr);
+        if (rc) return rc;
+        nS = mr.nS;
+        slist = mr.slist.as<int>();
+        tptr = mr.tptr.as<unsigned long long>();
+        tent = mr.tent.as<unsigned int>();
+        diagp = mr.diag.as<unsigned int>();
+      } else {
+        rc = build_transposed_rows(ctx);
+        if (rc) return rc;
+        tptr = pl.tptr.as<unsigned long long>();
+        tent = pl.tent.as<unsigned int>();
+      }
+    }
+    if (P.max_iterations > 1 && nS > 0) {
       OMNI_TRY_HIP(ctx, ctx->adj_s.alloc(sizeof(double) * 6 * (size_t)nS));
       double* ts = ctx->adj_s.as<double>();
       double* h[2] = {ts + nS, ts + 2 * (size_t)nS};
@@ -160,18 +245,16 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
       double* cs = ts + 5 * (size_t)nS;
       OMNI_TRY_HIP(ctx, launch_adj_scale(V, y, cnt, u, st));
       OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
-      OMNI_TRY_HIP(ctx, launch_adj_gather(false, pl.rays.as<RayRec>(), pl.nrays,
-                                          pl.dirs.as<DirConst>(), pl.g, nullptr, u, t, st));
-      OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, pl.slist.as<int>(), t, cnt, ts, h[0], w[0], cs, st));
+      OMNI_TRY_HIP(ctx, launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
+                                          pl.dirs.as<DirConst>(), pl.g, mask, u, t, st));
+      OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, slist, t, cnt, ts, h[0], w[0], cs, st));
       int cur = 0;
       for (int m = 2; m < P.max_iterations; m++) {
-        OMNI_TRY_HIP(ctx, launch_adj_src_iter(nS, pl.tptr.as<unsigned long long>(),
-                                              pl.tent.as<unsigned int>(),
-                                              pl.diag.as<unsigned int>(), cs, ts, w[cur],
-                                              h[cur ^ 1], w[cur ^ 1], st));
+        OMNI_TRY_HIP(ctx, launch_adj_src_iter(nS, tptr, tent, diagp, cs, ts, w[cur], h[cur ^ 1],
+                                              w[cur ^ 1], st));
         cur ^= 1;
       }
-      OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, pl.slist.as<int>(), h[cur], z, st));
+      OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, h[cur], z, st));
     }
   } else {
     for (int m = 1; m < P.max_iterations; m++) {

@@ -238,7 +238,8 @@ k_adj_narrow(const unsigned long long* __restrict__ key, int64_t n, unsigned int
     out[i] = (unsigned int)key[i];
 }
 
-// t_S = t at the source voxels; h_1 = t_S, w_1 = h_1 / C, and C per source.
+// t_S = t at the source voxels; h_1 = t_S, w_1 = h_1 / C (0 where C = 0), and
+// C per source.
 __global__ void __launch_bounds__(256)
 k_adj_src_init(int nS, const int* __restrict__ slist, const double* __restrict__ t,
                const unsigned int* __restrict__ cnt, double* __restrict__ ts,
@@ -250,7 +251,8 @@ k_adj_src_init(int nS, const int* __restrict__ slist, const double* __restrict__
   const double x = t[v];
   ts[s] = x;
   h[s] = x;
-  w[s] = x / c;
+  // a source no visit updates keeps p = 0 and passes nothing on
+  w[s] = c > 0.0 ? x / c : 0.0;
   cs[s] = c;
 }
 
@@ -272,7 +274,7 @@ k_adj_src_iter(int nS, const unsigned long long* __restrict__ tptr,
   if (lane == 0) {
     const double h = ts[r] + (double)diag[r] * w[r] + acc;
     hn[r] = h;
-    wn[r] = h / cs[r];
+    wn[r] = cs[r] > 0.0 ? h / cs[r] : 0.0;
   }
 }
 
@@ -283,6 +285,94 @@ k_adj_src_out(int nS, const int* __restrict__ slist, const double* __restrict__ 
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nS) return;
   z[slist[s]] += h[s];
+}
+
+// The plain-mask rule's version: its runs start at valid voxels after a
+// masked one (anywhere in the volume, not only at ray entries), have no
+// entry self-visit, and the sources S are the run-start voxels.
+
+// marks every run's first voxel
+__global__ void __launch_bounds__(256)
+k_adj_mask_starts(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+                  GridDesc g, const double* __restrict__ mask, uint8_t* __restrict__ flag) {
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  SegWalker<true> sw;
+  sw.w.init(rays[r], dirs, g);
+  sw.mask = mask;
+  bool first = true;
+  while (!sw.done && sw.next_source(first)) {
+    first = false;
+    flag[sw.w.v] = 1;
+    int a, s;
+    int64_t l;
+    while (sw.step(&a, &s, &l)) {
+    }
+  }
+}
+
+// pass 0: counts each run's visits to run-start voxels into its start's row;
+// pass 1: writes them (the run is walked twice from a saved walker state)
+template <int kPass>
+__global__ void __launch_bounds__(256)
+k_adj_mask_rows(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+                GridDesc g, const double* __restrict__ mask, const int* __restrict__ sid,
+                const unsigned long long* __restrict__ tptr, unsigned int* __restrict__ cnt,
+                unsigned long long* __restrict__ key) {
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  SegWalker<true> sw;
+  sw.w.init(rays[r], dirs, g);
+  sw.mask = mask;
+  bool first = true;
+  while (!sw.done && sw.next_source(first)) {
+    first = false;
+    const int row = sid[sw.w.v];
+    const Walker w0 = sw.w;
+    unsigned int n = 0;
+    int a, s;
+    int64_t l;
+    while (sw.step(&a, &s, &l))
+      if (sid[sw.w.v] >= 0) n++;
+    if (n == 0) continue;
+    if (kPass == 0) {
+      atomicAdd(&cnt[row], n);
+      continue;
+    }
+    const Walker wend = sw.w;
+    const bool end_done = sw.done;
+    const unsigned long long base = tptr[row] + atomicAdd(&cnt[row], n);
+    sw.w = w0;
+    sw.done = false;
+    unsigned int i = 0;
+    while (i < n && sw.step(&a, &s, &l)) {
+      const int q = sid[sw.w.v];
+      if (q >= 0) key[base + i++] = (unsigned long long)(unsigned int)q;
+    }
+    sw.w = wend;
+    sw.done = end_done;
+  }
+}
+
+hipError_t launch_adj_mask_starts(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                  GridDesc g, const double* mask, uint8_t* flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_mask_starts, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays,
+                     nrays, dirs, g, mask, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_mask_rows(int pass, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                GridDesc g, const double* mask, const int* sid,
+                                const unsigned long long* tptr, unsigned int* cnt,
+                                unsigned long long* key, hipStream_t st) {
+  const unsigned int grid = grid_for(nrays, 1u << 30);
+  if (pass == 0)
+    hipLaunchKernelGGL(k_adj_mask_rows<0>, dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g,
+                       mask, sid, tptr, cnt, key);
+  else
+    hipLaunchKernelGGL(k_adj_mask_rows<1>, dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g,
+                       mask, sid, tptr, cnt, key);
+  return hipGetLastError();
 }
 
 hipError_t launch_adj_transpose_rows(const RayRec* rays, int64_t nrays, const DirConst* dirs,

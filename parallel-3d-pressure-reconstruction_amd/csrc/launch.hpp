@@ -96,6 +96,12 @@ hipError_t launch_adj_transpose_fill(const RayRec* rays, int64_t nrays, const Di
                                      GridDesc g, const int* sid, const unsigned long long* tptr,
                                      unsigned int* cursor, unsigned long long* key,
                                      hipStream_t st);
+hipError_t launch_adj_mask_starts(const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                  GridDesc g, const double* mask, uint8_t* flag, hipStream_t st);
+hipError_t launch_adj_mask_rows(int pass, const RayRec* rays, int64_t nrays, const DirConst* dirs,
+                                GridDesc g, const double* mask, const int* sid,
+                                const unsigned long long* tptr, unsigned int* cnt,
+                                unsigned long long* key, hipStream_t st);
 hipError_t launch_adj_narrow(const unsigned long long* key, int64_t n, unsigned int* out,
                              hipStream_t st);
 hipError_t launch_adj_src_init(int nS, const int* slist, const double* t, const unsigned int* cnt,

@@ -7,9 +7,10 @@ pressure, K iterations):
     column is one oracle reconstruction of a unit gradient), rel <= 1e-12;
   * central finite differences on 64 random unit directions in grad-p space
     at the masked Fourier-sphere workload (32^3), rel <= 1e-6;
-  * the source-restricted backward of the plain method (Horner steps over
-    the source voxels with the transposed source rows) against the
-    full-volume Horner sum, rel <= 1e-12, at 48^3 with 162 directions;
+  * the source-restricted backward (Horner steps over the source voxels --
+    ray entries for plain, run starts for plain-mask -- with the transposed
+    source rows) against the full-volume Horner sum, rel <= 1e-12, at 48^3
+    with 162 directions;
   * FP32-Kahan forward/backward within 1e-5 of FP64;
   * a torch.autograd.Function over the pair fits a gradient correction to
     512 sparse pressure probes with Tikhonov lambda = 1e-3 (50 steps).
@@ -61,7 +62,7 @@ def sphere(shape, r):
 
 
 SOLVERS = [(omni3d.PLAIN, omni3d.NAIVE), (omni3d.PLAIN, omni3d.AFFINE),
-           (omni3d.PLAIN_MASK, omni3d.NAIVE)]
+           (omni3d.PLAIN_MASK, omni3d.NAIVE), (omni3d.PLAIN_MASK, omni3d.AFFINE)]
 
 
 @pytest.mark.parametrize("method,solver", SOLVERS)
@@ -139,15 +140,19 @@ def test_finite_differences_fourier_sphere(gpu_available, ref_dirs3):
     assert worst < 1e-6
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 25])
-def test_source_restricted_backward(gpu_available, ref_dirs3, K):
+@pytest.mark.parametrize("method,K", [(omni3d.PLAIN, 1), (omni3d.PLAIN, 2), (omni3d.PLAIN, 3),
+                                      (omni3d.PLAIN, 25), (omni3d.PLAIN_MASK, 2),
+                                      (omni3d.PLAIN_MASK, 25)])
+def test_source_restricted_backward(gpu_available, ref_dirs3, method, K):
     import torch
     n = 48
-    print(f"case adjoint source-restricted vs full-volume Horner K={K}", flush=True)
+    print(f"case adjoint source-restricted vs full-volume Horner method={method} K={K}",
+          flush=True)
     f = F.fourier_sphere(n, seed=3)
     shape = (n, n, n)
-    full = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN, K)
-    src = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN, K, solver=omni3d.AFFINE)
+    m = f["mask"] if method == omni3d.PLAIN_MASK else None
+    full = Adjoint(shape, ref_dirs3, f["delta"], method, K, mask=m)
+    src = Adjoint(shape, ref_dirs3, f["delta"], method, K, mask=m, solver=omni3d.AFFINE)
     gen = torch.Generator(device="cuda").manual_seed(5)
     y = torch.randn(full.V, dtype=torch.float64, device="cuda", generator=gen)
     a, b = src.backward(y), full.backward(y)
