@@ -14,7 +14,9 @@ reconstruction must have whatever its size, all through the C-ABI on the GPU:
 * the FP32 source iteration in increment form agrees with gathering p_S
   itself (128^3, rel-L2 <= 1e-7);
 * the masked methods' tile engine agrees with the one-lane-per-ray walk on
-  the 128^3 masked Fourier sphere (FP64, rel-L2 <= 1e-12, equal counts).
+  the 128^3 masked Fourier sphere (FP64, rel-L2 <= 1e-12, equal counts);
+* the affine solver agrees with the reference's schedule (a traversal per
+  iteration) on a 160 x 128 x 96 grid (FP64, rel-L2 <= 1e-12).
 """
 import numpy as np
 import pytest
@@ -153,4 +155,28 @@ def test_masked_tile_engine_128(gpu_available, dirs162, monkeypatch, method):
         assert F.rel_l2(wt, wn) < 1e-12
     err = F.rel_l2(pt, pn)
     print("tile engine vs lane walk rel-L2", err)
+    assert err < 1e-12
+
+
+def test_affine_vs_reference_schedule_noncubic(gpu_available, dirs162):
+    """The affine solver (source iteration + one final traversal) against the
+    reference's schedule (a full traversal per iteration, tile engine) on the
+    largest grid of the snapshot series, 160 x 128 x 96, FP64, 30 iterations:
+    two independent schedules of the same iteration."""
+    shape = (96, 128, 160)
+    print("case affine vs reference schedule 160x128x96 fp64", flush=True)
+    f = F.taylor_green(160, noise=0.03, seed=11, shape=shape, phase=(0.3, 1.1, 2.0))
+    p0 = np.random.default_rng(12).normal(size=shape)
+    out = []
+    for solver in (omni3d.AFFINE, omni3d.NAIVE):
+        with omni3d.Context(shape, f["delta"], density=f["rho"], precision=omni3d.FP64,
+                            solver=solver, max_iterations=30, tolerance=0.0) as ctx:
+            ctx.set_directions(dirs162)
+            ctx.set_field(f["acc"])
+            out.append(ctx.reconstruct(p0))
+    (pa, wa, ra), (pn, wn, rn) = out
+    assert ra.iterations == rn.iterations == 30
+    np.testing.assert_array_equal(wa, wn)
+    err = F.rel_l2(pa, pn)
+    print("affine vs reference schedule rel-L2", err)
     assert err < 1e-12
