@@ -576,6 +576,13 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   const void* field = ctx->field.as<void>();
   const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
   const WalkParams wp = walk_params(P);
+  // phase 1: up to the source iteration; phase 2: from its end (the batch
+  // runs the iterations of all snapshots in between); 0: everything
+  const int phase = ctx->affine_phase;
+  const size_t ns = (size_t)std::max(pl.nS, 1);
+  OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
+  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
+  if (phase != 2) {
   OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
   // max |a| over the packed field (the fourth component is 0)
   OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
@@ -584,10 +591,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
                                          (double)P.max_iterations, st));
   // pass A: trapezoid sums landing on source voxels
-  const size_t ns = (size_t)std::max(pl.nS, 1);
-  OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->bs.as<void>(), 2 * (int64_t)ns, 0u, st));
-  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
   record_event(ctx, 2);
   if (ctx->engine == 0) {
     // segment sums over all tiles, per-ray carries, then the source sums on
@@ -615,6 +619,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     record_event(ctx, 8);
   }
   record_event(ctx, 3);
+  }
   // source iteration
   // p_S padded to whole 16-source lines (the staged iteration copies lines)
   const size_t ns16 = (ns + 15) & ~(size_t)15;
@@ -625,9 +630,11 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                         std::max(iterate_grid(pl.nS),
                                  std::max(staged_grid(pl.nS),
                                           std::max(dense_grid(pl.nS), 2 * grouped_grid(pl.nS))))));
-  OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
-  // max |p_S| of the starting values: the fixed-point scale of iteration 1
-  OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->ps0.as<double>(), pl.nS, &sc->itmax_bits[0], st));
+  if (phase != 2) {
+    OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
+    // max |p_S| of the starting values: the fixed-point scale of iteration 1
+    OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->ps0.as<double>(), pl.nS, &sc->itmax_bits[0], st));
+  }
   double* pbuf[2] = {ctx->ps0.as<double>(), ctx->ps1.as<double>()};
   // the grouped dense iteration runs one launch ahead of its stop rule, so
   // p_S rotates over three buffers and p_S^{k*-1} is copied out after the loop
@@ -647,6 +654,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     OMNI_TRY_HIP(ctx, ctx->dinc1.alloc(sizeof(float) * ns16));
   }
   float* dbuf[2] = {ctx->dinc0.as<float>(), ctx->dinc1.as<float>()};
+  if (phase == 1) return OMNI3D_OK;
   auto iterations = [&](hipStream_t s) -> hipError_t {
     StagedIter si;
     si.nS = pl.nS;
@@ -705,7 +713,19 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     }
     return hipSuccess;
   };
-  if (ctx->capturing) {
+  if (phase == 2) {
+    // the iterations ran elsewhere: the stop rule of the last one, then
+    // p_S^{k*-1} for pass B
+    if (rot3) {
+      const int k = P.max_iterations;
+      const unsigned int gg = grouped_grid(pl.nS);
+      OMNI_TRY_HIP(ctx, launch_iter_finish(k, (int)gg,
+                                           ctx->partials.as<double>() + (size_t)(k & 1) * 2 * gg,
+                                           sc, P.max_iterations, P.tolerance, st));
+      OMNI_TRY_HIP(ctx, launch_select_ps(pl.nS, pb3[0], pb3[1], pb3[2], sc,
+                                         ctx->psel.as<double>(), st));
+    }
+  } else if (ctx->capturing) {
     OMNI_TRY_HIP(ctx, iterations(st));
   } else {
     // the iteration launches are replayed from a graph captured once per set
@@ -930,6 +950,45 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
 // Queues a whole reconstruction of the context's field on its stream (no
 // host synchronisation; every buffer must already have its size when this
 // runs under graph capture).  The result goes to pdst / wdst (device, dtype).
+// Whether a reconstruction's source iteration can run inside a
+// multi-snapshot launch: the plain method on the affine solver with the
+// grouped dense iteration (tile engine, p_S over three buffers).
+bool affine_fusable(const omni3d_ctx* ctx) {
+  const omni3d_params& P = ctx->prm;
+  const Plan& pl = *ctx->plan;
+  return P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN && pl.valid &&
+         pl.dense && pl.dense_group == 4 && !pl.staged && ctx->engine == 0 &&
+         P.max_iterations > 0 && pl.nS > 0;
+}
+
+// The iteration descriptor of a prepared context (buffers sized by an
+// earlier reconstruction), seedless (zero initial pressure).
+IterDesc affine_iter_desc(omni3d_ctx* ctx, int first) {
+  const Plan& pl = *ctx->plan;
+  const bool inc = ctx->prm.precision == OMNI3D_FP32_KAHAN && !std::getenv("OMNI3D_NO_INCREMENTS");
+  IterDesc d{};
+  d.nS = pl.nS;
+  d.nb = (int)grouped_grid(pl.nS);
+  d.first = first;
+  d.seed = 0;
+  d.wptr = pl.dense_wptr.as<unsigned int>();
+  d.dent = pl.dent.as<unsigned int>();
+  d.diag = pl.diag.as<unsigned int>();
+  d.slist = pl.slist.as<int>();
+  d.cnt = pl.cnt.as<unsigned int>();
+  d.bs = ctx->bs.as<long long>();
+  d.pinit = nullptr;
+  d.pb[0] = ctx->ps0.as<double>();
+  d.pb[1] = ctx->ps1.as<double>();
+  d.pb[2] = ctx->ps2.as<double>();
+  d.srow = inc ? ctx->srow.as<double>() : nullptr;
+  d.db[0] = inc ? ctx->dinc0.as<float>() : nullptr;
+  d.db[1] = inc ? ctx->dinc1.as<float>() : nullptr;
+  d.sc = ctx->scal.as<DevScalars>();
+  d.partials = ctx->partials.as<double>();
+  return d;
+}
+
 int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const omni3d_params& P = ctx->prm;
   hipStream_t st = ctx->stream;
@@ -942,6 +1001,7 @@ int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) 
   if (P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN) {
     rc = fp32 ? run_affine<float>(ctx, seed) : run_affine<double>(ctx, seed);
     if (rc) return rc;
+    if (ctx->affine_phase == 1) return OMNI3D_OK;
     OMNI_TRY_HIP(ctx, launch_finalize(V, ctx->sum.as<long long>(), ctx->used_copies,
                                       ctx->plan->cnt.as<unsigned int>(),
                                       seed >= 0 ? ctx->pinit.as<double>() : nullptr, seed > 0 ? 1 : 0,

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <string>
@@ -36,7 +37,11 @@ struct omni3d_batch {
   hipStream_t main = nullptr;
   std::vector<hipStream_t> br;
   std::vector<hipEvent_t> br_ev;
-  hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+  hipEvent_t fork = nullptr, fork2 = nullptr, t0 = nullptr, t1 = nullptr;
+  // one launch per source iteration for every snapshot (when all can):
+  // the snapshots' descriptors and the snapshot of each workgroup
+  DevBuf desc, blk;
+  unsigned int nblk = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool prepared = false;     // plans shared, buffers sized
@@ -49,6 +54,7 @@ struct omni3d_batch {
     for (hipEvent_t e : br_ev) (void)hipEventDestroy(e);
     for (hipStream_t s : br) (void)hipStreamDestroy(s);
     if (fork) (void)hipEventDestroy(fork);
+    if (fork2) (void)hipEventDestroy(fork2);
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
     if (main) (void)hipStreamDestroy(main);
@@ -147,34 +153,86 @@ int prepare(omni3d_batch* b) {
   return OMNI3D_OK;
 }
 
-int capture(omni3d_batch* b) {
-  drop_graph(b);
-  BATCH_TRY_HIP(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+// Uploads the iteration descriptors when every snapshot's source iteration
+// can share one launch per iteration (same precision for all); else 0.
+int fused_descriptors(omni3d_batch* b, bool* fused) {
+  *fused = false;
+  if (std::getenv("OMNI3D_BATCH_SEPARATE")) return OMNI3D_OK;
+  for (omni3d_ctx* c : b->snap)
+    if (!affine_fusable(c) || c->prm.precision != b->snap[0]->prm.precision) return OMNI3D_OK;
+  std::vector<IterDesc> d;
+  std::vector<unsigned int> blk;
+  for (int i = 0; i < (int)b->snap.size(); i++) {
+    d.push_back(affine_iter_desc(b->snap[i], (int)blk.size()));
+    blk.insert(blk.end(), (size_t)d.back().nb, (unsigned int)i);
+  }
+  BATCH_TRY_HIP(b, b->desc.alloc(sizeof(IterDesc) * d.size()));
+  BATCH_TRY_HIP(b, b->blk.alloc(sizeof(unsigned int) * blk.size()));
+  BATCH_TRY_HIP(b, hipMemcpy(b->desc.as<void>(), d.data(), sizeof(IterDesc) * d.size(),
+                             hipMemcpyHostToDevice));
+  BATCH_TRY_HIP(b, hipMemcpy(b->blk.as<void>(), blk.data(), sizeof(unsigned int) * blk.size(),
+                             hipMemcpyHostToDevice));
+  b->nblk = (unsigned int)blk.size();
+  *fused = true;
+  return OMNI3D_OK;
+}
+
+// Queues every snapshot's reconstruction (phase: as in run_affine) on its
+// branch, forked from and joined back into the main stream.
+int enqueue_branches(omni3d_batch* b, int phase, hipEvent_t fork_ev) {
+  hipError_t e = hipEventRecord(fork_ev, b->main);
+  for (int k = 0; e == hipSuccess && k < kBranches; k++) e = hipStreamWaitEvent(b->br[k], fork_ev, 0);
+  if (e != hipSuccess) return bfail(b, OMNI3D_ERR_HIP, std::string("fork: ") + hipGetErrorString(e));
   int rc = OMNI3D_OK;
-  hipError_t e = hipEventRecord(b->fork, b->main);
-  for (int k = 0; e == hipSuccess && k < kBranches; k++) e = hipStreamWaitEvent(b->br[k], b->fork, 0);
-  for (int i = 0; e == hipSuccess && rc == OMNI3D_OK && i < (int)b->snap.size(); i++) {
+  for (int i = 0; rc == OMNI3D_OK && i < (int)b->snap.size(); i++) {
     omni3d_ctx* c = b->snap[i];
     c->stream = b->br[b->branch[i]];
     c->capturing = true;
+    c->affine_phase = phase;
     rc = enqueue_solve(c, -1, b->out[i]->as<void>(), nullptr, OMNI3D_F64);
+    c->affine_phase = 0;
     c->capturing = false;
     c->stream = b->main;
   }
+  if (rc) return bfail(b, rc, "capture: " + g_last_error);
   for (int k = 0; e == hipSuccess && k < kBranches; k++) {
     e = hipEventRecord(b->br_ev[k], b->br[k]);
     if (e == hipSuccess) e = hipStreamWaitEvent(b->main, b->br_ev[k], 0);
+  }
+  if (e != hipSuccess) return bfail(b, OMNI3D_ERR_HIP, std::string("join: ") + hipGetErrorString(e));
+  return OMNI3D_OK;
+}
+
+// Captures the batch: passes A of all snapshots over the branches, then --
+// when all snapshots allow it -- one launch per source iteration for every
+// snapshot on the main stream, then passes B over the branches; otherwise
+// each branch runs its snapshots' whole reconstructions.
+int capture(omni3d_batch* b) {
+  drop_graph(b);
+  bool fused = false;
+  int rc = fused_descriptors(b, &fused);
+  if (rc) return rc;
+  BATCH_TRY_HIP(b, hipStreamBeginCapture(b->main, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_branches(b, fused ? 1 : 0, b->fork);
+  if (rc == OMNI3D_OK && fused) {
+    const omni3d_params& P = b->snap[0]->prm;
+    const int inc = P.precision == OMNI3D_FP32_KAHAN && !std::getenv("OMNI3D_NO_INCREMENTS");
+    hipError_t e = hipSuccess;
+    for (int k = 1; e == hipSuccess && k <= P.max_iterations; k++)
+      e = launch_iterate_multi(inc, k, b->desc.as<IterDesc>(), b->blk.as<unsigned int>(), b->nblk,
+                               P.max_iterations, P.tolerance, b->main);
+    if (e != hipSuccess) rc = bfail(b, OMNI3D_ERR_HIP, std::string("iterations: ") + hipGetErrorString(e));
+    if (rc == OMNI3D_OK) rc = enqueue_branches(b, 2, b->fork2);
   }
   hipGraph_t g = nullptr;
   const hipError_t e2 = hipStreamEndCapture(b->main, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
-    return bfail(b, rc, "capture: " + g_last_error);
+    return rc;
   }
-  if (e != hipSuccess || e2 != hipSuccess) {
+  if (e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
-    return bfail(b, OMNI3D_ERR_HIP, std::string("graph capture: ") +
-                                        hipGetErrorString(e != hipSuccess ? e : e2));
+    return bfail(b, OMNI3D_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e2));
   }
   b->graph = g;
   BATCH_TRY_HIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -215,6 +273,7 @@ int omni3d_batch_create(const omni3d_params* base, int count, const int* nx, con
     if (e == hipSuccess) b->br_ev.push_back(ev);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&b->t0);
   if (e == hipSuccess) e = hipEventCreate(&b->t1);
   for (int i = 0; e == hipSuccess && i < count; i++)

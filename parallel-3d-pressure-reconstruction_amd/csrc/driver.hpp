@@ -7,6 +7,7 @@
 #include <string>
 
 #include "host.hpp"
+#include "stage.hpp"
 
 namespace omni3d {
 
@@ -29,6 +30,9 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
 // A whole reconstruction queued on the context's stream without host
 // synchronisation (capturable once the buffers are sized); pdst / wdst device.
 int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype);
+// multi-snapshot source iteration (snapshot batches)
+bool affine_fusable(const omni3d_ctx* ctx);
+IterDesc affine_iter_desc(omni3d_ctx* ctx, int first);
 // Releases the context's cached graph of the iteration launches.
 void drop_iteration_graph(omni3d_ctx* ctx);
 // Phase timing event i of the context (skipped during graph capture).

@@ -139,6 +139,7 @@ struct omni3d_ctx {
   int dense_variant = 0;                     // dense iteration: wavefronts x round size
   int dense_group = 4;                       // dense iteration: chunks per row group
   bool capturing = false;                    // work is being captured into a graph
+  int affine_phase = 0;                      // 1: up to the source iteration, 2: after it
   // graph of the source-iteration launches, reused while its arguments stay
   hipStream_t cap_stream = nullptr;
   hipGraph_t it_graph = nullptr;

@@ -536,24 +536,26 @@ __device__ void finish_iteration(int k, int nparts, const double* __restrict__ p
 // - p_old; kInc == 2 gathers the previous increments din (4 bytes per entry)
 // and adds their sum to S.  The fixed-point scale then follows max |increment|
 // instead of max |p_S|, so the FP32 rounding applies to increments only.
+// The body of one workgroup: block lb of the nb workgroups of this
+// iteration (one snapshot's, in a multi-snapshot launch).
 template <int kG, int kInc>
-__global__ void __launch_bounds__(64 * kG)
-k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
-              const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
-              const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
-              const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
-              const double* __restrict__ pold, double* __restrict__ pnew,
-              double* __restrict__ srow, const float* __restrict__ din, float* __restrict__ dnew,
-              DevScalars* __restrict__ sc, double* __restrict__ partials, int max_iter,
-              double tol) {
+__device__ __forceinline__ void iterate_grp_body(
+    const unsigned int lb, const unsigned int nb, int k, int nS,
+    const unsigned int* __restrict__ wptr, const unsigned int* __restrict__ dent,
+    const unsigned int* __restrict__ diag, const int* __restrict__ slist,
+    const unsigned int* __restrict__ cnt, const long long* __restrict__ bs,
+    const double* __restrict__ pinit, int seed, const double* __restrict__ pold,
+    double* __restrict__ pnew, double* __restrict__ srow, const float* __restrict__ din,
+    float* __restrict__ dnew, DevScalars* __restrict__ sc, double* __restrict__ partials,
+    int max_iter, double tol) {
   // the stop rule of iteration k - 1 runs in workgroup 0 of this launch: the
   // launch computes iteration k before knowing it (the buffers keep p_S of
   // k - 1), and launch k + 1 returns at once if k - 1 stopped
   if (sc->converged) return;
-  const double* __restrict__ prev_part = partials + (size_t)((k - 1) & 1) * 2 * gridDim.x;
-  partials += (size_t)(k & 1) * 2 * gridDim.x;
-  if (blockIdx.x == 0) {
-    if (k > 1) finish_iteration<64 * kG>(k - 1, gridDim.x, prev_part, sc, max_iter, tol);
+  const double* __restrict__ prev_part = partials + (size_t)((k - 1) & 1) * 2 * nb;
+  partials += (size_t)(k & 1) * 2 * nb;
+  if (lb == 0) {
+    if (k > 1) finish_iteration<64 * kG>(k - 1, nb, prev_part, sc, max_iter, tol);
     // the max slot launch k + 1 fills (launch k - 1 read it)
     if (threadIdx.x == 0) sc->itmax_bits[(k + 1) % 3] = 0ull;
   }
@@ -579,7 +581,7 @@ k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
   double d2 = 0.0, n2 = 0.0;
   unsigned long long pm = 0ull;
   const int ngr = (nS + kRows - 1) / kRows;
-  for (int g = blockIdx.x; g < ngr; g += gridDim.x) {
+  for (int g = (int)lb; g < ngr; g += (int)nb) {
     s_acc[threadIdx.x] = 0ull;
     __syncthreads();
     const int rbeg = g * kRows;
@@ -679,10 +681,42 @@ k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
       c += s_red[1][q];
       m = s_max[q] > m ? s_max[q] : m;
     }
-    partials[2 * blockIdx.x] = a;
-    partials[2 * blockIdx.x + 1] = c;
+    partials[2 * lb] = a;
+    partials[2 * lb + 1] = c;
     atomicMax(&sc->itmax_bits[k % 3], m);
   }
+}
+
+template <int kG, int kInc>
+__global__ void __launch_bounds__(64 * kG)
+k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
+              const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
+              const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
+              const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
+              const double* __restrict__ pold, double* __restrict__ pnew,
+              double* __restrict__ srow, const float* __restrict__ din, float* __restrict__ dnew,
+              DevScalars* __restrict__ sc, double* __restrict__ partials, int max_iter,
+              double tol) {
+  iterate_grp_body<kG, kInc>(blockIdx.x, gridDim.x, k, nS, wptr, dent, diag, slist, cnt, bs,
+                             pinit, seed, pold, pnew, srow, din, dnew, sc, partials, max_iter,
+                             tol);
+}
+
+// Iteration k of many snapshots in one launch (a snapshot batch): workgroup
+// b works for snapshot blk_snap[b] as its block b - first of nb, on that
+// snapshot's buffers (p_S rotating over three, increments over two, as in
+// the one-snapshot launches), so every snapshot's result is the one its own
+// launches give; a snapshot that stopped returns at once.
+template <int kG, int kInc>
+__global__ void __launch_bounds__(64 * kG)
+k_iterate_multi(int k, const IterDesc* __restrict__ desc, const unsigned int* __restrict__ blk_snap,
+                int max_iter, double tol) {
+  const IterDesc& q = desc[blk_snap[blockIdx.x]];
+  const unsigned int lb = blockIdx.x - (unsigned int)q.first;
+  iterate_grp_body<kG, kInc>(lb, (unsigned int)q.nb, k, q.nS, q.wptr, q.dent, q.diag, q.slist,
+                             q.cnt, q.bs, q.pinit, q.seed, q.pb[(k - 1) % 3], q.pb[k % 3],
+                             q.srow, kInc == 2 ? q.db[(k - 1) & 1] : nullptr,
+                             kInc != 0 ? q.db[k & 1] : nullptr, q.sc, q.partials, max_iter, tol);
 }
 
 // Relative change of iteration k from the workgroups' partial sums (fixed
@@ -1031,6 +1065,19 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
 #undef OMNI_DENSE
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol, 2);
+  return hipGetLastError();
+}
+
+hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsigned int* blk_snap,
+                                unsigned int nblocks, int max_iter, double tol, hipStream_t st) {
+  const int v = inc ? (k == 1 ? 1 : 2) : 0;
+#define OMNI_MULTI(I)                                                                        \
+  hipLaunchKernelGGL((k_iterate_multi<4, I>), dim3(nblocks), dim3(256), 0, st, k, desc, blk_snap, \
+                     max_iter, tol)
+  if (v == 1) OMNI_MULTI(1);
+  else if (v == 2) OMNI_MULTI(2);
+  else OMNI_MULTI(0);
+#undef OMNI_MULTI
   return hipGetLastError();
 }
 

@@ -109,6 +109,28 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
                                 hipStream_t st, double* srow = nullptr,
                                 const float* din = nullptr, float* dnew = nullptr);
+// One snapshot's share of a multi-snapshot launch of the grouped iteration:
+// its nb = grouped_grid(nS) workgroups start at block `first`; p_S rotates
+// over pb[0..2] and the increments over db[0..1] as in its own launches.
+struct IterDesc {
+  int nS, nb, first, seed;
+  const unsigned int* wptr;
+  const unsigned int* dent;
+  const unsigned int* diag;
+  const int* slist;
+  const unsigned int* cnt;
+  const long long* bs;
+  const double* pinit;
+  double* pb[3];
+  double* srow;
+  float* db[2];
+  DevScalars* sc;
+  double* partials;
+};
+// iteration k of every snapshot described (inc: increment form, FP32 ray
+// sums); blk_snap[b] = the snapshot of workgroup b
+hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsigned int* blk_snap,
+                                unsigned int nblocks, int max_iter, double tol, hipStream_t st);
 hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
                                  const int* slist, const unsigned int* cnt, const long long* bs,
                                  const double* pinit, int seed, const double* pold, double* pnew,
