@@ -102,3 +102,50 @@ def test_ray_engine_matches_oracle(gpu_available, ref_dirs2, monkeypatch):
     po, co, it, rel = run_oracle(f, ref_dirs2, shape, max_iter=7)
     np.testing.assert_array_equal(w, co)
     assert F.rel_l2(p, po) < 1e-12
+
+
+LAYOUTS = [
+    {"OMNI3D_DENSE_GROUP": "1"},
+    {"OMNI3D_DENSE_GROUP": "1", "OMNI3D_DENSE_VARIANT": "1"},
+    {"OMNI3D_DENSE_GROUP": "1", "OMNI3D_DENSE_VARIANT": "2"},
+    {"OMNI3D_DENSE_GROUP": "1", "OMNI3D_DENSE_VARIANT": "3"},
+    {"OMNI3D_DENSE_GROUP": "1", "OMNI3D_DENSE_VARIANT": "4"},
+    {"OMNI3D_ITER": "slots"},
+    {"OMNI3D_ITER": "slots", "OMNI3D_ITER_WAVES": "1"},
+    {"OMNI3D_ITER": "slots", "OMNI3D_ITER_WAVES": "4"},
+    {"OMNI3D_ITER": "slots", "OMNI3D_ITER_WAVES": "-2"},
+    {"OMNI3D_ITER": "staged"},
+    {"OMNI3D_NO_INCREMENTS": "1"},
+]
+
+
+@pytest.mark.parametrize("env", LAYOUTS, ids=lambda e: ",".join(f"{k[7:]}={v}" for k, v in e.items()))
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_iteration_layouts_agree(gpu_available, ref_dirs3, monkeypatch, env, precision):
+    """Every selectable layout of the source iteration (dense words per chunk
+    and their variants, masked sliced-ELL slots, LDS-staged lines, p_S
+    gathers instead of increments) against the default (merged words in
+    groups of four chunks) from a random initial pressure: FP64 rel-L2 <=
+    1e-12, FP32 <= 1e-6, same iteration count."""
+    shape = (32, 34, 36)
+    print(f"case iteration layout {env} precision={precision}", flush=True)
+    f = F.taylor_green(36, noise=0.05, seed=13, shape=shape)
+    p0 = np.random.default_rng(14).normal(size=shape)
+    dtype = np.float64 if precision == omni3d.FP64 else np.float32
+
+    def go():
+        with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision,
+                            max_iterations=25, tolerance=0.0) as ctx:
+            ctx.set_directions(ref_dirs3)
+            ctx.set_field([np.asarray(a, dtype=dtype) for a in f["acc"]], dtype=dtype)
+            return ctx.reconstruct(p0)
+
+    pd, wd, rd = go()
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    pv, wv, rv = go()
+    assert rv.iterations == rd.iterations == 25
+    np.testing.assert_array_equal(wv, wd)
+    err = F.rel_l2(pv, pd)
+    print("layout vs default rel-L2", err)
+    assert err < (1e-12 if precision == omni3d.FP64 else 1e-6)
