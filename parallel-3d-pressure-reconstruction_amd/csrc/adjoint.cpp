@@ -10,6 +10,7 @@
 #include "driver.hpp"
 #include "host.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 
 using namespace omni3d;
 
@@ -210,6 +211,29 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   else
     OMNI_TRY_HIP(ctx, (launch_convert<float, double>(static_cast<const float*>(dldp_d), y, V, st)));
   OMNI_TRY_HIP(ctx, (launch_convert<double, double>(y, z, V, st)));
+  // the plain method's transposed traversals run on the tile engine with the
+  // affine solver (the NAIVE solver keeps one lane per ray: the cross-check)
+  omni3d_params pg = P;
+  pg.density = -1.0;
+  const WalkParams wp = walk_params(pg);
+  const bool tiles = !mask_rule && P.solver == OMNI3D_SOLVER_AFFINE && pl.ntiles > 0;
+  if (tiles) OMNI_TRY_HIP(ctx, ctx->adj_seg.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
+  // t = the ray sums of u gathered into the rays' sources
+  auto gather = [&](const double* uu, double* tt) -> hipError_t {
+    if (!tiles)
+      return launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                               pl.g, mask, uu, tt, st);
+    hipError_t e = launch_adj_tiles(0, pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
+                                    pl.segs_slot.as<SegRec>(), nullptr, nullptr, nullptr, 0,
+                                    nullptr, nullptr, pl.g, pl.tg, wp, uu,
+                                    ctx->adj_seg.as<double>(), nullptr, nullptr, st);
+    if (e == hipSuccess)
+      e = launch_adj_tiles(1, pl.ntiles, nullptr, nullptr, nullptr, nullptr, pl.rays.as<RayRec>(),
+                           pl.nrays, pl.nseg.as<unsigned int>(),
+                           pl.val_off.as<unsigned long long>(), pl.g, pl.tg, wp, nullptr,
+                           ctx->adj_seg.as<double>(), tt, nullptr, st);
+    return e;
+  };
   // z = sum_{j<K} (A^T)^j y
   MaskRows mr;
   if (P.solver == OMNI3D_SOLVER_AFFINE) {
@@ -245,8 +269,7 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
       double* cs = ts + 5 * (size_t)nS;
       OMNI_TRY_HIP(ctx, launch_adj_scale(V, y, cnt, u, st));
       OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
-      OMNI_TRY_HIP(ctx, launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
-                                          pl.dirs.as<DirConst>(), pl.g, mask, u, t, st));
+      OMNI_TRY_HIP(ctx, gather(u, t));
       OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, slist, t, cnt, ts, h[0], w[0], cs, st));
       int cur = 0;
       for (int m = 2; m < P.max_iterations; m++) {
@@ -267,12 +290,26 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   }
   // dL/dg = B^T (z / C)
   OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
-  OMNI_TRY_HIP(ctx, launch_fill_f64(dg, 3 * V, 0.0, st));
-  omni3d_params pg = P;
-  pg.density = -1.0;
-  const WalkParams wp = walk_params(pg);
-  OMNI_TRY_HIP(ctx, launch_adj_scatter(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
-                                       pl.dirs.as<DirConst>(), pl.g, wp, mask, u, dg, V, st));
+  if (tiles) {
+    // per-segment sums, per-ray suffix sums, then the tiles' scatter (every
+    // voxel of dg stored)
+    OMNI_TRY_HIP(ctx, launch_adj_tiles(0, pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
+                                       pl.segs_slot.as<SegRec>(), nullptr, nullptr, nullptr, 0,
+                                       nullptr, nullptr, pl.g, pl.tg, wp, u,
+                                       ctx->adj_seg.as<double>(), nullptr, nullptr, st));
+    OMNI_TRY_HIP(ctx, launch_adj_tiles(1, pl.ntiles, nullptr, nullptr, nullptr, nullptr,
+                                       pl.rays.as<RayRec>(), pl.nrays, pl.nseg.as<unsigned int>(),
+                                       pl.val_off.as<unsigned long long>(), pl.g, pl.tg, wp,
+                                       nullptr, ctx->adj_seg.as<double>(), nullptr, nullptr, st));
+    OMNI_TRY_HIP(ctx, launch_adj_tiles(2, pl.ntiles, nullptr, nullptr,
+                                       pl.tile_ptr.as<unsigned long long>(), pl.segs.as<SegRec>(),
+                                       nullptr, 0, nullptr, nullptr, pl.g, pl.tg, wp, u,
+                                       ctx->adj_seg.as<double>(), nullptr, dg, st));
+  } else {
+    OMNI_TRY_HIP(ctx, launch_fill_f64(dg, 3 * V, 0.0, st));
+    OMNI_TRY_HIP(ctx, launch_adj_scatter(mask_rule, pl.rays.as<RayRec>(), pl.nrays,
+                                         pl.dirs.as<DirConst>(), pl.g, wp, mask, u, dg, V, st));
+  }
   if (dt == OMNI3D_F64)
     OMNI_TRY_HIP(ctx, (launch_convert<double, double>(dg, static_cast<double*>(dldgrad_d), 3 * V, st)));
   else

@@ -19,6 +19,7 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 #include "walk.hpp"
 
 namespace omni3d {
@@ -416,6 +417,194 @@ hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const uns
 hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, double* z,
                               hipStream_t st) {
   hipLaunchKernelGGL(k_adj_src_out, dim3((nS + 255) / 256), dim3(256), 0, st, nS, slist, h, z);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The plain method's transposed traversals on the tile engine.  With u a
+// volume (z / C), step tau of a ray (from voxel l to n along axis a)
+// scatters h_a * S_tau into dL/dg_a at l and n, S_tau = the sum of u over the
+// ray's visits from n on; the gather adds the sum of u over all of a ray's
+// visits (its entry included) into its source.  Per segment (ray x tile):
+//   k_adj_seg_u      the sum U of u over the segment's visits (u in LDS);
+//   k_adj_rev_scan   per ray, T = the sum of U over the later segments (in
+//                    place), and the ray's total into its source voxel;
+//   k_adj_tile_scatter  per tile, S of every step from U, T and the running
+//                    prefix, added into LDS accumulators of the three
+//                    components (the step into a tile counts at its first
+//                    voxel here, at its last voxel in the tile it leaves).
+
+template <int kBits>
+__global__ void __launch_bounds__(1024)
+k_adj_seg_u(const unsigned long long* __restrict__ tile_ptr1, const SegRec* __restrict__ segs,
+            GridDesc g, TileGrid tg, const double* __restrict__ u, double* __restrict__ useg) {
+  constexpr int kTile = 1 << kBits, kTileMask = kTile - 1, kVox = kTile * kTile * kTile;
+  __shared__ double U[kVox];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    U[lds_slot<kBits>(l)] = (i < g.X && j < g.Y && k < g.Z) ? u[i + j * sy + k * sz] : 0.0;
+  }
+  __syncthreads();
+  const unsigned long long b = tile_ptr1[tile], e = tile_ptr1[tile + 1];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const int d0 = (sg & 1u) ? 1 : -1, d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    double sum = U[lds_slot<kBits>(lv)];
+    for (unsigned int q = 1; q < n; q++) {
+      const unsigned int c = q - 1;
+      const uint32_t w = c < 16 ? sr.code[0] : (c < 32 ? sr.code[1] : sr.code[2]);
+      const int a = (int)((w >> (2 * (c & 15))) & 3u);
+      lv += a == 0 ? d0 : (a == 1 ? d1 : d2);
+      sum += U[lds_slot<kBits>(lv)];
+    }
+    useg[sr.g] = sum;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_adj_rev_scan(const RayRec* __restrict__ rays, int64_t nrays, GridDesc g,
+               const unsigned int* __restrict__ nseg, const unsigned long long* __restrict__ val_off,
+               double* __restrict__ useg, double* __restrict__ tvol) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const unsigned long long b = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int n = nseg[r];
+  double t = 0.0;
+  for (unsigned int j = n; j-- > 0;) {
+    const double x = useg[b + 64ull * j];
+    useg[b + 64ull * j] = t;
+    t += x;
+  }
+  if (tvol) {
+    int c[3];
+    unpack_voxel(rays[r].in, c);
+    unsafeAtomicAdd(&tvol[(int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y], t);
+  }
+}
+
+template <int kBits>
+__global__ void __launch_bounds__(1024)
+k_adj_tile_scatter(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
+                   GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ u,
+                   const double* __restrict__ tafter, double* __restrict__ dg) {
+  constexpr int kTile = 1 << kBits, kTileMask = kTile - 1, kVox = kTile * kTile * kTile;
+  __shared__ double U[kVox];
+  __shared__ double A[3][kVox];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    const int sl = lds_slot<kBits>(l);
+    U[sl] = (i < g.X && j < g.Y && k < g.Z) ? u[i + j * sy + k * sz] : 0.0;
+    A[0][sl] = A[1][sl] = A[2][sl] = 0.0;
+  }
+  __syncthreads();
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    const int lv0 = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const bool first = (sr.info >> 21) & 1u, last = (sr.info >> 22) & 1u;
+    const bool up[3] = {(sg & 1u) != 0, (sg & 2u) != 0, (sg & 4u) != 0};
+    const int d0 = up[0] ? 1 : -1, d1 = up[1] ? kTile : -kTile;
+    const int d2 = up[2] ? kTile * kTile : -kTile * kTile;
+    auto code = [&](unsigned int c) {
+      const uint32_t w = c < 16 ? sr.code[0] : (c < 32 ? sr.code[1] : sr.code[2]);
+      return (int)((w >> (2 * (c & 15))) & 3u);
+    };
+    auto hfac = [&](int a) {
+      return a == 0 ? step_factor<double>(wp, 0, up[0])
+                    : (a == 1 ? step_factor<double>(wp, 1, up[1]) : step_factor<double>(wp, 2, up[2]));
+    };
+    const double T = tafter[sr.g];
+    // the segment's sum of u
+    double Useg = 0.0;
+    {
+      int lv = lv0;
+      Useg = U[lds_slot<kBits>(lv)];
+      for (unsigned int q = 1; q < n; q++) {
+        const int a = code(q - 1);
+        lv += a == 0 ? d0 : (a == 1 ? d1 : d2);
+        Useg += U[lds_slot<kBits>(lv)];
+      }
+    }
+    int lv = lv0;
+    int sl = lds_slot<kBits>(lv);
+    if (!first) {
+      // the step into the tile: S = every visit from this segment's first on
+      const int ae = (int)((sr.info >> 23) & 3u);
+      atomicAdd(&A[ae][sl], hfac(ae) * (Useg + T));
+    }
+    double P = U[sl];  // u of the visits before the next step's target
+    for (unsigned int q = 1; q < n; q++) {
+      const int a = code(q - 1);
+      const int ln = lv + (a == 0 ? d0 : (a == 1 ? d1 : d2));
+      const int sn = lds_slot<kBits>(ln);
+      const double hS = hfac(a) * ((Useg - P) + T);
+      atomicAdd(&A[a][sl], hS);
+      atomicAdd(&A[a][sn], hS);
+      P += U[sn];
+      lv = ln;
+      sl = sn;
+    }
+    if (!last) {
+      // the step out of the tile: S = the later segments' visits
+      const int ax = code(n - 1);
+      atomicAdd(&A[ax][sl], hfac(ax) * T);
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    if (i >= g.X || j >= g.Y || k >= g.Z) continue;
+    const int64_t v = i + j * sy + k * sz;
+    const int sl = lds_slot<kBits>(l);
+    dg[v] = A[0][sl];
+    dg[V + v] = A[1][sl];
+    dg[2 * V + v] = A[2][sl];
+  }
+}
+
+hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile_ptr1,
+                            const SegRec* segs_slot, const unsigned long long* tile_ptr,
+                            const SegRec* segs, const RayRec* rays, int64_t nrays,
+                            const unsigned int* nseg, const unsigned long long* val_off,
+                            GridDesc g, TileGrid tg, const WalkParams& wp, const double* u,
+                            double* useg, double* tvol, double* dg, hipStream_t st) {
+  // what 0: segment sums of u; 1: reverse scan (+ ray totals into tvol);
+  // 2: the scatter into dg
+  if (what == 1) {
+    hipLaunchKernelGGL(k_adj_rev_scan, dim3((unsigned int)((nrays + 255) / 256)), dim3(256), 0, st,
+                       rays, nrays, g, nseg, val_off, useg, tvol);
+    return hipGetLastError();
+  }
+  if (what == 0) {
+    if (tg.bits == 3)
+      hipLaunchKernelGGL(k_adj_seg_u<3>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs_slot, g,
+                         tg, u, useg);
+    else
+      hipLaunchKernelGGL(k_adj_seg_u<4>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs_slot, g,
+                         tg, u, useg);
+    return hipGetLastError();
+  }
+  if (tg.bits == 3)
+    hipLaunchKernelGGL(k_adj_tile_scatter<3>, dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g,
+                       tg, wp, u, useg, dg);
+  else
+    hipLaunchKernelGGL(k_adj_tile_scatter<4>, dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g,
+                       tg, wp, u, useg, dg);
   return hipGetLastError();
 }
 

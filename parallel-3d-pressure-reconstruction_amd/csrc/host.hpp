@@ -172,6 +172,8 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_y, adj_z, adj_u, adj_t, adj_g;
   // source-restricted backward: t_S, h (2), w (2), C_S over the nS sources
   omni3d::DevBuf adj_s;
+  // per-segment sums of the tile engine's transposed traversals
+  omni3d::DevBuf adj_seg;
   // masked methods on the tile engine: per value slot the first-visit record
   // (E, S, R, F, curl weights) of the field and the carries of an iteration
   omni3d::DevBuf mk_e, mk_s, mk_r, mk_f, mk_wc, mk_wn, mk_c;

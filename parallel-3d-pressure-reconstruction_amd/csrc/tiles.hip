@@ -124,6 +124,7 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
       nvis = 1;
       ncode = 0;
       start(w.c0, w.c1, w.c2);
+      sr.info |= (uint32_t)a << 23;  // the axis of the step into the tile
     } else {
       nvis++;
     }

@@ -63,7 +63,8 @@ struct SegRec {
   uint32_t src;      // source id of the ray (index of p_src)
   uint32_t info;     // local voxel of the first visit (12 bits) | visits (6) << 12 |
                      // step signs x,y,z (3) << 18 | first segment of the ray (1) << 21 |
-                     // last segment of the ray (1) << 22
+                     // last segment of the ray (1) << 22 | axis of the step into
+                     // the tile (2, not for the first segment) << 23
   uint32_t code[3];  // 2-bit axis codes: the steps between its visits, then the step
                      // out of the tile (not for the last segment), low bits first
 };
@@ -142,5 +143,17 @@ hipError_t launch_tile_masked(int method, int ntiles, const unsigned long long* 
                               const double* wc, const double* wn, const double* cval,
                               const double* p0, const double* p1, int k, const DevScalars* sc,
                               long long* sumfx, double* sumd, hipStream_t st);
+
+
+// adjoint.hip: the plain method's transposed traversals on the tile engine
+// (what 0: per-segment sums of u over the slot-ordered lists; 1: per-ray
+// suffix sums in place, and each ray's total into tvol at its entry voxel
+// when tvol is given; 2: the scatter of h * S into dg, every voxel stored)
+hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile_ptr1,
+                            const SegRec* segs_slot, const unsigned long long* tile_ptr,
+                            const SegRec* segs, const RayRec* rays, int64_t nrays,
+                            const unsigned int* nseg, const unsigned long long* val_off,
+                            GridDesc g, TileGrid tg, const WalkParams& wp, const double* u,
+                            double* useg, double* tvol, double* dg, hipStream_t st);
 
 }  // namespace omni3d
