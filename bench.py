@@ -461,6 +461,15 @@ def main():
                 k: {"kernel": kern[k][0], "ms": round(kern[k][1], 4),
                     "frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "traffic": traffic.get(k)} for k in kern if k != dom}}
+    # the source iteration (half of a reconstruction's time): per launch it
+    # streams one word and gathers one FP32 increment per source visit
+    it_ms = float(np.mean([x.ms_iterate for x in reps])) / max(1, reps[-1].iterations)
+    it_bytes = reps[-1].source_visits * 8
+    roof["iteration"] = {"kernel": "k_iterate_grp<4, increments> (one source iteration)",
+                         "ms_per_launch": round(it_ms, 5), "algorithmic_bytes": it_bytes,
+                         "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
+                         "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": traffic.get("iteration")}
 
     cpu = None
     if not args.no_cpu:
