@@ -61,6 +61,43 @@ int build_transposed_rows(omni3d_ctx* ctx) {
                                               key.as<unsigned long long>(), st));
   OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.tptr.as<unsigned long long>(),
                                      key.as<unsigned long long>(), st));
+  {
+    // merged words in groups of 4 chunks, as the forward iteration's rows
+    // (sources below 2^22, rows shorter than 4096 entries: int64 row sums)
+    std::vector<unsigned long long> hp((size_t)nS + 1);
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(hp.data(), pl.tptr.as<void>(),
+                                     sizeof(unsigned long long) * (nS + 1), hipMemcpyDeviceToHost,
+                                     st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    unsigned long long mx = 0;
+    for (int r = 0; r < nS; r++) mx = std::max(mx, hp[r + 1] - hp[r]);
+    pl.tdense = false;
+    if (nS < (1 << 22) && mx < 4096 && n > 0 && n < (1ull << 31)) {
+      DevBuf ka, kb, tmp, flag, excl;
+      OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * n));
+      OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * n));
+      OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)n)));
+      OMNI_TRY_HIP(ctx, pl.tdent.alloc(sizeof(unsigned int) * (n + 8)));
+      OMNI_TRY_HIP(ctx, launch_fill_u32(pl.tdent.as<void>(), (int64_t)n + 8, 0u, st));
+      OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)n, 4, pl.tptr.as<unsigned long long>(),
+                                            key.as<unsigned long long>(),
+                                            ka.as<unsigned long long>(), kb.as<unsigned long long>(),
+                                            tmp.as<void>(), tmp.size(), pl.tdent.as<unsigned int>(),
+                                            st));
+      OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * n));
+      OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * n));
+      const int ng = (nS + 255) / 256;
+      OMNI_TRY_HIP(ctx, pl.twptr.alloc(sizeof(unsigned int) * (ng + 1)));
+      unsigned int nw = 0;
+      OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)n, 4, pl.tptr.as<unsigned long long>(),
+                                            kb.as<unsigned long long>(), flag.as<unsigned int>(),
+                                            excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
+                                            pl.tdent.as<unsigned int>(),
+                                            pl.twptr.as<unsigned int>(), &nw, st));
+      OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+      pl.tdense = true;
+    }
+  }
   OMNI_TRY_HIP(ctx, pl.tent.alloc(sizeof(unsigned int) * (n + 1)));
   OMNI_TRY_HIP(ctx, launch_adj_narrow(key.as<unsigned long long>(), (int64_t)n,
                                       pl.tent.as<unsigned int>(), st));
@@ -262,22 +299,47 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
       }
     }
     if (P.max_iterations > 1 && nS > 0) {
-      OMNI_TRY_HIP(ctx, ctx->adj_s.alloc(sizeof(double) * 6 * (size_t)nS));
+      OMNI_TRY_HIP(ctx, ctx->adj_s.alloc(sizeof(double) * 7 * (size_t)nS));
       double* ts = ctx->adj_s.as<double>();
       double* h[2] = {ts + nS, ts + 2 * (size_t)nS};
       double* w[2] = {ts + 3 * (size_t)nS, ts + 4 * (size_t)nS};
       double* cs = ts + 5 * (size_t)nS;
+      double* w3 = ts + 6 * (size_t)nS;
       OMNI_TRY_HIP(ctx, launch_adj_scale(V, y, cnt, u, st));
       OMNI_TRY_HIP(ctx, launch_fill_f64(t, V, 0.0, st));
       OMNI_TRY_HIP(ctx, gather(u, t));
       OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, slist, t, cnt, ts, h[0], w[0], cs, st));
-      int cur = 0;
-      for (int m = 2; m < P.max_iterations; m++) {
-        OMNI_TRY_HIP(ctx, launch_adj_src_iter(nS, tptr, tent, diagp, cs, ts, w[cur], h[cur ^ 1],
-                                              w[cur ^ 1], st));
-        cur ^= 1;
+      const int M = P.max_iterations - 2;
+      if (!mask_rule && pl.tdense && M > 0) {
+        // w <- (t + diag w + sum w) / C is the forward iteration's form: the
+        // grouped kernel on the transposed merged words (exact fixed-point
+        // row sums), t as its source term, w over three buffers
+        const unsigned int gg = grouped_grid(nS);
+        OMNI_TRY_HIP(ctx, ctx->adj_sc.alloc(sizeof(DevScalars)));
+        OMNI_TRY_HIP(ctx, ctx->adj_part.alloc(sizeof(double) * 4 * (size_t)gg));
+        OMNI_TRY_HIP(ctx, ctx->adj_bs.alloc(sizeof(long long) * (size_t)nS));
+        DevScalars* asc = ctx->adj_sc.as<DevScalars>();
+        OMNI_TRY_HIP(ctx, launch_fill_u32(asc, sizeof(DevScalars) / 4, 0u, st));
+        OMNI_TRY_HIP(ctx, launch_absmax<double>(ts, nS, &asc->pmax_bits, st));
+        OMNI_TRY_HIP(ctx, launch_adj_fixed(nS, ts, asc, ctx->adj_bs.as<long long>(), st));
+        OMNI_TRY_HIP(ctx, launch_absmax<double>(w[0], nS, &asc->itmax_bits[0], st));
+        double* pb3[3] = {w[0], w[1], w3};
+        for (int k = 1; k <= M; k++)
+          OMNI_TRY_HIP(ctx, launch_iterate_dense(4, 0, k, nS, nullptr, pl.twptr.as<unsigned int>(),
+                                                 pl.tdent.as<unsigned int>(), diagp, slist, cnt,
+                                                 ctx->adj_bs.as<long long>(), nullptr, 0,
+                                                 pb3[(k - 1) % 3], pb3[k % 3], asc,
+                                                 ctx->adj_part.as<double>(), M, 0.0, st));
+        OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, pb3[M % 3], cs, z, st));
+      } else {
+        int cur = 0;
+        for (int m = 2; m < P.max_iterations; m++) {
+          OMNI_TRY_HIP(ctx, launch_adj_src_iter(nS, tptr, tent, diagp, cs, ts, w[cur], h[cur ^ 1],
+                                                w[cur ^ 1], st));
+          cur ^= 1;
+        }
+        OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, h[cur], nullptr, z, st));
       }
-      OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, h[cur], z, st));
     }
   } else {
     for (int m = 1; m < P.max_iterations; m++) {

@@ -17,6 +17,8 @@
 // twice from a saved walker state (no per-ray storage).
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "tiles.hpp"
@@ -279,13 +281,29 @@ k_adj_src_iter(int nS, const unsigned long long* __restrict__ tptr,
   }
 }
 
-// z = y + E^T h (source voxels are distinct, so plain stores).
+// z = y + E^T h (source voxels are distinct, so plain stores); with cs, the
+// vector given is w = h / C and h = w C.
 __global__ void __launch_bounds__(256)
 k_adj_src_out(int nS, const int* __restrict__ slist, const double* __restrict__ h,
-              double* __restrict__ z) {
+              const double* __restrict__ cs, double* __restrict__ z) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nS) return;
-  z[slist[s]] += h[s];
+  z[slist[s]] += cs ? h[s] * cs[s] : h[s];
+}
+
+// t_S -> the fixed-point source term of the grouped iteration: bs = round(t s)
+// with s = 2^(51 - e), |t| < 2^e from max |t| (in pmax_bits), inv_scale = 1 / s.
+__global__ void __launch_bounds__(256)
+k_adj_fixed(int nS, const double* __restrict__ t, DevScalars* __restrict__ sc,
+            long long* __restrict__ bs) {
+  const double B = __longlong_as_double((long long)sc->pmax_bits);
+  int ex = 0;
+  (void)frexp(B > 0.0 && B <= DBL_MAX ? B : 1.0, &ex);
+  ex = ex < -960 ? -960 : ex;
+  const double scale = ldexp(1.0, 51 - ex);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) sc->inv_scale = ldexp(1.0, ex - 51);
+  if (s < nS) bs[s] = to_fixed(t[s], scale);
 }
 
 // The plain-mask rule's version: its runs start at valid voxels after a
@@ -414,9 +432,15 @@ hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const uns
   return hipGetLastError();
 }
 
-hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, double* z,
-                              hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_src_out, dim3((nS + 255) / 256), dim3(256), 0, st, nS, slist, h, z);
+hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, const double* cs,
+                              double* z, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_src_out, dim3((nS + 255) / 256), dim3(256), 0, st, nS, slist, h, cs, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_fixed(int nS, const double* t, DevScalars* sc, long long* bs,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_fixed, dim3((nS + 255) / 256), dim3(256), 0, st, nS, t, sc, bs);
   return hipGetLastError();
 }
 

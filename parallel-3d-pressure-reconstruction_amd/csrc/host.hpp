@@ -121,6 +121,11 @@ struct Plan {
   bool trows = false;
   DevBuf tptr;  // uint64[nS+1]
   DevBuf tent;  // uint32[nnz+1]
+  // the same rows as merged words in groups of 4 chunks (the layout of the
+  // forward iteration), when sizes allow
+  bool tdense = false;
+  DevBuf tdent;   // uint32[nnz+8]
+  DevBuf twptr;   // uint32[ngroups+1]
 };
 
 }  // namespace omni3d
@@ -174,6 +179,9 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_s;
   // per-segment sums of the tile engine's transposed traversals
   omni3d::DevBuf adj_seg;
+  // the transposed iteration on merged words: its scalars, partial sums and
+  // fixed-point source term
+  omni3d::DevBuf adj_sc, adj_part, adj_bs;
   // masked methods on the tile engine: per value slot the first-visit record
   // (E, S, R, F, curl weights) of the field and the carries of an iteration
   omni3d::DevBuf mk_e, mk_s, mk_r, mk_f, mk_wc, mk_wn, mk_c;

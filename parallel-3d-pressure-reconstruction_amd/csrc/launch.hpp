@@ -109,8 +109,10 @@ hipError_t launch_adj_src_init(int nS, const int* slist, const double* t, const 
 hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const unsigned int* tent,
                                const unsigned int* diag, const double* cs, const double* ts,
                                const double* w, double* hn, double* wn, hipStream_t st);
-hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, double* z,
-                              hipStream_t st);
+hipError_t launch_adj_src_out(int nS, const int* slist, const double* h, const double* cs,
+                              double* z, hipStream_t st);
+hipError_t launch_adj_fixed(int nS, const double* t, DevScalars* sc, long long* bs,
+                            hipStream_t st);
 
 // solve.hip ----------------------------------------------------------------
 template <typename T>
