@@ -29,7 +29,14 @@ k_absmax(const T* __restrict__ x, int64_t n, unsigned long long* __restrict__ bi
     const unsigned long long o = __shfl_xor(b, off);
     b = o > b ? o : b;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(bits, b);
+  // one atomic per workgroup (atomics on one address serialise)
+  __shared__ unsigned long long s_b[4];
+  if ((threadIdx.x & 63) == 0) s_b[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) b = s_b[w] > b ? s_b[w] : b;
+    atomicMax(bits, b);
+  }
 }
 
 
@@ -945,7 +952,7 @@ k_max_len(const RayRec* __restrict__ rays, int64_t n, unsigned int* __restrict__
 
 template <typename T>
 hipError_t launch_absmax(const T* x, int64_t n, unsigned long long* bits, hipStream_t st) {
-  hipLaunchKernelGGL((k_absmax<T>), dim3(grid_for(n, 4096)), dim3(256), 0, st, x, n, bits);
+  hipLaunchKernelGGL((k_absmax<T>), dim3(grid_for(n, 1024)), dim3(256), 0, st, x, n, bits);
   return hipGetLastError();
 }
 template hipError_t launch_absmax<float>(const float*, int64_t, unsigned long long*, hipStream_t);
