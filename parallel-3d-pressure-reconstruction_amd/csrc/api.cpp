@@ -374,16 +374,15 @@ int build_plan(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, hipMemsetAsync(flag.as<void>(), 0, V, st));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(steps.as<void>(), 0, 16, st));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(mx.as<void>(), 0, 16, st));
-  OMNI_TRY_HIP(ctx, launch_plan_counts(pl.rays.as<RayRec>(), (int64_t)nrays, pl.dirs.as<DirConst>(),
-                                       g, pl.cnt.as<unsigned int>(), flag.as<uint8_t>(),
-                                       steps.as<unsigned long long>(), st));
-  OMNI_TRY_HIP(ctx, launch_max_u32(pl.cnt.as<unsigned int>(), V, mx.as<unsigned int>(), st));
+  // entry voxels and step totals from the ray records; the visit counts
+  // follow from the tile segments below
+  OMNI_TRY_HIP(ctx, launch_plan_flags(pl.rays.as<RayRec>(), (int64_t)nrays, g, flag.as<uint8_t>(),
+                                      steps.as<unsigned long long>(), st));
   OMNI_TRY_HIP(ctx, launch_max_len(pl.rays.as<RayRec>(), (int64_t)nrays, mx.as<unsigned int>() + 1, st));
   unsigned int mxh[2] = {0, 0};
   unsigned long long nsteps = 0;
   std::vector<uint8_t> hflag(V);
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(hflag.data(), flag.as<void>(), V, hipMemcpyDeviceToHost, st));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mxh, mx.as<unsigned int>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsteps, steps.as<void>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   // source numbering: 8x8 blocks of each box face, so that consecutive
@@ -397,6 +396,17 @@ int build_plan(omni3d_ctx* ctx) {
                                    hipMemcpyHostToDevice, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.slist.as<void>(), hslist.data(), sizeof(int) * nS,
                                    hipMemcpyHostToDevice, st));
+  // the tile engine's segments, then every voxel's visit count from them
+  {
+    const int rc = build_tiles(ctx, g, (int64_t)nrays);
+    if (rc) return rc;
+  }
+  OMNI_TRY_HIP(ctx, launch_tile_counts(pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
+                                       pl.segs_slot.as<SegRec>(), g, pl.tg,
+                                       pl.cnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, launch_max_u32(pl.cnt.as<unsigned int>(), V, mx.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mxh, mx.as<unsigned int>(), 8, hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   // CSR of visits landing on source voxels
   DevBuf rowcnt;
   OMNI_TRY_HIP(ctx, rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
@@ -516,8 +526,6 @@ int build_plan(omni3d_ctx* ctx) {
     }
   }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  int rc = build_tiles(ctx, g, (int64_t)nrays);
-  if (rc) return rc;
   pl.X = X;
   pl.Y = Y;
   pl.Z = Z;

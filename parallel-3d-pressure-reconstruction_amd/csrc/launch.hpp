@@ -24,9 +24,8 @@ hipError_t launch_enum_count(const DirConst* dirs, GridDesc g, int64_t npts, int
 hipError_t launch_enum_write(const DirConst* dirs, GridDesc g, int64_t npts, int64_t ncand,
                              const unsigned long long* block_offsets, RayRec* rays,
                              hipStream_t st);
-hipError_t launch_plan_counts(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                              GridDesc g, unsigned int* cnt, uint8_t* src,
-                              unsigned long long* steps, hipStream_t st);
+hipError_t launch_plan_flags(const RayRec* rays, int64_t nrays, GridDesc g, uint8_t* src,
+                             unsigned long long* steps, hipStream_t st);
 hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                 GridDesc g, const int* sid, unsigned int* rowcnt,
                                 unsigned int* diag, hipStream_t st);

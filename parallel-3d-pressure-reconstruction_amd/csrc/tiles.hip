@@ -371,6 +371,56 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
   }
 }
 
+// Visit counts of every voxel (plan build): a workgroup per tile counts its
+// segments' visits in LDS and stores the tile's counts (the entry visit of a
+// ray is its first segment's first visit).
+template <int kBits>
+__global__ void __launch_bounds__(1024)
+k_tile_counts(const unsigned long long* __restrict__ tile_ptr1, const SegRec* __restrict__ segs,
+              GridDesc g, TileGrid tg, unsigned int* __restrict__ cnt) {
+  constexpr int kTile = 1 << kBits, kTileMask = kTile - 1, kVox = kTile * kTile * kTile;
+  __shared__ unsigned int C[kVox];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) C[l] = 0u;
+  __syncthreads();
+  const unsigned long long b = tile_ptr1[tile], e = tile_ptr1[tile + 1];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const int d0 = (sg & 1u) ? 1 : -1, d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    atomicAdd(&C[lv], 1u);
+    for (unsigned int q = 1; q < n; q++) {
+      const unsigned int c = q - 1;
+      const uint32_t w = c < 16 ? sr.code[0] : (c < 32 ? sr.code[1] : sr.code[2]);
+      const int a = (int)((w >> (2 * (c & 15))) & 3u);
+      lv += a == 0 ? d0 : (a == 1 ? d1 : d2);
+      atomicAdd(&C[lv], 1u);
+    }
+  }
+  __syncthreads();
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    if (i < g.X && j < g.Y && k < g.Z) cnt[i + j * sy + k * sz] = C[l];
+  }
+}
+
+hipError_t launch_tile_counts(int ntiles, const unsigned long long* tile_ptr1, const SegRec* segs,
+                              GridDesc g, TileGrid tg, unsigned int* cnt, hipStream_t st) {
+  if (tg.bits == 3)
+    hipLaunchKernelGGL(k_tile_counts<3>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs, g, tg,
+                       cnt);
+  else
+    hipLaunchKernelGGL(k_tile_counts<4>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs, g, tg,
+                       cnt);
+  return hipGetLastError();
+}
+
 // Compact list of the segments of the source-sum pass (plan build).  One
 // workgroup per surface tile replays the step codes of the tile's segments
 // and keeps those whose visits reach a source voxel after the ray's entry

@@ -156,4 +156,9 @@ hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile
                             GridDesc g, TileGrid tg, const WalkParams& wp, const double* u,
                             double* useg, double* tvol, double* dg, hipStream_t st);
 
+
+// visit counts of every voxel from the tile segments (plan build)
+hipError_t launch_tile_counts(int ntiles, const unsigned long long* tile_ptr1, const SegRec* segs,
+                              GridDesc g, TileGrid tg, unsigned int* cnt, hipStream_t st);
+
 }  // namespace omni3d

@@ -120,32 +120,39 @@ k_enum_write(const DirConst* __restrict__ dirs, GridDesc g, int64_t npts, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Geometry-only walks of the plan: visit counts, source voxels, CSR of the
-// visits that land on source voxels.
+// Plan kernels over the rays: source voxels and step totals, the CSR of the
+// visits that land on source voxels (geometry-only walks).
 
-struct CountVisitor {
-  unsigned int* cnt;
-  uint8_t* src;
-  __device__ __forceinline__ void entry(int64_t v) {
-    atomicAdd(&cnt[v], 1u);
-    src[v] = 1;
-  }
-  __device__ __forceinline__ void visit(int64_t v, bool /*surface*/) { atomicAdd(&cnt[v], 1u); }
-};
-
+// Entry voxels (the plain method's sources) and the total staircase steps
+// (the L1 length of every ray) straight from the ray records; the visit
+// counts come from the tile engine (k_tile_counts).
 __global__ void __launch_bounds__(256)
-k_plan_counts(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-              GridDesc g, unsigned int* __restrict__ cnt, uint8_t* __restrict__ src,
-              unsigned long long* __restrict__ steps_total) {
+k_plan_flags(const RayRec* __restrict__ rays, int64_t nrays, GridDesc g, uint8_t* __restrict__ src,
+             unsigned long long* __restrict__ steps_total) {
   unsigned long long steps = 0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
        r += (int64_t)gridDim.x * blockDim.x) {
-    CountVisitor vis{cnt, src};
-    steps += (unsigned long long)walk_geometry(rays[r], dirs, g, vis);
+    const RayRec rr = rays[r];
+    int c[3];
+    unpack_voxel(rr.in, c);
+    src[(int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y] = 1;
+    steps += rr.dir_len >> 16;
   }
-  // wave-level sum then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off);
-  if ((threadIdx.x & 63) == 0) atomicAdd(steps_total, steps);
+  __shared__ unsigned long long s_st[4];
+  if ((threadIdx.x & 63) == 0) s_st[threadIdx.x >> 6] = steps;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) steps += s_st[w];
+    atomicAdd(steps_total, steps);
+  }
+}
+
+hipError_t launch_plan_flags(const RayRec* rays, int64_t nrays, GridDesc g, uint8_t* src,
+                             unsigned long long* steps, hipStream_t st) {
+  hipLaunchKernelGGL(k_plan_flags, dim3(grid_for(nrays, 2048)), dim3(256), 0, st, rays, nrays, g,
+                     src, steps);
+  return hipGetLastError();
 }
 
 // The entry visit of a plain ray reads its own source voxel: it is counted in
@@ -260,14 +267,6 @@ hipError_t launch_enum_write(const DirConst* dirs, GridDesc g, int64_t npts, int
                              hipStream_t st) {
   hipLaunchKernelGGL(k_enum_write, dim3(enum_blocks(ncand)), dim3(kEnumBlock), 0, st, dirs, g,
                      npts, ncand, block_offsets, rays);
-  return hipGetLastError();
-}
-
-hipError_t launch_plan_counts(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                              GridDesc g, unsigned int* cnt, uint8_t* src,
-                              unsigned long long* steps, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan_counts, dim3(grid_for(nrays)), dim3(256), 0, st, rays, nrays, dirs,
-                     g, cnt, src, steps);
   return hipGetLastError();
 }
 
