@@ -17,6 +17,8 @@
  *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select
  *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner
  *   kernel.cu:8352       calCurlofMaterialAccCPU     -> omni3d_curl
+ *   kernel.cu:8380       thredholdHistMaterialAccCPU -> thredholdHistMaterialAccCPU
+ *                        (same signature) / omni3d_threshold_hist
  *   sor3D.h:5            sor3D                       -> sor3D (same signature)
  *   calRHS.h:4           calRHS                      -> calRHS (same signature; the
  *                        result is written to the caller's RHS buffer, which
@@ -179,6 +181,21 @@ int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const fl
 int omni3d_curl(int X, int Y, int Z, float deltx, float delty, float deltz, const float* DuDt,
                 const float* DvDt, const float* DwDt, const float* mask, float* curl,
                 void* hip_stream);
+
+/* Select threshold from the curl histogram (thredholdHistMaterialAccCPU,
+ * kernel.cu:8380-8418): the value above which `percentage` of the voxels
+ * lie, on 10000 FP32 bins between min(curl, 1e4) and max(curl, -1e4), walked
+ * from the top bin with a FP32 running count; the reference's arithmetic,
+ * except that its bin walk (j++ from N-1, kernel.cu:8406) and the maximum's
+ * bin (N) leave the table: here the walk goes down and the maximum falls in
+ * the last bin.  The first form keeps the reference's signature (host
+ * buffers, void: failures only through omni3d_last_global_error()); the
+ * second takes curl and threshold (one float) as host or device buffers
+ * (mem_kind) and returns once *threshold is written. */
+void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, float percentage,
+                                 float* threshold);
+int omni3d_threshold_hist(int X, int Y, int Z, const float* curl, float percentage,
+                          float* threshold, int mem_kind, void* hip_stream);
 
 /* Adjoint pair of the linear operator g -> p (plain or plain-mask method,
  * zero initial pressure, the context's iteration count, no seed term).

@@ -719,3 +719,36 @@ void orc_sor_redblack(int X, int Y, int Z, int iters, double omega, double* p, c
             p[v] = p[v] + omega * (s - p[v]);
           }
 }
+
+/* Threshold from the curl histogram (thredholdHistMaterialAccCPU,
+ * kernel.cu:8380-8418): min / max of the values (seeded with 1e4 / -1e4 as
+ * there), 10000 bins over [min, max] in FP32, then from the top bin down the
+ * first bin where the running count (FP32) reaches count * percentage gives
+ * bin / N * (max - min) + min; the maximum itself when none does.  The
+ * reference walks the bins with j++ (kernel.cu:8406) and puts the maximum in
+ * bin N, both out of bounds; this restates the intended walk (j--) with the
+ * maximum in the last bin. */
+float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentage) {
+  const int64_t V = (int64_t)X * Y * Z;
+  float minv = 1e4f, maxv = -1e4f;
+  for (int64_t i = 0; i < V; i++) {
+    if (minv > curl[i]) minv = curl[i];
+    if (maxv < curl[i]) maxv = curl[i];
+  }
+  enum { N = 10000 };
+  static int hist[N];
+  memset(hist, 0, sizeof(hist));
+  for (int64_t i = 0; i < V; i++) {
+    const float q = (curl[i] - minv) / (maxv - minv) * N;
+    /* (int)q as kernel.cu:8401; the maximum (q == N) and NaN quotients
+     * (max == min, NaN values) are kept inside the table */
+    hist[q >= N - 1 ? N - 1 : (q > 0 ? (int)q : 0)]++;
+  }
+  float totnum = 0;
+  const float need = (float)(int)V * percentage;
+  for (int j = N - 1; j >= 0; j--) {
+    totnum += hist[j];
+    if (totnum >= need) return (float)j / N * (maxv - minv) + minv;
+  }
+  return maxv;
+}

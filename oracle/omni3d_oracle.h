@@ -102,6 +102,9 @@ void orc_sor_redblack(int X, int Y, int Z, int iters, double omega, double* p, c
 /* Interior residual of the 7-point operator, sum |sum_nb - 6p + rhs| / V (FP64). */
 double orc_residual(int X, int Y, int Z, const double* p, const double* rhs);
 
+/* kernel.cu:8380-8418 (intended bin walk, see the definition) */
+float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,8 @@ def _declare(L):
     L.orc_sor3d.argtypes = [i, i, i, i, _ip, _dp, _dp, _dp, d, _dp]
     L.orc_sor3d.restype = i
     L.orc_sor_redblack.argtypes = [i, i, i, i, d, _dp, _dp]
+    L.orc_threshold_hist.argtypes = [i, i, i, _fp, C.c_float]
+    L.orc_threshold_hist.restype = C.c_float
     L.orc_sor_redblack.restype = None
     L.orc_residual.argtypes = [i, i, i, _dp, _dp]
     L.orc_residual.restype = d
@@ -225,6 +227,13 @@ def sor_redblack(p, rhs, iters, omega):
     rhs = _f64(rhs)
     lib().orc_sor_redblack(X, Y, Z, iters, omega, _ptr(p, _dp), _ptr(rhs, _dp))
     return p
+
+
+def threshold_hist(curl, percentage):
+    """kernel.cu:8380-8418 with the intended bin walk; FP32 in and out."""
+    Z, Y, X = curl.shape
+    c = np.ascontiguousarray(curl, dtype=np.float32)
+    return float(lib().orc_threshold_hist(X, Y, Z, _ptr(c, _fp), percentage))
 
 
 def residual(p, rhs):

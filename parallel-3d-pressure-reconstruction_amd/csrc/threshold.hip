@@ -1,0 +1,183 @@
+// Curl threshold from a histogram: thredholdHistMaterialAccCPU
+// (kernel.cu:8380-8418), the helper that picks the Select method's curl
+// threshold as the value above which a given fraction of the voxels lie.
+//
+// Same arithmetic as the reference, in FP32: min / max seeded with 1e4 and
+// -1e4; bin = (int)((c - min) / (max - min) * 10000); from the top bin down,
+// a FP32 running count; the first bin j where it reaches (X*Y*Z) * percentage
+// gives j / 10000 * (max - min) + min, else max.  Two differences, both where
+// the reference leaves its arrays: the reference's bin walk counts j upwards
+// from N-1 (kernel.cu:8406) and the maximum lands in bin N; here the walk
+// goes down and the maximum goes into the last bin (NaN quotients, from
+// max == min or NaN values, into bin 0).  min / max treat -0 as +0.
+//
+// Kernels: k_thr_minmax (one pass, one atomic pair per workgroup on an
+// order-preserving integer image of the floats), k_thr_hist (per-workgroup
+// LDS histogram of the 10000 bins, non-empty bins added to the global one),
+// k_thr_pick (one workgroup: the serial FP32 walk from the top bin, so the
+// running count rounds exactly as the reference's).  HBM traffic: the volume
+// read twice (8 B per voxel) plus 40 KB of bins.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/omni3d.h"
+#include "driver.hpp"
+#include "launch.hpp"
+
+namespace omni3d {
+namespace {
+
+constexpr int kBins = 10000;
+constexpr int kThrBlock = 1024;
+
+// order-preserving map float -> uint32 (-0 folded onto +0)
+__device__ __forceinline__ unsigned int ord_of(float x) {
+  unsigned int u = __float_as_uint(x + 0.0f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float float_of(unsigned int o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+__global__ void k_thr_init(unsigned int* mm, unsigned int* hist) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) {
+    mm[0] = ord_of(1e4f);
+    mm[1] = ord_of(-1e4f);
+  }
+  for (int b = t; b < kBins; b += gridDim.x * blockDim.x) hist[b] = 0u;
+}
+
+__global__ void __launch_bounds__(kThrBlock) k_thr_minmax(const float* __restrict__ c, int64_t V,
+                                                         unsigned int* mm) {
+  float lo = 1e4f, hi = -1e4f;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride) {
+    const float x = c[v];
+    if (x < lo) lo = x;  // NaN compares false and is skipped, as in the reference
+    if (x > hi) hi = x;
+  }
+  unsigned int olo = ord_of(lo), ohi = ord_of(hi);
+  for (int off = 32; off > 0; off >>= 1) {
+    olo = min(olo, (unsigned int)__shfl_xor((int)olo, off));
+    ohi = max(ohi, (unsigned int)__shfl_xor((int)ohi, off));
+  }
+  __shared__ unsigned int slo[kThrBlock / 64], shi[kThrBlock / 64];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    slo[w] = olo;
+    shi[w] = ohi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+      olo = min(olo, slo[i]);
+      ohi = max(ohi, shi[i]);
+    }
+    atomicMin(&mm[0], olo);
+    atomicMax(&mm[1], ohi);
+  }
+}
+
+__device__ __forceinline__ int bin_of(float x, float lo, float range) {
+  const float q = __fmul_rn(__fdiv_rn(__fsub_rn(x, lo), range), (float)kBins);
+  return q >= (float)(kBins - 1) ? kBins - 1 : (q > 0.0f ? (int)q : 0);
+}
+
+__global__ void __launch_bounds__(kThrBlock) k_thr_hist(const float* __restrict__ c, int64_t V,
+                                                       const unsigned int* mm,
+                                                       unsigned int* hist) {
+  __shared__ unsigned int h[kBins];
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
+  const float lo = float_of(mm[0]), hi = float_of(mm[1]);
+  const float range = __fsub_rn(hi, lo);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V; v += stride)
+    atomicAdd(&h[bin_of(c[v], lo, range)], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+__global__ void k_thr_pick(const unsigned int* mm, const unsigned int* hist, float need,
+                           float* out) {
+  __shared__ unsigned int h[kBins];
+  for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = hist[b];
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const float lo = float_of(mm[0]), hi = float_of(mm[1]);
+  float tot = 0.0f, r = hi;
+  for (int j = kBins - 1; j >= 0; j--) {
+    tot = __fadd_rn(tot, (float)h[j]);
+    if (tot >= need) {
+      r = __fadd_rn(__fmul_rn(__fdiv_rn((float)j, (float)kBins), __fsub_rn(hi, lo)), lo);
+      break;
+    }
+  }
+  out[0] = r;
+}
+
+#define THR_TRY(expr)                                                           \
+  do {                                                                          \
+    hipError_t _e = (expr);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      return fail(nullptr, OMNI3D_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    }                                                                           \
+  } while (0)
+
+int threshold_impl(int X, int Y, int Z, const float* curl, float percentage, float* threshold,
+                   int mem_kind, hipStream_t st) {
+  if (X < 1 || Y < 1 || Z < 1 || (int64_t)X * Y * Z > INT32_MAX)
+    return fail(nullptr, OMNI3D_ERR_ARG, "bad grid dimensions");
+  if (!curl || !threshold) return fail(nullptr, OMNI3D_ERR_ARG, "null pointer");
+  if (mem_kind != OMNI3D_MEM_HOST && mem_kind != OMNI3D_MEM_DEVICE)
+    return fail(nullptr, OMNI3D_ERR_ARG, "mem_kind must be OMNI3D_MEM_HOST or OMNI3D_MEM_DEVICE");
+  const int64_t V = (int64_t)X * Y * Z;
+  // kernel.cu:8408: Imax*Jmax*Kmax*percentage, an int product times a float
+  const float need = (float)(int)V * percentage;
+  DevBuf scratch, dcurl;
+  THR_TRY(scratch.alloc(sizeof(unsigned int) * (2 + kBins) + sizeof(float)));
+  unsigned int* mm = scratch.as<unsigned int>();
+  unsigned int* hist = mm + 2;
+  float* dout = reinterpret_cast<float*>(hist + kBins);
+  const float* src = curl;
+  if (mem_kind == OMNI3D_MEM_HOST) {
+    THR_TRY(dcurl.alloc(sizeof(float) * V));
+    THR_TRY(hipMemcpyAsync(dcurl.as<void>(), curl, sizeof(float) * V, hipMemcpyHostToDevice, st));
+    src = dcurl.as<float>();
+  }
+  const unsigned int nb = grid_for((V + 15) / 16, 512);  // >= 16 voxels per lane
+  hipLaunchKernelGGL(k_thr_init, dim3(10), dim3(1024), 0, st, mm, hist);
+  hipLaunchKernelGGL(k_thr_minmax, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm);
+  hipLaunchKernelGGL(k_thr_hist, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm, hist);
+  hipLaunchKernelGGL(k_thr_pick, dim3(1), dim3(1024), 0, st, mm, hist, need,
+                     mem_kind == OMNI3D_MEM_HOST ? dout : threshold);
+  THR_TRY(hipGetLastError());
+  if (mem_kind == OMNI3D_MEM_HOST)
+    THR_TRY(hipMemcpyAsync(threshold, dout, sizeof(float), hipMemcpyDeviceToHost, st));
+  // the scratch bins are freed on return
+  THR_TRY(hipStreamSynchronize(st));
+  return OMNI3D_OK;
+}
+
+}  // namespace
+}  // namespace omni3d
+
+extern "C" {
+
+void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, float percentage,
+                                 float* threshold) {
+  (void)omni3d::threshold_impl(Imax, Jmax, Kmax, curl, percentage, threshold, OMNI3D_MEM_HOST,
+                               nullptr);
+}
+
+int omni3d_threshold_hist(int X, int Y, int Z, const float* curl, float percentage,
+                          float* threshold, int mem_kind, void* hip_stream) {
+  return omni3d::threshold_impl(X, Y, Z, curl, percentage, threshold, mem_kind,
+                                static_cast<hipStream_t>(hip_stream));
+}
+
+}  // extern "C"

@@ -33,6 +33,7 @@ EXPORTS = [
     "omni3d_batch_set_directions", "omni3d_batch_set_field", "omni3d_batch_tune",
     "omni3d_batch_run", "omni3d_batch_result", "omni3d_field_curl",
     "omni3d_read_parameter_file", "omni3d_run_parameter_file_ex",
+    "thredholdHistMaterialAccCPU", "omni3d_threshold_hist",
 ]
 
 
@@ -114,6 +115,10 @@ def _declare(L):
     L.omni3d_es_inner_select.argtypes = es + [vp, vp, vp, vp, f, vp]
     L.omni3d_devide_count_inner.argtypes = [i, i, i, vp, vp, vp, vp]
     L.omni3d_curl.argtypes = [i, i, i, f, f, f, vp, vp, vp, vp, vp, vp]
+    L.thredholdHistMaterialAccCPU.argtypes = [i, i, i, C.POINTER(C.c_float), f,
+                                              C.POINTER(C.c_float)]
+    L.thredholdHistMaterialAccCPU.restype = None
+    L.omni3d_threshold_hist.argtypes = [i, i, i, vp, f, vp, i, vp]
     L.omni3d_forward.argtypes = [vp, vp, vp]
     L.omni3d_backward.argtypes = [vp, vp, vp]
     L.omni3d_sor_solve.argtypes = [i, i, i, d, i, d, dp, dp, C.POINTER(C.c_int), dp]
@@ -212,6 +217,25 @@ def sor_solve_device(shape, rhs_ptr, p_ptr, omega=1.8, max_iter=1000, eps=0.0, s
     check(lib().omni3d_sor_solve_device(X, Y, Z, omega, max_iter, eps, rhs_ptr, p_ptr, stream,
                                         C.byref(it), C.byref(res)))
     return it.value, res.value
+
+
+def threshold_hist(curl, percentage, stream=None):
+    """Curl threshold above which `percentage` of the voxels lie (10000-bin
+    histogram, FP32).  curl: host numpy float32 (z-major) or a device torch
+    float32 tensor."""
+    Z, Y, X = curl.shape
+    out = C.c_float(0)
+    if hasattr(curl, "is_cuda") and curl.is_cuda:
+        import torch
+        c = curl.contiguous()
+        t = torch.empty(1, dtype=torch.float32, device=c.device)
+        check(lib().omni3d_threshold_hist(X, Y, Z, C.c_void_p(c.data_ptr()), percentage,
+                                          C.c_void_p(t.data_ptr()), DEVICE, stream))
+        return float(t.item())
+    c = np.ascontiguousarray(curl, dtype=np.float32)
+    check(lib().omni3d_threshold_hist(X, Y, Z, C.c_void_p(c.ctypes.data), percentage,
+                                      C.c_void_p(C.addressof(out)), HOST, stream))
+    return out.value
 
 
 def read_parameter_file(path):

@@ -9,7 +9,9 @@ C-ABI with the reference's argument lists:
     (kernel.cu:8352-8379);
   * calRHS, sor3D (same signatures as calRHS.h:4 and sor3D.h:5) and the
     red-black omni3d_sor_solve: bit-identical to the oracle's serial
-    evaluation (every voxel update has the same expression order).
+    evaluation (every voxel update has the same expression order);
+  * thredholdHistMaterialAccCPU (kernel.cu:8380-8418) and
+    omni3d_threshold_hist: the curl threshold bit-identical to the oracle's.
 """
 import ctypes as C
 
@@ -197,3 +199,41 @@ def test_sor_redblack_rows_bit_exact_device_buffers(gpu_available):
                                         max_iter=12)
     assert it2 == 12 and res2 == res
     np.testing.assert_array_equal(pd.cpu().numpy(), po)
+
+
+@pytest.mark.parametrize("shape,kind", [((1, 1, 1), "uniform"), ((19, 23, 37), "uniform"),
+                                        ((64, 64, 64), "curl"), ((128, 128, 128), "normal"),
+                                        ((40, 40, 40), "constant")])
+def test_threshold_hist_bit_exact(gpu_available, shape, kind):
+    """thredholdHistMaterialAccCPU (kernel.cu:8380-8418): min / max, the
+    10000-bin histogram and the FP32 top-down walk, bit-identical to the
+    oracle through both entry points (host signature of the reference and
+    the device-buffer form)."""
+    import torch
+    print(f"case threshold {kind} {shape}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    if kind == "uniform":
+        c = rng.random(shape).astype(np.float32)
+    elif kind == "normal":
+        c = np.abs(rng.normal(size=shape)).astype(np.float32) * np.float32(40.0)
+    elif kind == "constant":
+        c = np.full(shape, 3.5, np.float32)
+    else:
+        f = F.fourier_sphere(shape[0], seed=2)
+        a = [dev(x) for x in f["acc"]]
+        cd = torch.empty(shape, dtype=torch.float32, device="cuda")
+        Z, Y, X = shape
+        assert omni3d.lib().omni3d_curl(X, Y, Z, f["delta"][0], f["delta"][1], f["delta"][2],
+                                        a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(),
+                                        None, cd.data_ptr(), None) == 0
+        c = cd.cpu().numpy()
+    L = omni3d.lib()
+    for p in (0.0, 0.003, 0.05, 0.5, 0.97, 1.0, 2.0):
+        want = O.threshold_hist(c, p)
+        got = C.c_float(-1.0)
+        Z, Y, X = shape
+        L.thredholdHistMaterialAccCPU(X, Y, Z, np.ascontiguousarray(c).ctypes.data_as(
+            C.POINTER(C.c_float)), p, C.byref(got))
+        assert got.value == want, (p, got.value, want)
+        assert omni3d.threshold_hist(dev(c), p) == want
+        assert omni3d.threshold_hist(c, p) == want
