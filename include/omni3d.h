@@ -17,6 +17,7 @@
  *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select
  *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner
  *   kernel.cu:8352       calCurlofMaterialAccCPU     -> omni3d_curl
+ *   kernel.cu:6863       calCurlofMaterialAcc (GPU)  -> omni3d_curl, mask NULL
  *   kernel.cu:8380       thredholdHistMaterialAccCPU -> thredholdHistMaterialAccCPU
  *                        (same signature) / omni3d_threshold_hist
  *   sor3D.h:5            sor3D                       -> sor3D (same signature)
@@ -177,7 +178,9 @@ int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const fl
                               void* hip_stream);
 
 /* |curl a| * mask with the reference's one-sided wall differences
- * (kernel.cu:8352-8379).  Device pointers, float; mask may be NULL. */
+ * (kernel.cu:8352-8379).  Device pointers, float; mask may be NULL, which
+ * is the reference's GPU kernel calCurlofMaterialAcc (kernel.cu:6863-6888,
+ * the same differences without the mask). */
 int omni3d_curl(int X, int Y, int Z, float deltx, float delty, float deltz, const float* DuDt,
                 const float* DvDt, const float* DwDt, const float* mask, float* curl,
                 void* hip_stream);

@@ -6,7 +6,7 @@ C-ABI with the reference's argument lists:
     against one oracle pass on the same FP32 inputs, pn within FP32 rounding
     (rel <= 1e-6 per voxel), pcountinner exact for the plain method;
   * omni3d_devide_count_inner (kernel.cu:6746-6759), omni3d_curl
-    (kernel.cu:8352-8379);
+    (kernel.cu:8352-8379, and without a mask kernel.cu:6863-6888);
   * calRHS, sor3D (same signatures as calRHS.h:4 and sor3D.h:5) and the
     red-black omni3d_sor_solve: bit-identical to the oracle's serial
     evaluation (every voxel update has the same expression order);
@@ -119,6 +119,13 @@ def test_curl(gpu_available):
                                           md.data_ptr(), out.data_ptr(), None))
     torch.cuda.synchronize()
     want = O.curl(shape, tuple(float(np.float32(d)) for d in f["delta"]), acc, mask)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-6, atol=1e-7 * np.abs(want).max())
+    # no mask: the reference's GPU kernel calCurlofMaterialAcc (kernel.cu:6863-6888)
+    omni3d.check(omni3d.lib().omni3d_curl(X, Y, Z, f["delta"][0], f["delta"][1], f["delta"][2],
+                                          a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(),
+                                          None, out.data_ptr(), None))
+    torch.cuda.synchronize()
+    want = O.curl(shape, tuple(float(np.float32(d)) for d in f["delta"]), acc, np.ones(shape))
     np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-6, atol=1e-7 * np.abs(want).max())
 
 
