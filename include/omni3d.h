@@ -24,6 +24,9 @@
  *   calRHS.h:4           calRHS                      -> calRHS (same signature; the
  *                        result is written to the caller's RHS buffer, which
  *                        the reference loses, calRHS.cpp:10)
+ *   kernel.cu:3414,5572  EqualSpacing + BCiteration -> omni3d_boundary_pairs
+ *                        (the README's boundary-pair pipeline, commented out
+ *                        in main, kernel.cu:8830-8836)
  * Additions without a reference counterpart: the adjoint pair
  * (omni3d_forward / omni3d_backward), the GPU red-black SOR
  * (omni3d_sor_solve) and the snapshot batch (omni3d_batch_*).
@@ -199,6 +202,22 @@ void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, floa
                                  float* threshold);
 int omni3d_threshold_hist(int X, int Y, int Z, const float* curl, float percentage,
                           float* threshold, int mem_kind, void* hip_stream);
+
+/* Boundary-pair pipeline of the README (README:18-28; kernels
+ * omni3dparallellinesEqualSpacing kernel.cu:3414-3489 with bodyIntegral
+ * kernel.cu:1353-1467, devidecount kernel.cu:3896-3909, BCiteration
+ * kernel.cu:5572-5712), in sparse form: per ray the entry-to-exit integral,
+ * per (entry, exit) pair the mean over its rays, then `iterations` Jacobi
+ * sweeps over the exit voxels: p[out] = mean over the pairs with a non-zero
+ * value of p[in] + pint.  p (X*Y*Z, dtype, host or device by mem_kind): in =
+ * starting pressure, out = the same with the exit voxels updated (an exit
+ * voxel whose pairs are all zero keeps its value; the reference divides 0 by
+ * 0 there).  The README's inner pass follows as omni3d_reconstruct with the
+ * plain method and max_iterations = 1 from this p.  *npairs_out (may be
+ * NULL) gets the number of pairs.  Directions, field and the context's
+ * precision (ray sums) as for omni3d_reconstruct. */
+int omni3d_boundary_pairs(omni3d_ctx* ctx, int iterations, void* p, int dtype, int mem_kind,
+                          long long* npairs_out);
 
 /* Adjoint pair of the linear operator g -> p (plain or plain-mask method,
  * zero initial pressure, the context's iteration count, no seed term).

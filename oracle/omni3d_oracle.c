@@ -752,3 +752,113 @@ float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentag
   }
   return maxv;
 }
+
+/* ---- boundary-pair (PINT) pipeline in sparse per-ray form ----------------
+ * omni3dparallellinesEqualSpacing (kernel.cu:3414-3489): per accepted ray,
+ * bodyIntegral (kernel.cu:1353-1467, the plain staircase and trapezoid of
+ * bodyIntegralInner without the scatter) from the entry to the exit voxel,
+ * added into pint[in, out] with pcount[in, out]++; devidecount
+ * (kernel.cu:3896-3909) makes each pair's value the mean over its rays.
+ * BCiteration (kernel.cu:5572-5712): per exit node, the mean over the pairs
+ * with a non-zero value of p[in] + pint[in, out], then p[out] = that mean.
+ * Here the pairs are a sorted list instead of the n x n matrix, the
+ * iteration is a Jacobi sweep (every new value from the previous p; the
+ * reference has only __syncthreads() between its iterations, an inter-block
+ * race), and an exit node whose pairs are all zero keeps its p (the
+ * reference divides 0 by 0 there). */
+typedef struct {
+  int64_t out, in, ray;
+  double v;
+} bp_rec;
+
+static int bp_cmp(const void* a, const void* b) {
+  const bp_rec* x = (const bp_rec*)a;
+  const bp_rec* y = (const bp_rec*)b;
+  if (x->out != y->out) return x->out < y->out ? -1 : 1;
+  if (x->in != y->in) return x->in < y->in ? -1 : 1;
+  return x->ray < y->ray ? -1 : (x->ray > y->ray);
+}
+
+int64_t orc_boundary_pairs(int X, int Y, int Z, int D, const float* k1, const float* k2,
+                           const float* k3, double ls, double dx, double dy, double dz,
+                           double rho, const double* ax, const double* ay, const double* az,
+                           int iters, double* p) {
+  const int ng = orc_nogrid(X, Y, Z, ls);
+  const int64_t npts = (int64_t)ng * ng;
+  const double del[3] = {dx, dy, dz};
+  const double* acc[3] = {ax, ay, az};
+  int64_t cap = 1024, n = 0;
+  bp_rec* r = (bp_rec*)malloc(sizeof(bp_rec) * cap);
+  for (int ang = 0; ang < D; ang++) {
+    orc_dir d;
+    orc_make_dir(k1[ang], k2[ang], k3[ang], &d);
+    for (int64_t pt = 0; pt < npts; pt++) {
+      double o[3];
+      int in[3], out[3];
+      orc_origin(&d, ng, ls, pt, o);
+      if (!orc_clip(X, Y, Z, o, d.k, in, out)) continue;
+      walker w;
+      walker_init(&w, X, Y, Z, &d, o, in, out);
+      double pint = 0.0;
+      while (!walker_done(&w)) {
+        int s, a = walker_choose(&w, &s);
+        if (a < 0) break;
+        const int64_t vl = vidx(X, Y, w.c[0], w.c[1], w.c[2]);
+        walker_step(&w, a, s);
+        const int64_t vn = vidx(X, Y, w.c[0], w.c[1], w.c[2]);
+        pint += trap(rho, s, del[a], acc[a][vn], acc[a][vl]);
+      }
+      if (n == cap) {
+        cap *= 2;
+        r = (bp_rec*)realloc(r, sizeof(bp_rec) * cap);
+      }
+      r[n].out = vidx(X, Y, out[0], out[1], out[2]);
+      r[n].in = vidx(X, Y, in[0], in[1], in[2]);
+      r[n].ray = n;
+      r[n].v = pint;
+      n++;
+    }
+  }
+  qsort(r, (size_t)n, sizeof(bp_rec), bp_cmp);
+  /* pairs: (out, in, mean) in (out, in) order */
+  int64_t np = 0;
+  for (int64_t i = 0; i < n;) {
+    int64_t j = i;
+    double sum = 0.0;
+    while (j < n && r[j].out == r[i].out && r[j].in == r[i].in) sum += r[j++].v;
+    r[np].out = r[i].out;
+    r[np].in = r[i].in;
+    r[np].v = sum / (double)(j - i);
+    np++;
+    i = j;
+  }
+  const int64_t V = (int64_t)X * Y * Z;
+  double* pn = (double*)malloc(sizeof(double) * (size_t)(np > 0 ? np : 1));
+  for (int it = 0; it < iters; it++) {
+    int64_t nout = 0;
+    for (int64_t i = 0; i < np;) {
+      int64_t j = i;
+      double sum = 0.0;
+      int c = 0;
+      for (; j < np && r[j].out == r[i].out; j++) {
+        if (r[j].v != 0.0) {
+          sum += p[r[j].in] + r[j].v;
+          c++;
+        }
+      }
+      pn[nout++] = c ? sum / (double)c : p[r[i].out];
+      i = j;
+    }
+    nout = 0;
+    for (int64_t i = 0; i < np;) {
+      int64_t j = i;
+      while (j < np && r[j].out == r[i].out) j++;
+      p[r[i].out] = pn[nout++];
+      i = j;
+    }
+  }
+  (void)V;
+  free(pn);
+  free(r);
+  return np;
+}

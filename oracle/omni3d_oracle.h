@@ -102,6 +102,14 @@ void orc_sor_redblack(int X, int Y, int Z, int iters, double omega, double* p, c
 /* Interior residual of the 7-point operator, sum |sum_nb - 6p + rhs| / V (FP64). */
 double orc_residual(int X, int Y, int Z, const double* p, const double* rhs);
 
+/* Boundary-pair pipeline (kernel.cu:3414-3489, 1353-1467, 3896-3909,
+ * 5572-5712) on a sorted pair list, Jacobi boundary iterations: p (X*Y*Z)
+ * in/out, only exit voxels change.  Returns the number of (in, out) pairs. */
+int64_t orc_boundary_pairs(int X, int Y, int Z, int D, const float* k1, const float* k2,
+                           const float* k3, double ls, double dx, double dy, double dz,
+                           double rho, const double* ax, const double* ay, const double* az,
+                           int iters, double* p);
+
 /* kernel.cu:8380-8418 (intended bin walk, see the definition) */
 float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentage);
 

@@ -81,6 +81,9 @@ def _declare(L):
     L.orc_threshold_hist.argtypes = [i, i, i, _fp, C.c_float]
     L.orc_threshold_hist.restype = C.c_float
     L.orc_sor_redblack.restype = None
+    L.orc_boundary_pairs.argtypes = [i, i, i, i, _fp, _fp, _fp, d, d, d, d, d, _dp, _dp, _dp,
+                                     i, _dp]
+    L.orc_boundary_pairs.restype = i64
     L.orc_residual.argtypes = [i, i, i, _dp, _dp]
     L.orc_residual.restype = d
 
@@ -187,6 +190,19 @@ def reconstruct(shape, dirs, ls, delta, rho, acc, p0=None, curl=None, threshold=
                           _ptr(cu, _dp), threshold, method, faithful, max_iter, tol, seed_pn,
                           _ptr(p, _dp), _ptr(cnt, _dp), C.byref(rep))
     return p, cnt, rep.iterations, rep.rel_change
+
+
+def boundary_pairs(shape, dirs, ls, delta, rho, acc, p0, iters):
+    """Boundary-pair pipeline (kernel.cu:3414-3489, 3896-3909, 5572-5712),
+    sparse pairs, Jacobi boundary iterations; returns (p, number of pairs)."""
+    Z, Y, X = shape
+    k1, k2, k3, D = _dirs(dirs)
+    ax, ay, az = (_f64(a) for a in acc)
+    p = _f64(p0).copy()
+    n = lib().orc_boundary_pairs(X, Y, Z, D, _ptr(k1, _fp), _ptr(k2, _fp), _ptr(k3, _fp), ls,
+                                 delta[0], delta[1], delta[2], rho, _ptr(ax, _dp),
+                                 _ptr(ay, _dp), _ptr(az, _dp), iters, _ptr(p, _dp))
+    return p, int(n)
 
 
 def curl(shape, delta, acc, mask=None):

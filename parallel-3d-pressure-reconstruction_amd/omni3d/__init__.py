@@ -33,7 +33,7 @@ EXPORTS = [
     "omni3d_batch_set_directions", "omni3d_batch_set_field", "omni3d_batch_tune",
     "omni3d_batch_run", "omni3d_batch_result", "omni3d_field_curl",
     "omni3d_read_parameter_file", "omni3d_run_parameter_file_ex",
-    "thredholdHistMaterialAccCPU", "omni3d_threshold_hist",
+    "thredholdHistMaterialAccCPU", "omni3d_threshold_hist", "omni3d_boundary_pairs",
 ]
 
 
@@ -119,6 +119,7 @@ def _declare(L):
                                               C.POINTER(C.c_float)]
     L.thredholdHistMaterialAccCPU.restype = None
     L.omni3d_threshold_hist.argtypes = [i, i, i, vp, f, vp, i, vp]
+    L.omni3d_boundary_pairs.argtypes = [vp, i, vp, i, i, C.POINTER(C.c_longlong)]
     L.omni3d_forward.argtypes = [vp, vp, vp]
     L.omni3d_backward.argtypes = [vp, vp, vp]
     L.omni3d_sor_solve.argtypes = [i, i, i, d, i, d, dp, dp, C.POINTER(C.c_int), dp]
@@ -335,6 +336,16 @@ class Context:
                                            w.ctypes.data if w is not None else None, dt, HOST,
                                            C.byref(rep)))
         return p, w, rep
+
+    def boundary_pairs(self, p0, iterations, dtype=np.float64):
+        """Boundary-pair pipeline (README:18-28): host buffers; returns (p with
+        the exit voxels updated, number of pairs)."""
+        p = np.ascontiguousarray(p0, dtype=dtype).copy()
+        n = C.c_longlong(0)
+        self._chk(lib().omni3d_boundary_pairs(self._h, iterations, p.ctypes.data,
+                                              F64 if dtype == np.float64 else F32, HOST,
+                                              C.byref(n)))
+        return p, n.value
 
     def forward_device(self, grad_ptr, p_ptr):
         """p = F(g) for g = grad p (3 components, c*V + v), device buffers in
