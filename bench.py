@@ -31,6 +31,7 @@ GPU red-black SOR (omega 1.8) on the 256^3 Neumann right-hand side of the
 Burgers-vortex field, beside the oracle's serial red-black SOR.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -292,6 +293,72 @@ def sor_mode(args):
                    "residual": res, "iterations": it}}))
 
 
+def pairs_mode(args):
+    """Boundary-pair pipeline (README:18-28) at 256^3, 162 directions, device
+    buffers: per-ray entry-to-exit sums, pair list, 4 boundary iterations,
+    then the inner pass (one plain iteration from that boundary)."""
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    n = args.n
+    shape = (n, n, n)
+    V = n ** 3
+    f = F.burgers_vortex(n, noise=0.02, seed=0)
+    dirs = omni3d.generate_directions(args.divide)
+    out = {}
+    p_dev = torch.zeros(V, dtype=torch.float64, device="cuda")
+    for name, prec in (("fp32_kahan", omni3d.FP32_KAHAN), ("fp64", omni3d.FP64)):
+        with omni3d.Context(shape, f["delta"], density=f["rho"], line_spacing=1.0,
+                            precision=prec, max_iterations=1, seed_pn_with_p=0) as ctx:
+            ctx.set_directions(dirs)
+            ctx.set_field(f["acc"])
+            npairs = C.c_longlong(0)
+
+            def pairs():
+                p_dev.zero_()
+                omni3d.check(omni3d.lib().omni3d_boundary_pairs(
+                    ctx._h, 4, C.c_void_p(p_dev.data_ptr()), omni3d.F64, omni3d.DEVICE,
+                    C.byref(npairs)))
+            pairs()  # plan build
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                pairs()
+            torch.cuda.synchronize()
+            ms_pairs = (time.perf_counter() - t0) * 1e3 / args.steps
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                pairs()
+                ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
+            torch.cuda.synchronize()
+            ms_all = (time.perf_counter() - t0) * 1e3 / args.steps
+            p = p_dev.view(shape).cpu().numpy()
+            a = f["p"] - f["p"].mean()
+            err = float(np.linalg.norm((p - p.mean()) - a) / np.linalg.norm(a))
+            out[name] = {"ms_pairs_and_4_boundary_iterations": round(ms_pairs, 3),
+                         "ms_readme_pipeline": round(ms_all, 3), "pairs": npairs.value,
+                         "rel_l2_vs_analytic_demeaned": err}
+            log(f"pairs {name}: {out[name]}")
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        k = 2
+        t0 = time.perf_counter()
+        O.boundary_pairs(shape, dirs[:k], 1.0, f["delta"], f["rho"], f["acc"], np.zeros(shape), 4)
+        ts = time.perf_counter() - t0
+        cpu = {"value": round(ts * dirs.shape[0] / k, 1), "unit": "s", "cores": 1, "kind": "port",
+               "sample": f"oracle boundary pairs + 4 iterations over {k} of {dirs.shape[0]} "
+                         f"directions at {n}^3 took {ts:.2f}s, scaled by directions"}
+    print(json.dumps({
+        "metric": "ms per boundary-pair reconstruction (README pipeline), 256^3",
+        "value": out["fp32_kahan"]["ms_readme_pipeline"], "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "higher_is_better": False, "dtype": "fp32-kahan ray sums, f64 pairs",
+        "data": "synthetic", "results": out, "cpu_baseline": cpu,
+        "config": {"workload": f"burgers-vortex {n}^3, 2% noise, {dirs.shape[0]} directions, "
+                               "4 boundary iterations + one plain inner pass"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +378,7 @@ def main():
     ap.add_argument("--methods", action="store_true")
     ap.add_argument("--sor", action="store_true")
     ap.add_argument("--sor-iters", type=int, default=200)
+    ap.add_argument("--pairs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -330,6 +398,8 @@ def main():
         return methods_mode(args)
     if args.sor:
         return sor_mode(args)
+    if args.pairs:
+        return pairs_mode(args)
 
     import omni3d
     from omni3d import fields as F

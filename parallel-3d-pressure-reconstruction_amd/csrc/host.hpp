@@ -168,6 +168,7 @@ struct omni3d_ctx {
   omni3d::DevBuf segval;    // double[nval] segment sums, then carries
   int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
+  omni3d::DevBuf bp[13];    // boundary-pair pipeline: keys, sums, runs, pairs, pressure
   int used_copies = 1;
   omni3d::DevBuf pout, wout;  // staging for host outputs
   // naive schedule: pressure double buffer, aux field (mask or |curl a|*mask),

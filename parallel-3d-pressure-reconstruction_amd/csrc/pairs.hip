@@ -14,7 +14,9 @@
 // emits one (out, in) key and its integral, a radix sort groups the rays of
 // a pair (stable, so a pair's rays add in ray order), and a pair list sorted
 // by exit then entry voxel replaces the matrix: per exit node its pairs are
-// contiguous and the iteration is one lane per exit node.  The iteration is
+// contiguous and the iteration is one lane per exit node.  The ray sums come
+// from the tile engine's segment sums (or one lane per ray with the ray
+// engine).  The iteration is
 // a Jacobi sweep (the reference's only separation between iterations is a
 // __syncthreads(), an inter-block race); an exit node whose pairs are all
 // zero keeps its value (the reference divides 0 by 0).  Ray integrals use
@@ -29,6 +31,7 @@
 #include "../../include/omni3d.h"
 #include "driver.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 #include "walk.hpp"
 
 namespace omni3d {
@@ -67,6 +70,32 @@ k_bp_integral(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __
   val[r] = pint.value();
 }
 
+// Tile engine: the ray's sum is the sum of its segment sums (each step of
+// the ray is in exactly one segment's sum), read in segment order from the
+// value slots of its group of 64 rays.
+__global__ void __launch_bounds__(256)
+k_bp_total(const RayRec* __restrict__ rays, int64_t nrays, const unsigned int* __restrict__ nseg,
+           const unsigned long long* __restrict__ val_off, const double* __restrict__ segval,
+           unsigned long long* __restrict__ key, double* __restrict__ val) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const unsigned long long b = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int n = nseg[r];
+  double s = 0.0;
+  unsigned int k = 0;
+  for (; k + 4 <= n; k += 4) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[u] = segval[b + 64ull * (k + u)];
+#pragma unroll
+    for (int u = 0; u < 4; u++) s = s + x[u];
+  }
+  for (; k < n; k++) s = s + segval[b + 64ull * k];
+  const RayRec rr = rays[r];
+  key[r] = ((unsigned long long)rr.out << 30) | (unsigned long long)rr.in;
+  val[r] = s;
+}
+
 // 1 where a run of equal keys starts
 __global__ void k_bp_heads(const unsigned long long* __restrict__ key, int64_t n, int shift,
                            unsigned int* __restrict__ flag) {
@@ -100,24 +129,31 @@ __global__ void k_bp_pairs(const unsigned long long* __restrict__ key,
   pmean[p] = s / (double)(e - b);
 }
 
-// BCiteration for exit node o: pairs ostart[o] .. ostart[o+1]-1
-__global__ void k_bp_iterate(int64_t nout, const unsigned int* __restrict__ ostart,
-                             const unsigned long long* __restrict__ pkey,
-                             const double* __restrict__ pmean, int X, int Y,
-                             const double* __restrict__ p, double* __restrict__ pn) {
-  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nout) return;
+// BCiteration for exit node o: pairs ostart[o] .. ostart[o+1]-1, one
+// wavefront per node (coalesced pair reads, a fixed-order tree sum)
+constexpr int kBpNodesPerBlock = 4;
+__global__ void __launch_bounds__(64 * kBpNodesPerBlock)
+k_bp_iterate(int64_t nout, const unsigned int* __restrict__ ostart,
+             const unsigned long long* __restrict__ pkey, const double* __restrict__ pmean,
+             int X, int Y, const double* __restrict__ p, double* __restrict__ pn) {
+  const int64_t o = (int64_t)blockIdx.x * kBpNodesPerBlock + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (o >= nout) return;  // whole wavefronts leave together
   const unsigned int b = ostart[o], e = ostart[o + 1];
   double s = 0.0;
   int c = 0;
-  for (unsigned int i = b; i < e; i++) {
+  for (unsigned int i = b + lane; i < e; i += 64) {
     const double m = pmean[i];
     if (m != 0.0) {  // kernel.cu:5671: pairs with pint == 0 are skipped
       s += p[lin_of((uint32_t)(pkey[i] & kVoxMask), X, Y)] + m;
       c++;
     }
   }
-  pn[o] = c ? s / (double)c : p[lin_of((uint32_t)(pkey[b] >> 30), X, Y)];
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off);
+    c += __shfl_xor(c, off);
+  }
+  if (lane == 0) pn[o] = c ? s / (double)c : p[lin_of((uint32_t)(pkey[b] >> 30), X, Y)];
 }
 
 __global__ void k_bp_commit(int64_t nout, const unsigned int* __restrict__ ostart,
@@ -139,7 +175,16 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
   if (n > (int64_t)UINT32_MAX - 1) return fail(ctx, OMNI3D_ERR_ARG, "too many rays");
   *npairs_out = 0;
   if (n == 0) return OMNI3D_OK;
-  DevBuf k0, k1, v0, v1, flag, incl, start, tmp;
+  // work buffers stay with the context (freeing them costs more than the
+  // pipeline); sized for the largest call so far
+  DevBuf& k0 = ctx->bp[0];
+  DevBuf& k1 = ctx->bp[1];
+  DevBuf& v0 = ctx->bp[2];
+  DevBuf& v1 = ctx->bp[3];
+  DevBuf& flag = ctx->bp[4];
+  DevBuf& incl = ctx->bp[5];
+  DevBuf& start = ctx->bp[6];
+  DevBuf& tmp = ctx->bp[7];
   OMNI_TRY_HIP(ctx, k0.alloc(sizeof(unsigned long long) * n));
   OMNI_TRY_HIP(ctx, k1.alloc(sizeof(unsigned long long) * n));
   OMNI_TRY_HIP(ctx, v0.alloc(sizeof(double) * n));
@@ -148,9 +193,25 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
   OMNI_TRY_HIP(ctx, incl.alloc(sizeof(unsigned int) * n));
   OMNI_TRY_HIP(ctx, start.alloc(sizeof(unsigned int) * (n + 1)));
   using V4 = typename Vec4<T>::type;
-  hipLaunchKernelGGL(k_bp_integral<T>, dim3(blocks_for(n)), dim3(256), 0, st,
-                     pl.rays.as<RayRec>(), n, pl.dirs.as<DirConst>(), pl.g, walk_params(P),
-                     ctx->field.as<V4>(), k0.as<unsigned long long>(), v0.as<double>());
+  if (ctx->engine == 0 && pl.ntiles > 0 && pl.nval > 0) {
+    // segment sums on the tile engine (pass A's first traversal), then per
+    // ray the sum of its segments
+    OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * pl.nval));
+    OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
+                                          pl.tile_ptr1.as<unsigned long long>(),
+                                          pl.segs_slot.as<SegRec>(), pl.g, pl.tg, walk_params(P),
+                                          ctx->field.as<void>(), pl.sid.as<int>(),
+                                          pl.cnt.as<unsigned int>(), nullptr, nullptr,
+                                          ctx->scal.as<DevScalars>(), ctx->segval.as<double>(),
+                                          nullptr, st));
+    hipLaunchKernelGGL(k_bp_total, dim3(blocks_for(n)), dim3(256), 0, st, pl.rays.as<RayRec>(),
+                       n, pl.nseg.as<unsigned int>(), pl.val_off.as<unsigned long long>(),
+                       ctx->segval.as<double>(), k0.as<unsigned long long>(), v0.as<double>());
+  } else {
+    hipLaunchKernelGGL(k_bp_integral<T>, dim3(blocks_for(n)), dim3(256), 0, st,
+                       pl.rays.as<RayRec>(), n, pl.dirs.as<DirConst>(), pl.g, walk_params(P),
+                       ctx->field.as<V4>(), k0.as<unsigned long long>(), v0.as<double>());
+  }
   OMNI_TRY_HIP(ctx, hipGetLastError());
   // group the rays of each (exit, entry) pair; stable, so a pair's rays keep
   // the ray order
@@ -180,7 +241,10 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
                                    hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   const int64_t np = np32;
-  DevBuf pkey, pmean, ostart, pn;
+  DevBuf& pkey = ctx->bp[8];
+  DevBuf& pmean = ctx->bp[9];
+  DevBuf& ostart = ctx->bp[10];
+  DevBuf& pn = ctx->bp[11];
   OMNI_TRY_HIP(ctx, pkey.alloc(sizeof(unsigned long long) * np));
   OMNI_TRY_HIP(ctx, pmean.alloc(sizeof(double) * np));
   OMNI_TRY_HIP(ctx, ostart.alloc(sizeof(unsigned int) * (np + 1)));
@@ -210,7 +274,9 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
   const int64_t nout = no32;
   OMNI_TRY_HIP(ctx, pn.alloc(sizeof(double) * nout));
   for (int it = 0; it < iterations; it++) {
-    hipLaunchKernelGGL(k_bp_iterate, dim3(blocks_for(nout)), dim3(256), 0, st, nout,
+    hipLaunchKernelGGL(k_bp_iterate,
+                       dim3((unsigned int)((nout + kBpNodesPerBlock - 1) / kBpNodesPerBlock)),
+                       dim3(64 * kBpNodesPerBlock), 0, st, nout,
                        ostart.as<unsigned int>(), pkey.as<unsigned long long>(),
                        pmean.as<double>(), P.nx, P.ny, p, pn.as<double>());
     hipLaunchKernelGGL(k_bp_commit, dim3(blocks_for(nout)), dim3(256), 0, st, nout,
@@ -243,7 +309,8 @@ extern "C" int omni3d_boundary_pairs(omni3d_ctx* ctx, int iterations, void* p, i
   int rc = build_plan(ctx);
   if (rc) return rc;
   const size_t esz = elem_size(dtype);
-  DevBuf pd, stage;
+  DevBuf& pd = ctx->bp[12];
+  DevBuf stage;
   OMNI_TRY_HIP(ctx, pd.alloc(sizeof(double) * V));
   const void* src = p;
   if (mem_kind == OMNI3D_MEM_HOST) {

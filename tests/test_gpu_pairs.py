@@ -28,21 +28,38 @@ def boundary(shape):
     return b
 
 
-def gpu_pairs(f, dirs, shape, p0, iters, precision=omni3d.FP64, ls=1.0):
-    with omni3d.Context(shape, f["delta"], density=f["rho"], line_spacing=ls,
-                        precision=precision) as ctx:
-        ctx.set_directions(dirs)
-        ctx.set_field(f["acc"])
-        return ctx.boundary_pairs(p0, iters)
+def gpu_pairs(f, dirs, shape, p0, iters, precision=omni3d.FP64, ls=1.0, engine=None):
+    import os
+    old = os.environ.pop("OMNI3D_ENGINE", None)
+    if engine:
+        os.environ["OMNI3D_ENGINE"] = engine  # read when the context is created
+    try:
+        with omni3d.Context(shape, f["delta"], density=f["rho"], line_spacing=ls,
+                            precision=precision) as ctx:
+            ctx.set_directions(dirs)
+            ctx.set_field(f["acc"])
+            first = ctx.boundary_pairs(p0, iters)
+            # a second call on the same context reuses its work buffers
+            second = ctx.boundary_pairs(p0, iters)
+            np.testing.assert_array_equal(first[0], second[0])
+            return first
+    finally:
+        os.environ.pop("OMNI3D_ENGINE", None)
+        if old is not None:
+            os.environ["OMNI3D_ENGINE"] = old
 
 
-@pytest.mark.parametrize("shape,iters", [((9, 10, 11), 1), ((16, 12, 7), 3), ((20, 20, 20), 8)])
-def test_fp64_matches_oracle(gpu_available, ref_dirs2, shape, iters):
-    print(f"case boundary pairs {shape} x{iters}", flush=True)
+@pytest.mark.parametrize("shape,iters,engine", [((9, 10, 11), 1, None), ((16, 12, 7), 3, None),
+                                                ((20, 20, 20), 8, None),
+                                                ((16, 12, 7), 3, "rays")])
+def test_fp64_matches_oracle(gpu_available, ref_dirs2, shape, iters, engine):
+    """Ray sums from the tile engine's segment sums (default) or one lane per
+    ray (engine "rays")."""
+    print(f"case boundary pairs {shape} x{iters} {engine}", flush=True)
     f = F.taylor_green(max(shape), noise=0.05, seed=5, shape=shape)
     p0 = np.random.default_rng(7).normal(size=shape)
     want, nw = O.boundary_pairs(shape, ref_dirs2, 1.0, f["delta"], f["rho"], f["acc"], p0, iters)
-    got, ng = gpu_pairs(f, ref_dirs2, shape, p0, iters)
+    got, ng = gpu_pairs(f, ref_dirs2, shape, p0, iters, engine=engine)
     assert ng == nw
     err = np.linalg.norm(got - want) / np.linalg.norm(want)
     assert err <= 1e-12, err
