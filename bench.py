@@ -381,8 +381,8 @@ def main():
     ap.add_argument("--pairs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(direction sharding is described in DESIGN.md); run with --gpus 1.",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(directions would shard across GPUs; not built); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
