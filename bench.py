@@ -28,7 +28,9 @@ backward), and once with the plain-mask method (a traversal per iteration);
 --methods times one reference-schedule pass (a full traversal) of each
 integration method on the 128^3 masked Fourier-sphere field; --sor times the
 GPU red-black SOR (omega 1.8) on the 256^3 Neumann right-hand side of the
-Burgers-vortex field, beside the oracle's serial red-black SOR.
+Burgers-vortex field, beside the oracle's serial red-black SOR; --pairs times
+the README's boundary-pair pipeline at 256^3 (pairs, 4 boundary iterations,
+the inner pass); --threshold times the curl-histogram threshold at 256^3.
 """
 import argparse
 import ctypes as C
@@ -359,6 +361,58 @@ def pairs_mode(args):
                                "4 boundary iterations + one plain inner pass"}}))
 
 
+def threshold_mode(args):
+    """Curl-histogram threshold (kernel.cu:8380-8418) at 256^3 on a device
+    FP32 curl field: min/max pass, histogram pass, top-down walk."""
+    import torch
+    import omni3d
+    from omni3d import fields as F
+    n = args.n
+    shape = (n, n, n)
+    V = n ** 3
+    f = F.burgers_vortex(n, noise=0.02, seed=0)
+    with omni3d.Context(shape, f["delta"], density=f["rho"]) as ctx:
+        ctx.set_field(f["acc"])
+        curl = ctx.field_curl().astype(np.float32)
+    cd = torch.from_numpy(curl).cuda()
+    out = torch.empty(1, dtype=torch.float32, device="cuda")
+    L = omni3d.lib()
+
+    def call():
+        omni3d.check(L.omni3d_threshold_hist(n, n, n, C.c_void_p(cd.data_ptr()), 0.05,
+                                             C.c_void_p(out.data_ptr()), omni3d.DEVICE, None))
+    call()
+    torch.cuda.synchronize()
+    k = max(args.steps, 20)
+    t0 = time.perf_counter()
+    for _ in range(k):
+        call()
+    us = (time.perf_counter() - t0) * 1e6 / k
+    thr = float(out.item())
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        want = O.threshold_hist(curl, 0.05)
+        ts = time.perf_counter() - t0
+        assert want == thr, (want, thr)
+        cpu = {"value": round(ts * 1e6, 1), "unit": "us per call", "cores": 1, "kind": "port",
+               "sample": f"oracle threshold histogram at {n}^3 (one call, {ts:.3f}s)"}
+    bytes_call = 8 * V  # the curl read twice (min/max, then the histogram)
+    achieved = bytes_call / (us * 1e-6) / 1e9
+    print(json.dumps({
+        "metric": "us per curl-histogram threshold, 256^3", "value": round(us, 1), "unit": "us",
+        "n_gpus": 1, "steps": k, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+        "threshold": thr,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "whole call (k_thr_minmax + k_thr_hist + k_thr_pick, host sync)",
+                     "algorithmic_bytes": bytes_call},
+        "cpu_baseline": cpu,
+        "config": {"workload": f"|curl a| of the burgers-vortex {n}^3 field, percentage 0.05"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -379,6 +433,7 @@ def main():
     ap.add_argument("--sor", action="store_true")
     ap.add_argument("--sor-iters", type=int, default=200)
     ap.add_argument("--pairs", action="store_true")
+    ap.add_argument("--threshold", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
@@ -400,6 +455,8 @@ def main():
         return sor_mode(args)
     if args.pairs:
         return pairs_mode(args)
+    if args.threshold:
+        return threshold_mode(args)
 
     import omni3d
     from omni3d import fields as F

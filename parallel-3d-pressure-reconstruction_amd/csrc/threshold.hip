@@ -14,8 +14,8 @@
 // Kernels: k_thr_minmax (one pass, one atomic pair per workgroup on an
 // order-preserving integer image of the floats), k_thr_hist (per-workgroup
 // LDS histogram of the 10000 bins, non-empty bins added to the global one),
-// k_thr_pick (one workgroup: the serial FP32 walk from the top bin, so the
-// running count rounds exactly as the reference's).  HBM traffic: the volume
+// k_thr_pick (one workgroup: the top-down walk, as integer suffix sums over
+// the lanes where FP32 counts are exact, else serial in FP32).  HBM traffic: the volume
 // read twice (8 B per voxel) plus 40 KB of bins.
 #include <hip/hip_runtime.h>
 
@@ -102,22 +102,70 @@ __global__ void __launch_bounds__(kThrBlock) k_thr_hist(const float* __restrict_
     if (h[b]) atomicAdd(&hist[b], h[b]);
 }
 
-__global__ void k_thr_pick(const unsigned int* mm, const unsigned int* hist, float need,
-                           float* out) {
+// The top-down walk.  Up to 2^24 voxels every running count is an integer
+// that FP32 holds exactly, so the count above bin j is an integer suffix sum
+// and the walk's first hit is the largest j whose suffix sum reaches need:
+// kPickThreads lanes take kBins / kPickThreads bins each, a scan of their
+// totals gives each lane the count above its bins.  Larger volumes walk
+// serially in FP32 as the reference does.
+constexpr int kPickThreads = 1000;
+constexpr int kPickBins = kBins / kPickThreads;
+static_assert(kPickBins * kPickThreads == kBins, "bins split evenly over the lanes");
+
+__global__ void __launch_bounds__(1024) k_thr_pick(const unsigned int* mm,
+                                                   const unsigned int* hist, int64_t V,
+                                                   float need, float* out) {
   __shared__ unsigned int h[kBins];
-  for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = hist[b];
+  __shared__ unsigned int sfx[2][1024];
+  __shared__ int best;
+  const int t = threadIdx.x;
+  for (int b = t; b < kBins; b += blockDim.x) h[b] = hist[b];
+  if (t == 0) best = -1;
   __syncthreads();
-  if (threadIdx.x != 0) return;
   const float lo = float_of(mm[0]), hi = float_of(mm[1]);
-  float tot = 0.0f, r = hi;
-  for (int j = kBins - 1; j >= 0; j--) {
-    tot = __fadd_rn(tot, (float)h[j]);
-    if (tot >= need) {
-      r = __fadd_rn(__fmul_rn(__fdiv_rn((float)j, (float)kBins), __fsub_rn(hi, lo)), lo);
-      break;
+  auto edge = [&](int j) {
+    return __fadd_rn(__fmul_rn(__fdiv_rn((float)j, (float)kBins), __fsub_rn(hi, lo)), lo);
+  };
+  if (V > (1ll << 24)) {
+    if (t != 0) return;
+    float tot = 0.0f, r = hi;
+    for (int j = kBins - 1; j >= 0; j--) {
+      tot = __fadd_rn(tot, (float)h[j]);
+      if (tot >= need) {
+        r = edge(j);
+        break;
+      }
+    }
+    out[0] = r;
+    return;
+  }
+  // lane t owns bins [t * kPickBins, (t + 1) * kPickBins); inclusive suffix
+  // sums of the lane totals (Hillis-Steele over the 1024 slots, ping-pong)
+  unsigned int mine = 0u;
+  if (t < kPickThreads)
+    for (int q = 0; q < kPickBins; q++) mine += h[t * kPickBins + q];
+  sfx[0][t] = mine;
+  __syncthreads();
+  int cur = 0;
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int v = sfx[cur][t] + (t + off < 1024 ? sfx[cur][t + off] : 0u);
+    sfx[cur ^ 1][t] = v;
+    cur ^= 1;
+    __syncthreads();
+  }
+  if (t < kPickThreads) {
+    unsigned int run = sfx[cur][t] - mine;  // voxels in the bins above this lane's
+    for (int q = kPickBins - 1; q >= 0; q--) {
+      const int j = t * kPickBins + q;
+      run += h[j];
+      if ((float)run >= need) {
+        atomicMax(&best, j);
+        break;
+      }
     }
   }
-  out[0] = r;
+  __syncthreads();
+  if (t == 0) out[0] = best >= 0 ? edge(best) : hi;
 }
 
 #define THR_TRY(expr)                                                           \
@@ -138,7 +186,10 @@ int threshold_impl(int X, int Y, int Z, const float* curl, float percentage, flo
   const int64_t V = (int64_t)X * Y * Z;
   // kernel.cu:8408: Imax*Jmax*Kmax*percentage, an int product times a float
   const float need = (float)(int)V * percentage;
-  DevBuf scratch, dcurl;
+  // the 40 KB of bins stay allocated per host thread (the call synchronises
+  // before returning, so the next call on this thread may reuse them)
+  thread_local DevBuf scratch;
+  DevBuf dcurl;
   THR_TRY(scratch.alloc(sizeof(unsigned int) * (2 + kBins) + sizeof(float)));
   unsigned int* mm = scratch.as<unsigned int>();
   unsigned int* hist = mm + 2;
@@ -153,12 +204,11 @@ int threshold_impl(int X, int Y, int Z, const float* curl, float percentage, flo
   hipLaunchKernelGGL(k_thr_init, dim3(10), dim3(1024), 0, st, mm, hist);
   hipLaunchKernelGGL(k_thr_minmax, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm);
   hipLaunchKernelGGL(k_thr_hist, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm, hist);
-  hipLaunchKernelGGL(k_thr_pick, dim3(1), dim3(1024), 0, st, mm, hist, need,
+  hipLaunchKernelGGL(k_thr_pick, dim3(1), dim3(1024), 0, st, mm, hist, V, need,
                      mem_kind == OMNI3D_MEM_HOST ? dout : threshold);
   THR_TRY(hipGetLastError());
   if (mem_kind == OMNI3D_MEM_HOST)
     THR_TRY(hipMemcpyAsync(threshold, dout, sizeof(float), hipMemcpyDeviceToHost, st));
-  // the scratch bins are freed on return
   THR_TRY(hipStreamSynchronize(st));
   return OMNI3D_OK;
 }

@@ -210,12 +210,14 @@ def test_sor_redblack_rows_bit_exact_device_buffers(gpu_available):
 
 @pytest.mark.parametrize("shape,kind", [((1, 1, 1), "uniform"), ((19, 23, 37), "uniform"),
                                         ((64, 64, 64), "curl"), ((128, 128, 128), "normal"),
-                                        ((40, 40, 40), "constant")])
+                                        ((40, 40, 40), "constant"),
+                                        ((257, 256, 256), "normal")])
 def test_threshold_hist_bit_exact(gpu_available, shape, kind):
     """thredholdHistMaterialAccCPU (kernel.cu:8380-8418): min / max, the
     10000-bin histogram and the FP32 top-down walk, bit-identical to the
     oracle through both entry points (host signature of the reference and
-    the device-buffer form)."""
+    the device-buffer form); the 257 x 256 x 256 case (above 2^24 voxels)
+    takes the serial FP32 walk, the others the integer suffix sums."""
     import torch
     print(f"case threshold {kind} {shape}", flush=True)
     rng = np.random.default_rng(sum(shape))
