@@ -15,6 +15,8 @@
  *   kernel.cu:6151       omni3dparallellinesESInner  -> omni3d_es_inner
  *   kernel.cu:6281       ...ESInnerWeighted          -> omni3d_es_inner_weighted
  *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select
+ *   kernel.cu:6481       ...ESInnerSelectFixedBC     -> omni3d_es_inner_select_fixedbc
+ *   kernel.cu:6347       ...ESInnerWeightedMiniCurl  -> omni3d_es_inner_weighted_minicurl
  *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner
  *   kernel.cu:8352       calCurlofMaterialAccCPU     -> omni3d_curl
  *   kernel.cu:6863       calCurlofMaterialAcc (GPU)  -> omni3d_curl, mask NULL
@@ -58,6 +60,15 @@ extern "C" {
 #define OMNI3D_METHOD_PLAIN_MASK 1 /* Select walk with mask, unit weights, threshold +inf */
 #define OMNI3D_METHOD_SELECT 2     /* bodyIntegralInnerSelect, kernel.cu:2280 */
 #define OMNI3D_METHOD_WEIGHTED 3   /* bodyIntegralInnerWeighted, kernel.cu:1993 */
+/* bodyIntegralInnerSelectFixedBC, kernel.cu:2538: Select with its own step
+ * rule (x / y ties to y, no step along or onto the j = 0 plane, which keeps
+ * its prescribed pressure); one lane per ray, reference schedule */
+#define OMNI3D_METHOD_SELECT_FIXEDBC 4
+/* bodyIntegralInnerMiniCurl, kernel.cu:2797 (the pass of
+ * ESInnerWeightedMiniCurl): plain sums and counts, no entry self-add, along
+ * the path through the smallest curl toward the exit; one lane per ray,
+ * reference schedule */
+#define OMNI3D_METHOD_MINICURL 5
 
 /* Arithmetic of the ray sums. */
 #define OMNI3D_FP64 0        /* gradient and ray sums in FP64 */
@@ -175,6 +186,23 @@ int omni3d_es_inner_select(int X, int Y, int Z, int NoAngles, float linespacing,
                            const float* DuDt, const float* DvDt, const float* DwDt,
                            const float* curl, const float* p, float* pn, float* pcountinner,
                            float threshold, void* hip_stream);
+/* omni3dparallellinesESInnerSelectFixedBC, kernel.cu:6481-6549 (commented out
+ * in the reference's main): the Select pass with the FixedBC step rule. */
+int omni3d_es_inner_select_fixedbc(int X, int Y, int Z, int NoAngles, float linespacing,
+                                   const float* k1_d, const float* k2_d, const float* k3_d,
+                                   float deltx, float delty, float deltz, float density,
+                                   const float* DuDt, const float* DvDt, const float* DwDt,
+                                   const float* curl, const float* p, float* pn,
+                                   float* pcountinner, float threshold, void* hip_stream);
+/* omni3dparallellinesESInnerWeightedMiniCurl, kernel.cu:6347-6413 (commented
+ * out in the reference's main; it calls bodyIntegralInnerMiniCurl): one
+ * MiniCurl pass, unit counts into pcountinner. */
+int omni3d_es_inner_weighted_minicurl(int X, int Y, int Z, int NoAngles, float linespacing,
+                                      const float* k1_d, const float* k2_d, const float* k3_d,
+                                      float deltx, float delty, float deltz, float density,
+                                      const float* DuDt, const float* DvDt, const float* DwDt,
+                                      const float* curl, const float* p, float* pn,
+                                      float* pcountinner, void* hip_stream);
 /* devidecountInner, kernel.cu:6746-6759: where pcountinner > 0,
  * p = pn / pcountinner and pn = 0. */
 int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const float* pcountinner,

@@ -227,6 +227,7 @@ typedef struct {
   int c[3], e[3];
   double u[3];
   const orc_dir* d;
+  int fixedbc; /* the step rule of bodyIntegralInnerSelectFixedBC */
 } walker;
 
 static void walker_init(walker* w, int X, int Y, int Z, const orc_dir* d, const double o[3],
@@ -234,6 +235,7 @@ static void walker_init(walker* w, int X, int Y, int Z, const orc_dir* d, const 
   /* kernel.cu:1749-1751: the line passes through o + centre in index space */
   const double cc[3] = {(X - 1) / 2.0, (Y - 1) / 2.0, (Z - 1) / 2.0};
   w->X = X; w->Y = Y; w->Z = Z; w->d = d;
+  w->fixedbc = 0;
   double wv[3];
   for (int a = 0; a < 3; a++) {
     w->c[a] = in[a];
@@ -257,10 +259,22 @@ static int walker_choose(const walker* w, int* s) {
     f[a] = (2.0 * dd) * w->u[a] + (dd * dd) * w->d->M[a][a];
   }
   int a;
-  if (f[0] <= f[1] && f[0] <= f[2]) a = 0;
-  else if (f[1] < f[0] && f[1] <= f[2]) a = 1;
-  else if (f[2] < f[0] && f[2] < f[1]) a = 2;
-  else return -1;
+  if (w->fixedbc) {
+    /* kernel.cu:2628,2676,2722: an x / y tie goes to y; no x- or z-step
+     * while on the j = 0 plane, no y-step onto it (the plane keeps its
+     * prescribed pressure: only rays entering there read it) */
+    if (f[0] < f[1] && f[0] <= f[2]) a = 0;
+    else if (f[1] <= f[0] && f[1] <= f[2]) a = 1;
+    else if (f[2] < f[0] && f[2] < f[1]) a = 2;
+    else return -1;
+    if (a != 1 && w->c[1] == 0) return -1;
+    if (a == 1 && w->c[1] + st[1] == 0) return -1;
+  } else {
+    if (f[0] <= f[1] && f[0] <= f[2]) a = 0;
+    else if (f[1] < f[0] && f[1] <= f[2]) a = 1;
+    else if (f[2] < f[0] && f[2] < f[1]) a = 2;
+    else return -1;
+  }
   int n = w->c[a] + st[a];
   if (n < 0 || n >= size[a]) return -1;
   *s = st[a];
@@ -312,6 +326,50 @@ static int walk_plain(const pass_ctx* P, walker* w) {
     walker_step(w, a, s);
     int64_t vn = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
     pint += trap(P->rho, s, P->del[a], P->acc[a][vn], P->acc[a][vl]);
+    P->pn[vn] += pin + pint;
+    P->cnt[vn] += 1.0;
+    steps++;
+  }
+  return steps;
+}
+
+/* bodyIntegralInnerMiniCurl, kernel.cu:2797-2936 (the pass of
+ * omni3dparallellinesESInnerWeightedMiniCurl, kernel.cu:6347-6413): the
+ * plain sums and unit counts without the entry self-add, along a path that
+ * steps toward the exit through the candidate with the smallest curl (ties x,
+ * then y, then z, kernel.cu:2883,2893,2903).  A candidate on an axis already
+ * at the exit is the reference's sentinel i - 60000 (or j, k): its curl is
+ * read at the aliased linear index when that lies inside the volume, else
+ * it counts as 1e10, and choosing it ends the walk (kernel.cu:2867-2879). */
+static int walk_minicurl(const pass_ctx* P, walker* w) {
+  const int X = P->X, Y = P->Y, Z = P->Z;
+  const int64_t V = (int64_t)X * Y * Z;
+  const int size[3] = {X, Y, Z};
+  const int64_t stride[3] = {1, X, (int64_t)X * Y};
+  const int64_t vin = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+  const double pin = P->p[vin];
+  if (P->src) P->src[vin] = 1;
+  double pint = 0.0;
+  int steps = 0;
+  while (!walker_done(w)) {
+    const int64_t v = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+    int st[3];
+    double d[3];
+    for (int a = 0; a < 3; a++) {
+      st[a] = w->c[a] < w->e[a] ? 1 : (w->c[a] > w->e[a] ? -1 : -60000);
+      const int64_t idx = v + (int64_t)st[a] * stride[a];
+      d[a] = (idx >= 0 && idx < V) ? P->curl[idx] : 1e10;
+    }
+    int a;
+    if (d[0] <= d[1] && d[0] <= d[2]) a = 0;
+    else if (d[1] < d[0] && d[1] <= d[2]) a = 1;
+    else if (d[2] < d[0] && d[2] < d[1]) a = 2;
+    else break;
+    const int n = w->c[a] + st[a];
+    if (n < 0 || n >= size[a]) break;
+    w->c[a] = n;
+    const int64_t vn = v + (int64_t)st[a] * stride[a];
+    pint += trap(P->rho, st[a], P->del[a], P->acc[a][vn], P->acc[a][v]);
     P->pn[vn] += pin + pint;
     P->cnt[vn] += 1.0;
     steps++;
@@ -447,6 +505,11 @@ void orc_pass(int X, int Y, int Z, int D, const float* k1, const float* k2, cons
         case ORC_WEIGHTED: walk_weighted(&P, &w); break;
         case ORC_SELECT: walk_select(&P, &w, 0); break;
         case ORC_PLAIN_MASK: walk_select(&P, &w, 1); break;
+        case ORC_SELECT_FIXEDBC:
+          w.fixedbc = 1;
+          walk_select(&P, &w, 0);
+          break;
+        case ORC_MINICURL: walk_minicurl(&P, &w); break;
         default: break;
       }
     }
@@ -480,7 +543,7 @@ int64_t orc_enumerate_rays(int X, int Y, int Z, int D, const float* k1, const fl
 }
 
 int orc_ray_path(int X, int Y, int Z, float k1, float k2, float k3, double ls, int64_t point,
-                 int64_t* path, int capacity) {
+                 int64_t* path, int capacity, int fixedbc) {
   orc_dir d;
   orc_make_dir(k1, k2, k3, &d);
   const int ng = orc_nogrid(X, Y, Z, ls);
@@ -490,6 +553,7 @@ int orc_ray_path(int X, int Y, int Z, float k1, float k2, float k3, double ls, i
   if (!orc_clip(X, Y, Z, o, d.k, in, out)) return 0;
   walker w;
   walker_init(&w, X, Y, Z, &d, o, in, out);
+  w.fixedbc = fixedbc;
   int n = 0;
   if (n < capacity) path[n] = vidx(X, Y, w.c[0], w.c[1], w.c[2]);
   n++;

@@ -18,7 +18,9 @@
 extern "C" {
 #endif
 
-enum { ORC_PLAIN = 0, ORC_PLAIN_MASK = 1, ORC_SELECT = 2, ORC_WEIGHTED = 3 };
+enum { ORC_PLAIN = 0, ORC_PLAIN_MASK = 1, ORC_SELECT = 2, ORC_WEIGHTED = 3,
+       ORC_SELECT_FIXEDBC = 4, /* bodyIntegralInnerSelectFixedBC, kernel.cu:2538-2796 */
+       ORC_MINICURL = 5 /* bodyIntegralInnerMiniCurl, kernel.cu:2797-2936 */ };
 
 /* Per-direction constants derived from one unit vector. */
 typedef struct {
@@ -49,9 +51,10 @@ int orc_clip(int X, int Y, int Z, const double o[3], const double k[3], int in[3
 int64_t orc_enumerate_rays(int X, int Y, int Z, int D, const float* k1, const float* k2,
                            const float* k3, double ls, int32_t* rays, int64_t capacity);
 
-/* Staircase path of one ray as voxel indices (entry first); returns its length. */
+/* Staircase path of one ray as voxel indices (entry first); returns its
+ * length.  fixedbc: the step rule of bodyIntegralInnerSelectFixedBC. */
 int orc_ray_path(int X, int Y, int Z, float k1, float k2, float k3, double ls, int64_t point,
-                 int64_t* path, int capacity);
+                 int64_t* path, int capacity, int fixedbc);
 
 /* One full pass (kernel.cu:6151-6215 / 6281-6346 / 6414-6480 with full
  * (line x direction) coverage): pn += contributions, cnt += weights.

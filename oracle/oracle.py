@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-PLAIN, PLAIN_MASK, SELECT, WEIGHTED = 0, 1, 2, 3
+PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
 
 _lib = None
 
@@ -61,7 +61,7 @@ def _declare(L):
     L.orc_nogrid.restype = i
     L.orc_enumerate_rays.argtypes = [i, i, i, i, _fp, _fp, _fp, d, _i32p, i64]
     L.orc_enumerate_rays.restype = i64
-    L.orc_ray_path.argtypes = [i, i, i, C.c_float, C.c_float, C.c_float, d, i64, _i64p, i]
+    L.orc_ray_path.argtypes = [i, i, i, C.c_float, C.c_float, C.c_float, d, i64, _i64p, i, i]
     L.orc_ray_path.restype = i
     L.orc_pass.argtypes = [i, i, i, i, _fp, _fp, _fp, d, d, d, d, d, _dp, _dp, _dp, _dp, d, i,
                            i, _dp, _dp, _dp, _u8p]
@@ -149,11 +149,11 @@ def enumerate_rays(shape, dirs, ls):
     return np.concatenate([buf[:, :1], pt[:, None], buf[:, 3:]], axis=1)
 
 
-def ray_path(shape, kvec, ls, point, capacity=1 << 16):
+def ray_path(shape, kvec, ls, point, capacity=1 << 16, fixedbc=0):
     Z, Y, X = shape
     out = np.zeros(capacity, dtype=np.int64)
     n = lib().orc_ray_path(X, Y, Z, float(kvec[0]), float(kvec[1]), float(kvec[2]), ls, point,
-                           _ptr(out, _i64p), capacity)
+                           _ptr(out, _i64p), capacity, fixedbc)
     return out[:n]
 
 

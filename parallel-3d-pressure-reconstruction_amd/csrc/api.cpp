@@ -44,7 +44,7 @@ bool params_ok(const omni3d_params& p, std::string* why) {
     *why = "line spacing must be positive";
     return false;
   }
-  if (p.method < OMNI3D_METHOD_PLAIN || p.method > OMNI3D_METHOD_WEIGHTED) {
+  if (p.method < OMNI3D_METHOD_PLAIN || p.method > OMNI3D_METHOD_MINICURL) {
     *why = "unknown method";
     return false;
   }
@@ -841,11 +841,15 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
     }
     aux = ctx->aux.as<double>();
   }
-  const bool fixed = m == OMNI3D_METHOD_PLAIN || m == OMNI3D_METHOD_PLAIN_MASK;
+  // unit weights: plain, plain-mask and MiniCurl (its counts come from its own walk)
+  const bool unitw = m == OMNI3D_METHOD_PLAIN_MASK || m == OMNI3D_METHOD_MINICURL;
+  const bool fixed = m == OMNI3D_METHOD_PLAIN || unitw;
   // every method runs on the tile engine (OMNI3D_MASKED=naive: one lane per
   // ray with global atomics)
   const char* menv = std::getenv("OMNI3D_MASKED");
-  const bool tiled = !(menv && std::strcmp(menv, "naive") == 0);
+  // SelectFixedBC steps by its own rule, not along the plan's staircase
+  const bool tiled = !(menv && std::strcmp(menv, "naive") == 0) &&
+                     m != OMNI3D_METHOD_SELECT_FIXEDBC && m != OMNI3D_METHOD_MINICURL;
   if (tiled) {
     const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
     const bool curlw = !fixed;
@@ -863,7 +867,7 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * naive_update_grid(V)));
   if (fixed) {
     OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
-    if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+    if (unitw) OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
   } else {
     OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
@@ -876,7 +880,7 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
     OMNI_TRY_HIP(ctx, launch_fill_f64(pa, V, 0.0, st));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
   // the weights depend on the field only: accumulated by the first iteration
-  if (m == OMNI3D_METHOD_PLAIN_MASK) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+  if (unitw) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
   if (!fixed) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
@@ -929,8 +933,8 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
     OMNI_TRY_HIP(ctx, launch_walk_naive<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
                                            pl.g, wp, field, aux, pa, pb, k, sc,
                                            fixed ? ctx->sum.as<long long>() : nullptr,
-                                           (first && m == OMNI3D_METHOD_PLAIN_MASK)
-                                               ? ctx->wcnt.as<unsigned int>() : nullptr,
+                                           (first && unitw) ? ctx->wcnt.as<unsigned int>()
+                                                            : nullptr,
                                            fixed ? nullptr : ctx->sumd.as<double>(),
                                            (first && !fixed) ? ctx->wsum.as<double>() : nullptr,
                                            first ? ctx->srcf.as<uint8_t>() : nullptr, st));

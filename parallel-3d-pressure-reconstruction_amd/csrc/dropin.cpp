@@ -110,7 +110,10 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   const Plan& pl = *ctx->plan;
   const int64_t V = (int64_t)X * Y * Z;
   DevScalars* sc = ctx->scal.as<DevScalars>();
-  const bool fixed = method == OMNI3D_METHOD_PLAIN;
+  // fixed-point sums with unit weights: plain (counts of the plan) and
+  // MiniCurl (counts of its own paths, counted by the walk)
+  const bool minicurl = method == OMNI3D_METHOD_MINICURL;
+  const bool fixed = method == OMNI3D_METHOD_PLAIN || minicurl;
   OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
   OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
   OMNI_TRY_HIP(ctx, (launch_convert<float, double>(p, ctx->pa.as<double>(), V, st)));
@@ -121,10 +124,16 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   OMNI_TRY_HIP(ctx, launch_compute_scale(sc, std::fabs(gfac), (double)pl.lmax, (double)pl.cmax,
                                          1.0, st));
   const double* aux = nullptr;
-  if (!fixed) {
+  if (method != OMNI3D_METHOD_PLAIN) {
     OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, (launch_convert<float, double>(curl, ctx->aux.as<double>(), V, st)));
     aux = ctx->aux.as<double>();
+  }
+  if (minicurl) {
+    OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+  }
+  if (!fixed) {
     OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
@@ -137,7 +146,9 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   // the pass runs on the tile engine (rule replay, carry chain, tile pass);
   // OMNI3D_MASKED=naive walks one lane per ray with global atomics
   const char* menv = std::getenv("OMNI3D_MASKED");
-  if (!(menv && std::strcmp(menv, "naive") == 0)) {
+  // SelectFixedBC's step rule differs from the plan's staircase: one lane per ray
+  if (!(menv && std::strcmp(menv, "naive") == 0) && method != OMNI3D_METHOD_SELECT_FIXEDBC &&
+      !minicurl) {
     const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
     OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
     OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
@@ -178,13 +189,15 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
                                                 ctx->field.as<void>(), aux, ctx->pa.as<double>(),
                                                 ctx->pa.as<double>(), 1, sc,
                                                 fixed ? ctx->sum.as<long long>() : nullptr,
-                                                nullptr, fixed ? nullptr : ctx->sumd.as<double>(),
+                                                minicurl ? ctx->wcnt.as<unsigned int>() : nullptr,
+                                                fixed ? nullptr : ctx->sumd.as<double>(),
                                                 fixed ? nullptr : ctx->wsum.as<double>(),
                                                 ctx->srcf.as<uint8_t>(), st));
   }
   OMNI_TRY_HIP(ctx, launch_accumulate_out(V, fixed ? 1 : 0,
                                           fixed ? ctx->sum.as<long long>() : nullptr,
-                                          pl.cnt.as<unsigned int>(),
+                                          minicurl ? ctx->wcnt.as<unsigned int>()
+                                                   : pl.cnt.as<unsigned int>(),
                                           fixed ? nullptr : ctx->sumd.as<double>(),
                                           fixed ? nullptr : ctx->wsum.as<double>(), sc, pn, pc, st));
   // the pass reads the context's work buffers: finish before another call reuses them
@@ -226,6 +239,28 @@ int omni3d_es_inner_select(int X, int Y, int Z, int NoAngles, float linespacing,
                            float* pn, float* pcountinner, float threshold, void* hip_stream) {
   return es_pass(OMNI3D_METHOD_SELECT, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
                  delty, deltz, density, DuDt, DvDt, DwDt, curl, threshold, p, pn, pcountinner,
+                 hip_stream);
+}
+
+int omni3d_es_inner_select_fixedbc(int X, int Y, int Z, int NoAngles, float linespacing,
+                                   const float* k1_d, const float* k2_d, const float* k3_d,
+                                   float deltx, float delty, float deltz, float density,
+                                   const float* DuDt, const float* DvDt, const float* DwDt,
+                                   const float* curl, const float* p, float* pn,
+                                   float* pcountinner, float threshold, void* hip_stream) {
+  return es_pass(OMNI3D_METHOD_SELECT_FIXEDBC, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d,
+                 deltx, delty, deltz, density, DuDt, DvDt, DwDt, curl, threshold, p, pn,
+                 pcountinner, hip_stream);
+}
+
+int omni3d_es_inner_weighted_minicurl(int X, int Y, int Z, int NoAngles, float linespacing,
+                                      const float* k1_d, const float* k2_d, const float* k3_d,
+                                      float deltx, float delty, float deltz, float density,
+                                      const float* DuDt, const float* DvDt, const float* DwDt,
+                                      const float* curl, const float* p, float* pn,
+                                      float* pcountinner, void* hip_stream) {
+  return es_pass(OMNI3D_METHOD_MINICURL, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
+                 delty, deltz, density, DuDt, DvDt, DwDt, curl, 0.0f, p, pn, pcountinner,
                  hip_stream);
 }
 

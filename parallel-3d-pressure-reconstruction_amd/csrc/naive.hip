@@ -57,7 +57,7 @@ __device__ __forceinline__ void naive_plain(const RayRec& rr, const DirConst* __
 // voxel and do not treat curl == 0 as leaving the valid zone
 // (kernel.cu:2422,2442,2468,2488), and a z-step restart keeps the weight
 // accumulators (kernel.cu:2497-2506).
-template <typename T, bool kUnit>
+template <typename T, bool kUnit, bool kFixedBC = false>
 __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* __restrict__ dirs,
                                              const GridDesc& g, const WalkParams& wp,
                                              const typename Vec4<T>::type* __restrict__ field,
@@ -87,7 +87,7 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
   V4 cur = field[w.v];
   while (!w.done()) {
     int s;
-    const int a = w.advance(&s);
+    const int a = kFixedBC ? w.advance_fixedbc(&s) : w.advance(&s);
     if (a < 0) break;
     const V4 nxt = field[w.v];
     const double cn = aux[w.v];
@@ -169,6 +169,57 @@ __device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst*
   }
 }
 
+// MiniCurl walk (bodyIntegralInnerMiniCurl, kernel.cu:2797-2936, the pass of
+// omni3dparallellinesESInnerWeightedMiniCurl): plain sums and unit counts,
+// no entry self-add, along a path that steps toward the exit through the
+// candidate voxel with the smallest curl (ties x, then y, then z).  An axis
+// already at the exit offers the reference's sentinel (coordinate - 60000):
+// its curl is read at the aliased linear index when that lies in the volume,
+// else it counts as 1e10; choosing it ends the walk.
+template <typename T>
+__device__ __forceinline__ void naive_minicurl(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                               const GridDesc& g, const WalkParams& wp,
+                                               const typename Vec4<T>::type* __restrict__ field,
+                                               const double* __restrict__ aux,
+                                               const double* __restrict__ p, double scale,
+                                               long long* __restrict__ sumfx,
+                                               unsigned int* __restrict__ wcnt,
+                                               uint8_t* __restrict__ src) {
+  using V4 = typename Vec4<T>::type;
+  Walker w;
+  w.init(rr, dirs, g);
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  const double pin = p[w.v];
+  if (src) src[w.v] = 1;
+  V4 cur = field[w.v];
+  RaySum<T> pint;
+  while (!w.done()) {
+    const int s0 = w.c0 < w.e0 ? 1 : (w.c0 > w.e0 ? -1 : -60000);
+    const int s1 = w.c1 < w.e1 ? 1 : (w.c1 > w.e1 ? -1 : -60000);
+    const int s2 = w.c2 < w.e2 ? 1 : (w.c2 > w.e2 ? -1 : -60000);
+    const int64_t i0 = w.v + s0, i1 = w.v + (int64_t)s1 * w.sy, i2 = w.v + (int64_t)s2 * w.sz;
+    const double d0 = (i0 >= 0 && i0 < V) ? aux[i0] : 1e10;
+    const double d1 = (i1 >= 0 && i1 < V) ? aux[i1] : 1e10;
+    const double d2 = (i2 >= 0 && i2 < V) ? aux[i2] : 1e10;
+    int a, s, nc, lim;
+    int64_t vn;
+    if (d0 <= d1 && d0 <= d2) { a = 0; s = s0; nc = w.c0 + s0; lim = g.X; vn = i0; }
+    else if (d1 < d0 && d1 <= d2) { a = 1; s = s1; nc = w.c1 + s1; lim = g.Y; vn = i1; }
+    else if (d2 < d0 && d2 < d1) { a = 2; s = s2; nc = w.c2 + s2; lim = g.Z; vn = i2; }
+    else break;
+    if (nc < 0 || nc >= lim) break;
+    if (a == 0) w.c0 = nc;
+    else if (a == 1) w.c1 = nc;
+    else w.c2 = nc;
+    w.v = vn;
+    const V4 nxt = field[vn];
+    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+    fx_add(&sumfx[vn], to_fixed(pin + pint.value(), scale));
+    if (wcnt) atomicAdd(&wcnt[vn], 1u);
+    cur = nxt;
+  }
+}
+
 template <typename T, int kMethod>
 __global__ void __launch_bounds__(256)
 k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
@@ -190,6 +241,11 @@ k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
   else if (kMethod == 2)
     naive_select<T, false>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd, wsum,
                            src);
+  else if (kMethod == 4)
+    naive_select<T, false, true>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd,
+                                 wsum, src);
+  else if (kMethod == 5)
+    naive_minicurl<T>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, src);
   else
     naive_weighted<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
 }
@@ -443,6 +499,8 @@ hipError_t launch_walk_naive(int method, const RayRec* rays, int64_t nrays, cons
     case 0: OMNI_NAIVE(0); break;
     case 1: OMNI_NAIVE(1); break;
     case 2: OMNI_NAIVE(2); break;
+    case 4: OMNI_NAIVE(4); break;
+    case 5: OMNI_NAIVE(5); break;
     default: OMNI_NAIVE(3); break;
   }
 #undef OMNI_NAIVE

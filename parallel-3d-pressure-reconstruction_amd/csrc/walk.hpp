@@ -79,6 +79,36 @@ struct Walker {
     return a;
   }
 
+  // The step rule of bodyIntegralInnerSelectFixedBC (kernel.cu:2628, 2676,
+  // 2722): an x / y tie goes to y, no x- or z-step while on the j = 0 plane
+  // and no y-step onto it; a refused step ends the walk.
+  __device__ __forceinline__ int advance_fixedbc(int* s_out) {
+    const int s0 = sgn_step(c0, e0), s1 = sgn_step(c1, e1), s2 = sgn_step(c2, e2);
+    const double f0 = step_metric(s0, u0, m00);
+    const double f1 = step_metric(s1, u1, m11);
+    const double f2 = step_metric(s2, u2, m22);
+    int a, s, nc, lim;
+    if (f0 < f1 && f0 <= f2) { a = 0; s = s0; nc = c0 + s0; lim = X; }
+    else if (f1 <= f0 && f1 <= f2) { a = 1; s = s1; nc = c1 + s1; lim = Y; }
+    else if (f2 < f0 && f2 < f1) { a = 2; s = s2; nc = c2 + s2; lim = Z; }
+    else return -1;
+    if (a != 1 && c1 == 0) return -1;
+    if (a == 1 && nc == 0) return -1;
+    if (nc < 0 || nc >= lim) return -1;
+    const double r0 = a == 0 ? m00 : (a == 1 ? m01 : m02);
+    const double r1 = a == 0 ? m01 : (a == 1 ? m11 : m12);
+    const double r2 = a == 0 ? m02 : (a == 1 ? m12 : m22);
+    const double sd = (double)s;
+    u0 = u0 + sd * r0;
+    u1 = u1 + sd * r1;
+    u2 = u2 + sd * r2;
+    if (a == 0) { c0 = nc; v += s; }
+    else if (a == 1) { c1 = nc; v += s * sy; }
+    else { c2 = nc; v += s * sz; }
+    *s_out = s;
+    return a;
+  }
+
   __device__ __forceinline__ bool on_surface() const {
     return c0 == 0 || c0 == X - 1 || c1 == 0 || c1 == Y - 1 || c2 == 0 || c2 == Z - 1;
   }

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libomni3d.so")
 
 OK = 0
 ERR_ARG, ERR_STATE, ERR_HIP, ERR_IO, ERR_NONFINITE, ERR_NODEVICE = 1, 2, 3, 4, 5, 6
-PLAIN, PLAIN_MASK, SELECT, WEIGHTED = 0, 1, 2, 3
+PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
 FP64, FP32_KAHAN = 0, 1
 AFFINE, NAIVE = 0, 1
 F32, F64 = 0, 1
@@ -34,6 +34,7 @@ EXPORTS = [
     "omni3d_batch_run", "omni3d_batch_result", "omni3d_field_curl",
     "omni3d_read_parameter_file", "omni3d_run_parameter_file_ex",
     "thredholdHistMaterialAccCPU", "omni3d_threshold_hist", "omni3d_boundary_pairs",
+    "omni3d_es_inner_select_fixedbc", "omni3d_es_inner_weighted_minicurl",
 ]
 
 
@@ -113,6 +114,8 @@ def _declare(L):
     L.omni3d_es_inner.argtypes = es + [vp, vp, vp, vp]
     L.omni3d_es_inner_weighted.argtypes = es + [vp, vp, vp, vp, vp]
     L.omni3d_es_inner_select.argtypes = es + [vp, vp, vp, vp, f, vp]
+    L.omni3d_es_inner_select_fixedbc.argtypes = es + [vp, vp, vp, vp, f, vp]
+    L.omni3d_es_inner_weighted_minicurl.argtypes = es + [vp, vp, vp, vp, vp]
     L.omni3d_devide_count_inner.argtypes = [i, i, i, vp, vp, vp, vp]
     L.omni3d_curl.argtypes = [i, i, i, f, f, f, vp, vp, vp, vp, vp, vp]
     L.thredholdHistMaterialAccCPU.argtypes = [i, i, i, C.POINTER(C.c_float), f,

@@ -103,6 +103,127 @@ def test_select(gpu_available, ref_dirs2, faithful, precision):
     assert err < (1e-10 if precision == omni3d.FP64 else 1e-5)
 
 
+@pytest.mark.parametrize("faithful", [1, 0])
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_select_fixedbc(gpu_available, ref_dirs2, faithful, precision):
+    """SelectFixedBC (kernel.cu:2538-2796): the oracle's result, the j = 0
+    plane kept at its starting pressure, and the kernel-level drop-in
+    (kernel.cu:6481) equal to one pass of the oracle."""
+    shape = (18, 20, 19)
+    print(f"case select-fixedbc faithful={faithful} precision={precision}", flush=True)
+    f = F.taylor_green(20, noise=0.05, seed=6, shape=shape)
+    acc = [np.asarray(a, np.float32).astype(np.float64) if precision else a for a in f["acc"]]
+    f = dict(f, acc=acc)
+    m = sphere(shape, 4.0)
+    curl = O.curl(shape, f["delta"], acc, m)
+    thr = float(np.quantile(curl[m > 0], 0.8))
+    p0 = np.random.default_rng(8).normal(size=shape)
+    p, w, rep = gpu(f, ref_dirs2, shape, mask=m, precision=precision,
+                    method=omni3d.SELECT_FIXEDBC, threshold=thr, faithful_select=faithful,
+                    max_iterations=5, p0=p0, seed_pn_with_p=0)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc, p0=p0,
+                                    curl=curl, threshold=thr, method=O.SELECT_FIXEDBC,
+                                    faithful=faithful, max_iter=5, seed_pn=0)
+    assert rep.iterations == it
+    assert F.rel_l2(w, co) < 1e-12
+    err = F.rel_l2(p, po)
+    print("select-fixedbc rel-L2 vs oracle", err)
+    assert err < (1e-10 if precision == omni3d.FP64 else 1e-5)
+    np.testing.assert_array_equal(p[:, 0, :], p0[:, 0, :])
+
+
+def test_es_inner_select_fixedbc_dropin(gpu_available, ref_dirs2):
+    import torch
+    shape = (12, 13, 14)
+    print("case es_inner_select_fixedbc", flush=True)
+    Z, Y, X = shape
+    f = F.taylor_green(14, noise=0.05, seed=7, shape=shape)
+    acc = [np.asarray(a, np.float32) for a in f["acc"]]
+    curl = O.curl(shape, tuple(float(np.float32(d)) for d in f["delta"]),
+                  [a.astype(np.float64) for a in acc])
+    thr = float(np.float32(np.quantile(curl, 0.85)))
+    p = np.random.default_rng(9).normal(size=shape).astype(np.float32)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    k = [dev(ref_dirs2[:, i]) for i in range(3)]
+    a = [dev(c) for c in acc]
+    cd, pd = dev(curl), dev(p)
+    pnd = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    pcd = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    dl = [float(np.float32(d)) for d in f["delta"]]
+    omni3d.check(omni3d.lib().omni3d_es_inner_select_fixedbc(
+        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(),
+        dl[0], dl[1], dl[2], float(np.float32(f["rho"])), a[0].data_ptr(), a[1].data_ptr(),
+        a[2].data_ptr(), cd.data_ptr(), pd.data_ptr(), pnd.data_ptr(), pcd.data_ptr(), thr, None))
+    torch.cuda.synchronize()
+    pn, cnt, _ = O.one_pass(shape, ref_dirs2, 1.0, dl, float(np.float32(f["rho"])),
+                            [x.astype(np.float64) for x in acc], p.astype(np.float64),
+                            curl=curl.astype(np.float32).astype(np.float64), threshold=thr,
+                            method=O.SELECT_FIXEDBC, faithful=1)
+    np.testing.assert_allclose(pcd.cpu().numpy(), cnt, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(pnd.cpu().numpy(), pn, rtol=1e-5,
+                               atol=1e-6 * np.abs(pn).max())
+    assert np.all(pcd.cpu().numpy()[:, 0, :] == 0)
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_minicurl(gpu_available, ref_dirs2, precision):
+    """MiniCurl (kernel.cu:2797-2936): against the oracle fed the curl the
+    GPU computed (its step rule compares curl values, so both sides must see
+    the same ones); counts exact."""
+    shape = (18, 20, 19)
+    print(f"case minicurl precision={precision}", flush=True)
+    f = F.taylor_green(20, noise=0.05, seed=9, shape=shape)
+    acc = [np.asarray(a, np.float32).astype(np.float64) if precision else a for a in f["acc"]]
+    m = sphere(shape, 4.0)
+    p0 = np.random.default_rng(10).normal(size=shape)
+    with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision,
+                        method=omni3d.MINICURL, max_iterations=5, seed_pn_with_p=0) as ctx:
+        ctx.set_directions(ref_dirs2)
+        ctx.set_field(acc, mask=m)
+        curl = ctx.field_curl()
+        p, w, rep = ctx.reconstruct(p0)
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc, p0=p0,
+                                    curl=curl, method=O.MINICURL, max_iter=5, seed_pn=0)
+    assert rep.iterations == it
+    np.testing.assert_array_equal(w, co)
+    err = F.rel_l2(p, po)
+    print("minicurl rel-L2 vs oracle", err)
+    assert err < (1e-12 if precision == omni3d.FP64 else 1e-5)
+
+
+def test_es_inner_weighted_minicurl_dropin(gpu_available, ref_dirs2):
+    import torch
+    shape = (12, 13, 14)
+    print("case es_inner_weighted_minicurl", flush=True)
+    Z, Y, X = shape
+    f = F.taylor_green(14, noise=0.05, seed=11, shape=shape)
+    acc = [np.asarray(a, np.float32) for a in f["acc"]]
+    dl = [float(np.float32(d)) for d in f["delta"]]
+    rho = float(np.float32(f["rho"]))
+    curl = O.curl(shape, dl, [a.astype(np.float64) for a in acc]).astype(np.float32)
+    p = np.random.default_rng(12).normal(size=shape).astype(np.float32)
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+    k = [dev(ref_dirs2[:, i]) for i in range(3)]
+    a = [dev(c) for c in acc]
+    cd, pd = dev(curl), dev(p)
+    pnd = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    pcd = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    omni3d.check(omni3d.lib().omni3d_es_inner_weighted_minicurl(
+        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(),
+        dl[0], dl[1], dl[2], rho, a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(),
+        cd.data_ptr(), pd.data_ptr(), pnd.data_ptr(), pcd.data_ptr(), None))
+    torch.cuda.synchronize()
+    pn, cnt, _ = O.one_pass(shape, ref_dirs2, 1.0, dl, rho, [x.astype(np.float64) for x in acc],
+                            p.astype(np.float64), curl=curl.astype(np.float64),
+                            method=O.MINICURL)
+    np.testing.assert_array_equal(pcd.cpu().numpy(), cnt)
+    np.testing.assert_allclose(pnd.cpu().numpy(), pn, rtol=1e-5, atol=1e-6 * np.abs(pn).max())
+
+
 @pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
 def test_weighted(gpu_available, ref_dirs2, precision):
     shape = (16, 17, 18)
