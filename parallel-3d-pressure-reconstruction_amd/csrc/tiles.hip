@@ -639,13 +639,23 @@ k_carry_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
   const unsigned int n = nseg[r];
   double run = 0.0;
   unsigned int k = 0;
-  // eight loads in flight per lane; the sum order is the segment order
-  for (; k + 8 <= n; k += 8) {
-    double x[8];
+  // sixteen loads in flight per lane; the sum order is the segment order
+  for (; k + 16 <= n; k += 16) {
+    double x[16];
 #pragma unroll
-    for (int u = 0; u < 8; u++) x[u] = segval[b + 64ull * (k + u)];
+    for (int u = 0; u < 16; u++) x[u] = segval[b + 64ull * (k + u)];
 #pragma unroll
-    for (int u = 0; u < 8; u++) {
+    for (int u = 0; u < 16; u++) {
+      segval[b + 64ull * (k + u)] = run;
+      run = run + x[u];
+    }
+  }
+  for (; k + 4 <= n; k += 4) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) x[u] = segval[b + 64ull * (k + u)];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
       segval[b + 64ull * (k + u)] = run;
       run = run + x[u];
     }
