@@ -172,7 +172,8 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
   Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   const int64_t n = pl.nrays;
-  if (n > (int64_t)UINT32_MAX - 1) return fail(ctx, OMNI3D_ERR_ARG, "too many rays");
+  // the sort and scans take an int item count
+  if (n > (int64_t)INT32_MAX - 1) return fail(ctx, OMNI3D_ERR_ARG, "too many rays");
   *npairs_out = 0;
   if (n == 0) return OMNI3D_OK;
   // work buffers stay with the context (freeing them costs more than the

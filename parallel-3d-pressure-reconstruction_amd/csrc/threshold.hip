@@ -187,8 +187,16 @@ int threshold_impl(int X, int Y, int Z, const float* curl, float percentage, flo
   // kernel.cu:8408: Imax*Jmax*Kmax*percentage, an int product times a float
   const float need = (float)(int)V * percentage;
   // the 40 KB of bins stay allocated per host thread (the call synchronises
-  // before returning, so the next call on this thread may reuse them)
+  // before returning, so the next call on this thread may reuse them); a
+  // call on another device than the last one allocates them there
   thread_local DevBuf scratch;
+  thread_local int scratch_dev = -1;
+  int dev = 0;
+  THR_TRY(hipGetDevice(&dev));
+  if (dev != scratch_dev) {
+    scratch.release();
+    scratch_dev = dev;
+  }
   DevBuf dcurl;
   THR_TRY(scratch.alloc(sizeof(unsigned int) * (2 + kBins) + sizeof(float)));
   unsigned int* mm = scratch.as<unsigned int>();
