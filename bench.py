@@ -15,8 +15,8 @@ Also measured in the same run: the FP64 GPU path on the same input (rel-L2
 of the FP32 result against it, speed ratio), the cost of one reference-
 schedule FP64 pass (a full traversal per iteration, as the reference
 iterates), the per-phase times from the library's HIP events on its stream,
-the roofline of the dominant traversal kernel (the slower of pass A's
-segment-sum traversal and the final full traversal), and the
+the roofline of the dominant kernel, the source iteration (with the two
+ray traversals beside it in the per-direction unit), and the
 oracle's single-thread CPU time on a bounded sample (cpu_baseline).
 
 Other modes (not the headline): --copy-calibration runs a plain 1 GiB device
@@ -578,25 +578,27 @@ def main():
     kern = {"local_sum": ("k_tile_pass<float, local-sum, 16^3 tiles + halo, 1024 threads> "
                           "(segment sums of pass A)", ms_s),
             "full": ("k_tile_pass<float, full, 16^3 tiles, 1024 threads> (final traversal)", ms_b)}
-    dom = max(kern, key=lambda k: kern[k][1])
-    name, ms_dom = kern[dom]
-    achieved = bytes_pass / (ms_dom * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom),
-            "kernel": name, "algorithmic_bytes": bytes_pass, "ms": round(ms_dom, 4),
-            "other_traversal": {
-                k: {"kernel": kern[k][0], "ms": round(kern[k][1], 4),
-                    "frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "traffic": traffic.get(k)} for k in kern if k != dom}}
-    # the source iteration (half of a reconstruction's time): per launch it
-    # streams one word and gathers one FP32 increment per source visit
+    # the source iteration is the dominant kernel by time (200 launches, half
+    # of a reconstruction): one launch must read the iteration matrix once,
+    # 4 B per word (equal entries of a row merged), and per source row its
+    # vectors: row sum, p_S, constant, diagonal, source index and visit count
+    # read, row sum, p_S and FP32 increment written, the increment gathered
+    # (60 B).  Its duration is the iteration phase's HIP events / launches.
     it_ms = float(np.mean([x.ms_iterate for x in reps])) / max(1, reps[-1].iterations)
-    it_bytes = reps[-1].source_visits * 8
-    roof["iteration"] = {"kernel": "k_iterate_grp<4, increments> (one source iteration)",
-                         "ms_per_launch": round(it_ms, 5), "algorithmic_bytes": it_bytes,
-                         "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
-                         "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                         "traffic": traffic.get("iteration")}
+    it_bytes = reps[-1].matrix_words * 4 + reps[-1].sources * 60
+    it_ach = it_bytes / (it_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(it_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(it_ach / HBM_PEAK_GBS, 4), "traffic": traffic.get("iteration"),
+            "kernel": "k_iterate_grp<4, increments> (one source iteration)",
+            "algorithmic_bytes": it_bytes, "ms": round(it_ms, 5),
+            "matrix_words": reps[-1].matrix_words, "source_visits": reps[-1].source_visits,
+            # the two ray traversals, in the per-direction unit D*V*12 + 20V
+            "traversals": {
+                k: {"kernel": kern[k][0], "ms": round(kern[k][1], 4),
+                    "algorithmic_bytes": bytes_pass,
+                    "achieved": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9, 1),
+                    "frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "traffic": traffic.get(k)} for k in kern}}
 
     cpu = None
     if not args.no_cpu:

@@ -118,6 +118,8 @@ typedef struct omni3d_report {
   int64_t segments;        /* ray segments of the tile engine (one per ray and tile) */
   int64_t sell_cells;      /* stored cells of the iteration matrix incl. padding */
   double ms_segment_sums;  /* the segment-sum traversal of pass A alone (tile engine) */
+  int64_t matrix_words;    /* 4-byte words of the iteration matrix one iteration reads
+                              (equal entries of a row merged) */
 } omni3d_report;
 
 typedef struct omni3d_ctx omni3d_ctx;

@@ -1340,6 +1340,7 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->source_visits = ctx->plan->nnz;
     rep->segments = ctx->plan->nsegs;
     rep->sell_cells = ctx->plan->ncell;
+    rep->matrix_words = ctx->plan->nwords;
     rep->ms_segment_sums = ms(2, 8);
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");

@@ -430,6 +430,7 @@ int omni3d_batch_result(omni3d_batch* b, int index, void* p, int dtype, int mem_
     rep->source_visits = c->plan->nnz;
     rep->segments = c->plan->nsegs;
     rep->sell_cells = c->plan->ncell;
+    rep->matrix_words = c->plan->nwords;
   }
   return OMNI3D_OK;
 }

@@ -67,7 +67,7 @@ class Report(C.Structure):
         ("ms_plan", C.c_double), ("ms_solve", C.c_double), ("ms_pass_a", C.c_double),
         ("ms_iterate", C.c_double), ("ms_pass_b", C.c_double), ("ms_finalize", C.c_double),
         ("source_visits", C.c_int64), ("segments", C.c_int64), ("sell_cells", C.c_int64),
-        ("ms_segment_sums", C.c_double),
+        ("ms_segment_sums", C.c_double), ("matrix_words", C.c_int64),
     ]
 
 
