@@ -424,7 +424,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-dirs", type=int, default=12)
+    ap.add_argument("--cpu-dirs", type=int, default=18)
     ap.add_argument("--naive-passes", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", action="store_true")
