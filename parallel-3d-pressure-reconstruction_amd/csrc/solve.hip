@@ -632,12 +632,12 @@ __device__ __forceinline__ void iterate_grp_body(
         else x[u] = pold[w[u] & kSrcMask];
       }
 #pragma unroll
-      for (int u = 0; u < kU; u++)
-        if (ok[u]) {
-          // a word stands for 1..4 equal entries (the same source in one row)
-          const unsigned long long m = ((w[u] >> 22) & 3u) + 1u;
-          atomicAdd(&s_acc[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale) * m);
-        }
+      for (int u = 0; u < kU; u++) {
+        // a word stands for 1..4 equal entries (the same source in one row);
+        // lanes outside the group's words add 0 (no branch around the add)
+        const unsigned long long m = ok[u] ? ((w[u] >> 22) & 3u) + 1u : 0u;
+        atomicAdd(&s_acc[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale) * m);
+      }
       __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
