@@ -165,10 +165,14 @@ def methods_mode(args):
     passes = args.naive_passes
     out = {}
     thr = None
+    # the two reference variants walk one lane per ray (their step rules
+    # leave the plan's staircase)
     for name, method in (("plain", omni3d.PLAIN), ("plain-mask", omni3d.PLAIN_MASK),
-                         ("select", omni3d.SELECT), ("weighted", omni3d.WEIGHTED)):
+                         ("select", omni3d.SELECT), ("weighted", omni3d.WEIGHTED),
+                         ("select-fixedbc", omni3d.SELECT_FIXEDBC),
+                         ("minicurl", omni3d.MINICURL)):
         kw = {}
-        if method == omni3d.SELECT:
+        if method in (omni3d.SELECT, omni3d.SELECT_FIXEDBC):
             kw["threshold"] = thr
         ctx = omni3d.Context(shape, f["delta"], density=f["rho"], precision=omni3d.FP32_KAHAN,
                              method=method, solver=omni3d.NAIVE, max_iterations=passes,
