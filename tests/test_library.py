@@ -61,6 +61,28 @@ def test_bad_arguments_rejected_before_any_device_work():
     with pytest.raises(omni3d.Omni3DError) as e:
         omni3d.Context((8, 8, 8), line_spacing=0.0)
     assert e.value.code == omni3d.ERR_ARG
+    with pytest.raises(omni3d.Omni3DError) as e:
+        omni3d.Context((8, 8, 8), method=omni3d.MINICURL + 1)
+    assert e.value.code == omni3d.ERR_ARG
+    L = omni3d.lib()
+    out = C.c_float(0)
+    # threshold histogram: grid and pointers checked first
+    assert L.omni3d_threshold_hist(0, 4, 4, None, 0.5, C.c_void_p(C.addressof(out)),
+                                   omni3d.HOST, None) == omni3d.ERR_ARG
+    buf = np.zeros(8, np.float32)
+    assert L.omni3d_threshold_hist(2, 2, 2, C.c_void_p(buf.ctypes.data), 0.5, None,
+                                   omni3d.HOST, None) == omni3d.ERR_ARG
+    assert L.omni3d_threshold_hist(2, 2, 2, C.c_void_p(buf.ctypes.data), 0.5,
+                                   C.c_void_p(C.addressof(out)), 7, None) == omni3d.ERR_ARG
+    # boundary pairs: a context is required
+    assert L.omni3d_boundary_pairs(None, 4, C.c_void_p(buf.ctypes.data), omni3d.F32,
+                                   omni3d.HOST, None) == omni3d.ERR_ARG
+    # the variant launchers check their pointers before any device work
+    es = [8, 8, 8, 1, 1.0, None, None, None, 1.0, 1.0, 1.0, 1.0, None, None, None]
+    assert L.omni3d_es_inner_select_fixedbc(*es, None, None, None, None, 0.5,
+                                            None) == omni3d.ERR_ARG
+    assert L.omni3d_es_inner_weighted_minicurl(*es, None, None, None, None,
+                                               None) == omni3d.ERR_ARG
 
 
 @pytest.mark.parametrize("divide", [1, 2, 3, 4, 5])
