@@ -110,3 +110,25 @@ def test_readme_pipeline_inner_pass(gpu_available, ref_dirs2):
     # reconstructs the field
     d = (p - p.mean()) - (f["p"] - f["p"].mean())
     assert np.linalg.norm(d) / np.linalg.norm(f["p"] - f["p"].mean()) < 0.2
+
+
+def test_device_f32_buffers(gpu_available, ref_dirs2):
+    """The device-buffer, FP32 form of the call gives the host FP64 result
+    rounded to FP32."""
+    import ctypes as C
+    import torch
+    shape = (12, 11, 10)
+    print("case boundary pairs device f32", flush=True)
+    f = F.taylor_green(12, noise=0.05, seed=13, shape=shape)
+    p0 = np.random.default_rng(14).normal(size=shape).astype(np.float32)
+    with omni3d.Context(shape, f["delta"], density=f["rho"]) as ctx:
+        ctx.set_directions(ref_dirs2)
+        ctx.set_field(f["acc"])
+        want, nw = ctx.boundary_pairs(p0.astype(np.float64), 3)
+        pd = torch.from_numpy(p0).cuda()
+        n = C.c_longlong(0)
+        omni3d.check(omni3d.lib().omni3d_boundary_pairs(ctx._h, 3, C.c_void_p(pd.data_ptr()),
+                                                        omni3d.F32, omni3d.DEVICE, C.byref(n)))
+        got = pd.cpu().numpy()
+    assert n.value == nw
+    np.testing.assert_array_equal(got, want.astype(np.float32))
