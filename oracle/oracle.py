@@ -32,6 +32,14 @@ def lib():
     return _lib
 
 
+def load(path):
+    """A separately built copy of the oracle (e.g. a mutant the tests compile
+    to show that a check fails on a one-line change), bound like lib()."""
+    L = C.CDLL(path)
+    _declare(L)
+    return L
+
+
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
@@ -158,8 +166,9 @@ def ray_path(shape, kvec, ls, point, capacity=1 << 16, fixedbc=0):
 
 
 def one_pass(shape, dirs, ls, delta, rho, acc, p, curl=None, threshold=0.0, method=PLAIN,
-             faithful=1, pn=None, cnt=None):
-    """One full pass; returns (pn, cnt, src_mark) with pn/cnt accumulated in place."""
+             faithful=1, pn=None, cnt=None, L=None):
+    """One full pass; returns (pn, cnt, src_mark) with pn/cnt accumulated in place.
+    L: a library from load() instead of the default build."""
     Z, Y, X = shape
     k1, k2, k3, D = _dirs(dirs)
     ax, ay, az = (_f64(a) for a in acc)
@@ -168,10 +177,10 @@ def one_pass(shape, dirs, ls, delta, rho, acc, p, curl=None, threshold=0.0, meth
     cnt = np.zeros(shape) if cnt is None else cnt
     src = np.zeros(shape, dtype=np.uint8)
     cu = _f64(curl) if curl is not None else None
-    lib().orc_pass(X, Y, Z, D, _ptr(k1, _fp), _ptr(k2, _fp), _ptr(k3, _fp), ls, delta[0],
-                   delta[1], delta[2], rho, _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp),
-                   _ptr(cu, _dp), threshold, method, faithful, _ptr(p, _dp), _ptr(pn, _dp),
-                   _ptr(cnt, _dp), _ptr(src, _u8p))
+    (L or lib()).orc_pass(X, Y, Z, D, _ptr(k1, _fp), _ptr(k2, _fp), _ptr(k3, _fp), ls,
+                          delta[0], delta[1], delta[2], rho, _ptr(ax, _dp), _ptr(ay, _dp),
+                          _ptr(az, _dp), _ptr(cu, _dp), threshold, method, faithful,
+                          _ptr(p, _dp), _ptr(pn, _dp), _ptr(cnt, _dp), _ptr(src, _u8p))
     return pn, cnt, src
 
 
