@@ -333,6 +333,98 @@ static int walk_plain(const pass_ctx* P, walker* w) {
   return steps;
 }
 
+/* Trilinear sample of the acceleration at the index-space point x (0-based
+ * voxel centres).  The cell's lower corner is floor(x) clamped to
+ * [0, size - 2] and the weights are not clamped, so a point up to one voxel
+ * outside the centres of the faces is extrapolated linearly: the sample of a
+ * linear field is exact everywhere.  Interpolation order x, then y, then z. */
+static void tri_sample(const pass_ctx* P, const double x[3], double out[3]) {
+  const int size[3] = {P->X, P->Y, P->Z};
+  int i0[3];
+  double f[3], g[3];
+  for (int a = 0; a < 3; a++) {
+    int fl = (int)floor(x[a]);
+    if (fl > size[a] - 2) fl = size[a] - 2;
+    if (fl < 0) fl = 0;
+    i0[a] = fl;
+    f[a] = x[a] - (double)fl;
+    g[a] = 1.0 - f[a];
+  }
+  const int64_t b = vidx(P->X, P->Y, i0[0], i0[1], i0[2]);
+  const int64_t sy = P->X, sz = (int64_t)P->X * P->Y;
+  for (int c = 0; c < 3; c++) {
+    const double* A = P->acc[c];
+    const double c00 = A[b] * g[0] + A[b + 1] * f[0];
+    const double c10 = A[b + sy] * g[0] + A[b + sy + 1] * f[0];
+    const double c01 = A[b + sz] * g[0] + A[b + sz + 1] * f[0];
+    const double c11 = A[b + sy + sz] * g[0] + A[b + sy + sz + 1] * f[0];
+    const double c0 = c00 * g[1] + c10 * f[1];
+    const double c1 = c01 * g[1] + c11 * f[1];
+    out[c] = c0 * g[2] + c1 * f[2];
+  }
+}
+
+/* Trapezoid of grad p = -rho a over a straight path: -rho/2 (a_start +
+ * a_end) . (disp * delta), disp in index units. */
+static inline double tri_term(const pass_ctx* P, const double ga[3], const double gb[3],
+                              const double disp[3]) {
+  const double hr = -P->rho * 0.5;
+  return hr * (((ga[0] + gb[0]) * (disp[0] * P->del[0]) + (ga[1] + gb[1]) * (disp[1] * P->del[1])) +
+               (ga[2] + gb[2]) * (disp[2] * P->del[2]));
+}
+
+/* The plain method (kernel.cu:1743-1863: entry self-add, the staircase's
+ * visited voxels, p_in + integral at each visit) with the integral taken
+ * along the straight ray instead of the staircase.  Each voxel centre c
+ * projects onto the line at x = c - u (u: its perpendicular offset, which the
+ * walker tracks for the step rule).  The integral from the entry voxel to a
+ * visited voxel v follows in -> x_in (leg), x_in -> x_v along the line, one
+ * trapezoid per staircase step (the projection advances by s k_a along k),
+ * and x_v -> v (leg); the gradient at the points on the line is sampled
+ * trilinearly, at voxel centres it is the grid value.  Every piece is exact
+ * for a linear gradient, so a quadratic pressure is reproduced exactly. */
+static int walk_plain_trilinear(const pass_ctx* P, walker* w) {
+  const int X = P->X, Y = P->Y;
+  const double* k = w->d->k;
+  int64_t vin = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+  double pin = P->p[vin];
+  if (P->src) P->src[vin] = 1;
+  P->pn[vin] += pin;
+  P->cnt[vin] += 1.0;
+  double x[3], sl[3], ac[3], disp[3];
+  for (int a = 0; a < 3; a++) {
+    x[a] = (double)w->c[a] - w->u[a];
+    ac[a] = P->acc[a][vin];
+    disp[a] = -w->u[a];
+  }
+  tri_sample(P, x, sl);
+  double pint = tri_term(P, ac, sl, disp); /* leg in -> x_in */
+  int steps = 0;
+  while (!walker_done(w)) {
+    int s, a = walker_choose(w, &s);
+    if (a < 0) break;
+    walker_step(w, a, s);
+    int64_t vn = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+    double sn[3];
+    for (int b = 0; b < 3; b++) {
+      x[b] = (double)w->c[b] - w->u[b];
+      disp[b] = (double)s * (k[a] * k[b]);
+    }
+    tri_sample(P, x, sn);
+    pint += tri_term(P, sl, sn, disp); /* x_l -> x_n along the line */
+    for (int b = 0; b < 3; b++) {
+      ac[b] = P->acc[b][vn];
+      disp[b] = w->u[b];
+    }
+    const double leg = tri_term(P, sn, ac, disp); /* x_n -> v */
+    P->pn[vn] += pin + (pint + leg);
+    P->cnt[vn] += 1.0;
+    for (int b = 0; b < 3; b++) sl[b] = sn[b];
+    steps++;
+  }
+  return steps;
+}
+
 /* bodyIntegralInnerMiniCurl, kernel.cu:2797-2936 (the pass of
  * omni3dparallellinesESInnerWeightedMiniCurl, kernel.cu:6347-6413): the
  * plain sums and unit counts without the entry self-add, along a path that
@@ -510,6 +602,7 @@ void orc_pass(int X, int Y, int Z, int D, const float* k1, const float* k2, cons
           walk_select(&P, &w, 0);
           break;
         case ORC_MINICURL: walk_minicurl(&P, &w); break;
+        case ORC_PLAIN_TRILINEAR: walk_plain_trilinear(&P, &w); break;
         default: break;
       }
     }
@@ -792,6 +885,53 @@ void orc_sor_redblack(int X, int Y, int Z, int iters, double omega, double* p, c
  * reference walks the bins with j++ (kernel.cu:8406) and puts the maximum in
  * bin N, both out of bounds; this restates the intended walk (j--) with the
  * maximum in the last bin. */
+/* initialIntegration, initialIntegration.cpp:9-408.  p[0] = pn[0] = pref
+ * (lines 11-12); then eight sweeps, one per octant around the centre voxel
+ * (X/2, Y/2, Z/2), in the file's order (No.1 .. No.8: x, then y, then z
+ * toward the low side first), each with k outermost, then j, then i, every
+ * index running from the centre toward its face.  A voxel gets the mean of
+ * the one-step trapezoid predictions from its neighbours toward the centre
+ * along the axes on which it is off the centre, added in the order x, y, z:
+ *   neighbour on the high side:  pn[nb] + 0.5 * (a[v] + a[nb]) * delta * rho
+ *   neighbour on the low side:   pn[nb] - 0.5 * (a[v] + a[nb]) * delta * rho
+ * (the first assigned, the others added with +=, then / 2.0 or / 3.0), and
+ * p[v] = pn[v].  The centre keeps the caller's pn.  Voxels on a centre plane
+ * lie in two or four octants and are recomputed with the same expression. */
+void orc_initial_integration(int X, int Y, int Z, double dx, double dy, double dz, double rho,
+                             const double* ax, const double* ay, const double* az, double pref,
+                             double* p, double* pn) {
+  p[0] = pref;
+  pn[0] = pref;
+  const int ctr[3] = {X / 2, Y / 2, Z / 2};
+  const int size[3] = {X, Y, Z};
+  const int64_t stride[3] = {1, X, (int64_t)X * Y};
+  const double del[3] = {dx, dy, dz};
+  const double* acc[3] = {ax, ay, az};
+  for (int oc = 0; oc < 8; oc++) {
+    const int sg[3] = {(oc & 1) ? 1 : -1, (oc & 2) ? 1 : -1, (oc & 4) ? 1 : -1};
+    for (int k = ctr[2]; k >= 0 && k < size[2]; k += sg[2])
+      for (int j = ctr[1]; j >= 0 && j < size[1]; j += sg[1])
+        for (int i = ctr[0]; i >= 0 && i < size[0]; i += sg[0]) {
+          const int c[3] = {i, j, k};
+          const int64_t v = vidx(X, Y, i, j, k);
+          int n = 0;
+          double val = 0.0;
+          for (int a = 0; a < 3; a++) {
+            if (c[a] == ctr[a]) continue;
+            const int64_t nb = v - (int64_t)sg[a] * stride[a];
+            const double t = 0.5 * (acc[a][v] + acc[a][nb]) * del[a] * rho;
+            const double pr = sg[a] < 0 ? pn[nb] + t : pn[nb] - t;
+            val = n == 0 ? pr : val + pr;
+            n++;
+          }
+          if (n == 2) val = val / 2.0;
+          if (n == 3) val = val / 3.0;
+          if (n > 0) pn[v] = val;
+          p[v] = pn[v];
+        }
+  }
+}
+
 float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentage) {
   const int64_t V = (int64_t)X * Y * Z;
   float minv = 1e4f, maxv = -1e4f;

@@ -20,7 +20,12 @@ extern "C" {
 
 enum { ORC_PLAIN = 0, ORC_PLAIN_MASK = 1, ORC_SELECT = 2, ORC_WEIGHTED = 3,
        ORC_SELECT_FIXEDBC = 4, /* bodyIntegralInnerSelectFixedBC, kernel.cu:2538-2796 */
-       ORC_MINICURL = 5 /* bodyIntegralInnerMiniCurl, kernel.cu:2797-2936 */ };
+       ORC_MINICURL = 5, /* bodyIntegralInnerMiniCurl, kernel.cu:2797-2936 */
+       /* The plain method's visits and sources with the line integral taken
+        * along the straight ray through trilinear samples of the gradient
+        * (no reference counterpart: the accuracy mode beside the staircase
+        * of kernel.cu:1743-1863; see walk_plain_trilinear). */
+       ORC_PLAIN_TRILINEAR = 6 };
 
 /* Per-direction constants derived from one unit vector. */
 typedef struct {
@@ -112,6 +117,12 @@ int64_t orc_boundary_pairs(int X, int Y, int Z, int D, const float* k1, const fl
                            const float* k3, double ls, double dx, double dy, double dz,
                            double rho, const double* ax, const double* ay, const double* az,
                            int iters, double* p);
+
+/* initialIntegration, initialIntegration.cpp:9-408 (FP64, host buffers;
+ * pBC is unused there and omitted here). */
+void orc_initial_integration(int X, int Y, int Z, double dx, double dy, double dz, double rho,
+                             const double* ax, const double* ay, const double* az, double pref,
+                             double* p, double* pn);
 
 /* kernel.cu:8380-8418 (intended bin walk, see the definition) */
 float orc_threshold_hist(int X, int Y, int Z, const float* curl, float percentage);

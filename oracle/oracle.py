@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
+# the plain method integrated along the straight ray (trilinear samples)
+PLAIN_TRILINEAR = 6
 
 _lib = None
 
@@ -94,6 +96,8 @@ def _declare(L):
     L.orc_boundary_pairs.restype = i64
     L.orc_residual.argtypes = [i, i, i, _dp, _dp]
     L.orc_residual.restype = d
+    L.orc_initial_integration.argtypes = [i, i, i, d, d, d, d, _dp, _dp, _dp, d, _dp, _dp]
+    L.orc_initial_integration.restype = None
 
 
 def _f64(a):
@@ -264,3 +268,16 @@ def threshold_hist(curl, percentage):
 def residual(p, rhs):
     Z, Y, X = p.shape
     return lib().orc_residual(X, Y, Z, _ptr(_f64(p), _dp), _ptr(_f64(rhs), _dp))
+
+
+def initial_integration(shape, delta, rho, acc, pref, pn=None, L=None):
+    """initialIntegration.cpp:9-408; returns (p, pn).  pn: the caller's
+    starting pn (its centre voxel is kept), zeros if None."""
+    Z, Y, X = shape
+    ax, ay, az = (_f64(a) for a in acc)
+    pn = np.zeros(shape) if pn is None else _f64(pn).copy()
+    p = np.zeros(shape)
+    (L or lib()).orc_initial_integration(X, Y, Z, delta[0], delta[1], delta[2], rho,
+                                         _ptr(ax, _dp), _ptr(ay, _dp), _ptr(az, _dp), pref,
+                                         _ptr(p, _dp), _ptr(pn, _dp))
+    return p, pn
