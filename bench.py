@@ -3,9 +3,10 @@
 Workload: swirl-only Burgers vortex pressure on a 256^3 grid (L = 1, core
 radius 0.1 L, axis z through the centre, Gamma = 1) with 2 % Gaussian noise
 on the gradient; 162 icosahedral directions (twice subdivided), line spacing
-1 voxel; plain method; ray sums in FP32 with Kahan compensation, averaging,
-source update and residual in FP64; stop at a relative change < 1e-6 of the
-source voxels or 200 iterations.  The field is resident on the GPU before
+1 voxel; plain method; gradient and trapezoid terms in FP32, added into FP64
+ray sums by the tile engine (the compensated FP32 mode: OMNI3D_FP32_KAHAN),
+averaging, source update and residual in FP64; stop at a relative change
+< 1e-6 of the source voxels or 200 iterations.  The field is resident on the GPU before
 the timed region; a step is one reconstruction of that snapshot with the
 geometry plan (rays, visit counts, source CSR, tile segments) of the grid
 already built, as in the reference's loop over the snapshots of one grid.
@@ -16,8 +17,9 @@ of the FP32 result against it, speed ratio), the cost of one reference-
 schedule FP64 pass (a full traversal per iteration, as the reference
 iterates), the per-phase times from the library's HIP events on its stream,
 the roofline of the dominant kernel, the source iteration (with the two
-ray traversals beside it in the per-direction unit), and the
-oracle's single-thread CPU time on a bounded sample (cpu_baseline).
+ray traversals beside it), the trilinear traversal's accuracy against the
+analytic pressure beside the staircase's, and the oracle's single-thread CPU
+time on a bounded sample (cpu_baseline; --cpu-full times a whole pass).
 
 Other modes (not the headline): --copy-calibration runs a plain 1 GiB device
 copy for scaling the FETCH_SIZE / WRITE_SIZE counters; --batch times the
@@ -46,6 +48,30 @@ sys.path.insert(0, os.path.join(ROOT, "parallel-3d-pressure-reconstruction_amd")
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tile_passes_burgers256.json")
+
+
+def host_info():
+    """CPU model, threads visible and the compiler the oracle is built with."""
+    import platform
+    import subprocess
+    model = platform.processor() or "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cc = subprocess.run(["gcc", "--version"], capture_output=True, text=True,
+                            timeout=20).stdout.splitlines()[0]
+    except Exception:
+        cc = "unknown"
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncpu = os.cpu_count()
+    return {"cpu_model": model, "nproc": ncpu, "compiler": f"{cc} -O2 -ffp-contract=off"}
 
 
 def log(msg):
@@ -95,7 +121,8 @@ def batch_mode(args):
         "metric": "snapshots per second, 64 uneven snapshots (one hipGraph)",
         "value": round(64 / (ms_graph * 1e-3), 3), "unit": "snapshots/s", "n_gpus": 1,
         "steps": args.steps, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
-        "dtype": "fp32-kahan ray sums, fp64 averaging", "data": "synthetic",
+        "dtype": "f32 gradient and trapezoid terms, f64 ray sums and averaging",
+        "data": "synthetic",
         "config": {"workload": "taylor-green series, 64 snapshots from {64^3, 96^3, 128x96x64, "
                                "160x128x96}, 0-5% noise, 162 directions, plain affine",
                    "voxels": vox, "tile_choices": picks, "tune_s": round(tune_s, 3),
@@ -284,7 +311,26 @@ def sor_mode(args):
         ts = time.perf_counter() - t0
         cpu = {"value": round(ts / k * 1e3, 2), "unit": "ms per iteration", "cores": 1,
                "kind": "port",
-               "sample": f"oracle serial red-black SOR, {k} iterations at {n}^3 took {ts:.2f}s"}
+               "sample": f"oracle serial red-black SOR, {k} iterations at {n}^3 took {ts:.2f}s",
+               **host_info()}
+        # the reference's own CPU Poisson solver, sor3D.cpp: damped Jacobi
+        # (weight 0.8, sor3D.cpp:7,739) with its 26 boundary stencils
+        kj = 4
+        t0 = time.perf_counter()
+        O.sor3d(np.zeros(shape), rhs, kj, 0.0)
+        tj = time.perf_counter() - t0
+        cpu["reference_jacobi_w0.8"] = {
+            "value": round(tj / kj * 1e3, 2), "unit": "ms per sweep", "cores": 1,
+            "sample": f"oracle restatement of sor3D (damped Jacobi, weight 0.8), {kj} sweeps "
+                      f"at {n}^3 took {tj:.2f}s"}
+    # sor3D's drop-in on the GPU (same signature, host buffers: the PCIe copies
+    # of p, pn and the right-hand side are included)
+    kg = 40
+    t0 = time.perf_counter()
+    _, _, sw, _ = omni3d.sor3d(np.zeros(shape), rhs, kg, 0.0)
+    tg = time.perf_counter() - t0
+    jacobi_gpu = {"sweeps": sw, "s_total_host_buffers": round(tg, 4),
+                  "note": "sor3D signature on the GPU: damped Jacobi, weight 0.8, host buffers"}
     print(json.dumps({
         "metric": "ms per red-black SOR iteration, 256^3", "value": round(ms_it, 4), "unit": "ms",
         "n_gpus": 1, "steps": iters, "higher_is_better": False, "dtype": "f64",
@@ -293,7 +339,7 @@ def sor_mode(args):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_redblack_fused<64, 8, 32, 256> (red and black in one pass)",
                      "algorithmic_bytes": bytes_it},
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "gpu_sor3D_dropin": jacobi_gpu,
         "config": {"workload": f"Neumann right-hand side of the burgers-vortex {n}^3 field, "
                                f"omega 1.8, {iters} iterations from p = 0",
                    "residual": res, "iterations": it}}))
@@ -429,6 +475,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-dirs", type=int, default=18)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the oracle's whole pass (every direction) instead of a sample")
+    ap.add_argument("--no-trilinear", action="store_true")
     ap.add_argument("--naive-passes", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", action="store_true")
@@ -565,6 +614,27 @@ def main():
     p32h = p32.cpu().numpy()
     err_analytic = F.rel_l2(F.demean(p32h), F.demean(pa))
 
+    # the trilinear traversal (straight-ray integral, the accuracy mode) on the
+    # same input and schedule, beside the staircase's error
+    tri = {}
+    if not args.no_trilinear:
+        log("bench: trilinear traversal")
+        ctxt = make_ctx(omni3d.FP32_KAHAN, traversal=omni3d.TRILINEAR)
+        step(ctxt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rt = step(ctxt)
+        torch.cuda.synchronize()
+        st_tri = time.perf_counter() - t0
+        ptri = p_dev.cpu().numpy()
+        ctxt.close()
+        tri = {"s_per_recon": round(st_tri, 4), "iterations": rt.iterations,
+               "ms_pass_a": round(rt.ms_pass_a, 3), "ms_pass_b": round(rt.ms_pass_b, 3),
+               "rel_l2_vs_analytic_demeaned": F.rel_l2(F.demean(ptri), F.demean(pa)),
+               "staircase_rel_l2_vs_analytic_demeaned": err_analytic}
+        log(f"bench: trilinear {st_tri * 1e3:.1f} ms, rel-L2 vs analytic "
+            f"{tri['rel_l2_vs_analytic_demeaned']:.3e} (staircase {err_analytic:.3e})")
+
     # roofline of the dominant traversal kernel: the slower of the segment-sum
     # traversal of pass A (k_tile_pass, local-sum mode) and the final full
     # traversal (k_tile_pass, full mode), each one ray pass.  A ray pass must
@@ -608,26 +678,32 @@ def main():
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        sub = dirs[:: max(1, D // args.cpu_dirs)][: args.cpu_dirs]
-        log(f"bench: CPU oracle sample, {len(sub)} of {D} directions, one pass at {n}^3")
+        sub = dirs if args.cpu_full else dirs[:: max(1, D // args.cpu_dirs)][: args.cpu_dirs]
+        log(f"bench: CPU oracle, {len(sub)} of {D} directions, one pass at {n}^3")
         t0 = time.perf_counter()
         O.one_pass(shape, sub, 1.0, f["delta"], f["rho"], f["acc"], np.zeros(shape))
         ts = time.perf_counter() - t0
         per_pass = ts * D / len(sub)
         iters = reps[-1].iterations
+        what = (f"oracle FP64 serial pass over all {D} directions at {n}^3 took {ts:.2f}s"
+                if len(sub) == D else
+                f"oracle FP64 serial pass over {len(sub)} of {D} directions (every "
+                f"{D // len(sub)}th) at {n}^3 took {ts:.2f}s; scaled x{D / len(sub):.1f} to one "
+                f"pass ({per_pass:.1f}s)")
         cpu = {"value": round(per_pass * iters, 2), "unit": "s per reconstruction", "cores": 1,
                "kind": "port",
-               "sample": f"oracle FP64 serial pass over {len(sub)} of {D} directions at {n}^3 "
-                         f"took {ts:.2f}s; scaled x{D / len(sub):.1f} to one pass "
-                         f"({per_pass:.1f}s) and x{iters} passes (the reference's CPU loop "
-                         f"runs one full pass per iteration)"}
+               "sample": what + f", x{iters} passes (the reference's CPU loop runs one full pass "
+                                f"per iteration)",
+               "measured_s": round(ts, 3), "s_per_pass": round(per_pass, 3),
+               "directions_timed": len(sub), **host_info()}
 
     out = {
         "metric": "s per 256^3 reconstruction at rel-L2<=1e-3 vs FP64 oracle; % HBM roofline",
         "value": round(s_per, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(s_per * 1e3, 4),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "fp32-kahan ray sums, fp64 averaging", "data": "synthetic",
+        "dtype": "f32 gradient and trapezoid terms, f64 ray sums and averaging",
+        "data": "synthetic",
         "config": {"workload": f"burgers-vortex {n}^3, 2% noise, {D} icosahedral directions, "
                                f"line spacing 1, plain method, affine solver, tol {args.tol:g} / "
                                f"{args.max_iter} iterations",
@@ -643,6 +719,17 @@ def main():
                    "ms_pass_b": round(ms_b, 4),
                    "ms_finalize": round(float(np.mean([x.ms_finalize for x in reps])), 4),
                    "rel_l2_vs_analytic_demeaned": err_analytic, **fp64},
+        # one snapshot on a fresh context: plan build + first reconstruction
+        "single_snapshot_s": round(cold_s, 4),
+        "fp32_vs_fp64_target": {
+            "target": 5.0, "measured": fp64.get("speedup_vs_fp64"),
+            "met": bool(fp64) and fp64.get("speedup_vs_fp64", 0) >= 5.0,
+            "note": "both precisions run the same affine schedule: the FP64 source iteration "
+                    "(fixed-point row sums, the dominant kernel) is shared, and the tile "
+                    "engine's traversals are bound by LDS and instruction issue, not by FP64 "
+                    "arithmetic; against the reference's schedule in FP64 (a full traversal "
+                    "per iteration) the ratio is speedup_vs_fp64_reference_schedule"},
+        "trilinear": tri,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

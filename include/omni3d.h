@@ -12,12 +12,14 @@
  *   kernel.cu:8467-8983  main()                      -> omni3d_run_parameter_file
  *   kernel.cu:8622-8648  direction-file reader       -> omni3d_load_directions_file
  *   kernel.cu:8805-8861  iteration driver            -> omni3d_reconstruct
- *   kernel.cu:6151       omni3dparallellinesESInner  -> omni3d_es_inner
- *   kernel.cu:6281       ...ESInnerWeighted          -> omni3d_es_inner_weighted
- *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select
- *   kernel.cu:6481       ...ESInnerSelectFixedBC     -> omni3d_es_inner_select_fixedbc
- *   kernel.cu:6347       ...ESInnerWeightedMiniCurl  -> omni3d_es_inner_weighted_minicurl
- *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner
+ *   kernel.cu:6151       omni3dparallellinesESInner  -> omni3d_es_inner, and the
+ *                        same name as a host function (call site minus <<<...>>>)
+ *   kernel.cu:6281       ...ESInnerWeighted          -> omni3d_es_inner_weighted (+ same name)
+ *   kernel.cu:6414       ...ESInnerSelect            -> omni3d_es_inner_select (+ same name)
+ *   kernel.cu:6481       ...ESInnerSelectFixedBC     -> omni3d_es_inner_select_fixedbc (+ same name)
+ *   kernel.cu:6347       ...ESInnerWeightedMiniCurl  -> omni3d_es_inner_weighted_minicurl (+ same name)
+ *   kernel.cu:6746       devidecountInner            -> omni3d_devide_count_inner (+ same name)
+ *   kernel.cu:8806-8811  Bernoulli seed of the j = 0 face -> omni3d_set_velocity
  *   kernel.cu:8352       calCurlofMaterialAccCPU     -> omni3d_curl
  *   kernel.cu:6863       calCurlofMaterialAcc (GPU)  -> omni3d_curl, mask NULL
  *   kernel.cu:8380       thredholdHistMaterialAccCPU -> thredholdHistMaterialAccCPU
@@ -26,6 +28,7 @@
  *   calRHS.h:4           calRHS                      -> calRHS (same signature; the
  *                        result is written to the caller's RHS buffer, which
  *                        the reference loses, calRHS.cpp:10)
+ *   initialIntegration.h:3 initialIntegration        -> initialIntegration (same signature)
  *   kernel.cu:3414,5572  EqualSpacing + BCiteration -> omni3d_boundary_pairs
  *                        (the README's boundary-pair pipeline, commented out
  *                        in main, kernel.cu:8830-8836)
@@ -72,8 +75,24 @@ extern "C" {
 
 /* Arithmetic of the ray sums. */
 #define OMNI3D_FP64 0        /* gradient and ray sums in FP64 */
-#define OMNI3D_FP32_KAHAN 1  /* gradient in FP32, Kahan-compensated FP32 ray sums;
-                                averaging, boundary update, residual in FP64 */
+/* Gradient and trapezoid terms in FP32, ray sums compensated: the tile engine
+ * (the default traversals) adds the FP32 terms into FP64 accumulators, the
+ * one-lane-per-ray walks (reference schedule, trilinear traversal, the
+ * cross-checks) into Kahan-compensated FP32 sums.  Averaging, the source
+ * iteration's row sums, the boundary update and the residual stay FP64. */
+#define OMNI3D_FP32_KAHAN 1
+#define OMNI3D_FP32 OMNI3D_FP32_KAHAN
+
+/* How a ray's line integral is formed. */
+#define OMNI3D_TRAVERSAL_STAIRCASE 0 /* the voxel staircase with one trapezoid per
+                                        axis step (kernel.cu:1764-1840) */
+/* The staircase's visited voxels (same counts, sources and averaging), with
+ * the integral taken along the straight ray: one trapezoid of grad p . k per
+ * step between the projections of consecutive voxel centres, the gradient
+ * sampled trilinearly there, plus a trapezoid leg from each end voxel to its
+ * projection.  Exact for a linear gradient.  No reference counterpart (the
+ * accuracy mode beside the staircase); plain method, affine solver. */
+#define OMNI3D_TRAVERSAL_TRILINEAR 1
 
 /* Iteration schedule. */
 #define OMNI3D_SOLVER_AFFINE 0 /* iterate only the voxels the rays read, then one full pass */
@@ -99,6 +118,7 @@ typedef struct omni3d_params {
   double tolerance;        /* stop when ||dp_S|| / ||p_S|| < tolerance over the voxels the
                               rays read; 0 = always run max_iterations (reference) */
   int seed_pn_with_p;      /* 1 = the first pass's sums start from p (kernel.cu:8813) */
+  int traversal;           /* OMNI3D_TRAVERSAL_* (default staircase) */
 } omni3d_params;
 
 typedef struct omni3d_report {
@@ -125,7 +145,8 @@ typedef struct omni3d_report {
 typedef struct omni3d_ctx omni3d_ctx;
 
 /* Defaults: rho 1, line spacing 1, plain FP64 affine, 10 iterations (the
- * shipped Parameter_Omni3D.dat line 14), tolerance 0, seed_pn_with_p 1. */
+ * shipped Parameter_Omni3D.dat line 14), tolerance 0, seed_pn_with_p 1,
+ * staircase traversal. */
 void omni3d_default_params(omni3d_params* p);
 
 int omni3d_create(const omni3d_params* params, omni3d_ctx** out);
@@ -160,6 +181,15 @@ int omni3d_generate_directions(int divide, float* k1, float* k2, float* k3, int 
 int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const void* dwdt,
                      const void* mask, int dtype, int mem_kind);
 
+/* Velocity of the snapshot for the Bernoulli seed (kernel.cu:8806-8811): every
+ * later omni3d_reconstruct starts from p with its j = 0 face replaced by
+ * pref - rho/2 (u^2 + v^2 + w^2) (from zeros when p is NULL), as the
+ * reference's driver does before its first pass.  u, v, w: X*Y*Z values
+ * (only the j = 0 face is read), dtype / mem_kind as for the field; the
+ * context keeps the face values.  u == NULL removes the seed. */
+int omni3d_set_velocity(omni3d_ctx* ctx, const void* u, const void* v, const void* w,
+                        double pref, int dtype, int mem_kind);
+
 /* Iterate to convergence on the GPU (kernel.cu:8805-8861).  p: in = initial
  * pressure (NULL = zeros), out = result, dtype/mem_kind as given.
  * weight_out (may be NULL): the per-voxel weight sum of the last pass
@@ -168,47 +198,101 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
 int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, int mem_kind,
                        omni3d_report* rep);
 
-/* Kernel-level drop-ins (device pointers, float, reference semantics with
- * full line x direction coverage and deterministic accumulation):
- * pn += sum over visits, pcountinner += weights (kernel.cu:6151/6281/6414). */
-int omni3d_es_inner(int X, int Y, int Z, int NoAngles, float linespacing, const float* k1_d,
-                    const float* k2_d, const float* k3_d, float deltx, float delty,
-                    float deltz, float density, const float* DuDt, const float* DvDt,
-                    const float* DwDt, const float* p, float* pn, float* pcountinner,
-                    void* hip_stream);
-int omni3d_es_inner_weighted(int X, int Y, int Z, int NoAngles, float linespacing,
+/* Kernel-level drop-ins: the reference kernels' argument lists in their order
+ * (kernel.cu:6151, 6281, 6414, 6481, 6347), device pointers, float, plus the
+ * HIP stream (NULL = the null stream) at the end.  One pass over every line x
+ * direction pair with deterministic accumulation: pn += sums over visits,
+ * pcountinner += weights.  index (the boundary numbering of calIndexGPU) is
+ * accepted and not read; NULL is allowed.
+ *
+ * The work is queued on the stream and the call returns without waiting for
+ * it.  Each host thread keeps a hidden context: the geometry plan is rebuilt
+ * only when the grid, line spacing or directions change, and the staged field
+ * with the Select / Weighted rule replay only when the field, curl, spacing,
+ * density, method or threshold change.  Changes are found by a fingerprint of
+ * the directions, field and curl computed on the GPU (one 16-byte read-back
+ * per call, which also reports a non-finite input or p as
+ * OMNI3D_ERR_NONFINITE before anything is queued), so buffers may be reused
+ * or rewritten between calls.  omni3d_dropin_release frees the thread's
+ * hidden context. */
+int omni3d_es_inner(long X, long Y, long Z, int NoAngles, float linespacing, const float* k1_d,
+                    const float* k2_d, const float* k3_d, const long* index, float deltx,
+                    float delty, float deltz, float density, const float* DuDt,
+                    const float* DvDt, const float* DwDt, const float* p, float* pn,
+                    float* pcountinner, void* hip_stream);
+int omni3d_es_inner_weighted(long X, long Y, long Z, int NoAngles, float linespacing,
                              const float* k1_d, const float* k2_d, const float* k3_d,
-                             float deltx, float delty, float deltz, float density,
-                             const float* DuDt, const float* DvDt, const float* DwDt,
-                             const float* curl, const float* p, float* pn,
+                             const long* index, float deltx, float delty, float deltz,
+                             float density, const float* DuDt, const float* DvDt,
+                             const float* DwDt, const float* curl, const float* p, float* pn,
                              float* pcountinner, void* hip_stream);
-int omni3d_es_inner_select(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_select(long X, long Y, long Z, int NoAngles, float linespacing,
                            const float* k1_d, const float* k2_d, const float* k3_d,
-                           float deltx, float delty, float deltz, float density,
-                           const float* DuDt, const float* DvDt, const float* DwDt,
-                           const float* curl, const float* p, float* pn, float* pcountinner,
-                           float threshold, void* hip_stream);
+                           const long* index, float deltx, float delty, float deltz,
+                           float density, const float* DuDt, const float* DvDt,
+                           const float* DwDt, const float* curl, const float* p, float* pn,
+                           float* pcountinner, float threshold, void* hip_stream);
 /* omni3dparallellinesESInnerSelectFixedBC, kernel.cu:6481-6549 (commented out
  * in the reference's main): the Select pass with the FixedBC step rule. */
-int omni3d_es_inner_select_fixedbc(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_select_fixedbc(long X, long Y, long Z, int NoAngles, float linespacing,
                                    const float* k1_d, const float* k2_d, const float* k3_d,
-                                   float deltx, float delty, float deltz, float density,
-                                   const float* DuDt, const float* DvDt, const float* DwDt,
-                                   const float* curl, const float* p, float* pn,
-                                   float* pcountinner, float threshold, void* hip_stream);
+                                   const long* index, float deltx, float delty, float deltz,
+                                   float density, const float* DuDt, const float* DvDt,
+                                   const float* DwDt, const float* curl, const float* p,
+                                   float* pn, float* pcountinner, float threshold,
+                                   void* hip_stream);
 /* omni3dparallellinesESInnerWeightedMiniCurl, kernel.cu:6347-6413 (commented
  * out in the reference's main; it calls bodyIntegralInnerMiniCurl): one
  * MiniCurl pass, unit counts into pcountinner. */
-int omni3d_es_inner_weighted_minicurl(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_weighted_minicurl(long X, long Y, long Z, int NoAngles, float linespacing,
                                       const float* k1_d, const float* k2_d, const float* k3_d,
-                                      float deltx, float delty, float deltz, float density,
-                                      const float* DuDt, const float* DvDt, const float* DwDt,
-                                      const float* curl, const float* p, float* pn,
-                                      float* pcountinner, void* hip_stream);
+                                      const long* index, float deltx, float delty, float deltz,
+                                      float density, const float* DuDt, const float* DvDt,
+                                      const float* DwDt, const float* curl, const float* p,
+                                      float* pn, float* pcountinner, void* hip_stream);
 /* devidecountInner, kernel.cu:6746-6759: where pcountinner > 0,
  * p = pn / pcountinner and pn = 0. */
-int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const float* pcountinner,
-                              void* hip_stream);
+int omni3d_devide_count_inner(long X, long Y, long Z, float* p, float* pn,
+                              const float* pcountinner, void* hip_stream);
+/* Frees the calling thread's hidden drop-in context (plan, staged field). */
+void omni3d_dropin_release(void);
+
+/* The same six under the reference kernels' own names and argument lists
+ * (kernel.cu:6151, 6281, 6414, 6481, 6347, 6746): a reference call site
+ * binds unchanged once its <<<grid, block>>> is dropped.  Null stream; void,
+ * as a kernel launch is: failures only through omni3d_last_global_error(). */
+void omni3dparallellinesESInner(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                float linespacing, float* k1_d, float* k2_d, float* k3_d,
+                                long* index, float deltx, float delty, float deltz,
+                                float density, float* DuDt, float* DvDt, float* DwDt, float* p,
+                                float* pn, float* pcountinner);
+void omni3dparallellinesESInnerWeighted(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                        float linespacing, float* k1_d, float* k2_d,
+                                        float* k3_d, long* index, float deltx, float delty,
+                                        float deltz, float density, float* DuDt, float* DvDt,
+                                        float* DwDt, float* curl, float* p, float* pn,
+                                        float* pcountinner);
+void omni3dparallellinesESInnerSelect(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                      float linespacing, float* k1_d, float* k2_d, float* k3_d,
+                                      long* index, float deltx, float delty, float deltz,
+                                      float density, float* DuDt, float* DvDt, float* DwDt,
+                                      float* curl, float* p, float* pn, float* pcountinner,
+                                      float threshold);
+void omni3dparallellinesESInnerSelectFixedBC(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                             float linespacing, float* k1_d, float* k2_d,
+                                             float* k3_d, long* index, float deltx, float delty,
+                                             float deltz, float density, float* DuDt,
+                                             float* DvDt, float* DwDt, float* curl, float* p,
+                                             float* pn, float* pcountinner, float threshold);
+void omni3dparallellinesESInnerWeightedMiniCurl(long Xsize, long Ysize, long Zsize,
+                                                int NoAngles, float linespacing, float* k1_d,
+                                                float* k2_d, float* k3_d, long* index,
+                                                float deltx, float delty, float deltz,
+                                                float density, float* DuDt, float* DvDt,
+                                                float* DwDt, float* curl, float* p, float* pn,
+                                                float* pcountinner);
+void devidecountInner(long Xsize, long Ysize, long Zsize, float* p, float* pn,
+                      float* pcountinner);
 
 /* |curl a| * mask with the reference's one-sided wall differences
  * (kernel.cu:8352-8379).  Device pointers, float; mask may be NULL, which
@@ -232,7 +316,6 @@ void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, floa
                                  float* threshold);
 int omni3d_threshold_hist(int X, int Y, int Z, const float* curl, float percentage,
                           float* threshold, int mem_kind, void* hip_stream);
-
 /* Boundary-pair pipeline of the README (README:18-28; kernels
  * omni3dparallellinesEqualSpacing kernel.cu:3414-3489 with bodyIntegral
  * kernel.cu:1353-1467, devidecount kernel.cu:3896-3909, BCiteration
@@ -290,6 +373,21 @@ int omni3d_sor3d_ex(int X, int Y, int Z, int NoItr, const int* BCScheme, double*
                     const double* rhs, double eps, int* sweeps_out, double* residue_out);
 int omni3d_calrhs(int X, int Y, int Z, double dx, double dy, double dz, double density,
                   const double* DuDt, const double* DvDt, const double* DwDt, double* rhs);
+
+/* initialIntegration (initialIntegration.h:3, initialIntegration.cpp:9-408),
+ * same signature, host buffers, run on the GPU: p[0] = pn[0] = pref, then
+ * from the centre voxel (X/2, Y/2, Z/2), which keeps the caller's pn, every
+ * voxel becomes the mean of the one-step trapezoid predictions (rho * delta *
+ * (a[v] + a[nb]) / 2) from its neighbours toward the centre along the axes on
+ * which it is off the centre; p = pn on return.  pBC is not read (nor is it
+ * by the reference).  Void: failures only through
+ * omni3d_last_global_error(); omni3d_initial_integration returns a status. */
+void initialIntegration(long Xsize, long Ysize, long Zsize, double deltx, double delty,
+                        double deltz, double density, double* DuDt, double* DvDt, double* DwDt,
+                        double pref, double* p, double* pn, double* pBC);
+int omni3d_initial_integration(long X, long Y, long Z, double dx, double dy, double dz,
+                               double density, const double* DuDt, const double* DvDt,
+                               const double* DwDt, double pref, double* p, double* pn);
 
 /* Snapshot batches (no reference counterpart; the reference loops over the
  * files of a folder, kernel.cu:8650-8960).  `count` snapshots of grids

@@ -22,6 +22,8 @@ int adjoint_checks(omni3d_ctx* ctx) {
   if (P.method != OMNI3D_METHOD_PLAIN && P.method != OMNI3D_METHOD_PLAIN_MASK)
     return fail(ctx, OMNI3D_ERR_ARG,
                 "the adjoint exists for the linear methods (plain, plain-mask) only");
+  if (P.traversal != OMNI3D_TRAVERSAL_STAIRCASE)
+    return fail(ctx, OMNI3D_ERR_ARG, "the adjoint is built for the staircase traversal");
   if (ctx->k.empty()) return fail(ctx, OMNI3D_ERR_STATE, "no directions set");
   return OMNI3D_OK;
 }

@@ -60,6 +60,15 @@ bool params_ok(const omni3d_params& p, std::string* why) {
     *why = "max_iterations must be >= 1";
     return false;
   }
+  if (p.traversal != OMNI3D_TRAVERSAL_STAIRCASE && p.traversal != OMNI3D_TRAVERSAL_TRILINEAR) {
+    *why = "unknown traversal";
+    return false;
+  }
+  if (p.traversal == OMNI3D_TRAVERSAL_TRILINEAR &&
+      (p.method != OMNI3D_METHOD_PLAIN || p.solver != OMNI3D_SOLVER_AFFINE)) {
+    *why = "the trilinear traversal runs the plain method with the affine solver";
+    return false;
+  }
   return true;
 }
 
@@ -104,6 +113,7 @@ void number_sources(int X, int Y, int Z, const std::vector<uint8_t>& flag, std::
   }
 }
 
+#ifdef OMNI3D_LAYOUT_STUDY
 // LDS-staged layout of the iteration (stage.hip).  Left off (pl.staged
 // false, the slot-ordered gathers are used) when a (group, direction) needs
 // more lines than a stage holds or the sizes exceed the 32-bit indices.
@@ -178,6 +188,8 @@ int build_staged(omni3d_ctx* ctx, int nS, int D, int64_t nnz) {
                  pl.staged ? "" : " (stage overflow: slot-ordered gathers)");
   return OMNI3D_OK;
 }
+
+#endif  // OMNI3D_LAYOUT_STUDY
 
 // Tile engine part of the plan: compiled step codes and per-tile segments.
 int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
@@ -432,7 +444,23 @@ int build_plan(omni3d_ctx* ctx) {
                                         rowcnt.as<unsigned int>(), key.as<unsigned long long>(), st));
   OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.rowptr.as<unsigned long long>(),
                                      key.as<unsigned long long>(), st));
-  {
+  // the dense layout (default iteration) unless sizes exceed its word format:
+  // 24-bit source ids, rows shorter than 4096 entries (int64 row sums of
+  // terms below 2^51); otherwise (e.g. thousands of directions) the masked
+  // sliced-ELL layout
+  const char* iter_env = std::getenv("OMNI3D_ITER");
+  // groups of 4 chunks use merged words with 22-bit sources
+  const int grp = ctx->dense_group == 4 && nS < (1 << 22) ? 4 : 1;
+#ifdef OMNI3D_LAYOUT_STUDY
+  const bool want_slots = iter_env && (std::strcmp(iter_env, "slots") == 0 ||
+                                       std::strcmp(iter_env, "staged") == 0);
+#else
+  const bool want_slots = iter_env && std::strcmp(iter_env, "slots") == 0;
+#endif
+  const bool use_dense = !want_slots && nS < (1 << (32 - dense_row_bits(grp))) &&
+                         mxh[0] < 4096u && nnz > 0 && nnz < (1ull << 31);
+  pl.ncell = 0;
+  if (!use_dense) {
     // masked sliced-ELL copy of the sorted rows for the iteration
     if (D > 4096) return fail(ctx, OMNI3D_ERR_ARG, "at most 4096 directions");
     const int nch = (nS + 63) / 64;
@@ -462,15 +490,8 @@ int build_plan(omni3d_ctx* ctx) {
     pl.ncell = (int64_t)nm;
   }
   {
-    // dense layout (default iteration) unless sizes exceed its word format:
-    // 24-bit source ids, rows shorter than 4096 entries (int64 row sums of
-    // terms below 2^51)
-    const char* env = std::getenv("OMNI3D_ITER");
     pl.dense = false;
-    // groups of 4 chunks use merged words with 22-bit sources
-    const int grp = ctx->dense_group == 4 && nS < (1 << 22) ? 4 : 1;
-    if (!(env && std::strcmp(env, "slots") == 0) && nS < (1 << (32 - dense_row_bits(grp))) &&
-        mxh[0] < 4096u && nnz > 0 && nnz < (1ull << 31)) {
+    if (use_dense) {
       DevBuf ka, kb, tmp;
       OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
       OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * nnz));
@@ -520,10 +541,12 @@ int build_plan(omni3d_ctx* ctx) {
       }
     }
     pl.staged = false;
-    if (env && std::strcmp(env, "staged") == 0) {
+#ifdef OMNI3D_LAYOUT_STUDY
+    if (iter_env && std::strcmp(iter_env, "staged") == 0) {
       const int rc = build_staged(ctx, nS, D, (int64_t)nnz);
       if (rc) return rc;
     }
+#endif
   }
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   pl.X = X;
@@ -556,6 +579,33 @@ void record_event(omni3d_ctx* ctx, int i) {
   if (!ctx->capturing) (void)hipEventRecord(ctx->ev[i], ctx->stream);
 }
 
+// partials of the measurement-only iteration layouts (none without them)
+unsigned int study_grid(int nS) {
+#ifdef OMNI3D_LAYOUT_STUDY
+  return staged_grid(nS);
+#else
+  (void)nS;
+  return 1;
+#endif
+}
+
+bool trilinear(const omni3d_params& P) { return P.traversal == OMNI3D_TRAVERSAL_TRILINEAR; }
+
+// The traversals of the plain method run one lane per ray (ray engine) on
+// request, and always for the trilinear traversal, whose samples need the
+// ray's line.
+bool ray_engine(const omni3d_ctx* ctx) { return ctx->engine != 0 || trilinear(ctx->prm); }
+
+// Bound on a visit's path, in units of rho * max(delta) * max|a|, for the
+// fixed-point scale.  Staircase: the longest ray's steps.  Trilinear: the
+// line (at most the steps) plus two legs below one voxel, each term a dot
+// product of up to three components (sqrt 3), and trilinear samples that
+// extrapolate up to a voxel beyond the face centres (at most 20 max|a| at a
+// corner): 40 (L + 2).
+double path_bound(const omni3d_params& P, const Plan& pl) {
+  return trilinear(P) ? 40.0 * ((double)pl.lmax + 2.0) : (double)pl.lmax;
+}
+
 WalkParams walk_params(const omni3d_params& P) {
   WalkParams wp;
   wp.rho = P.density;
@@ -567,6 +617,7 @@ WalkParams walk_params(const omni3d_params& P) {
     wp.hs[a][1] = -P.density * 1.0 * del[a] * 0.5;
     wp.hsf[a][0] = (float)wp.hs[a][0];
     wp.hsf[a][1] = (float)wp.hs[a][1];
+    wp.del[a] = del[a];
   }
   wp.thr = P.threshold;
   wp.method = P.method;
@@ -596,12 +647,12 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
   if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
   const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
-  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
+  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, path_bound(P, pl), (double)pl.cmax,
                                          (double)P.max_iterations, st));
   // pass A: trapezoid sums landing on source voxels
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->bs.as<void>(), 2 * (int64_t)ns, 0u, st));
   record_event(ctx, 2);
-  if (ctx->engine == 0) {
+  if (!ray_engine(ctx)) {
     // segment sums over all tiles, per-ray carries, then the source sums on
     // the surface tiles
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
@@ -621,7 +672,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           ctx->segval.as<double>(),
                                           ctx->bs.as<long long>(), st));
   } else {
-    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, pl.rays.as<RayRec>(), pl.nrays,
+    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, trilinear(P), pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
                                            nullptr, nullptr, sc, ctx->bs.as<long long>(), 0, 1, st));
     record_event(ctx, 8);
@@ -636,7 +687,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   OMNI_TRY_HIP(ctx, ctx->partials.alloc(
                         sizeof(double) * 2 *
                         std::max(iterate_grid(pl.nS),
-                                 std::max(staged_grid(pl.nS),
+                                 std::max(study_grid(pl.nS),
                                           std::max(dense_grid(pl.nS), 2 * grouped_grid(pl.nS))))));
   if (phase != 2) {
     OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
@@ -701,6 +752,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         }
         continue;
       }
+#ifdef OMNI3D_LAYOUT_STUDY
       if (pl.staged) {
         const hipError_t e = launch_iterate_staged(
             si, k, pl.diag.as<unsigned int>(), pl.slist.as<int>(), pl.cnt.as<unsigned int>(),
@@ -709,6 +761,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
         if (e != hipSuccess) return e;
         continue;
       }
+#endif
       const hipError_t e = launch_iterate(ctx->iter_waves, k, pl.nS, pl.moff.as<unsigned long long>(),
                                           pl.masks.as<unsigned long long>(),
                                           pl.cell_eb.as<unsigned int>(), pl.ent.as<int>(),
@@ -775,10 +828,10 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   }
   record_event(ctx, 4);
   // pass B: every voxel from p_S^{k*-1}
-  const int nc = ctx->engine == 0 ? 1 : ctx->acc_copies;
+  const int nc = ray_engine(ctx) ? ctx->acc_copies : 1;
   ctx->used_copies = nc;
   OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V * nc));
-  if (ctx->engine == 0) {
+  if (!ray_engine(ctx)) {
     // every voxel belongs to exactly one tile, which stores its sum
     record_event(ctx, 5);
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kFull, ctx->tile_threads, pl.ntiles, nullptr,
@@ -792,7 +845,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   } else {
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V * nc, 0u, st));
     record_event(ctx, 5);
-    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, pl.rays.as<RayRec>(), pl.nrays,
+    OMNI_TRY_HIP(ctx, launch_walk_plain<T>(true, trilinear(P), pl.rays.as<RayRec>(), pl.nrays,
                                            pl.dirs.as<DirConst>(), pl.g, wp, field, pl.sid.as<int>(),
                                            rot3 ? ctx->psel.as<double>() : pbuf[0],
                                            rot3 ? ctx->psel.as<double>() : pbuf[1], sc,
@@ -969,7 +1022,7 @@ bool affine_fusable(const omni3d_ctx* ctx) {
   const omni3d_params& P = ctx->prm;
   const Plan& pl = *ctx->plan;
   return P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN && pl.valid &&
-         pl.dense && pl.dense_group == 4 && !pl.staged && ctx->engine == 0 &&
+         pl.dense && pl.dense_group == 4 && !pl.staged && !ray_engine(ctx) &&
          P.max_iterations > 0 && pl.nS > 0;
 }
 
@@ -1080,18 +1133,22 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
     const int t = std::atoi(e);
     if (t == 256 || t == 512 || t == 1024) ctx->tile_threads = t;
   }
+#ifdef OMNI3D_LAYOUT_STUDY
   if (const char* e = std::getenv("OMNI3D_ITER_WAVES")) {
     const int w = std::atoi(e);
     if (w == 1 || w == 2 || w == 4 || w == -1 || w == -2 || w == -4) ctx->iter_waves = w;
   }
+#endif
   if (const char* e = std::getenv("OMNI3D_DENSE_GROUP")) {
     const int v = std::atoi(e);
     if (v == 1 || v == 4) ctx->dense_group = v;
   }
+#ifdef OMNI3D_LAYOUT_STUDY
   if (const char* e = std::getenv("OMNI3D_DENSE_VARIANT")) {
     const int v = std::atoi(e);
     if (v >= 0 && v <= 4) ctx->dense_variant = v;
   }
+#endif
   for (auto& e : ctx->ev) {
     if (hipEventCreate(&e) != hipSuccess) {
       delete ctx;
@@ -1227,6 +1284,47 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
   return OMNI3D_OK;
 }
 
+int omni3d_set_velocity(omni3d_ctx* ctx, const void* u, const void* v, const void* w,
+                        double pref, int dtype, int mem_kind) {
+  if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
+  if (!u) {
+    ctx->have_seed = false;
+    return OMNI3D_OK;
+  }
+  if (!v || !w) return fail(ctx, OMNI3D_ERR_ARG, "null velocity component");
+  if (dtype != OMNI3D_F32 && dtype != OMNI3D_F64) return fail(ctx, OMNI3D_ERR_ARG, "bad dtype");
+  if (mem_kind != OMNI3D_MEM_HOST && mem_kind != OMNI3D_MEM_DEVICE)
+    return fail(ctx, OMNI3D_ERR_ARG, "bad mem_kind");
+  if (!std::isfinite(pref)) return fail(ctx, OMNI3D_ERR_ARG, "pref must be finite");
+  const omni3d_params& P = ctx->prm;
+  const int64_t V = voxels(P);
+  const size_t esz = elem_size(dtype);
+  hipStream_t st = ctx->stream;
+  DevBuf stage;
+  const void* src[3] = {u, v, w};
+  if (mem_kind == OMNI3D_MEM_HOST) {
+    OMNI_TRY_HIP(ctx, stage.alloc(esz * V * 3));
+    for (int c = 0; c < 3; c++) {
+      void* d = static_cast<char*>(stage.as<void>()) + esz * V * c;
+      OMNI_TRY_HIP(ctx, hipMemcpyAsync(d, src[c], esz * V, hipMemcpyHostToDevice, st));
+      src[c] = d;
+    }
+  }
+  OMNI_TRY_HIP(ctx, ctx->seed_face.alloc(sizeof(double) * (size_t)P.nx * P.nz));
+  hipError_t e =
+      dtype == OMNI3D_F64
+          ? launch_bernoulli_face<double>(P.nx, P.ny, P.nz, (const double*)src[0],
+                                          (const double*)src[1], (const double*)src[2], pref,
+                                          P.density, ctx->seed_face.as<double>(), st)
+          : launch_bernoulli_face<float>(P.nx, P.ny, P.nz, (const float*)src[0],
+                                         (const float*)src[1], (const float*)src[2], pref,
+                                         P.density, ctx->seed_face.as<double>(), st);
+  OMNI_TRY_HIP(ctx, e);
+  if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  ctx->have_seed = true;
+  return OMNI3D_OK;
+}
+
 int omni3d_field_curl(omni3d_ctx* ctx, void* out, int dtype, int mem_kind) {
   if (!ctx || !out) return fail(ctx, OMNI3D_ERR_ARG, "null argument");
   if (!ctx->have_field) return fail(ctx, OMNI3D_ERR_STATE, "no field set");
@@ -1289,6 +1387,17 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
                        : launch_convert<float, double>((const float*)src, ctx->pinit.as<double>(), V, st);
     OMNI_TRY_HIP(ctx, e);
     if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    seed = P.seed_pn_with_p ? 1 : 0;
+  }
+  if (ctx->have_seed) {
+    // the j = 0 face gets the Bernoulli pressure (kernel.cu:8806-8811), over
+    // the caller's p or zeros
+    if (!p) {
+      OMNI_TRY_HIP(ctx, ctx->pinit.alloc(sizeof(double) * V));
+      OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->pinit.as<double>(), V, 0.0, st));
+    }
+    OMNI_TRY_HIP(ctx, launch_apply_face(P.nx, P.ny, P.nz, ctx->seed_face.as<double>(),
+                                        ctx->pinit.as<double>(), st));
     seed = P.seed_pn_with_p ? 1 : 0;
   }
   // averaging into the caller's buffers (device) or a staging buffer (host)

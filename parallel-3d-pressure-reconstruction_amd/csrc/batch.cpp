@@ -315,10 +315,19 @@ int omni3d_batch_set_field(omni3d_batch* b, int index, const void* dudt, const v
   omni3d_ctx* c = b->snap[index];
   c->stream = b->main;
   const void* old = c->field.as<void>();
+  const void* old_mask = c->mask.as<void>();
+  const bool had_mask = c->have_mask;
   const int rc = omni3d_set_field(c, dudt, dvdt, dwdt, mask, dtype, mem_kind);
   if (rc) return bfail(b, rc, c->err);
-  // the graph holds buffer addresses: a reallocated field invalidates it
-  if (c->field.as<void>() != old) drop_graph(b);
+  // the graph holds buffer addresses and was captured with or without the
+  // mask: a reallocated field or mask, or a mask added or removed,
+  // invalidates it
+  if (c->field.as<void>() != old || c->have_mask != had_mask ||
+      (c->have_mask && c->mask.as<void>() != old_mask))
+    drop_graph(b);
+  // a mask added or removed changes the work buffers the masked methods
+  // use: size them again with an eager run before the next capture
+  if (c->have_mask != had_mask) b->prepared = false;
   return OMNI3D_OK;
 }
 

@@ -22,6 +22,12 @@ bool params_ok(const omni3d_params& p, std::string* why);
 // (rebuilt only when one of them changed).
 int build_plan(omni3d_ctx* ctx);
 WalkParams walk_params(const omni3d_params& P);
+// traversal == OMNI3D_TRAVERSAL_TRILINEAR
+bool trilinear(const omni3d_params& P);
+// whether the plain method's traversals run one lane per ray
+bool ray_engine(const omni3d_ctx* ctx);
+// bound on a visit's path for the fixed-point scale (see launch_compute_scale)
+double path_bound(const omni3d_params& P, const Plan& pl);
 // Reference-schedule iteration; the result of iteration k* goes to
 // pdst/wdst (device, dtype), seed: -1 no initial pressure, 0 p only,
 // 1 p and the seed term.

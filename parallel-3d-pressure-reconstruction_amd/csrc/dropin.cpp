@@ -1,13 +1,19 @@
 // Kernel-level drop-ins for the reference's device entry points
 // (kernel.cu:6151 omni3dparallellinesESInner, 6281 ...Weighted, 6414
-// ...Select, 6746 devidecountInner, 8352 calCurlofMaterialAccCPU): the same
-// arguments as device pointers, one pass over every line x direction pair
-// accumulated into the caller's pn / pcountinner.
+// ...Select, 6481 ...SelectFixedBC, 6347 ...WeightedMiniCurl, 6746
+// devidecountInner, 8352 calCurlofMaterialAccCPU): the reference kernels'
+// argument lists as device pointers, one pass over every line x direction
+// pair accumulated into the caller's pn / pcountinner.
 //
-// Each host thread keeps one hidden context whose geometry plan (rays, visit
-// counts, tile segments) is reused while the grid, line spacing and
-// direction set stay the same, as they do across the reference's iteration
-// loop.  A pass runs on the tile engine of the reference's schedule.
+// Each host thread keeps one hidden context.  Its geometry plan (rays, visit
+// counts, tile segments) is rebuilt only when the grid, line spacing or
+// direction set change, and its staged field with the Select / Weighted rule
+// replay (which depends on the field, curl, spacing, density, method and
+// threshold, never on p) only when one of those changes -- as across the
+// reference's iteration loop, where only p changes.  Changes are detected by
+// a fingerprint of the inputs computed on the GPU; its 24-byte read-back is
+// the only host wait of a call.  A pass runs on the tile engine of the
+// reference's schedule and is queued without waiting for it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,49 +31,121 @@ using namespace omni3d;
 
 namespace {
 
+// Everything the staged field and the rule replay depend on.
+struct FieldKey {
+  unsigned long long field = 0, dirs = 0;
+  int method = -1;
+  float thr = 0, dx = 0, dy = 0, dz = 0, rho = 0;
+  bool operator==(const FieldKey& o) const {
+    return field == o.field && dirs == o.dirs && method == o.method &&
+           std::memcmp(&thr, &o.thr, sizeof(float)) == 0 && dx == o.dx && dy == o.dy &&
+           dz == o.dz && rho == o.rho;
+  }
+};
+
 struct KernelCache {
   omni3d_ctx* ctx = nullptr;
-  ~KernelCache() {
+  hipStream_t last = nullptr;  // stream of the last queued pass
+  bool pending = false;        // a pass may still read the context's buffers
+  DevBuf print;                // InputPrint, device
+  InputPrint* host = nullptr;  // its pinned host copy
+  bool dirs_valid = false;
+  unsigned long long dirs_hash = 0;
+  int D = 0;
+  bool field_valid = false;
+  FieldKey fkey;
+  void release() {
+    if (pending) (void)hipStreamSynchronize(last);
+    pending = false;
     if (ctx) omni3d_destroy(ctx);
+    ctx = nullptr;
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    print.release();
+    dirs_valid = field_valid = false;
   }
+  ~KernelCache() { release(); }
 };
 thread_local KernelCache g_kc;
 
-bool dims_ok(int X, int Y, int Z) {
+bool dims_ok(long X, long Y, long Z) {
   return X >= 2 && Y >= 2 && Z >= 2 && X <= kMaxDim && Y <= kMaxDim && Z <= kMaxDim;
 }
 
-// Returns the cached context configured for this call (plan rebuilt only if
-// the grid, line spacing or directions changed).
-int kernel_context(int X, int Y, int Z, int D, float ls, const float* k1_d, const float* k2_d,
-                   const float* k3_d, float dx, float dy, float dz, float rho, int method,
-                   float thr, hipStream_t st, omni3d_ctx** out) {
+// One pass of `method` over every ray, added into pn / pcountinner.
+int es_pass(int method, long X, long Y, long Z, int D, float ls, const float* k1_d,
+            const float* k2_d, const float* k3_d, float dx, float dy, float dz, float rho,
+            const float* ax, const float* ay, const float* az, const float* curl, float thr,
+            const float* p, float* pn, float* pc, void* stream) {
+  hipStream_t st = static_cast<hipStream_t>(stream);
   if (!dims_ok(X, Y, Z)) return fail(nullptr, OMNI3D_ERR_ARG, "grid dimensions must be in [2, 1023]");
   if (D < 1 || D > 4096) return fail(nullptr, OMNI3D_ERR_ARG, "NoAngles must be in [1, 4096]");
   if (!k1_d || !k2_d || !k3_d) return fail(nullptr, OMNI3D_ERR_ARG, "null direction array");
-  std::vector<float> a(D), b(D), c(D);
-  hipError_t e = hipMemcpyAsync(a.data(), k1_d, sizeof(float) * D, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(b.data(), k2_d, sizeof(float) * D, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(c.data(), k3_d, sizeof(float) * D, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return fail(nullptr, OMNI3D_ERR_HIP, std::string("direction copy: ") + hipGetErrorString(e));
-  omni3d_ctx*& ctx = g_kc.ctx;
+  if (!ax || !ay || !az || !p || !pn || !pc)
+    return fail(nullptr, OMNI3D_ERR_ARG, "null field / pressure / count pointer");
+  if (method != OMNI3D_METHOD_PLAIN && !curl) return fail(nullptr, OMNI3D_ERR_ARG, "null curl");
+  if (!(dx > 0 && dy > 0 && dz > 0)) return fail(nullptr, OMNI3D_ERR_ARG, "grid spacing must be positive");
+  if (!(ls > 0)) return fail(nullptr, OMNI3D_ERR_ARG, "line spacing must be positive");
+  KernelCache& kc = g_kc;
+  // a pass queued on another stream still reads the cached buffers
+  if (kc.pending && kc.last != st) {
+    const hipError_t e = hipStreamSynchronize(kc.last);
+    if (e != hipSuccess) return fail(nullptr, OMNI3D_ERR_HIP, hipGetErrorString(e));
+    kc.pending = false;
+  }
+  omni3d_ctx*& ctx = kc.ctx;
   if (ctx && (ctx->prm.nx != X || ctx->prm.ny != Y || ctx->prm.nz != Z ||
               ctx->prm.line_spacing != (double)ls)) {
     omni3d_destroy(ctx);
     ctx = nullptr;
+    kc.dirs_valid = kc.field_valid = false;
   }
   if (!ctx) {
     omni3d_params P;
     omni3d_default_params(&P);
-    P.nx = X;
-    P.ny = Y;
-    P.nz = Z;
+    P.nx = (int)X;
+    P.ny = (int)Y;
+    P.nz = (int)Z;
     P.line_spacing = ls;
     P.solver = OMNI3D_SOLVER_NAIVE;
     P.max_iterations = 1;
     int rc = omni3d_create(&P, &ctx);
     if (rc) return rc;
+    kc.dirs_valid = kc.field_valid = false;
+  }
+  ctx->stream = st;
+  const int64_t V = (int64_t)X * Y * Z;
+  // fingerprint of the inputs (and a non-finite check) before anything else
+  if (!kc.host) {
+    OMNI_TRY_HIP(ctx, kc.print.alloc(sizeof(InputPrint)));
+    OMNI_TRY_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&kc.host), sizeof(InputPrint),
+                                    hipHostMallocDefault));
+  }
+  InputPrint* dprint = kc.print.as<InputPrint>();
+  OMNI_TRY_HIP(ctx, launch_fill_u32(dprint, sizeof(InputPrint) / 4, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_fingerprint(D, k1_d, k2_d, k3_d, V, ax, ay, az,
+                                       method != OMNI3D_METHOD_PLAIN ? curl : nullptr, p, dprint,
+                                       st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(kc.host, dprint, sizeof(InputPrint), hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  kc.pending = false;
+  const InputPrint ip = *kc.host;
+  if (ip.nonfinite)
+    return fail(nullptr, OMNI3D_ERR_NONFINITE, "non-finite value in the directions, field, curl or p");
+  // directions: copied to the host (the plan build needs them) only when
+  // their fingerprint changed
+  if (!kc.dirs_valid || kc.D != D || kc.dirs_hash != ip.dirs) {
+    std::vector<float> a(D), b(D), c(D);
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(a.data(), k1_d, sizeof(float) * D, hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(b.data(), k2_d, sizeof(float) * D, hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(c.data(), k3_d, sizeof(float) * D, hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    kc.dirs_valid = kc.field_valid = false;
+    int rc = omni3d_set_directions(ctx, a.data(), b.data(), c.data(), D);
+    if (rc) return fail(nullptr, rc, ctx->err);
+    kc.dirs_valid = true;
+    kc.D = D;
+    kc.dirs_hash = ip.dirs;
   }
   omni3d_params& P = ctx->prm;
   P.dx = dx;
@@ -78,42 +156,33 @@ int kernel_context(int X, int Y, int Z, int D, float ls, const float* k1_d, cons
   P.threshold = thr;
   P.faithful_select = 1;
   P.precision = OMNI3D_FP64;
-  if (!(P.dx > 0 && P.dy > 0 && P.dz > 0)) return fail(nullptr, OMNI3D_ERR_ARG, "grid spacing must be positive");
-  ctx->stream = st;
-  bool same = (int)(ctx->k.size() / 3) == D;
-  for (int i = 0; same && i < D; i++)
-    same = ctx->k[3 * i] == a[i] && ctx->k[3 * i + 1] == b[i] && ctx->k[3 * i + 2] == c[i];
-  if (!same) {
-    int rc = omni3d_set_directions(ctx, a.data(), b.data(), c.data(), D);
-    if (rc) return fail(nullptr, rc, ctx->err);
-  }
-  *out = ctx;
-  return OMNI3D_OK;
-}
-
-// One pass of `method` over every ray, added into pn / pcountinner.
-int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d, const float* k2_d,
-            const float* k3_d, float dx, float dy, float dz, float rho, const float* ax,
-            const float* ay, const float* az, const float* curl, float thr, const float* p,
-            float* pn, float* pc, void* stream) {
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (!ax || !ay || !az || !p || !pn || !pc)
-    return fail(nullptr, OMNI3D_ERR_ARG, "null field / pressure / count pointer");
-  if (method != OMNI3D_METHOD_PLAIN && !curl) return fail(nullptr, OMNI3D_ERR_ARG, "null curl");
-  omni3d_ctx* ctx = nullptr;
-  int rc = kernel_context(X, Y, Z, D, ls, k1_d, k2_d, k3_d, dx, dy, dz, rho, method, thr, st, &ctx);
-  if (rc) return rc;
-  rc = omni3d_set_field(ctx, ax, ay, az, nullptr, OMNI3D_F32, OMNI3D_MEM_DEVICE);
-  if (rc) return fail(nullptr, rc, ctx->err);
-  rc = build_plan(ctx);
+  int rc = build_plan(ctx);
   if (rc) return fail(nullptr, rc, ctx->err);
   const Plan& pl = *ctx->plan;
-  const int64_t V = (int64_t)X * Y * Z;
+  FieldKey key;
+  key.field = ip.field;
+  key.dirs = kc.dirs_hash;
+  key.method = method;
+  key.thr = thr;
+  key.dx = dx;
+  key.dy = dy;
+  key.dz = dz;
+  key.rho = rho;
+  const bool restage = !kc.field_valid || !(key == kc.fkey);
+  kc.field_valid = false;
   DevScalars* sc = ctx->scal.as<DevScalars>();
   // fixed-point sums with unit weights: plain (counts of the plan) and
   // MiniCurl (counts of its own paths, counted by the walk)
   const bool minicurl = method == OMNI3D_METHOD_MINICURL;
   const bool fixed = method == OMNI3D_METHOD_PLAIN || minicurl;
+  if (restage) {
+    rc = omni3d_set_field(ctx, ax, ay, az, nullptr, OMNI3D_F32, OMNI3D_MEM_DEVICE);
+    if (rc) return fail(nullptr, rc, ctx->err);
+    if (method != OMNI3D_METHOD_PLAIN) {
+      OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
+      OMNI_TRY_HIP(ctx, (launch_convert<float, double>(curl, ctx->aux.as<double>(), V, st)));
+    }
+  }
   OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
   OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
   OMNI_TRY_HIP(ctx, (launch_convert<float, double>(p, ctx->pa.as<double>(), V, st)));
@@ -123,12 +192,7 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   const double gfac = (double)rho * std::max((double)dx, std::max((double)dy, (double)dz));
   OMNI_TRY_HIP(ctx, launch_compute_scale(sc, std::fabs(gfac), (double)pl.lmax, (double)pl.cmax,
                                          1.0, st));
-  const double* aux = nullptr;
-  if (method != OMNI3D_METHOD_PLAIN) {
-    OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
-    OMNI_TRY_HIP(ctx, (launch_convert<float, double>(curl, ctx->aux.as<double>(), V, st)));
-    aux = ctx->aux.as<double>();
-  }
+  const double* aux = method != OMNI3D_METHOD_PLAIN ? ctx->aux.as<double>() : nullptr;
   if (minicurl) {
     OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
@@ -137,7 +201,6 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
     OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
-    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
   } else {
     OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
@@ -150,25 +213,29 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
   if (!(menv && std::strcmp(menv, "naive") == 0) && method != OMNI3D_METHOD_SELECT_FIXEDBC &&
       !minicurl) {
     const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
-    OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
-    OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
-    OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
-    OMNI_TRY_HIP(ctx, ctx->mk_r.alloc(sizeof(uint32_t) * nv));
-    OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
-    OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (fixed ? 1 : nv)));
-    OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (fixed ? 1 : nv)));
+    if (restage) {
+      // the rule replay and its setup tile pass: per segment the first-visit
+      // record, the sums after its last restart and the visit weights
+      OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
+      OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
+      OMNI_TRY_HIP(ctx, ctx->mk_r.alloc(sizeof(uint32_t) * nv));
+      OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
+      OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (fixed ? 1 : nv)));
+      OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (fixed ? 1 : nv)));
+      OMNI_TRY_HIP(ctx, launch_mask_walk<double>(method, pl.rays.as<RayRec>(), pl.nrays,
+                                                 pl.dirs.as<DirConst>(), pl.g, pl.tg, wp,
+                                                 ctx->field.as<void>(), aux,
+                                                 pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                                 pl.tile_ptr.as<unsigned long long>(),
+                                                 pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
+                                                 ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                                 ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                                 ctx->mk_wn.as<double>(), nullptr,
+                                                 fixed ? nullptr : ctx->wsum.as<double>(),
+                                                 ctx->srcf.as<uint8_t>(), st));
+    }
     const double* pa = ctx->pa.as<double>();
-    OMNI_TRY_HIP(ctx, launch_mask_walk<double>(method, pl.rays.as<RayRec>(), pl.nrays,
-                                               pl.dirs.as<DirConst>(), pl.g, pl.tg, wp,
-                                               ctx->field.as<void>(), aux,
-                                               pl.val_off.as<unsigned long long>(), pl.ntiles,
-                                               pl.tile_ptr.as<unsigned long long>(),
-                                               pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
-                                               ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
-                                               ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
-                                               ctx->mk_wn.as<double>(), nullptr,
-                                               fixed ? nullptr : ctx->wsum.as<double>(),
-                                               ctx->srcf.as<uint8_t>(), st));
+    OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
     OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
                                         pl.val_off.as<unsigned long long>(),
                                         ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
@@ -184,6 +251,7 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
                                                  fixed ? ctx->sum.as<long long>() : nullptr,
                                                  fixed ? nullptr : ctx->sumd.as<double>(), st));
   } else {
+    if (!fixed) OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
     OMNI_TRY_HIP(ctx, launch_walk_naive<double>(method, pl.rays.as<RayRec>(), pl.nrays,
                                                 pl.dirs.as<DirConst>(), pl.g, wp,
                                                 ctx->field.as<void>(), aux, ctx->pa.as<double>(),
@@ -200,11 +268,12 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
                                                    : pl.cnt.as<unsigned int>(),
                                           fixed ? nullptr : ctx->sumd.as<double>(),
                                           fixed ? nullptr : ctx->wsum.as<double>(), sc, pn, pc, st));
-  // the pass reads the context's work buffers: finish before another call reuses them
-  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  DevScalars hs;
-  OMNI_TRY_HIP(ctx, hipMemcpy(&hs, sc, sizeof(hs), hipMemcpyDeviceToHost));
-  if (hs.status) return fail(nullptr, hs.status, "non-finite value in the field or pressure");
+  // queued: the next call on this stream is ordered after it, a call on
+  // another stream waits for it first
+  kc.pending = true;
+  kc.last = st;
+  kc.field_valid = true;
+  kc.fkey = key;
   return OMNI3D_OK;
 }
 
@@ -212,66 +281,137 @@ int es_pass(int method, int X, int Y, int Z, int D, float ls, const float* k1_d,
 
 extern "C" {
 
-int omni3d_es_inner(int X, int Y, int Z, int NoAngles, float linespacing, const float* k1_d,
-                    const float* k2_d, const float* k3_d, float deltx, float delty, float deltz,
-                    float density, const float* DuDt, const float* DvDt, const float* DwDt,
-                    const float* p, float* pn, float* pcountinner, void* hip_stream) {
+int omni3d_es_inner(long X, long Y, long Z, int NoAngles, float linespacing, const float* k1_d,
+                    const float* k2_d, const float* k3_d, const long* index, float deltx,
+                    float delty, float deltz, float density, const float* DuDt,
+                    const float* DvDt, const float* DwDt, const float* p, float* pn,
+                    float* pcountinner, void* hip_stream) {
+  (void)index;
   return es_pass(OMNI3D_METHOD_PLAIN, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
                  delty, deltz, density, DuDt, DvDt, DwDt, nullptr, 0.0f, p, pn, pcountinner,
                  hip_stream);
 }
 
-int omni3d_es_inner_weighted(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_weighted(long X, long Y, long Z, int NoAngles, float linespacing,
                              const float* k1_d, const float* k2_d, const float* k3_d,
-                             float deltx, float delty, float deltz, float density,
-                             const float* DuDt, const float* DvDt, const float* DwDt,
-                             const float* curl, const float* p, float* pn, float* pcountinner,
-                             void* hip_stream) {
+                             const long* index, float deltx, float delty, float deltz,
+                             float density, const float* DuDt, const float* DvDt,
+                             const float* DwDt, const float* curl, const float* p, float* pn,
+                             float* pcountinner, void* hip_stream) {
+  (void)index;
   return es_pass(OMNI3D_METHOD_WEIGHTED, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
                  delty, deltz, density, DuDt, DvDt, DwDt, curl, 0.0f, p, pn, pcountinner,
                  hip_stream);
 }
 
-int omni3d_es_inner_select(int X, int Y, int Z, int NoAngles, float linespacing,
-                           const float* k1_d, const float* k2_d, const float* k3_d, float deltx,
-                           float delty, float deltz, float density, const float* DuDt,
-                           const float* DvDt, const float* DwDt, const float* curl, const float* p,
-                           float* pn, float* pcountinner, float threshold, void* hip_stream) {
+int omni3d_es_inner_select(long X, long Y, long Z, int NoAngles, float linespacing,
+                           const float* k1_d, const float* k2_d, const float* k3_d,
+                           const long* index, float deltx, float delty, float deltz,
+                           float density, const float* DuDt, const float* DvDt,
+                           const float* DwDt, const float* curl, const float* p, float* pn,
+                           float* pcountinner, float threshold, void* hip_stream) {
+  (void)index;
   return es_pass(OMNI3D_METHOD_SELECT, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
                  delty, deltz, density, DuDt, DvDt, DwDt, curl, threshold, p, pn, pcountinner,
                  hip_stream);
 }
 
-int omni3d_es_inner_select_fixedbc(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_select_fixedbc(long X, long Y, long Z, int NoAngles, float linespacing,
                                    const float* k1_d, const float* k2_d, const float* k3_d,
-                                   float deltx, float delty, float deltz, float density,
-                                   const float* DuDt, const float* DvDt, const float* DwDt,
-                                   const float* curl, const float* p, float* pn,
-                                   float* pcountinner, float threshold, void* hip_stream) {
+                                   const long* index, float deltx, float delty, float deltz,
+                                   float density, const float* DuDt, const float* DvDt,
+                                   const float* DwDt, const float* curl, const float* p,
+                                   float* pn, float* pcountinner, float threshold,
+                                   void* hip_stream) {
+  (void)index;
   return es_pass(OMNI3D_METHOD_SELECT_FIXEDBC, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d,
                  deltx, delty, deltz, density, DuDt, DvDt, DwDt, curl, threshold, p, pn,
                  pcountinner, hip_stream);
 }
 
-int omni3d_es_inner_weighted_minicurl(int X, int Y, int Z, int NoAngles, float linespacing,
+int omni3d_es_inner_weighted_minicurl(long X, long Y, long Z, int NoAngles, float linespacing,
                                       const float* k1_d, const float* k2_d, const float* k3_d,
-                                      float deltx, float delty, float deltz, float density,
-                                      const float* DuDt, const float* DvDt, const float* DwDt,
-                                      const float* curl, const float* p, float* pn,
-                                      float* pcountinner, void* hip_stream) {
+                                      const long* index, float deltx, float delty, float deltz,
+                                      float density, const float* DuDt, const float* DvDt,
+                                      const float* DwDt, const float* curl, const float* p,
+                                      float* pn, float* pcountinner, void* hip_stream) {
+  (void)index;
   return es_pass(OMNI3D_METHOD_MINICURL, X, Y, Z, NoAngles, linespacing, k1_d, k2_d, k3_d, deltx,
                  delty, deltz, density, DuDt, DvDt, DwDt, curl, 0.0f, p, pn, pcountinner,
                  hip_stream);
 }
 
-int omni3d_devide_count_inner(int X, int Y, int Z, float* p, float* pn, const float* pcountinner,
-                              void* hip_stream) {
+int omni3d_devide_count_inner(long X, long Y, long Z, float* p, float* pn,
+                              const float* pcountinner, void* hip_stream) {
   if (X < 1 || Y < 1 || Z < 1) return fail(nullptr, OMNI3D_ERR_ARG, "bad grid dimensions");
   if (!p || !pn || !pcountinner) return fail(nullptr, OMNI3D_ERR_ARG, "null pointer");
   hipError_t e = launch_divide_count((int64_t)X * Y * Z, p, pn, pcountinner,
                                      static_cast<hipStream_t>(hip_stream));
   if (e != hipSuccess) return fail(nullptr, OMNI3D_ERR_HIP, hipGetErrorString(e));
   return OMNI3D_OK;
+}
+
+void omni3d_dropin_release(void) { g_kc.release(); }
+
+// The reference kernels' own names (a call site minus its <<<grid, block>>>).
+void omni3dparallellinesESInner(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                float linespacing, float* k1_d, float* k2_d, float* k3_d,
+                                long* index, float deltx, float delty, float deltz,
+                                float density, float* DuDt, float* DvDt, float* DwDt, float* p,
+                                float* pn, float* pcountinner) {
+  (void)omni3d_es_inner(Xsize, Ysize, Zsize, NoAngles, linespacing, k1_d, k2_d, k3_d, index,
+                        deltx, delty, deltz, density, DuDt, DvDt, DwDt, p, pn, pcountinner,
+                        nullptr);
+}
+
+void omni3dparallellinesESInnerWeighted(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                        float linespacing, float* k1_d, float* k2_d,
+                                        float* k3_d, long* index, float deltx, float delty,
+                                        float deltz, float density, float* DuDt, float* DvDt,
+                                        float* DwDt, float* curl, float* p, float* pn,
+                                        float* pcountinner) {
+  (void)omni3d_es_inner_weighted(Xsize, Ysize, Zsize, NoAngles, linespacing, k1_d, k2_d, k3_d,
+                                 index, deltx, delty, deltz, density, DuDt, DvDt, DwDt, curl, p,
+                                 pn, pcountinner, nullptr);
+}
+
+void omni3dparallellinesESInnerSelect(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                      float linespacing, float* k1_d, float* k2_d, float* k3_d,
+                                      long* index, float deltx, float delty, float deltz,
+                                      float density, float* DuDt, float* DvDt, float* DwDt,
+                                      float* curl, float* p, float* pn, float* pcountinner,
+                                      float threshold) {
+  (void)omni3d_es_inner_select(Xsize, Ysize, Zsize, NoAngles, linespacing, k1_d, k2_d, k3_d,
+                               index, deltx, delty, deltz, density, DuDt, DvDt, DwDt, curl, p,
+                               pn, pcountinner, threshold, nullptr);
+}
+
+void omni3dparallellinesESInnerSelectFixedBC(long Xsize, long Ysize, long Zsize, int NoAngles,
+                                             float linespacing, float* k1_d, float* k2_d,
+                                             float* k3_d, long* index, float deltx, float delty,
+                                             float deltz, float density, float* DuDt,
+                                             float* DvDt, float* DwDt, float* curl, float* p,
+                                             float* pn, float* pcountinner, float threshold) {
+  (void)omni3d_es_inner_select_fixedbc(Xsize, Ysize, Zsize, NoAngles, linespacing, k1_d, k2_d,
+                                       k3_d, index, deltx, delty, deltz, density, DuDt, DvDt,
+                                       DwDt, curl, p, pn, pcountinner, threshold, nullptr);
+}
+
+void omni3dparallellinesESInnerWeightedMiniCurl(long Xsize, long Ysize, long Zsize,
+                                                int NoAngles, float linespacing, float* k1_d,
+                                                float* k2_d, float* k3_d, long* index,
+                                                float deltx, float delty, float deltz,
+                                                float density, float* DuDt, float* DvDt,
+                                                float* DwDt, float* curl, float* p, float* pn,
+                                                float* pcountinner) {
+  (void)omni3d_es_inner_weighted_minicurl(Xsize, Ysize, Zsize, NoAngles, linespacing, k1_d, k2_d,
+                                          k3_d, index, deltx, delty, deltz, density, DuDt, DvDt,
+                                          DwDt, curl, p, pn, pcountinner, nullptr);
+}
+
+void devidecountInner(long Xsize, long Ysize, long Zsize, float* p, float* pn,
+                      float* pcountinner) {
+  (void)omni3d_devide_count_inner(Xsize, Ysize, Zsize, p, pn, pcountinner, nullptr);
 }
 
 int omni3d_curl(int X, int Y, int Z, float deltx, float delty, float deltz, const float* DuDt,

@@ -154,6 +154,10 @@ struct omni3d_ctx {
   omni3d::DevBuf field, mask;
   bool have_field = false;
   bool have_mask = false;
+  // Bernoulli seed of the j = 0 face (X*Z doubles), written into the
+  // starting pressure of every reconstruction while set
+  omni3d::DevBuf seed_face;
+  bool have_seed = false;
   // per-reconstruction work buffers
   omni3d::DevBuf scal;      // DevScalars
   omni3d::DevBuf pinit;     // double[V]

@@ -44,11 +44,21 @@ struct DevScalars {
                                      // (k % 3 with the finish folded into the next launch)
 };
 
+// Fingerprint of a kernel-level drop-in's inputs (directions; field and
+// curl) and a flag for a non-finite input or p.
+struct InputPrint {
+  unsigned long long dirs;
+  unsigned long long field;
+  unsigned int nonfinite;
+  unsigned int pad;
+};
+
 // Method-specific walk parameters.
 struct WalkParams {
   double rho;
   double hs[3][2];   // ((-rho * s) * delta_a) * 0.5 for s = -1 (index 0) / +1 (index 1)
   float hsf[3][2];   // the same factors rounded to FP32
+  double del[3];     // grid spacing (the trilinear walk's displacements)
   double thr;        // Select threshold (+inf for plain-mask)
   int method;
   int faithful;

@@ -34,11 +34,25 @@ hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst
                                unsigned int* cursor, unsigned long long* key, hipStream_t st);
 // field: float4 / double4 per voxel (a_x, a_y, a_z, 0); acc: ncopies accumulator
 // copies copy_stride apart (block b adds into copy b % ncopies).
+// trilinear: the straight-ray integral of the trilinear traversal instead of
+// the staircase's trapezoids (same visits).
 template <typename T>
-hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                             GridDesc g, const WalkParams& wp, const void* field, const int* sid,
-                             const double* p0, const double* p1, const DevScalars* sc,
-                             long long* acc, int64_t copy_stride, int ncopies, hipStream_t st);
+hipError_t launch_walk_plain(bool full, bool trilinear, const RayRec* rays, int64_t nrays,
+                             const DirConst* dirs, GridDesc g, const WalkParams& wp,
+                             const void* field, const int* sid, const double* p0,
+                             const double* p1, const DevScalars* sc, long long* acc,
+                             int64_t copy_stride, int ncopies, hipStream_t st);
+
+// inputs.hip ---------------------------------------------------------------
+// out must be zeroed before the launch (it accumulates).
+hipError_t launch_fingerprint(int D, const float* k1, const float* k2, const float* k3, int64_t V,
+                              const float* ax, const float* ay, const float* az,
+                              const float* curl, const float* p, InputPrint* out,
+                              hipStream_t st);
+template <typename S>
+hipError_t launch_bernoulli_face(int X, int Y, int Z, const S* u, const S* v, const S* w,
+                                 double pref, double rho, double* face, hipStream_t st);
+hipError_t launch_apply_face(int X, int Y, int Z, const double* face, double* p, hipStream_t st);
 
 // naive.hip ----------------------------------------------------------------
 // method: 0 plain, 1 plain-mask, 2 select, 3 weighted.  Iteration k reads

@@ -299,6 +299,8 @@ extern "C" int omni3d_boundary_pairs(omni3d_ctx* ctx, int iterations, void* p, i
   if (!ctx) return fail(nullptr, OMNI3D_ERR_ARG, "null context");
   if (!p) return fail(ctx, OMNI3D_ERR_ARG, "null pressure buffer");
   if (iterations < 0) return fail(ctx, OMNI3D_ERR_ARG, "iterations must be >= 0");
+  if (ctx->prm.traversal != OMNI3D_TRAVERSAL_STAIRCASE)
+    return fail(ctx, OMNI3D_ERR_ARG, "the boundary pairs are built for the staircase traversal");
   if (dtype != OMNI3D_F32 && dtype != OMNI3D_F64) return fail(ctx, OMNI3D_ERR_ARG, "bad dtype");
   if (mem_kind != OMNI3D_MEM_HOST && mem_kind != OMNI3D_MEM_DEVICE)
     return fail(ctx, OMNI3D_ERR_ARG, "bad mem_kind");

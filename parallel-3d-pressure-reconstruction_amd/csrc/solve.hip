@@ -207,6 +207,7 @@ k_iterate(int k, int nS, const unsigned long long* __restrict__ moff,
   }
 }
 
+#ifdef OMNI3D_LAYOUT_STUDY
 // The same iteration over the LDS-staged layout: a workgroup of kG
 // wavefronts owns kG consecutive chunks.  Per stage, the workgroup copies the
 // stage's distinct source lines (16 doubles each) into LDS with 16-byte
@@ -346,6 +347,8 @@ k_iterate_staged(StagedIter si, int k, const unsigned int* __restrict__ diag,
     partials[2 * blockIdx.x + 1] = c;
   }
 }
+
+#endif  // OMNI3D_LAYOUT_STUDY
 
 // The same iteration over the dense layout: each 64-row chunk's entries
 // sorted by source, one word (row in the chunk << 24 | source) each.  A
@@ -987,6 +990,7 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
 #define OMNI_ITER(W, M)                                                                        \
   hipLaunchKernelGGL((k_iterate<W, M>), dim3(grid), dim3(64 * W), 0, st, k, nS, moff, masks,     \
                      cell_eb, ent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
+#ifdef OMNI3D_LAYOUT_STUDY
   const bool masked = waves < 0;
   if (waves < 0) waves = -waves;
   if (masked) {
@@ -998,6 +1002,10 @@ hipError_t launch_iterate(int waves, int k, int nS, const unsigned long long* mo
     else if (waves == 2) OMNI_ITER(2, false);
     else OMNI_ITER(4, false);
   }
+#else
+  (void)waves;
+  OMNI_ITER(2, false);
+#endif
 #undef OMNI_ITER
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol, 2);
@@ -1060,6 +1068,7 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
   hipLaunchKernelGGL((k_iterate_dense<W, U>), dim3(grid), dim3(64 * W), 0, st, k, nS, rowptr,    \
                      dent, diag, slist, cnt, bs, pinit, seed, pold, pnew, sc, partials)
   const int nch = (nS + 63) / 64;
+#ifdef OMNI3D_LAYOUT_STUDY
   const int w = variant == 1 || variant == 2 ? 8 : 4;
   grid = (unsigned int)std::max(1, std::min((nch + w - 1) / w, 65536));
   switch (variant) {
@@ -1069,6 +1078,11 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
     case 4: OMNI_DENSE(4, 16); break;
     default: OMNI_DENSE(4, 8); break;
   }
+#else
+  (void)variant;
+  grid = (unsigned int)std::max(1, std::min((nch + 3) / 4, 65536));
+  OMNI_DENSE(4, 8);
+#endif
 #undef OMNI_DENSE
   hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, (int)grid, partials, sc,
                      max_iter, tol, 2);
@@ -1088,6 +1102,7 @@ hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsi
   return hipGetLastError();
 }
 
+#ifdef OMNI3D_LAYOUT_STUDY
 unsigned int staged_grid(int nS) {
   const int nch = (nS + 63) / 64;
   const int ng = (nch + kStageGroup - 1) / kStageGroup;
@@ -1106,6 +1121,7 @@ hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int
                      max_iter, tol, 2);
   return hipGetLastError();
 }
+#endif  // OMNI3D_LAYOUT_STUDY
 
 hipError_t launch_sort_rows(int nS, const unsigned long long* rowptr, unsigned long long* key,
                             hipStream_t st) {

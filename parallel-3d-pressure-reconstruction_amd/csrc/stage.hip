@@ -1,6 +1,11 @@
-// Plan construction of the LDS-staged source iteration.
+// Plan construction of the source iteration's word layouts.
 //
-// Rows (sources) are grouped kStageGroup chunks of 64 at a time.  For every
+// Dense words (the default): per group of 64-row chunks, the CSR entries
+// sorted by (source, row) and equal entries merged into one word
+// (row << 24) | (multiplicity - 1) << 22 | source.
+//
+// LDS-staged lines (measurement only, built with OMNI3D_LAYOUT_STUDY): rows
+// (sources) are grouped kStageGroup chunks of 64 at a time.  For every
 // (group, direction) the distinct 16-source lines (128 bytes of p) that the
 // group's entries of that direction gather are listed once; consecutive
 // directions are packed into stages of at most kStageCap lines, which the
@@ -23,6 +28,7 @@ __host__ __device__ inline int bits_for(unsigned long long n) {
   return b;
 }
 
+#ifdef OMNI3D_LAYOUT_STUDY
 // One wavefront per chunk: key (group, direction, line) and the entry
 // index of every packed entry.
 __global__ void __launch_bounds__(64)
@@ -133,6 +139,8 @@ __global__ void k_stage_offsets(int64_t n, int lb, int db, int D,
   const unsigned int base = sbase[g * (unsigned long long)D + d];
   off[e] = (unsigned short)((u - base) * 16u + ((unsigned int)ent[e] & 15u));
 }
+
+#endif  // OMNI3D_LAYOUT_STUDY
 
 unsigned int grid1(int64_t n) { return (unsigned int)((n + 255) / 256); }
 
@@ -258,6 +266,7 @@ hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
   return hipGetLastError();
 }
 
+#ifdef OMNI3D_LAYOUT_STUDY
 int stage_key_bits(int nS, int D) {
   const int nch = (nS + 63) / 64;
   const int ngroups = (nch + kStageGroup - 1) / kStageGroup;
@@ -327,5 +336,7 @@ hipError_t stage_finish(const StageBuild& sb, int ngroups, unsigned int* ulines,
                      sb.keys_b, sb.vals_b, sb.incl, sbase, sb.ent, off);
   return hipGetLastError();
 }
+
+#endif  // OMNI3D_LAYOUT_STUDY
 
 }  // namespace omni3d

@@ -33,6 +33,7 @@ struct StageBuild {
   size_t tmp_bytes = 0;
 };
 
+#ifdef OMNI3D_LAYOUT_STUDY
 int stage_key_bits(int nS, int D);
 size_t stage_tmp_bytes(int64_t nnz, int64_t ngd);
 // sorted (group, direction, line) keys and the inclusive count of distinct keys
@@ -42,6 +43,7 @@ hipError_t stage_finish(const StageBuild& sb, int ngroups, unsigned int* ulines,
                         unsigned int* st_hi, unsigned short* st_dend, unsigned int* sbase,
                         unsigned int* cse, unsigned short* off, unsigned int* overflow,
                         hipStream_t st);
+#endif
 
 // One source iteration k over the staged layout, then the relative change
 // and stop rule (as launch_iterate).
@@ -57,7 +59,9 @@ struct StagedIter {
   const unsigned int* cell_eb = nullptr;
   const unsigned short* off = nullptr;
 };
+#ifdef OMNI3D_LAYOUT_STUDY
 unsigned int staged_grid(int nS);
+#endif
 
 // Dense layout: per 64-row chunk, entries sorted by source, one word
 // (row in chunk << 24 | source) each; the chunk's words occupy the CSR range
@@ -131,10 +135,12 @@ struct IterDesc {
 // sums); blk_snap[b] = the snapshot of workgroup b
 hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsigned int* blk_snap,
                                 unsigned int nblocks, int max_iter, double tol, hipStream_t st);
+#ifdef OMNI3D_LAYOUT_STUDY
 hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
                                  const int* slist, const unsigned int* cnt, const long long* bs,
                                  const double* pinit, int seed, const double* pold, double* pnew,
                                  DevScalars* sc, double* partials, int max_iter, double tol,
                                  hipStream_t st);
+#endif
 
 }  // namespace omni3d

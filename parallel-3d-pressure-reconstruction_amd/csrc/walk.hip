@@ -223,7 +223,7 @@ k_plan_rowfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* _
 //   (bs[sid(v)] += fix(pint)).
 // Pass B: every visit, p_src + pint (sum[v] += fix(p_src + pint)).
 
-template <typename T, bool kFull>
+template <typename T, bool kFull, bool kTri>
 __global__ void __launch_bounds__(256)
 k_walk_plain(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
              GridDesc g, WalkParams wp, const typename Vec4<T>::type* __restrict__ field,
@@ -243,7 +243,8 @@ k_walk_plain(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
   const int64_t vin = (int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y;
   const double pin = kFull ? ps[sid[vin]] : 0.0;
   PlainAccum<kFull> vis{mine, sid, scale, pin};
-  walk_gradient<T>(rr, dirs, g, wp, field, vis);
+  if constexpr (kTri) walk_gradient_tri<T>(rr, dirs, g, wp, field, vis);
+  else walk_gradient<T>(rr, dirs, g, wp, field, vis);
 }
 
 // ---------------------------------------------------------------------------
@@ -287,31 +288,36 @@ hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst
 }
 
 template <typename T>
-hipError_t launch_walk_plain(bool full, const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                             GridDesc g, const WalkParams& wp, const void* field, const int* sid,
-                             const double* p0, const double* p1, const DevScalars* sc,
-                             long long* acc, int64_t copy_stride, int ncopies, hipStream_t st) {
+hipError_t launch_walk_plain(bool full, bool trilinear, const RayRec* rays, int64_t nrays,
+                             const DirConst* dirs, GridDesc g, const WalkParams& wp,
+                             const void* field, const int* sid, const double* p0,
+                             const double* p1, const DevScalars* sc, long long* acc,
+                             int64_t copy_stride, int ncopies, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const unsigned int grid = grid_for(nrays, 1u << 30);
   const V4* f = static_cast<const V4*>(field);
-  if (full)
-    hipLaunchKernelGGL((k_walk_plain<T, true>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs,
-                       g, wp, f, sid, p0, p1, sc, acc, copy_stride, ncopies);
-  else
-    hipLaunchKernelGGL((k_walk_plain<T, false>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs,
-                       g, wp, f, sid, p0, p1, sc, acc, copy_stride, ncopies);
+#define OMNI_WP(F, R)                                                                          \
+  hipLaunchKernelGGL((k_walk_plain<T, F, R>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, \
+                     g, wp, f, sid, p0, p1, sc, acc, copy_stride, ncopies)
+  if (full) {
+    if (trilinear) OMNI_WP(true, true);
+    else OMNI_WP(true, false);
+  } else {
+    if (trilinear) OMNI_WP(false, true);
+    else OMNI_WP(false, false);
+  }
+#undef OMNI_WP
   return hipGetLastError();
 }
 
-template hipError_t launch_walk_plain<float>(bool, const RayRec*, int64_t, const DirConst*,
-                                             GridDesc, const WalkParams&, const void*,
-                                             const int*, const double*, const double*,
-                                             const DevScalars*, long long*, int64_t, int,
-                                             hipStream_t);
-template hipError_t launch_walk_plain<double>(bool, const RayRec*, int64_t, const DirConst*,
-                                              GridDesc, const WalkParams&, const void*,
-                                              const int*, const double*, const double*,
-                                              const DevScalars*, long long*, int64_t, int,
-                                              hipStream_t);
+#define OMNI_WP_INST(T)                                                                        \
+  template hipError_t launch_walk_plain<T>(bool, bool, const RayRec*, int64_t, const DirConst*, \
+                                           GridDesc, const WalkParams&, const void*,           \
+                                           const int*, const double*, const double*,           \
+                                           const DevScalars*, long long*, int64_t, int,        \
+                                           hipStream_t);
+OMNI_WP_INST(float)
+OMNI_WP_INST(double)
+#undef OMNI_WP_INST
 
 }  // namespace omni3d

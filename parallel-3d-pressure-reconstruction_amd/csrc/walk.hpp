@@ -240,6 +240,119 @@ __device__ __forceinline__ int walk_gradient(const RayRec& rr, const DirConst* _
   return steps;
 }
 
+// Trilinear sample of the packed field (a_x, a_y, a_z) at the index-space
+// point x (0-based voxel centres).  The cell's lower corner is floor(x)
+// clamped to [0, size - 2] and the weights are not clamped, so points within
+// a voxel of the centres of the faces are extrapolated linearly and a linear
+// field is sampled exactly.  Interpolation order x, then y, then z.
+template <typename T>
+__device__ __forceinline__ void tri_sample(const typename Vec4<T>::type* __restrict__ field,
+                                           const Walker& w, const double x[3], T out[3]) {
+  using V4 = typename Vec4<T>::type;
+  const int size[3] = {w.X, w.Y, w.Z};
+  int lo[3];
+  T f[3], gw[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int fl = (int)floor(x[a]);
+    fl = fl > size[a] - 2 ? size[a] - 2 : fl;
+    fl = fl < 0 ? 0 : fl;
+    lo[a] = fl;
+    const double fd = x[a] - (double)fl;
+    f[a] = (T)fd;
+    gw[a] = (T)(1.0 - fd);
+  }
+  const int64_t b = lo[0] + lo[1] * w.sy + lo[2] * w.sz;
+  const V4 q000 = field[b], q100 = field[b + 1];
+  const V4 q010 = field[b + w.sy], q110 = field[b + w.sy + 1];
+  const V4 q001 = field[b + w.sz], q101 = field[b + w.sz + 1];
+  const V4 q011 = field[b + w.sy + w.sz], q111 = field[b + w.sy + w.sz + 1];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const T c00 = component(q000, c) * gw[0] + component(q100, c) * f[0];
+    const T c10 = component(q010, c) * gw[0] + component(q110, c) * f[0];
+    const T c01 = component(q001, c) * gw[0] + component(q101, c) * f[0];
+    const T c11 = component(q011, c) * gw[0] + component(q111, c) * f[0];
+    const T c0 = c00 * gw[1] + c10 * f[1];
+    const T c1 = c01 * gw[1] + c11 * f[1];
+    out[c] = c0 * gw[2] + c1 * f[2];
+  }
+}
+
+// Trapezoid of grad p = -rho a over a straight piece of path with index-space
+// displacement disp: -rho/2 (ga + gb) . (disp * delta).
+template <typename T>
+__device__ __forceinline__ T tri_term(const WalkParams& wp, const T ga[3], const T gb[3],
+                                      const T disp[3]) {
+  const T hr = (T)(-wp.rho * 0.5);
+  const T d0 = (T)wp.del[0], d1 = (T)wp.del[1], d2 = (T)wp.del[2];
+  return hr * (((ga[0] + gb[0]) * (disp[0] * d0) + (ga[1] + gb[1]) * (disp[1] * d1)) +
+               (ga[2] + gb[2]) * (disp[2] * d2));
+}
+
+// Gradient walk of the plain method along the straight ray (the accuracy
+// mode beside kernel.cu:1743-1863).  The visited voxels are the staircase's;
+// the integral to a visited voxel v runs in -> x_in (leg), along the line
+// from x_in to x_v with one trapezoid per staircase step (the projection
+// x = c - u of a voxel centre moves by s k_a along k), then x_v -> v (leg),
+// with the gradient sampled trilinearly on the line.  vis.visit receives the
+// carried line integral plus v's own leg.
+template <typename T, typename Visitor>
+__device__ __forceinline__ int walk_gradient_tri(const RayRec& rr,
+                                                 const DirConst* __restrict__ dirs,
+                                                 const GridDesc& g, const WalkParams& wp,
+                                                 const typename Vec4<T>::type* __restrict__ field,
+                                                 Visitor& vis) {
+  using V4 = typename Vec4<T>::type;
+  Walker w;
+  w.init(rr, dirs, g);
+  const DirConst& d = dirs[rr.dir_len & 0xffffu];
+  const double k0 = d.k[0], k1 = d.k[1], k2 = d.k[2];
+  vis.entry(w.v);
+  double x[3] = {(double)w.c0 - w.u0, (double)w.c1 - w.u1, (double)w.c2 - w.u2};
+  T sl[3], sn[3], ac[3], disp[3];
+  tri_sample<T>(field, w, x, sl);
+  V4 av = field[w.v];
+  ac[0] = av.x;
+  ac[1] = av.y;
+  ac[2] = av.z;
+  disp[0] = (T)(-w.u0);
+  disp[1] = (T)(-w.u1);
+  disp[2] = (T)(-w.u2);
+  RaySum<T> pint;
+  pint.add(tri_term<T>(wp, ac, sl, disp));  // in -> x_in
+  int steps = 0;
+  while (!w.done()) {
+    int s;
+    const int a = w.advance(&s);
+    if (a < 0) break;
+    x[0] = (double)w.c0 - w.u0;
+    x[1] = (double)w.c1 - w.u1;
+    x[2] = (double)w.c2 - w.u2;
+    tri_sample<T>(field, w, x, sn);
+    const double ka = a == 0 ? k0 : (a == 1 ? k1 : k2);
+    const double sd = (double)s;
+    disp[0] = (T)(sd * (ka * k0));
+    disp[1] = (T)(sd * (ka * k1));
+    disp[2] = (T)(sd * (ka * k2));
+    pint.add(tri_term<T>(wp, sl, sn, disp));  // x_last -> x_next along the line
+    av = field[w.v];
+    ac[0] = av.x;
+    ac[1] = av.y;
+    ac[2] = av.z;
+    disp[0] = (T)w.u0;
+    disp[1] = (T)w.u1;
+    disp[2] = (T)w.u2;
+    const T leg = tri_term<T>(wp, sn, ac, disp);  // x_next -> v
+    vis.visit(w.v, w.on_surface(), pint.value() + (double)leg);
+    sl[0] = sn[0];
+    sl[1] = sn[1];
+    sl[2] = sn[2];
+    steps++;
+  }
+  return steps;
+}
+
 // Logical block index that gives each XCD a contiguous range of blocks
 // (hardware dispatches workgroups round-robin over the 8 XCDs), so that
 // neighbouring ray patches share an L2.

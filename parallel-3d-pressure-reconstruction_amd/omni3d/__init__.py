@@ -17,6 +17,8 @@ OK = 0
 ERR_ARG, ERR_STATE, ERR_HIP, ERR_IO, ERR_NONFINITE, ERR_NODEVICE = 1, 2, 3, 4, 5, 6
 PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
 FP64, FP32_KAHAN = 0, 1
+FP32 = FP32_KAHAN
+STAIRCASE, TRILINEAR = 0, 1
 AFFINE, NAIVE = 0, 1
 F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
@@ -35,6 +37,11 @@ EXPORTS = [
     "omni3d_read_parameter_file", "omni3d_run_parameter_file_ex",
     "thredholdHistMaterialAccCPU", "omni3d_threshold_hist", "omni3d_boundary_pairs",
     "omni3d_es_inner_select_fixedbc", "omni3d_es_inner_weighted_minicurl",
+    "omni3d_set_velocity", "omni3d_dropin_release", "initialIntegration",
+    "omni3d_initial_integration", "omni3dparallellinesESInner",
+    "omni3dparallellinesESInnerWeighted", "omni3dparallellinesESInnerSelect",
+    "omni3dparallellinesESInnerSelectFixedBC", "omni3dparallellinesESInnerWeightedMiniCurl",
+    "devidecountInner",
 ]
 
 
@@ -45,7 +52,7 @@ class Params(C.Structure):
         ("density", C.c_double), ("line_spacing", C.c_double),
         ("method", C.c_int), ("threshold", C.c_double), ("faithful_select", C.c_int),
         ("precision", C.c_int), ("solver", C.c_int), ("max_iterations", C.c_int),
-        ("tolerance", C.c_double), ("seed_pn_with_p", C.c_int),
+        ("tolerance", C.c_double), ("seed_pn_with_p", C.c_int), ("traversal", C.c_int),
     ]
 
 
@@ -110,13 +117,31 @@ def _declare(L):
     L.omni3d_generate_directions.argtypes = [i, fp, fp, fp, i]
     L.omni3d_set_field.argtypes = [vp, vp, vp, vp, vp, i, i]
     L.omni3d_reconstruct.argtypes = [vp, vp, vp, i, i, C.POINTER(Report)]
-    es = [i, i, i, i, f, vp, vp, vp, f, f, f, f, vp, vp, vp]
+    lg = C.c_long
+    # the reference kernels' argument lists (kernel.cu:6151...): long dims,
+    # direction arrays, long* index, spacing, density, field
+    es = [lg, lg, lg, i, f, vp, vp, vp, vp, f, f, f, f, vp, vp, vp]
     L.omni3d_es_inner.argtypes = es + [vp, vp, vp, vp]
     L.omni3d_es_inner_weighted.argtypes = es + [vp, vp, vp, vp, vp]
     L.omni3d_es_inner_select.argtypes = es + [vp, vp, vp, vp, f, vp]
     L.omni3d_es_inner_select_fixedbc.argtypes = es + [vp, vp, vp, vp, f, vp]
     L.omni3d_es_inner_weighted_minicurl.argtypes = es + [vp, vp, vp, vp, vp]
-    L.omni3d_devide_count_inner.argtypes = [i, i, i, vp, vp, vp, vp]
+    L.omni3d_devide_count_inner.argtypes = [lg, lg, lg, vp, vp, vp, vp]
+    L.omni3d_dropin_release.argtypes = []
+    L.omni3d_dropin_release.restype = None
+    for name, extra in (("omni3dparallellinesESInner", [vp, vp, vp]),
+                        ("omni3dparallellinesESInnerWeighted", [vp, vp, vp, vp]),
+                        ("omni3dparallellinesESInnerSelect", [vp, vp, vp, vp, f]),
+                        ("omni3dparallellinesESInnerSelectFixedBC", [vp, vp, vp, vp, f]),
+                        ("omni3dparallellinesESInnerWeightedMiniCurl", [vp, vp, vp, vp])):
+        getattr(L, name).argtypes = es + extra
+        getattr(L, name).restype = None
+    L.devidecountInner.argtypes = [lg, lg, lg, vp, vp, vp]
+    L.devidecountInner.restype = None
+    L.omni3d_set_velocity.argtypes = [vp, vp, vp, vp, d, i, i]
+    L.initialIntegration.argtypes = [lg, lg, lg, d, d, d, d, dp, dp, dp, d, dp, dp, dp]
+    L.initialIntegration.restype = None
+    L.omni3d_initial_integration.argtypes = [lg, lg, lg, d, d, d, d, dp, dp, dp, d, dp, dp]
     L.omni3d_curl.argtypes = [i, i, i, f, f, f, vp, vp, vp, vp, vp, vp]
     L.thredholdHistMaterialAccCPU.argtypes = [i, i, i, C.POINTER(C.c_float), f,
                                               C.POINTER(C.c_float)]
@@ -184,6 +209,19 @@ def calrhs(shape, delta, density, acc):
     check(lib().omni3d_calrhs(X, Y, Z, delta[0], delta[1], delta[2], density, _dptr(a[0]),
                               _dptr(a[1]), _dptr(a[2]), _dptr(out)))
     return out
+
+
+def initial_integration(shape, delta, density, acc, pref, pn=None):
+    """initialIntegration on the GPU (host arrays, FP64); returns (p, pn).
+    pn: the starting pn (its centre voxel is kept), zeros if None."""
+    Z, Y, X = shape
+    a = [np.ascontiguousarray(c, dtype=np.float64) for c in acc]
+    pn = np.zeros(shape) if pn is None else np.ascontiguousarray(pn, dtype=np.float64).copy()
+    p = np.zeros(shape)
+    check(lib().omni3d_initial_integration(X, Y, Z, delta[0], delta[1], delta[2], density,
+                                           _dptr(a[0]), _dptr(a[1]), _dptr(a[2]), pref,
+                                           _dptr(p), _dptr(pn)))
+    return p, pn
 
 
 def sor3d(p, rhs, noitr, eps, bcscheme=(0, 0, 0, 0, 0, 0), pn=None):
@@ -323,6 +361,17 @@ class Context:
         self._chk(lib().omni3d_set_field(self._h, self._keep[0].ctypes.data,
                                          self._keep[1].ctypes.data, self._keep[2].ctypes.data,
                                          None if m is None else m.ctypes.data, dt, HOST))
+
+    def set_velocity(self, uvw, pref, dtype=np.float64):
+        """Bernoulli seed of the j = 0 face from host velocity arrays (uvw =
+        None removes it)."""
+        if uvw is None:
+            self._chk(lib().omni3d_set_velocity(self._h, None, None, None, 0.0, F64, HOST))
+            return
+        vel = [np.ascontiguousarray(a, dtype=dtype) for a in uvw]
+        dt = F64 if dtype == np.float64 else F32
+        self._chk(lib().omni3d_set_velocity(self._h, vel[0].ctypes.data, vel[1].ctypes.data,
+                                            vel[2].ctypes.data, pref, dt, HOST))
 
     def set_field_device(self, ptrs, mask_ptr=None, dtype=F32):
         self._chk(lib().omni3d_set_field(self._h, ptrs[0], ptrs[1], ptrs[2], mask_ptr, dtype,

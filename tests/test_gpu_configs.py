@@ -1,0 +1,248 @@
+"""The benchmark configurations run on the GPU at their stated sizes:
+
+  * Taylor-Green 32^3 (h = 2 pi / 32, 5 % noise, 42 directions of the
+    Divide2 fixture, line spacing one voxel, stop at a relative change below
+    1e-6 or 200 iterations): the FP64 affine solve equals the oracle's
+    reference schedule (rel-L2 <= 1e-12, same iteration count); FP32 ray sums
+    <= 1e-5; the red-black SOR (omega 1.8) of calRHS on the same field is
+    bit-identical to the oracle's;
+  * snapshot batches: eight uneven small snapshots (Taylor-Green, random
+    phase, 0-5 % noise) replayed from one hipGraph, each against the oracle's
+    reconstruction (FP64 <= 1e-12 with equal iteration counts, FP32 <= 1e-5);
+    one graph over the four grids 64^3, 96^3, 128 x 96 x 64 and
+    160 x 128 x 96 with 162 directions, each snapshot bit-identical to a lone
+    context, and per grid a quadratic pressure reproduced from its exact
+    seed;
+  * the adjoint of the 128^3 FP32 plain-mask reconstruction of the masked
+    Fourier sphere (30 iterations): central differences on 64 random unit
+    directions in grad-p space against the backward (rel <= 1e-5 in FP32),
+    the FP64 transpose identity (rel <= 1e-12), and a 50-step fit of a
+    gradient correction to 512 probes with Tikhonov lambda = 1e-3.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import omni3d
+from omni3d import fields as F
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_taylor_green_32(gpu_available, ref_dirs2, precision):
+    shape = (32, 32, 32)
+    print(f"case taylor-green 32^3 precision={precision}", flush=True)
+    f = F.taylor_green(32, noise=0.05, seed=0)
+    acc = f["acc"]
+    if precision == omni3d.FP32_KAHAN:
+        acc = [np.asarray(a, np.float32).astype(np.float64) for a in acc]
+    with omni3d.Context(shape, f["delta"], precision=precision, max_iterations=200,
+                        tolerance=1e-6) as ctx:
+        ctx.set_directions(ref_dirs2)
+        ctx.set_field(acc)
+        p, w, rep = ctx.reconstruct()
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc,
+                                    max_iter=200, tol=1e-6)
+    np.testing.assert_array_equal(w, co)
+    err = F.rel_l2(p, po)
+    e_an = F.rel_l2(F.demean(p), F.demean(f["p"]))
+    print(f"taylor-green 32^3: {rep.iterations} iterations (oracle {it}), rel-L2 vs oracle "
+          f"{err:.2e}, vs analytic {e_an:.3e}")
+    if precision == omni3d.FP64:
+        assert rep.iterations == it
+        assert err < 1e-12
+    else:
+        assert abs(rep.iterations - it) <= 1
+        assert err < 1e-5
+    assert e_an < 0.05
+
+
+def test_taylor_green_32_sor(gpu_available):
+    shape = (32, 32, 32)
+    print("case taylor-green 32^3 red-black SOR", flush=True)
+    f = F.taylor_green(32, noise=0.05, seed=0)
+    rhs_g = omni3d.calrhs(shape, f["delta"], f["rho"], f["acc"])
+    rhs_o = O.calrhs(shape, f["delta"], f["rho"], f["acc"])
+    np.testing.assert_array_equal(rhs_g, rhs_o)
+    rhs = rhs_o - rhs_o.mean()
+    p, it, res = omni3d.sor_solve(rhs, omega=1.8, max_iter=150)
+    po = O.sor_redblack(np.zeros(shape), rhs, 150, 1.8)
+    np.testing.assert_array_equal(p, po)
+    e_an = F.rel_l2(F.demean(p), F.demean(f["p"]))
+    print(f"SOR 150 iterations: residual {res:.3e}, rel-L2 vs analytic {e_an:.3e}")
+
+
+def small_series(seed):
+    rng = np.random.default_rng(seed)
+    shapes = [(16, 18, 20), (20, 16, 18), (18, 20, 16), (16, 16, 16), (20, 20, 18),
+              (16, 18, 20), (18, 16, 20), (20, 18, 16)]
+    fields = []
+    for s in shapes:
+        ph = tuple(rng.uniform(0, 2 * np.pi, 3))
+        fields.append(F.taylor_green(20, noise=rng.uniform(0, 0.05),
+                                     seed=int(rng.integers(1 << 30)), shape=s, phase=ph))
+    return shapes, fields
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_batch_against_oracle(gpu_available, ref_dirs2, precision):
+    print(f"case batch vs oracle precision={precision}", flush=True)
+    shapes, fields = small_series(5)
+    h = 2 * np.pi / 20
+    kw = dict(max_iterations=40, tolerance=1e-5, precision=precision)
+    accs = []
+    for fl in fields:
+        a = fl["acc"]
+        if precision == omni3d.FP32_KAHAN:
+            a = [np.asarray(c, np.float32).astype(np.float64) for c in a]
+        accs.append(a)
+    with omni3d.Batch(shapes, (h, h, h), **kw) as b:
+        b.set_directions(ref_dirs2)
+        for i, a in enumerate(accs):
+            b.set_field(i, a)
+        b.run(use_graph=True)
+        b.run(use_graph=True)
+        res = [b.result(i) for i in range(len(shapes))]
+    for i, s in enumerate(shapes):
+        po, _, it, _ = O.reconstruct(s, ref_dirs2, 1.0, (h, h, h), 1.0, accs[i],
+                                     max_iter=40, tol=1e-5)
+        p, rep = res[i]
+        err = F.rel_l2(p, po)
+        if precision == omni3d.FP64:
+            assert rep.iterations == it, (i, rep.iterations, it)
+            assert err < 1e-12, (i, err)
+        else:
+            assert abs(rep.iterations - it) <= 1, (i, rep.iterations, it)
+            assert err < 1e-5, (i, err)
+
+
+CFG4_SHAPES = [(64, 64, 64), (96, 96, 96), (64, 96, 128), (96, 128, 160)]
+
+
+def test_batch_cfg4_grids(gpu_available, ref_dirs3):
+    print("case batch cfg4 grids", flush=True)
+    rng = np.random.default_rng(11)
+    shapes = CFG4_SHAPES + CFG4_SHAPES[::-1]
+    fields = []
+    for s in shapes:
+        ph = tuple(rng.uniform(0, 2 * np.pi, 3))
+        fields.append(F.taylor_green(s[2], noise=rng.uniform(0, 0.05),
+                                     seed=int(rng.integers(1 << 30)), shape=s, phase=ph))
+    kw = dict(max_iterations=200, tolerance=1e-6, precision=omni3d.FP32_KAHAN)
+    # one spacing for the batch (the snapshots share every parameter but the grid)
+    h = (2 * np.pi / 64,) * 3
+    with omni3d.Batch(shapes, h, **kw) as b:
+        b.set_directions(ref_dirs3)
+        for i, fl in enumerate(fields):
+            b.set_field(i, fl["acc"])
+        b.run(use_graph=True)
+        res = [b.result(i) for i in range(len(shapes))]
+    for i in (0, 1, 2, 3):
+        with omni3d.Context(shapes[i], h, **kw) as ctx:
+            ctx.set_directions(ref_dirs3)
+            ctx.set_field(fields[i]["acc"])
+            p, w, rep = ctx.reconstruct()
+        np.testing.assert_array_equal(res[i][0], p)
+        assert res[i][1].iterations == rep.iterations
+        print(f"cfg4 grid {shapes[i]}: {rep.iterations} iterations, bit-equal to a lone context")
+    for s in CFG4_SHAPES:
+        q = F.quadratic(s, seed=3, delta=(0.01, 0.012, 0.014), rho=1.0)
+        with omni3d.Context(s, q["delta"], max_iterations=2, seed_pn_with_p=0) as ctx:
+            ctx.set_directions(ref_dirs3)
+            ctx.set_field(q["acc"])
+            p, w, rep = ctx.reconstruct(q["p"])
+        m = w > 0
+        err = np.abs(p[m] - q["p"][m]).max() / np.abs(q["p"]).max()
+        assert err < 1e-9, (s, err)
+
+
+class Adjoint:
+    def __init__(self, shape, dirs, delta, K, mask, precision):
+        import torch
+        self.V = int(np.prod(shape))
+        self.dt = torch.float64 if precision == omni3d.FP64 else torch.float32
+        self.ctx = omni3d.Context(shape, delta, method=omni3d.PLAIN_MASK, max_iterations=K,
+                                  precision=precision)
+        self.ctx.set_directions(dirs)
+        self.ctx.set_field([np.zeros(shape)] * 3, mask=mask)
+
+    def forward(self, g):
+        import torch
+        g = g.to(self.dt).contiguous()
+        p = torch.empty(self.V, dtype=self.dt, device="cuda")
+        self.ctx.forward_device(g.data_ptr(), p.data_ptr())
+        torch.cuda.synchronize()
+        return p
+
+    def backward(self, y):
+        import torch
+        y = y.to(self.dt).contiguous()
+        dg = torch.empty(3 * self.V, dtype=self.dt, device="cuda")
+        self.ctx.backward_device(y.data_ptr(), dg.data_ptr())
+        torch.cuda.synchronize()
+        return dg
+
+
+def test_adjoint_fourier_sphere_128_fp32(gpu_available, ref_dirs3):
+    import torch
+    n = 128
+    print("case adjoint 128^3 FP32 plain-mask", flush=True)
+    f = F.fourier_sphere(n, seed=0)
+    shape = (n, n, n)
+    K = 30
+    A = Adjoint(shape, ref_dirs3, f["delta"], K, f["mask"], omni3d.FP32_KAHAN)
+    g0 = torch.from_numpy(np.concatenate([c.reshape(-1) for c in f["grad"]])).cuda().float()
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    y = torch.randn(A.V, dtype=torch.float32, device="cuda", generator=gen)
+    dLdg = A.backward(y).double()
+    eps = float(torch.linalg.norm(g0))
+    worst = 0.0
+    for _ in range(64):
+        d = torch.randn(3 * A.V, dtype=torch.float32, device="cuda", generator=gen)
+        d /= torch.linalg.norm(d)
+        fd = float(torch.dot(y.double(),
+                             (A.forward(g0 + eps * d).double() - A.forward(g0 - eps * d).double()))
+                   ) / (2 * eps)
+        an = float(torch.dot(dLdg, d.double()))
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-30))
+    print(f"128^3 FP32 finite differences, 64 directions: worst rel error {worst:.2e}")
+    assert worst < 1e-5
+    # FP64 transpose identity at the same size
+    A64 = Adjoint(shape, ref_dirs3, f["delta"], K, f["mask"], omni3d.FP64)
+    g = torch.randn(3 * A.V, dtype=torch.float64, device="cuda", generator=gen)
+    yy = torch.randn(A.V, dtype=torch.float64, device="cuda", generator=gen)
+    lhs = float(torch.dot(yy, A64.forward(g)))
+    rhs = float(torch.dot(A64.backward(yy), g))
+    print(f"128^3 FP64 transpose identity: {lhs:.15e} vs {rhs:.15e}")
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs) * 10
+    del A64
+
+    class Recon(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, g):
+            return A.forward(g)
+
+        @staticmethod
+        def backward(ctx, dp):
+            return A.backward(dp)
+
+    p_true = torch.from_numpy(f["p"].reshape(-1)).cuda().float()
+    rng = np.random.default_rng(4)
+    valid = np.nonzero(f["mask"].reshape(-1) > 0)[0]
+    probes = torch.from_numpy(rng.choice(valid, 512, replace=False)).cuda()
+    delta = torch.zeros_like(g0, requires_grad=True)
+    opt = torch.optim.Adam([delta], lr=0.02)
+    lam = 1e-3
+    losses = []
+    for _ in range(50):
+        opt.zero_grad()
+        p = Recon.apply(g0 + delta)
+        r = p[probes] - p_true[probes]
+        r = r - r.mean()
+        loss = (r * r).mean() + lam * (delta * delta).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss.detach()))
+    print(f"128^3 probe fit: loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.5 * losses[0]

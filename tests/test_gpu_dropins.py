@@ -39,7 +39,7 @@ def es_call(which, shape, dirs, f, p, curl=None, thr=0.0, pn=None, pc=None):
     pnd = dev(np.zeros(shape) if pn is None else pn)
     pcd = dev(np.zeros(shape) if pc is None else pc)
     L = omni3d.lib()
-    args = [X, Y, Z, len(dirs), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(),
+    args = [X, Y, Z, len(dirs), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(), None,
             f["delta"][0], f["delta"][1], f["delta"][2], f["rho"],
             a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr()]
     if which == "plain":

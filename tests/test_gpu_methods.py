@@ -153,7 +153,7 @@ def test_es_inner_select_fixedbc_dropin(gpu_available, ref_dirs2):
     pcd = torch.zeros(shape, dtype=torch.float32, device="cuda")
     dl = [float(np.float32(d)) for d in f["delta"]]
     omni3d.check(omni3d.lib().omni3d_es_inner_select_fixedbc(
-        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(),
+        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(), None,
         dl[0], dl[1], dl[2], float(np.float32(f["rho"])), a[0].data_ptr(), a[1].data_ptr(),
         a[2].data_ptr(), cd.data_ptr(), pd.data_ptr(), pnd.data_ptr(), pcd.data_ptr(), thr, None))
     torch.cuda.synchronize()
@@ -213,7 +213,7 @@ def test_es_inner_weighted_minicurl_dropin(gpu_available, ref_dirs2):
     pnd = torch.zeros(shape, dtype=torch.float32, device="cuda")
     pcd = torch.zeros(shape, dtype=torch.float32, device="cuda")
     omni3d.check(omni3d.lib().omni3d_es_inner_weighted_minicurl(
-        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(),
+        X, Y, Z, len(ref_dirs2), 1.0, k[0].data_ptr(), k[1].data_ptr(), k[2].data_ptr(), None,
         dl[0], dl[1], dl[2], rho, a[0].data_ptr(), a[1].data_ptr(), a[2].data_ptr(),
         cd.data_ptr(), pd.data_ptr(), pnd.data_ptr(), pcd.data_ptr(), None))
     torch.cuda.synchronize()

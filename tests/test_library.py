@@ -25,7 +25,11 @@ def test_header_declares_the_reference_entry_points():
     names = declared_functions()
     for n in ("omni3d_create", "omni3d_reconstruct", "omni3d_es_inner", "omni3d_es_inner_select",
               "omni3d_devide_count_inner", "sor3D", "calRHS", "omni3d_forward",
-              "omni3d_backward", "omni3d_run_parameter_file"):
+              "omni3d_backward", "omni3d_run_parameter_file", "initialIntegration",
+              "omni3dparallellinesESInner", "omni3dparallellinesESInnerWeighted",
+              "omni3dparallellinesESInnerSelect", "omni3dparallellinesESInnerSelectFixedBC",
+              "omni3dparallellinesESInnerWeightedMiniCurl", "devidecountInner",
+              "omni3d_set_velocity"):
         assert n in names
 
 
@@ -43,6 +47,30 @@ def test_default_params_match_reference_defaults():
     assert p.method == omni3d.PLAIN and p.precision == omni3d.FP64
     assert p.max_iterations == 10            # Parameter_Omni3D.dat line 14
     assert p.seed_pn_with_p == 1             # kernel.cu:8813
+    assert p.traversal == omni3d.STAIRCASE   # kernel.cu:1764-1840
+
+
+def test_trilinear_needs_plain_affine():
+    for kw in ({"method": omni3d.SELECT}, {"method": omni3d.PLAIN_MASK},
+               {"solver": omni3d.NAIVE}, {"traversal": 2}):
+        with pytest.raises(omni3d.Omni3DError) as e:
+            omni3d.Context((8, 8, 8), **dict({"traversal": omni3d.TRILINEAR}, **kw))
+        assert e.value.code == omni3d.ERR_ARG
+
+
+def test_reference_signatures_checked_before_device_work():
+    L = omni3d.lib()
+    # initialIntegration / omni3d_initial_integration: grid and pointers first
+    assert L.omni3d_initial_integration(0, 4, 4, 1.0, 1.0, 1.0, 1.0, None, None, None, 0.0,
+                                        None, None) == omni3d.ERR_ARG
+    # the reference-order drop-ins: long dims, then the index pointer (not
+    # read); a null field pointer is an argument error
+    es = [8, 8, 8, 1, 1.0, None, None, None, None, 1.0, 1.0, 1.0, 1.0, None, None, None]
+    assert L.omni3d_es_inner(*es, None, None, None, None) == omni3d.ERR_ARG
+    assert L.omni3d_es_inner_select(*es, None, None, None, None, 0.5, None) == omni3d.ERR_ARG
+    # the void reference-named forms report through the global error
+    L.omni3dparallellinesESInner(*es, None, None, None)
+    assert b"null" in L.omni3d_last_global_error()
 
 
 def test_no_gpu_is_an_error_not_a_fallback():
@@ -78,7 +106,7 @@ def test_bad_arguments_rejected_before_any_device_work():
     assert L.omni3d_boundary_pairs(None, 4, C.c_void_p(buf.ctypes.data), omni3d.F32,
                                    omni3d.HOST, None) == omni3d.ERR_ARG
     # the variant launchers check their pointers before any device work
-    es = [8, 8, 8, 1, 1.0, None, None, None, 1.0, 1.0, 1.0, 1.0, None, None, None]
+    es = [8, 8, 8, 1, 1.0, None, None, None, None, 1.0, 1.0, 1.0, 1.0, None, None, None]
     assert L.omni3d_es_inner_select_fixedbc(*es, None, None, None, None, 0.5,
                                             None) == omni3d.ERR_ARG
     assert L.omni3d_es_inner_weighted_minicurl(*es, None, None, None, None,
