@@ -316,6 +316,18 @@ void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, floa
                                  float* threshold);
 int omni3d_threshold_hist(int X, int Y, int Z, const float* curl, float percentage,
                           float* threshold, int mem_kind, void* hip_stream);
+/* The same threshold with nothing allocated, for callers that cannot call
+ * hipMalloc (code under HIP graph capture, serving loops); no counterpart in
+ * the reference.  curl_d and threshold_d (one float) are device buffers; the
+ * work is queued on hip_stream and the call returns without waiting.  The
+ * min / max pair and the 10000 bins live in a fixed 40,012-byte scratch array
+ * in device memory, one per GPU, looked up on the first call on that GPU and
+ * cleared by every call; calls that overlap on one GPU (e.g. on two streams
+ * at once) race on it.  Volumes above 2^31 - 1 voxels are rejected
+ * (OMNI3D_ERR_ARG). */
+int omni3d_threshold_hist_noalloc(int X, int Y, int Z, const float* curl_d, float percentage,
+                                  float* threshold_d, void* hip_stream);
+
 /* Boundary-pair pipeline of the README (README:18-28; kernels
  * omni3dparallellinesEqualSpacing kernel.cu:3414-3489 with bodyIntegral
  * kernel.cu:1353-1467, devidecount kernel.cu:3896-3909, BCiteration

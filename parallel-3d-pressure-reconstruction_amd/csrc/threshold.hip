@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "../../include/omni3d.h"
@@ -168,6 +169,37 @@ __global__ void __launch_bounds__(1024) k_thr_pick(const unsigned int* mm,
   if (t == 0) out[0] = best >= 0 ? edge(best) : hi;
 }
 
+// Scratch of the allocation-free entry point: the min / max pair and the
+// kBins bins, then one spare word (40,012 bytes).  Each GPU the library runs
+// on holds its own copy until the process ends.  Host code uses only the
+// device address hipGetSymbolAddress returns for the current device (the
+// array's name in host code is a stand-in, not a device address).
+constexpr int kScratchWords = 2 + kBins + 1;
+static __device__ __align__(256) unsigned int scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned int* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's scratch address, looked up once per device.
+int scratch_address(unsigned int** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(nullptr, OMNI3D_ERR_NODEVICE, hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, OMNI3D_ERR_ARG, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(nullptr, OMNI3D_ERR_HIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned int*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return OMNI3D_OK;
+}
+
 #define THR_TRY(expr)                                                           \
   do {                                                                          \
     hipError_t _e = (expr);                                                     \
@@ -221,10 +253,43 @@ int threshold_impl(int X, int Y, int Z, const float* curl, float percentage, flo
   return OMNI3D_OK;
 }
 
+// The four kernels on device buffers with the given bins / min-max / result
+// scratch (mm: 2 words, then kBins bins); the result goes to out (device).
+void launch_threshold(const float* src, int64_t V, float need, unsigned int* mm, float* out,
+                      hipStream_t st) {
+  unsigned int* hist = mm + 2;
+  const unsigned int nb = grid_for((V + 15) / 16, 512);  // >= 16 voxels per lane
+  hipLaunchKernelGGL(k_thr_init, dim3(10), dim3(1024), 0, st, mm, hist);
+  hipLaunchKernelGGL(k_thr_minmax, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm);
+  hipLaunchKernelGGL(k_thr_hist, dim3(nb), dim3(kThrBlock), 0, st, src, V, mm, hist);
+  hipLaunchKernelGGL(k_thr_pick, dim3(1), dim3(1024), 0, st, mm, hist, V, need, out);
+}
+
+int threshold_noalloc(int X, int Y, int Z, const float* curl, float percentage, float* threshold,
+                      hipStream_t st) {
+  if (X < 1 || Y < 1 || Z < 1 || (int64_t)X * Y * Z > INT32_MAX)
+    return fail(nullptr, OMNI3D_ERR_ARG,
+                "bad grid dimensions (at most 2^31 - 1 voxels: the bins count in 32 bits)");
+  if (!curl || !threshold) return fail(nullptr, OMNI3D_ERR_ARG, "null pointer");
+  unsigned int* mm = nullptr;
+  const int rc = scratch_address(&mm);
+  if (rc) return rc;
+  const int64_t V = (int64_t)X * Y * Z;
+  launch_threshold(curl, V, (float)(int)V * percentage, mm, threshold, st);
+  THR_TRY(hipGetLastError());
+  return OMNI3D_OK;
+}
+
 }  // namespace
 }  // namespace omni3d
 
 extern "C" {
+
+int omni3d_threshold_hist_noalloc(int X, int Y, int Z, const float* curl_d, float percentage,
+                                  float* threshold_d, void* hip_stream) {
+  return omni3d::threshold_noalloc(X, Y, Z, curl_d, percentage, threshold_d,
+                                   static_cast<hipStream_t>(hip_stream));
+}
 
 void thredholdHistMaterialAccCPU(int Imax, int Jmax, int Kmax, float* curl, float percentage,
                                  float* threshold) {
