@@ -225,6 +225,40 @@ def methods_mode(args):
         # setup tile pass), once per reconstruction
         out[name]["ms_rule_replay_per_field"] = round(float(np.mean(setup)), 3)
         log(f"bench: {name} {m:.2f} ms per pass")
+    # whole reconstructions of the masked methods: the affine schedule (the
+    # iteration on the run-start voxels, then one full pass) against the
+    # reference schedule (a full traversal per iteration), same field
+    recon = {}
+    for name, method in (("plain-mask", omni3d.PLAIN_MASK), ("select", omni3d.SELECT),
+                         ("weighted", omni3d.WEIGHTED)):
+        kw = {"threshold": thr} if method == omni3d.SELECT else {}
+        row = {}
+        for iters in (10, 200):
+            for solver, sname in ((omni3d.AFFINE, "affine"), (omni3d.NAIVE, "reference")):
+                ctx = omni3d.Context(shape, f["delta"], density=f["rho"],
+                                     precision=omni3d.FP32_KAHAN, method=method, solver=solver,
+                                     max_iterations=iters, tolerance=0.0, **kw)
+                ctx.set_directions(dirs)
+                ctx.set_field(f["acc"], mask=f["mask"])
+                ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r = ctx.reconstruct_device(p_dev.data_ptr(), None, omni3d.F64)
+                torch.cuda.synchronize()
+                wall = time.perf_counter() - t0
+                ctx.close()
+                e = {"ms_solve": round(r.ms_solve, 3), "ms_wall": round(wall * 1e3, 3)}
+                if solver == omni3d.AFFINE:
+                    e.update({"ms_rows_and_b": round(r.ms_pass_a, 3),
+                              "ms_iterate": round(r.ms_iterate, 3),
+                              "ms_final_pass": round(r.ms_pass_b, 3),
+                              "run_starts": int(r.sources), "row_entries": int(r.source_visits)})
+                row[f"{sname}_K{iters}"] = e
+                log(f"bench: {name} {sname} K={iters}: {r.ms_solve:.2f} ms")
+        row["speedup_K10"] = round(row["reference_K10"]["ms_solve"] / row["affine_K10"]["ms_solve"], 2)
+        row["speedup_K200"] = round(row["reference_K200"]["ms_solve"] /
+                                    row["affine_K200"]["ms_solve"], 2)
+        recon[name] = row
     first = out["select"]
     cpu = None
     if not args.no_cpu:
@@ -264,7 +298,8 @@ def methods_mode(args):
         "config": {"workload": f"fourier-sphere 128^3 with the sphere masked, {D} directions, "
                                f"one full traversal per iteration, {passes} iterations timed; "
                                f"tile engine (rule replay once per field, carry chain + tile pass per iteration)",
-                   "select_threshold": thr, "methods": out}}))
+                   "select_threshold": thr, "methods": out,
+                   "reconstructions": recon}}))
 
 
 SOR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sor256.json")
@@ -272,6 +307,15 @@ SOR_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_sor256.json")
 
 def sor_mode(args):
     """GPU red-black SOR on device buffers: 256^3, omega 1.8, fixed sweeps."""
+    print(json.dumps(sor_measure(args)))
+
+
+def sor_measure(args):
+    """The Poisson path's numbers (the --sor line; a summary rides on the
+    default line as "poisson"): GPU red-black SOR per iteration with its
+    roofline, the oracle's serial red-black SOR and its restatement of the
+    reference's CPU solver sor3D (damped Jacobi, weight 0.8) per sweep on
+    the same right-hand side, and sor3D's drop-in on host buffers."""
     import torch
     import omni3d
     from omni3d import fields as F
@@ -331,7 +375,7 @@ def sor_mode(args):
     tg = time.perf_counter() - t0
     jacobi_gpu = {"sweeps": sw, "s_total_host_buffers": round(tg, 4),
                   "note": "sor3D signature on the GPU: damped Jacobi, weight 0.8, host buffers"}
-    print(json.dumps({
+    return {
         "metric": "ms per red-black SOR iteration, 256^3", "value": round(ms_it, 4), "unit": "ms",
         "n_gpus": 1, "steps": iters, "higher_is_better": False, "dtype": "f64",
         "data": "synthetic",
@@ -342,7 +386,7 @@ def sor_mode(args):
         "cpu_baseline": cpu, "gpu_sor3D_dropin": jacobi_gpu,
         "config": {"workload": f"Neumann right-hand side of the burgers-vortex {n}^3 field, "
                                f"omega 1.8, {iters} iterations from p = 0",
-                   "residual": res, "iterations": it}}))
+                   "residual": res, "iterations": it}}
 
 
 def pairs_mode(args):
@@ -478,6 +522,7 @@ def main():
     ap.add_argument("--cpu-full", action="store_true",
                     help="time the oracle's whole pass (every direction) instead of a sample")
     ap.add_argument("--no-trilinear", action="store_true")
+    ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--naive-passes", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--batch", action="store_true")
@@ -697,6 +742,23 @@ def main():
                "measured_s": round(ts, 3), "s_per_pass": round(per_pass, 3),
                "directions_timed": len(sub), **host_info()}
 
+    # the Poisson path beside it: GPU red-black SOR and the CPU solvers
+    poisson = None
+    if not args.no_poisson:
+        log("bench: Poisson path (red-black SOR, CPU red-black SOR and sor3D)")
+        sm = sor_measure(args)
+        c = sm.get("cpu_baseline") or {}
+        poisson = {
+            "workload": sm["config"]["workload"],
+            "gpu_ms_per_sor_iteration": sm["value"],
+            "roofline_frac": sm["roofline"]["frac"],
+            "cpu_red_black_sor_ms_per_iteration": c.get("value"),
+            "cpu_reference_jacobi_w0.8_ms_per_sweep":
+                (c.get("reference_jacobi_w0.8") or {}).get("value"),
+            "cpu_cores": c.get("cores"), "cpu_kind": c.get("kind"),
+            "gpu_sor3D_dropin": sm["gpu_sor3D_dropin"],
+        }
+
     out = {
         "metric": "s per 256^3 reconstruction at rel-L2<=1e-3 vs FP64 oracle; % HBM roofline",
         "value": round(s_per, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
@@ -730,6 +792,7 @@ def main():
                     "arithmetic; against the reference's schedule in FP64 (a full traversal "
                     "per iteration) the ratio is speedup_vs_fp64_reference_schedule"},
         "trilinear": tri,
+        "poisson": poisson,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

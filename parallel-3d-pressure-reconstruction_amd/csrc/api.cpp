@@ -1012,9 +1012,306 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   return OMNI3D_OK;
 }
 
-// Queues a whole reconstruction of the context's field on its stream (no
-// host synchronisation; every buffer must already have its size when this
-// runs under graph capture).  The result goes to pdst / wdst (device, dtype).
+// The masked methods (plain-mask, Select, Weighted) on their affine
+// schedule (masked.hip): the rule replay of the field, the run starts S and
+// the rows of the visits landing on them, the pass from p = 0 (the constant
+// b), the iteration on S, and one full pass from p_S^{k*-1}.  Sets
+// *fallback (nothing of the result queued) when the rows do not fit the
+// iteration's fixed-point sums (a voxel visited 4096 times or more) or there
+// is no run start, for the caller to run the reference schedule.
+template <typename T>
+int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype,
+                      bool* fallback) {
+  *fallback = false;
+  const omni3d_params& P = ctx->prm;
+  Plan& pl = *ctx->plan;
+  if (pl.cmax >= 4096u || pl.nval <= 0 || P.max_iterations <= 0) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  const int m = P.method;
+  DevScalars* sc = ctx->scal.as<DevScalars>();
+  const void* field = ctx->field.as<void>();
+  const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
+  WalkParams wp = walk_params(P);
+  OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
+  if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
+  const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
+  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
+                                         (double)P.max_iterations, st));
+  // the field the path-break rule tests
+  OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
+  const double* mask = ctx->have_mask ? ctx->mask.as<double>() : nullptr;
+  if (m == OMNI3D_METHOD_PLAIN_MASK) {
+    if (mask)
+      OMNI_TRY_HIP(ctx, (launch_convert<double, double>(mask, ctx->aux.as<double>(), V, st)));
+    else
+      OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->aux.as<double>(), V, 1.0, st));
+    wp.thr = INFINITY;
+    wp.faithful = 0;
+  } else {
+    OMNI_TRY_HIP(ctx, launch_curl_packed<T>(P.nx, P.ny, P.nz, P.dx, P.dy, P.dz, field, mask,
+                                            ctx->aux.as<double>(), st));
+  }
+  const double* aux = ctx->aux.as<double>();
+  const bool fixed = m == OMNI3D_METHOD_PLAIN_MASK;  // unit weights
+  const size_t nv = (size_t)pl.nval;
+  OMNI_TRY_HIP(ctx, ctx->mk_e.alloc(sizeof(double) * nv));
+  OMNI_TRY_HIP(ctx, ctx->mk_s.alloc(sizeof(double) * nv));
+  OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));
+  OMNI_TRY_HIP(ctx, ctx->mk_r.alloc(sizeof(uint32_t) * nv));
+  OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
+  OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (fixed ? 1 : nv)));
+  OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (fixed ? 1 : nv)));
+  OMNI_TRY_HIP(ctx, ctx->am_run.alloc(sizeof(uint32_t) * nv));
+  OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->pb.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
+  if (fixed) {
+    OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
+    OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+  } else {
+    OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
+    OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
+  }
+  long long* sumfx = fixed ? ctx->sum.as<long long>() : nullptr;
+  double* sumd = fixed ? nullptr : ctx->sumd.as<double>();
+  unsigned int* wcnt = fixed ? ctx->wcnt.as<unsigned int>() : nullptr;
+  double* wsum = fixed ? nullptr : ctx->wsum.as<double>();
+  double* pa = ctx->pa.as<double>();
+  double* pz = ctx->pb.as<double>();
+  if (pinit)
+    OMNI_TRY_HIP(ctx, (launch_convert<double, double>(pinit, pa, V, st)));
+  else
+    OMNI_TRY_HIP(ctx, launch_fill_f64(pa, V, 0.0, st));
+  OMNI_TRY_HIP(ctx, launch_fill_f64(pz, V, 0.0, st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
+  record_event(ctx, 2);
+  // the rule replay: first-visit records, weights, run starts
+  OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                        pl.g, pl.tg, wp, field, aux,
+                                        pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                        pl.tile_ptr.as<unsigned long long>(),
+                                        pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
+                                        ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                        ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                        ctx->mk_wn.as<double>(), wcnt, wsum,
+                                        ctx->srcf.as<uint8_t>(), st));
+  record_event(ctx, 8);
+  // S: compact ids of the run starts (the voxels a pass reads p at)
+  OMNI_TRY_HIP(ctx, ctx->am_sid.alloc(sizeof(int) * V));
+  OMNI_TRY_HIP(ctx, ctx->am_slist.alloc(sizeof(int) * (V + 1)));
+  DevBuf dns;
+  OMNI_TRY_HIP(ctx, dns.alloc(sizeof(int)));
+  OMNI_TRY_HIP(ctx, launch_source_ids(ctx->srcf.as<uint8_t>(), V, ctx->am_sid.as<int>(),
+                                      ctx->am_slist.as<int>(), dns.as<int>(), st));
+  int nS = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, dns.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (nS <= 0) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  const int* sid = ctx->am_sid.as<int>();
+  const int* slist = ctx->am_slist.as<int>();
+  // the run each segment continues, then the rows (count, scan, fill)
+  OMNI_TRY_HIP(ctx, launch_mask_runsrc(pl.nrays, pl.nseg.as<unsigned int>(),
+                                       pl.val_off.as<unsigned long long>(),
+                                       ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
+                                       ctx->am_run.as<uint32_t>(), st));
+  DevBuf rcnt;
+  OMNI_TRY_HIP(ctx, rcnt.alloc(sizeof(unsigned int) * ((size_t)nS + 1)));
+  OMNI_TRY_HIP(ctx, ctx->am_ptr.alloc(sizeof(unsigned long long) * ((size_t)nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(rcnt.as<void>(), (int64_t)nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_tile_rows(false, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                     pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                     ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                     ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(), sid,
+                                     rcnt.as<unsigned int>(), nullptr, nullptr, nullptr, st));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(rcnt.as<unsigned int>(), nS,
+                                       ctx->am_ptr.as<unsigned long long>(), st));
+  unsigned long long nnz = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, ctx->am_ptr.as<unsigned long long>() + nS, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (nnz >= (1ull << 31)) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  OMNI_TRY_HIP(ctx, ctx->am_src.alloc(sizeof(unsigned int) * (nnz + 1)));
+  OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (fixed ? 1 : nnz + 1)));
+  OMNI_TRY_HIP(ctx, launch_tile_rows(true, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                     pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                     ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                     ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(), sid,
+                                     rcnt.as<unsigned int>(), ctx->am_ptr.as<unsigned long long>(),
+                                     ctx->am_src.as<unsigned int>(), ctx->am_w.as<double>(), st));
+  // unit weights (plain-mask): the rows as the merged-word layout of the
+  // plain method's source iteration (k_iterate_grp: groups of 256 rows,
+  // words sorted by source, equal entries merged); curl weights: per-row
+  // lists (k_masked_iterate)
+  const bool grouped = fixed && nS < (1 << 22) && nnz > 0;
+  unsigned int nwords = 0;
+  if (grouped) {
+    DevBuf keys, ka, kb, tmp, flag, excl;
+    OMNI_TRY_HIP(ctx, keys.alloc(sizeof(unsigned long long) * nnz));
+    OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
+    OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * nnz));
+    OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)nnz)));
+    OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * nnz));
+    OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * nnz));
+    // slack after the words: the iteration stages them with 16-byte loads
+    OMNI_TRY_HIP(ctx, ctx->am_dent.alloc(sizeof(unsigned int) * (nnz + 8)));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_dent.as<void>(), (int64_t)nnz + 8, 0u, st));
+    const int ng = (nS + 255) / 256;
+    OMNI_TRY_HIP(ctx, ctx->am_wptr.alloc(sizeof(unsigned int) * ((size_t)ng + 1)));
+    OMNI_TRY_HIP(ctx, launch_widen_keys((int64_t)nnz, ctx->am_src.as<unsigned int>(),
+                                        keys.as<unsigned long long>(), st));
+    OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, 4, ctx->am_ptr.as<unsigned long long>(),
+                                          keys.as<unsigned long long>(),
+                                          ka.as<unsigned long long>(), kb.as<unsigned long long>(),
+                                          tmp.as<void>(), tmp.size(),
+                                          ctx->am_dent.as<unsigned int>(), st));
+    OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)nnz, 4, ctx->am_ptr.as<unsigned long long>(),
+                                          kb.as<unsigned long long>(), flag.as<unsigned int>(),
+                                          excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
+                                          ctx->am_dent.as<unsigned int>(),
+                                          ctx->am_wptr.as<unsigned int>(), &nwords, st));
+  } else {
+    OMNI_TRY_HIP(ctx, ctx->am_wmax.alloc(sizeof(unsigned long long)));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_wmax.as<void>(), 2, 0u, st));
+    if (nnz > 0 && !fixed)
+      OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->am_w.as<double>(), (int64_t)nnz,
+                                              ctx->am_wmax.as<unsigned long long>(), st));
+    if (fixed) {
+      // unit weights beyond the word format: w = 1 per entry
+      OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (nnz + 1)));
+      OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->am_w.as<double>(), (int64_t)nnz + 1, 1.0, st));
+      OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->am_w.as<double>(), 1,
+                                              ctx->am_wmax.as<unsigned long long>(), st));
+    }
+  }
+  // b: the pass from p = 0, read at the run starts, and the weights W there
+  OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
+                                      pl.val_off.as<unsigned long long>(), ctx->mk_e.as<double>(),
+                                      ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                      ctx->mk_f.as<uint32_t>(), pz, pz, 1, sc,
+                                      ctx->mk_c.as<double>(), st));
+  OMNI_TRY_HIP(ctx, launch_tile_masked<T>(m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field, aux,
+                                          ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                          ctx->mk_wn.as<double>(), ctx->mk_c.as<double>(), pz, pz,
+                                          1, sc, sumfx, sumd, st));
+  const size_t ns = (size_t)nS;
+  const size_t ns16 = (ns + 15) & ~(size_t)15;
+  double* q0 = nullptr;
+  double* q1 = nullptr;
+  if (grouped) {
+    // the plain method's iteration on these rows: p_new = (bs * inv + M p) / W
+    // (no diagonal: a run start's own visits are entries), p_S over three
+    // buffers (each launch runs the previous iteration's stop rule), the
+    // increment form for FP32 ray sums
+    OMNI_TRY_HIP(ctx, ctx->am_bs.alloc(sizeof(long long) * ns));
+    OMNI_TRY_HIP(ctx, ctx->am_diag.alloc(sizeof(unsigned int) * ns));
+    OMNI_TRY_HIP(ctx, launch_masked_bs(nS, slist, sumfx, ctx->am_bs.as<long long>(), st));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_diag.as<void>(), (int64_t)ns, 0u, st));
+    record_event(ctx, 3);
+    OMNI_TRY_HIP(ctx, ctx->ps0.alloc(sizeof(double) * ns16));
+    OMNI_TRY_HIP(ctx, ctx->ps1.alloc(sizeof(double) * ns16));
+    OMNI_TRY_HIP(ctx, ctx->ps2.alloc(sizeof(double) * ns16));
+    OMNI_TRY_HIP(ctx, ctx->psel.alloc(sizeof(double) * ns16));
+    double* pb3[3] = {ctx->ps0.as<double>(), ctx->ps1.as<double>(), ctx->ps2.as<double>()};
+    const bool inc = std::is_same<T, float>::value && !std::getenv("OMNI3D_NO_INCREMENTS");
+    if (inc) {
+      OMNI_TRY_HIP(ctx, ctx->srow.alloc(sizeof(double) * ns));
+      OMNI_TRY_HIP(ctx, ctx->dinc0.alloc(sizeof(float) * ns16));
+      OMNI_TRY_HIP(ctx, ctx->dinc1.alloc(sizeof(float) * ns16));
+    }
+    float* dbuf[2] = {ctx->dinc0.as<float>(), ctx->dinc1.as<float>()};
+    const unsigned int gg = grouped_grid(nS);
+    OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 4 * gg));
+    OMNI_TRY_HIP(ctx, launch_gather_sources(slist, nS, pinit, pb3[0], st));
+    OMNI_TRY_HIP(ctx, launch_absmax<double>(pb3[0], nS, &sc->itmax_bits[0], st));
+    for (int k = 1; k <= P.max_iterations; k++)
+      OMNI_TRY_HIP(ctx, launch_iterate_dense(4, 0, k, nS, ctx->am_ptr.as<unsigned long long>(),
+                                             ctx->am_wptr.as<unsigned int>(),
+                                             ctx->am_dent.as<unsigned int>(),
+                                             ctx->am_diag.as<unsigned int>(), slist, wcnt,
+                                             ctx->am_bs.as<long long>(), pinit, seed > 0 ? 1 : 0,
+                                             pb3[(k - 1) % 3], pb3[k % 3], sc,
+                                             ctx->partials.as<double>(), P.max_iterations,
+                                             P.tolerance, st, inc ? ctx->srow.as<double>() : nullptr,
+                                             inc ? dbuf[(k - 1) & 1] : nullptr,
+                                             inc ? dbuf[k & 1] : nullptr));
+    const int K = P.max_iterations;
+    OMNI_TRY_HIP(ctx, launch_iter_finish(K, (int)gg,
+                                         ctx->partials.as<double>() + (size_t)(K & 1) * 2 * gg, sc,
+                                         K, P.tolerance, st));
+    OMNI_TRY_HIP(ctx, launch_select_ps(nS, pb3[0], pb3[1], pb3[2], sc, ctx->psel.as<double>(), st));
+    q0 = q1 = ctx->psel.as<double>();
+  } else {
+    OMNI_TRY_HIP(ctx, ctx->am_b.alloc(sizeof(double) * ns));
+    OMNI_TRY_HIP(ctx, ctx->am_wv.alloc(sizeof(double) * ns));
+    OMNI_TRY_HIP(ctx, launch_masked_rowvec(nS, slist, fixed ? 1 : 0, sumfx, wcnt, sumd, wsum, sc,
+                                           ctx->am_b.as<double>(), ctx->am_wv.as<double>(), st));
+    record_event(ctx, 3);
+    // the iteration on S from p_S^0
+    OMNI_TRY_HIP(ctx, ctx->am_p0.alloc(sizeof(double) * ns));
+    OMNI_TRY_HIP(ctx, ctx->am_p1.alloc(sizeof(double) * ns));
+    q0 = ctx->am_p0.as<double>();
+    q1 = ctx->am_p1.as<double>();
+    OMNI_TRY_HIP(ctx, launch_gather_sources(slist, nS, pinit, q0, st));
+    OMNI_TRY_HIP(ctx, launch_absmax<double>(q0, nS, &sc->itmax_bits[0], st));
+    const unsigned int grid = masked_iterate_grid(nS);
+    OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * grid));
+    for (int k = 1; k <= P.max_iterations; k++) {
+      OMNI_TRY_HIP(ctx, launch_masked_iterate(k, nS, ctx->am_ptr.as<unsigned long long>(),
+                                              ctx->am_src.as<unsigned int>(),
+                                              ctx->am_w.as<double>(), ctx->am_b.as<double>(),
+                                              ctx->am_wv.as<double>(), slist, pinit,
+                                              seed > 0 ? 1 : 0, q0, q1, sc,
+                                              ctx->am_wmax.as<unsigned long long>(),
+                                              ctx->partials.as<double>(), st));
+      OMNI_TRY_HIP(ctx, launch_iter_finish2(k, (int)grid, ctx->partials.as<double>(), sc,
+                                            P.max_iterations, P.tolerance, st));
+    }
+  }
+  record_event(ctx, 4);
+  // the final pass (pass k*) from p_S^{k*-1}, on scalars whose stop flag is
+  // clear; pa keeps the starting p off S (voxels no visit updates)
+  OMNI_TRY_HIP(ctx, ctx->am_sc.alloc(sizeof(DevScalars)));
+  DevScalars* scf = ctx->am_sc.as<DevScalars>();
+  OMNI_TRY_HIP(ctx, launch_masked_final_scalars(sc, scf, st));
+  OMNI_TRY_HIP(ctx, launch_masked_scatter(nS, slist, sc, q0, q1, pa, st));
+  record_event(ctx, 5);
+  OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
+                                      pl.val_off.as<unsigned long long>(), ctx->mk_e.as<double>(),
+                                      ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                      ctx->mk_f.as<uint32_t>(), pa, pa, 1, scf,
+                                      ctx->mk_c.as<double>(), st));
+  OMNI_TRY_HIP(ctx, launch_tile_masked<T>(m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field, aux,
+                                          ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                          ctx->mk_wn.as<double>(), ctx->mk_c.as<double>(), pa, pa,
+                                          1, scf, sumfx, sumd, st));
+  record_event(ctx, 6);
+  const bool f64 = dtype == OMNI3D_F64;
+  OMNI_TRY_HIP(ctx, launch_masked_final(V, fixed ? 1 : 0, sumfx, wcnt, sumd, wsum, pinit,
+                                        seed > 0 ? 1 : 0, sc, pa, f64 ? (double*)pdst : nullptr,
+                                        f64 ? nullptr : (float*)pdst,
+                                        (wdst && f64) ? (double*)wdst : nullptr,
+                                        (wdst && !f64) ? (float*)wdst : nullptr, st));
+  ctx->am_nnz = (int64_t)nnz;
+  ctx->am_words = grouped ? (int64_t)nwords : (int64_t)nnz;
+  ctx->am_ns = nS;
+  return OMNI3D_OK;
+}
+
 // Whether a reconstruction's source iteration can run inside a
 // multi-snapshot launch: the plain method on the affine solver with the
 // grouped dense iteration (tile engine, p_S over three buffers).
@@ -1054,12 +1351,18 @@ IterDesc affine_iter_desc(omni3d_ctx* ctx, int first) {
   return d;
 }
 
+// Queues a whole reconstruction of the context's field on its stream (no
+// host synchronisation, except the masked methods' affine schedule, which
+// sizes its rows on the host; every buffer must already have its size when
+// this runs under graph capture).  The result goes to pdst / wdst (device,
+// dtype).
 int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const omni3d_params& P = ctx->prm;
   hipStream_t st = ctx->stream;
   const int64_t V = voxels(P);
   int rc;
   const bool f64 = dtype == OMNI3D_F64;
+  ctx->am_ns = -1;
   const bool fp32 = P.precision == OMNI3D_FP32_KAHAN;
   // the source-voxel (affine) schedule exists for the plain method; every
   // other method, and the NAIVE solver, walks the whole volume per iteration
@@ -1075,9 +1378,23 @@ int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) 
                                       (wdst && f64) ? (double*)wdst : nullptr,
                                       (wdst && !f64) ? (float*)wdst : nullptr, st));
   } else {
-    rc = fp32 ? run_naive<float>(ctx, seed, pdst, wdst, dtype)
-              : run_naive<double>(ctx, seed, pdst, wdst, dtype);
-    if (rc) return rc;
+    // the masked methods' affine schedule, where it applies (the tile engine;
+    // not inside a captured graph: it sizes its rows on the host)
+    const char* menv = std::getenv("OMNI3D_MASKED");
+    bool fallback = true;
+    if (P.solver == OMNI3D_SOLVER_AFFINE && ctx->affine_masked && !ctx->capturing &&
+        !(menv && std::strcmp(menv, "naive") == 0) &&
+        (P.method == OMNI3D_METHOD_PLAIN_MASK || P.method == OMNI3D_METHOD_SELECT ||
+         P.method == OMNI3D_METHOD_WEIGHTED)) {
+      rc = fp32 ? run_affine_masked<float>(ctx, seed, pdst, wdst, dtype, &fallback)
+                : run_affine_masked<double>(ctx, seed, pdst, wdst, dtype, &fallback);
+      if (rc) return rc;
+    }
+    if (fallback) {
+      rc = fp32 ? run_naive<float>(ctx, seed, pdst, wdst, dtype)
+                : run_naive<double>(ctx, seed, pdst, wdst, dtype);
+      if (rc) return rc;
+    }
   }
   return OMNI3D_OK;
 }
@@ -1450,6 +1767,11 @@ int omni3d_reconstruct(omni3d_ctx* ctx, void* p, void* weight_out, int dtype, in
     rep->segments = ctx->plan->nsegs;
     rep->sell_cells = ctx->plan->ncell;
     rep->matrix_words = ctx->plan->nwords;
+    if (ctx->am_ns >= 0) {  // the masked methods' run starts and their rows
+      rep->sources = ctx->am_ns;
+      rep->source_visits = ctx->am_nnz;
+      rep->matrix_words = ctx->am_words;
+    }
     rep->ms_segment_sums = ms(2, 8);
   }
   if (hs.status) return fail(ctx, hs.status, "non-finite value in the input field or pressure");

@@ -260,6 +260,10 @@ int omni3d_batch_create(const omni3d_params* base, int count, const int* nx, con
       delete b;
       return rc;
     }
+    // the masked methods' affine schedule sizes its rows from the field on
+    // the host (a synchronisation per field): a captured graph replays the
+    // reference schedule instead
+    c->affine_masked = false;
     b->snap.push_back(c);
   }
   for (int i = 0; i < count; i++) b->out.emplace_back(new DevBuf());

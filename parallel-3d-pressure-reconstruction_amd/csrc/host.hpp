@@ -190,6 +190,16 @@ struct omni3d_ctx {
   // masked methods on the tile engine: per value slot the first-visit record
   // (E, S, R, F, curl weights) of the field and the carries of an iteration
   omni3d::DevBuf mk_e, mk_s, mk_r, mk_f, mk_wc, mk_wn, mk_c;
+  // the masked methods' affine schedule: per value slot the start voxel of
+  // the run a segment continues; source ids and list of the run starts S;
+  // the rows (pointers, source ids, weights); per row b, W and p_S (two);
+  // max weight; the final pass's scalars
+  omni3d::DevBuf am_run, am_sid, am_slist, am_ptr, am_src, am_w, am_b, am_wv, am_p0, am_p1;
+  omni3d::DevBuf am_wmax, am_sc, am_dent, am_wptr, am_bs, am_diag;
+  bool affine_masked = true;  // off for snapshot batches (their graphs replay the reference schedule)
+  int am_ns = -1;             // run starts of the last reconstruction (-1: reference schedule)
+  int64_t am_nnz = 0;         // its row entries
+  int64_t am_words = 0;       // and the words the iteration reads
   // 0 start, 1 plan done, 2 pass A start, 3 pass A done, 4 iterations done,
   // 5 pass B start, 6 pass B done, 7 finalize done
   hipEvent_t ev[9] = {};

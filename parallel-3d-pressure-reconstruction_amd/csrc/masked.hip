@@ -30,6 +30,7 @@
 // pass walks the visits after it.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 
 #include "kernels.hpp"
@@ -515,5 +516,410 @@ template hipError_t launch_tile_masked<double>(int, int, const unsigned long lon
                                                const double*, const double*, const double*,
                                                const double*, int, const DevScalars*, long long*,
                                                double*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// The masked methods' affine schedule.  A run's value at a visit is
+// p[start] + (the terms since the start), and the weights, the runs and their
+// start voxels depend on the curl / mask alone.  So with S = the run-start
+// voxels (the source flags of k_mask_walk) every pass maps p on S affinely:
+//   p_new[s] = (b[s] + sum over the visits of s: w p[start of the run]) / W[s]
+// where W[s] > 0 (else p[s] stays), b = the pass's sums from p = 0 and W the
+// weights (kernel.cu:2385-2392 Select, 2071-2079 Weighted; unit weights for
+// plain-mask).  Per field the visits landing on S are listed once as rows:
+// k_mask_runsrc gives each segment the start voxel of the run it continues,
+// k_tile_rows replays the rule per tile (first counting each row's entries,
+// then filling them); the iteration (k_masked_iterate) then runs on S alone,
+// and one full pass from p_S^{k*-1} gives every voxel (k_masked_final).
+
+constexpr uint32_t kNoRun = 0xffffffffu;
+
+// Per segment (value slot): the start voxel of the run a segment in the
+// "continue" mode carries -- the last restart before it along its ray.
+__global__ void __launch_bounds__(256)
+k_mask_runsrc(int64_t nrays, const unsigned int* __restrict__ nseg,
+              const unsigned long long* __restrict__ val_off, const uint32_t* __restrict__ R,
+              const uint32_t* __restrict__ F, uint32_t* __restrict__ runsrc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  unsigned long long gs = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int n = nseg[r];
+  uint32_t cur = kNoRun;
+  for (unsigned int j = 0; j < n; j++, gs += 64) {
+    const uint32_t f = F[gs];
+    runsrc[gs] = (f & 3u) == kModeCont ? cur : kNoRun;
+    if (f & kFlagRestart) cur = R[gs];
+  }
+}
+
+// The rows: per tile, the rule replayed over the tile's segments (the
+// curl / mask and the source ids staged in LDS); every visit that adds and
+// lands on a source voxel v is an entry (row sid[v], source sid[start],
+// weight).  Every visit of a voxel is walked by the tile that owns it, so the
+// rows of the tile's voxels are counted and filled in LDS: kFill == false
+// stores each row's entry count into cnt; kFill takes a slot per entry from
+// an LDS cursor and writes the entry at rowptr[row] + slot (esrc, and ew for
+// the curl weights).  Entries of a row are in no fixed order: the iteration's
+// fixed-point sums do not depend on it.
+template <int kMethod, int kBits, bool kFill>
+__global__ void __launch_bounds__(1024)
+k_tile_rows(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
+            GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ aux,
+            const uint32_t* __restrict__ F, const double* __restrict__ wc,
+            const double* __restrict__ wn, const uint32_t* __restrict__ runsrc,
+            const int* __restrict__ sid, unsigned int* __restrict__ cnt,
+            const unsigned long long* __restrict__ rowptr, unsigned int* __restrict__ esrc,
+            double* __restrict__ ew) {
+  constexpr int kTile = 1 << kBits;
+  constexpr int kTileMask = kTile - 1;
+  constexpr int kVox = kTile * kTile * kTile;
+  __shared__ double s_aux[kVox];
+  __shared__ int s_sid[kVox];
+  __shared__ unsigned int s_n[kVox];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), kk = oz + (l >> (2 * kBits));
+    const bool in = i < g.X && j < g.Y && kk < g.Z;
+    const int64_t gv = i + j * sy + kk * sz;
+    const int sl = lds_slot<kBits>(l);
+    s_aux[sl] = in ? aux[gv] : 0.0;
+    s_sid[sl] = in ? sid[gv] : -1;
+    s_n[sl] = 0u;
+  }
+  __syncthreads();
+  auto gvox = [&](int l) -> int64_t {
+    return (int64_t)(ox + (l & kTileMask)) + (int64_t)(oy + ((l >> kBits) & kTileMask)) * sy +
+           (int64_t)(oz + (l >> (2 * kBits))) * sz;
+  };
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    const uint32_t f = F[sr.g];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const int d0 = (sg & 1u) ? 1 : -1;
+    const int d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    bool p0set = (f & kFlagP0) != 0, outthr = (f & kFlagOut) != 0;
+    const uint32_t mode = f & 3u;
+    uint32_t t = mode == kModeCont ? runsrc[sr.g]
+                                   : (mode == kModeRestart ? (uint32_t)gvox(lv) : kNoRun);
+    // the source id of the run's start voxel (every run start is a source:
+    // k_mask_walk flags them all)
+    int src = t == kNoRun ? -1 : sid[t];
+    double curl = 0.0, num = 0.0;
+    if (kCurlW<kMethod>) {
+      curl = wc[sr.g];
+      num = wn[sr.g];
+    }
+    auto emit = [&](int l) {
+      const int sl = lds_slot<kBits>(l);
+      const int row = s_sid[sl];
+      if (row < 0 || src < 0) return;
+      const unsigned int slot = atomicAdd(&s_n[sl], 1u);
+      if constexpr (kFill) {
+        const unsigned long long at = rowptr[row] + slot;
+        esrc[at] = (unsigned int)src;
+        if constexpr (kCurlW<kMethod>) ew[at] = curl_weight(curl, num);
+      }
+    };
+    if (f & kFlagAdd) emit(lv);
+    uint32_t cw = sr.code[0];
+    int rem = (int)n - 1;
+#pragma unroll 1
+    for (int wi = 0; wi < 3 && rem > 0; wi++) {
+      const int m = rem < 16 ? rem : 16;
+      for (int q = 0; q < m; q++) {
+        const int a = (int)(cw & 3u);
+        cw >>= 2;
+        const int ln = lv + (a == 0 ? d0 : (a == 1 ? d1 : d2));
+        const double cn = s_aux[lds_slot<kBits>(ln)];
+        const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
+        if (d.add) {
+          bool added = true;
+          if (kCurlW<kMethod>) {
+            curl += cn / 10000;
+            num += 1.0;
+            added = kMethod != kMaskWeighted || curl != 0.0;
+          }
+          if (added) emit(ln);
+        }
+        if (d.restart) {
+          src = s_sid[lds_slot<kBits>(ln)];
+          if (!d.keep_w) {
+            curl = 0.0;
+            num = 0.0;
+          }
+        }
+        lv = ln;
+      }
+      rem -= m;
+      cw = wi == 0 ? sr.code[1] : sr.code[2];
+    }
+  }
+  if constexpr (!kFill) {
+    __syncthreads();
+    for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+      const int sl = lds_slot<kBits>(l);
+      const int row = s_sid[sl];
+      if (row >= 0) cnt[row] = s_n[sl];
+    }
+  }
+}
+
+// Per row of S: the constant b (the sums of the pass from p = 0) and the
+// weight W, from the full-volume sums and weights.
+__global__ void __launch_bounds__(256)
+k_masked_rowvec(int nS, const int* __restrict__ slist, int fixed, const long long* __restrict__ sumfx,
+                const unsigned int* __restrict__ wcnt, const double* __restrict__ sumd,
+                const double* __restrict__ wsum, const DevScalars* __restrict__ sc,
+                double* __restrict__ bvec, double* __restrict__ wvec) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nS) return;
+  const int v = slist[r];
+  bvec[r] = fixed ? (double)sumfx[v] * sc->inv_scale : sumd[v];
+  wvec[r] = fixed ? (double)wcnt[v] : wsum[v];
+}
+
+// Unit weights: the rows' constant in fixed point (the grouped iteration's
+// form, bs * inv_scale) and the rows' entries as 64-bit sort keys (source in
+// the low bits) for the merged-word layout.
+__global__ void __launch_bounds__(256)
+k_masked_bs(int nS, const int* __restrict__ slist, const long long* __restrict__ sumfx,
+            long long* __restrict__ bs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < nS) bs[r] = sumfx[slist[r]];
+}
+
+__global__ void __launch_bounds__(256)
+k_widen_keys(int64_t n, const unsigned int* __restrict__ esrc, unsigned long long* __restrict__ key) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = esrc[i];
+}
+
+// One iteration on S: a wavefront per row, its lanes over the row's entries;
+// the products w p[src] rounded to 64-bit fixed point with the scale
+// 2^(51 - e), 2^e > max w * max |p_S^{k-1}|, so every term is below 2^51
+// and a row's sum (at most 4095 entries: the visits of one voxel) below
+// 2^63, and added as integers (the same sum in any order).  Then
+// p_new = (b + sum [+ the starting p on the first pass, kernel.cu:8813]) / W
+// where W > 0, the relative-change partials per workgroup, and max |p_new|
+// (slot k & 1) for the next iteration's scale; k_iter_finish (two slots)
+// applies the stop rule.
+constexpr int kRowWaves = 4;
+__global__ void __launch_bounds__(64 * kRowWaves)
+k_masked_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
+                 const unsigned int* __restrict__ esrc, const double* __restrict__ ew,
+                 const double* __restrict__ bvec, const double* __restrict__ wvec,
+                 const int* __restrict__ slist, const double* __restrict__ pinit, int seed,
+                 double* __restrict__ p0, double* __restrict__ p1, DevScalars* __restrict__ sc,
+                 const unsigned long long* __restrict__ wmax_bits, double* __restrict__ partials) {
+  if (sc->converged) return;
+  __shared__ double s_d2[kRowWaves], s_n2[kRowWaves];
+  __shared__ unsigned long long s_pm[kRowWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
+  double* __restrict__ pnew = (k & 1) ? p1 : p0;
+  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]) *
+                   __longlong_as_double((long long)*wmax_bits);
+  const bool finite = B <= DBL_MAX;
+  int ex = 0;
+  (void)frexp(finite ? B : 1.0, &ex);
+  ex = ex < -960 ? -960 : ex;
+  const double scale = ldexp(1.0, 51 - ex), iscale = ldexp(1.0, ex - 51);
+  double d2 = 0.0, n2 = 0.0;
+  unsigned long long pm = 0ull;
+  for (int r = blockIdx.x * kRowWaves + wave; r < nS; r += gridDim.x * kRowWaves) {
+    long long acc = 0;
+    for (unsigned long long e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64)
+      acc += to_fixed(ew[e] * pold[esrc[e]], scale);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    const double po = pold[r];
+    double tot = bvec[r] + (finite ? (double)acc * iscale : B - B);
+    if (k == 1 && seed && pinit) tot += pinit[slist[r]];
+    const double w = wvec[r];
+    const double pn = w > 0.0 ? tot / w : po;
+    if (lane == 0) pnew[r] = pn;
+    const double df = pn - po;
+    d2 += lane == 0 ? df * df : 0.0;
+    n2 += lane == 0 ? pn * pn : 0.0;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(pn));
+    pm = bits > pm ? bits : pm;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+  }
+  if (lane == 0) {
+    s_d2[wave] = d2;
+    s_n2[wave] = n2;
+    s_pm[wave] = pm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    unsigned long long m = 0ull;
+    for (int q = 0; q < kRowWaves; q++) {
+      a += s_d2[q];
+      c += s_n2[q];
+      m = s_pm[q] > m ? s_pm[q] : m;
+    }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = c;
+    atomicMax(&sc->itmax_bits[k & 1], m);
+  }
+}
+
+// p_S^{k*-1} (the buffer iteration k* read) into the full-volume pressure
+// the final pass reads at the run starts.
+__global__ void __launch_bounds__(256)
+k_masked_scatter(int nS, const int* __restrict__ slist, const DevScalars* __restrict__ sc,
+                 const double* __restrict__ p0, const double* __restrict__ p1,
+                 double* __restrict__ pfull) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nS) return;
+  const double* __restrict__ p = ((sc->kstar - 1) & 1) ? p1 : p0;
+  pfull[slist[r]] = p[r];
+}
+
+// The scalars of the final pass: the iteration's scale, the stop flag
+// cleared (the pass kernels return at once on a stopped iteration).
+__global__ void k_masked_final_scalars(const DevScalars* __restrict__ sc,
+                                       DevScalars* __restrict__ out) {
+  *out = *sc;
+  out->converged = 0;
+}
+
+// Averaging of the final pass (devidecountInner, kernel.cu:6746-6759):
+// where the weight is positive p = sum / weight (plus the starting p when the
+// output is the first pass and pn started as p), elsewhere p keeps
+// p^{k*-1} (pprev).  Outputs in FP64 or FP32, the weights beside.
+__global__ void __launch_bounds__(256)
+k_masked_final(int64_t V, int fixed, const long long* __restrict__ sumfx,
+               const unsigned int* __restrict__ wcnt, const double* __restrict__ sumd,
+               const double* __restrict__ wsum, const double* __restrict__ pinit, int seed,
+               const DevScalars* __restrict__ sc, const double* __restrict__ pprev,
+               double* __restrict__ pd, float* __restrict__ pf, double* __restrict__ wd,
+               float* __restrict__ wf) {
+  const double inv = sc->inv_scale;
+  const bool add_seed = seed && pinit && sc->kstar == 1;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const double w = fixed ? (double)wcnt[v] : wsum[v];
+    double p = pprev[v];
+    if (w > 0.0) {
+      double tot = fixed ? (double)sumfx[v] * inv : sumd[v];
+      if (add_seed) tot += pinit[v];
+      p = tot / w;
+    }
+    if (pd) pd[v] = p;
+    if (pf) pf[v] = (float)p;
+    if (wd) wd[v] = w;
+    if (wf) wf[v] = (float)w;
+  }
+}
+
+hipError_t launch_mask_runsrc(int64_t nrays, const unsigned int* nseg,
+                              const unsigned long long* val_off, const uint32_t* R,
+                              const uint32_t* F, uint32_t* runsrc, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_runsrc, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, nrays, nseg,
+                     val_off, R, F, runsrc);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_rows(bool fill, int method, int ntiles, const unsigned long long* tile_ptr,
+                            const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                            const double* aux, const uint32_t* F, const double* wc,
+                            const double* wn, const uint32_t* runsrc, const int* sid,
+                            unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, double* ew, hipStream_t st) {
+#define OMNI_TR(M, B, FL)                                                                       \
+  hipLaunchKernelGGL((k_tile_rows<M, B, FL>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, \
+                     tg, wp, aux, F, wc, wn, runsrc, sid, cnt, rowptr, esrc, ew)
+#define OMNI_TR_B(M, FL)                  \
+  do {                                    \
+    if (tg.bits == 3) OMNI_TR(M, 3, FL);  \
+    else OMNI_TR(M, 4, FL);               \
+  } while (0)
+#define OMNI_TR_M(FL)                                 \
+  do {                                                \
+    switch (method) {                                 \
+      case kMaskUnit: OMNI_TR_B(kMaskUnit, FL); break;     \
+      case kMaskSelect: OMNI_TR_B(kMaskSelect, FL); break; \
+      default: OMNI_TR_B(kMaskWeighted, FL); break;        \
+    }                                                 \
+  } while (0)
+  if (fill) OMNI_TR_M(true);
+  else OMNI_TR_M(false);
+#undef OMNI_TR_M
+#undef OMNI_TR_B
+#undef OMNI_TR
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_bs(int nS, const int* slist, const long long* sumfx, long long* bs,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_bs, dim3(grid_for(nS, 1u << 30)), dim3(256), 0, st, nS, slist, sumfx,
+                     bs);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen_keys(int64_t n, const unsigned int* esrc, unsigned long long* key,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_widen_keys, dim3(grid_for(n, 1u << 30)), dim3(256), 0, st, n, esrc, key);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_rowvec(int nS, const int* slist, int fixed, const long long* sumfx,
+                                const unsigned int* wcnt, const double* sumd, const double* wsum,
+                                const DevScalars* sc, double* bvec, double* wvec,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_rowvec, dim3(grid_for(nS, 1u << 30)), dim3(256), 0, st, nS, slist,
+                     fixed, sumfx, wcnt, sumd, wsum, sc, bvec, wvec);
+  return hipGetLastError();
+}
+
+unsigned int masked_iterate_grid(int nS) {
+  // a wavefront walks many rows: the per-workgroup partials and max stay few
+  const int b = (nS + kRowWaves - 1) / kRowWaves;
+  return (unsigned int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
+
+hipError_t launch_masked_iterate(int k, int nS, const unsigned long long* rowptr,
+                                 const unsigned int* esrc, const double* ew, const double* bvec,
+                                 const double* wvec, const int* slist, const double* pinit,
+                                 int seed, double* p0, double* p1, DevScalars* sc,
+                                 const unsigned long long* wmax_bits, double* partials,
+                                 hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_iterate, dim3(masked_iterate_grid(nS)), dim3(64 * kRowWaves), 0, st,
+                     k, nS, rowptr, esrc, ew, bvec, wvec, slist, pinit, seed, p0, p1, sc,
+                     wmax_bits, partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_scatter(int nS, const int* slist, const DevScalars* sc, const double* p0,
+                                 const double* p1, double* pfull, hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_scatter, dim3(grid_for(nS, 1u << 30)), dim3(256), 0, st, nS, slist,
+                     sc, p0, p1, pfull);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_final_scalars(const DevScalars* sc, DevScalars* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_final_scalars, dim3(1), dim3(1), 0, st, sc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_final(int64_t V, int fixed, const long long* sumfx,
+                               const unsigned int* wcnt, const double* sumd, const double* wsum,
+                               const double* pinit, int seed, const DevScalars* sc,
+                               const double* pprev, double* pd, float* pf, double* wd, float* wf,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_final, dim3(grid_for(V)), dim3(256), 0, st, V, fixed, sumfx, wcnt,
+                     sumd, wsum, pinit, seed, sc, pprev, pd, pf, wd, wf);
+  return hipGetLastError();
+}
 
 }  // namespace omni3d

@@ -618,8 +618,12 @@ __device__ __forceinline__ void iterate_grp_body(
           if (q < q1) pre[j] = d4[q];
         }
       }
-      const unsigned long long lo = e0 > base ? e0 - base : 0ull;
-      const unsigned long long hi = e1 - base;
+      // this round's words i with lo <= i < hi belong to the group (lo <= 3:
+      // the group's first round may start up to 3 words early; both bounds
+      // fit 32 bits)
+      const unsigned int lo = (unsigned int)(e0 > base ? e0 - base : 0ull);
+      const unsigned long long hl = e1 - base;
+      const unsigned int hi = hl < (unsigned long long)(kU * 64) ? (unsigned int)hl : kU * 64;
       unsigned int w[kU];
       double x[kU];
       bool ok[kU];
@@ -636,9 +640,11 @@ __device__ __forceinline__ void iterate_grp_body(
       }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
-        // a word stands for 1..4 equal entries (the same source in one row);
+        // a word stands for 1..4 equal entries (the same source in one row):
+        // the fixed-point value times its multiplicity (an integer product,
+        // exact; scaling x first would leave the bias rounding's 2^51 range);
         // lanes outside the group's words add 0 (no branch around the add)
-        const unsigned long long m = ok[u] ? ((w[u] >> 22) & 3u) + 1u : 0u;
+        const unsigned int m = ok[u] ? ((w[u] >> 22) & 3u) + 1u : 0u;
         atomicAdd(&s_acc[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale) * m);
       }
       __builtin_amdgcn_wave_barrier();
@@ -1026,6 +1032,13 @@ __global__ void k_select_ps(int nS, const double* __restrict__ b0, const double*
 hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const double* b2,
                             const DevScalars* sc, double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_select_ps, dim3(grid_for(nS)), dim3(256), 0, st, nS, b0, b1, b2, sc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_iter_finish2(int k, int nparts, const double* partials, DevScalars* sc,
+                               int max_iter, double tol, hipStream_t st) {
+  hipLaunchKernelGGL(k_iter_finish, dim3(1), dim3(kFinishBlock), 0, st, k, nparts, partials, sc,
+                     max_iter, tol, 2);
   return hipGetLastError();
 }
 

@@ -161,4 +161,43 @@ hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile
 hipError_t launch_tile_counts(int ntiles, const unsigned long long* tile_ptr1, const SegRec* segs,
                               GridDesc g, TileGrid tg, unsigned int* cnt, hipStream_t st);
 
+// masked.hip: the masked methods' affine schedule ---------------------------
+hipError_t launch_mask_runsrc(int64_t nrays, const unsigned int* nseg,
+                              const unsigned long long* val_off, const uint32_t* R,
+                              const uint32_t* F, uint32_t* runsrc, hipStream_t st);
+// method 1 plain-mask, 2 Select, 3 Weighted; fill = false counts each row's
+// entries into cnt, fill = true writes them (cnt zeroed: the rows' cursors)
+hipError_t launch_tile_rows(bool fill, int method, int ntiles, const unsigned long long* tile_ptr,
+                            const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                            const double* aux, const uint32_t* F, const double* wc,
+                            const double* wn, const uint32_t* runsrc, const int* sid,
+                            unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, double* ew, hipStream_t st);
+hipError_t launch_masked_bs(int nS, const int* slist, const long long* sumfx, long long* bs,
+                           hipStream_t st);
+hipError_t launch_widen_keys(int64_t n, const unsigned int* esrc, unsigned long long* key,
+                             hipStream_t st);
+hipError_t launch_masked_rowvec(int nS, const int* slist, int fixed, const long long* sumfx,
+                                const unsigned int* wcnt, const double* sumd, const double* wsum,
+                                const DevScalars* sc, double* bvec, double* wvec,
+                                hipStream_t st);
+unsigned int masked_iterate_grid(int nS);
+hipError_t launch_masked_iterate(int k, int nS, const unsigned long long* rowptr,
+                                 const unsigned int* esrc, const double* ew, const double* bvec,
+                                 const double* wvec, const int* slist, const double* pinit,
+                                 int seed, double* p0, double* p1, DevScalars* sc,
+                                 const unsigned long long* wmax_bits, double* partials,
+                                 hipStream_t st);
+hipError_t launch_masked_scatter(int nS, const int* slist, const DevScalars* sc, const double* p0,
+                                 const double* p1, double* pfull, hipStream_t st);
+hipError_t launch_masked_final_scalars(const DevScalars* sc, DevScalars* out, hipStream_t st);
+hipError_t launch_masked_final(int64_t V, int fixed, const long long* sumfx,
+                               const unsigned int* wcnt, const double* sumd, const double* wsum,
+                               const double* pinit, int seed, const DevScalars* sc,
+                               const double* pprev, double* pd, float* pf, double* wd, float* wf,
+                               hipStream_t st);
+// the stop rule of iteration k for an iteration with two max |p| slots
+hipError_t launch_iter_finish2(int k, int nparts, const double* partials, DevScalars* sc,
+                               int max_iter, double tol, hipStream_t st);
+
 }  // namespace omni3d

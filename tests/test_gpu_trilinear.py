@@ -45,7 +45,12 @@ def test_trilinear_vs_oracle(gpu_available, ref_dirs2, precision, tol):
     assert rep.iterations == it
     err = F.rel_l2(p, po)
     print("trilinear rel-L2 vs oracle", err, "iterations", it)
-    assert err < (1e-12 if precision == omni3d.FP64 else 1e-5)
+    if precision == omni3d.FP64:
+        # per-pass rounding differences (~1e-15) add up over the passes: the
+        # averaging keeps a constant shift of p and damps smooth modes slowly
+        assert err < (1e-12 if K <= 12 else 1e-11)
+    else:
+        assert err < 1e-5
 
 
 def test_trilinear_quadratic_exact(gpu_available, ref_dirs3):
