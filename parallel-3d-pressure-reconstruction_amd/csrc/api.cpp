@@ -1103,13 +1103,18 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                         ctx->mk_wn.as<double>(), wcnt, wsum,
                                         ctx->srcf.as<uint8_t>(), st));
   record_event(ctx, 8);
-  // S: compact ids of the run starts (the voxels a pass reads p at)
+  // S: compact ids of the run starts (the voxels a pass reads p at), in tile
+  // order: tile t's rows are [tfirst[t], tfirst[t + 1])
   OMNI_TRY_HIP(ctx, ctx->am_sid.alloc(sizeof(int) * V));
   OMNI_TRY_HIP(ctx, ctx->am_slist.alloc(sizeof(int) * (V + 1)));
-  DevBuf dns;
+  OMNI_TRY_HIP(ctx, ctx->am_tfirst.alloc(sizeof(int) * ((size_t)pl.ntiles + 1)));
+  DevBuf dns, bcnt;
   OMNI_TRY_HIP(ctx, dns.alloc(sizeof(int)));
-  OMNI_TRY_HIP(ctx, launch_source_ids(ctx->srcf.as<uint8_t>(), V, ctx->am_sid.as<int>(),
-                                      ctx->am_slist.as<int>(), dns.as<int>(), st));
+  OMNI_TRY_HIP(ctx, bcnt.alloc(sizeof(unsigned int) * (size_t)source_ids_tiled_blocks(pl.tg)));
+  OMNI_TRY_HIP(ctx, launch_source_ids_tiled(ctx->srcf.as<uint8_t>(), pl.g, pl.tg,
+                                            bcnt.as<unsigned int>(), ctx->am_sid.as<int>(),
+                                            ctx->am_slist.as<int>(), dns.as<int>(),
+                                            ctx->am_tfirst.as<int>(), st));
   int nS = 0;
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, dns.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
@@ -1119,45 +1124,47 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   }
   const int* sid = ctx->am_sid.as<int>();
   const int* slist = ctx->am_slist.as<int>();
-  // the run each segment continues, then the rows (count, scan, fill)
+  const int* tfirst = ctx->am_tfirst.as<int>();
+  // the run each segment continues
   OMNI_TRY_HIP(ctx, launch_mask_runsrc(pl.nrays, pl.nseg.as<unsigned int>(),
                                        pl.val_off.as<unsigned long long>(),
                                        ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
                                        ctx->am_run.as<uint32_t>(), st));
-  DevBuf rcnt;
-  OMNI_TRY_HIP(ctx, rcnt.alloc(sizeof(unsigned int) * ((size_t)nS + 1)));
-  OMNI_TRY_HIP(ctx, ctx->am_ptr.alloc(sizeof(unsigned long long) * ((size_t)nS + 1)));
-  OMNI_TRY_HIP(ctx, launch_fill_u32(rcnt.as<void>(), (int64_t)nS + 1, 0u, st));
-  OMNI_TRY_HIP(ctx, launch_tile_rows(false, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
-                                     pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
-                                     ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
-                                     ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(), sid,
-                                     rcnt.as<unsigned int>(), nullptr, nullptr, nullptr, st));
-  OMNI_TRY_HIP(ctx, launch_scan_counts(rcnt.as<unsigned int>(), nS,
-                                       ctx->am_ptr.as<unsigned long long>(), st));
-  unsigned long long nnz = 0;
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, ctx->am_ptr.as<unsigned long long>() + nS, 8,
-                                   hipMemcpyDeviceToHost, st));
-  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  if (nnz >= (1ull << 31)) {
-    *fallback = true;
-    return OMNI3D_OK;
-  }
-  OMNI_TRY_HIP(ctx, ctx->am_src.alloc(sizeof(unsigned int) * (nnz + 1)));
-  OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (fixed ? 1 : nnz + 1)));
-  OMNI_TRY_HIP(ctx, launch_tile_rows(true, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
-                                     pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
-                                     ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
-                                     ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(), sid,
-                                     rcnt.as<unsigned int>(), ctx->am_ptr.as<unsigned long long>(),
-                                     ctx->am_src.as<unsigned int>(), ctx->am_w.as<double>(), st));
   // unit weights (plain-mask): the rows as the merged-word layout of the
-  // plain method's source iteration (k_iterate_grp: groups of 256 rows,
-  // words sorted by source, equal entries merged); curl weights: per-row
-  // lists (k_masked_iterate)
-  const bool grouped = fixed && nS < (1 << 22) && nnz > 0;
+  // plain method's source iteration (k_iterate_grp: groups of 256 rows, words
+  // sorted by source, equal entries merged), built from CSR rows; otherwise
+  // the rows in tile layout (k_mtile_accum / k_mtile_finish)
+  const bool grouped = fixed && nS < (1 << 22);
+  unsigned long long nnz = 0;
   unsigned int nwords = 0;
+  int nchunks = 0;
+  DevBuf stmp;
   if (grouped) {
+    DevBuf rcnt;
+    OMNI_TRY_HIP(ctx, rcnt.alloc(sizeof(unsigned int) * ((size_t)nS + 1)));
+    OMNI_TRY_HIP(ctx, ctx->am_ptr.alloc(sizeof(unsigned long long) * ((size_t)nS + 1)));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(rcnt.as<void>(), (int64_t)nS + 1, 0u, st));
+    OMNI_TRY_HIP(ctx, launch_tile_rows(false, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                       pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                       ctx->mk_f.as<uint32_t>(), ctx->am_run.as<uint32_t>(), sid,
+                                       rcnt.as<unsigned int>(), nullptr, nullptr, st));
+    OMNI_TRY_HIP(ctx, stmp.alloc(scan_u32_tmp_bytes(nS)));
+    OMNI_TRY_HIP(ctx, launch_scan_u32(rcnt.as<unsigned int>(), nS,
+                                      ctx->am_ptr.as<unsigned long long>(), 0, stmp.as<void>(),
+                                      stmp.size(), st));
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, ctx->am_ptr.as<unsigned long long>() + nS, 8,
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (nnz == 0 || nnz >= (1ull << 31)) {
+      *fallback = true;
+      return OMNI3D_OK;
+    }
+    OMNI_TRY_HIP(ctx, ctx->am_src.alloc(sizeof(unsigned int) * (nnz + 1)));
+    OMNI_TRY_HIP(ctx, launch_tile_rows(true, m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                       pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                       ctx->mk_f.as<uint32_t>(), ctx->am_run.as<uint32_t>(), sid,
+                                       rcnt.as<unsigned int>(), ctx->am_ptr.as<unsigned long long>(),
+                                       ctx->am_src.as<unsigned int>(), st));
     DevBuf keys, ka, kb, tmp, flag, excl;
     OMNI_TRY_HIP(ctx, keys.alloc(sizeof(unsigned long long) * nnz));
     OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * nnz));
@@ -1183,18 +1190,66 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                           ctx->am_dent.as<unsigned int>(),
                                           ctx->am_wptr.as<unsigned int>(), &nwords, st));
   } else {
+    // entry counts per tile, their offsets, then the entries
+    const int nt = pl.ntiles;
+    DevBuf tcnt;
+    OMNI_TRY_HIP(ctx, tcnt.alloc(sizeof(unsigned int) * ((size_t)nt + 1)));
+    OMNI_TRY_HIP(ctx, ctx->am_tbase.alloc(sizeof(unsigned long long) * ((size_t)nt + 1)));
+    OMNI_TRY_HIP(ctx, launch_tile_entries(false, m, nt, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                          ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                          ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(),
+                                          sid, tfirst, tcnt.as<unsigned int>(), nullptr, nullptr,
+                                          nullptr, nullptr, st));
+    OMNI_TRY_HIP(ctx, stmp.alloc(scan_u32_tmp_bytes(nt)));
+    OMNI_TRY_HIP(ctx, launch_scan_u32(tcnt.as<unsigned int>(), nt,
+                                      ctx->am_tbase.as<unsigned long long>(), 0, stmp.as<void>(),
+                                      stmp.size(), st));
+    std::vector<unsigned long long> hbase((size_t)nt + 1);
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(hbase.data(), ctx->am_tbase.as<void>(),
+                                     sizeof(unsigned long long) * ((size_t)nt + 1),
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    nnz = hbase[nt];
+    if (nnz == 0) {
+      *fallback = true;
+      return OMNI3D_OK;
+    }
+    OMNI_TRY_HIP(ctx, ctx->am_src.alloc(sizeof(unsigned int) * nnz));
+    OMNI_TRY_HIP(ctx, ctx->am_erow.alloc(sizeof(unsigned short) * nnz));
+    OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (fixed ? 1 : nnz)));
+    OMNI_TRY_HIP(ctx, launch_tile_entries(true, m, nt, pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), pl.g, pl.tg, wp, aux,
+                                          ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                          ctx->mk_wn.as<double>(), ctx->am_run.as<uint32_t>(),
+                                          sid, tfirst, nullptr,
+                                          ctx->am_tbase.as<unsigned long long>(),
+                                          ctx->am_src.as<unsigned int>(),
+                                          ctx->am_erow.as<unsigned short>(),
+                                          fixed ? nullptr : ctx->am_w.as<double>(), st));
+    // chunks of at most csz entries of one tile, a workgroup each
+    const unsigned long long csz = std::max<unsigned long long>(
+        2048ull, std::min<unsigned long long>(65536ull, nnz / 4096ull));
+    std::vector<MaskChunk> chunks;
+    for (int t = 0; t < nt; t++)
+      for (unsigned long long b = hbase[t]; b < hbase[t + 1]; b += csz)
+        chunks.push_back(MaskChunk{b, std::min(hbase[t + 1], b + csz), t, 0});
+    nchunks = (int)chunks.size();
+    OMNI_TRY_HIP(ctx, ctx->am_chunks.alloc(sizeof(MaskChunk) * chunks.size()));
+    OMNI_TRY_HIP(ctx, hipMemcpy(ctx->am_chunks.as<void>(), chunks.data(),
+                                sizeof(MaskChunk) * chunks.size(), hipMemcpyHostToDevice));
+    // max w of the fixed-point scale
     OMNI_TRY_HIP(ctx, ctx->am_wmax.alloc(sizeof(unsigned long long)));
-    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_wmax.as<void>(), 2, 0u, st));
-    if (nnz > 0 && !fixed)
+    if (fixed) {
+      const double one = 1.0;
+      OMNI_TRY_HIP(ctx, hipMemcpy(ctx->am_wmax.as<void>(), &one, 8, hipMemcpyHostToDevice));
+    } else {
+      OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_wmax.as<void>(), 2, 0u, st));
       OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->am_w.as<double>(), (int64_t)nnz,
                                               ctx->am_wmax.as<unsigned long long>(), st));
-    if (fixed) {
-      // unit weights beyond the word format: w = 1 per entry
-      OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (nnz + 1)));
-      OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->am_w.as<double>(), (int64_t)nnz + 1, 1.0, st));
-      OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->am_w.as<double>(), 1,
-                                              ctx->am_wmax.as<unsigned long long>(), st));
     }
+    OMNI_TRY_HIP(ctx, ctx->am_rowsum.alloc(sizeof(unsigned long long) * (size_t)nS));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_rowsum.as<void>(), 2 * (int64_t)nS, 0u, st));
   }
   // b: the pass from p = 0, read at the run starts, and the weights W there
   OMNI_TRY_HIP(ctx, launch_mask_carry(pl.nrays, pl.nseg.as<unsigned int>(),
@@ -1267,16 +1322,20 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
     q1 = ctx->am_p1.as<double>();
     OMNI_TRY_HIP(ctx, launch_gather_sources(slist, nS, pinit, q0, st));
     OMNI_TRY_HIP(ctx, launch_absmax<double>(q0, nS, &sc->itmax_bits[0], st));
-    const unsigned int grid = masked_iterate_grid(nS);
+    const unsigned int grid = mtile_finish_grid(nS);
     OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * grid));
     for (int k = 1; k <= P.max_iterations; k++) {
-      OMNI_TRY_HIP(ctx, launch_masked_iterate(k, nS, ctx->am_ptr.as<unsigned long long>(),
-                                              ctx->am_src.as<unsigned int>(),
-                                              ctx->am_w.as<double>(), ctx->am_b.as<double>(),
-                                              ctx->am_wv.as<double>(), slist, pinit,
-                                              seed > 0 ? 1 : 0, q0, q1, sc,
-                                              ctx->am_wmax.as<unsigned long long>(),
-                                              ctx->partials.as<double>(), st));
+      OMNI_TRY_HIP(ctx, launch_mtile_accum(k, nchunks, ctx->am_chunks.as<MaskChunk>(), tfirst,
+                                           ctx->am_src.as<unsigned int>(),
+                                           ctx->am_erow.as<unsigned short>(),
+                                           fixed ? nullptr : ctx->am_w.as<double>(), q0, q1, sc,
+                                           ctx->am_wmax.as<unsigned long long>(),
+                                           ctx->am_rowsum.as<unsigned long long>(), st));
+      OMNI_TRY_HIP(ctx, launch_mtile_finish(k, nS, ctx->am_b.as<double>(), ctx->am_wv.as<double>(),
+                                            slist, pinit, seed > 0 ? 1 : 0, q0, q1,
+                                            ctx->am_rowsum.as<unsigned long long>(), sc,
+                                            ctx->am_wmax.as<unsigned long long>(),
+                                            ctx->partials.as<double>(), st));
       OMNI_TRY_HIP(ctx, launch_iter_finish2(k, (int)grid, ctx->partials.as<double>(), sc,
                                             P.max_iterations, P.tolerance, st));
     }

@@ -196,6 +196,7 @@ struct omni3d_ctx {
   // max weight; the final pass's scalars
   omni3d::DevBuf am_run, am_sid, am_slist, am_ptr, am_src, am_w, am_b, am_wv, am_p0, am_p1;
   omni3d::DevBuf am_wmax, am_sc, am_dent, am_wptr, am_bs, am_diag;
+  omni3d::DevBuf am_tfirst, am_tbase, am_erow, am_chunks, am_rowsum;
   bool affine_masked = true;  // off for snapshot batches (their graphs replay the reference schedule)
   int am_ns = -1;             // run starts of the last reconstruction (-1: reference schedule)
   int64_t am_nnz = 0;         // its row entries

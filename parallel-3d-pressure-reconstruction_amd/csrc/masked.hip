@@ -527,9 +527,12 @@ template hipError_t launch_tile_masked<double>(int, int, const unsigned long lon
 // weights (kernel.cu:2385-2392 Select, 2071-2079 Weighted; unit weights for
 // plain-mask).  Per field the visits landing on S are listed once as rows:
 // k_mask_runsrc gives each segment the start voxel of the run it continues,
-// k_tile_rows replays the rule per tile (first counting each row's entries,
-// then filling them); the iteration (k_masked_iterate) then runs on S alone,
-// and one full pass from p_S^{k*-1} gives every voxel (k_masked_final).
+// then a tile pass replays the rule and counts, then fills, the entries --
+// as CSR rows for the unit weights (k_tile_rows; the merged-word iteration
+// of the plain method takes them from there) or in tile layout for the curl
+// weights (k_tile_entries; iterated by k_mtile_accum + k_mtile_finish).  The
+// iteration runs on S alone, and one full pass from p_S^{k*-1} gives every
+// voxel (k_masked_final).
 
 constexpr uint32_t kNoRun = 0xffffffffu;
 
@@ -557,18 +560,16 @@ k_mask_runsrc(int64_t nrays, const unsigned int* __restrict__ nseg,
 // weight).  Every visit of a voxel is walked by the tile that owns it, so the
 // rows of the tile's voxels are counted and filled in LDS: kFill == false
 // stores each row's entry count into cnt; kFill takes a slot per entry from
-// an LDS cursor and writes the entry at rowptr[row] + slot (esrc, and ew for
-// the curl weights).  Entries of a row are in no fixed order: the iteration's
-// fixed-point sums do not depend on it.
+// an LDS cursor and writes the entry's source at rowptr[row] + slot.  Entries
+// of a row are in no fixed order: the iteration's fixed-point sums do not
+// depend on it.  Launched for unit weights (plain-mask).
 template <int kMethod, int kBits, bool kFill>
 __global__ void __launch_bounds__(1024)
 k_tile_rows(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
             GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ aux,
-            const uint32_t* __restrict__ F, const double* __restrict__ wc,
-            const double* __restrict__ wn, const uint32_t* __restrict__ runsrc,
+            const uint32_t* __restrict__ F, const uint32_t* __restrict__ runsrc,
             const int* __restrict__ sid, unsigned int* __restrict__ cnt,
-            const unsigned long long* __restrict__ rowptr, unsigned int* __restrict__ esrc,
-            double* __restrict__ ew) {
+            const unsigned long long* __restrict__ rowptr, unsigned int* __restrict__ esrc) {
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
   constexpr int kVox = kTile * kTile * kTile;
@@ -610,19 +611,122 @@ k_tile_rows(const unsigned long long* __restrict__ tile_ptr, const SegRec* __res
     // the source id of the run's start voxel (every run start is a source:
     // k_mask_walk flags them all)
     int src = t == kNoRun ? -1 : sid[t];
+    auto emit = [&](int l) {
+      const int sl = lds_slot<kBits>(l);
+      const int row = s_sid[sl];
+      if (row < 0 || src < 0) return;
+      const unsigned int slot = atomicAdd(&s_n[sl], 1u);
+      if constexpr (kFill) esrc[rowptr[row] + slot] = (unsigned int)src;
+    };
+    if (f & kFlagAdd) emit(lv);
+    uint32_t cw = sr.code[0];
+    int rem = (int)n - 1;
+#pragma unroll 1
+    for (int wi = 0; wi < 3 && rem > 0; wi++) {
+      const int m = rem < 16 ? rem : 16;
+      for (int q = 0; q < m; q++) {
+        const int a = (int)(cw & 3u);
+        cw >>= 2;
+        const int ln = lv + (a == 0 ? d0 : (a == 1 ? d1 : d2));
+        const double cn = s_aux[lds_slot<kBits>(ln)];
+        const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
+        if (d.add) emit(ln);
+        if (d.restart) src = s_sid[lds_slot<kBits>(ln)];
+        lv = ln;
+      }
+      rem -= m;
+      cw = wi == 0 ? sr.code[1] : sr.code[2];
+    }
+  }
+  if constexpr (!kFill) {
+    __syncthreads();
+    for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+      const int sl = lds_slot<kBits>(l);
+      const int row = s_sid[sl];
+      if (row >= 0) cnt[row] = s_n[sl];
+    }
+  }
+}
+
+// The rows in tile layout (curl weights, and unit weights past the merged-
+// word format): the entries whose row is a voxel of tile t sit in
+// [tbase[t], tbase[t + 1]) in the order the tile pass emits them (source id,
+// the row's index among the tile's rows, weight), so the pass writes them
+// coalesced (a wavefront takes consecutive slots from one LDS counter) and
+// the iteration streams a tile's entries with neighbouring rays' sources side
+// by side.  kFill == false stores each tile's entry count into tcnt.
+template <int kMethod, int kBits, bool kFill>
+__global__ void __launch_bounds__(1024)
+k_tile_entries(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
+               GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ aux,
+               const uint32_t* __restrict__ F, const double* __restrict__ wc,
+               const double* __restrict__ wn, const uint32_t* __restrict__ runsrc,
+               const int* __restrict__ sid, const int* __restrict__ tfirst,
+               unsigned int* __restrict__ tcnt, const unsigned long long* __restrict__ tbase,
+               unsigned int* __restrict__ esrc, unsigned short* __restrict__ erow,
+               double* __restrict__ ew) {
+  constexpr int kTile = 1 << kBits;
+  constexpr int kTileMask = kTile - 1;
+  constexpr int kVox = kTile * kTile * kTile;
+  __shared__ double s_aux[kVox];
+  __shared__ int s_sid[kVox];
+  __shared__ unsigned int s_total;
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  if (threadIdx.x == 0) s_total = 0u;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), kk = oz + (l >> (2 * kBits));
+    const bool in = i < g.X && j < g.Y && kk < g.Z;
+    const int64_t gv = i + j * sy + kk * sz;
+    const int sl = lds_slot<kBits>(l);
+    s_aux[sl] = in ? aux[gv] : 0.0;
+    s_sid[sl] = in ? sid[gv] : -1;
+  }
+  __syncthreads();
+  const int first = kFill ? tfirst[tile] : 0;
+  const unsigned long long base = kFill ? tbase[tile] : 0ull;
+  const int lane = threadIdx.x & 63;
+  auto gvox = [&](int l) -> int64_t {
+    return (int64_t)(ox + (l & kTileMask)) + (int64_t)(oy + ((l >> kBits) & kTileMask)) * sy +
+           (int64_t)(oz + (l >> (2 * kBits))) * sz;
+  };
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    const uint32_t f = F[sr.g];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const int d0 = (sg & 1u) ? 1 : -1;
+    const int d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    bool p0set = (f & kFlagP0) != 0, outthr = (f & kFlagOut) != 0;
+    const uint32_t mode = f & 3u;
+    const uint32_t t = mode == kModeCont ? runsrc[sr.g]
+                                         : (mode == kModeRestart ? (uint32_t)gvox(lv) : kNoRun);
+    int src = t == kNoRun ? -1 : sid[t];
     double curl = 0.0, num = 0.0;
     if (kCurlW<kMethod>) {
       curl = wc[sr.g];
       num = wn[sr.g];
     }
     auto emit = [&](int l) {
-      const int sl = lds_slot<kBits>(l);
-      const int row = s_sid[sl];
-      if (row < 0 || src < 0) return;
-      const unsigned int slot = atomicAdd(&s_n[sl], 1u);
+      const int row = s_sid[lds_slot<kBits>(l)];
+      const bool want = row >= 0 && src >= 0;
+      // the lanes emitting now take consecutive slots
+      const unsigned long long act = __ballot(want);
+      if (!want) return;
+      const int leader = __ffsll((unsigned long long)act) - 1;
+      const unsigned int rank = (unsigned int)__popcll(act & ((1ull << lane) - 1ull));
+      unsigned int b0 = 0u;
+      if (lane == leader) b0 = atomicAdd(&s_total, (unsigned int)__popcll(act));
+      b0 = __shfl(b0, leader);
       if constexpr (kFill) {
-        const unsigned long long at = rowptr[row] + slot;
+        const unsigned long long at = base + b0 + rank;
         esrc[at] = (unsigned int)src;
+        erow[at] = (unsigned short)(row - first);
         if constexpr (kCurlW<kMethod>) ew[at] = curl_weight(curl, num);
       }
     };
@@ -662,11 +766,120 @@ k_tile_rows(const unsigned long long* __restrict__ tile_ptr, const SegRec* __res
   }
   if constexpr (!kFill) {
     __syncthreads();
-    for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
-      const int sl = lds_slot<kBits>(l);
-      const int row = s_sid[sl];
-      if (row >= 0) cnt[row] = s_n[sl];
+    if (threadIdx.x == 0) tcnt[tile] = s_total;
+  }
+}
+
+// One iteration on S in tile layout, part 1: a workgroup per chunk of one
+// tile's entries adds w p[src] (64-bit fixed point, scale 2^(51 - e),
+// 2^e > max w * max |p_S^{k-1}|) into LDS sums of the tile's rows, then adds
+// each nonzero sum into the row's global fixed-point sum (a row's total over
+// all its entries stays below 2^63: fewer than 4096 entries per row).
+constexpr int kChunkThreads = 256;
+constexpr int kTileRowsMax = 4096;
+
+__device__ __forceinline__ bool masked_scale(const DevScalars* sc, int k,
+                                             const unsigned long long* wmax_bits, double* scale,
+                                             double* iscale, double* B) {
+  const double b = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]) *
+                   __longlong_as_double((long long)*wmax_bits);
+  const bool finite = b <= DBL_MAX;
+  int ex = 0;
+  (void)frexp(finite ? b : 1.0, &ex);
+  ex = ex < -960 ? -960 : ex;
+  *scale = ldexp(1.0, 51 - ex);
+  *iscale = ldexp(1.0, ex - 51);
+  *B = b;
+  return finite;
+}
+
+template <bool kUnit>
+__global__ void __launch_bounds__(kChunkThreads)
+k_mtile_accum(int k, const MaskChunk* __restrict__ chunks, const int* __restrict__ tfirst,
+              const unsigned int* __restrict__ esrc, const unsigned short* __restrict__ erow,
+              const double* __restrict__ ew, const double* __restrict__ p0,
+              const double* __restrict__ p1, const DevScalars* __restrict__ sc,
+              const unsigned long long* __restrict__ wmax_bits,
+              unsigned long long* __restrict__ rowsum) {
+  if (sc->converged) return;
+  __shared__ unsigned long long s_acc[kTileRowsMax];
+  const MaskChunk c = chunks[xcd_block()];
+  const int r0 = tfirst[c.tile], nr = tfirst[c.tile + 1] - r0;
+  for (int i = threadIdx.x; i < nr; i += kChunkThreads) s_acc[i] = 0ull;
+  __syncthreads();
+  const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
+  double scale, iscale, B;
+  (void)masked_scale(sc, k, wmax_bits, &scale, &iscale, &B);
+  for (unsigned long long e = c.begin + threadIdx.x; e < c.end; e += kChunkThreads) {
+    const double x = pold[esrc[e]];
+    const double w = kUnit ? 1.0 : ew[e];
+    atomicAdd(&s_acc[erow[e]], (unsigned long long)to_fixed(w * x, scale));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nr; i += kChunkThreads) {
+    const unsigned long long a = s_acc[i];
+    if (a) atomicAdd(&rowsum[r0 + i], a);
+  }
+}
+
+// Part 2, per row: p_new = (b + sum [+ the starting p on the first pass,
+// kernel.cu:8813]) / W where W > 0 (else p stays), the sum cleared for the
+// next iteration; the relative-change partials per workgroup and max |p_new|
+// (slot k & 1); k_iter_finish (two slots) applies the stop rule.
+__global__ void __launch_bounds__(256)
+k_mtile_finish(int k, int nS, const double* __restrict__ bvec, const double* __restrict__ wvec,
+               const int* __restrict__ slist, const double* __restrict__ pinit, int seed,
+               double* __restrict__ p0, double* __restrict__ p1,
+               unsigned long long* __restrict__ rowsum, DevScalars* __restrict__ sc,
+               const unsigned long long* __restrict__ wmax_bits, double* __restrict__ partials) {
+  if (sc->converged) return;
+  __shared__ double s_d2[4], s_n2[4];
+  __shared__ unsigned long long s_pm[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
+  double* __restrict__ pnew = (k & 1) ? p1 : p0;
+  double scale, iscale, B;
+  const bool finite = masked_scale(sc, k, wmax_bits, &scale, &iscale, &B);
+  double d2 = 0.0, n2 = 0.0;
+  unsigned long long pm = 0ull;
+  for (int r = blockIdx.x * 256 + threadIdx.x; r < nS; r += gridDim.x * 256) {
+    const unsigned long long a = rowsum[r];
+    rowsum[r] = 0ull;
+    double tot = bvec[r] + (finite ? (double)(long long)a * iscale : B - B);
+    if (k == 1 && seed && pinit) tot += pinit[slist[r]];
+    const double po = pold[r];
+    const double w = wvec[r];
+    const double pn = w > 0.0 ? tot / w : po;
+    pnew[r] = pn;
+    const double df = pn - po;
+    d2 += df * df;
+    n2 += pn * pn;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(pn));
+    pm = bits > pm ? bits : pm;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    d2 += __shfl_xor(d2, off);
+    n2 += __shfl_xor(n2, off);
+    const unsigned long long o = __shfl_xor(pm, off);
+    pm = o > pm ? o : pm;
+  }
+  if (lane == 0) {
+    s_d2[wave] = d2;
+    s_n2[wave] = n2;
+    s_pm[wave] = pm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    unsigned long long m = 0ull;
+    for (int q = 0; q < 4; q++) {
+      a += s_d2[q];
+      c += s_n2[q];
+      m = s_pm[q] > m ? s_pm[q] : m;
     }
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = c;
+    atomicMax(&sc->itmax_bits[k & 1], m);
   }
 }
 
@@ -684,6 +897,105 @@ k_masked_rowvec(int nS, const int* __restrict__ slist, int fixed, const long lon
   wvec[r] = fixed ? (double)wcnt[v] : wsum[v];
 }
 
+// Compact ids of the flagged voxels (the run starts) in tile order (the
+// voxels of tile 0 in in-tile order, then tile 1, ...): the rows a tile pass
+// builds are then one contiguous id range, tile t's [tfirst[t],
+// tfirst[t + 1]), and consecutive rows are nearby voxels.  Over many workgroups: per 1024 positions the flag count, an
+// exclusive scan of the counts in one workgroup, then each block's ids from
+// its base and ballots.  Positions past the grid (partial tiles) are unflagged.
+constexpr int kIdBlock = 1024;
+
+__device__ __forceinline__ int64_t tile_order_voxel(int64_t q, GridDesc g, TileGrid tg) {
+  const int b = tg.bits;
+  const int64_t tv = 1ll << (3 * b);
+  const int64_t t = q >> (3 * b);
+  const int l = (int)(q & (tv - 1));
+  const int tx = (int)(t % tg.ntx), ty = (int)((t / tg.ntx) % tg.nty), tz = (int)(t / ((int64_t)tg.ntx * tg.nty));
+  const int m = (1 << b) - 1;
+  const int i = (tx << b) + (l & m), j = (ty << b) + ((l >> b) & m), k = (tz << b) + (l >> (2 * b));
+  if (i >= g.X || j >= g.Y || k >= g.Z) return -1;
+  return (int64_t)i + (int64_t)j * g.X + (int64_t)k * g.X * g.Y;
+}
+
+__global__ void __launch_bounds__(kIdBlock)
+k_srcid_partial(const uint8_t* __restrict__ flag, int64_t nq, GridDesc g, TileGrid tg,
+                unsigned int* __restrict__ bcnt) {
+  __shared__ unsigned int s_w[kIdBlock / 64];
+  const int64_t q = (int64_t)blockIdx.x * kIdBlock + threadIdx.x;
+  const int64_t v = q < nq ? tile_order_voxel(q, g, tg) : -1;
+  const unsigned long long m = __ballot(v >= 0 && flag[v]);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = (unsigned int)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int w = 0; w < kIdBlock / 64; w++) t += s_w[w];
+    bcnt[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(kIdBlock)
+k_srcid_scan(unsigned int* __restrict__ bcnt, int64_t nb, int* __restrict__ nS,
+             int* __restrict__ tfirst, int ntiles) {
+  __shared__ unsigned int s_w[kIdBlock / 64];
+  __shared__ unsigned int s_base;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (int64_t b0 = 0; b0 < nb; b0 += kIdBlock) {
+    const int64_t i = b0 + threadIdx.x;
+    const unsigned int v = i < nb ? bcnt[i] : 0u;
+    unsigned int x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    unsigned int wb = 0, tot = 0;
+    for (int w = 0; w < kIdBlock / 64; w++) {
+      if (w < wave) wb += s_w[w];
+      tot += s_w[w];
+    }
+    const unsigned int base = s_base;
+    if (i < nb) bcnt[i] = base + wb + x - v;
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = base + tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *nS = (int)s_base;
+    tfirst[ntiles] = (int)s_base;
+  }
+}
+
+__global__ void __launch_bounds__(kIdBlock)
+k_srcid_final(const uint8_t* __restrict__ flag, int64_t nq, GridDesc g, TileGrid tg,
+              const unsigned int* __restrict__ bbase, int* __restrict__ sid,
+              int* __restrict__ slist, int* __restrict__ tfirst) {
+  __shared__ unsigned int s_w[kIdBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t q = (int64_t)blockIdx.x * kIdBlock + threadIdx.x;
+  const int64_t v = q < nq ? tile_order_voxel(q, g, tg) : -1;
+  const bool f = v >= 0 && flag[v];
+  const unsigned long long m = __ballot(f);
+  if (lane == 0) s_w[wave] = (unsigned int)__popcll(m);
+  __syncthreads();
+  unsigned int wb = 0;
+  for (int w = 0; w < wave; w++) wb += s_w[w];
+  const unsigned int before = bbase[blockIdx.x] + wb + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+  // a tile's first position: the first id of its rows
+  if (q < nq && (q & ((1ll << (3 * tg.bits)) - 1)) == 0) tfirst[q >> (3 * tg.bits)] = (int)before;
+  if (v >= 0) {
+    if (f) {
+      const int id = (int)before;
+      sid[v] = id;
+      slist[id] = (int)v;
+    } else {
+      sid[v] = -1;
+    }
+  }
+}
+
 // Unit weights: the rows' constant in fixed point (the grouped iteration's
 // form, bs * inv_scale) and the rows' entries as 64-bit sort keys (source in
 // the low bits) for the merged-word layout.
@@ -698,79 +1010,6 @@ __global__ void __launch_bounds__(256)
 k_widen_keys(int64_t n, const unsigned int* __restrict__ esrc, unsigned long long* __restrict__ key) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) key[i] = esrc[i];
-}
-
-// One iteration on S: a wavefront per row, its lanes over the row's entries;
-// the products w p[src] rounded to 64-bit fixed point with the scale
-// 2^(51 - e), 2^e > max w * max |p_S^{k-1}|, so every term is below 2^51
-// and a row's sum (at most 4095 entries: the visits of one voxel) below
-// 2^63, and added as integers (the same sum in any order).  Then
-// p_new = (b + sum [+ the starting p on the first pass, kernel.cu:8813]) / W
-// where W > 0, the relative-change partials per workgroup, and max |p_new|
-// (slot k & 1) for the next iteration's scale; k_iter_finish (two slots)
-// applies the stop rule.
-constexpr int kRowWaves = 4;
-__global__ void __launch_bounds__(64 * kRowWaves)
-k_masked_iterate(int k, int nS, const unsigned long long* __restrict__ rowptr,
-                 const unsigned int* __restrict__ esrc, const double* __restrict__ ew,
-                 const double* __restrict__ bvec, const double* __restrict__ wvec,
-                 const int* __restrict__ slist, const double* __restrict__ pinit, int seed,
-                 double* __restrict__ p0, double* __restrict__ p1, DevScalars* __restrict__ sc,
-                 const unsigned long long* __restrict__ wmax_bits, double* __restrict__ partials) {
-  if (sc->converged) return;
-  __shared__ double s_d2[kRowWaves], s_n2[kRowWaves];
-  __shared__ unsigned long long s_pm[kRowWaves];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
-  double* __restrict__ pnew = (k & 1) ? p1 : p0;
-  const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) & 1]) *
-                   __longlong_as_double((long long)*wmax_bits);
-  const bool finite = B <= DBL_MAX;
-  int ex = 0;
-  (void)frexp(finite ? B : 1.0, &ex);
-  ex = ex < -960 ? -960 : ex;
-  const double scale = ldexp(1.0, 51 - ex), iscale = ldexp(1.0, ex - 51);
-  double d2 = 0.0, n2 = 0.0;
-  unsigned long long pm = 0ull;
-  for (int r = blockIdx.x * kRowWaves + wave; r < nS; r += gridDim.x * kRowWaves) {
-    long long acc = 0;
-    for (unsigned long long e = rowptr[r] + lane; e < rowptr[r + 1]; e += 64)
-      acc += to_fixed(ew[e] * pold[esrc[e]], scale);
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    const double po = pold[r];
-    double tot = bvec[r] + (finite ? (double)acc * iscale : B - B);
-    if (k == 1 && seed && pinit) tot += pinit[slist[r]];
-    const double w = wvec[r];
-    const double pn = w > 0.0 ? tot / w : po;
-    if (lane == 0) pnew[r] = pn;
-    const double df = pn - po;
-    d2 += lane == 0 ? df * df : 0.0;
-    n2 += lane == 0 ? pn * pn : 0.0;
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(pn));
-    pm = bits > pm ? bits : pm;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    d2 += __shfl_xor(d2, off);
-    n2 += __shfl_xor(n2, off);
-  }
-  if (lane == 0) {
-    s_d2[wave] = d2;
-    s_n2[wave] = n2;
-    s_pm[wave] = pm;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0, c = 0.0;
-    unsigned long long m = 0ull;
-    for (int q = 0; q < kRowWaves; q++) {
-      a += s_d2[q];
-      c += s_n2[q];
-      m = s_pm[q] > m ? s_pm[q] : m;
-    }
-    partials[2 * blockIdx.x] = a;
-    partials[2 * blockIdx.x + 1] = c;
-    atomicMax(&sc->itmax_bits[k & 1], m);
-  }
 }
 
 // p_S^{k*-1} (the buffer iteration k* read) into the full-volume pressure
@@ -832,32 +1071,46 @@ hipError_t launch_mask_runsrc(int64_t nrays, const unsigned int* nseg,
 
 hipError_t launch_tile_rows(bool fill, int method, int ntiles, const unsigned long long* tile_ptr,
                             const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
-                            const double* aux, const uint32_t* F, const double* wc,
-                            const double* wn, const uint32_t* runsrc, const int* sid,
-                            unsigned int* cnt, const unsigned long long* rowptr,
-                            unsigned int* esrc, double* ew, hipStream_t st) {
+                            const double* aux, const uint32_t* F, const uint32_t* runsrc,
+                            const int* sid, unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, hipStream_t st) {
 #define OMNI_TR(M, B, FL)                                                                       \
   hipLaunchKernelGGL((k_tile_rows<M, B, FL>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, \
-                     tg, wp, aux, F, wc, wn, runsrc, sid, cnt, rowptr, esrc, ew)
+                     tg, wp, aux, F, runsrc, sid, cnt, rowptr, esrc)
 #define OMNI_TR_B(M, FL)                  \
   do {                                    \
     if (tg.bits == 3) OMNI_TR(M, 3, FL);  \
     else OMNI_TR(M, 4, FL);               \
   } while (0)
-#define OMNI_TR_M(FL)                                 \
-  do {                                                \
-    switch (method) {                                 \
-      case kMaskUnit: OMNI_TR_B(kMaskUnit, FL); break;     \
-      case kMaskSelect: OMNI_TR_B(kMaskSelect, FL); break; \
-      default: OMNI_TR_B(kMaskWeighted, FL); break;        \
-    }                                                 \
-  } while (0)
-  if (fill) OMNI_TR_M(true);
-  else OMNI_TR_M(false);
-#undef OMNI_TR_M
+  // (unit weights only: the curl-weighted rows use the tile layout)
+  if (method != kMaskUnit) return hipErrorInvalidValue;
+  if (fill) OMNI_TR_B(kMaskUnit, true);
+  else OMNI_TR_B(kMaskUnit, false);
 #undef OMNI_TR_B
 #undef OMNI_TR
   return hipGetLastError();
+}
+
+static int64_t tile_positions(const TileGrid& tg) {
+  return (int64_t)tg.ntx * tg.nty * tg.ntz << (3 * tg.bits);
+}
+
+hipError_t launch_source_ids_tiled(const uint8_t* flag, GridDesc g, TileGrid tg,
+                                  unsigned int* bcnt, int* sid, int* slist, int* nS,
+                                  int* tfirst, hipStream_t st) {
+  const int64_t nq = tile_positions(tg);
+  const int64_t nb = (nq + kIdBlock - 1) / kIdBlock;
+  const int ntiles = tg.ntx * tg.nty * tg.ntz;
+  hipLaunchKernelGGL(k_srcid_partial, dim3((unsigned int)nb), dim3(kIdBlock), 0, st, flag, nq, g,
+                     tg, bcnt);
+  hipLaunchKernelGGL(k_srcid_scan, dim3(1), dim3(kIdBlock), 0, st, bcnt, nb, nS, tfirst, ntiles);
+  hipLaunchKernelGGL(k_srcid_final, dim3((unsigned int)nb), dim3(kIdBlock), 0, st, flag, nq, g,
+                     tg, bcnt, sid, slist, tfirst);
+  return hipGetLastError();
+}
+
+int64_t source_ids_tiled_blocks(const TileGrid& tg) {
+  return (tile_positions(tg) + kIdBlock - 1) / kIdBlock;
 }
 
 hipError_t launch_masked_bs(int nS, const int* slist, const long long* sumfx, long long* bs,
@@ -873,30 +1126,73 @@ hipError_t launch_widen_keys(int64_t n, const unsigned int* esrc, unsigned long 
   return hipGetLastError();
 }
 
+hipError_t launch_tile_entries(bool fill, int method, int ntiles,
+                               const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                               TileGrid tg, const WalkParams& wp, const double* aux,
+                               const uint32_t* F, const double* wc, const double* wn,
+                               const uint32_t* runsrc, const int* sid, const int* tfirst,
+                               unsigned int* tcnt, const unsigned long long* tbase,
+                               unsigned int* esrc, unsigned short* erow, double* ew,
+                               hipStream_t st) {
+#define OMNI_TE(M, B, FL)                                                                       \
+  hipLaunchKernelGGL((k_tile_entries<M, B, FL>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, \
+                     g, tg, wp, aux, F, wc, wn, runsrc, sid, tfirst, tcnt, tbase, esrc, erow, ew)
+#define OMNI_TE_B(M, FL)                  \
+  do {                                    \
+    if (tg.bits == 3) OMNI_TE(M, 3, FL);  \
+    else OMNI_TE(M, 4, FL);               \
+  } while (0)
+#define OMNI_TE_M(FL)                                      \
+  do {                                                     \
+    switch (method) {                                      \
+      case kMaskUnit: OMNI_TE_B(kMaskUnit, FL); break;     \
+      case kMaskSelect: OMNI_TE_B(kMaskSelect, FL); break; \
+      default: OMNI_TE_B(kMaskWeighted, FL); break;        \
+    }                                                      \
+  } while (0)
+  if (fill) OMNI_TE_M(true);
+  else OMNI_TE_M(false);
+#undef OMNI_TE_M
+#undef OMNI_TE_B
+#undef OMNI_TE
+  return hipGetLastError();
+}
+
+hipError_t launch_mtile_accum(int k, int nchunks, const MaskChunk* chunks, const int* tfirst,
+                              const unsigned int* esrc, const unsigned short* erow,
+                              const double* ew, const double* p0, const double* p1,
+                              const DevScalars* sc, const unsigned long long* wmax_bits,
+                              unsigned long long* rowsum, hipStream_t st) {
+  if (ew)
+    hipLaunchKernelGGL(k_mtile_accum<false>, dim3(nchunks), dim3(kChunkThreads), 0, st, k, chunks,
+                       tfirst, esrc, erow, ew, p0, p1, sc, wmax_bits, rowsum);
+  else
+    hipLaunchKernelGGL(k_mtile_accum<true>, dim3(nchunks), dim3(kChunkThreads), 0, st, k, chunks,
+                       tfirst, esrc, erow, ew, p0, p1, sc, wmax_bits, rowsum);
+  return hipGetLastError();
+}
+
+unsigned int mtile_finish_grid(int nS) {
+  const int b = (nS + 255) / 256;
+  return (unsigned int)(b < 1 ? 1 : (b > 1024 ? 1024 : b));
+}
+
+hipError_t launch_mtile_finish(int k, int nS, const double* bvec, const double* wvec,
+                               const int* slist, const double* pinit, int seed, double* p0,
+                               double* p1, unsigned long long* rowsum, DevScalars* sc,
+                               const unsigned long long* wmax_bits, double* partials,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_mtile_finish, dim3(mtile_finish_grid(nS)), dim3(256), 0, st, k, nS, bvec,
+                     wvec, slist, pinit, seed, p0, p1, rowsum, sc, wmax_bits, partials);
+  return hipGetLastError();
+}
+
 hipError_t launch_masked_rowvec(int nS, const int* slist, int fixed, const long long* sumfx,
                                 const unsigned int* wcnt, const double* sumd, const double* wsum,
                                 const DevScalars* sc, double* bvec, double* wvec,
                                 hipStream_t st) {
   hipLaunchKernelGGL(k_masked_rowvec, dim3(grid_for(nS, 1u << 30)), dim3(256), 0, st, nS, slist,
                      fixed, sumfx, wcnt, sumd, wsum, sc, bvec, wvec);
-  return hipGetLastError();
-}
-
-unsigned int masked_iterate_grid(int nS) {
-  // a wavefront walks many rows: the per-workgroup partials and max stay few
-  const int b = (nS + kRowWaves - 1) / kRowWaves;
-  return (unsigned int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
-}
-
-hipError_t launch_masked_iterate(int k, int nS, const unsigned long long* rowptr,
-                                 const unsigned int* esrc, const double* ew, const double* bvec,
-                                 const double* wvec, const int* slist, const double* pinit,
-                                 int seed, double* p0, double* p1, DevScalars* sc,
-                                 const unsigned long long* wmax_bits, double* partials,
-                                 hipStream_t st) {
-  hipLaunchKernelGGL(k_masked_iterate, dim3(masked_iterate_grid(nS)), dim3(64 * kRowWaves), 0, st,
-                     k, nS, rowptr, esrc, ew, bvec, wvec, slist, pinit, seed, p0, p1, sc,
-                     wmax_bits, partials);
   return hipGetLastError();
 }
 

@@ -169,10 +169,40 @@ hipError_t launch_mask_runsrc(int64_t nrays, const unsigned int* nseg,
 // entries into cnt, fill = true writes them (cnt zeroed: the rows' cursors)
 hipError_t launch_tile_rows(bool fill, int method, int ntiles, const unsigned long long* tile_ptr,
                             const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
-                            const double* aux, const uint32_t* F, const double* wc,
-                            const double* wn, const uint32_t* runsrc, const int* sid,
-                            unsigned int* cnt, const unsigned long long* rowptr,
-                            unsigned int* esrc, double* ew, hipStream_t st);
+                            const double* aux, const uint32_t* F, const uint32_t* runsrc,
+                            const int* sid, unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, hipStream_t st);
+// compact ids of the flagged voxels in tile order, over many workgroups
+// (bcnt: source_ids_tiled_blocks(tg) u32); tile t's rows are
+// [tfirst[t], tfirst[t + 1]) (tfirst: ntiles + 1 ints)
+hipError_t launch_source_ids_tiled(const uint8_t* flag, GridDesc g, TileGrid tg,
+                                  unsigned int* bcnt, int* sid, int* slist, int* nS,
+                                  int* tfirst, hipStream_t st);
+// the masked rows in tile layout and their iteration
+struct MaskChunk {
+  unsigned long long begin, end;  // entries [begin, end) of one tile
+  int tile, pad;
+};
+hipError_t launch_tile_entries(bool fill, int method, int ntiles,
+                               const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                               TileGrid tg, const WalkParams& wp, const double* aux,
+                               const uint32_t* F, const double* wc, const double* wn,
+                               const uint32_t* runsrc, const int* sid, const int* tfirst,
+                               unsigned int* tcnt, const unsigned long long* tbase,
+                               unsigned int* esrc, unsigned short* erow, double* ew,
+                               hipStream_t st);
+hipError_t launch_mtile_accum(int k, int nchunks, const MaskChunk* chunks, const int* tfirst,
+                              const unsigned int* esrc, const unsigned short* erow,
+                              const double* ew, const double* p0, const double* p1,
+                              const DevScalars* sc, const unsigned long long* wmax_bits,
+                              unsigned long long* rowsum, hipStream_t st);
+unsigned int mtile_finish_grid(int nS);
+hipError_t launch_mtile_finish(int k, int nS, const double* bvec, const double* wvec,
+                               const int* slist, const double* pinit, int seed, double* p0,
+                               double* p1, unsigned long long* rowsum, DevScalars* sc,
+                               const unsigned long long* wmax_bits, double* partials,
+                               hipStream_t st);
+int64_t source_ids_tiled_blocks(const TileGrid& tg);
 hipError_t launch_masked_bs(int nS, const int* slist, const long long* sumfx, long long* bs,
                            hipStream_t st);
 hipError_t launch_widen_keys(int64_t n, const unsigned int* esrc, unsigned long long* key,
@@ -181,13 +211,6 @@ hipError_t launch_masked_rowvec(int nS, const int* slist, int fixed, const long 
                                 const unsigned int* wcnt, const double* sumd, const double* wsum,
                                 const DevScalars* sc, double* bvec, double* wvec,
                                 hipStream_t st);
-unsigned int masked_iterate_grid(int nS);
-hipError_t launch_masked_iterate(int k, int nS, const unsigned long long* rowptr,
-                                 const unsigned int* esrc, const double* ew, const double* bvec,
-                                 const double* wvec, const int* slist, const double* pinit,
-                                 int seed, double* p0, double* p1, DevScalars* sc,
-                                 const unsigned long long* wmax_bits, double* partials,
-                                 hipStream_t st);
 hipError_t launch_masked_scatter(int nS, const int* slist, const DevScalars* sc, const double* p0,
                                  const double* p1, double* pfull, hipStream_t st);
 hipError_t launch_masked_final_scalars(const DevScalars* sc, DevScalars* out, hipStream_t st);
