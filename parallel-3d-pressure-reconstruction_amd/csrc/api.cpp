@@ -499,17 +499,42 @@ int build_plan(omni3d_ctx* ctx) {
       // slack after the words: the iteration stages them with 16-byte loads
       OMNI_TRY_HIP(ctx, pl.dent.alloc(sizeof(unsigned int) * (nnz + 8)));
       OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.dent.as<void>(), 0, sizeof(unsigned int) * (nnz + 8), st));
+      // groups of at most 256 consecutive rows holding about equal entries:
+      // one workgroup per group, as many groups as fill the CUs evenly, so
+      // the launch does not wait on the largest uniform group
+      DevBuf rgroup;
+      if (grp > 1) {
+        std::vector<unsigned long long> hrp((size_t)nS + 1);
+        OMNI_TRY_HIP(ctx, hipMemcpy(hrp.data(), pl.rowptr.as<void>(),
+                                    sizeof(unsigned long long) * ((size_t)nS + 1),
+                                    hipMemcpyDeviceToHost));
+        int dev = 0, ncu = 256;
+        if (hipGetDevice(&dev) == hipSuccess)
+          (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        ncu = std::max(ncu, 1);
+        std::vector<int> hgrow, hrg((size_t)nS);
+        balance_groups(hrp, 256, ncu, &hgrow, &hrg);
+        pl.dense_ngroups = (int)hgrow.size() - 1;
+        OMNI_TRY_HIP(ctx, pl.dense_grow.alloc(sizeof(int) * hgrow.size()));
+        OMNI_TRY_HIP(ctx, rgroup.alloc(sizeof(int) * (size_t)std::max(nS, 1)));
+        OMNI_TRY_HIP(ctx, hipMemcpy(pl.dense_grow.as<void>(), hgrow.data(),
+                                    sizeof(int) * hgrow.size(), hipMemcpyHostToDevice));
+        OMNI_TRY_HIP(ctx, hipMemcpy(rgroup.as<void>(), hrg.data(), sizeof(int) * (size_t)nS,
+                                    hipMemcpyHostToDevice));
+      }
       OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, grp, pl.rowptr.as<unsigned long long>(),
                                             key.as<unsigned long long>(),
                                             ka.as<unsigned long long>(), kb.as<unsigned long long>(),
                                             tmp.as<void>(), tmp.size(), pl.dent.as<unsigned int>(),
-                                            st));
+                                            st, grp > 1 ? rgroup.as<int>() : nullptr,
+                                            grp > 1 ? pl.dense_grow.as<int>() : nullptr,
+                                            pl.dense_ngroups));
       pl.nwords = (int64_t)nnz;
       if (grp > 1) {
         DevBuf flag, excl;
         OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * nnz));
         OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * nnz));
-        const int ng = (nS + 64 * grp - 1) / (64 * grp);
+        const int ng = pl.dense_ngroups;
         OMNI_TRY_HIP(ctx, pl.dense_wptr.alloc(sizeof(unsigned int) * (ng + 1)));
         unsigned int nw = 0;
         OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)nnz, grp,
@@ -517,7 +542,8 @@ int build_plan(omni3d_ctx* ctx) {
                                               kb.as<unsigned long long>(), flag.as<unsigned int>(),
                                               excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
                                               pl.dent.as<unsigned int>(),
-                                              pl.dense_wptr.as<unsigned int>(), &nw, st));
+                                              pl.dense_wptr.as<unsigned int>(), &nw, st,
+                                              pl.dense_grow.as<int>(), ng));
         pl.nwords = nw;
       }
       OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
@@ -527,7 +553,7 @@ int build_plan(omni3d_ctx* ctx) {
         std::fprintf(stderr, "omni3d plan: %lld entries in %lld dense words (groups of %d chunks)\n",
                      (long long)nnz, (long long)pl.nwords, grp);
         if (grp > 1) {
-          const int ng = (nS + 64 * grp - 1) / (64 * grp);
+          const int ng = pl.dense_ngroups;
           std::vector<unsigned int> wp(ng + 1);
           if (hipMemcpy(wp.data(), pl.dense_wptr.as<void>(), sizeof(unsigned int) * (ng + 1),
                         hipMemcpyDeviceToHost) == hipSuccess) {
@@ -625,6 +651,53 @@ WalkParams walk_params(const omni3d_params& P) {
   return wp;
 }
 
+// Groups of consecutive rows for the grouped iteration: at most maxrows rows
+// each and G = the uniform group count rounded up to a multiple of ncu
+// groups at most, with the smallest entry budget per group for which the
+// greedy split fits in G.  The launch runs one workgroup per group and the
+// CUs take G / ncu groups each, so equal groups make equal work per CU.
+// grow: first row of each group and nS at the end; rgroup: the group of
+// every row.
+void balance_groups(const std::vector<unsigned long long>& rowptr, int maxrows, int ncu,
+                    std::vector<int>* grow, std::vector<int>* rgroup) {
+  const int nS = (int)rowptr.size() - 1;
+  const unsigned long long nnz = rowptr[nS];
+  long long G = std::max<long long>((nS + maxrows - 1) / maxrows, 1);
+  G = (G + ncu - 1) / ncu * ncu;
+  auto split = [&](unsigned long long budget, bool store) -> long long {
+    long long groups = 1;
+    unsigned long long acc = 0;
+    int rows = 0;
+    if (store) {
+      grow->clear();
+      grow->push_back(0);
+    }
+    for (int r = 0; r < nS; r++) {
+      const unsigned long long e = rowptr[r + 1] - rowptr[r];
+      if (rows > 0 && (rows == maxrows || acc + e > budget)) {
+        groups++;
+        if (store) grow->push_back(r);
+        acc = 0;
+        rows = 0;
+      }
+      if (store) (*rgroup)[r] = (int)grow->size() - 1;
+      acc += e;
+      rows++;
+    }
+    if (store) grow->push_back(nS);
+    return groups;
+  };
+  // the smallest budget whose split has at most G groups (a budget of nnz
+  // gives the uniform count, which is at most G)
+  unsigned long long lo = std::max<unsigned long long>(1ull, nnz / (unsigned long long)G), hi = std::max(nnz, 1ull);
+  while (lo < hi) {
+    const unsigned long long mid = lo + (hi - lo) / 2;
+    if (split(mid, false) <= G) hi = mid;
+    else lo = mid + 1;
+  }
+  (void)split(lo, true);
+}
+
 template <typename T>
 int run_affine(omni3d_ctx* ctx, int seed) {
   const omni3d_params& P = ctx->prm;
@@ -688,7 +761,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                         sizeof(double) * 2 *
                         std::max(iterate_grid(pl.nS),
                                  std::max(study_grid(pl.nS),
-                                          std::max(dense_grid(pl.nS), 2 * grouped_grid(pl.nS))))));
+                                          std::max(dense_grid(pl.nS),
+                                                   2 * grouped_grid_n(pl.dense_ngroups))))));
   if (phase != 2) {
     OMNI_TRY_HIP(ctx, launch_gather_sources(pl.slist.as<int>(), pl.nS, pinit, ctx->ps0.as<double>(), st));
     // max |p_S| of the starting values: the fixed-point scale of iteration 1
@@ -737,12 +811,13 @@ int run_affine(omni3d_ctx* ctx, int seed) {
             rot3 ? pb3[(k - 1) % 3] : pbuf[(k - 1) & 1], rot3 ? pb3[k % 3] : pbuf[k & 1], sc,
             ctx->partials.as<double>(), P.max_iterations, P.tolerance, s,
             inc ? ctx->srow.as<double>() : nullptr, inc ? dbuf[(k - 1) & 1] : nullptr,
-            inc ? dbuf[k & 1] : nullptr);
+            inc ? dbuf[k & 1] : nullptr, pl.dense_group > 1 ? pl.dense_grow.as<int>() : nullptr,
+            pl.dense_ngroups);
         if (e != hipSuccess) return e;
         if (rot3 && k == P.max_iterations) {
           // the stop rule of the last iteration (each grouped launch runs
           // the previous one's), then p_S^{k*-1} for pass B
-          const unsigned int gg = grouped_grid(pl.nS);
+          const unsigned int gg = grouped_grid_n(pl.dense_ngroups);
           hipError_t e2 = launch_iter_finish(
               k, (int)gg, ctx->partials.as<double>() + (size_t)(k & 1) * 2 * gg, sc,
               P.max_iterations, P.tolerance, s);
@@ -779,7 +854,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
     // p_S^{k*-1} for pass B
     if (rot3) {
       const int k = P.max_iterations;
-      const unsigned int gg = grouped_grid(pl.nS);
+      const unsigned int gg = grouped_grid_n(pl.dense_ngroups);
       OMNI_TRY_HIP(ctx, launch_iter_finish(k, (int)gg,
                                            ctx->partials.as<double>() + (size_t)(k & 1) * 2 * gg,
                                            sc, P.max_iterations, P.tolerance, st));
@@ -1389,7 +1464,9 @@ IterDesc affine_iter_desc(omni3d_ctx* ctx, int first) {
   const bool inc = ctx->prm.precision == OMNI3D_FP32_KAHAN && !std::getenv("OMNI3D_NO_INCREMENTS");
   IterDesc d{};
   d.nS = pl.nS;
-  d.nb = (int)grouped_grid(pl.nS);
+  d.nb = (int)grouped_grid_n(pl.dense_ngroups);
+  d.grow = pl.dense_grow.as<int>();
+  d.ngr = pl.dense_ngroups;
   d.first = first;
   d.seed = 0;
   d.wptr = pl.dense_wptr.as<unsigned int>();

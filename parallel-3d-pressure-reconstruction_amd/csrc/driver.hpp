@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "host.hpp"
 #include "stage.hpp"
@@ -28,6 +29,11 @@ bool trilinear(const omni3d_params& P);
 bool ray_engine(const omni3d_ctx* ctx);
 // bound on a visit's path for the fixed-point scale (see launch_compute_scale)
 double path_bound(const omni3d_params& P, const Plan& pl);
+// Row groups of the grouped iteration with about equal entries (rowptr:
+// nS + 1 CSR offsets on the host): at most maxrows rows each, grow the
+// first row of every group (nS last), rgroup[nS] the group of every row.
+void balance_groups(const std::vector<unsigned long long>& rowptr, int maxrows, int ncu,
+                    std::vector<int>* grow, std::vector<int>* rgroup);
 // Reference-schedule iteration; the result of iteration k* goes to
 // pdst/wdst (device, dtype), seed: -1 no initial pressure, 0 p only,
 // 1 p and the seed term.

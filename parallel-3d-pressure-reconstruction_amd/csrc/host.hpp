@@ -99,6 +99,8 @@ struct Plan {
   int64_t nwords = 0;  // dense words (merged equal entries for groups of 4)
   DevBuf dent;         // uint32[nnz+8]
   DevBuf dense_wptr;   // uint32[ngroups+1] first word of each group (groups of 4)
+  DevBuf dense_grow;   // int32[ngroups+1] first row of each group (groups of 4)
+  int dense_ngroups = 0;
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
   int ntiles = 0;

@@ -552,7 +552,7 @@ template <int kG, int kInc>
 __device__ __forceinline__ void iterate_grp_body(
     const unsigned int lb, const unsigned int nb, int k, int nS,
     const unsigned int* __restrict__ wptr, const unsigned int* __restrict__ dent,
-    const unsigned int* __restrict__ diag, const int* __restrict__ slist,
+    const int* __restrict__ grow, int ngr, const unsigned int* __restrict__ diag, const int* __restrict__ slist,
     const unsigned int* __restrict__ cnt, const long long* __restrict__ bs,
     const double* __restrict__ pinit, int seed, const double* __restrict__ pold,
     double* __restrict__ pnew, double* __restrict__ srow, const float* __restrict__ din,
@@ -590,11 +590,13 @@ __device__ __forceinline__ void iterate_grp_body(
   const double scale = ldexp(1.0, 51 - ex), iscale = ldexp(1.0, ex - 51);
   double d2 = 0.0, n2 = 0.0;
   unsigned long long pm = 0ull;
-  const int ngr = (nS + kRows - 1) / kRows;
   for (int g = (int)lb; g < ngr; g += (int)nb) {
     s_acc[threadIdx.x] = 0ull;
     __syncthreads();
-    const int rbeg = g * kRows;
+    // the group's rows: [grow[g], grow[g + 1]) (at most 256, chosen so that
+    // groups hold about equal words), or 256 consecutive rows
+    const int rbeg = grow ? grow[g] : g * kRows;
+    const int nrows = grow ? grow[g + 1] - rbeg : min(kRows, nS - rbeg);
     const unsigned long long e0 = wptr[g], e1 = wptr[g + 1];
     const unsigned long long q1 = (e1 + 3) >> 2;
     unsigned long long base = (e0 & ~3ull) + (unsigned long long)wave * (kU * 64);
@@ -651,7 +653,7 @@ __device__ __forceinline__ void iterate_grp_body(
     }
     __syncthreads();
     const int row = rbeg + wave * 64 + lane;
-    if (row < nS) {
+    if (wave * 64 + lane < nrows) {
       double acc = (double)(long long)s_acc[wave * 64 + lane] * iscale;
       if (!finite) acc = B - B;
       if constexpr (kInc == 2) acc += srow[row];
@@ -706,16 +708,17 @@ __device__ __forceinline__ void iterate_grp_body(
 template <int kG, int kInc>
 __global__ void __launch_bounds__(64 * kG)
 k_iterate_grp(int k, int nS, const unsigned int* __restrict__ wptr,
-              const unsigned int* __restrict__ dent, const unsigned int* __restrict__ diag,
+              const unsigned int* __restrict__ dent, const int* __restrict__ grow, int ngr,
+              const unsigned int* __restrict__ diag,
               const int* __restrict__ slist, const unsigned int* __restrict__ cnt,
               const long long* __restrict__ bs, const double* __restrict__ pinit, int seed,
               const double* __restrict__ pold, double* __restrict__ pnew,
               double* __restrict__ srow, const float* __restrict__ din, float* __restrict__ dnew,
               DevScalars* __restrict__ sc, double* __restrict__ partials, int max_iter,
               double tol) {
-  iterate_grp_body<kG, kInc>(blockIdx.x, gridDim.x, k, nS, wptr, dent, diag, slist, cnt, bs,
-                             pinit, seed, pold, pnew, srow, din, dnew, sc, partials, max_iter,
-                             tol);
+  iterate_grp_body<kG, kInc>(blockIdx.x, gridDim.x, k, nS, wptr, dent, grow, ngr, diag, slist,
+                             cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc, partials,
+                             max_iter, tol);
 }
 
 // Iteration k of many snapshots in one launch (a snapshot batch): workgroup
@@ -729,7 +732,8 @@ k_iterate_multi(int k, const IterDesc* __restrict__ desc, const unsigned int* __
                 int max_iter, double tol) {
   const IterDesc& q = desc[blk_snap[blockIdx.x]];
   const unsigned int lb = blockIdx.x - (unsigned int)q.first;
-  iterate_grp_body<kG, kInc>(lb, (unsigned int)q.nb, k, q.nS, q.wptr, q.dent, q.diag, q.slist,
+  iterate_grp_body<kG, kInc>(lb, (unsigned int)q.nb, k, q.nS, q.wptr, q.dent, q.grow, q.ngr,
+                             q.diag, q.slist,
                              q.cnt, q.bs, q.pinit, q.seed, q.pb[(k - 1) % 3], q.pb[k % 3],
                              q.srow, kInc == 2 ? q.db[(k - 1) & 1] : nullptr,
                              kInc != 0 ? q.db[k & 1] : nullptr, q.sc, q.partials, max_iter, tol);
@@ -1061,16 +1065,18 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 const int* slist, const unsigned int* cnt, const long long* bs,
                                 const double* pinit, int seed, const double* pold, double* pnew,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
-                                hipStream_t st, double* srow, const float* din, float* dnew) {
+                                hipStream_t st, double* srow, const float* din, float* dnew,
+                                const int* grow, int ngroups) {
   unsigned int grid = dense_grid(nS);
   if (gchunks > 1) {
     const int rows = 64 * gchunks;
-    grid = (unsigned int)std::max(1, std::min((nS + rows - 1) / rows, 65536));
+    const int ngr = grow ? ngroups : (nS + rows - 1) / rows;
+    grid = grouped_grid_n(ngr);
     const int inc = srow ? (k == 1 ? 1 : 2) : 0;
 #define OMNI_GRP(I)                                                                            \
   hipLaunchKernelGGL((k_iterate_grp<4, I>), dim3(grid), dim3(256), 0, st, k, nS, wptr, dent,     \
-                     diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew, sc, partials, \
-                     max_iter, tol)
+                     grow, ngr, diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew,   \
+                     sc, partials, max_iter, tol)
     if (inc == 1) OMNI_GRP(1);
     else if (inc == 2) OMNI_GRP(2);
     else OMNI_GRP(0);

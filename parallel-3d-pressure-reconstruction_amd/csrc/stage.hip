@@ -148,11 +148,13 @@ unsigned int grid1(int64_t n) { return (unsigned int)((n + 255) / 256); }
 // (CSR keys are (direction << 32) | source); a group is 2^gshift rows.
 __global__ void k_dense_keys(int nS, int gshift, int rb, const unsigned long long* __restrict__ rowptr,
                              const unsigned long long* __restrict__ ckey,
+                             const int* __restrict__ rgroup, const int* __restrict__ grow,
                              unsigned long long* __restrict__ keys) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nS) return;
-  const unsigned long long hi = ((unsigned long long)(r >> gshift) << 32) |
-                                (unsigned long long)(r & ((1 << gshift) - 1));
+  const int g = rgroup ? rgroup[r] : r >> gshift;
+  const int rin = rgroup ? r - grow[g] : r & ((1 << gshift) - 1);
+  const unsigned long long hi = ((unsigned long long)g << 32) | (unsigned long long)rin;
   const unsigned long long smask = (1ull << (32 - rb)) - 1ull;
   for (unsigned long long j = rowptr[r]; j < rowptr[r + 1]; j++)
     keys[j] = hi | ((ckey[j] & smask) << rb);
@@ -197,10 +199,10 @@ __global__ void k_merge_write(int64_t n, int rb, const unsigned long long* __res
 __global__ void k_merge_gptr(int ng, int rows, int nS, int64_t n,
                              const unsigned long long* __restrict__ rowptr,
                              const unsigned int* __restrict__ excl, unsigned int nwords,
-                             unsigned int* __restrict__ wptr) {
+                             const int* __restrict__ grow, unsigned int* __restrict__ wptr) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g > ng) return;
-  const long long r = (long long)g * rows;
+  const long long r = grow ? (long long)grow[g] : (long long)g * rows;
   const unsigned long long pos = rowptr[r < nS ? r : nS];
   wptr[g] = pos < (unsigned long long)n ? excl[pos] : nwords;
 }
@@ -219,7 +221,8 @@ size_t dense_tmp_bytes(int64_t nnz) {
 hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* keys, unsigned int* flag,
                                unsigned int* excl, void* tmp, size_t tmp_bytes, unsigned int* dent,
-                               unsigned int* wptr, unsigned int* nwords, hipStream_t st) {
+                               unsigned int* wptr, unsigned int* nwords, hipStream_t st,
+                               const int* grow, int ngroups) {
   const int rb = dense_row_bits(gchunks);
   hipLaunchKernelGGL(k_merge_flags, dim3(grid1(nnz)), dim3(256), 0, st, nnz, keys, flag);
   hipError_t e = hipGetLastError();
@@ -240,23 +243,24 @@ hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int rows = 64 * gchunks;
-  const int ng = (nS + rows - 1) / rows;
+  const int ng = grow ? ngroups : (nS + rows - 1) / rows;
   hipLaunchKernelGGL(k_merge_gptr, dim3(grid1(ng + 1)), dim3(256), 0, st, ng, rows, nS, nnz, rowptr,
-                     excl, *nwords, wptr);
+                     excl, *nwords, grow, wptr);
   return hipGetLastError();
 }
 
 hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* csr_key, unsigned long long* keys_a,
                                unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
-                               unsigned int* dent, hipStream_t st) {
+                               unsigned int* dent, hipStream_t st, const int* rgroup,
+                               const int* grow, int ngroups) {
   const int gshift = 6 + bits_for((unsigned long long)gchunks);
   const int rb = dense_row_bits(gchunks);
   hipLaunchKernelGGL(k_dense_keys, dim3(grid1(nS)), dim3(256), 0, st, nS, gshift, rb, rowptr,
-                     csr_key, keys_a);
+                     csr_key, rgroup, grow, keys_a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int ng = (nS + (1 << gshift) - 1) >> gshift;
+  const int ng = rgroup ? ngroups : (nS + (1 << gshift) - 1) >> gshift;
   const int end_bit = 32 + bits_for((unsigned long long)ng);
   size_t bytes = tmp_bytes;
   e = hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys_a, keys_b, (int)nnz, 0, end_bit, st);

@@ -76,19 +76,24 @@ hipError_t launch_iter_finish(int k, int nparts, const double* partials, DevScal
 hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const double* b2,
                             const DevScalars* sc, double* out, hipStream_t st);
 // grid of the grouped dense iteration (4 chunks per workgroup)
-inline unsigned int grouped_grid(int nS) {
-  const int g = (nS + 255) / 256;
-  return (unsigned int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+inline unsigned int grouped_grid_n(int ngroups) {
+  return (unsigned int)(ngroups < 1 ? 1 : (ngroups > 65536 ? 65536 : ngroups));
 }
+// workgroups of the grouped iteration over uniform groups of 256 rows
+inline unsigned int grouped_grid(int nS) { return grouped_grid_n((nS + 255) / 256); }
 // Row bits of a dense word for groups of gchunks chunks: (row << (32 - bits)) | source.
 constexpr int dense_row_bits(int gchunks) { return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10); }
 // gchunks: chunks per group sharing the LDS row sums (1, 4, 8 or 16); the
 // group's words are sorted by source.  For gchunks > 1 this leaves the sorted
 // keys in keys_b for merge_dense_entries.
+// With rgroup / grow (groups > 1 only): row r belongs to group rgroup[r],
+// whose rows are [grow[g], grow[g + 1]) (at most 64 * gchunks), ngroups in
+// all; without them, groups of 64 * gchunks consecutive rows.
 hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* csr_key, unsigned long long* keys_a,
                                unsigned long long* keys_b, void* tmp, size_t tmp_bytes,
-                               unsigned int* dent, hipStream_t st);
+                               unsigned int* dent, hipStream_t st, const int* rgroup = nullptr,
+                               const int* grow = nullptr, int ngroups = 0);
 size_t dense_tmp_bytes(int64_t nnz);
 // Groups (gchunks > 1): equal (group, source, row) keys of the sorted keys
 // merged into words (row << 24) | (multiplicity - 1) << 22 | source (sources
@@ -96,7 +101,8 @@ size_t dense_tmp_bytes(int64_t nnz);
 hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* keys, unsigned int* flag,
                                unsigned int* excl, void* tmp, size_t tmp_bytes, unsigned int* dent,
-                               unsigned int* wptr, unsigned int* nwords, hipStream_t st);
+                               unsigned int* wptr, unsigned int* nwords, hipStream_t st,
+                               const int* grow = nullptr, int ngroups = 0);
 // gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
 // entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
 // 4: 4 x 1024); gchunks 4: one group per workgroup of 4 wavefronts sharing
@@ -112,14 +118,17 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 const double* pinit, int seed, const double* pold, double* pnew,
                                 DevScalars* sc, double* partials, int max_iter, double tol,
                                 hipStream_t st, double* srow = nullptr,
-                                const float* din = nullptr, float* dnew = nullptr);
+                                const float* din = nullptr, float* dnew = nullptr,
+                                const int* grow = nullptr, int ngroups = 0);
 // One snapshot's share of a multi-snapshot launch of the grouped iteration:
-// its nb = grouped_grid(nS) workgroups start at block `first`; p_S rotates
+// its nb = grouped_grid_n(ngr) workgroups start at block `first`; p_S rotates
 // over pb[0..2] and the increments over db[0..1] as in its own launches.
 struct IterDesc {
   int nS, nb, first, seed;
   const unsigned int* wptr;
   const unsigned int* dent;
+  const int* grow;  // rows of each group (nullptr: 256 consecutive rows)
+  int ngr;          // groups
   const unsigned int* diag;
   const int* slist;
   const unsigned int* cnt;
