@@ -17,7 +17,8 @@ of the FP32 result against it, speed ratio), the cost of one reference-
 schedule FP64 pass (a full traversal per iteration, as the reference
 iterates), the per-phase times from the library's HIP events on its stream,
 the roofline of the dominant kernel, the source iteration (with the two
-ray traversals beside it), the trilinear traversal's accuracy against the
+ray traversals beside it: their counter traffic over their time), the
+Poisson path (GPU red-black SOR, the CPU solvers), the trilinear traversal's accuracy against the
 analytic pressure beside the staircase's, and the oracle's single-thread CPU
 time on a bounded sample (cpu_baseline; --cpu-full times a whole pass).
 
@@ -711,13 +712,20 @@ def main():
             "kernel": "k_iterate_grp<4, increments> (one source iteration)",
             "algorithmic_bytes": it_bytes, "ms": round(it_ms, 5),
             "matrix_words": reps[-1].matrix_words, "source_visits": reps[-1].source_visits,
-            # the two ray traversals, in the per-direction unit D*V*12 + 20V
+            # the two ray traversals: the bytes they move (counter traffic per
+            # launch) over their time against the HBM peak, and beside it the
+            # per-direction model D*V*12 + 20V (above 1: the tile engine
+            # streams each tile's gradient once for every direction crossing it)
             "traversals": {
                 k: {"kernel": kern[k][0], "ms": round(kern[k][1], 4),
-                    "algorithmic_bytes": bytes_pass,
-                    "achieved": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9, 1),
-                    "frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "traffic": traffic.get(k)} for k in kern}}
+                    "traffic": traffic.get(k),
+                    "achieved": (round(traffic[k] / (kern[k][1] * 1e-3) / 1e9, 1)
+                                 if traffic.get(k) else None),
+                    "frac": (round(traffic[k] / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                             if traffic.get(k) else None),
+                    "model_bytes": bytes_pass,
+                    "model_frac": round(bytes_pass / (kern[k][1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                for k in kern}}
 
     cpu = None
     if not args.no_cpu:
