@@ -72,6 +72,17 @@ extern "C" {
  * the path through the smallest curl toward the exit; one lane per ray,
  * reference schedule */
 #define OMNI3D_METHOD_MINICURL 5
+/* bodyIntegralInnerWeightedFixedBC, kernel.cu:2136-2279 (no launcher calls
+ * it): the Weighted sums and weights along the plain staircase, except that
+ * a step onto the j = 0 plane ends the ray; one lane per ray, reference
+ * schedule */
+#define OMNI3D_METHOD_WEIGHTED_FIXEDBC 6
+/* bodyIntegralInnerWeightedMiniCurl, kernel.cu:2937-3085 (no launcher calls
+ * it): the MiniCurl path with curl weights counttmp / curltmp, curltmp the
+ * running sum of curl (not / 10^4) starting at the entry voxel, which adds
+ * its own p with weight 1 / curl[entry]; a zero curl sum gives infinite
+ * weights, as in the reference; one lane per ray, reference schedule */
+#define OMNI3D_METHOD_WEIGHTED_MINICURL 7
 
 /* Arithmetic of the ray sums. */
 #define OMNI3D_FP64 0        /* gradient and ray sums in FP64 */

@@ -227,7 +227,8 @@ typedef struct {
   int c[3], e[3];
   double u[3];
   const orc_dir* d;
-  int fixedbc; /* the step rule of bodyIntegralInnerSelectFixedBC */
+  int fixedbc; /* 1: the step rule of bodyIntegralInnerSelectFixedBC; 2: the
+                  plain rule without a y-step onto j = 0 (WeightedFixedBC) */
 } walker;
 
 static void walker_init(walker* w, int X, int Y, int Z, const orc_dir* d, const double o[3],
@@ -259,7 +260,7 @@ static int walker_choose(const walker* w, int* s) {
     f[a] = (2.0 * dd) * w->u[a] + (dd * dd) * w->d->M[a][a];
   }
   int a;
-  if (w->fixedbc) {
+  if (w->fixedbc == 1) {
     /* kernel.cu:2628,2676,2722: an x / y tie goes to y; no x- or z-step
      * while on the j = 0 plane, no y-step onto it (the plane keeps its
      * prescribed pressure: only rays entering there read it) */
@@ -274,6 +275,9 @@ static int walker_choose(const walker* w, int* s) {
     else if (f[1] < f[0] && f[1] <= f[2]) a = 1;
     else if (f[2] < f[0] && f[2] < f[1]) a = 2;
     else return -1;
+    /* kernel.cu:2226: bodyIntegralInnerWeightedFixedBC takes the y-step only
+     * when jnext2 > 0; otherwise no branch runs and the walk ends */
+    if (w->fixedbc == 2 && a == 1 && w->c[1] + st[1] == 0) return -1;
   }
   int n = w->c[a] + st[a];
   if (n < 0 || n >= size[a]) return -1;
@@ -469,8 +473,56 @@ static int walk_minicurl(const pass_ctx* P, walker* w) {
   return steps;
 }
 
+/* bodyIntegralInnerWeightedMiniCurl, kernel.cu:2937-3085: the path of
+ * walk_minicurl; curltmp = curl[entry] and counttmp = 1 at the entry, which
+ * adds p[entry] * 1 / curltmp * counttmp (kernel.cu:2956-2960); each step adds
+ * curl[next] (not / 1e4) to curltmp, 1 to counttmp, and (p[entry] + pint) *
+ * 1 / curltmp * counttmp with no test of curltmp (kernel.cu:3029-3060). */
+static int walk_weighted_minicurl(const pass_ctx* P, walker* w) {
+  const int X = P->X, Y = P->Y, Z = P->Z;
+  const int64_t V = (int64_t)X * Y * Z;
+  const int size[3] = {X, Y, Z};
+  const int64_t stride[3] = {1, X, (int64_t)X * Y};
+  const int64_t vin = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+  const double pin = P->p[vin];
+  if (P->src) P->src[vin] = 1;
+  double curltmp = P->curl[vin], counttmp = 1.0;
+  P->pn[vin] += pin * 1 / curltmp * counttmp;
+  P->cnt[vin] += 1 / curltmp * counttmp;
+  double pint = 0.0;
+  int steps = 0;
+  while (!walker_done(w)) {
+    const int64_t v = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
+    int st[3];
+    double d[3];
+    for (int a = 0; a < 3; a++) {
+      st[a] = w->c[a] < w->e[a] ? 1 : (w->c[a] > w->e[a] ? -1 : -60000);
+      const int64_t idx = v + (int64_t)st[a] * stride[a];
+      d[a] = (idx >= 0 && idx < V) ? P->curl[idx] : 1e10;
+    }
+    int a;
+    if (d[0] <= d[1] && d[0] <= d[2]) a = 0;
+    else if (d[1] < d[0] && d[1] <= d[2]) a = 1;
+    else if (d[2] < d[0] && d[2] < d[1]) a = 2;
+    else break;
+    const int n = w->c[a] + st[a];
+    if (n < 0 || n >= size[a]) break;
+    w->c[a] = n;
+    const int64_t vn = v + (int64_t)st[a] * stride[a];
+    pint += trap(P->rho, st[a], P->del[a], P->acc[a][vn], P->acc[a][v]);
+    curltmp += P->curl[vn];
+    counttmp += 1.0;
+    P->pn[vn] += (pin + pint) * 1 / curltmp * counttmp;
+    P->cnt[vn] += 1 / curltmp * counttmp;
+    steps++;
+  }
+  return steps;
+}
+
 /* bodyIntegralInnerWeighted, kernel.cu:1993-2135: no entry self-add; weight
- * counttmp/curltmp with curltmp = sum(curl/1e4) over the path so far. */
+ * counttmp/curltmp with curltmp = sum(curl/1e4) over the path so far.  With
+ * w->fixedbc == 2 it is bodyIntegralInnerWeightedFixedBC (kernel.cu:2136-2279):
+ * the same walk ended by a y-step onto the j = 0 plane. */
 static int walk_weighted(const pass_ctx* P, walker* w) {
   const int X = P->X, Y = P->Y;
   int64_t vin = vidx(X, Y, w->c[0], w->c[1], w->c[2]);
@@ -602,6 +654,11 @@ void orc_pass(int X, int Y, int Z, int D, const float* k1, const float* k2, cons
           walk_select(&P, &w, 0);
           break;
         case ORC_MINICURL: walk_minicurl(&P, &w); break;
+        case ORC_WEIGHTED_FIXEDBC:
+          w.fixedbc = 2;
+          walk_weighted(&P, &w);
+          break;
+        case ORC_WEIGHTED_MINICURL: walk_weighted_minicurl(&P, &w); break;
         case ORC_PLAIN_TRILINEAR: walk_plain_trilinear(&P, &w); break;
         default: break;
       }

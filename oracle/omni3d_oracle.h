@@ -25,7 +25,9 @@ enum { ORC_PLAIN = 0, ORC_PLAIN_MASK = 1, ORC_SELECT = 2, ORC_WEIGHTED = 3,
         * along the straight ray through trilinear samples of the gradient
         * (no reference counterpart: the accuracy mode beside the staircase
         * of kernel.cu:1743-1863; see walk_plain_trilinear). */
-       ORC_PLAIN_TRILINEAR = 6 };
+       ORC_PLAIN_TRILINEAR = 6,
+       ORC_WEIGHTED_FIXEDBC = 7, /* bodyIntegralInnerWeightedFixedBC, kernel.cu:2136-2279 */
+       ORC_WEIGHTED_MINICURL = 8 /* bodyIntegralInnerWeightedMiniCurl, kernel.cu:2937-3085 */ };
 
 /* Per-direction constants derived from one unit vector. */
 typedef struct {

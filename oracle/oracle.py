@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
 # the plain method integrated along the straight ray (trilinear samples)
 PLAIN_TRILINEAR = 6
+# the two integrators no reference launcher calls
+WEIGHTED_FIXEDBC, WEIGHTED_MINICURL = 7, 8
 
 _lib = None
 

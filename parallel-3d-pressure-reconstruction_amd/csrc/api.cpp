@@ -44,7 +44,7 @@ bool params_ok(const omni3d_params& p, std::string* why) {
     *why = "line spacing must be positive";
     return false;
   }
-  if (p.method < OMNI3D_METHOD_PLAIN || p.method > OMNI3D_METHOD_MINICURL) {
+  if (p.method < OMNI3D_METHOD_PLAIN || p.method > OMNI3D_METHOD_WEIGHTED_MINICURL) {
     *why = "unknown method";
     return false;
   }
@@ -664,25 +664,24 @@ void balance_groups(const std::vector<unsigned long long>& rowptr, int maxrows, 
   const unsigned long long nnz = rowptr[nS];
   long long G = std::max<long long>((nS + maxrows - 1) / maxrows, 1);
   G = (G + ncu - 1) / ncu * ncu;
+  // greedy split: each group takes rows while its entries stay within the
+  // budget (at least one row, at most maxrows), found by binary search on
+  // the offsets, so a split costs O(groups log nS)
   auto split = [&](unsigned long long budget, bool store) -> long long {
-    long long groups = 1;
-    unsigned long long acc = 0;
-    int rows = 0;
-    if (store) {
-      grow->clear();
-      grow->push_back(0);
-    }
-    for (int r = 0; r < nS; r++) {
-      const unsigned long long e = rowptr[r + 1] - rowptr[r];
-      if (rows > 0 && (rows == maxrows || acc + e > budget)) {
-        groups++;
-        if (store) grow->push_back(r);
-        acc = 0;
-        rows = 0;
+    long long groups = 0;
+    if (store) grow->clear();
+    int r = 0;
+    while (r < nS) {
+      const int cap = std::min(nS, r + maxrows);
+      const auto it = std::upper_bound(rowptr.begin() + r + 1, rowptr.begin() + cap + 1,
+                                       rowptr[r] + budget);
+      const int t = std::max(r + 1, (int)(it - rowptr.begin()) - 1);
+      if (store) {
+        grow->push_back(r);
+        for (int q = r; q < t; q++) (*rgroup)[q] = (int)groups;
       }
-      if (store) (*rgroup)[r] = (int)grow->size() - 1;
-      acc += e;
-      rows++;
+      groups++;
+      r = t;
     }
     if (store) grow->push_back(nS);
     return groups;
@@ -977,7 +976,8 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const char* menv = std::getenv("OMNI3D_MASKED");
   // SelectFixedBC steps by its own rule, not along the plan's staircase
   const bool tiled = !(menv && std::strcmp(menv, "naive") == 0) &&
-                     m != OMNI3D_METHOD_SELECT_FIXEDBC && m != OMNI3D_METHOD_MINICURL;
+                     m != OMNI3D_METHOD_SELECT_FIXEDBC && m != OMNI3D_METHOD_MINICURL &&
+                     m != OMNI3D_METHOD_WEIGHTED_FIXEDBC && m != OMNI3D_METHOD_WEIGHTED_MINICURL;
   if (tiled) {
     const size_t nv = (size_t)std::max<int64_t>(pl.nval, 1);
     const bool curlw = !fixed;

@@ -135,8 +135,9 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
 
 // Weighted walk (bodyIntegralInnerWeighted, kernel.cu:1993-2135): no entry
 // self-add, no path break; weight counttmp / curltmp with curltmp the running
-// sum of curl / 10^4.
-template <typename T>
+// sum of curl / 10^4.  kNoJ0: bodyIntegralInnerWeightedFixedBC
+// (kernel.cu:2136-2279), the same walk ended by a step onto the j = 0 plane.
+template <typename T, bool kNoJ0 = false>
 __device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst* __restrict__ dirs,
                                                const GridDesc& g, const WalkParams& wp,
                                                const typename Vec4<T>::type* __restrict__ field,
@@ -155,7 +156,7 @@ __device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst*
   V4 cur = field[w.v];
   while (!w.done()) {
     int s;
-    const int a = w.advance(&s);
+    const int a = kNoJ0 ? w.advance_noj0(&s) : w.advance(&s);
     if (a < 0) break;
     const V4 nxt = field[w.v];
     pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
@@ -220,6 +221,57 @@ __device__ __forceinline__ void naive_minicurl(const RayRec& rr, const DirConst*
   }
 }
 
+// WeightedMiniCurl walk (bodyIntegralInnerWeightedMiniCurl,
+// kernel.cu:2937-3085): the MiniCurl path; the entry adds its own p with
+// weight 1 / curl[entry] (curltmp = curl there, counttmp = 1, kernel.cu:2956-
+// 2960), every step adds (p_in + pint) * counttmp / curltmp with curltmp the
+// running sum of curl (no / 10^4) and no test of curltmp against zero.
+template <typename T>
+__device__ __forceinline__ void naive_weighted_minicurl(
+    const RayRec& rr, const DirConst* __restrict__ dirs, const GridDesc& g, const WalkParams& wp,
+    const typename Vec4<T>::type* __restrict__ field, const double* __restrict__ aux,
+    const double* __restrict__ p, double* __restrict__ sumd, double* __restrict__ wsum,
+    uint8_t* __restrict__ src) {
+  using V4 = typename Vec4<T>::type;
+  Walker w;
+  w.init(rr, dirs, g);
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  const double pin = p[w.v];
+  if (src) src[w.v] = 1;
+  double curltmp = aux[w.v], counttmp = 1.0;
+  add_f64(&sumd[w.v], pin * 1 / curltmp * counttmp);
+  if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+  V4 cur = field[w.v];
+  RaySum<T> pint;
+  while (!w.done()) {
+    const int s0 = w.c0 < w.e0 ? 1 : (w.c0 > w.e0 ? -1 : -60000);
+    const int s1 = w.c1 < w.e1 ? 1 : (w.c1 > w.e1 ? -1 : -60000);
+    const int s2 = w.c2 < w.e2 ? 1 : (w.c2 > w.e2 ? -1 : -60000);
+    const int64_t i0 = w.v + s0, i1 = w.v + (int64_t)s1 * w.sy, i2 = w.v + (int64_t)s2 * w.sz;
+    const double d0 = (i0 >= 0 && i0 < V) ? aux[i0] : 1e10;
+    const double d1 = (i1 >= 0 && i1 < V) ? aux[i1] : 1e10;
+    const double d2 = (i2 >= 0 && i2 < V) ? aux[i2] : 1e10;
+    int a, s, nc, lim;
+    int64_t vn;
+    if (d0 <= d1 && d0 <= d2) { a = 0; s = s0; nc = w.c0 + s0; lim = g.X; vn = i0; }
+    else if (d1 < d0 && d1 <= d2) { a = 1; s = s1; nc = w.c1 + s1; lim = g.Y; vn = i1; }
+    else if (d2 < d0 && d2 < d1) { a = 2; s = s2; nc = w.c2 + s2; lim = g.Z; vn = i2; }
+    else break;
+    if (nc < 0 || nc >= lim) break;
+    if (a == 0) w.c0 = nc;
+    else if (a == 1) w.c1 = nc;
+    else w.c2 = nc;
+    w.v = vn;
+    const V4 nxt = field[vn];
+    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+    curltmp += aux[vn];
+    counttmp += 1.0;
+    add_f64(&sumd[vn], (pin + pint.value()) * 1 / curltmp * counttmp);
+    if (wsum) add_f64(&wsum[vn], 1 / curltmp * counttmp);
+    cur = nxt;
+  }
+}
+
 template <typename T, int kMethod>
 __global__ void __launch_bounds__(256)
 k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
@@ -246,6 +298,10 @@ k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
                                  wsum, src);
   else if (kMethod == 5)
     naive_minicurl<T>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, src);
+  else if (kMethod == 6)
+    naive_weighted<T, true>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
+  else if (kMethod == 7)
+    naive_weighted_minicurl<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
   else
     naive_weighted<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
 }
@@ -501,6 +557,8 @@ hipError_t launch_walk_naive(int method, const RayRec* rays, int64_t nrays, cons
     case 2: OMNI_NAIVE(2); break;
     case 4: OMNI_NAIVE(4); break;
     case 5: OMNI_NAIVE(5); break;
+    case 6: OMNI_NAIVE(6); break;
+    case 7: OMNI_NAIVE(7); break;
     default: OMNI_NAIVE(3); break;
   }
 #undef OMNI_NAIVE

@@ -109,6 +109,21 @@ struct Walker {
     return a;
   }
 
+  // The plain staircase step (ties x, y, z) refusing a y-step onto the
+  // j = 0 plane (bodyIntegralInnerWeightedFixedBC, kernel.cu:2226: jnext2 > 0):
+  // such a step ends the walk, as in the reference.
+  __device__ __forceinline__ int advance_noj0(int* s_out) {
+    const int s1 = sgn_step(c1, e1);
+    if (c1 + s1 == 0) {
+      const int s0 = sgn_step(c0, e0), s2 = sgn_step(c2, e2);
+      const double f0 = step_metric(s0, u0, m00);
+      const double f1 = step_metric(s1, u1, m11);
+      const double f2 = step_metric(s2, u2, m22);
+      if (f1 < f0 && f1 <= f2) return -1;
+    }
+    return advance(s_out);
+  }
+
   __device__ __forceinline__ bool on_surface() const {
     return c0 == 0 || c0 == X - 1 || c1 == 0 || c1 == Y - 1 || c2 == 0 || c2 == Z - 1;
   }

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libomni3d.so")
 OK = 0
 ERR_ARG, ERR_STATE, ERR_HIP, ERR_IO, ERR_NONFINITE, ERR_NODEVICE = 1, 2, 3, 4, 5, 6
 PLAIN, PLAIN_MASK, SELECT, WEIGHTED, SELECT_FIXEDBC, MINICURL = 0, 1, 2, 3, 4, 5
+# the integrators no reference launcher calls
+WEIGHTED_FIXEDBC, WEIGHTED_MINICURL = 6, 7
 FP64, FP32_KAHAN = 0, 1
 FP32 = FP32_KAHAN
 STAIRCASE, TRILINEAR = 0, 1

@@ -313,3 +313,43 @@ def test_masked_tile_engine_matches_lane_walk(gpu_available, ref_dirs3, monkeypa
     err = F.rel_l2(pt, pn)
     print("tile engine vs lane walk rel-L2", err)
     assert err < (1e-12 if precision == omni3d.FP64 else 1e-5)
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+@pytest.mark.parametrize("method", [omni3d.WEIGHTED_FIXEDBC, omni3d.WEIGHTED_MINICURL])
+def test_weighted_variants(gpu_available, ref_dirs2, method, precision):
+    """WeightedFixedBC (kernel.cu:2136-2279) and WeightedMiniCurl
+    (kernel.cu:2937-3085) against the oracle from a random initial pressure:
+    weights <= 1e-12, FP64 <= 1e-10 (FP64 atomics), FP32 <= 1e-5.  The curl
+    is kept positive (WeightedMiniCurl divides by the running curl sum from
+    the entry on, which the reference does not test for zero)."""
+    shape = (16, 17, 18)
+    print(f"case weighted variant method={method} precision={precision}", flush=True)
+    f = F.taylor_green(18, noise=0.05, seed=11, shape=shape)
+    acc = [np.asarray(a, np.float32).astype(np.float64) if precision else a for a in f["acc"]]
+    f = dict(f, acc=acc)
+    p0 = np.random.default_rng(12).normal(size=shape)
+    with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision, method=method,
+                        solver=omni3d.NAIVE, max_iterations=5) as ctx:
+        ctx.set_directions(ref_dirs2)
+        ctx.set_field([np.asarray(a, dtype=np.float64) for a in acc])
+        curl = ctx.field_curl()
+        assert curl.min() > 0
+        p, w, rep = ctx.reconstruct(p0)
+    om = O.WEIGHTED_FIXEDBC if method == omni3d.WEIGHTED_FIXEDBC else O.WEIGHTED_MINICURL
+    po, co, it, rel = O.reconstruct(shape, ref_dirs2, 1.0, f["delta"], f["rho"], acc, p0=p0,
+                                    curl=curl, method=om, max_iter=5)
+    assert rep.iterations == it == 5
+    assert F.rel_l2(w, co) < 1e-12
+    err = F.rel_l2(p, po)
+    print("weighted variant rel-L2 vs oracle", err)
+    assert err < (1e-10 if precision == omni3d.FP64 else 1e-5)
+    if method == omni3d.WEIGHTED_FIXEDBC:
+        # rays stepping onto j = 0 end there: the plane gets no visits but
+        # the entries' (Weighted visits it)
+        with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision,
+                            method=omni3d.WEIGHTED, solver=omni3d.NAIVE, max_iterations=1) as c2:
+            c2.set_directions(ref_dirs2)
+            c2.set_field([np.asarray(a, dtype=np.float64) for a in acc])
+            _, w2, _ = c2.reconstruct(p0)
+        assert w[:, 0, :].sum() < w2[:, 0, :].sum()

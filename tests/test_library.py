@@ -90,7 +90,7 @@ def test_bad_arguments_rejected_before_any_device_work():
         omni3d.Context((8, 8, 8), line_spacing=0.0)
     assert e.value.code == omni3d.ERR_ARG
     with pytest.raises(omni3d.Omni3DError) as e:
-        omni3d.Context((8, 8, 8), method=omni3d.MINICURL + 1)
+        omni3d.Context((8, 8, 8), method=omni3d.WEIGHTED_MINICURL + 1)
     assert e.value.code == omni3d.ERR_ARG
     L = omni3d.lib()
     out = C.c_float(0)

@@ -241,7 +241,8 @@ MUTANTS = [
     ("weighted: curl not scaled by 1e4",
      "curltmp += P->curl[vn] / 10000;", "curltmp += P->curl[vn];"),
     ("weighted: weight without counttmp",
-     "P->pn[vn] += (pin + pint) * 1 / curltmp * counttmp;", "P->pn[vn] += (pin + pint) / curltmp;"),
+     "      P->pn[vn] += (pin + pint) * 1 / curltmp * counttmp;\n      P->cnt[vn]",
+     "      P->pn[vn] += (pin + pint) / curltmp;\n      P->cnt[vn]"),
     ("weighted: count without counttmp",
      "counttmp;\n      P->cnt[vn] += 1 / curltmp * counttmp;",
      "counttmp;\n      P->cnt[vn] += 1 / curltmp;"),
