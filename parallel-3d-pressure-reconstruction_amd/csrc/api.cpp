@@ -1013,10 +1013,15 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
   const unsigned int* cntw =
       m == OMNI3D_METHOD_PLAIN ? pl.cnt.as<unsigned int>() : ctx->wcnt.as<unsigned int>();
   record_event(ctx, 2);
+  void* rule = nullptr;
+  if (tiled && !mask_walk_lane()) {
+    OMNI_TRY_HIP(ctx, ctx->mk_rule.alloc(mask_rule_bytes(m, pl.nval)));
+    rule = ctx->mk_rule.as<void>();
+  }
   if (tiled)
     OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
-                                          pl.g, pl.tg, wp, field, aux,
-                                          pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                          pl.g, pl.tg, wp, field, aux, pl.nseg.as<unsigned int>(),
+                                          pl.val_off.as<unsigned long long>(), pl.nval, pl.ntiles,
                                           pl.tile_ptr.as<unsigned long long>(),
                                           pl.segs.as<SegRec>(), sc,
                                           ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
@@ -1025,7 +1030,7 @@ int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
                                           m == OMNI3D_METHOD_PLAIN_MASK
                                               ? ctx->wcnt.as<unsigned int>() : nullptr,
                                           fixed ? nullptr : ctx->wsum.as<double>(),
-                                          ctx->srcf.as<uint8_t>(), st));
+                                          ctx->srcf.as<uint8_t>(), rule, st));
   record_event(ctx, 8);
   record_event(ctx, 3);
   for (int k = 1; k <= P.max_iterations; k++) {
@@ -1168,15 +1173,20 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
   record_event(ctx, 2);
   // the rule replay: first-visit records, weights, run starts
+  void* rule = nullptr;
+  if (!mask_walk_lane()) {
+    OMNI_TRY_HIP(ctx, ctx->mk_rule.alloc(mask_rule_bytes(m, pl.nval)));
+    rule = ctx->mk_rule.as<void>();
+  }
   OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
-                                        pl.g, pl.tg, wp, field, aux,
-                                        pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                        pl.g, pl.tg, wp, field, aux, pl.nseg.as<unsigned int>(),
+                                        pl.val_off.as<unsigned long long>(), pl.nval, pl.ntiles,
                                         pl.tile_ptr.as<unsigned long long>(),
                                         pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
                                         ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
                                         ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
                                         ctx->mk_wn.as<double>(), wcnt, wsum,
-                                        ctx->srcf.as<uint8_t>(), st));
+                                        ctx->srcf.as<uint8_t>(), rule, st));
   record_event(ctx, 8);
   // S: compact ids of the run starts (the voxels a pass reads p at), in tile
   // order: tile t's rows are [tfirst[t], tfirst[t + 1])

@@ -222,17 +222,24 @@ int es_pass(int method, long X, long Y, long Z, int D, float ls, const float* k1
       OMNI_TRY_HIP(ctx, ctx->mk_f.alloc(sizeof(uint32_t) * nv));
       OMNI_TRY_HIP(ctx, ctx->mk_wc.alloc(sizeof(double) * (fixed ? 1 : nv)));
       OMNI_TRY_HIP(ctx, ctx->mk_wn.alloc(sizeof(double) * (fixed ? 1 : nv)));
+      void* rule = nullptr;
+      if (!mask_walk_lane()) {
+        OMNI_TRY_HIP(ctx, ctx->mk_rule.alloc(mask_rule_bytes(method, pl.nval)));
+        rule = ctx->mk_rule.as<void>();
+      }
       OMNI_TRY_HIP(ctx, launch_mask_walk<double>(method, pl.rays.as<RayRec>(), pl.nrays,
                                                  pl.dirs.as<DirConst>(), pl.g, pl.tg, wp,
                                                  ctx->field.as<void>(), aux,
-                                                 pl.val_off.as<unsigned long long>(), pl.ntiles,
+                                                 pl.nseg.as<unsigned int>(),
+                                                 pl.val_off.as<unsigned long long>(), pl.nval,
+                                                 pl.ntiles,
                                                  pl.tile_ptr.as<unsigned long long>(),
                                                  pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
                                                  ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
                                                  ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
                                                  ctx->mk_wn.as<double>(), nullptr,
                                                  fixed ? nullptr : ctx->wsum.as<double>(),
-                                                 ctx->srcf.as<uint8_t>(), st));
+                                                 ctx->srcf.as<uint8_t>(), rule, st));
     }
     const double* pa = ctx->pa.as<double>();
     OMNI_TRY_HIP(ctx, ctx->mk_c.alloc(sizeof(double) * nv));

@@ -192,6 +192,7 @@ struct omni3d_ctx {
   // masked methods on the tile engine: per value slot the first-visit record
   // (E, S, R, F, curl weights) of the field and the carries of an iteration
   omni3d::DevBuf mk_e, mk_s, mk_r, mk_f, mk_wc, mk_wn, mk_c;
+  omni3d::DevBuf mk_rule;  // the tile rule replay's per-entry-state records
   // the masked methods' affine schedule: per value slot the start voxel of
   // the run a segment continues; source ids and list of the run starts S;
   // the rows (pointers, source ids, weights); per row b, W and p_S (two);

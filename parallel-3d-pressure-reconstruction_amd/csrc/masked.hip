@@ -30,8 +30,11 @@
 // pass walks the visits after it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 
 #include "kernels.hpp"
 #include "launch.hpp"
@@ -209,6 +212,230 @@ k_mask_walk(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   store();
 }
 
+// ---------------------------------------------------------------------------
+// The rule replay on the tile engine (the records of k_mask_walk without a
+// lane per ray).  The rule's state between two visits is (p0set, outthr) --
+// N (no run), L (in a run), H (in a high / masked stretch) -- plus, for the
+// curl weights, the running (curltmp, counttmp).  Per segment, a tile pass
+// (k_rule_tile: the curl / mask in LDS) replays the segment from each entry
+// state and stores what it does: the first visit's flags, the state it
+// leaves in, its last restart, and how it changes the weights (the first
+// visit adds cn / 1e4, resets them or keeps them; later visits add to sums
+// kept since the first visit or since the segment's last reset).  A scan per
+// ray (k_rule_scan) then follows the actual states along the ray's segments
+// and writes the records k_mask_walk writes.  The source flags are set by the
+// setup tile pass, which replays every restart.  The weights come out as
+// (weights at the segment start) + (the segment's sums): the same terms as
+// the lane walk's running sum, added in another order.
+
+// per entry state: F flags (bits 0-5), exit state (6-7), the first visit's
+// weight change (8-9: 0 keep, 1 add, 2 reset), a reset after the first
+// visit (10), the visits added after the first or the last reset (11-16)
+constexpr int kRuleStates = 3;
+struct RuleSegs {
+  double* E;      // the trapezoid term of the step into the tile, in place in MaskSegs::E
+  double* x;      // curl / 1e4 at the first visit, in place in MaskSegs::wc (curl weights)
+  uint32_t* f;    // [planes][nval] per entry state, bits as above
+  uint32_t* r;    // [planes][nval] voxel of the last restart per entry state
+  double* cs;     // [planes][nval] curl sum since the first visit or the last reset
+  uint64_t nval;  // value slots per entry-state plane
+};
+
+__device__ __forceinline__ uint32_t rule_state(bool p0set, bool outthr) {
+  return p0set ? (outthr ? 2u : 1u) : 0u;
+}
+
+// Select and plain-mask enter a segment in one of three states; the plain
+// and Weighted walks are always in a run after the entry voxel.
+template <int kMethod>
+constexpr int kRulePlanes = (kMethod == kMaskUnit || kMethod == kMaskSelect) ? kRuleStates : 1;
+
+bool mask_walk_lane() {
+  const char* e = std::getenv("OMNI3D_MASK_WALK");
+  return e && std::strcmp(e, "lane") == 0;
+}
+
+size_t mask_rule_bytes(int method, int64_t nval) {
+  const int planes = (method == kMaskUnit || method == kMaskSelect) ? kRuleStates : 1;
+  const bool curl = method == kMaskSelect || method == kMaskWeighted;
+  return (size_t)planes * (size_t)std::max<int64_t>(nval, 1) * (curl ? 16 : 8);
+}
+
+template <typename T, int kMethod, int kBits>
+__global__ void __launch_bounds__(1024)
+k_rule_tile(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
+            GridDesc g, TileGrid tg, WalkParams wp,
+            const typename Vec4<T>::type* __restrict__ field, const double* __restrict__ aux,
+            RuleSegs rs) {
+  constexpr int kTile = 1 << kBits;
+  constexpr int kTileMask = kTile - 1;
+  constexpr int kVox = kTile * kTile * kTile;
+  constexpr bool kAux = kMethod != kMaskPlain;
+  __shared__ double s_aux[kAux ? kVox : 1];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  if constexpr (kAux) {
+    for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+      const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), kk = oz + (l >> (2 * kBits));
+      const bool in = i < g.X && j < g.Y && kk < g.Z;
+      s_aux[lds_slot<kBits>(l)] = in ? aux[i + j * sy + kk * sz] : 0.0;
+    }
+    __syncthreads();
+  }
+  auto gvox = [&](int l) -> int64_t {
+    return (int64_t)(ox + (l & kTileMask)) + (int64_t)(oy + ((l >> kBits) & kTileMask)) * sy +
+           (int64_t)(oz + (l >> (2 * kBits))) * sz;
+  };
+  auto cval = [&](int l) -> double {
+    if constexpr (kAux) return s_aux[lds_slot<kBits>(l)];
+    else return 0.0;
+  };
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    const int lv0 = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const bool first_seg = (sr.info >> 21) & 1u;
+    const int ae = (int)((sr.info >> 23) & 3u);  // axis of the step into the tile
+    const int d0 = (sg & 1u) ? 1 : -1;
+    const int d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    const double c_first = cval(lv0);
+    // the step into the tile: its term (whether it joins the run depends on
+    // the entry state; the scan decides)
+    double E = 0.0;
+    if (!first_seg) {
+      const int se = ((sg >> ae) & 1u) ? 1 : -1;
+      const int64_t vf = gvox(lv0);
+      const int64_t vp = vf - (ae == 0 ? (int64_t)se : (ae == 1 ? se * sy : se * sz));
+      E = (double)trapezoid<T>(wp, ae, se, component(field[vf], ae), component(field[vp], ae));
+    }
+    rs.E[sr.g] = E;
+    if constexpr (kCurlW<kMethod>) rs.x[sr.g] = c_first / 10000;
+    const int nplanes = first_seg ? 1 : kRulePlanes<kMethod>;
+    for (int st = 0; st < nplanes; st++) {
+      // entry state: N, L, H; the single state of the plain / Weighted walks is L
+      const int es = kRulePlanes<kMethod> == 1 ? 1 : st;
+      bool p0set = es >= 1, outthr = es == 2;
+      uint32_t F, wop = 0u, rst = 0u;
+      bool hasR = false;
+      uint32_t R = 0u;
+      if (first_seg) {
+        // the ray's entry voxel (kernel.cu:2302 Select; every entry for the
+        // plain and Weighted walks)
+        p0set = outthr = false;
+        if (rule_entry<kMethod>(wp, c_first)) {
+          p0set = true;
+          hasR = true;
+          R = (uint32_t)gvox(lv0);
+          F = kModeRestart | (kMethod == kMaskPlain ? kFlagAdd : 0u);
+        } else {
+          F = kModeNone;
+        }
+      } else {
+        const StepRule d = rule_step<kMethod>(wp, ae, c_first, p0set, outthr);
+        if (d.restart) {
+          hasR = true;
+          R = (uint32_t)gvox(lv0);
+          if (!d.keep_w) wop = 2u;
+        }
+        if (d.add) wop = 1u;  // a step adds or restarts, never both
+        F = (d.restart ? kModeRestart : (p0set ? kModeCont : kModeNone)) | (d.add ? kFlagAdd : 0u);
+      }
+      F |= (p0set ? kFlagP0 : 0u) | (outthr ? kFlagOut : 0u);
+      // the visits after the first
+      double csum = 0.0;
+      uint32_t ccnt = 0u;
+      int lv = lv0;
+      uint32_t cw = sr.code[0];
+      int rem = (int)n - 1;
+#pragma unroll 1
+      for (int wi = 0; wi < 3 && rem > 0; wi++) {
+        const int m = rem < 16 ? rem : 16;
+        for (int q = 0; q < m; q++) {
+          const int a = (int)(cw & 3u);
+          cw >>= 2;
+          const int ln = lv + (a == 0 ? d0 : (a == 1 ? d1 : d2));
+          const double cn = cval(ln);
+          const StepRule d = rule_step<kMethod>(wp, a, cn, p0set, outthr);
+          if (d.add && kCurlW<kMethod>) {
+            csum += cn / 10000;
+            ccnt += 1u;
+          }
+          if (d.restart) {
+            hasR = true;
+            R = (uint32_t)gvox(ln);
+            if (!d.keep_w) {
+              csum = 0.0;
+              ccnt = 0u;
+              rst = 1u;
+            }
+          }
+          lv = ln;
+        }
+        rem -= m;
+        cw = wi == 0 ? sr.code[1] : sr.code[2];
+      }
+      const uint64_t o = (uint64_t)st * rs.nval + sr.g;
+      rs.f[o] = F | (hasR ? kFlagRestart : 0u) | (rule_state(p0set, outthr) << 6) |
+                (wop << 8) | (rst << 10) | (ccnt << 11);
+      rs.r[o] = R;
+      if constexpr (kCurlW<kMethod>) rs.cs[o] = csum;
+    }
+  }
+}
+
+// Per ray: the segments' records from the actual entry states (one lane per
+// ray, its slots 64 apart: every load and store is coalesced).
+template <int kMethod>
+__global__ void __launch_bounds__(256)
+k_rule_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
+            const unsigned long long* __restrict__ val_off, RuleSegs rs, MaskSegs ms) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  unsigned long long gs = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int ns = nseg[r];
+  int state = 0;
+  double curl = 0.0, cnt = 0.0;
+  for (unsigned int j = 0; j < ns; j++, gs += 64) {
+    const int st = (j == 0 || kRulePlanes<kMethod> == 1) ? 0 : state;
+    const uint64_t o = (uint64_t)st * rs.nval + gs;
+    const uint32_t f = rs.f[o];
+    const uint32_t wop = (f >> 8) & 3u;
+    uint32_t F = f & 63u;
+    if constexpr (kCurlW<kMethod>) {
+      double c0 = curl, n0 = cnt;
+      if (j > 0 && kMethod == kMaskWeighted) {
+        // every step adds; it counts as added once the curl sum is nonzero
+        c0 = curl + rs.x[gs];
+        n0 = cnt + 1.0;
+        if (c0 == 0.0) F &= ~kFlagAdd;
+      } else if (wop == 1u) {
+        c0 = curl + rs.x[gs];
+        n0 = cnt + 1.0;
+      } else if (wop == 2u) {
+        c0 = 0.0;
+        n0 = 0.0;
+      }
+      const bool reset = (f >> 10) & 1u;
+      curl = (reset ? 0.0 : c0) + rs.cs[o];
+      cnt = (reset ? 0.0 : n0) + (double)((f >> 11) & 63u);
+      ms.wc[gs] = c0;
+      ms.wn[gs] = n0;
+    }
+    // the step into the tile joins the run where it adds
+    const bool step_adds =
+        j > 0 && (kMethod == kMaskWeighted || kMethod == kMaskPlain || wop == 1u);
+    if (!step_adds) ms.E[gs] = 0.0;
+    ms.F[gs] = F;
+    ms.R[gs] = rs.r[o];
+    state = (int)((f >> 6) & 3u);
+  }
+}
+
 // Carry chain of one iteration: the value each segment's first visit
 // continues from (mode "continue": the run's value at the previous
 // segment's exit plus the step into the tile).  A segment that restarts
@@ -259,7 +486,8 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
               const double* __restrict__ p0, const double* __restrict__ p1, int k,
               const DevScalars* __restrict__ sc, long long* __restrict__ sumfx,
               double* __restrict__ sumd, double* __restrict__ segS,
-              unsigned int* __restrict__ wcnt, double* __restrict__ wsum) {
+              unsigned int* __restrict__ wcnt, double* __restrict__ wsum,
+              uint8_t* __restrict__ src) {
   using V4 = typename Vec4<T>::type;
   constexpr int kTile = 1 << kBits;
   constexpr int kTileMask = kTile - 1;
@@ -340,6 +568,9 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
       if (mode == kModeCont) c = cval[sr.g];
       else if (mode == kModeRestart) c = kAuxLds ? L.pl[lds_slot<kBits>(lv)] : p[gvox(lv)];
     }
+    // the run starts (source flags): every voxel a run (re)starts at
+    if constexpr (kSetup)
+      if (mode == kModeRestart) src[gvox(lv)] = 1;
     double curl = 0.0, cnt = 0.0;
     if (kCurlW<kMethod>) {
       curl = wc[sr.g];
@@ -376,8 +607,10 @@ k_tile_masked(const unsigned long long* __restrict__ tile_ptr, const SegRec* __r
           if (added) add(sln, c, curl, cnt);
         }
         if (d.restart) {
-          if constexpr (kSetup) S = 0.0;
-          else if constexpr (kAuxLds) c = L.pl[sln];
+          if constexpr (kSetup) {
+            S = 0.0;
+            src[gvox(ln)] = 1;
+          } else if constexpr (kAuxLds) c = L.pl[sln];
           else c = p[gvox(ln)];
           if (!d.keep_w) {
             curl = 0.0;
@@ -417,14 +650,14 @@ hipError_t launch_tile_pass_masked(int method, int ntiles, const unsigned long l
                                    const double* cval, const double* p0, const double* p1, int k,
                                    const DevScalars* sc, long long* sumfx, double* sumd,
                                    double* segS, unsigned int* wcnt, double* wsum,
-                                   hipStream_t st) {
+                                   uint8_t* src, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   constexpr int kThreads = 1024;
 #define OMNI_TM(M, B)                                                                           \
   hipLaunchKernelGGL((k_tile_masked<T, M, B, kThreads, kSetup>), dim3(ntiles), dim3(kThreads), 0, \
                      st, tile_ptr, segs, g, tg, wp, f, aux, F, wc, wn, cval, p0, p1, k, sc, sumfx, \
-                     sumd, segS, wcnt, wsum)
+                     sumd, segS, wcnt, wsum, src)
   if (tg.bits == 3) {
     switch (method) {
       case kMaskPlain: OMNI_TM(kMaskPlain, 3); break;
@@ -447,38 +680,87 @@ hipError_t launch_tile_pass_masked(int method, int ntiles, const unsigned long l
 template <typename T>
 hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
-                            const double* aux, const unsigned long long* val_off, int ntiles,
+                            const double* aux, const unsigned int* nseg,
+                            const unsigned long long* val_off, int64_t nval, int ntiles,
                             const unsigned long long* tile_ptr, const SegRec* segs,
                             const DevScalars* sc, double* E, double* S, uint32_t* R, uint32_t* F,
                             double* wc, double* wn, unsigned int* wcnt, double* wsum,
-                            uint8_t* src, hipStream_t st) {
+                            uint8_t* src, void* rule, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   const MaskSegs ms{E, R, F, wc, wn};
-  const unsigned int grid = grid_for(nrays, 1u << 30);
+  if (rule) {
+    // the rule replay on the tile engine: per segment and entry state, then
+    // per ray
+    const int planes = (method == kMaskUnit || method == kMaskSelect) ? kRuleStates : 1;
+    const bool curl = method == kMaskSelect || method == kMaskWeighted;
+    const uint64_t nv = (uint64_t)std::max<int64_t>(nval, 1);
+    char* base = static_cast<char*>(rule);
+    RuleSegs rs;
+    rs.E = E;
+    rs.x = wc;
+    rs.cs = curl ? reinterpret_cast<double*>(base) : nullptr;
+    if (curl) base += sizeof(double) * planes * nv;
+    rs.f = reinterpret_cast<uint32_t*>(base);
+    rs.r = rs.f + planes * nv;
+    rs.nval = nv;
+#define OMNI_RT(M, B)                                                                         \
+  hipLaunchKernelGGL((k_rule_tile<T, M, B>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, \
+                     tg, wp, f, aux, rs)
+#define OMNI_RT_M(B)                                  \
+  switch (method) {                                   \
+    case kMaskPlain: OMNI_RT(kMaskPlain, B); break;   \
+    case kMaskUnit: OMNI_RT(kMaskUnit, B); break;     \
+    case kMaskSelect: OMNI_RT(kMaskSelect, B); break; \
+    default: OMNI_RT(kMaskWeighted, B); break;        \
+  }
+    if (tg.bits == 3) {
+      OMNI_RT_M(3)
+    } else {
+      OMNI_RT_M(4)
+    }
+#undef OMNI_RT_M
+#undef OMNI_RT
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const unsigned int sgrid = grid_for(nrays, 1u << 30);
+#define OMNI_RS(M) \
+  hipLaunchKernelGGL((k_rule_scan<M>), dim3(sgrid), dim3(256), 0, st, nrays, nseg, val_off, rs, ms)
+    switch (method) {
+      case kMaskPlain: OMNI_RS(kMaskPlain); break;
+      case kMaskUnit: OMNI_RS(kMaskUnit); break;
+      case kMaskSelect: OMNI_RS(kMaskSelect); break;
+      default: OMNI_RS(kMaskWeighted); break;
+    }
+#undef OMNI_RS
+  } else {
+    // one lane per ray (the records without the tile engine)
+    const unsigned int grid = grid_for(nrays, 1u << 30);
 #define OMNI_MW(M)                                                                              \
   hipLaunchKernelGGL((k_mask_walk<T, M>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g, tg, \
                      wp, f, aux, val_off, ms, src)
-  switch (method) {
-    case kMaskPlain: OMNI_MW(kMaskPlain); break;
-    case kMaskUnit: OMNI_MW(kMaskUnit); break;
-    case kMaskSelect: OMNI_MW(kMaskSelect); break;
-    default: OMNI_MW(kMaskWeighted); break;
-  }
+    switch (method) {
+      case kMaskPlain: OMNI_MW(kMaskPlain); break;
+      case kMaskUnit: OMNI_MW(kMaskUnit); break;
+      case kMaskSelect: OMNI_MW(kMaskSelect); break;
+      default: OMNI_MW(kMaskWeighted); break;
+    }
 #undef OMNI_MW
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // setup tile pass: in-tile sums after the last restart, and the weights
+  // setup tile pass: in-tile sums after the last restart, the weights and
+  // the run starts
   return launch_tile_pass_masked<T, true>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux, F,
                                           wc, wn, nullptr, nullptr, nullptr, 1, sc, nullptr,
-                                          nullptr, S, wcnt, wsum, st);
+                                          nullptr, S, wcnt, wsum, src, st);
 }
 #define OMNI_MW_INST(T)                                                                        \
   template hipError_t launch_mask_walk<T>(                                                      \
       int, const RayRec*, int64_t, const DirConst*, GridDesc, TileGrid, const WalkParams&,       \
-      const void*, const double*, const unsigned long long*, int, const unsigned long long*,    \
-      const SegRec*, const DevScalars*, double*, double*, uint32_t*, uint32_t*, double*, double*, \
-      unsigned int*, double*, uint8_t*, hipStream_t);
+      const void*, const double*, const unsigned int*, const unsigned long long*, int64_t, int,  \
+      const unsigned long long*, const SegRec*, const DevScalars*, double*, double*, uint32_t*,  \
+      uint32_t*, double*, double*, unsigned int*, double*, uint8_t*, void*, hipStream_t);
 OMNI_MW_INST(float)
 OMNI_MW_INST(double)
 #undef OMNI_MW_INST
@@ -502,7 +784,7 @@ hipError_t launch_tile_masked(int method, int ntiles, const unsigned long long* 
                               long long* sumfx, double* sumd, hipStream_t st) {
   return launch_tile_pass_masked<T, false>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux,
                                            F, wc, wn, cval, p0, p1, k, sc, sumfx, sumd, nullptr,
-                                           nullptr, nullptr, st);
+                                           nullptr, nullptr, nullptr, st);
 }
 template hipError_t launch_tile_masked<float>(int, int, const unsigned long long*, const SegRec*,
                                               GridDesc, TileGrid, const WalkParams&, const void*,

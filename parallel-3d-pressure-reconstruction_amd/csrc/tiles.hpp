@@ -121,16 +121,23 @@ size_t scan_u32_tmp_bytes(int64_t n);
 
 // masked.hip: the masked methods' traversal per iteration (method 1
 // plain-mask, 2 Select, 3 Weighted as in the naive walk)
-// the per-field rule replay: k_mask_walk, then the setup tile pass (segment
-// sums S and the weights wcnt (plain-mask) or wsum (curl methods))
+// the per-field rule replay -- on the tile engine (k_rule_tile per segment
+// and entry state, k_rule_scan per ray; `rule` = mask_rule_bytes of scratch)
+// or, with rule == nullptr, one lane per ray (k_mask_walk) -- then the setup
+// tile pass (segment sums S, the weights wcnt (plain-mask) or wsum (curl
+// methods) and the run starts src)
 template <typename T>
 hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
-                            const double* aux, const unsigned long long* val_off, int ntiles,
+                            const double* aux, const unsigned int* nseg,
+                            const unsigned long long* val_off, int64_t nval, int ntiles,
                             const unsigned long long* tile_ptr, const SegRec* segs,
                             const DevScalars* sc, double* E, double* S, uint32_t* R, uint32_t* F,
                             double* wc, double* wn, unsigned int* wcnt, double* wsum,
-                            uint8_t* src, hipStream_t st);
+                            uint8_t* src, void* rule, hipStream_t st);
+size_t mask_rule_bytes(int method, int64_t nval);
+// OMNI3D_MASK_WALK=lane selects the one-lane-per-ray rule replay
+bool mask_walk_lane();
 hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, const double* E, const double* S,
                              const uint32_t* R, const uint32_t* F, const double* p0,
