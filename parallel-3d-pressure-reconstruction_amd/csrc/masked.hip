@@ -1057,7 +1057,8 @@ k_tile_entries(const unsigned long long* __restrict__ tile_ptr, const SegRec* __
 // 2^e > max w * max |p_S^{k-1}|) into LDS sums of the tile's rows, then adds
 // each nonzero sum into the row's global fixed-point sum (a row's total over
 // all its entries stays below 2^63: fewer than 4096 entries per row).
-constexpr int kChunkThreads = 256;
+constexpr int kChunkThreads = 1024;
+constexpr int kChunkUnroll = 4;
 constexpr int kTileRowsMax = 4096;
 
 __device__ __forceinline__ bool masked_scale(const DevScalars* sc, int k,
@@ -1092,10 +1093,35 @@ k_mtile_accum(int k, const MaskChunk* __restrict__ chunks, const int* __restrict
   const double* __restrict__ pold = ((k - 1) & 1) ? p1 : p0;
   double scale, iscale, B;
   (void)masked_scale(sc, k, wmax_bits, &scale, &iscale, &B);
-  for (unsigned long long e = c.begin + threadIdx.x; e < c.end; e += kChunkThreads) {
-    const double x = pold[esrc[e]];
-    const double w = kUnit ? 1.0 : ew[e];
-    atomicAdd(&s_acc[erow[e]], (unsigned long long)to_fixed(w * x, scale));
+  // kChunkUnroll entries per thread and round: their loads, then their
+  // gathers, then their adds (the integer sums do not depend on the order)
+  constexpr unsigned long long kSpan = (unsigned long long)kChunkThreads * kChunkUnroll;
+  for (unsigned long long e0 = c.begin + threadIdx.x; e0 < c.end; e0 += kSpan) {
+    unsigned int src[kChunkUnroll];
+    unsigned short row[kChunkUnroll];
+    double w[kChunkUnroll], x[kChunkUnroll];
+#pragma unroll
+    for (int q = 0; q < kChunkUnroll; q++) {
+      const unsigned long long e = e0 + (unsigned long long)q * kChunkThreads;
+      src[q] = 0u;
+      row[q] = 0;
+      w[q] = 0.0;
+      if (e < c.end) {
+        src[q] = esrc[e];
+        row[q] = erow[e];
+        w[q] = kUnit ? 1.0 : ew[e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kChunkUnroll; q++) {
+      const unsigned long long e = e0 + (unsigned long long)q * kChunkThreads;
+      x[q] = e < c.end ? pold[src[q]] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kChunkUnroll; q++) {
+      const unsigned long long e = e0 + (unsigned long long)q * kChunkThreads;
+      if (e < c.end) atomicAdd(&s_acc[row[q]], (unsigned long long)to_fixed(w[q] * x[q], scale));
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nr; i += kChunkThreads) {
