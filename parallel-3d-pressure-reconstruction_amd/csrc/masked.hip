@@ -9,15 +9,20 @@
 // start and stop, which visits add, and their curl weights are fixed for a
 // field.  Only the value a run carries depends on p, through the pressure at
 // the voxel the run (re)starts from.  So
-//   once per field:  k_mask_walk   one lane per ray replays the rule on the
-//                    curl / mask alone and stores, per segment (ray x tile,
-//                    in its value slot), how its first visit is taken (with
-//                    the term of the step into the tile, the only gradient
-//                    it reads) and the voxel of its last restart, and sets
-//                    the source flags; then k_tile_masked<setup> replays the
-//                    rule per tile from those records with the gradient in
-//                    LDS: each segment's sum of added terms after its last
-//                    restart, and the weights (visit counts or curl weights);
+//   once per field:  k_rule_tile   one workgroup per tile replays the rule on
+//                    the curl / mask (in LDS) for each segment from each
+//                    state a ray can enter it in, and k_rule_scan (one lane
+//                    per ray) follows the states the rays actually enter
+//                    with: per segment (ray x tile, in its value slot), how
+//                    its first visit is taken (with the term of the step
+//                    into the tile, the only gradient it reads), the voxel
+//                    of its last restart and the curl weights at its first
+//                    visit (k_mask_walk, one lane per ray, writes the same
+//                    records: OMNI3D_MASK_WALK=lane); then k_tile_masked<setup>
+//                    replays the rule per tile from those records with the
+//                    gradient in LDS: each segment's sum of added terms after
+//                    its last restart, the weights (visit counts or curl
+//                    weights) and the run starts (source flags);
 //   per iteration:   k_mask_carry  one lane per ray chains its segments: the
 //                    value a run enters each segment with;
 //                    k_tile_masked one workgroup per tile stages the tile's
@@ -25,9 +30,9 @@
 //                    the tile's segments and adds every visit's value in LDS
 //                    (64-bit fixed point for unit weights, FP64 otherwise),
 //                    then stores the tile's sums -- no global atomics.
-// The first visit of a segment (the step into the tile) is taken by
-// k_mask_walk, which reads the voxel outside the tile from memory; the tile
-// pass walks the visits after it.
+// The first visit of a segment (the step into the tile) is decided by the
+// replay, which reads the voxel outside the tile from memory; the tile pass
+// walks the visits after it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
