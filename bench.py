@@ -229,10 +229,15 @@ def methods_mode(args):
     # whole reconstructions of the masked methods: the affine schedule (the
     # iteration on the run-start voxels, then one full pass) against the
     # reference schedule (a full traversal per iteration), same field
+    # (the reference's variants whose walks leave the staircase too: their
+    # affine schedule walks one lane per ray)
     recon = {}
     for name, method in (("plain-mask", omni3d.PLAIN_MASK), ("select", omni3d.SELECT),
-                         ("weighted", omni3d.WEIGHTED)):
-        kw = {"threshold": thr} if method == omni3d.SELECT else {}
+                         ("weighted", omni3d.WEIGHTED), ("select-fixedbc", omni3d.SELECT_FIXEDBC),
+                         ("minicurl", omni3d.MINICURL),
+                         ("weighted-fixedbc", omni3d.WEIGHTED_FIXEDBC),
+                         ("weighted-minicurl", omni3d.WEIGHTED_MINICURL)):
+        kw = {"threshold": thr} if method in (omni3d.SELECT, omni3d.SELECT_FIXEDBC) else {}
         row = {}
         for iters in (10, 200):
             for solver, sname in ((omni3d.AFFINE, "affine"), (omni3d.NAIVE, "reference")):

@@ -1456,6 +1456,234 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   return OMNI3D_OK;
 }
 
+// The walks that leave the plan's staircase (SelectFixedBC, MiniCurl,
+// WeightedFixedBC, WeightedMiniCurl) on the same affine schedule, one lane
+// per ray: their runs, run starts and weights also depend on the curl field
+// alone.  One pass from p = 0 gives b (the sums), W (the weights) and the
+// run starts S; two walks with an entry sink count, then write, the visits
+// landing on S as rows in tile layout (S numbered in tile order); the
+// iteration on S is the curl-weighted masked methods' (k_mtile_accum +
+// k_mtile_finish, unit weights for MiniCurl); one full pass from
+// p_S^{k*-1} gives every voxel.  Sets *fallback (nothing of the result
+// queued) where the rows do not fit the iteration (a row of 4096 entries or
+// more) or there is no run start.
+template <typename T>
+int run_affine_lane(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype,
+                    bool* fallback) {
+  *fallback = false;
+  const omni3d_params& P = ctx->prm;
+  Plan& pl = *ctx->plan;
+  if (!pl.valid || pl.ntiles <= 0 || pl.nrays <= 0 || P.max_iterations <= 0) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  hipStream_t st = ctx->stream;
+  const int64_t V = voxels(P);
+  const int m = P.method;
+  DevScalars* sc = ctx->scal.as<DevScalars>();
+  const void* field = ctx->field.as<void>();
+  const double* pinit = seed >= 0 ? ctx->pinit.as<double>() : nullptr;
+  WalkParams wp = walk_params(P);
+  OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<T>(ctx->field.as<T>(), 4 * V, &sc->amax_bits, st));
+  if (pinit) OMNI_TRY_HIP(ctx, launch_absmax<double>(pinit, V, &sc->pmax_bits, st));
+  const double gfac = P.density * std::max(P.dx, std::max(P.dy, P.dz));
+  OMNI_TRY_HIP(ctx, launch_compute_scale(sc, gfac, (double)pl.lmax, (double)pl.cmax,
+                                         (double)P.max_iterations, st));
+  OMNI_TRY_HIP(ctx, ctx->aux.alloc(sizeof(double) * V));
+  const double* mask = ctx->have_mask ? ctx->mask.as<double>() : nullptr;
+  OMNI_TRY_HIP(ctx, launch_curl_packed<T>(P.nx, P.ny, P.nz, P.dx, P.dy, P.dz, field, mask,
+                                          ctx->aux.as<double>(), st));
+  const double* aux = ctx->aux.as<double>();
+  const bool fixed = m == OMNI3D_METHOD_MINICURL;  // unit weights
+  OMNI_TRY_HIP(ctx, ctx->pa.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->pb.alloc(sizeof(double) * V));
+  OMNI_TRY_HIP(ctx, ctx->srcf.alloc((size_t)((V + 3) / 4) * 4));
+  if (fixed) {
+    OMNI_TRY_HIP(ctx, ctx->sum.alloc(sizeof(long long) * V));
+    OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
+  } else {
+    OMNI_TRY_HIP(ctx, ctx->sumd.alloc(sizeof(double) * V));
+    OMNI_TRY_HIP(ctx, ctx->wsum.alloc(sizeof(double) * V));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wsum.as<void>(), 2 * V, 0u, st));
+  }
+  long long* sumfx = fixed ? ctx->sum.as<long long>() : nullptr;
+  double* sumd = fixed ? nullptr : ctx->sumd.as<double>();
+  unsigned int* wcnt = fixed ? ctx->wcnt.as<unsigned int>() : nullptr;
+  double* wsum = fixed ? nullptr : ctx->wsum.as<double>();
+  double* pa = ctx->pa.as<double>();
+  double* pz = ctx->pb.as<double>();
+  if (pinit)
+    OMNI_TRY_HIP(ctx, (launch_convert<double, double>(pinit, pa, V, st)));
+  else
+    OMNI_TRY_HIP(ctx, launch_fill_f64(pa, V, 0.0, st));
+  OMNI_TRY_HIP(ctx, launch_fill_f64(pz, V, 0.0, st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
+  record_event(ctx, 2);
+  // the pass from p = 0: b, W and the run starts
+  OMNI_TRY_HIP(ctx, launch_walk_naive<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                         pl.g, wp, field, aux, pz, pz, 1, sc, sumfx, wcnt, sumd,
+                                         wsum, ctx->srcf.as<uint8_t>(), st));
+  record_event(ctx, 8);
+  // S in tile order: tile t's rows are [tfirst[t], tfirst[t + 1])
+  OMNI_TRY_HIP(ctx, ctx->am_sid.alloc(sizeof(int) * V));
+  OMNI_TRY_HIP(ctx, ctx->am_slist.alloc(sizeof(int) * (V + 1)));
+  OMNI_TRY_HIP(ctx, ctx->am_tfirst.alloc(sizeof(int) * ((size_t)pl.ntiles + 1)));
+  DevBuf dns, bcnt;
+  OMNI_TRY_HIP(ctx, dns.alloc(sizeof(int)));
+  OMNI_TRY_HIP(ctx, bcnt.alloc(sizeof(unsigned int) * (size_t)source_ids_tiled_blocks(pl.tg)));
+  OMNI_TRY_HIP(ctx, launch_source_ids_tiled(ctx->srcf.as<uint8_t>(), pl.g, pl.tg,
+                                            bcnt.as<unsigned int>(), ctx->am_sid.as<int>(),
+                                            ctx->am_slist.as<int>(), dns.as<int>(),
+                                            ctx->am_tfirst.as<int>(), st));
+  int nS = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, dns.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  if (nS <= 0) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  const int* sid = ctx->am_sid.as<int>();
+  const int* slist = ctx->am_slist.as<int>();
+  const int* tfirst = ctx->am_tfirst.as<int>();
+  // the rows: counts, offsets (CSR over S, so each tile's entries are one
+  // range), then the entries
+  const int nt = pl.ntiles;
+  DevBuf rcnt, dmax, stmp;
+  OMNI_TRY_HIP(ctx, rcnt.alloc(sizeof(unsigned int) * ((size_t)nS + 1)));
+  OMNI_TRY_HIP(ctx, dmax.alloc(sizeof(unsigned int)));
+  OMNI_TRY_HIP(ctx, ctx->am_ptr.alloc(sizeof(unsigned long long) * ((size_t)nS + 1)));
+  OMNI_TRY_HIP(ctx, ctx->am_tbase.alloc(sizeof(unsigned long long) * ((size_t)nt + 1)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(rcnt.as<void>(), (int64_t)nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(dmax.as<void>(), 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_lane_rows(false, m, pl.rays.as<RayRec>(), pl.nrays,
+                                     pl.dirs.as<DirConst>(), pl.g, pl.tg, wp, aux, sid, tfirst,
+                                     rcnt.as<unsigned int>(), nullptr, nullptr, nullptr, nullptr,
+                                     st));
+  OMNI_TRY_HIP(ctx, launch_max_u32(rcnt.as<unsigned int>(), nS, dmax.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, stmp.alloc(scan_u32_tmp_bytes(nS)));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(rcnt.as<unsigned int>(), nS,
+                                    ctx->am_ptr.as<unsigned long long>(), 0, stmp.as<void>(),
+                                    stmp.size(), st));
+  OMNI_TRY_HIP(ctx, launch_tile_bases(nt, tfirst, ctx->am_ptr.as<unsigned long long>(),
+                                      ctx->am_tbase.as<unsigned long long>(), st));
+  std::vector<unsigned long long> hbase((size_t)nt + 1);
+  unsigned int rmax = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hbase.data(), ctx->am_tbase.as<void>(),
+                                   sizeof(unsigned long long) * ((size_t)nt + 1),
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&rmax, dmax.as<void>(), sizeof(unsigned int),
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  const unsigned long long nnz = hbase[nt];
+  if (nnz == 0 || rmax >= 4096u) {
+    *fallback = true;
+    return OMNI3D_OK;
+  }
+  OMNI_TRY_HIP(ctx, ctx->am_src.alloc(sizeof(unsigned int) * nnz));
+  OMNI_TRY_HIP(ctx, ctx->am_erow.alloc(sizeof(unsigned short) * nnz));
+  OMNI_TRY_HIP(ctx, ctx->am_w.alloc(sizeof(double) * (fixed ? 1 : nnz)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(rcnt.as<void>(), (int64_t)nS + 1, 0u, st));
+  OMNI_TRY_HIP(ctx, launch_lane_rows(true, m, pl.rays.as<RayRec>(), pl.nrays,
+                                     pl.dirs.as<DirConst>(), pl.g, pl.tg, wp, aux, sid, tfirst,
+                                     rcnt.as<unsigned int>(), ctx->am_ptr.as<unsigned long long>(),
+                                     ctx->am_src.as<unsigned int>(),
+                                     ctx->am_erow.as<unsigned short>(),
+                                     fixed ? nullptr : ctx->am_w.as<double>(), st));
+  // chunks of at most csz entries of one tile, a workgroup each
+  const unsigned long long csz = std::max<unsigned long long>(
+      2048ull, std::min<unsigned long long>(65536ull, nnz / 4096ull));
+  std::vector<MaskChunk> chunks;
+  for (int t = 0; t < nt; t++)
+    for (unsigned long long b = hbase[t]; b < hbase[t + 1]; b += csz)
+      chunks.push_back(MaskChunk{b, std::min(hbase[t + 1], b + csz), t, 0});
+  const int nchunks = (int)chunks.size();
+  OMNI_TRY_HIP(ctx, ctx->am_chunks.alloc(sizeof(MaskChunk) * chunks.size()));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(ctx->am_chunks.as<void>(), chunks.data(),
+                                   sizeof(MaskChunk) * chunks.size(), hipMemcpyHostToDevice, st));
+  OMNI_TRY_HIP(ctx, ctx->am_wmax.alloc(sizeof(unsigned long long)));
+  if (fixed) {
+    OMNI_TRY_HIP(ctx, launch_fill_f64(reinterpret_cast<double*>(ctx->am_wmax.as<void>()), 1, 1.0,
+                                      st));
+  } else {
+    // a weight 1 / curltmp * counttmp with curltmp = 0 (WeightedMiniCurl
+    // tests no curltmp against zero) is not finite: such a field runs the
+    // reference schedule, whose sums carry it as the reference's do
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_wmax.as<void>(), 2, 0u, st));
+    OMNI_TRY_HIP(ctx, launch_absmax<double>(ctx->am_w.as<double>(), (int64_t)nnz,
+                                            ctx->am_wmax.as<unsigned long long>(), st));
+    double wmax = 0.0;
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&wmax, ctx->am_wmax.as<void>(), sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (!std::isfinite(wmax)) {
+      *fallback = true;
+      return OMNI3D_OK;
+    }
+  }
+  OMNI_TRY_HIP(ctx, ctx->am_rowsum.alloc(sizeof(unsigned long long) * (size_t)nS));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_rowsum.as<void>(), 2 * (int64_t)nS, 0u, st));
+  const size_t ns = (size_t)nS;
+  OMNI_TRY_HIP(ctx, ctx->am_b.alloc(sizeof(double) * ns));
+  OMNI_TRY_HIP(ctx, ctx->am_wv.alloc(sizeof(double) * ns));
+  OMNI_TRY_HIP(ctx, launch_masked_rowvec(nS, slist, fixed ? 1 : 0, sumfx, wcnt, sumd, wsum, sc,
+                                         ctx->am_b.as<double>(), ctx->am_wv.as<double>(), st));
+  record_event(ctx, 3);
+  // the iteration on S from p_S^0
+  OMNI_TRY_HIP(ctx, ctx->am_p0.alloc(sizeof(double) * ns));
+  OMNI_TRY_HIP(ctx, ctx->am_p1.alloc(sizeof(double) * ns));
+  double* q0 = ctx->am_p0.as<double>();
+  double* q1 = ctx->am_p1.as<double>();
+  OMNI_TRY_HIP(ctx, launch_gather_sources(slist, nS, pinit, q0, st));
+  OMNI_TRY_HIP(ctx, launch_absmax<double>(q0, nS, &sc->itmax_bits[0], st));
+  const unsigned int grid = mtile_finish_grid(nS);
+  OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 2 * grid));
+  for (int k = 1; k <= P.max_iterations; k++) {
+    OMNI_TRY_HIP(ctx, launch_mtile_accum(k, nchunks, ctx->am_chunks.as<MaskChunk>(), tfirst,
+                                         ctx->am_src.as<unsigned int>(),
+                                         ctx->am_erow.as<unsigned short>(),
+                                         fixed ? nullptr : ctx->am_w.as<double>(), q0, q1, sc,
+                                         ctx->am_wmax.as<unsigned long long>(),
+                                         ctx->am_rowsum.as<unsigned long long>(), st));
+    OMNI_TRY_HIP(ctx, launch_mtile_finish(k, nS, ctx->am_b.as<double>(), ctx->am_wv.as<double>(),
+                                          slist, pinit, seed > 0 ? 1 : 0, q0, q1,
+                                          ctx->am_rowsum.as<unsigned long long>(), sc,
+                                          ctx->am_wmax.as<unsigned long long>(),
+                                          ctx->partials.as<double>(), st));
+    OMNI_TRY_HIP(ctx, launch_iter_finish2(k, (int)grid, ctx->partials.as<double>(), sc,
+                                          P.max_iterations, P.tolerance, st));
+  }
+  record_event(ctx, 4);
+  // the final pass (pass k*) from p_S^{k*-1} on scalars whose stop flag is
+  // clear; pa keeps the starting p off S; the sums cleared, the weights kept
+  OMNI_TRY_HIP(ctx, ctx->am_sc.alloc(sizeof(DevScalars)));
+  DevScalars* scf = ctx->am_sc.as<DevScalars>();
+  OMNI_TRY_HIP(ctx, launch_masked_final_scalars(sc, scf, st));
+  OMNI_TRY_HIP(ctx, launch_masked_scatter(nS, slist, sc, q0, q1, pa, st));
+  if (fixed)
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V, 0u, st));
+  else
+    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sumd.as<void>(), 2 * V, 0u, st));
+  record_event(ctx, 5);
+  OMNI_TRY_HIP(ctx, launch_walk_naive<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                         pl.g, wp, field, aux, pa, pa, 1, scf, sumfx, nullptr, sumd,
+                                         nullptr, nullptr, st));
+  record_event(ctx, 6);
+  const bool f64 = dtype == OMNI3D_F64;
+  OMNI_TRY_HIP(ctx, launch_masked_final(V, fixed ? 1 : 0, sumfx, wcnt, sumd, wsum, pinit,
+                                        seed > 0 ? 1 : 0, sc, pa, f64 ? (double*)pdst : nullptr,
+                                        f64 ? nullptr : (float*)pdst,
+                                        (wdst && f64) ? (double*)wdst : nullptr,
+                                        (wdst && !f64) ? (float*)wdst : nullptr, st));
+  ctx->am_nnz = (int64_t)nnz;
+  ctx->am_words = (int64_t)nnz;
+  ctx->am_ns = nS;
+  return OMNI3D_OK;
+}
+
 // Whether a reconstruction's source iteration can run inside a
 // multi-snapshot launch: the plain method on the affine solver with the
 // grouped dense iteration (tile engine, p_S over three buffers).
@@ -1534,6 +1762,16 @@ int enqueue_solve(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) 
          P.method == OMNI3D_METHOD_WEIGHTED)) {
       rc = fp32 ? run_affine_masked<float>(ctx, seed, pdst, wdst, dtype, &fallback)
                 : run_affine_masked<double>(ctx, seed, pdst, wdst, dtype, &fallback);
+      if (rc) return rc;
+    }
+    // the walks off the plan's staircase: the same schedule, one lane per ray
+    if (P.solver == OMNI3D_SOLVER_AFFINE && ctx->affine_masked && !ctx->capturing &&
+        !(menv && std::strcmp(menv, "naive") == 0) &&
+        (P.method == OMNI3D_METHOD_SELECT_FIXEDBC || P.method == OMNI3D_METHOD_MINICURL ||
+         P.method == OMNI3D_METHOD_WEIGHTED_FIXEDBC ||
+         P.method == OMNI3D_METHOD_WEIGHTED_MINICURL)) {
+      rc = fp32 ? run_affine_lane<float>(ctx, seed, pdst, wdst, dtype, &fallback)
+                : run_affine_lane<double>(ctx, seed, pdst, wdst, dtype, &fallback);
       if (rc) return rc;
     }
     if (fallback) {

@@ -18,6 +18,7 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "tiles.hpp"
 #include "walk.hpp"
 
 namespace omni3d {
@@ -50,53 +51,130 @@ __device__ __forceinline__ void naive_plain(const RayRec& rr, const DirConst* __
   }
 }
 
+// What a walk does where a run (re)starts and where a visit adds.  The
+// walks below take the rule once; the sink decides what a visit becomes:
+//   PassSink   the pass itself: p is read where a run starts (and the voxel
+//              flagged as a run start), each visit adds its value and weight
+//              to the sums in the reference's expressions;
+//   EntrySink  the visits landing on a run-start voxel as entries (row =
+//              that voxel's id, source = the id of the voxel the run
+//              started from, weight) of the affine map a pass applies to p
+//              on the run starts: counted per row (kFill false) or written
+//              at the row's offset plus a slot from its cursor, with the row
+//              relative to its tile's first row (the tile layout of
+//              k_mtile_accum).  No gradient is read.
+struct PassSink {
+  using Start = double;
+  static constexpr bool kValues = true;
+  const double* __restrict__ p;
+  double scale;
+  long long* __restrict__ sumfx;
+  unsigned int* __restrict__ wcnt;
+  double* __restrict__ sumd;
+  double* __restrict__ wsum;
+  uint8_t* __restrict__ src;
+  __device__ __forceinline__ double start(int64_t v) const {
+    if (src) src[v] = 1;
+    return p[v];
+  }
+  // a visit with unit weight: p0 + pint in 64-bit fixed point, a count
+  __device__ __forceinline__ void unit(int64_t v, double p0, double pv) const {
+    fx_add(&sumfx[v], to_fixed(p0 + pv, scale));
+    if (wcnt) atomicAdd(&wcnt[v], 1u);
+  }
+  // a visit with curl weight counttmp / curltmp
+  __device__ __forceinline__ void curl(int64_t v, double p0, double pv, double c, double n) const {
+    add_f64(&sumd[v], (p0 + pv) * 1 / c * n);
+    if (wsum) add_f64(&wsum[v], 1 / c * n);
+  }
+  // the entry voxel's own value (pint = 0) with curl weight
+  __device__ __forceinline__ void curl_entry(int64_t v, double p0, double c, double n) const {
+    add_f64(&sumd[v], p0 * 1 / c * n);
+    if (wsum) add_f64(&wsum[v], 1 / c * n);
+  }
+};
+
+template <bool kFill>
+struct EntrySink {
+  using Start = int;
+  static constexpr bool kValues = false;
+  const int* __restrict__ sid;
+  GridDesc g;
+  TileGrid tg;
+  const int* __restrict__ tfirst;
+  unsigned int* __restrict__ cnt;  // entries per row (count) or the rows' cursors (fill)
+  const unsigned long long* __restrict__ rowptr;
+  unsigned int* __restrict__ esrc;
+  unsigned short* __restrict__ erow;
+  double* __restrict__ ew;
+  __device__ __forceinline__ int start(int64_t v) const { return sid[v]; }
+  __device__ __forceinline__ void put(int64_t v, int s0, double w) const {
+    const int row = sid[v];
+    if (row < 0 || s0 < 0) return;
+    if constexpr (!kFill) {
+      atomicAdd(&cnt[row], 1u);
+    } else {
+      const unsigned long long at = rowptr[row] + atomicAdd(&cnt[row], 1u);
+      const int i = (int)(v % g.X), j = (int)((v / g.X) % g.Y);
+      const int k = (int)(v / ((int64_t)g.X * g.Y));
+      esrc[at] = (unsigned int)s0;
+      erow[at] = (unsigned short)(row - tfirst[tile_of(i, j, k, tg)]);
+      if (ew) ew[at] = w;
+    }
+  }
+  __device__ __forceinline__ void unit(int64_t v, int s0, double) const { put(v, s0, 1.0); }
+  __device__ __forceinline__ void curl(int64_t v, int s0, double, double c, double n) const {
+    put(v, s0, 1 / c * n);
+  }
+  __device__ __forceinline__ void curl_entry(int64_t v, int s0, double c, double n) const {
+    put(v, s0, 1 / c * n);
+  }
+};
+
 // Select walk (bodyIntegralInnerSelect, kernel.cu:2280-2537).  aux is the
 // |curl a| * mask field (Select) or the mask (plain-mask, thr = +inf, unit
 // weights, symmetric rule).  With faithful the per-axis asymmetries of the
 // reference are kept: y/z steps test only curl != 0 for the first valid
 // voxel and do not treat curl == 0 as leaving the valid zone
 // (kernel.cu:2422,2442,2468,2488), and a z-step restart keeps the weight
-// accumulators (kernel.cu:2497-2506).
-template <typename T, bool kUnit, bool kFixedBC = false>
-__device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* __restrict__ dirs,
-                                             const GridDesc& g, const WalkParams& wp,
-                                             const typename Vec4<T>::type* __restrict__ field,
-                                             const double* __restrict__ aux,
-                                             const double* __restrict__ p, double scale,
-                                             long long* __restrict__ sumfx,
-                                             unsigned int* __restrict__ wcnt,
-                                             double* __restrict__ sumd, double* __restrict__ wsum,
-                                             uint8_t* __restrict__ src) {
+// accumulators (kernel.cu:2497-2506).  kFixedBC: the step rule of
+// bodyIntegralInnerSelectFixedBC (kernel.cu:2538-2796).
+template <typename T, bool kUnit, bool kFixedBC, typename Sink>
+__device__ __forceinline__ void walk_select(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                            const GridDesc& g, const WalkParams& wp,
+                                            const typename Vec4<T>::type* __restrict__ field,
+                                            const double* __restrict__ aux, const Sink& sk) {
   using V4 = typename Vec4<T>::type;
   const double thr = wp.thr;
   const bool faithful = wp.faithful != 0;
   Walker w;
   w.init(rr, dirs, g);
-  double p0 = 0.0, curltmp = 0.0, counttmp = 0.0;
+  typename Sink::Start p0{};
+  double curltmp = 0.0, counttmp = 0.0;
   bool p0set = false, outthr = false;
   RaySum<T> pint;
   {
     const double ci = aux[w.v];
     const bool entry_valid = faithful ? !(ci == 0.0 || ci > thr) : (ci != 0.0 && ci < thr);
     if (entry_valid) {
-      p0 = p[w.v];
-      if (src) src[w.v] = 1;
+      p0 = sk.start(w.v);
       p0set = true;
     }
   }
-  V4 cur = field[w.v];
+  V4 cur{};
+  if constexpr (Sink::kValues) cur = field[w.v];
   while (!w.done()) {
     int s;
     const int a = kFixedBC ? w.advance_fixedbc(&s) : w.advance(&s);
     if (a < 0) break;
-    const V4 nxt = field[w.v];
+    V4 nxt{};
+    if constexpr (Sink::kValues) nxt = field[w.v];
     const double cn = aux[w.v];
     const bool valid = cn < thr && cn != 0.0;
     if (!p0set) {
       const bool ok = (faithful && a != 0) ? (cn != 0.0) : valid;
       if (ok) {
-        p0 = p[w.v];
-        if (src) src[w.v] = 1;
+        p0 = sk.start(w.v);
         p0set = true;
         pint.reset();
         counttmp = 0.0;
@@ -104,23 +182,22 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
       }
     } else {
       if (!outthr && valid) {
-        pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+        if constexpr (Sink::kValues)
+          pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+        const double pv = Sink::kValues ? pint.value() : 0.0;
         if (kUnit) {
-          fx_add(&sumfx[w.v], to_fixed(p0 + pint.value(), scale));
-          if (wcnt) atomicAdd(&wcnt[w.v], 1u);
+          sk.unit(w.v, p0, pv);
         } else {
           curltmp += cn / 10000;
           counttmp += 1.0;
-          add_f64(&sumd[w.v], (p0 + pint.value()) * 1 / curltmp * counttmp);
-          if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
+          sk.curl(w.v, p0, pv, curltmp, counttmp);
         }
       }
       const bool enter_high = (faithful && a != 0) ? (!outthr && cn >= thr)
                                                    : ((!outthr && cn >= thr) || cn == 0.0);
       if (enter_high) outthr = true;
       if (outthr && valid) {
-        p0 = p[w.v];
-        if (src) src[w.v] = 1;
+        p0 = sk.start(w.v);
         pint.reset();
         outthr = false;
         if (!(faithful && a == 2)) {
@@ -129,7 +206,7 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
         }
       }
     }
-    cur = nxt;
+    if constexpr (Sink::kValues) cur = nxt;
   }
 }
 
@@ -137,87 +214,90 @@ __device__ __forceinline__ void naive_select(const RayRec& rr, const DirConst* _
 // self-add, no path break; weight counttmp / curltmp with curltmp the running
 // sum of curl / 10^4.  kNoJ0: bodyIntegralInnerWeightedFixedBC
 // (kernel.cu:2136-2279), the same walk ended by a step onto the j = 0 plane.
-template <typename T, bool kNoJ0 = false>
-__device__ __forceinline__ void naive_weighted(const RayRec& rr, const DirConst* __restrict__ dirs,
-                                               const GridDesc& g, const WalkParams& wp,
-                                               const typename Vec4<T>::type* __restrict__ field,
-                                               const double* __restrict__ aux,
-                                               const double* __restrict__ p,
-                                               double* __restrict__ sumd,
-                                               double* __restrict__ wsum,
-                                               uint8_t* __restrict__ src) {
+template <typename T, bool kNoJ0, typename Sink>
+__device__ __forceinline__ void walk_weighted(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                              const GridDesc& g, const WalkParams& wp,
+                                              const typename Vec4<T>::type* __restrict__ field,
+                                              const double* __restrict__ aux, const Sink& sk) {
   using V4 = typename Vec4<T>::type;
   Walker w;
   w.init(rr, dirs, g);
-  const double pin = p[w.v];
-  if (src) src[w.v] = 1;
+  const typename Sink::Start pin = sk.start(w.v);
   double curltmp = 0.0, counttmp = 0.0;
   RaySum<T> pint;
-  V4 cur = field[w.v];
+  V4 cur{};
+  if constexpr (Sink::kValues) cur = field[w.v];
   while (!w.done()) {
     int s;
     const int a = kNoJ0 ? w.advance_noj0(&s) : w.advance(&s);
     if (a < 0) break;
-    const V4 nxt = field[w.v];
-    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+    if constexpr (Sink::kValues) {
+      const V4 nxt = field[w.v];
+      pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+      cur = nxt;
+    }
     curltmp += aux[w.v] / 10000;
     counttmp += 1.0;
-    if (curltmp != 0.0) {
-      add_f64(&sumd[w.v], (pin + pint.value()) * 1 / curltmp * counttmp);
-      if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
-    }
-    cur = nxt;
+    if (curltmp != 0.0) sk.curl(w.v, pin, Sink::kValues ? pint.value() : 0.0, curltmp, counttmp);
   }
+}
+
+// One step of the MiniCurl path (kernel.cu:2797-2936): toward the exit
+// through the candidate voxel with the smallest curl (ties x, then y, then
+// z).  An axis already at the exit offers the reference's sentinel
+// (coordinate - 60000): its curl is read at the aliased linear index when
+// that lies in the volume, else it counts as 1e10; choosing it ends the
+// walk.  Returns the axis (sign in *s_out) or -1.
+__device__ __forceinline__ int minicurl_step(Walker& w, const GridDesc& g, int64_t V,
+                                             const double* __restrict__ aux, int* s_out) {
+  const int s0 = w.c0 < w.e0 ? 1 : (w.c0 > w.e0 ? -1 : -60000);
+  const int s1 = w.c1 < w.e1 ? 1 : (w.c1 > w.e1 ? -1 : -60000);
+  const int s2 = w.c2 < w.e2 ? 1 : (w.c2 > w.e2 ? -1 : -60000);
+  const int64_t i0 = w.v + s0, i1 = w.v + (int64_t)s1 * w.sy, i2 = w.v + (int64_t)s2 * w.sz;
+  const double d0 = (i0 >= 0 && i0 < V) ? aux[i0] : 1e10;
+  const double d1 = (i1 >= 0 && i1 < V) ? aux[i1] : 1e10;
+  const double d2 = (i2 >= 0 && i2 < V) ? aux[i2] : 1e10;
+  int a, s, nc, lim;
+  int64_t vn;
+  if (d0 <= d1 && d0 <= d2) { a = 0; s = s0; nc = w.c0 + s0; lim = g.X; vn = i0; }
+  else if (d1 < d0 && d1 <= d2) { a = 1; s = s1; nc = w.c1 + s1; lim = g.Y; vn = i1; }
+  else if (d2 < d0 && d2 < d1) { a = 2; s = s2; nc = w.c2 + s2; lim = g.Z; vn = i2; }
+  else return -1;
+  if (nc < 0 || nc >= lim) return -1;
+  if (a == 0) w.c0 = nc;
+  else if (a == 1) w.c1 = nc;
+  else w.c2 = nc;
+  w.v = vn;
+  *s_out = s;
+  return a;
 }
 
 // MiniCurl walk (bodyIntegralInnerMiniCurl, kernel.cu:2797-2936, the pass of
 // omni3dparallellinesESInnerWeightedMiniCurl): plain sums and unit counts,
-// no entry self-add, along a path that steps toward the exit through the
-// candidate voxel with the smallest curl (ties x, then y, then z).  An axis
-// already at the exit offers the reference's sentinel (coordinate - 60000):
-// its curl is read at the aliased linear index when that lies in the volume,
-// else it counts as 1e10; choosing it ends the walk.
-template <typename T>
-__device__ __forceinline__ void naive_minicurl(const RayRec& rr, const DirConst* __restrict__ dirs,
-                                               const GridDesc& g, const WalkParams& wp,
-                                               const typename Vec4<T>::type* __restrict__ field,
-                                               const double* __restrict__ aux,
-                                               const double* __restrict__ p, double scale,
-                                               long long* __restrict__ sumfx,
-                                               unsigned int* __restrict__ wcnt,
-                                               uint8_t* __restrict__ src) {
+// no entry self-add, along the MiniCurl path.
+template <typename T, typename Sink>
+__device__ __forceinline__ void walk_minicurl(const RayRec& rr, const DirConst* __restrict__ dirs,
+                                              const GridDesc& g, const WalkParams& wp,
+                                              const typename Vec4<T>::type* __restrict__ field,
+                                              const double* __restrict__ aux, const Sink& sk) {
   using V4 = typename Vec4<T>::type;
   Walker w;
   w.init(rr, dirs, g);
   const int64_t V = (int64_t)g.X * g.Y * g.Z;
-  const double pin = p[w.v];
-  if (src) src[w.v] = 1;
-  V4 cur = field[w.v];
+  const typename Sink::Start pin = sk.start(w.v);
+  V4 cur{};
+  if constexpr (Sink::kValues) cur = field[w.v];
   RaySum<T> pint;
   while (!w.done()) {
-    const int s0 = w.c0 < w.e0 ? 1 : (w.c0 > w.e0 ? -1 : -60000);
-    const int s1 = w.c1 < w.e1 ? 1 : (w.c1 > w.e1 ? -1 : -60000);
-    const int s2 = w.c2 < w.e2 ? 1 : (w.c2 > w.e2 ? -1 : -60000);
-    const int64_t i0 = w.v + s0, i1 = w.v + (int64_t)s1 * w.sy, i2 = w.v + (int64_t)s2 * w.sz;
-    const double d0 = (i0 >= 0 && i0 < V) ? aux[i0] : 1e10;
-    const double d1 = (i1 >= 0 && i1 < V) ? aux[i1] : 1e10;
-    const double d2 = (i2 >= 0 && i2 < V) ? aux[i2] : 1e10;
-    int a, s, nc, lim;
-    int64_t vn;
-    if (d0 <= d1 && d0 <= d2) { a = 0; s = s0; nc = w.c0 + s0; lim = g.X; vn = i0; }
-    else if (d1 < d0 && d1 <= d2) { a = 1; s = s1; nc = w.c1 + s1; lim = g.Y; vn = i1; }
-    else if (d2 < d0 && d2 < d1) { a = 2; s = s2; nc = w.c2 + s2; lim = g.Z; vn = i2; }
-    else break;
-    if (nc < 0 || nc >= lim) break;
-    if (a == 0) w.c0 = nc;
-    else if (a == 1) w.c1 = nc;
-    else w.c2 = nc;
-    w.v = vn;
-    const V4 nxt = field[vn];
-    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
-    fx_add(&sumfx[vn], to_fixed(pin + pint.value(), scale));
-    if (wcnt) atomicAdd(&wcnt[vn], 1u);
-    cur = nxt;
+    int s;
+    const int a = minicurl_step(w, g, V, aux, &s);
+    if (a < 0) break;
+    if constexpr (Sink::kValues) {
+      const V4 nxt = field[w.v];
+      pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+      cur = nxt;
+    }
+    sk.unit(w.v, pin, Sink::kValues ? pint.value() : 0.0);
   }
 }
 
@@ -226,49 +306,33 @@ __device__ __forceinline__ void naive_minicurl(const RayRec& rr, const DirConst*
 // weight 1 / curl[entry] (curltmp = curl there, counttmp = 1, kernel.cu:2956-
 // 2960), every step adds (p_in + pint) * counttmp / curltmp with curltmp the
 // running sum of curl (no / 10^4) and no test of curltmp against zero.
-template <typename T>
-__device__ __forceinline__ void naive_weighted_minicurl(
+template <typename T, typename Sink>
+__device__ __forceinline__ void walk_weighted_minicurl(
     const RayRec& rr, const DirConst* __restrict__ dirs, const GridDesc& g, const WalkParams& wp,
     const typename Vec4<T>::type* __restrict__ field, const double* __restrict__ aux,
-    const double* __restrict__ p, double* __restrict__ sumd, double* __restrict__ wsum,
-    uint8_t* __restrict__ src) {
+    const Sink& sk) {
   using V4 = typename Vec4<T>::type;
   Walker w;
   w.init(rr, dirs, g);
   const int64_t V = (int64_t)g.X * g.Y * g.Z;
-  const double pin = p[w.v];
-  if (src) src[w.v] = 1;
+  const typename Sink::Start pin = sk.start(w.v);
   double curltmp = aux[w.v], counttmp = 1.0;
-  add_f64(&sumd[w.v], pin * 1 / curltmp * counttmp);
-  if (wsum) add_f64(&wsum[w.v], 1 / curltmp * counttmp);
-  V4 cur = field[w.v];
+  sk.curl_entry(w.v, pin, curltmp, counttmp);
+  V4 cur{};
+  if constexpr (Sink::kValues) cur = field[w.v];
   RaySum<T> pint;
   while (!w.done()) {
-    const int s0 = w.c0 < w.e0 ? 1 : (w.c0 > w.e0 ? -1 : -60000);
-    const int s1 = w.c1 < w.e1 ? 1 : (w.c1 > w.e1 ? -1 : -60000);
-    const int s2 = w.c2 < w.e2 ? 1 : (w.c2 > w.e2 ? -1 : -60000);
-    const int64_t i0 = w.v + s0, i1 = w.v + (int64_t)s1 * w.sy, i2 = w.v + (int64_t)s2 * w.sz;
-    const double d0 = (i0 >= 0 && i0 < V) ? aux[i0] : 1e10;
-    const double d1 = (i1 >= 0 && i1 < V) ? aux[i1] : 1e10;
-    const double d2 = (i2 >= 0 && i2 < V) ? aux[i2] : 1e10;
-    int a, s, nc, lim;
-    int64_t vn;
-    if (d0 <= d1 && d0 <= d2) { a = 0; s = s0; nc = w.c0 + s0; lim = g.X; vn = i0; }
-    else if (d1 < d0 && d1 <= d2) { a = 1; s = s1; nc = w.c1 + s1; lim = g.Y; vn = i1; }
-    else if (d2 < d0 && d2 < d1) { a = 2; s = s2; nc = w.c2 + s2; lim = g.Z; vn = i2; }
-    else break;
-    if (nc < 0 || nc >= lim) break;
-    if (a == 0) w.c0 = nc;
-    else if (a == 1) w.c1 = nc;
-    else w.c2 = nc;
-    w.v = vn;
-    const V4 nxt = field[vn];
-    pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
-    curltmp += aux[vn];
+    int s;
+    const int a = minicurl_step(w, g, V, aux, &s);
+    if (a < 0) break;
+    if constexpr (Sink::kValues) {
+      const V4 nxt = field[w.v];
+      pint.add(trapezoid<T>(wp, a, s, component(nxt, a), component(cur, a)));
+      cur = nxt;
+    }
+    curltmp += aux[w.v];
     counttmp += 1.0;
-    add_f64(&sumd[vn], (pin + pint.value()) * 1 / curltmp * counttmp);
-    if (wsum) add_f64(&wsum[vn], 1 / curltmp * counttmp);
-    cur = nxt;
+    sk.curl(w.v, pin, Sink::kValues ? pint.value() : 0.0, curltmp, counttmp);
   }
 }
 
@@ -286,24 +350,51 @@ k_walk_naive(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __r
   const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   const RayRec rr = rays[r];
+  const PassSink sk{p, sc->scale, sumfx, wcnt, sumd, wsum, src};
   if (kMethod == 0)
     naive_plain<T>(rr, dirs, g, wp, field, p, sc->scale, sumfx);
   else if (kMethod == 1)
-    naive_select<T, true>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd, wsum, src);
+    walk_select<T, true, false>(rr, dirs, g, wp, field, aux, sk);
   else if (kMethod == 2)
-    naive_select<T, false>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd, wsum,
-                           src);
+    walk_select<T, false, false>(rr, dirs, g, wp, field, aux, sk);
   else if (kMethod == 4)
-    naive_select<T, false, true>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, sumd,
-                                 wsum, src);
+    walk_select<T, false, true>(rr, dirs, g, wp, field, aux, sk);
   else if (kMethod == 5)
-    naive_minicurl<T>(rr, dirs, g, wp, field, aux, p, sc->scale, sumfx, wcnt, src);
+    walk_minicurl<T>(rr, dirs, g, wp, field, aux, sk);
   else if (kMethod == 6)
-    naive_weighted<T, true>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
+    walk_weighted<T, true>(rr, dirs, g, wp, field, aux, sk);
   else if (kMethod == 7)
-    naive_weighted_minicurl<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
+    walk_weighted_minicurl<T>(rr, dirs, g, wp, field, aux, sk);
   else
-    naive_weighted<T>(rr, dirs, g, wp, field, aux, p, sumd, wsum, src);
+    walk_weighted<T, false>(rr, dirs, g, wp, field, aux, sk);
+}
+
+// The entries of the affine map of one pass on the run starts, one lane per
+// ray, for the methods whose walks leave the plan's staircase (SelectFixedBC,
+// MiniCurl, WeightedFixedBC, WeightedMiniCurl): the walk of the pass with an
+// EntrySink (no gradient read).
+template <int kMethod, bool kFill>
+__global__ void __launch_bounds__(256)
+k_lane_rows(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
+            GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ aux,
+            const int* __restrict__ sid, const int* __restrict__ tfirst,
+            unsigned int* __restrict__ cnt, const unsigned long long* __restrict__ rowptr,
+            unsigned int* __restrict__ esrc, unsigned short* __restrict__ erow,
+            double* __restrict__ ew) {
+  const int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const RayRec rr = rays[r];
+  const EntrySink<kFill> sk{sid, g, tg, tfirst, cnt, rowptr, esrc, erow, ew};
+  using V4 = Vec4<float>::type;
+  const V4* nofield = nullptr;
+  if (kMethod == 4)
+    walk_select<float, false, true>(rr, dirs, g, wp, nofield, aux, sk);
+  else if (kMethod == 5)
+    walk_minicurl<float>(rr, dirs, g, wp, nofield, aux, sk);
+  else if (kMethod == 6)
+    walk_weighted<float, true>(rr, dirs, g, wp, nofield, aux, sk);
+  else
+    walk_weighted_minicurl<float>(rr, dirs, g, wp, nofield, aux, sk);
 }
 
 // Averaging of one iteration (devidecountInner, kernel.cu:6746-6759: where
@@ -574,6 +665,48 @@ template hipError_t launch_walk_naive<double>(int, const RayRec*, int64_t, const
                                               const double*, const double*, const double*, int,
                                               const DevScalars*, long long*, unsigned int*,
                                               double*, double*, uint8_t*, hipStream_t);
+
+hipError_t launch_lane_rows(bool fill, int method, const RayRec* rays, int64_t nrays,
+                            const DirConst* dirs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                            const double* aux, const int* sid, const int* tfirst,
+                            unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, unsigned short* erow, double* ew, hipStream_t st) {
+  const unsigned int grid = grid_for(nrays, 1u << 30);
+#define OMNI_LR(M, FL)                                                                         \
+  hipLaunchKernelGGL((k_lane_rows<M, FL>), dim3(grid), dim3(256), 0, st, rays, nrays, dirs, g, \
+                     tg, wp, aux, sid, tfirst, cnt, rowptr, esrc, erow, ew)
+#define OMNI_LR_M(FL)                      \
+  switch (method) {                        \
+    case 4: OMNI_LR(4, FL); break;         \
+    case 5: OMNI_LR(5, FL); break;         \
+    case 6: OMNI_LR(6, FL); break;         \
+    case 7: OMNI_LR(7, FL); break;         \
+    default: return hipErrorInvalidValue;  \
+  }
+  if (fill) {
+    OMNI_LR_M(true)
+  } else {
+    OMNI_LR_M(false)
+  }
+#undef OMNI_LR_M
+#undef OMNI_LR
+  return hipGetLastError();
+}
+
+// tbase[t] = rowptr[tfirst[t]], t = 0..nt: the first entry of each tile's rows
+__global__ void __launch_bounds__(256)
+k_tile_bases(int nt, const int* __restrict__ tfirst, const unsigned long long* __restrict__ rowptr,
+             unsigned long long* __restrict__ tbase) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= nt) tbase[t] = rowptr[tfirst[t]];
+}
+
+hipError_t launch_tile_bases(int nt, const int* tfirst, const unsigned long long* rowptr,
+                             unsigned long long* tbase, hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_bases, dim3((unsigned int)((nt + 1 + 255) / 256)), dim3(256), 0, st, nt,
+                     tfirst, rowptr, tbase);
+  return hipGetLastError();
+}
 
 unsigned int naive_update_grid(int64_t V) { return grid_for(V, 2048); }
 

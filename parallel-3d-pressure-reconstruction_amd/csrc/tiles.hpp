@@ -136,6 +136,18 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
                             double* wc, double* wn, unsigned int* wcnt, double* wsum,
                             uint8_t* src, void* rule, hipStream_t st);
 size_t mask_rule_bytes(int method, int64_t nval);
+// naive.hip: the rows of the affine map of one pass for the walks that leave
+// the plan's staircase (methods 4-7), one lane per ray: entries per run-start
+// row (fill false: counted into cnt) or written (fill true: cnt = zeroed
+// cursors; source id, row within its tile, weight unless ew is null)
+hipError_t launch_lane_rows(bool fill, int method, const RayRec* rays, int64_t nrays,
+                            const DirConst* dirs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                            const double* aux, const int* sid, const int* tfirst,
+                            unsigned int* cnt, const unsigned long long* rowptr,
+                            unsigned int* esrc, unsigned short* erow, double* ew, hipStream_t st);
+// tbase[t] = rowptr[tfirst[t]] for t = 0..nt
+hipError_t launch_tile_bases(int nt, const int* tfirst, const unsigned long long* rowptr,
+                             unsigned long long* tbase, hipStream_t st);
 // OMNI3D_MASK_WALK=lane selects the one-lane-per-ray rule replay
 bool mask_walk_lane();
 hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
