@@ -387,7 +387,7 @@ def sor_measure(args):
         "data": "synthetic",
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_redblack_fused<64, 8, 32, 256> (red and black in one pass)",
+                     "traffic": traffic, "kernel": "k_redblack_multi<64, 16, 64, 2, 1024> (two red-black iterations per pass)",
                      "algorithmic_bytes": bytes_it},
         "cpu_baseline": cpu, "gpu_sor3D_dropin": jacobi_gpu,
         "config": {"workload": f"Neumann right-hand side of the burgers-vortex {n}^3 field, "

@@ -275,6 +275,142 @@ k_redblack_fused(int X, int Y, int Z, double omega, const double* __restrict__ p
   }
 }
 
+// kK whole red-black iterations in one pass (temporal blocking): the same
+// z-march with 2 kK half sweeps ("stages") in flight.  Stage s (red for even
+// s, black for odd s, iteration s / 2 + 1) takes plane k - s at step k over
+// the column plus a halo of h = 2 kK - 1 - s voxels in x, y and z: each
+// stage reads the previous stage's colour on the planes around it, which
+// that stage finished one step earlier or earlier in this step, so the
+// stages run in order within a step, a barrier apart.  The column's planes
+// load with a halo of 2 kK (a ring of 2 kK + 3 planes; the plane two steps
+// ahead loads into registers during a step); a plane is stored one step
+// after the last stage finished it.  Reads come from p_in and writes go to
+// p_out; every voxel update is the same expression on the same values as in
+// the half-sweep kernels, so kK iterations here equal kK single ones bit for
+// bit.  The halo voxels are updated redundantly by each column that needs
+// them (the price of one read and one write of p per kK iterations).
+template <int kTX, int kTY, int kZ, int kK, int kThreads>
+__global__ void __launch_bounds__(kThreads)
+k_redblack_multi(int X, int Y, int Z, double omega, const double* __restrict__ pin,
+                 const double* __restrict__ rhs, double* __restrict__ pout,
+                 const SorState* __restrict__ st) {
+  if (st->stop_at) return;
+  constexpr int kStages = 2 * kK;
+  constexpr int kHalo = 2 * kK;
+  constexpr int kW = kTX + 2 * kHalo, kH = kTY + 2 * kHalo;
+  constexpr int kRing = 2 * kK + 3;
+  constexpr int kLoads = (kW * kH + kThreads - 1) / kThreads;
+  static_assert(kTX % 2 == 0, "even column width");
+  __shared__ double P[kRing][kH][kW];
+  const int x0 = (int)blockIdx.x * kTX, y0 = (int)blockIdx.y * kTY;
+  const int z0 = (int)blockIdx.z * kZ, z1 = min(Z, z0 + kZ);
+  const int64_t sy = X, sz = (int64_t)X * Y;
+  const int tid = (int)threadIdx.x;
+  auto slot = [](int k) { return (k + 8 * kRing) % kRing; };  // k >= -2 kHalo
+  double pf[kLoads];
+  auto issue = [&](int k) {
+    if (k < 0 || k >= Z) return;
+#pragma unroll
+    for (int m = 0; m < kLoads; m++) {
+      const int l = tid + m * kThreads;
+      const int i = x0 - kHalo + l % kW, j = y0 - kHalo + l / kW;
+      if (l < kW * kH && i >= 0 && i < X && j >= 0 && j < Y) pf[m] = pin[i + j * sy + k * sz];
+    }
+  };
+  auto commit = [&](int k) {
+    if (k < 0 || k >= Z) return;
+#pragma unroll
+    for (int m = 0; m < kLoads; m++) {
+      const int l = tid + m * kThreads;
+      const int i = x0 - kHalo + l % kW, j = y0 - kHalo + l / kW;
+      if (l < kW * kH && i >= 0 && i < X && j >= 0 && j < Y) P[slot(k)][l / kW][l % kW] = pf[m];
+    }
+  };
+  // in-box neighbour sum in the reference's order (x-, x+, y-, y+, z-, z+)
+  auto update = [&](int i, int j, int k, double r) {
+    const int xx = i - x0 + kHalo, yy = j - y0 + kHalo;
+    double(*Pc)[kW] = P[slot(k)];
+    double s = 0.0;
+    int n = 0;
+    if (i > 0) { s = n ? s + Pc[yy][xx - 1] : Pc[yy][xx - 1]; n++; }
+    if (i < X - 1) { s = n ? s + Pc[yy][xx + 1] : Pc[yy][xx + 1]; n++; }
+    if (j > 0) { s = n ? s + Pc[yy - 1][xx] : Pc[yy - 1][xx]; n++; }
+    if (j < Y - 1) { s = n ? s + Pc[yy + 1][xx] : Pc[yy + 1][xx]; n++; }
+    if (k > 0) { const double q = P[slot(k - 1)][yy][xx]; s = n ? s + q : q; n++; }
+    if (k < Z - 1) { const double q = P[slot(k + 1)][yy][xx]; s = n ? s + q : q; n++; }
+    s += r;
+    s = s / n;
+    const double old = Pc[yy][xx];
+    Pc[yy][xx] = old + omega * (s - old);
+  };
+  // stage s on plane m: the voxels of its colour in the column plus halo h;
+  // thread tid takes voxels tid + q kThreads, q < kPer
+  constexpr int kPer = ((kTX / 2 + kStages - 1) * (kTY + 2 * (kStages - 1)) + kThreads - 1) /
+                       kThreads;
+  auto voxel = [&](int s, int m, int q, int* i, int* j) {
+    const int h = kStages - 1 - s;
+    const int half = kTX / 2 + h, rows = kTY + 2 * h;
+    const int l = tid + q * kThreads;
+    *j = y0 - h + l / half;
+    *i = x0 - h + ((x0 - h + *j + m + (s & 1)) & 1) + 2 * (l % half);
+    return l < half * rows && m >= 0 && m < Z && m >= z0 - h && m < z1 + h && *i >= 0 &&
+           *i < X && *j >= 0 && *j < Y;
+  };
+  // the right-hand sides of a step's stages load into registers one step
+  // ahead
+  double rr[kStages][kPer];
+  auto issue_rhs = [&](int k) {
+#pragma unroll
+    for (int s = 0; s < kStages; s++) {
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        int i, j;
+        if (voxel(s, k - s, q, &i, &j)) rr[s][q] = rhs[i + j * sy + (int64_t)(k - s) * sz];
+      }
+    }
+  };
+  auto stage = [&](int s, int m, const double (&r)[kPer]) {
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      int i, j;
+      if (voxel(s, m, q, &i, &j)) update(i, j, m, r[q]);
+    }
+  };
+  const int kb = z0 - kStages + 1;  // the first step: stage 0 on its lowest plane
+  issue(kb - 1);
+  commit(kb - 1);
+  issue(kb);
+  commit(kb);
+  issue(kb + 1);
+  issue_rhs(kb);
+  for (int k = kb; k <= z1 + kStages - 1; k++) {
+    commit(k + 1);
+    issue(k + 2);
+    double r0[kStages][kPer];
+#pragma unroll
+    for (int s = 0; s < kStages; s++)
+#pragma unroll
+      for (int q = 0; q < kPer; q++) r0[s][q] = rr[s][q];
+    issue_rhs(k + 1);
+    __syncthreads();
+    stage(0, k, r0[0]);
+    // the plane the last stage finished at the previous step
+    const int ms = k - kStages;
+    if (ms >= z0 && ms < z1) {
+      const double(*Pb)[kW] = P[slot(ms)];
+      for (int l = tid; l < kTX * kTY; l += kThreads) {
+        const int i = x0 + (l % kTX), j = y0 + l / kTX;
+        if (i < X && j < Y) pout[i + j * sy + ms * sz] = Pb[j - y0 + kHalo][i - x0 + kHalo];
+      }
+    }
+#pragma unroll
+    for (int s = 1; s < kStages; s++) {
+      __syncthreads();
+      stage(s, k - s, r0[s]);
+    }
+  }
+}
+
 // Interior residual sum |sum_nb - 6 p + rhs| (sor3D.cpp:741-781) reduced
 // over the grid; the last block applies the stop rule of sweep / iteration
 // `it`: residue = sum / X / Y / Z, stop when residue < eps.  fp32 rounds the
@@ -559,13 +695,32 @@ int omni3d_sor_solve_device(int X, int Y, int Z, double omega, int max_iter, dou
   const bool rows = (Y + 3) / 4 <= 65535 && Z <= 65535;
   const dim3 rgrid((nh + 63) / 64, (unsigned int)((Y + 3) / 4), (unsigned int)Z), rblock(64, 4);
   const unsigned int g = grid_for(V);
+  // with no stop test between iterations (eps <= 0) pairs of iterations run
+  // as one temporally blocked pass (k_redblack_multi, two iterations, same
+  // columns); OMNI3D_SOR=single keeps one pass per iteration
+  // column 64 x 16 over 64 planes, 1024 threads, one workgroup per CU (97 KB
+  // of LDS): the fastest of 32-128 x 8-16 columns, 16-64 planes, 256-1024
+  // threads and two or three iterations per pass at 256^3
+  constexpr int mtx = 64, mty = 16, mtz = 64, mthreads = 1024;
+  const dim3 mgrid((unsigned int)((X + mtx - 1) / mtx), (unsigned int)((Y + mty - 1) / mty),
+                   (unsigned int)((Z + mtz - 1) / mtz));
+  const bool pairs = fused && eps <= 0.0 && !(env && std::strcmp(env, "single") == 0) &&
+                     mgrid.y <= 65535 && mgrid.z <= 65535;
+  double* src = p_d;  // the buffer holding the latest iterate (fused passes)
   for (int it = 1; it <= max_iter; it++) {
     double* cur = p_d;
     if (fused) {
-      cur = buf[it & 1];
-      const double* prev = buf[(it - 1) & 1];
-      hipLaunchKernelGGL((k_redblack_fused<tx, ty, tz, 256>), fgrid, dim3(256), 0, s, X, Y, Z,
-                         omega, prev, rhs_d, cur, st);
+      double* dst = src == buf[0] ? buf[1] : buf[0];
+      if (pairs && it + 1 <= max_iter) {
+        hipLaunchKernelGGL((k_redblack_multi<mtx, mty, mtz, 2, mthreads>), mgrid, dim3(mthreads), 0,
+                           s, X, Y, Z, omega, src, rhs_d, dst, st);
+        it++;
+      } else {
+        hipLaunchKernelGGL((k_redblack_fused<tx, ty, tz, 256>), fgrid, dim3(256), 0, s, X, Y, Z,
+                           omega, src, rhs_d, dst, st);
+      }
+      src = dst;
+      cur = dst;
     } else {
       for (int color = 0; color < 2; color++) {
         if (rows)
@@ -584,9 +739,12 @@ int omni3d_sor_solve_device(int X, int Y, int Z, double omega, int max_iter, dou
   POIS_TRY(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
   POIS_TRY(hipStreamSynchronize(s));
   const int last = hs.stop_at ? hs.stop_at : max_iter;
-  if (fused && (last & 1)) {
-    // the last iteration wrote the second buffer
-    POIS_TRY(hipMemcpyAsync(p_d, buf[1], sizeof(double) * V, hipMemcpyDeviceToDevice, s));
+  // the buffer the last executed iteration wrote: with single passes
+  // iteration i writes buf[i & 1] (later launches return at once after a
+  // stop); with pairs (no stop test) the latest one
+  const double* res = pairs ? src : buf[last & 1];
+  if (fused && res != p_d) {
+    POIS_TRY(hipMemcpyAsync(p_d, res, sizeof(double) * V, hipMemcpyDeviceToDevice, s));
     POIS_TRY(hipStreamSynchronize(s));
   }
   if (iterations_out) *iterations_out = last;

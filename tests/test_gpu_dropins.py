@@ -208,6 +208,28 @@ def test_sor_redblack_rows_bit_exact_device_buffers(gpu_available):
     np.testing.assert_array_equal(pd.cpu().numpy(), po)
 
 
+@pytest.mark.parametrize("iters", [1, 2, 7, 16])
+def test_sor_paired_passes_equal_single_passes(gpu_available, monkeypatch, iters):
+    """With no stop test between iterations, pairs of red-black iterations
+    run as one temporally blocked pass: the same voxel updates on the same
+    values, so the result equals one pass per iteration (OMNI3D_SOR=single)
+    and the serial oracle bit for bit, on a grid that is no multiple of the
+    column (64 x 8 x 32) in any axis; an odd count ends with a single pass."""
+    shape = (37, 29, 71)
+    print(f"case red-black pairs {iters} iterations", flush=True)
+    rng = np.random.default_rng(21)
+    rhs = rng.normal(size=shape)
+    rhs -= rhs.mean()
+    p2, it2, res2 = omni3d.sor_solve(rhs, omega=1.8, max_iter=iters)
+    monkeypatch.setenv("OMNI3D_SOR", "single")
+    p1, it1, res1 = omni3d.sor_solve(rhs, omega=1.8, max_iter=iters)
+    monkeypatch.delenv("OMNI3D_SOR")
+    assert it1 == it2 == iters and res1 == res2
+    np.testing.assert_array_equal(p2, p1)
+    if iters <= 7:
+        np.testing.assert_array_equal(p2, O.sor_redblack(np.zeros(shape), rhs, iters, 1.8))
+
+
 @pytest.mark.parametrize("shape,kind", [((1, 1, 1), "uniform"), ((19, 23, 37), "uniform"),
                                         ((64, 64, 64), "curl"), ((128, 128, 128), "normal"),
                                         ((40, 40, 40), "constant"),
