@@ -20,7 +20,9 @@
 // a Jacobi sweep (the reference's only separation between iterations is a
 // __syncthreads(), an inter-block race); an exit node whose pairs are all
 // zero keeps its value (the reference divides 0 by 0).  Ray integrals use
-// the context's precision (FP64, or FP32 with Kahan compensation).
+// the context's precision (FP64, or FP32 gradient and trapezoid terms,
+// added into FP64 segment sums by the tile engine and into
+// Kahan-compensated FP32 sums by the ray engine).
 #include <hip/hip_runtime.h>
 
 #include <hipcub/hipcub.hpp>

@@ -413,8 +413,11 @@ k_redblack_multi(int X, int Y, int Z, double omega, const double* __restrict__ p
 
 // Interior residual sum |sum_nb - 6 p + rhs| (sor3D.cpp:741-781) reduced
 // over the grid; the last block applies the stop rule of sweep / iteration
-// `it`: residue = sum / X / Y / Z, stop when residue < eps.  fp32 rounds the
-// residue to float as the reference's accumulator does.
+// `it`: residue = sum / X / Y / Z, stop when residue < eps.  The sum is FP64
+// in a fixed tree order; fp32 rounds it to float once, where the reference
+// adds each |temp| into a float (sor3D.cpp:774): the residues can differ in
+// their last float bits, which moves the stop sweep only when the residue
+// ties with eps.
 constexpr int kResBlock = 256;
 constexpr int kResPlanes = 4;
 
@@ -454,8 +457,8 @@ k_residual(int X, int Y, int Z, const double* __restrict__ p, const double* __re
   }
 }
 
-// One workgroup: residue = sum / X / Y / Z (fp32: rounded to float as the
-// reference's accumulator is), stop when residue < eps.
+// One workgroup: residue = sum / X / Y / Z (fp32: the FP64 sum rounded to
+// float once, then divided in float), stop when residue < eps.
 __global__ void __launch_bounds__(kResBlock)
 k_residual_finish(int X, int Y, int Z, int64_t nblk, const double* __restrict__ partials, int it,
                   double eps, int fp32, SorState* __restrict__ st) {
