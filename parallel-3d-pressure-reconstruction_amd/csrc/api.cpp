@@ -511,7 +511,7 @@ int build_plan(omni3d_ctx* ctx) {
         int dev = 0, ncu = 256;
         if (hipGetDevice(&dev) == hipSuccess)
           (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        ncu = std::max(ncu, 1);
+        ncu = ctx->groups_fill_cus ? std::max(ncu, 1) : 1;
         std::vector<int> hgrow, hrg((size_t)nS);
         balance_groups(hrp, 256, ncu, &hgrow, &hrg);
         pl.dense_ngroups = (int)hgrow.size() - 1;

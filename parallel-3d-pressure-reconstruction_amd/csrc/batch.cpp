@@ -264,6 +264,9 @@ int omni3d_batch_create(const omni3d_params* base, int count, const int* nx, con
     // the host (a synchronisation per field): a captured graph replays the
     // reference schedule instead
     c->affine_masked = false;
+    // the snapshots' iterations share one launch: no group count rounded up
+    // to fill the CUs per snapshot
+    c->groups_fill_cus = false;
     b->snap.push_back(c);
   }
   for (int i = 0; i < count; i++) b->out.emplace_back(new DevBuf());

@@ -145,6 +145,10 @@ struct omni3d_ctx {
   int iter_waves = 2;                        // wavefronts per 64-row chunk of the iteration
   int dense_variant = 0;                     // dense iteration: wavefronts x round size
   int dense_group = 4;                       // dense iteration: chunks per row group
+  // the row groups of the grouped iteration are as many as fill the CUs
+  // evenly (a lone reconstruction); a batch, whose snapshots share one
+  // launch per iteration, keeps the uniform count
+  bool groups_fill_cus = true;
   bool capturing = false;                    // work is being captured into a graph
   int affine_phase = 0;                      // 1: up to the source iteration, 2: after it
   // graph of the source-iteration launches, reused while its arguments stay
