@@ -111,10 +111,7 @@ int build_transposed_rows(omni3d_ctx* ctx) {
 // Sources and transposed rows of the plain-mask rule for the current mask:
 // S = the run-start voxels (numbered in voxel order), row s' = the run-start
 // voxels that runs starting at s' visit (sorted).  No self visits: diag = 0.
-struct MaskRows {
-  int nS = 0;
-  DevBuf sid, slist, tptr, tent, diag;
-};
+using MaskRows = omni3d_ctx::AdjMask;
 
 int build_mask_rows(omni3d_ctx* ctx, const double* mask, MaskRows* mr) {
   const Plan& pl = *ctx->plan;
@@ -188,8 +185,10 @@ int omni3d_forward(omni3d_ctx* ctx, const void* grad_d, void* p_d) {
   // the gradient becomes the context's field, integrated with density -1
   // (-rho * a = g); the mask set before stays
   const bool had_mask = ctx->have_mask;
+  const uint64_t epoch = ctx->mask_epoch;
   rc = omni3d_set_field(ctx, g, g + esz * V, g + 2 * esz * V, nullptr, dt, OMNI3D_MEM_DEVICE);
   ctx->have_mask = had_mask;
+  ctx->mask_epoch = epoch;
   if (rc) return rc;
   const omni3d_params saved = ctx->prm;
   ctx->prm.density = -1.0;
@@ -228,14 +227,24 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
       OMNI_TRY_HIP(ctx, launch_fill_f64(ctx->aux.as<double>(), V, 1.0, st));
     mask = ctx->aux.as<double>();
   }
-  // visit counts C of the method
+  // visit counts C of the method (the plain-mask rule's: built once per
+  // mask and plan, with its run-start rows below)
   const unsigned int* cnt = pl.cnt.as<unsigned int>();
+  MaskRows& mr = ctx->adj_mask;
   if (mask_rule) {
-    OMNI_TRY_HIP(ctx, ctx->wcnt.alloc(sizeof(unsigned int) * V));
-    OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->wcnt.as<void>(), V, 0u, st));
-    OMNI_TRY_HIP(ctx, launch_adj_count(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(), pl.g,
-                                       mask, ctx->wcnt.as<unsigned int>(), st));
-    cnt = ctx->wcnt.as<unsigned int>();
+    if (mr.mask_epoch != ctx->mask_epoch || mr.plan_epoch != pl.epoch) {
+      mr.counts = mr.rows = false;
+      mr.mask_epoch = ctx->mask_epoch;
+      mr.plan_epoch = pl.epoch;
+    }
+    if (!mr.counts) {
+      OMNI_TRY_HIP(ctx, mr.cnt.alloc(sizeof(unsigned int) * V));
+      OMNI_TRY_HIP(ctx, launch_fill_u32(mr.cnt.as<void>(), V, 0u, st));
+      OMNI_TRY_HIP(ctx, launch_adj_count(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
+                                         pl.g, mask, mr.cnt.as<unsigned int>(), st));
+      mr.counts = true;
+    }
+    cnt = mr.cnt.as<unsigned int>();
   }
   for (DevBuf* b : {&ctx->adj_y, &ctx->adj_z, &ctx->adj_u, &ctx->adj_t})
     OMNI_TRY_HIP(ctx, b->alloc(sizeof(double) * V));
@@ -274,7 +283,6 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
     return e;
   };
   // z = sum_{j<K} (A^T)^j y
-  MaskRows mr;
   if (P.solver == OMNI3D_SOLVER_AFFINE) {
     // after the first gather only the source voxels (ray entries; for the
     // plain-mask rule the run starts) carry values, so the Horner steps run
@@ -286,8 +294,11 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
     const unsigned int* diagp = pl.diag.as<unsigned int>();
     if (P.max_iterations > 1) {
       if (mask_rule) {
-        rc = build_mask_rows(ctx, mask, &mr);
-        if (rc) return rc;
+        if (!mr.rows) {
+          rc = build_mask_rows(ctx, mask, &mr);
+          if (rc) return rc;
+          mr.rows = true;
+        }
         nS = mr.nS;
         slist = mr.slist.as<int>();
         tptr = mr.tptr.as<unsigned long long>();

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -587,6 +588,8 @@ int build_plan(omni3d_ctx* ctx) {
   pl.nnz = (int64_t)nnz;
   pl.cmax = mxh[0];
   pl.lmax = mxh[1];
+  static std::atomic<uint64_t> builds{0};
+  pl.epoch = ++builds;
   pl.valid = true;
   ctx->dirs_dirty = false;
   return OMNI3D_OK;
@@ -1981,6 +1984,7 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
   }
   if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   ctx->have_field = true;
+  if (mask || ctx->have_mask) ctx->mask_epoch++;
   ctx->have_mask = mask != nullptr;
   return OMNI3D_OK;
 }

@@ -61,6 +61,7 @@ class DevBuf {
 // and the line spacing (not on the field), reused across snapshots.
 struct Plan {
   bool valid = false;
+  uint64_t epoch = 0;  // distinct per build: what a cache keyed on the plan compares
   int X = 0, Y = 0, Z = 0, D = 0;
   double ls = 0;
   GridDesc g{};
@@ -190,6 +191,17 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_s;
   // per-segment sums of the tile engine's transposed traversals
   omni3d::DevBuf adj_seg;
+  // the plain-mask backward's per-mask structures, reused while the mask
+  // (mask_epoch: bumped when set_field sets or removes a mask) and the plan
+  // are unchanged: visit counts of the rule, and the run starts with their
+  // transposed rows (sorted run-start ids per row, no self visits)
+  uint64_t mask_epoch = 0;
+  struct AdjMask {
+    bool counts = false, rows = false;
+    uint64_t mask_epoch = 0, plan_epoch = 0;
+    int nS = 0;
+    omni3d::DevBuf cnt, sid, slist, tptr, tent, diag;
+  } adj_mask;
   // the transposed iteration on merged words: its scalars, partial sums and
   // fixed-point source term
   omni3d::DevBuf adj_sc, adj_part, adj_bs;

@@ -165,6 +165,42 @@ def test_source_restricted_backward(gpu_available, ref_dirs3, method, K):
     assert again < 1e-13
 
 
+def test_masked_backward_cache_follows_the_mask(gpu_available, ref_dirs3):
+    """The plain-mask backward keeps its per-mask visit counts and run-start
+    rows between calls: a forward in between keeps them, a new mask set with
+    set_field rebuilds them (the result equals a fresh context's)."""
+    import torch
+    n = 40
+    print("case adjoint plain-mask cache", flush=True)
+    f = F.fourier_sphere(n, seed=4)
+    shape = (n, n, n)
+    m1 = f["mask"]
+    m2 = m1.copy()
+    m2[: n // 3] = 1.0
+    m2[:, : n // 4] = 0.0
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    y = torch.randn(n ** 3, dtype=torch.float64, device="cuda", generator=gen)
+    g = torch.randn(3 * n ** 3, dtype=torch.float64, device="cuda", generator=gen)
+    A = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN_MASK, 6, mask=m1, solver=omni3d.AFFINE)
+    b1 = A.backward(y)
+    A.forward(g)
+    b1b = A.backward(y)
+    assert float(torch.linalg.norm(b1b - b1) / torch.linalg.norm(b1)) < 1e-13
+    A.ctx.set_field([np.zeros(shape)] * 3, mask=m2)
+    b2 = A.backward(y)
+    fresh = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN_MASK, 6, mask=m2,
+                    solver=omni3d.AFFINE)
+    ref = fresh.backward(y)
+    assert float(torch.linalg.norm(b2 - ref) / torch.linalg.norm(ref)) < 1e-13
+    assert float(torch.linalg.norm(b2 - b1) / torch.linalg.norm(b1)) > 1e-3
+    # the mask removed: the rule's counts and rows follow (all-ones mask)
+    A.ctx.set_field([np.zeros(shape)] * 3)
+    ones = Adjoint(shape, ref_dirs3, f["delta"], omni3d.PLAIN_MASK, 6, mask=np.ones(shape),
+                   solver=omni3d.AFFINE)
+    b3, ref3 = A.backward(y), ones.backward(y)
+    assert float(torch.linalg.norm(b3 - ref3) / torch.linalg.norm(ref3)) < 1e-13
+
+
 def test_finite_differences_plain_affine(gpu_available, ref_dirs3):
     import torch
     n = 32
