@@ -32,6 +32,46 @@ int io_dtype(const omni3d_ctx* ctx) {
   return ctx->prm.precision == OMNI3D_FP64 ? OMNI3D_F64 : OMNI3D_F32;
 }
 
+// Merged words of sorted CSR rows (groups of 4 chunks: the forward
+// iteration's layout) when they fit it -- sources below 2^22, rows shorter
+// than 4096 entries (int64 row sums); *dense tells whether they were built.
+int build_row_words(omni3d_ctx* ctx, int nS, unsigned long long n, const DevBuf& tptr,
+                    const DevBuf& key, DevBuf* tdent, DevBuf* twptr, bool* dense) {
+  hipStream_t st = ctx->stream;
+  *dense = false;
+  std::vector<unsigned long long> hp((size_t)nS + 1);
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hp.data(), tptr.as<void>(),
+                                   sizeof(unsigned long long) * (nS + 1), hipMemcpyDeviceToHost,
+                                   st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  unsigned long long mx = 0;
+  for (int r = 0; r < nS; r++) mx = std::max(mx, hp[r + 1] - hp[r]);
+  if (!(nS < (1 << 22) && mx < 4096 && n > 0 && n < (1ull << 31))) return OMNI3D_OK;
+  DevBuf ka, kb, tmp, flag, excl;
+  OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * n));
+  OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * n));
+  OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)n)));
+  OMNI_TRY_HIP(ctx, tdent->alloc(sizeof(unsigned int) * (n + 8)));
+  OMNI_TRY_HIP(ctx, launch_fill_u32(tdent->as<void>(), (int64_t)n + 8, 0u, st));
+  OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)n, 4, tptr.as<unsigned long long>(),
+                                        key.as<unsigned long long>(), ka.as<unsigned long long>(),
+                                        kb.as<unsigned long long>(), tmp.as<void>(), tmp.size(),
+                                        tdent->as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * n));
+  OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * n));
+  const int ng = (nS + 255) / 256;
+  OMNI_TRY_HIP(ctx, twptr->alloc(sizeof(unsigned int) * (ng + 1)));
+  unsigned int nw = 0;
+  OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)n, 4, tptr.as<unsigned long long>(),
+                                        kb.as<unsigned long long>(), flag.as<unsigned int>(),
+                                        excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
+                                        tdent->as<unsigned int>(), twptr->as<unsigned int>(), &nw,
+                                        st));
+  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  *dense = true;
+  return OMNI3D_OK;
+}
+
 // Transposed source rows of the plain method, kept with the plan: row s'
 // lists the source voxels that rays entering at s' visit after their entry
 // (the entries of the forward source rows, regrouped by their source).
@@ -64,41 +104,10 @@ int build_transposed_rows(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.tptr.as<unsigned long long>(),
                                      key.as<unsigned long long>(), st));
   {
-    // merged words in groups of 4 chunks, as the forward iteration's rows
-    // (sources below 2^22, rows shorter than 4096 entries: int64 row sums)
-    std::vector<unsigned long long> hp((size_t)nS + 1);
-    OMNI_TRY_HIP(ctx, hipMemcpyAsync(hp.data(), pl.tptr.as<void>(),
-                                     sizeof(unsigned long long) * (nS + 1), hipMemcpyDeviceToHost,
-                                     st));
-    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-    unsigned long long mx = 0;
-    for (int r = 0; r < nS; r++) mx = std::max(mx, hp[r + 1] - hp[r]);
-    pl.tdense = false;
-    if (nS < (1 << 22) && mx < 4096 && n > 0 && n < (1ull << 31)) {
-      DevBuf ka, kb, tmp, flag, excl;
-      OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * n));
-      OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * n));
-      OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)n)));
-      OMNI_TRY_HIP(ctx, pl.tdent.alloc(sizeof(unsigned int) * (n + 8)));
-      OMNI_TRY_HIP(ctx, launch_fill_u32(pl.tdent.as<void>(), (int64_t)n + 8, 0u, st));
-      OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)n, 4, pl.tptr.as<unsigned long long>(),
-                                            key.as<unsigned long long>(),
-                                            ka.as<unsigned long long>(), kb.as<unsigned long long>(),
-                                            tmp.as<void>(), tmp.size(), pl.tdent.as<unsigned int>(),
-                                            st));
-      OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * n));
-      OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * n));
-      const int ng = (nS + 255) / 256;
-      OMNI_TRY_HIP(ctx, pl.twptr.alloc(sizeof(unsigned int) * (ng + 1)));
-      unsigned int nw = 0;
-      OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)n, 4, pl.tptr.as<unsigned long long>(),
-                                            kb.as<unsigned long long>(), flag.as<unsigned int>(),
-                                            excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
-                                            pl.tdent.as<unsigned int>(),
-                                            pl.twptr.as<unsigned int>(), &nw, st));
-      OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-      pl.tdense = true;
-    }
+    bool dense = false;
+    const int rc = build_row_words(ctx, nS, n, pl.tptr, key, &pl.tdent, &pl.twptr, &dense);
+    if (rc) return rc;
+    pl.tdense = dense;
   }
   OMNI_TRY_HIP(ctx, pl.tent.alloc(sizeof(unsigned int) * (n + 1)));
   OMNI_TRY_HIP(ctx, launch_adj_narrow(key.as<unsigned long long>(), (int64_t)n,
@@ -259,18 +268,42 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   else
     OMNI_TRY_HIP(ctx, (launch_convert<float, double>(static_cast<const float*>(dldp_d), y, V, st)));
   OMNI_TRY_HIP(ctx, (launch_convert<double, double>(y, z, V, st)));
-  // the plain method's transposed traversals run on the tile engine with the
-  // affine solver (the NAIVE solver keeps one lane per ray: the cross-check)
+  // the transposed traversals (plain and plain-mask) run on the tile engine
+  // with the affine solver (the NAIVE solver keeps one lane per ray: the
+  // cross-check)
   omni3d_params pg = P;
   pg.density = -1.0;
   const WalkParams wp = walk_params(pg);
-  const bool tiles = !mask_rule && P.solver == OMNI3D_SOLVER_AFFINE && pl.ntiles > 0;
+  const bool tiles = P.solver == OMNI3D_SOLVER_AFFINE && pl.ntiles > 0;
   if (tiles) OMNI_TRY_HIP(ctx, ctx->adj_seg.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
+  if (tiles && mask_rule)
+    OMNI_TRY_HIP(ctx, ctx->adj_span.alloc((size_t)std::max<int64_t>(pl.nval, 1)));
+  // the plain-mask rule's transposed traversals on the tile engine: the
+  // entering-run sums per segment, the reverse scan per ray, then the tiles'
+  // gather (what 3) or scatter (what 2)
+  auto mask_tiles = [&](const double* uu, int what, double* out) -> hipError_t {
+    hipError_t e = launch_adjm_tiles(0, pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
+                                     pl.segs_slot.as<SegRec>(), nullptr, nullptr, 0, nullptr,
+                                     nullptr, pl.g, pl.tg, wp, uu, mask,
+                                     ctx->adj_seg.as<double>(), ctx->adj_span.as<uint8_t>(),
+                                     nullptr, st);
+    if (e == hipSuccess)
+      e = launch_adjm_tiles(1, pl.ntiles, nullptr, nullptr, nullptr, nullptr, pl.nrays,
+                            pl.nseg.as<unsigned int>(), pl.val_off.as<unsigned long long>(), pl.g,
+                            pl.tg, wp, nullptr, nullptr, ctx->adj_seg.as<double>(),
+                            ctx->adj_span.as<uint8_t>(), nullptr, st);
+    if (e == hipSuccess)
+      e = launch_adjm_tiles(what, pl.ntiles, nullptr, nullptr, pl.tile_ptr.as<unsigned long long>(),
+                            pl.segs.as<SegRec>(), 0, nullptr, nullptr, pl.g, pl.tg, wp, uu, mask,
+                            ctx->adj_seg.as<double>(), nullptr, out, st);
+    return e;
+  };
   // t = the ray sums of u gathered into the rays' sources
   auto gather = [&](const double* uu, double* tt) -> hipError_t {
     if (!tiles)
       return launch_adj_gather(mask_rule, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
                                pl.g, mask, uu, tt, st);
+    if (mask_rule) return mask_tiles(uu, 3, tt);
     hipError_t e = launch_adj_tiles(0, pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
                                     pl.segs_slot.as<SegRec>(), nullptr, nullptr, nullptr, 0,
                                     nullptr, nullptr, pl.g, pl.tg, wp, uu,
@@ -365,7 +398,9 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   }
   // dL/dg = B^T (z / C)
   OMNI_TRY_HIP(ctx, launch_adj_scale(V, z, cnt, u, st));
-  if (tiles) {
+  if (tiles && mask_rule) {
+    OMNI_TRY_HIP(ctx, mask_tiles(u, 2, dg));
+  } else if (tiles) {
     // per-segment sums, per-ray suffix sums, then the tiles' scatter (every
     // voxel of dg stored)
     OMNI_TRY_HIP(ctx, launch_adj_tiles(0, pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),

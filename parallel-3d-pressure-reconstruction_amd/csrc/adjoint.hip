@@ -632,6 +632,244 @@ hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The plain-mask rule's transposed traversals on the tile engine (the rule of
+// SegWalker<true>: a run starts at a valid voxel -- mask != 0 -- that is the
+// ray's entry or follows a masked voxel, and its later valid voxels are its
+// adding visits, up to the next masked one).  A visit v adds iff v and the
+// voxel before it are valid, so a step into an adding visit scatters h_a S
+// with S = the sum of u over the run's adding visits from its target on, and
+// a run's start gathers the sum of u over all of the run's adding visits.
+// Per segment (the voxel before its first visit read from the mask):
+//   k_adjm_seg_u     Upre = the sum of u over the leading adding visits (the
+//                    run the segment enters with), and whether that run
+//                    spans the whole segment;
+//   k_adjm_rev_scan  per ray, T = what the later segments add to the run the
+//                    segment ends in (Upre of the next segment, plus its T if
+//                    the run spans it), in place;
+//   k_adjm_tile      per tile, each segment walked backwards from T: S of
+//                    every adding step into LDS accumulators of the three
+//                    components (kScatter), or each run's total into its
+//                    start (the gather); complete values stored per voxel.
+
+// validity of the voxel before a segment's first visit (none for a ray's
+// first segment): the first voxel minus the step into the tile
+template <int kBits>
+__device__ __forceinline__ bool adjm_prev_valid(const SegRec& sr, int ox, int oy, int oz, int64_t sy,
+                                                int64_t sz, const double* __restrict__ mask) {
+  constexpr int kTileMask = (1 << kBits) - 1;
+  if ((sr.info >> 21) & 1u) return false;
+  const int lv0 = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+  const unsigned int sg = (sr.info >> 18) & 7u;
+  const int ae = (int)((sr.info >> 23) & 3u);
+  const int se = ((sg >> ae) & 1u) ? 1 : -1;
+  const int64_t vf = (int64_t)(ox + (lv0 & kTileMask)) + (int64_t)(oy + ((lv0 >> kBits) & kTileMask)) * sy +
+                     (int64_t)(oz + (lv0 >> (2 * kBits))) * sz;
+  const int64_t vp = vf - (ae == 0 ? (int64_t)se : (ae == 1 ? se * sy : se * sz));
+  return mask[vp] != 0.0;
+}
+
+template <int kBits>
+__global__ void __launch_bounds__(1024)
+k_adjm_seg_u(const unsigned long long* __restrict__ tile_ptr1, const SegRec* __restrict__ segs,
+             GridDesc g, TileGrid tg, const double* __restrict__ u,
+             const double* __restrict__ mask, double* __restrict__ useg,
+             uint8_t* __restrict__ span) {
+  constexpr int kTile = 1 << kBits, kTileMask = kTile - 1, kVox = kTile * kTile * kTile;
+  __shared__ double U[kVox];
+  __shared__ uint8_t M[kVox];
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    const bool in = i < g.X && j < g.Y && k < g.Z;
+    const int64_t v = i + j * sy + k * sz;
+    U[lds_slot<kBits>(l)] = in ? u[v] : 0.0;
+    M[lds_slot<kBits>(l)] = in && mask[v] != 0.0 ? 1 : 0;
+  }
+  __syncthreads();
+  const unsigned long long b = tile_ptr1[tile], e = tile_ptr1[tile + 1];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    int lv = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const int d0 = (sg & 1u) ? 1 : -1, d1 = (sg & 2u) ? kTile : -kTile;
+    const int d2 = (sg & 4u) ? kTile * kTile : -kTile * kTile;
+    // the entering run continues through a visit while the visits are valid
+    bool alive = M[lds_slot<kBits>(lv)] && adjm_prev_valid<kBits>(sr, ox, oy, oz, sy, sz, mask);
+    double pre = alive ? U[lds_slot<kBits>(lv)] : 0.0;
+    for (unsigned int q = 1; q < n && alive; q++) {
+      const unsigned int c = q - 1;
+      const uint32_t w = c < 16 ? sr.code[0] : (c < 32 ? sr.code[1] : sr.code[2]);
+      const int a = (int)((w >> (2 * (c & 15))) & 3u);
+      lv += a == 0 ? d0 : (a == 1 ? d1 : d2);
+      const int sl = lds_slot<kBits>(lv);
+      if (M[sl]) pre += U[sl];
+      else alive = false;
+    }
+    useg[sr.g] = pre;
+    span[sr.g] = alive ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_adjm_rev_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
+                const unsigned long long* __restrict__ val_off, double* __restrict__ useg,
+                const uint8_t* __restrict__ span) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  const unsigned long long b = val_off[r >> 6] + (unsigned long long)(r & 63);
+  const unsigned int n = nseg[r];
+  double t = 0.0;
+  for (unsigned int j = n; j-- > 0;) {
+    const unsigned long long o = b + 64ull * j;
+    const double x = useg[o];
+    const bool whole = span[o] != 0;
+    useg[o] = t;
+    t = x + (whole ? t : 0.0);
+  }
+}
+
+template <int kBits, bool kScatter>
+__global__ void __launch_bounds__(1024)
+k_adjm_tile(const unsigned long long* __restrict__ tile_ptr, const SegRec* __restrict__ segs,
+            GridDesc g, TileGrid tg, WalkParams wp, const double* __restrict__ u,
+            const double* __restrict__ mask, const double* __restrict__ tafter,
+            double* __restrict__ out) {
+  constexpr int kTile = 1 << kBits, kTileMask = kTile - 1, kVox = kTile * kTile * kTile;
+  __shared__ double U[kVox];
+  __shared__ uint8_t M[kVox];
+  __shared__ double A[kScatter ? 3 : 1][kVox];  // dg components, or the gathered totals
+  const int tile = (int)xcd_block();
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    const bool in = i < g.X && j < g.Y && k < g.Z;
+    const int64_t v = i + j * sy + k * sz;
+    const int sl = lds_slot<kBits>(l);
+    U[sl] = in ? u[v] : 0.0;
+    M[sl] = in && mask[v] != 0.0 ? 1 : 0;
+#pragma unroll
+    for (int c = 0; c < (kScatter ? 3 : 1); c++) A[c][sl] = 0.0;
+  }
+  __syncthreads();
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + threadIdx.x; jj < e; jj += blockDim.x) {
+    const SegRec sr = segs[jj];
+    const int lv0 = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+    const unsigned int n = (sr.info >> kSegLvBits) & 63u;
+    const unsigned int sg = (sr.info >> 18) & 7u;
+    const bool last = (sr.info >> 22) & 1u;
+    const bool up[3] = {(sg & 1u) != 0, (sg & 2u) != 0, (sg & 4u) != 0};
+    const int d0 = up[0] ? 1 : -1, d1 = up[1] ? kTile : -kTile;
+    const int d2 = up[2] ? kTile * kTile : -kTile * kTile;
+    auto code = [&](unsigned int c) {
+      const uint32_t w = c < 16 ? sr.code[0] : (c < 32 ? sr.code[1] : sr.code[2]);
+      return (int)((w >> (2 * (c & 15))) & 3u);
+    };
+    auto delta = [&](int a) { return a == 0 ? d0 : (a == 1 ? d1 : d2); };
+    auto hfac = [&](int a) {
+      return a == 0 ? step_factor<double>(wp, 0, up[0])
+                    : (a == 1 ? step_factor<double>(wp, 1, up[1]) : step_factor<double>(wp, 2, up[2]));
+    };
+    // forward: which visits add and which start a run (at most 46 visits)
+    unsigned long long addb = 0ull, startb = 0ull;
+    bool prev = adjm_prev_valid<kBits>(sr, ox, oy, oz, sy, sz, mask);
+    int lv = lv0;
+    for (unsigned int q = 0; q < n; q++) {
+      if (q > 0) lv += delta(code(q - 1));
+      const bool val = M[lds_slot<kBits>(lv)] != 0;
+      if (val && prev) addb |= 1ull << q;
+      if (val && !prev) startb |= 1ull << q;
+      prev = val;
+    }
+    // backward from what the later segments add to the run the segment ends in
+    const double T = tafter[sr.g];
+    double S = T;
+    if (kScatter && !last) {
+      // the step out of the tile (into an adding visit whenever T != 0)
+      const int ax = code(n - 1);
+      atomicAdd(&A[ax][lds_slot<kBits>(lv)], hfac(ax) * T);
+    }
+    for (unsigned int q = n; q-- > 0;) {
+      const int sl = lds_slot<kBits>(lv);
+      const int a = q > 0 ? code(q - 1) : (int)((sr.info >> 23) & 3u);
+      const int lp = q > 0 ? lv - delta(a) : lv;
+      if ((addb >> q) & 1ull) {
+        S += U[sl];
+        if constexpr (kScatter) {
+          // the step into visit q (for q = 0 the step into the tile; its
+          // other end is counted by the segment it leaves)
+          const double hS = hfac(a) * S;
+          atomicAdd(&A[a][sl], hS);
+          if (q > 0) atomicAdd(&A[a][lds_slot<kBits>(lp)], hS);
+        }
+      }
+      if ((startb >> q) & 1ull) {
+        if constexpr (!kScatter) atomicAdd(&A[0][sl], S);
+        S = 0.0;
+      }
+      lv = lp;
+    }
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < kVox; l += blockDim.x) {
+    const int i = ox + (l & kTileMask), j = oy + ((l >> kBits) & kTileMask), k = oz + (l >> (2 * kBits));
+    if (i >= g.X || j >= g.Y || k >= g.Z) continue;
+    const int64_t v = i + j * sy + k * sz;
+    const int sl = lds_slot<kBits>(l);
+    if constexpr (kScatter) {
+      out[v] = A[0][sl];
+      out[V + v] = A[1][sl];
+      out[2 * V + v] = A[2][sl];
+    } else {
+      out[v] = A[0][sl];
+    }
+  }
+}
+
+hipError_t launch_adjm_tiles(int what, int ntiles, const unsigned long long* tile_ptr1,
+                             const SegRec* segs_slot, const unsigned long long* tile_ptr,
+                             const SegRec* segs, int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, GridDesc g, TileGrid tg,
+                             const WalkParams& wp, const double* u, const double* mask,
+                             double* useg, uint8_t* span, double* out, hipStream_t st) {
+  // what 0: segment sums of the entering runs; 1: reverse scan; 2: the
+  // scatter into out (dg, 3 V); 3: the gather into out (V)
+  if (what == 1) {
+    hipLaunchKernelGGL(k_adjm_rev_scan, dim3((unsigned int)((nrays + 255) / 256)), dim3(256), 0, st,
+                       nrays, nseg, val_off, useg, span);
+    return hipGetLastError();
+  }
+  if (what == 0) {
+    if (tg.bits == 3)
+      hipLaunchKernelGGL(k_adjm_seg_u<3>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs_slot, g,
+                         tg, u, mask, useg, span);
+    else
+      hipLaunchKernelGGL(k_adjm_seg_u<4>, dim3(ntiles), dim3(1024), 0, st, tile_ptr1, segs_slot, g,
+                         tg, u, mask, useg, span);
+    return hipGetLastError();
+  }
+#define OMNI_AM(B, SC)                                                                       \
+  hipLaunchKernelGGL((k_adjm_tile<B, SC>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, tg, \
+                     wp, u, mask, useg, out)
+  if (tg.bits == 3) {
+    if (what == 2) OMNI_AM(3, true);
+    else OMNI_AM(3, false);
+  } else {
+    if (what == 2) OMNI_AM(4, true);
+    else OMNI_AM(4, false);
+  }
+#undef OMNI_AM
+  return hipGetLastError();
+}
+
 hipError_t launch_adj_gather(bool mask_rule, const RayRec* rays, int64_t nrays,
                              const DirConst* dirs, GridDesc g, const double* mask,
                              const double* u, double* t, hipStream_t st) {

@@ -191,6 +191,7 @@ struct omni3d_ctx {
   omni3d::DevBuf adj_s;
   // per-segment sums of the tile engine's transposed traversals
   omni3d::DevBuf adj_seg;
+  omni3d::DevBuf adj_span;  // plain-mask: per segment, whether its entering run spans it
   // the plain-mask backward's per-mask structures, reused while the mask
   // (mask_epoch: bumped when set_field sets or removes a mask) and the plan
   // are unchanged: visit counts of the rule, and the run starts with their

@@ -43,81 +43,11 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "rule.hpp"
 #include "tiles.hpp"
 #include "walk.hpp"
 
 namespace omni3d {
-
-// method ids as in k_walk_naive: 0 plain (bodyIntegralInner: every step
-// adds, the entry adds its own p, weights = the plan's visit counts),
-// 1 plain-mask (unit weights), 2 Select, 3 Weighted
-enum { kMaskPlain = 0, kMaskUnit = 1, kMaskSelect = 2, kMaskWeighted = 3 };
-// integer (fixed-point) sums for whole-count weights, FP64 for curl weights
-template <int kMethod>
-constexpr bool kCurlW = kMethod == kMaskSelect || kMethod == kMaskWeighted;
-
-// flags of a segment's first visit
-constexpr uint32_t kModeNone = 0u;     // no run: nothing carried
-constexpr uint32_t kModeCont = 1u;     // continues the run: carry + E
-constexpr uint32_t kModeRestart = 2u;  // (re)starts at its first voxel: p there
-constexpr uint32_t kFlagAdd = 4u;      // the first visit adds
-constexpr uint32_t kFlagP0 = 8u;       // run started (p0set) after the first visit
-constexpr uint32_t kFlagOut = 16u;     // inside a high / masked stretch (outthr)
-constexpr uint32_t kFlagRestart = 32u; // some visit of the segment restarts the run
-
-// The rule of one step onto a voxel with curl / mask value cn along axis a
-// (naive_select: the reference's asymmetries when faithful).  add: the visit
-// adds p0 + pint (the step's term joins pint first); restart: p0 = p here,
-// pint = 0, and the curl weights restart unless keep_w.
-struct StepRule {
-  bool add, restart, keep_w;
-};
-
-template <int kMethod>
-__device__ __forceinline__ StepRule rule_step(const WalkParams& wp, int a, double cn, bool& p0set,
-                                              bool& outthr) {
-  StepRule r{false, false, false};
-  if (kMethod == kMaskWeighted || kMethod == kMaskPlain) {
-    r.add = true;  // every step adds (Weighted: when the curl sum is nonzero)
-    return r;
-  }
-  const double thr = wp.thr;
-  const bool faithful = wp.faithful != 0;
-  const bool valid = cn < thr && cn != 0.0;
-  if (!p0set) {
-    const bool ok = (faithful && a != 0) ? (cn != 0.0) : valid;
-    if (ok) {
-      p0set = true;
-      r.restart = true;
-    }
-  } else {
-    r.add = !outthr && valid;
-    const bool enter_high = (faithful && a != 0) ? (!outthr && cn >= thr)
-                                                 : ((!outthr && cn >= thr) || cn == 0.0);
-    if (enter_high) outthr = true;
-    if (outthr && valid) {
-      r.restart = true;
-      outthr = false;
-      r.keep_w = faithful && a == 2;
-    }
-  }
-  return r;
-}
-
-// The entry voxel of a ray: a run starts there (p0 = p, no add) when valid.
-template <int kMethod>
-__device__ __forceinline__ bool rule_entry(const WalkParams& wp, double ci) {
-  if (kMethod == kMaskWeighted || kMethod == kMaskPlain) return true;
-  const double thr = wp.thr;
-  return wp.faithful ? !(ci == 0.0 || ci > thr) : (ci != 0.0 && ci < thr);
-}
-
-// The visit's weight (curl methods): 1 / curltmp * counttmp as the
-// reference writes it, and the visit's value times it.
-__device__ __forceinline__ double curl_weight(double curl, double cnt) { return 1 / curl * cnt; }
-__device__ __forceinline__ double curl_value(double c, double curl, double cnt) {
-  return c * 1 / curl * cnt;
-}
 
 // Per-segment record of a field (value-slot indexed); S comes from the
 // setup tile pass, the rest from k_mask_walk.
@@ -682,6 +612,74 @@ hipError_t launch_tile_pass_masked(int method, int ntiles, const unsigned long l
   return hipGetLastError();
 }
 
+// The rule replay on the tile engine alone (k_rule_tile + k_rule_scan): the
+// records E, R, F (and the curl weights wc, wn) of every segment.
+template <typename T>
+hipError_t launch_rule_replay(int method, int64_t nrays, const unsigned int* nseg,
+                              const unsigned long long* val_off, int64_t nval, int ntiles,
+                              const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                              TileGrid tg, const WalkParams& wp, const void* field,
+                              const double* aux, double* E, uint32_t* R, uint32_t* F, double* wc,
+                              double* wn, void* rule, hipStream_t st) {
+  using V4 = typename Vec4<T>::type;
+  const V4* f = static_cast<const V4*>(field);
+  const MaskSegs ms{E, R, F, wc, wn};
+  const int planes = (method == kMaskUnit || method == kMaskSelect) ? kRuleStates : 1;
+  const bool curl = method == kMaskSelect || method == kMaskWeighted;
+  const uint64_t nv = (uint64_t)std::max<int64_t>(nval, 1);
+  char* base = static_cast<char*>(rule);
+  RuleSegs rs;
+  rs.E = E;
+  rs.x = wc;
+  rs.cs = curl ? reinterpret_cast<double*>(base) : nullptr;
+  if (curl) base += sizeof(double) * planes * nv;
+  rs.f = reinterpret_cast<uint32_t*>(base);
+  rs.r = rs.f + planes * nv;
+  rs.nval = nv;
+#define OMNI_RT(M, B)                                                                         \
+  hipLaunchKernelGGL((k_rule_tile<T, M, B>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, \
+                     tg, wp, f, aux, rs)
+#define OMNI_RT_M(B)                                  \
+  switch (method) {                                   \
+    case kMaskPlain: OMNI_RT(kMaskPlain, B); break;   \
+    case kMaskUnit: OMNI_RT(kMaskUnit, B); break;     \
+    case kMaskSelect: OMNI_RT(kMaskSelect, B); break; \
+    default: OMNI_RT(kMaskWeighted, B); break;        \
+  }
+  if (tg.bits == 3) {
+    OMNI_RT_M(3)
+  } else {
+    OMNI_RT_M(4)
+  }
+#undef OMNI_RT_M
+#undef OMNI_RT
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned int sgrid = grid_for(nrays, 1u << 30);
+#define OMNI_RS(M) \
+  hipLaunchKernelGGL((k_rule_scan<M>), dim3(sgrid), dim3(256), 0, st, nrays, nseg, val_off, rs, ms)
+  switch (method) {
+    case kMaskPlain: OMNI_RS(kMaskPlain); break;
+    case kMaskUnit: OMNI_RS(kMaskUnit); break;
+    case kMaskSelect: OMNI_RS(kMaskSelect); break;
+    default: OMNI_RS(kMaskWeighted); break;
+  }
+#undef OMNI_RS
+  return hipGetLastError();
+}
+template hipError_t launch_rule_replay<float>(int, int64_t, const unsigned int*,
+                                              const unsigned long long*, int64_t, int,
+                                              const unsigned long long*, const SegRec*, GridDesc,
+                                              TileGrid, const WalkParams&, const void*,
+                                              const double*, double*, uint32_t*, uint32_t*,
+                                              double*, double*, void*, hipStream_t);
+template hipError_t launch_rule_replay<double>(int, int64_t, const unsigned int*,
+                                               const unsigned long long*, int64_t, int,
+                                               const unsigned long long*, const SegRec*, GridDesc,
+                                               TileGrid, const WalkParams&, const void*,
+                                               const double*, double*, uint32_t*, uint32_t*,
+                                               double*, double*, void*, hipStream_t);
+
 template <typename T>
 hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const DirConst* dirs,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const void* field,
@@ -694,50 +692,13 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   const MaskSegs ms{E, R, F, wc, wn};
+  hipError_t e;
   if (rule) {
     // the rule replay on the tile engine: per segment and entry state, then
     // per ray
-    const int planes = (method == kMaskUnit || method == kMaskSelect) ? kRuleStates : 1;
-    const bool curl = method == kMaskSelect || method == kMaskWeighted;
-    const uint64_t nv = (uint64_t)std::max<int64_t>(nval, 1);
-    char* base = static_cast<char*>(rule);
-    RuleSegs rs;
-    rs.E = E;
-    rs.x = wc;
-    rs.cs = curl ? reinterpret_cast<double*>(base) : nullptr;
-    if (curl) base += sizeof(double) * planes * nv;
-    rs.f = reinterpret_cast<uint32_t*>(base);
-    rs.r = rs.f + planes * nv;
-    rs.nval = nv;
-#define OMNI_RT(M, B)                                                                         \
-  hipLaunchKernelGGL((k_rule_tile<T, M, B>), dim3(ntiles), dim3(1024), 0, st, tile_ptr, segs, g, \
-                     tg, wp, f, aux, rs)
-#define OMNI_RT_M(B)                                  \
-  switch (method) {                                   \
-    case kMaskPlain: OMNI_RT(kMaskPlain, B); break;   \
-    case kMaskUnit: OMNI_RT(kMaskUnit, B); break;     \
-    case kMaskSelect: OMNI_RT(kMaskSelect, B); break; \
-    default: OMNI_RT(kMaskWeighted, B); break;        \
-  }
-    if (tg.bits == 3) {
-      OMNI_RT_M(3)
-    } else {
-      OMNI_RT_M(4)
-    }
-#undef OMNI_RT_M
-#undef OMNI_RT
-    hipError_t e = hipGetLastError();
+    e = launch_rule_replay<T>(method, nrays, nseg, val_off, nval, ntiles, tile_ptr, segs, g, tg,
+                              wp, field, aux, E, R, F, wc, wn, rule, st);
     if (e != hipSuccess) return e;
-    const unsigned int sgrid = grid_for(nrays, 1u << 30);
-#define OMNI_RS(M) \
-  hipLaunchKernelGGL((k_rule_scan<M>), dim3(sgrid), dim3(256), 0, st, nrays, nseg, val_off, rs, ms)
-    switch (method) {
-      case kMaskPlain: OMNI_RS(kMaskPlain); break;
-      case kMaskUnit: OMNI_RS(kMaskUnit); break;
-      case kMaskSelect: OMNI_RS(kMaskSelect); break;
-      default: OMNI_RS(kMaskWeighted); break;
-    }
-#undef OMNI_RS
   } else {
     // one lane per ray (the records without the tile engine)
     const unsigned int grid = grid_for(nrays, 1u << 30);
@@ -751,9 +712,9 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
       default: OMNI_MW(kMaskWeighted); break;
     }
 #undef OMNI_MW
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
   // setup tile pass: in-tile sums after the last restart, the weights and
   // the run starts
   return launch_tile_pass_masked<T, true>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux, F,

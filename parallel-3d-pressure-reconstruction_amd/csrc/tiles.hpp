@@ -136,6 +136,14 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
                             double* wc, double* wn, unsigned int* wcnt, double* wsum,
                             uint8_t* src, void* rule, hipStream_t st);
 size_t mask_rule_bytes(int method, int64_t nval);
+// the tile rule replay alone (no setup pass): E, R, F (wc, wn) per segment
+template <typename T>
+hipError_t launch_rule_replay(int method, int64_t nrays, const unsigned int* nseg,
+                              const unsigned long long* val_off, int64_t nval, int ntiles,
+                              const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
+                              TileGrid tg, const WalkParams& wp, const void* field,
+                              const double* aux, double* E, uint32_t* R, uint32_t* F, double* wc,
+                              double* wn, void* rule, hipStream_t st);
 // naive.hip: the rows of the affine map of one pass for the walks that leave
 // the plan's staircase (methods 4-7), one lane per ray: entries per run-start
 // row (fill false: counted into cnt) or written (fill true: cnt = zeroed
@@ -174,6 +182,15 @@ hipError_t launch_adj_tiles(int what, int ntiles, const unsigned long long* tile
                             const unsigned int* nseg, const unsigned long long* val_off,
                             GridDesc g, TileGrid tg, const WalkParams& wp, const double* u,
                             double* useg, double* tvol, double* dg, hipStream_t st);
+// the plain-mask rule's transposed traversals on the tile engine (what 0:
+// entering-run sums and span flags per segment; 1: per-ray reverse scan;
+// 2: scatter into out = dg (3 V); 3: gather of run totals into out (V))
+hipError_t launch_adjm_tiles(int what, int ntiles, const unsigned long long* tile_ptr1,
+                             const SegRec* segs_slot, const unsigned long long* tile_ptr,
+                             const SegRec* segs, int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, GridDesc g, TileGrid tg,
+                             const WalkParams& wp, const double* u, const double* mask,
+                             double* useg, uint8_t* span, double* out, hipStream_t st);
 
 
 // visit counts of every voxel from the tile segments (plan build)
