@@ -241,9 +241,12 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
   const unsigned int* cnt = pl.cnt.as<unsigned int>();
   MaskRows& mr = ctx->adj_mask;
   if (mask_rule) {
-    if (mr.mask_epoch != ctx->mask_epoch || mr.plan_epoch != pl.epoch) {
+    uint64_t mepoch = 0;
+    rc = mask_epoch_now(ctx, &mepoch);
+    if (rc) return rc;
+    if (mr.mask_epoch != mepoch || mr.plan_epoch != pl.epoch) {
       mr.counts = mr.rows = false;
-      mr.mask_epoch = ctx->mask_epoch;
+      mr.mask_epoch = mepoch;
       mr.plan_epoch = pl.epoch;
     }
     if (!mr.counts) {

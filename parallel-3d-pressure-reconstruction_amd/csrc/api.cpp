@@ -938,6 +938,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
 // method other than PLAIN.  Writes the result of iteration k* to pdst/wdst.
 template <typename T>
 int run_naive(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype) {
+  ctx->am_cache.valid = false;  // the records and rows it keeps are rewritten here
   const omni3d_params& P = ctx->prm;
   Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
@@ -1175,49 +1176,75 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   OMNI_TRY_HIP(ctx, launch_fill_f64(pz, V, 0.0, st));
   OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->srcf.as<void>(), (V + 3) / 4, 0u, st));
   record_event(ctx, 2);
-  // the rule replay: first-visit records, weights, run starts
-  void* rule = nullptr;
-  if (!mask_walk_lane()) {
-    OMNI_TRY_HIP(ctx, ctx->mk_rule.alloc(mask_rule_bytes(m, pl.nval)));
-    rule = ctx->mk_rule.as<void>();
+  // plain-mask: the runs depend on the mask alone, so the mask-only
+  // structures of the last field (the rule records F and R, the run-start
+  // ids, the run sources, the rows as merged words) serve this one while the
+  // mask and the plan are unchanged: then only E, the setup pass, b, the
+  // iteration and the final pass run
+  omni3d_ctx::AmCache& cache = ctx->am_cache;
+  uint64_t mepoch = 0;
+  if (m == OMNI3D_METHOD_PLAIN_MASK) {
+    const int rc = mask_epoch_now(ctx, &mepoch);
+    if (rc) return rc;
   }
-  OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
-                                        pl.g, pl.tg, wp, field, aux, pl.nseg.as<unsigned int>(),
-                                        pl.val_off.as<unsigned long long>(), pl.nval, pl.ntiles,
-                                        pl.tile_ptr.as<unsigned long long>(),
-                                        pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
-                                        ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
-                                        ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
-                                        ctx->mk_wn.as<double>(), wcnt, wsum,
-                                        ctx->srcf.as<uint8_t>(), rule, st));
+  const bool reuse = m == OMNI3D_METHOD_PLAIN_MASK && cache.valid &&
+                     cache.mask_epoch == mepoch && cache.plan_epoch == pl.epoch;
+  cache.valid = false;
+  if (reuse) {
+    OMNI_TRY_HIP(ctx, launch_mask_reuse<T>(m, pl.ntiles, pl.tile_ptr.as<unsigned long long>(),
+                                           pl.segs.as<SegRec>(), pl.g, pl.tg, wp, field, aux, sc,
+                                           ctx->mk_e.as<double>(), ctx->mk_s.as<double>(),
+                                           ctx->mk_f.as<uint32_t>(), wcnt,
+                                           ctx->srcf.as<uint8_t>(), st));
+  } else {
+    // the rule replay: first-visit records, weights, run starts
+    void* rule = nullptr;
+    if (!mask_walk_lane()) {
+      OMNI_TRY_HIP(ctx, ctx->mk_rule.alloc(mask_rule_bytes(m, pl.nval)));
+      rule = ctx->mk_rule.as<void>();
+    }
+    OMNI_TRY_HIP(ctx, launch_mask_walk<T>(m, pl.rays.as<RayRec>(), pl.nrays,
+                                          pl.dirs.as<DirConst>(), pl.g, pl.tg, wp, field, aux,
+                                          pl.nseg.as<unsigned int>(),
+                                          pl.val_off.as<unsigned long long>(), pl.nval, pl.ntiles,
+                                          pl.tile_ptr.as<unsigned long long>(),
+                                          pl.segs.as<SegRec>(), sc, ctx->mk_e.as<double>(),
+                                          ctx->mk_s.as<double>(), ctx->mk_r.as<uint32_t>(),
+                                          ctx->mk_f.as<uint32_t>(), ctx->mk_wc.as<double>(),
+                                          ctx->mk_wn.as<double>(), wcnt, wsum,
+                                          ctx->srcf.as<uint8_t>(), rule, st));
+  }
   record_event(ctx, 8);
   // S: compact ids of the run starts (the voxels a pass reads p at), in tile
   // order: tile t's rows are [tfirst[t], tfirst[t + 1])
-  OMNI_TRY_HIP(ctx, ctx->am_sid.alloc(sizeof(int) * V));
-  OMNI_TRY_HIP(ctx, ctx->am_slist.alloc(sizeof(int) * (V + 1)));
-  OMNI_TRY_HIP(ctx, ctx->am_tfirst.alloc(sizeof(int) * ((size_t)pl.ntiles + 1)));
-  DevBuf dns, bcnt;
-  OMNI_TRY_HIP(ctx, dns.alloc(sizeof(int)));
-  OMNI_TRY_HIP(ctx, bcnt.alloc(sizeof(unsigned int) * (size_t)source_ids_tiled_blocks(pl.tg)));
-  OMNI_TRY_HIP(ctx, launch_source_ids_tiled(ctx->srcf.as<uint8_t>(), pl.g, pl.tg,
-                                            bcnt.as<unsigned int>(), ctx->am_sid.as<int>(),
-                                            ctx->am_slist.as<int>(), dns.as<int>(),
-                                            ctx->am_tfirst.as<int>(), st));
-  int nS = 0;
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, dns.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
-  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  if (nS <= 0) {
-    *fallback = true;
-    return OMNI3D_OK;
+  int nS = reuse ? cache.nS : 0;
+  if (!reuse) {
+    OMNI_TRY_HIP(ctx, ctx->am_sid.alloc(sizeof(int) * V));
+    OMNI_TRY_HIP(ctx, ctx->am_slist.alloc(sizeof(int) * (V + 1)));
+    OMNI_TRY_HIP(ctx, ctx->am_tfirst.alloc(sizeof(int) * ((size_t)pl.ntiles + 1)));
+    DevBuf dns, bcnt;
+    OMNI_TRY_HIP(ctx, dns.alloc(sizeof(int)));
+    OMNI_TRY_HIP(ctx, bcnt.alloc(sizeof(unsigned int) * (size_t)source_ids_tiled_blocks(pl.tg)));
+    OMNI_TRY_HIP(ctx, launch_source_ids_tiled(ctx->srcf.as<uint8_t>(), pl.g, pl.tg,
+                                              bcnt.as<unsigned int>(), ctx->am_sid.as<int>(),
+                                              ctx->am_slist.as<int>(), dns.as<int>(),
+                                              ctx->am_tfirst.as<int>(), st));
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nS, dns.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+    if (nS <= 0) {
+      *fallback = true;
+      return OMNI3D_OK;
+    }
   }
   const int* sid = ctx->am_sid.as<int>();
   const int* slist = ctx->am_slist.as<int>();
   const int* tfirst = ctx->am_tfirst.as<int>();
   // the run each segment continues
-  OMNI_TRY_HIP(ctx, launch_mask_runsrc(pl.nrays, pl.nseg.as<unsigned int>(),
-                                       pl.val_off.as<unsigned long long>(),
-                                       ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
-                                       ctx->am_run.as<uint32_t>(), st));
+  if (!reuse)
+    OMNI_TRY_HIP(ctx, launch_mask_runsrc(pl.nrays, pl.nseg.as<unsigned int>(),
+                                         pl.val_off.as<unsigned long long>(),
+                                         ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
+                                         ctx->am_run.as<uint32_t>(), st));
   // unit weights (plain-mask): the rows as the merged-word layout of the
   // plain method's source iteration (k_iterate_grp: groups of 256 rows, words
   // sorted by source, equal entries merged), built from CSR rows; otherwise
@@ -1227,7 +1254,10 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   unsigned int nwords = 0;
   int nchunks = 0;
   DevBuf stmp;
-  if (grouped) {
+  if (grouped && reuse) {
+    nnz = cache.nnz;
+    nwords = cache.nwords;
+  } else if (grouped) {
     DevBuf rcnt;
     OMNI_TRY_HIP(ctx, rcnt.alloc(sizeof(unsigned int) * ((size_t)nS + 1)));
     OMNI_TRY_HIP(ctx, ctx->am_ptr.alloc(sizeof(unsigned long long) * ((size_t)nS + 1)));
@@ -1277,6 +1307,13 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                           excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
                                           ctx->am_dent.as<unsigned int>(),
                                           ctx->am_wptr.as<unsigned int>(), &nwords, st));
+    if (m == OMNI3D_METHOD_PLAIN_MASK) {
+      cache.mask_epoch = mepoch;
+      cache.plan_epoch = pl.epoch;
+      cache.nS = nS;
+      cache.nnz = nnz;
+      cache.nwords = nwords;
+    }
   } else {
     // entry counts per tile, their offsets, then the entries
     const int nt = pl.ntiles;
@@ -1456,6 +1493,7 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
   ctx->am_nnz = (int64_t)nnz;
   ctx->am_words = grouped ? (int64_t)nwords : (int64_t)nnz;
   ctx->am_ns = nS;
+  cache.valid = m == OMNI3D_METHOD_PLAIN_MASK && grouped;
   return OMNI3D_OK;
 }
 
@@ -1474,6 +1512,7 @@ template <typename T>
 int run_affine_lane(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype,
                     bool* fallback) {
   *fallback = false;
+  ctx->am_cache.valid = false;  // its rows share the masked schedule's buffers
   const omni3d_params& P = ctx->prm;
   Plan& pl = *ctx->plan;
   if (!pl.valid || pl.ntiles <= 0 || pl.nrays <= 0 || P.max_iterations <= 0) {
@@ -1684,6 +1723,21 @@ int run_affine_lane(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dtype
   ctx->am_nnz = (int64_t)nnz;
   ctx->am_words = (int64_t)nnz;
   ctx->am_ns = nS;
+  return OMNI3D_OK;
+}
+
+// The mask epoch with a pending comparison resolved (one flag read, on the
+// context's stream): caches keyed on the mask compare against this.
+int mask_epoch_now(omni3d_ctx* ctx, uint64_t* epoch) {
+  if (ctx->mask_cmp_pending) {
+    unsigned int diff = 0;
+    OMNI_TRY_HIP(ctx, hipMemcpyAsync(&diff, ctx->mask_diff.as<void>(), sizeof(diff),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    OMNI_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (diff) ctx->mask_epoch++;
+    ctx->mask_cmp_pending = false;
+  }
+  *epoch = ctx->mask_epoch;
   return OMNI3D_OK;
 }
 
@@ -1976,15 +2030,36 @@ int omni3d_set_field(omni3d_ctx* ctx, const void* dudt, const void* dvdt, const 
                                                (const float*)dsrc[2], ctx->field.as<void>(), V, st);
   OMNI_TRY_HIP(ctx, e);
   if (mask) {
-    OMNI_TRY_HIP(ctx, ctx->mask.alloc(sizeof(double) * V));
+    // a mask over a mask of the same grid: converted beside it and compared
+    // on the device, so that structures kept for the old mask stay valid when
+    // the values are the same (a snapshot series with one mask)
+    const bool compare = ctx->have_mask && ctx->mask.size() >= sizeof(double) * (size_t)V;
+    DevBuf& dst = compare ? ctx->mask_next : ctx->mask;
+    OMNI_TRY_HIP(ctx, dst.alloc(sizeof(double) * V));
     e = dtype == OMNI3D_F64
-            ? launch_convert<double, double>((const double*)dsrc[3], ctx->mask.as<double>(), V, st)
-            : launch_convert<float, double>((const float*)dsrc[3], ctx->mask.as<double>(), V, st);
+            ? launch_convert<double, double>((const double*)dsrc[3], dst.as<double>(), V, st)
+            : launch_convert<float, double>((const float*)dsrc[3], dst.as<double>(), V, st);
     OMNI_TRY_HIP(ctx, e);
+    if (compare) {
+      if (!ctx->mask_cmp_pending) {
+        OMNI_TRY_HIP(ctx, ctx->mask_diff.alloc(sizeof(unsigned int)));
+        OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->mask_diff.as<void>(), 1, 0u, st));
+      }
+      // (a comparison still pending accumulates into the same flag)
+      OMNI_TRY_HIP(ctx, launch_mask_diff(ctx->mask.as<double>(), ctx->mask_next.as<double>(), V,
+                                         ctx->mask_diff.as<unsigned int>(), st));
+      // copied, not swapped: graphs captured over the mask keep its address
+      OMNI_TRY_HIP(ctx, (launch_convert<double, double>(ctx->mask_next.as<double>(),
+                                                        ctx->mask.as<double>(), V, st)));
+      ctx->mask_cmp_pending = true;
+    } else {
+      ctx->mask_epoch++;
+    }
+  } else if (ctx->have_mask) {
+    ctx->mask_epoch++;
   }
   if (mem_kind == OMNI3D_MEM_HOST) OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   ctx->have_field = true;
-  if (mask || ctx->have_mask) ctx->mask_epoch++;
   ctx->have_mask = mask != nullptr;
   return OMNI3D_OK;
 }

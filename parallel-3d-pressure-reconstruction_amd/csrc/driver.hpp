@@ -22,6 +22,9 @@ bool params_ok(const omni3d_params& p, std::string* why);
 // Geometry plan of the context's grid, direction set and line spacing
 // (rebuilt only when one of them changed).
 int build_plan(omni3d_ctx* ctx);
+// The context's mask epoch with a pending device comparison of a newly set
+// mask resolved (a flag read): what caches keyed on the mask compare.
+int mask_epoch_now(omni3d_ctx* ctx, uint64_t* epoch);
 WalkParams walk_params(const omni3d_params& P);
 // traversal == OMNI3D_TRAVERSAL_TRILINEAR
 bool trilinear(const omni3d_params& P);

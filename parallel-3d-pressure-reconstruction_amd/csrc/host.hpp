@@ -197,6 +197,11 @@ struct omni3d_ctx {
   // are unchanged: visit counts of the rule, and the run starts with their
   // transposed rows (sorted run-start ids per row, no self visits)
   uint64_t mask_epoch = 0;
+  // a mask set over one of the same size is compared with it on the device
+  // (mask_diff: nonzero if any value differs); the epoch moves only if so,
+  // resolved by mask_epoch_now() when a cache asks
+  bool mask_cmp_pending = false;
+  omni3d::DevBuf mask_next, mask_diff;
   struct AdjMask {
     bool counts = false, rows = false;
     uint64_t mask_epoch = 0, plan_epoch = 0;
@@ -218,6 +223,16 @@ struct omni3d_ctx {
   omni3d::DevBuf am_wmax, am_sc, am_dent, am_wptr, am_bs, am_diag;
   omni3d::DevBuf am_tfirst, am_tbase, am_erow, am_chunks, am_rowsum;
   bool affine_masked = true;  // off for snapshot batches (their graphs replay the reference schedule)
+  // plain-mask on the affine schedule: the mask-only structures (rule
+  // records F and R, run-start ids, run sources, the rows as merged words)
+  // kept for the next field while the mask and the plan are unchanged
+  struct AmCache {
+    bool valid = false;
+    uint64_t mask_epoch = 0, plan_epoch = 0;
+    int nS = 0;
+    unsigned long long nnz = 0;
+    unsigned int nwords = 0;
+  } am_cache;
   int am_ns = -1;             // run starts of the last reconstruction (-1: reference schedule)
   int64_t am_nnz = 0;         // its row entries
   int64_t am_words = 0;       // and the words the iteration reads

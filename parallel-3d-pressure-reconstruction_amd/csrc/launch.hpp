@@ -82,6 +82,9 @@ hipError_t launch_curl_f32(int X, int Y, int Z, double dx, double dy, double dz,
                            hipStream_t st);
 hipError_t launch_fill_u32(void* x, int64_t words, unsigned int v, hipStream_t st);
 hipError_t launch_fill_f64(double* x, int64_t n, double v, hipStream_t st);
+// *diff = 1 when a and b differ in any of their n values (bitwise); else untouched
+hipError_t launch_mask_diff(const double* a, const double* b, int64_t n, unsigned int* diff,
+                            hipStream_t st);
 hipError_t launch_accumulate_out(int64_t V, int fixed, const long long* sumfx,
                                  const unsigned int* cnt, const double* sumd, const double* wsum,
                                  const DevScalars* sc, float* pn, float* pc, hipStream_t st);

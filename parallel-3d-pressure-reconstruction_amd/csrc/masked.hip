@@ -371,6 +371,43 @@ k_rule_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
   }
 }
 
+// The terms of the steps into the tiles of a field, for records F kept from
+// an earlier field with the same mask (plain-mask: the runs, and so F, depend
+// on the mask alone): E = the step's trapezoid where the segment's first
+// visit adds (not for a ray's first segment), else 0 -- what the rule replay
+// writes for E.
+template <typename T, int kBits>
+__global__ void __launch_bounds__(256)
+k_mask_entry_terms(const unsigned long long* __restrict__ tile_ptr,
+                   const SegRec* __restrict__ segs, GridDesc g, TileGrid tg, WalkParams wp,
+                   const typename Vec4<T>::type* __restrict__ field,
+                   const uint32_t* __restrict__ F, double* __restrict__ E) {
+  constexpr int kTileMask = (1 << kBits) - 1;
+  const int tile = (int)blockIdx.y;
+  const int tx = tile % tg.ntx, ty = (tile / tg.ntx) % tg.nty, tz = tile / (tg.ntx * tg.nty);
+  const int ox = tx << kBits, oy = ty << kBits, oz = tz << kBits;
+  const int64_t sy = g.X, sz = (int64_t)g.X * g.Y;
+  const unsigned long long b = tile_ptr[tile * kLenBuckets], e = tile_ptr[(tile + 1) * kLenBuckets];
+  for (unsigned long long jj = b + blockIdx.x * 256ull + threadIdx.x; jj < e;
+       jj += (unsigned long long)gridDim.x * 256ull) {
+    const SegRec sr = segs[jj];
+    const bool first_seg = (sr.info >> 21) & 1u;
+    double x = 0.0;
+    if (!first_seg && (F[sr.g] & kFlagAdd)) {
+      const int lv0 = (int)(sr.info & ((1u << kSegLvBits) - 1u));
+      const unsigned int sg = (sr.info >> 18) & 7u;
+      const int ae = (int)((sr.info >> 23) & 3u);
+      const int se = ((sg >> ae) & 1u) ? 1 : -1;
+      const int64_t vf = (int64_t)(ox + (lv0 & kTileMask)) +
+                         (int64_t)(oy + ((lv0 >> kBits) & kTileMask)) * sy +
+                         (int64_t)(oz + (lv0 >> (2 * kBits))) * sz;
+      const int64_t vp = vf - (ae == 0 ? (int64_t)se : (ae == 1 ? se * sy : se * sz));
+      x = (double)trapezoid<T>(wp, ae, se, component(field[vf], ae), component(field[vp], ae));
+    }
+    E[sr.g] = x;
+  }
+}
+
 // Carry chain of one iteration: the value each segment's first visit
 // continues from (mode "continue": the run's value at the previous
 // segment's exit plus the step into the tile).  A segment that restarts
@@ -730,6 +767,40 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
 OMNI_MW_INST(float)
 OMNI_MW_INST(double)
 #undef OMNI_MW_INST
+
+template <typename T>
+hipError_t launch_mask_reuse(int method, int ntiles, const unsigned long long* tile_ptr,
+                             const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                             const void* field, const double* aux, const DevScalars* sc,
+                             double* E, double* S, const uint32_t* F, unsigned int* wcnt,
+                             uint8_t* src, hipStream_t st) {
+  if (method != kMaskUnit) return hipErrorInvalidValue;
+  using V4 = typename Vec4<T>::type;
+  const V4* f = static_cast<const V4*>(field);
+  // a few workgroups per tile: the tiles' segment lists average hundreds
+  const dim3 grid(4, (unsigned int)ntiles);
+  if (tg.bits == 3)
+    hipLaunchKernelGGL((k_mask_entry_terms<T, 3>), grid, dim3(256), 0, st, tile_ptr, segs, g, tg,
+                       wp, f, F, E);
+  else
+    hipLaunchKernelGGL((k_mask_entry_terms<T, 4>), grid, dim3(256), 0, st, tile_ptr, segs, g, tg,
+                       wp, f, F, E);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_tile_pass_masked<T, true>(method, ntiles, tile_ptr, segs, g, tg, wp, field, aux, F,
+                                          nullptr, nullptr, nullptr, nullptr, nullptr, 1, sc,
+                                          nullptr, nullptr, S, wcnt, nullptr, src, st);
+}
+template hipError_t launch_mask_reuse<float>(int, int, const unsigned long long*, const SegRec*,
+                                             GridDesc, TileGrid, const WalkParams&, const void*,
+                                             const double*, const DevScalars*, double*, double*,
+                                             const uint32_t*, unsigned int*, uint8_t*,
+                                             hipStream_t);
+template hipError_t launch_mask_reuse<double>(int, int, const unsigned long long*, const SegRec*,
+                                              GridDesc, TileGrid, const WalkParams&, const void*,
+                                              const double*, const DevScalars*, double*, double*,
+                                              const uint32_t*, unsigned int*, uint8_t*,
+                                              hipStream_t);
 
 hipError_t launch_mask_carry(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, const double* E, const double* S,

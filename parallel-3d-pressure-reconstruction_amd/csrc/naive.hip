@@ -773,6 +773,22 @@ hipError_t launch_divide_count(int64_t V, float* p, float* pn, const float* pc, 
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(256)
+k_mask_diff(const double* __restrict__ a, const double* __restrict__ b, int64_t n,
+            unsigned int* __restrict__ diff) {
+  bool d = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d |= __double_as_longlong(a[i]) != __double_as_longlong(b[i]);
+  if (__any(d) && (threadIdx.x & 63) == 0) *diff = 1u;
+}
+
+hipError_t launch_mask_diff(const double* a, const double* b, int64_t n, unsigned int* diff,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_diff, dim3(grid_for(n, 4096)), dim3(256), 0, st, a, b, n, diff);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_f64(double* x, int64_t n, double v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n)), dim3(256), 0, st, x, n, v);
   return hipGetLastError();

@@ -136,6 +136,15 @@ hipError_t launch_mask_walk(int method, const RayRec* rays, int64_t nrays, const
                             double* wc, double* wn, unsigned int* wcnt, double* wsum,
                             uint8_t* src, void* rule, hipStream_t st);
 size_t mask_rule_bytes(int method, int64_t nval);
+// plain-mask records F kept from an earlier field with the same mask: the new
+// field's entering-step terms E, then the setup pass (segment sums S, the
+// weights wcnt, the run starts src)
+template <typename T>
+hipError_t launch_mask_reuse(int method, int ntiles, const unsigned long long* tile_ptr,
+                             const SegRec* segs, GridDesc g, TileGrid tg, const WalkParams& wp,
+                             const void* field, const double* aux, const DevScalars* sc,
+                             double* E, double* S, const uint32_t* F, unsigned int* wcnt,
+                             uint8_t* src, hipStream_t st);
 // the tile rule replay alone (no setup pass): E, R, F (wc, wn) per segment
 template <typename T>
 hipError_t launch_rule_replay(int method, int64_t nrays, const unsigned int* nseg,
