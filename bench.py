@@ -714,7 +714,7 @@ def main():
     it_ach = it_bytes / (it_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(it_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(it_ach / HBM_PEAK_GBS, 4), "traffic": traffic.get("iteration"),
-            "kernel": "k_iterate_grp<4, increments> (one source iteration)",
+            "kernel": "k_iterate_grp<8, increments> (one source iteration)",
             "algorithmic_bytes": it_bytes, "ms": round(it_ms, 5),
             "matrix_words": reps[-1].matrix_words, "source_visits": reps[-1].source_visits,
             # the two ray traversals: the bytes they move (counter traffic per

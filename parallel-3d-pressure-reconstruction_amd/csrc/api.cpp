@@ -450,8 +450,10 @@ int build_plan(omni3d_ctx* ctx) {
   // terms below 2^51); otherwise (e.g. thousands of directions) the masked
   // sliced-ELL layout
   const char* iter_env = std::getenv("OMNI3D_ITER");
-  // groups of 4 chunks use merged words with 22-bit sources
-  const int grp = ctx->dense_group == 4 && nS < (1 << 22) ? 4 : 1;
+  // groups of 8 (4) chunks use merged words with 21-bit (22-bit) sources
+  int grp = ctx->dense_group;
+  if (grp == 8 && nS >= (1 << 21)) grp = 4;
+  if (grp == 4 && nS >= (1 << 22)) grp = 1;
 #ifdef OMNI3D_LAYOUT_STUDY
   const bool want_slots = iter_env && (std::strcmp(iter_env, "slots") == 0 ||
                                        std::strcmp(iter_env, "staged") == 0);
@@ -500,7 +502,7 @@ int build_plan(omni3d_ctx* ctx) {
       // slack after the words: the iteration stages them with 16-byte loads
       OMNI_TRY_HIP(ctx, pl.dent.alloc(sizeof(unsigned int) * (nnz + 8)));
       OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.dent.as<void>(), 0, sizeof(unsigned int) * (nnz + 8), st));
-      // groups of at most 256 consecutive rows holding about equal entries:
+      // groups of at most 64 * grp consecutive rows holding about equal entries:
       // one workgroup per group, as many groups as fill the CUs evenly, so
       // the launch does not wait on the largest uniform group
       DevBuf rgroup;
@@ -514,7 +516,7 @@ int build_plan(omni3d_ctx* ctx) {
           (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         ncu = ctx->groups_fill_cus ? std::max(ncu, 1) : 1;
         std::vector<int> hgrow, hrg((size_t)nS);
-        balance_groups(hrp, 256, ncu, &hgrow, &hrg);
+        balance_groups(hrp, 64 * grp, ncu, &hgrow, &hrg);
         pl.dense_ngroups = (int)hgrow.size() - 1;
         OMNI_TRY_HIP(ctx, pl.dense_grow.alloc(sizeof(int) * hgrow.size()));
         OMNI_TRY_HIP(ctx, rgroup.alloc(sizeof(int) * (size_t)std::max(nS, 1)));
@@ -1246,8 +1248,9 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                          ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
                                          ctx->am_run.as<uint32_t>(), st));
   // unit weights (plain-mask): the rows as the merged-word layout of the
-  // plain method's source iteration (k_iterate_grp: groups of 256 rows, words
-  // sorted by source, equal entries merged), built from CSR rows; otherwise
+  // plain method's source iteration (k_iterate_grp, here uniform groups of
+  // 256 rows, so that ~10^5 run starts still make more groups than CUs;
+  // words sorted by source, equal entries merged), built from CSR rows; otherwise
   // the rows in tile layout (k_mtile_accum / k_mtile_finish)
   const bool grouped = fixed && nS < (1 << 22);
   unsigned long long nnz = 0;
@@ -1748,7 +1751,7 @@ bool affine_fusable(const omni3d_ctx* ctx) {
   const omni3d_params& P = ctx->prm;
   const Plan& pl = *ctx->plan;
   return P.solver == OMNI3D_SOLVER_AFFINE && P.method == OMNI3D_METHOD_PLAIN && pl.valid &&
-         pl.dense && pl.dense_group == 4 && !pl.staged && !ray_engine(ctx) &&
+         pl.dense && pl.dense_group > 1 && !pl.staged && !ray_engine(ctx) &&
          P.max_iterations > 0 && pl.nS > 0;
 }
 
@@ -1899,7 +1902,7 @@ int omni3d_create(const omni3d_params* params, omni3d_ctx** out) {
 #endif
   if (const char* e = std::getenv("OMNI3D_DENSE_GROUP")) {
     const int v = std::atoi(e);
-    if (v == 1 || v == 4) ctx->dense_group = v;
+    if (v == 1 || v == 4 || v == 8) ctx->dense_group = v;
   }
 #ifdef OMNI3D_LAYOUT_STUDY
   if (const char* e = std::getenv("OMNI3D_DENSE_VARIANT")) {

@@ -159,7 +159,9 @@ int fused_descriptors(omni3d_batch* b, bool* fused) {
   *fused = false;
   if (std::getenv("OMNI3D_BATCH_SEPARATE")) return OMNI3D_OK;
   for (omni3d_ctx* c : b->snap)
-    if (!affine_fusable(c) || c->prm.precision != b->snap[0]->prm.precision) return OMNI3D_OK;
+    if (!affine_fusable(c) || c->prm.precision != b->snap[0]->prm.precision ||
+        c->plan->dense_group != b->snap[0]->plan->dense_group)
+      return OMNI3D_OK;
   std::vector<IterDesc> d;
   std::vector<unsigned int> blk;
   for (int i = 0; i < (int)b->snap.size(); i++) {
@@ -219,7 +221,8 @@ int capture(omni3d_batch* b) {
     const int inc = P.precision == OMNI3D_FP32_KAHAN && !std::getenv("OMNI3D_NO_INCREMENTS");
     hipError_t e = hipSuccess;
     for (int k = 1; e == hipSuccess && k <= P.max_iterations; k++)
-      e = launch_iterate_multi(inc, k, b->desc.as<IterDesc>(), b->blk.as<unsigned int>(), b->nblk,
+      e = launch_iterate_multi(b->snap[0]->plan->dense_group, inc, k, b->desc.as<IterDesc>(),
+                               b->blk.as<unsigned int>(), b->nblk,
                                P.max_iterations, P.tolerance, b->main);
     if (e != hipSuccess) rc = bfail(b, OMNI3D_ERR_HIP, std::string("iterations: ") + hipGetErrorString(e));
     if (rc == OMNI3D_OK) rc = enqueue_branches(b, 2, b->fork2);

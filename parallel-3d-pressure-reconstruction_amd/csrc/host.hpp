@@ -96,11 +96,11 @@ struct Plan {
   // dense iteration layout (valid when dense): per chunk, entries sorted by
   // source, word (row in chunk << 24 | source), in the CSR range of its rows
   bool dense = false;
-  int dense_group = 1; // chunks per group of the words (1 or 4)
-  int64_t nwords = 0;  // dense words (merged equal entries for groups of 4)
+  int dense_group = 1; // chunks per group of the words (1, 4 or 8)
+  int64_t nwords = 0;  // dense words (merged equal entries for groups)
   DevBuf dent;         // uint32[nnz+8]
-  DevBuf dense_wptr;   // uint32[ngroups+1] first word of each group (groups of 4)
-  DevBuf dense_grow;   // int32[ngroups+1] first row of each group (groups of 4)
+  DevBuf dense_wptr;   // uint32[ngroups+1] first word of each group (groups only)
+  DevBuf dense_grow;   // int32[ngroups+1] first row of each group (groups only)
   int dense_ngroups = 0;
   // tile engine: compiled paths and per-tile segment lists
   TileGrid tg{};
@@ -145,7 +145,7 @@ struct omni3d_ctx {
   int tile_threads = 1024;                   // workgroup size of the tile passes
   int iter_waves = 2;                        // wavefronts per 64-row chunk of the iteration
   int dense_variant = 0;                     // dense iteration: wavefronts x round size
-  int dense_group = 4;                       // dense iteration: chunks per row group
+  int dense_group = 8;                       // dense iteration: chunks per row group (1, 4, 8)
   // the row groups of the grouped iteration are as many as fill the CUs
   // evenly (a lone reconstruction); a batch, whose snapshots share one
   // launch per iteration, keeps the uniform count

@@ -533,9 +533,10 @@ __device__ void finish_iteration(int k, int nparts, const double* __restrict__ p
 }
 
 // The dense iteration over groups of kG chunks: a workgroup of kG
-// wavefronts owns a group (kG * 64 = 256 rows, the group's words sorted by
-// source, word (row in group << 24) | (multiplicity - 1) << 22 | source, the
-// group's words from wptr[g]), its wavefronts take the
+// wavefronts owns a group (up to kG * 64 = 256 or 512 rows, the group's
+// words sorted by source, word (row in group << (32 - b)) | (multiplicity -
+// 1) << (30 - b) | source with b = 8 or 9 row bits, the group's words from
+// wptr[g]), its wavefronts take the
 // group's rounds of 512 words in turn and add into the group's LDS row sums;
 // each wavefront then finishes the 64 rows of its chunk.  Sources shared by
 // neighbouring chunks touch the same 128-byte lines in one gather.
@@ -571,8 +572,10 @@ __device__ __forceinline__ void iterate_grp_body(
   }
   constexpr int kU = 8;
   constexpr int kRows = 64 * kG;
-  static_assert(kRows == 256, "merged words hold 8-bit rows");
-  constexpr unsigned int kSrcMask = (1u << 22) - 1u;
+  static_assert(kRows == 256 || kRows == 512, "merged words hold 8- or 9-bit rows");
+  constexpr int kRowShift = kRows == 256 ? 24 : 23;
+  constexpr int kMulShift = kRowShift - 2;
+  constexpr unsigned int kSrcMask = (1u << kMulShift) - 1u;
   __shared__ unsigned long long s_acc[kRows];
   __shared__ __attribute__((aligned(16))) unsigned int s_ent[kG][kU * 64];
   __shared__ double s_red[2][kG];
@@ -593,8 +596,8 @@ __device__ __forceinline__ void iterate_grp_body(
   for (int g = (int)lb; g < ngr; g += (int)nb) {
     s_acc[threadIdx.x] = 0ull;
     __syncthreads();
-    // the group's rows: [grow[g], grow[g + 1]) (at most 256, chosen so that
-    // groups hold about equal words), or 256 consecutive rows
+    // the group's rows: [grow[g], grow[g + 1]) (at most kRows, chosen so
+    // that groups hold about equal words), or kRows consecutive rows
     const int rbeg = grow ? grow[g] : g * kRows;
     const int nrows = grow ? grow[g + 1] - rbeg : min(kRows, nS - rbeg);
     const unsigned long long e0 = wptr[g], e1 = wptr[g + 1];
@@ -646,8 +649,8 @@ __device__ __forceinline__ void iterate_grp_body(
         // the fixed-point value times its multiplicity (an integer product,
         // exact; scaling x first would leave the bias rounding's 2^51 range);
         // lanes outside the group's words add 0 (no branch around the add)
-        const unsigned int m = ok[u] ? ((w[u] >> 22) & 3u) + 1u : 0u;
-        atomicAdd(&s_acc[w[u] >> 24], (unsigned long long)to_fixed(x[u], scale) * m);
+        const unsigned int m = ok[u] ? ((w[u] >> kMulShift) & 3u) + 1u : 0u;
+        atomicAdd(&s_acc[w[u] >> kRowShift], (unsigned long long)to_fixed(x[u], scale) * m);
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -1073,13 +1076,19 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
     const int ngr = grow ? ngroups : (nS + rows - 1) / rows;
     grid = grouped_grid_n(ngr);
     const int inc = srow ? (k == 1 ? 1 : 2) : 0;
-#define OMNI_GRP(I)                                                                            \
-  hipLaunchKernelGGL((k_iterate_grp<4, I>), dim3(grid), dim3(256), 0, st, k, nS, wptr, dent,     \
+#define OMNI_GRP(G, I)                                                                         \
+  hipLaunchKernelGGL((k_iterate_grp<G, I>), dim3(grid), dim3(64 * G), 0, st, k, nS, wptr, dent,  \
                      grow, ngr, diag, slist, cnt, bs, pinit, seed, pold, pnew, srow, din, dnew,   \
                      sc, partials, max_iter, tol)
-    if (inc == 1) OMNI_GRP(1);
-    else if (inc == 2) OMNI_GRP(2);
-    else OMNI_GRP(0);
+    if (gchunks == 8) {
+      if (inc == 1) OMNI_GRP(8, 1);
+      else if (inc == 2) OMNI_GRP(8, 2);
+      else OMNI_GRP(8, 0);
+    } else {
+      if (inc == 1) OMNI_GRP(4, 1);
+      else if (inc == 2) OMNI_GRP(4, 2);
+      else OMNI_GRP(4, 0);
+    }
 #undef OMNI_GRP
     return hipGetLastError();
   }
@@ -1108,15 +1117,22 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
   return hipGetLastError();
 }
 
-hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsigned int* blk_snap,
-                                unsigned int nblocks, int max_iter, double tol, hipStream_t st) {
+hipError_t launch_iterate_multi(int gchunks, int inc, int k, const IterDesc* desc,
+                                const unsigned int* blk_snap, unsigned int nblocks, int max_iter,
+                                double tol, hipStream_t st) {
   const int v = inc ? (k == 1 ? 1 : 2) : 0;
-#define OMNI_MULTI(I)                                                                        \
-  hipLaunchKernelGGL((k_iterate_multi<4, I>), dim3(nblocks), dim3(256), 0, st, k, desc, blk_snap, \
-                     max_iter, tol)
-  if (v == 1) OMNI_MULTI(1);
-  else if (v == 2) OMNI_MULTI(2);
-  else OMNI_MULTI(0);
+#define OMNI_MULTI(G, I)                                                                      \
+  hipLaunchKernelGGL((k_iterate_multi<G, I>), dim3(nblocks), dim3(64 * G), 0, st, k, desc,       \
+                     blk_snap, max_iter, tol)
+  if (gchunks == 8) {
+    if (v == 1) OMNI_MULTI(8, 1);
+    else if (v == 2) OMNI_MULTI(8, 2);
+    else OMNI_MULTI(8, 0);
+  } else {
+    if (v == 1) OMNI_MULTI(4, 1);
+    else if (v == 2) OMNI_MULTI(4, 2);
+    else OMNI_MULTI(4, 0);
+  }
 #undef OMNI_MULTI
   return hipGetLastError();
 }

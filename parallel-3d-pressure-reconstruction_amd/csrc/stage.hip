@@ -2,7 +2,8 @@
 //
 // Dense words (the default): per group of 64-row chunks, the CSR entries
 // sorted by (source, row) and equal entries merged into one word
-// (row << 24) | (multiplicity - 1) << 22 | source.
+// (row << (32 - b)) | (multiplicity - 1) << (30 - b) | source, with b = 8 row
+// bits for groups of 4 chunks and 9 for groups of 8.
 //
 // LDS-staged lines (measurement only, built with OMNI3D_LAYOUT_STUDY): rows
 // (sources) are grouped kStageGroup chunks of 64 at a time.  For every
@@ -181,7 +182,8 @@ __global__ void k_merge_flags(int64_t n, const unsigned long long* __restrict__ 
   flag[i] = (c & 3) == 0 ? 1u : 0u;
 }
 
-// word (row << 24) | (multiplicity - 1) << 22 | source at the run's slot
+// word (row << (32 - rb)) | (multiplicity - 1) << (30 - rb) | source at the
+// run's slot
 __global__ void k_merge_write(int64_t n, int rb, const unsigned long long* __restrict__ k,
                               const unsigned int* __restrict__ flag,
                               const unsigned int* __restrict__ excl, unsigned int* __restrict__ dent) {
@@ -191,8 +193,8 @@ __global__ void k_merge_write(int64_t n, int rb, const unsigned long long* __res
   while (m < 4 && i + m < n && k[i + m] == k[i]) m++;
   const unsigned long long key = k[i];
   const unsigned int row = (unsigned int)(key & ((1ull << rb) - 1ull));
-  const unsigned int src = (unsigned int)((key >> rb) & 0x3fffffull);
-  dent[excl[i]] = (row << 24) | ((m - 1u) << 22) | src;
+  const unsigned int src = (unsigned int)((key >> rb) & ((1ull << (30 - rb)) - 1ull));
+  dent[excl[i]] = (row << (32 - rb)) | ((m - 1u) << (30 - rb)) | src;
 }
 
 // first word of every group: the merged index of the group's first entry

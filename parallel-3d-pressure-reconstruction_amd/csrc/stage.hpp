@@ -75,7 +75,7 @@ hipError_t launch_iter_finish(int k, int nparts, const double* partials, DevScal
 // one iteration ahead of the stop rule): out = p_S of iteration k* - 1.
 hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const double* b2,
                             const DevScalars* sc, double* out, hipStream_t st);
-// grid of the grouped dense iteration (4 chunks per workgroup)
+// grid of the grouped dense iteration (one group per workgroup)
 inline unsigned int grouped_grid_n(int ngroups) {
   return (unsigned int)(ngroups < 1 ? 1 : (ngroups > 65536 ? 65536 : ngroups));
 }
@@ -96,8 +96,9 @@ hipError_t build_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
                                const int* grow = nullptr, int ngroups = 0);
 size_t dense_tmp_bytes(int64_t nnz);
 // Groups (gchunks > 1): equal (group, source, row) keys of the sorted keys
-// merged into words (row << 24) | (multiplicity - 1) << 22 | source (sources
-// below 2^22), wptr[ngroups+1] the first word of every group.
+// merged into words (row << (32 - b)) | (multiplicity - 1) << (30 - b) |
+// source, b = dense_row_bits(gchunks) (sources below 2^(30 - b)),
+// wptr[ngroups+1] the first word of every group.
 hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned long long* rowptr,
                                const unsigned long long* keys, unsigned int* flag,
                                unsigned int* excl, void* tmp, size_t tmp_bytes, unsigned int* dent,
@@ -105,8 +106,8 @@ hipError_t merge_dense_entries(int nS, int64_t nnz, int gchunks, const unsigned 
                                const int* grow = nullptr, int ngroups = 0);
 // gchunks 1: one chunk per wavefront, variant = chunks per workgroup x
 // entries per round (0: 4 x 512, 1: 8 x 512, 2: 8 x 256, 3: 4 x 256,
-// 4: 4 x 1024); gchunks 4: one group per workgroup of 4 wavefronts sharing
-// the group's LDS row sums, merged words from wptr; partials then hold two
+// 4: 4 x 1024); gchunks 4 or 8: one group per workgroup of 4 or 8
+// wavefronts sharing the group's LDS row sums, merged words from wptr; partials then hold two
 // halves (iterations alternate) and each launch runs the stop rule of the
 // previous iteration, max |.| slots k % 3 -- launch_iter_finish ends the loop.  With srow (groups only) the
 // increment form: srow[nS] FP64 row sums of p_S kept across iterations,
@@ -127,7 +128,7 @@ struct IterDesc {
   int nS, nb, first, seed;
   const unsigned int* wptr;
   const unsigned int* dent;
-  const int* grow;  // rows of each group (nullptr: 256 consecutive rows)
+  const int* grow;  // rows of each group (nullptr: 64 * chunks consecutive rows)
   int ngr;          // groups
   const unsigned int* diag;
   const int* slist;
@@ -140,10 +141,12 @@ struct IterDesc {
   DevScalars* sc;
   double* partials;
 };
-// iteration k of every snapshot described (inc: increment form, FP32 ray
-// sums); blk_snap[b] = the snapshot of workgroup b
-hipError_t launch_iterate_multi(int inc, int k, const IterDesc* desc, const unsigned int* blk_snap,
-                                unsigned int nblocks, int max_iter, double tol, hipStream_t st);
+// iteration k of every snapshot described (gchunks: chunks per group, 4 or 8,
+// the same for all; inc: increment form, FP32 ray sums); blk_snap[b] = the
+// snapshot of workgroup b
+hipError_t launch_iterate_multi(int gchunks, int inc, int k, const IterDesc* desc,
+                                const unsigned int* blk_snap, unsigned int nblocks, int max_iter,
+                                double tol, hipStream_t st);
 #ifdef OMNI3D_LAYOUT_STUDY
 hipError_t launch_iterate_staged(const StagedIter& si, int k, const unsigned int* diag,
                                  const int* slist, const unsigned int* cnt, const long long* bs,

@@ -120,7 +120,7 @@ LAYOUTS = [
 def test_iteration_layouts_agree(gpu_available, ref_dirs3, monkeypatch, env, precision):
     """Every fallback layout of the source iteration (dense words per chunk,
     masked sliced-ELL slots, p_S gathers instead of increments) against the
-    default (merged words in groups of four chunks) from a random initial
+    default (merged words in groups of eight chunks) from a random initial
     pressure: FP64 rel-L2 <= 1e-12, FP32 <= 1e-6, same iteration count."""
     shape = (32, 34, 36)
     print(f"case iteration layout {env} precision={precision}", flush=True)
@@ -144,3 +144,34 @@ def test_iteration_layouts_agree(gpu_available, ref_dirs3, monkeypatch, env, pre
     err = F.rel_l2(pv, pd)
     print("layout vs default rel-L2", err)
     assert err < (1e-12 if precision == omni3d.FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("precision", [omni3d.FP64, omni3d.FP32_KAHAN])
+def test_word_groups_of_four_and_eight_chunks_agree_bitwise(gpu_available, ref_dirs3,
+                                                            monkeypatch, precision):
+    """The merged words in groups of 4 chunks (256 rows, 8-bit rows) against
+    the default groups of 8 chunks (512 rows, 9-bit rows).  A row's sum is an
+    exact 64-bit fixed-point sum whatever group holds it, so p is bit-equal
+    (only the stop rule's partial sums are added in another order: tolerance
+    0 here).  86 k sources, so that the 256- and 512-row caps split the rows
+    differently (smaller grids give both the same CU-filling groups)."""
+    shape = (120, 112, 128)
+    print(f"case word groups 4 vs 8 precision={precision}", flush=True)
+    f = F.taylor_green(128, noise=0.05, seed=21, shape=shape)
+    p0 = np.random.default_rng(22).normal(size=shape)
+    dtype = np.float64 if precision == omni3d.FP64 else np.float32
+
+    def go():
+        with omni3d.Context(shape, f["delta"], density=f["rho"], precision=precision,
+                            max_iterations=12, tolerance=0.0) as ctx:
+            ctx.set_directions(ref_dirs3)
+            ctx.set_field([np.asarray(a, dtype=dtype) for a in f["acc"]], dtype=dtype)
+            return ctx.reconstruct(p0)
+
+    p8, w8, r8 = go()
+    monkeypatch.setenv("OMNI3D_DENSE_GROUP", "4")
+    p4, w4, r4 = go()
+    assert r4.iterations == r8.iterations == 12
+    assert r4.sources == r8.sources > 65536
+    np.testing.assert_array_equal(w4, w8)
+    np.testing.assert_array_equal(p4, p8)
