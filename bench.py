@@ -4,7 +4,8 @@ Workload: swirl-only Burgers vortex pressure on a 256^3 grid (L = 1, core
 radius 0.1 L, axis z through the centre, Gamma = 1) with 2 % Gaussian noise
 on the gradient; 162 icosahedral directions (twice subdivided), line spacing
 1 voxel; plain method; gradient and trapezoid terms in FP32, added into FP64
-ray sums by the tile engine (the compensated FP32 mode: OMNI3D_FP32_KAHAN),
+accumulators by the tile engine, a ray's partial sums stored rounded once to
+FP32 (the compensated FP32 mode: OMNI3D_FP32_KAHAN),
 averaging, source update and residual in FP64; stop at a relative change
 < 1e-6 of the source voxels or 200 iterations.  The field is resident on the GPU before
 the timed region; a step is one reconstruction of that snapshot with the
@@ -122,7 +123,7 @@ def batch_mode(args):
         "metric": "snapshots per second, 64 uneven snapshots (one hipGraph)",
         "value": round(64 / (ms_graph * 1e-3), 3), "unit": "snapshots/s", "n_gpus": 1,
         "steps": args.steps, "ms_per_step": round(ms_graph, 3), "higher_is_better": True,
-        "dtype": "f32 gradient and trapezoid terms, f64 ray sums and averaging",
+        "dtype": "f32 gradient, trapezoid terms and stored ray partial sums; f64 accumulation, iteration and averaging",
         "data": "synthetic",
         "config": {"workload": "taylor-green series, 64 snapshots from {64^3, 96^3, 128x96x64, "
                                "160x128x96}, 0-5% noise, 162 directions, plain affine",
@@ -777,7 +778,7 @@ def main():
         "value": round(s_per, 6), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(s_per * 1e3, 4),
         "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f32 gradient and trapezoid terms, f64 ray sums and averaging",
+        "dtype": "f32 gradient, trapezoid terms and stored ray partial sums; f64 accumulation, iteration and averaging",
         "data": "synthetic",
         "config": {"workload": f"burgers-vortex {n}^3, 2% noise, {D} icosahedral directions, "
                                f"line spacing 1, plain method, affine solver, tol {args.tol:g} / "

@@ -86,11 +86,13 @@ extern "C" {
 
 /* Arithmetic of the ray sums. */
 #define OMNI3D_FP64 0        /* gradient and ray sums in FP64 */
-/* Gradient and trapezoid terms in FP32, ray sums compensated: the tile engine
- * (the default traversals) adds the FP32 terms into FP64 accumulators, the
- * one-lane-per-ray walks (reference schedule, trilinear traversal, the
- * cross-checks) into Kahan-compensated FP32 sums.  Averaging, the source
- * iteration's row sums, the boundary update and the residual stay FP64. */
+/* Gradient and trapezoid terms in FP32, ray sums compensated.  The tile
+ * engine (the default traversals) adds the FP32 terms into FP64 accumulators
+ * and stores a ray's partial sums (per segment, and the prefix sums carried
+ * between tiles) rounded once to FP32; the one-lane-per-ray walks (reference
+ * schedule, trilinear traversal, the cross-checks) add them into
+ * Kahan-compensated FP32 sums.  Averaging, the source iteration's row sums,
+ * the boundary update and the residual stay FP64. */
 #define OMNI3D_FP32_KAHAN 1
 #define OMNI3D_FP32 OMNI3D_FP32_KAHAN
 

@@ -717,7 +717,8 @@ int run_affine(omni3d_ctx* ctx, int seed) {
   const int phase = ctx->affine_phase;
   const size_t ns = (size_t)std::max(pl.nS, 1);
   OMNI_TRY_HIP(ctx, ctx->bs.alloc(sizeof(long long) * ns));
-  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * std::max<int64_t>(pl.nval, 1)));
+  // segment sums, then carries, in the compute precision
+  OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(T) * std::max<int64_t>(pl.nval, 1)));
   if (phase != 2) {
   OMNI_TRY_HIP(ctx, launch_fill_u32(sc, sizeof(DevScalars) / 4, 0u, st));
   // max |a| over the packed field (the fourth component is 0)
@@ -736,17 +737,17 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.tile_ptr1.as<unsigned long long>(),
                                           pl.segs_slot.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), pl.cnt.as<unsigned int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(),
+                                          ctx->segval.as<T>(),
                                           nullptr, st));
     record_event(ctx, 8);
     OMNI_TRY_HIP(ctx, launch_carry_scan(pl.nrays, pl.nseg.as<unsigned int>(),
                                         pl.val_off.as<unsigned long long>(),
-                                        ctx->segval.as<double>(), st));
+                                        ctx->segval.as<T>(), st));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kSourceSums, ctx->tile_threads, pl.nsurf,
                                           pl.surf.as<int>(), pl.sseg_ptr.as<unsigned long long>(),
                                           pl.ssegs.as<SegRec>(), pl.g, pl.tg, wp, field,
                                           pl.sid.as<int>(), pl.cnt.as<unsigned int>(), nullptr, nullptr, sc,
-                                          ctx->segval.as<double>(),
+                                          ctx->segval.as<T>(),
                                           ctx->bs.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_walk_plain<T>(false, trilinear(P), pl.rays.as<RayRec>(), pl.nrays,
@@ -919,7 +920,7 @@ int run_affine(omni3d_ctx* ctx, int seed) {
                                           pl.sid.as<int>(), pl.cnt.as<unsigned int>(),
                                           rot3 ? ctx->psel.as<double>() : pbuf[0],
                                           rot3 ? ctx->psel.as<double>() : pbuf[1], sc,
-                                          ctx->segval.as<double>(),
+                                          ctx->segval.as<T>(),
                                           ctx->sum.as<long long>(), st));
   } else {
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->sum.as<void>(), 2 * V * nc, 0u, st));

@@ -176,7 +176,7 @@ struct omni3d_ctx {
   omni3d::DevBuf srow, dinc0, dinc1;
   omni3d::DevBuf partials;  // double[2*grid]
   omni3d::DevBuf sum;       // int64[ncopies*V]
-  omni3d::DevBuf segval;    // double[nval] segment sums, then carries
+  omni3d::DevBuf segval;    // T[nval] segment sums, then carries (T: compute precision)
   int acc_copies = 1;       // private accumulator copies of the ray-engine final pass
   int engine = 0;           // 0 = tile engine, 1 = ray engine (one lane per ray, atomics)
   omni3d::DevBuf bp[13];    // boundary-pair pipeline: keys, sums, runs, pairs, pressure

@@ -73,11 +73,13 @@ k_bp_integral(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __
 }
 
 // Tile engine: the ray's sum is the sum of its segment sums (each step of
-// the ray is in exactly one segment's sum), read in segment order from the
-// value slots of its group of 64 rays.
+// the ray is in exactly one segment's sum; stored in the compute precision
+// C, added in FP64), read in segment order from the value slots of its
+// group of 64 rays.
+template <typename C>
 __global__ void __launch_bounds__(256)
 k_bp_total(const RayRec* __restrict__ rays, int64_t nrays, const unsigned int* __restrict__ nseg,
-           const unsigned long long* __restrict__ val_off, const double* __restrict__ segval,
+           const unsigned long long* __restrict__ val_off, const C* __restrict__ segval,
            unsigned long long* __restrict__ key, double* __restrict__ val) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
@@ -88,11 +90,11 @@ k_bp_total(const RayRec* __restrict__ rays, int64_t nrays, const unsigned int* _
   for (; k + 4 <= n; k += 4) {
     double x[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) x[u] = segval[b + 64ull * (k + u)];
+    for (int u = 0; u < 4; u++) x[u] = (double)segval[b + 64ull * (k + u)];
 #pragma unroll
     for (int u = 0; u < 4; u++) s = s + x[u];
   }
-  for (; k < n; k++) s = s + segval[b + 64ull * k];
+  for (; k < n; k++) s = s + (double)segval[b + 64ull * k];
   const RayRec rr = rays[r];
   key[r] = ((unsigned long long)rr.out << 30) | (unsigned long long)rr.in;
   val[r] = s;
@@ -199,17 +201,17 @@ int boundary_pairs(omni3d_ctx* ctx, int iterations, double* p, int64_t* npairs_o
   if (ctx->engine == 0 && pl.ntiles > 0 && pl.nval > 0) {
     // segment sums on the tile engine (pass A's first traversal), then per
     // ray the sum of its segments
-    OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(double) * pl.nval));
+    OMNI_TRY_HIP(ctx, ctx->segval.alloc(sizeof(T) * pl.nval));
     OMNI_TRY_HIP(ctx, launch_tile_pass<T>(kLocalSum, ctx->tile_threads, pl.ntiles, nullptr,
                                           pl.tile_ptr1.as<unsigned long long>(),
                                           pl.segs_slot.as<SegRec>(), pl.g, pl.tg, walk_params(P),
                                           ctx->field.as<void>(), pl.sid.as<int>(),
                                           pl.cnt.as<unsigned int>(), nullptr, nullptr,
-                                          ctx->scal.as<DevScalars>(), ctx->segval.as<double>(),
+                                          ctx->scal.as<DevScalars>(), ctx->segval.as<T>(),
                                           nullptr, st));
-    hipLaunchKernelGGL(k_bp_total, dim3(blocks_for(n)), dim3(256), 0, st, pl.rays.as<RayRec>(),
+    hipLaunchKernelGGL(k_bp_total<T>, dim3(blocks_for(n)), dim3(256), 0, st, pl.rays.as<RayRec>(),
                        n, pl.nseg.as<unsigned int>(), pl.val_off.as<unsigned long long>(),
-                       ctx->segval.as<double>(), k0.as<unsigned long long>(), v0.as<double>());
+                       ctx->segval.as<T>(), k0.as<unsigned long long>(), v0.as<double>());
   } else {
     hipLaunchKernelGGL(k_bp_integral<T>, dim3(blocks_for(n)), dim3(256), 0, st,
                        pl.rays.as<RayRec>(), n, pl.dirs.as<DirConst>(), pl.g, walk_params(P),

@@ -165,7 +165,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
             const typename Vec4<T>::type* __restrict__ field, const int* __restrict__ sid,
             const unsigned int* __restrict__ vcnt,
             const double* __restrict__ pbuf0, const double* __restrict__ pbuf1,
-            const DevScalars* __restrict__ sc, double* __restrict__ segval,
+            const DevScalars* __restrict__ sc, T* __restrict__ segval,
             long long* __restrict__ out) {
   using V4 = typename Vec4<T>::type;
   constexpr int kTileBits = kBits;
@@ -233,7 +233,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
     const bool fs = (r.info >> 21) & 1u;
     if constexpr (kMode == kFull) q.pin = ps[r.src];
     if constexpr (kMode != kLocalSum) {
-      if (!fs) q.cv = segval[r.g];
+      if (!fs) q.cv = (double)segval[r.g];
     }
     (void)jj;
     return q;
@@ -349,7 +349,7 @@ k_tile_pass(const int* __restrict__ tile_list, const unsigned long long* __restr
         const int dn = a == 0 ? d0 : (a == 1 ? d1 : d2);
         loc += (double)trapezoid<T>(wp, a, sgn, L.g[a][lv + dn], L.g[a][lv]);
       }
-      segval[sr.g] = loc;
+      segval[sr.g] = (T)loc;
     }
   }
   if (kMode == kLocalSum) return;
@@ -629,10 +629,13 @@ k_u64_to_u32(const unsigned long long* __restrict__ in, int64_t n, unsigned int*
 
 // Per-ray exclusive scan of the segment sums into carries, in place; the
 // lanes of a wavefront are the 64 rays of one group, so every step reads 64
-// consecutive slots.
+// consecutive slots.  The running sum is FP64; the sums and carries are
+// stored in the compute precision C (FP32 ray sums: each carry is the FP64
+// prefix rounded once to FP32).
+template <typename C>
 __global__ void __launch_bounds__(256)
 k_carry_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
-             const unsigned long long* __restrict__ val_off, double* __restrict__ segval) {
+             const unsigned long long* __restrict__ val_off, C* __restrict__ segval) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   const unsigned long long b = val_off[r >> 6] + (unsigned long long)(r & 63);
@@ -643,26 +646,26 @@ k_carry_scan(int64_t nrays, const unsigned int* __restrict__ nseg,
   for (; k + 16 <= n; k += 16) {
     double x[16];
 #pragma unroll
-    for (int u = 0; u < 16; u++) x[u] = segval[b + 64ull * (k + u)];
+    for (int u = 0; u < 16; u++) x[u] = (double)segval[b + 64ull * (k + u)];
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-      segval[b + 64ull * (k + u)] = run;
+      segval[b + 64ull * (k + u)] = (C)run;
       run = run + x[u];
     }
   }
   for (; k + 4 <= n; k += 4) {
     double x[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) x[u] = segval[b + 64ull * (k + u)];
+    for (int u = 0; u < 4; u++) x[u] = (double)segval[b + 64ull * (k + u)];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      segval[b + 64ull * (k + u)] = run;
+      segval[b + 64ull * (k + u)] = (C)run;
       run = run + x[u];
     }
   }
   for (; k < n; k++) {
-    const double x = segval[b + 64ull * k];
-    segval[b + 64ull * k] = run;
+    const double x = (double)segval[b + 64ull * k];
+    segval[b + 64ull * k] = (C)run;
     run = run + x;
   }
 }
@@ -842,7 +845,7 @@ static void tile_pass_threads(int threads, int ntiles, const int* tile_list,
                               TileGrid tg, const WalkParams& wp,
                               const typename Vec4<T>::type* f, const int* sid,
                               const unsigned int* vcnt, const double* p0,
-                              const double* p1, const DevScalars* sc, double* segval,
+                              const double* p1, const DevScalars* sc, T* segval,
                               long long* out, hipStream_t st) {
 #define OMNI_TILE(NT)                                                                        \
   hipLaunchKernelGGL((k_tile_pass<T, kMode, kBits, NT>), dim3(ntiles), dim3(NT), 0, st,       \
@@ -858,7 +861,7 @@ static void tile_pass_bits(int threads, int ntiles, const int* tile_list,
                            const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                            TileGrid tg, const WalkParams& wp, const typename Vec4<T>::type* f,
                            const int* sid, const unsigned int* vcnt, const double* p0,
-                           const double* p1, const DevScalars* sc, double* segval,
+                           const double* p1, const DevScalars* sc, T* segval,
                            long long* out, hipStream_t st) {
   if (tg.bits == 3)
     tile_pass_threads<T, kMode, 3>(threads, ntiles, tile_list, tile_ptr, segs, g, tg, wp, f, sid,
@@ -873,7 +876,7 @@ hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_l
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const unsigned int* vcnt, const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, long long* out, hipStream_t st) {
+                            T* segval, long long* out, hipStream_t st) {
   using V4 = typename Vec4<T>::type;
   const V4* f = static_cast<const V4*>(field);
   if (ntiles <= 0) return hipSuccess;
@@ -894,7 +897,7 @@ template hipError_t launch_tile_pass<float>(int, int, int, const int*, const uns
                                             const SegRec*, GridDesc, TileGrid, const WalkParams&,
                                             const void*, const int*, const unsigned int*,
                                             const double*, const double*,
-                                            const DevScalars*, double*, long long*,
+                                            const DevScalars*, float*, long long*,
                                             hipStream_t);
 template hipError_t launch_tile_pass<double>(int, int, int, const int*, const unsigned long long*,
                                              const SegRec*, GridDesc, TileGrid,
@@ -920,8 +923,15 @@ hipError_t launch_scan_u32(const unsigned int* cnt, int64_t n, unsigned long lon
 
 hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, double* segval, hipStream_t st) {
-  hipLaunchKernelGGL(k_carry_scan, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, nrays,
-                     nseg, val_off, segval);
+  hipLaunchKernelGGL(k_carry_scan<double>, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st,
+                     nrays, nseg, val_off, segval);
+  return hipGetLastError();
+}
+
+hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, float* segval, hipStream_t st) {
+  hipLaunchKernelGGL(k_carry_scan<float>, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st,
+                     nrays, nseg, val_off, segval);
   return hipGetLastError();
 }
 

@@ -94,14 +94,16 @@ hipError_t launch_src_segs(int pass, int nsurf, const int* tile_list,
                            TileGrid tg, const int* sid, unsigned int* cnt,
                            const unsigned long long* sptr, SegRec* out, hipStream_t st);
 // kSourceSums: tile_ptr / segs are the compact lists, kLocalSum the slot-
-// ordered copy (one range per tile); kFull the length-grouped lists
+// ordered copy (one range per tile); kFull the length-grouped lists.
+// segval[nval]: per value slot the segment sums (kLocalSum writes them),
+// then the carries (kSourceSums, kFull read them), in the compute precision
 template <typename T>
 hipError_t launch_tile_pass(int mode, int threads, int ntiles, const int* tile_list,
                             const unsigned long long* tile_ptr, const SegRec* segs, GridDesc g,
                             TileGrid tg, const WalkParams& wp, const void* field, const int* sid,
                             const unsigned int* vcnt,
                             const double* p0, const double* p1, const DevScalars* sc,
-                            double* segval, long long* out, hipStream_t st);
+                            T* segval, long long* out, hipStream_t st);
 // masked sliced-ELL copy of the source CSR (64-row chunks, per-direction
 // slots with lane masks, entries packed without padding); D <= 4096
 hipError_t launch_sell_layout(int nS, int D, const unsigned long long* rowptr,
@@ -110,8 +112,12 @@ hipError_t launch_sell_fill(int nS, int D, const unsigned long long* rowptr,
                             const unsigned long long* key, const unsigned long long* moff,
                             unsigned long long* masks, unsigned int* cell_eb,
                             unsigned short* sdir, int* ent, hipStream_t st);
+// per-ray exclusive scan of the segment sums into carries, in place (FP64
+// running sum; storage in the compute precision)
 hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
                              const unsigned long long* val_off, double* segval, hipStream_t st);
+hipError_t launch_carry_scan(int64_t nrays, const unsigned int* nseg,
+                             const unsigned long long* val_off, float* segval, hipStream_t st);
 // exclusive scan of n uint32 counts into n+1 uint64 offsets (multi-block);
 // `words` != 0 scans ceil(count/16) instead of count
 hipError_t launch_scan_u32(const unsigned int* cnt, int64_t n, unsigned long long* out,
