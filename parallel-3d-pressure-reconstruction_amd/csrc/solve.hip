@@ -654,17 +654,29 @@ __device__ __forceinline__ void iterate_grp_body(
       }
       __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
+    // the row's own values do not depend on the row sums: loaded before the
+    // barrier, they arrive while the group's last wavefronts finish
     const int row = rbeg + wave * 64 + lane;
-    if (wave * 64 + lane < nrows) {
+    const bool own = wave * 64 + lane < nrows;
+    int v = 0;
+    unsigned int c = 0, dg = 0;
+    double po = 0.0, sr = 0.0;
+    long long bv = 0;
+    if (own) {
+      v = slist[row];
+      c = cnt[v];
+      po = pold[row];
+      bv = bs[row];
+      dg = diag[row];
+      if constexpr (kInc == 2) sr = srow[row];
+    }
+    __syncthreads();
+    if (own) {
       double acc = (double)(long long)s_acc[wave * 64 + lane] * iscale;
       if (!finite) acc = B - B;
-      if constexpr (kInc == 2) acc += srow[row];
+      if constexpr (kInc == 2) acc += sr;
       if constexpr (kInc != 0) srow[row] = acc;
-      const int v = slist[row];
-      const unsigned int c = cnt[v];
-      const double po = pold[row];
-      double tot = (double)bs[row] * inv + (double)diag[row] * po + acc;
+      double tot = (double)bv * inv + (double)dg * po + acc;
       if (k == 1 && seed && pinit) tot += pinit[v];
       const double pn = c > 0 ? tot / (double)c : po;
       pnew[row] = pn;
