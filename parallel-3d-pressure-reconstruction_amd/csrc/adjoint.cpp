@@ -32,11 +32,14 @@ int io_dtype(const omni3d_ctx* ctx) {
   return ctx->prm.precision == OMNI3D_FP64 ? OMNI3D_F64 : OMNI3D_F32;
 }
 
-// Merged words of sorted CSR rows (groups of 4 chunks: the forward
-// iteration's layout) when they fit it -- sources below 2^22, rows shorter
-// than 4096 entries (int64 row sums); *dense tells whether they were built.
+// Merged words of sorted CSR rows in the forward iteration's layout (groups
+// of 8 chunks, 4 from 2^21 sources on, sized to about equal entries and as
+// many as fill the CUs evenly) when they fit it -- sources below 2^22, rows
+// shorter than 4096 entries (int64 row sums); *dense tells whether they
+// were built, *gchunks / *ngroups / grow the groups.
 int build_row_words(omni3d_ctx* ctx, int nS, unsigned long long n, const DevBuf& tptr,
-                    const DevBuf& key, DevBuf* tdent, DevBuf* twptr, bool* dense) {
+                    const DevBuf& key, DevBuf* tdent, DevBuf* twptr, DevBuf* grow,
+                    int* gchunks, int* ngroups, bool* dense) {
   hipStream_t st = ctx->stream;
   *dense = false;
   std::vector<unsigned long long> hp((size_t)nS + 1);
@@ -47,27 +50,43 @@ int build_row_words(omni3d_ctx* ctx, int nS, unsigned long long n, const DevBuf&
   unsigned long long mx = 0;
   for (int r = 0; r < nS; r++) mx = std::max(mx, hp[r + 1] - hp[r]);
   if (!(nS < (1 << 22) && mx < 4096 && n > 0 && n < (1ull << 31))) return OMNI3D_OK;
+  const int grp = nS < (1 << 21) ? 8 : 4;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  std::vector<int> hgrow, hrg((size_t)nS);
+  balance_groups(hp, 64 * grp, std::max(ncu, 1), &hgrow, &hrg);
+  const int ng = (int)hgrow.size() - 1;
+  DevBuf rgroup;
+  OMNI_TRY_HIP(ctx, grow->alloc(sizeof(int) * hgrow.size()));
+  OMNI_TRY_HIP(ctx, rgroup.alloc(sizeof(int) * (size_t)nS));
+  OMNI_TRY_HIP(ctx, hipMemcpy(grow->as<void>(), hgrow.data(), sizeof(int) * hgrow.size(),
+                              hipMemcpyHostToDevice));
+  OMNI_TRY_HIP(ctx, hipMemcpy(rgroup.as<void>(), hrg.data(), sizeof(int) * (size_t)nS,
+                              hipMemcpyHostToDevice));
   DevBuf ka, kb, tmp, flag, excl;
   OMNI_TRY_HIP(ctx, ka.alloc(sizeof(unsigned long long) * n));
   OMNI_TRY_HIP(ctx, kb.alloc(sizeof(unsigned long long) * n));
   OMNI_TRY_HIP(ctx, tmp.alloc(dense_tmp_bytes((int64_t)n)));
   OMNI_TRY_HIP(ctx, tdent->alloc(sizeof(unsigned int) * (n + 8)));
   OMNI_TRY_HIP(ctx, launch_fill_u32(tdent->as<void>(), (int64_t)n + 8, 0u, st));
-  OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)n, 4, tptr.as<unsigned long long>(),
+  OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)n, grp, tptr.as<unsigned long long>(),
                                         key.as<unsigned long long>(), ka.as<unsigned long long>(),
                                         kb.as<unsigned long long>(), tmp.as<void>(), tmp.size(),
-                                        tdent->as<unsigned int>(), st));
+                                        tdent->as<unsigned int>(), st, rgroup.as<int>(),
+                                        grow->as<int>(), ng));
   OMNI_TRY_HIP(ctx, flag.alloc(sizeof(unsigned int) * n));
   OMNI_TRY_HIP(ctx, excl.alloc(sizeof(unsigned int) * n));
-  const int ng = (nS + 255) / 256;
   OMNI_TRY_HIP(ctx, twptr->alloc(sizeof(unsigned int) * (ng + 1)));
   unsigned int nw = 0;
-  OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)n, 4, tptr.as<unsigned long long>(),
+  OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)n, grp, tptr.as<unsigned long long>(),
                                         kb.as<unsigned long long>(), flag.as<unsigned int>(),
                                         excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
                                         tdent->as<unsigned int>(), twptr->as<unsigned int>(), &nw,
-                                        st));
+                                        st, grow->as<int>(), ng));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
+  *gchunks = grp;
+  *ngroups = ng;
   *dense = true;
   return OMNI3D_OK;
 }
@@ -105,7 +124,8 @@ int build_transposed_rows(omni3d_ctx* ctx) {
                                      key.as<unsigned long long>(), st));
   {
     bool dense = false;
-    const int rc = build_row_words(ctx, nS, n, pl.tptr, key, &pl.tdent, &pl.twptr, &dense);
+    const int rc = build_row_words(ctx, nS, n, pl.tptr, key, &pl.tdent, &pl.twptr, &pl.tgrow,
+                                   &pl.tgroup, &pl.tngroups, &dense);
     if (rc) return rc;
     pl.tdense = dense;
   }
@@ -363,7 +383,7 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
         // w <- (t + diag w + sum w) / C is the forward iteration's form: the
         // grouped kernel on the transposed merged words (exact fixed-point
         // row sums), t as its source term, w over three buffers
-        const unsigned int gg = grouped_grid(nS);
+        const unsigned int gg = grouped_grid_n(pl.tngroups);
         OMNI_TRY_HIP(ctx, ctx->adj_sc.alloc(sizeof(DevScalars)));
         OMNI_TRY_HIP(ctx, ctx->adj_part.alloc(sizeof(double) * 4 * (size_t)gg));
         OMNI_TRY_HIP(ctx, ctx->adj_bs.alloc(sizeof(long long) * (size_t)nS));
@@ -374,11 +394,14 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
         OMNI_TRY_HIP(ctx, launch_absmax<double>(w[0], nS, &asc->itmax_bits[0], st));
         double* pb3[3] = {w[0], w[1], w3};
         for (int k = 1; k <= M; k++)
-          OMNI_TRY_HIP(ctx, launch_iterate_dense(4, 0, k, nS, nullptr, pl.twptr.as<unsigned int>(),
+          OMNI_TRY_HIP(ctx, launch_iterate_dense(pl.tgroup, 0, k, nS, nullptr,
+                                                 pl.twptr.as<unsigned int>(),
                                                  pl.tdent.as<unsigned int>(), diagp, slist, cnt,
                                                  ctx->adj_bs.as<long long>(), nullptr, 0,
                                                  pb3[(k - 1) % 3], pb3[k % 3], asc,
-                                                 ctx->adj_part.as<double>(), M, 0.0, st));
+                                                 ctx->adj_part.as<double>(), M, 0.0, st, nullptr,
+                                                 nullptr, nullptr, pl.tgrow.as<int>(),
+                                                 pl.tngroups));
         OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, pb3[M % 3], cs, z, st));
       } else {
         int cur = 0;

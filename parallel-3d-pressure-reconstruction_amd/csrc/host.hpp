@@ -124,11 +124,14 @@ struct Plan {
   bool trows = false;
   DevBuf tptr;  // uint64[nS+1]
   DevBuf tent;  // uint32[nnz+1]
-  // the same rows as merged words in groups of 4 chunks (the layout of the
-  // forward iteration), when sizes allow
+  // the same rows as merged words in groups of tgroup chunks sized to equal
+  // entries (the layout of the forward iteration), when sizes allow
   bool tdense = false;
+  int tgroup = 8;     // chunks per group (8, or 4 for 2^21 sources or more)
+  int tngroups = 0;   // groups
   DevBuf tdent;   // uint32[nnz+8]
   DevBuf twptr;   // uint32[ngroups+1]
+  DevBuf tgrow;   // int32[ngroups+1] first row of each group
 };
 
 }  // namespace omni3d
