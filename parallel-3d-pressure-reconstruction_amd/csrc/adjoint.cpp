@@ -162,6 +162,7 @@ int build_mask_rows(omni3d_ctx* ctx, const double* mask, MaskRows* mr) {
     }
   const int nS = (int)hsl.size();
   mr->nS = nS;
+  mr->tdense = false;
   if (nS == 0) return OMNI3D_OK;
   OMNI_TRY_HIP(ctx, mr->sid.alloc(sizeof(int) * V));
   OMNI_TRY_HIP(ctx, mr->slist.alloc(sizeof(int) * nS));
@@ -190,6 +191,13 @@ int build_mask_rows(omni3d_ctx* ctx, const double* mask, MaskRows* mr) {
                                          key.as<unsigned long long>(), st));
   OMNI_TRY_HIP(ctx, launch_sort_rows(nS, mr->tptr.as<unsigned long long>(),
                                      key.as<unsigned long long>(), st));
+  {
+    bool dense = false;
+    const int rc = build_row_words(ctx, nS, n, mr->tptr, key, &mr->tdent, &mr->twptr, &mr->tgrow,
+                                   &mr->tgroup, &mr->tngroups, &dense);
+    if (rc) return rc;
+    mr->tdense = dense;
+  }
   OMNI_TRY_HIP(ctx, mr->tent.alloc(sizeof(unsigned int) * (n + 1)));
   OMNI_TRY_HIP(ctx, launch_adj_narrow(key.as<unsigned long long>(), (int64_t)n,
                                       mr->tent.as<unsigned int>(), st));
@@ -379,11 +387,20 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
       OMNI_TRY_HIP(ctx, gather(u, t));
       OMNI_TRY_HIP(ctx, launch_adj_src_init(nS, slist, t, cnt, ts, h[0], w[0], cs, st));
       const int M = P.max_iterations - 2;
-      if (!mask_rule && pl.tdense && M > 0) {
+      // the transposed rows as merged words: the plain method's (with the
+      // plan) or the plain-mask rule's (with the mask)
+      const bool tdense = mask_rule ? mr.tdense : pl.tdense;
+      const int tgroup = mask_rule ? mr.tgroup : pl.tgroup;
+      const int tngroups = mask_rule ? mr.tngroups : pl.tngroups;
+      const DevBuf& twptr = mask_rule ? mr.twptr : pl.twptr;
+      const DevBuf& tdent = mask_rule ? mr.tdent : pl.tdent;
+      const DevBuf& tgrow = mask_rule ? mr.tgrow : pl.tgrow;
+      if (tdense && M > 0) {
         // w <- (t + diag w + sum w) / C is the forward iteration's form: the
         // grouped kernel on the transposed merged words (exact fixed-point
-        // row sums), t as its source term, w over three buffers
-        const unsigned int gg = grouped_grid_n(pl.tngroups);
+        // row sums), t as its source term, w over three buffers (a source
+        // with C = 0 keeps w = 0, as in the one-wavefront-per-row steps)
+        const unsigned int gg = grouped_grid_n(tngroups);
         OMNI_TRY_HIP(ctx, ctx->adj_sc.alloc(sizeof(DevScalars)));
         OMNI_TRY_HIP(ctx, ctx->adj_part.alloc(sizeof(double) * 4 * (size_t)gg));
         OMNI_TRY_HIP(ctx, ctx->adj_bs.alloc(sizeof(long long) * (size_t)nS));
@@ -394,14 +411,13 @@ int omni3d_backward(omni3d_ctx* ctx, const void* dldp_d, void* dldgrad_d) {
         OMNI_TRY_HIP(ctx, launch_absmax<double>(w[0], nS, &asc->itmax_bits[0], st));
         double* pb3[3] = {w[0], w[1], w3};
         for (int k = 1; k <= M; k++)
-          OMNI_TRY_HIP(ctx, launch_iterate_dense(pl.tgroup, 0, k, nS, nullptr,
-                                                 pl.twptr.as<unsigned int>(),
-                                                 pl.tdent.as<unsigned int>(), diagp, slist, cnt,
+          OMNI_TRY_HIP(ctx, launch_iterate_dense(tgroup, 0, k, nS, nullptr,
+                                                 twptr.as<unsigned int>(),
+                                                 tdent.as<unsigned int>(), diagp, slist, cnt,
                                                  ctx->adj_bs.as<long long>(), nullptr, 0,
                                                  pb3[(k - 1) % 3], pb3[k % 3], asc,
                                                  ctx->adj_part.as<double>(), M, 0.0, st, nullptr,
-                                                 nullptr, nullptr, pl.tgrow.as<int>(),
-                                                 pl.tngroups));
+                                                 nullptr, nullptr, tgrow.as<int>(), tngroups));
         OMNI_TRY_HIP(ctx, launch_adj_src_out(nS, slist, pb3[M % 3], cs, z, st));
       } else {
         int cur = 0;

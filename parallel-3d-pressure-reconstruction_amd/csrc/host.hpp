@@ -210,6 +210,11 @@ struct omni3d_ctx {
     uint64_t mask_epoch = 0, plan_epoch = 0;
     int nS = 0;
     omni3d::DevBuf cnt, sid, slist, tptr, tent, diag;
+    // the rows as merged words (the forward iteration's layout), when sizes
+    // allow: chunks per group, groups, words, first word and row per group
+    bool tdense = false;
+    int tgroup = 8, tngroups = 0;
+    omni3d::DevBuf tdent, twptr, tgrow;
   } adj_mask;
   // the transposed iteration on merged words: its scalars, partial sums and
   // fixed-point source term
