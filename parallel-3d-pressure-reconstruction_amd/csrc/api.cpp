@@ -1249,10 +1249,10 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                          ctx->mk_r.as<uint32_t>(), ctx->mk_f.as<uint32_t>(),
                                          ctx->am_run.as<uint32_t>(), st));
   // unit weights (plain-mask): the rows as the merged-word layout of the
-  // plain method's source iteration (k_iterate_grp, here uniform groups of
-  // 256 rows, so that ~10^5 run starts still make more groups than CUs;
-  // words sorted by source, equal entries merged), built from CSR rows; otherwise
-  // the rows in tile layout (k_mtile_accum / k_mtile_finish)
+  // plain method's source iteration (k_iterate_grp: groups of up to 512
+  // rows sized to about equal entries, as many as fill the CUs evenly;
+  // words sorted by source, equal entries merged), built from CSR rows;
+  // otherwise the rows in tile layout (k_mtile_accum / k_mtile_finish)
   const bool grouped = fixed && nS < (1 << 22);
   unsigned long long nnz = 0;
   unsigned int nwords = 0;
@@ -1297,20 +1297,45 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
     // slack after the words: the iteration stages them with 16-byte loads
     OMNI_TRY_HIP(ctx, ctx->am_dent.alloc(sizeof(unsigned int) * (nnz + 8)));
     OMNI_TRY_HIP(ctx, launch_fill_u32(ctx->am_dent.as<void>(), (int64_t)nnz + 8, 0u, st));
-    const int ng = (nS + 255) / 256;
+    // groups of 8 chunks (4 from 2^21 run starts on), balanced as the plain
+    // method's
+    const int grp = nS < (1 << 21) ? 8 : 4;
+    std::vector<unsigned long long> hrp((size_t)nS + 1);
+    OMNI_TRY_HIP(ctx, hipMemcpy(hrp.data(), ctx->am_ptr.as<void>(),
+                                sizeof(unsigned long long) * ((size_t)nS + 1),
+                                hipMemcpyDeviceToHost));
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    std::vector<int> hgrow, hrg((size_t)nS);
+    balance_groups(hrp, 64 * grp, std::max(ncu, 1), &hgrow, &hrg);
+    const int ng = (int)hgrow.size() - 1;
+    DevBuf rgroup;
+    OMNI_TRY_HIP(ctx, ctx->am_grow.alloc(sizeof(int) * hgrow.size()));
+    OMNI_TRY_HIP(ctx, rgroup.alloc(sizeof(int) * (size_t)nS));
+    OMNI_TRY_HIP(ctx, hipMemcpy(ctx->am_grow.as<void>(), hgrow.data(), sizeof(int) * hgrow.size(),
+                                hipMemcpyHostToDevice));
+    OMNI_TRY_HIP(ctx, hipMemcpy(rgroup.as<void>(), hrg.data(), sizeof(int) * (size_t)nS,
+                                hipMemcpyHostToDevice));
+    ctx->am_group = grp;
+    ctx->am_ngroups = ng;
     OMNI_TRY_HIP(ctx, ctx->am_wptr.alloc(sizeof(unsigned int) * ((size_t)ng + 1)));
     OMNI_TRY_HIP(ctx, launch_widen_keys((int64_t)nnz, ctx->am_src.as<unsigned int>(),
                                         keys.as<unsigned long long>(), st));
-    OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, 4, ctx->am_ptr.as<unsigned long long>(),
+    OMNI_TRY_HIP(ctx, build_dense_entries(nS, (int64_t)nnz, grp,
+                                          ctx->am_ptr.as<unsigned long long>(),
                                           keys.as<unsigned long long>(),
                                           ka.as<unsigned long long>(), kb.as<unsigned long long>(),
                                           tmp.as<void>(), tmp.size(),
-                                          ctx->am_dent.as<unsigned int>(), st));
-    OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)nnz, 4, ctx->am_ptr.as<unsigned long long>(),
+                                          ctx->am_dent.as<unsigned int>(), st, rgroup.as<int>(),
+                                          ctx->am_grow.as<int>(), ng));
+    OMNI_TRY_HIP(ctx, merge_dense_entries(nS, (int64_t)nnz, grp,
+                                          ctx->am_ptr.as<unsigned long long>(),
                                           kb.as<unsigned long long>(), flag.as<unsigned int>(),
                                           excl.as<unsigned int>(), tmp.as<void>(), tmp.size(),
                                           ctx->am_dent.as<unsigned int>(),
-                                          ctx->am_wptr.as<unsigned int>(), &nwords, st));
+                                          ctx->am_wptr.as<unsigned int>(), &nwords, st,
+                                          ctx->am_grow.as<int>(), ng));
     if (m == OMNI3D_METHOD_PLAIN_MASK) {
       cache.mask_epoch = mepoch;
       cache.plan_epoch = pl.epoch;
@@ -1417,12 +1442,13 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
       OMNI_TRY_HIP(ctx, ctx->dinc1.alloc(sizeof(float) * ns16));
     }
     float* dbuf[2] = {ctx->dinc0.as<float>(), ctx->dinc1.as<float>()};
-    const unsigned int gg = grouped_grid(nS);
+    const unsigned int gg = grouped_grid_n(ctx->am_ngroups);
     OMNI_TRY_HIP(ctx, ctx->partials.alloc(sizeof(double) * 4 * gg));
     OMNI_TRY_HIP(ctx, launch_gather_sources(slist, nS, pinit, pb3[0], st));
     OMNI_TRY_HIP(ctx, launch_absmax<double>(pb3[0], nS, &sc->itmax_bits[0], st));
     for (int k = 1; k <= P.max_iterations; k++)
-      OMNI_TRY_HIP(ctx, launch_iterate_dense(4, 0, k, nS, ctx->am_ptr.as<unsigned long long>(),
+      OMNI_TRY_HIP(ctx, launch_iterate_dense(ctx->am_group, 0, k, nS,
+                                             ctx->am_ptr.as<unsigned long long>(),
                                              ctx->am_wptr.as<unsigned int>(),
                                              ctx->am_dent.as<unsigned int>(),
                                              ctx->am_diag.as<unsigned int>(), slist, wcnt,
@@ -1431,7 +1457,8 @@ int run_affine_masked(omni3d_ctx* ctx, int seed, void* pdst, void* wdst, int dty
                                              ctx->partials.as<double>(), P.max_iterations,
                                              P.tolerance, st, inc ? ctx->srow.as<double>() : nullptr,
                                              inc ? dbuf[(k - 1) & 1] : nullptr,
-                                             inc ? dbuf[k & 1] : nullptr));
+                                             inc ? dbuf[k & 1] : nullptr,
+                                             ctx->am_grow.as<int>(), ctx->am_ngroups));
     const int K = P.max_iterations;
     OMNI_TRY_HIP(ctx, launch_iter_finish(K, (int)gg,
                                          ctx->partials.as<double>() + (size_t)(K & 1) * 2 * gg, sc,

@@ -230,6 +230,10 @@ struct omni3d_ctx {
   omni3d::DevBuf am_run, am_sid, am_slist, am_ptr, am_src, am_w, am_b, am_wv, am_p0, am_p1;
   omni3d::DevBuf am_wmax, am_sc, am_dent, am_wptr, am_bs, am_diag;
   omni3d::DevBuf am_tfirst, am_tbase, am_erow, am_chunks, am_rowsum;
+  // plain-mask rows as merged words: chunks per group, groups, first row of
+  // each group (int32[ngroups+1])
+  int am_group = 8, am_ngroups = 0;
+  omni3d::DevBuf am_grow;
   bool affine_masked = true;  // off for snapshot batches (their graphs replay the reference schedule)
   // plain-mask on the affine schedule: the mask-only structures (rule
   // records F and R, run-start ids, run sources, the rows as merged words)
