@@ -394,21 +394,51 @@ int build_plan(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, launch_max_len(pl.rays.as<RayRec>(), (int64_t)nrays, mx.as<unsigned int>() + 1, st));
   unsigned int mxh[2] = {0, 0};
   unsigned long long nsteps = 0;
-  std::vector<uint8_t> hflag(V);
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hflag.data(), flag.as<void>(), V, hipMemcpyDeviceToHost, st));
+  // source numbering: 8x8 blocks of each box face, so that consecutive
+  // source ids are neighbouring voxels in both face directions -- on the
+  // device (face slots flagged, scanned, scattered); on the host, the same
+  // order, only if a flagged voxel lies off the surface
+  const FaceSlots fs = face_slots(X, Y, Z);
+  const unsigned long long nslot = fs.off[6];
+  DevBuf svalid, soff, stmp, sint;
+  OMNI_TRY_HIP(ctx, svalid.alloc(sizeof(unsigned int) * nslot));
+  OMNI_TRY_HIP(ctx, soff.alloc(sizeof(unsigned long long) * (nslot + 1)));
+  OMNI_TRY_HIP(ctx, stmp.alloc(scan_u32_tmp_bytes((int64_t)nslot)));
+  OMNI_TRY_HIP(ctx, sint.alloc(sizeof(unsigned int)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(sint.as<void>(), 0, sizeof(unsigned int), st));
+  OMNI_TRY_HIP(ctx, launch_face_slot_flags(fs, g, flag.as<uint8_t>(), svalid.as<unsigned int>(),
+                                           sint.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(svalid.as<unsigned int>(), (int64_t)nslot,
+                                    soff.as<unsigned long long>(), 0, stmp.as<void>(), stmp.size(),
+                                    st));
+  unsigned long long nsurf_src = 0;
+  unsigned int ninterior = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsurf_src, soff.as<unsigned long long>() + nslot, 8,
+                                   hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&ninterior, sint.as<void>(), 4, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsteps, steps.as<void>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  // source numbering: 8x8 blocks of each box face, so that consecutive
-  // source ids are neighbouring voxels in both face directions
-  std::vector<int> hsid, hslist;
-  number_sources(X, Y, Z, hflag, &hsid, &hslist);
-  const int nS = (int)hslist.size();
+  int nS = 0;
   OMNI_TRY_HIP(ctx, pl.sid.alloc(sizeof(int) * V));
-  OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * std::max(nS, 1)));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.sid.as<void>(), hsid.data(), sizeof(int) * V,
-                                   hipMemcpyHostToDevice, st));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(pl.slist.as<void>(), hslist.data(), sizeof(int) * nS,
-                                   hipMemcpyHostToDevice, st));
+  if (ninterior == 0) {
+    nS = (int)nsurf_src;
+    OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * std::max(nS, 1)));
+    OMNI_TRY_HIP(ctx, launch_fill_u32(pl.sid.as<void>(), V, 0xffffffffu, st));
+    OMNI_TRY_HIP(ctx, launch_face_slot_ids(fs, g, svalid.as<unsigned int>(),
+                                           soff.as<unsigned long long>(), pl.sid.as<int>(),
+                                           pl.slist.as<int>(), st));
+  } else {
+    std::vector<uint8_t> hflag(V);
+    OMNI_TRY_HIP(ctx, hipMemcpy(hflag.data(), flag.as<void>(), V, hipMemcpyDeviceToHost));
+    std::vector<int> hsid, hslist;
+    number_sources(X, Y, Z, hflag, &hsid, &hslist);
+    nS = (int)hslist.size();
+    OMNI_TRY_HIP(ctx, pl.slist.alloc(sizeof(int) * std::max(nS, 1)));
+    OMNI_TRY_HIP(ctx, hipMemcpy(pl.sid.as<void>(), hsid.data(), sizeof(int) * V,
+                                hipMemcpyHostToDevice));
+    OMNI_TRY_HIP(ctx, hipMemcpy(pl.slist.as<void>(), hslist.data(), sizeof(int) * nS,
+                                hipMemcpyHostToDevice));
+  }
   // the tile engine's segments, then every voxel's visit count from them
   {
     const int rc = build_tiles(ctx, g, (int64_t)nrays);

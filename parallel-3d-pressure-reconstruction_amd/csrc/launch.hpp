@@ -26,6 +26,22 @@ hipError_t launch_enum_write(const DirConst* dirs, GridDesc g, int64_t npts, int
                              hipStream_t st);
 hipError_t launch_plan_flags(const RayRec* rays, int64_t nrays, GridDesc g, uint8_t* src,
                              unsigned long long* steps, hipStream_t st);
+// Source numbering of the flagged surface voxels: face f's slots are
+// [off[f], off[f + 1]), 8x8 blocks of its (u, w) plane (nu x nw voxels).
+struct FaceSlots {
+  int nu[6], nw[6];
+  unsigned long long off[7];
+};
+FaceSlots face_slots(int X, int Y, int Z);
+// valid[q] = 1 where slot q holds a flagged voxel of its face; *interior +=
+// the flagged voxels off the surface
+hipError_t launch_face_slot_flags(const FaceSlots& fs, GridDesc g, const uint8_t* flag,
+                                  unsigned int* valid, unsigned int* interior, hipStream_t st);
+// sid[v] = off[q], slist[off[q]] = v for the valid slots (sid set to -1
+// beforehand)
+hipError_t launch_face_slot_ids(const FaceSlots& fs, GridDesc g, const unsigned int* valid,
+                                const unsigned long long* off, int* sid, int* slist,
+                                hipStream_t st);
 hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
                                 GridDesc g, const int* sid, unsigned int* rowcnt,
                                 unsigned int* diag, hipStream_t st);
@@ -155,8 +171,6 @@ hipError_t launch_finalize(int64_t V, const long long* sum, int ncopies, const u
                            float* pf, double* wd, float* wf, hipStream_t st);
 template <typename S, typename D>
 hipError_t launch_convert(const S* s, D* d, int64_t n, hipStream_t st);
-hipError_t launch_source_ids(const uint8_t* flag, int64_t V, int* sid, int* slist, int* nS,
-                             hipStream_t st);
 hipError_t launch_scan_counts(const unsigned int* cnt, int n, unsigned long long* ptr,
                               hipStream_t st);
 hipError_t launch_max_u32(const unsigned int* x, int64_t n, unsigned int* out, hipStream_t st);

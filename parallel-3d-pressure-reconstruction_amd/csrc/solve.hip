@@ -871,44 +871,6 @@ __global__ void __launch_bounds__(256) k_convert(const S* __restrict__ s, D* __r
 }
 
 
-// Exclusive scan of 0/1 source flags into compact ids (single block; the
-// flag array is only scanned once per plan).
-__global__ void __launch_bounds__(1024)
-k_source_ids(const uint8_t* __restrict__ flag, int64_t V, int* __restrict__ sid,
-             int* __restrict__ slist, int* __restrict__ nS) {
-  __shared__ int s_wave[16];
-  __shared__ int s_base;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_base = 0;
-  __syncthreads();
-  for (int64_t b0 = 0; b0 < V; b0 += 1024) {
-    const int64_t v = b0 + threadIdx.x;
-    const bool f = v < V && flag[v];
-    const unsigned long long m = __ballot(f);
-    const int below = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) s_wave[wave] = __popcll(m);
-    __syncthreads();
-    int wb = 0, tot = 0;
-    for (int w = 0; w < 16; w++) {
-      if (w < wave) wb += s_wave[w];
-      tot += s_wave[w];
-    }
-    const int base = s_base;
-    if (v < V) {
-      if (f) {
-        sid[v] = base + wb + below;
-        slist[base + wb + below] = (int)v;
-      } else {
-        sid[v] = -1;
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) s_base = base + tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *nS = s_base;
-}
-
 // Exclusive scan of counts into 64-bit pointers (one block of 1024 threads);
 // ptr[n] gets the total.
 __device__ void scan_counts(const unsigned int* __restrict__ cnt, int n,
@@ -1211,12 +1173,6 @@ template hipError_t launch_convert<float, double>(const float*, double*, int64_t
 template hipError_t launch_convert<double, float>(const double*, float*, int64_t, hipStream_t);
 template hipError_t launch_convert<float, float>(const float*, float*, int64_t, hipStream_t);
 template hipError_t launch_convert<double, double>(const double*, double*, int64_t, hipStream_t);
-
-hipError_t launch_source_ids(const uint8_t* flag, int64_t V, int* sid, int* slist, int* nS,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(k_source_ids, dim3(1), dim3(1024), 0, st, flag, V, sid, slist, nS);
-  return hipGetLastError();
-}
 
 hipError_t launch_scan_counts(const unsigned int* cnt, int n, unsigned long long* ptr,
                               hipStream_t st) {

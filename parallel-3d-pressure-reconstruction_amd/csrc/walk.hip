@@ -155,6 +155,119 @@ hipError_t launch_plan_flags(const RayRec* rays, int64_t nrays, GridDesc g, uint
   return hipGetLastError();
 }
 
+// Source numbering on the device.  The flagged (entry) voxels of the box
+// surface are numbered face by face (k = 0, k = Z - 1, j = 0, j = Y - 1,
+// i = 0, i = X - 1; a voxel on several faces belongs to the first), each face
+// in 8x8 blocks of its (u, w) coordinates, blocks row by row, inside a block
+// w-major: slot q of face f is block q / 64 at (u, w) = (8 (b % nbu) + q % 8,
+// 8 (b / nbu) + q % 64 / 8).  Consecutive ids are thus neighbouring voxels.
+__device__ __forceinline__ int surface_face(int i, int j, int k, int X, int Y, int Z) {
+  if (k == 0) return 0;
+  if (k == Z - 1) return 1;
+  if (j == 0) return 2;
+  if (j == Y - 1) return 3;
+  if (i == 0) return 4;
+  if (i == X - 1) return 5;
+  return 6;
+}
+
+// The voxel of slot q, or -1 where the slot lies outside its face or its
+// voxel belongs to an earlier face.
+__device__ __forceinline__ int64_t face_slot_voxel(const FaceSlots& fs, GridDesc g,
+                                                    unsigned long long q) {
+  int f = 0;
+  while (f < 5 && q >= fs.off[f + 1]) f++;
+  const unsigned long long pos = q - fs.off[f];
+  const unsigned long long blk = pos >> 6;
+  const int in = (int)(pos & 63ull);
+  const int nbu = (fs.nu[f] + 7) >> 3;
+  const int u = (int)(blk % (unsigned long long)nbu) * 8 + (in & 7);
+  const int w = (int)(blk / (unsigned long long)nbu) * 8 + (in >> 3);
+  if (u >= fs.nu[f] || w >= fs.nw[f]) return -1;
+  int i, j, k;
+  switch (f) {
+    case 0: i = u; j = w; k = 0; break;
+    case 1: i = u; j = w; k = g.Z - 1; break;
+    case 2: i = u; j = 0; k = w; break;
+    case 3: i = u; j = g.Y - 1; k = w; break;
+    case 4: i = 0; j = u; k = w; break;
+    default: i = g.X - 1; j = u; k = w; break;
+  }
+  if (surface_face(i, j, k, g.X, g.Y, g.Z) != f) return -1;
+  return (int64_t)i + (int64_t)j * g.X + (int64_t)k * g.X * g.Y;
+}
+
+__global__ void __launch_bounds__(256)
+k_face_slot_flags(FaceSlots fs, GridDesc g, const uint8_t* __restrict__ flag,
+                  unsigned int* __restrict__ valid) {
+  const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= fs.off[6]) return;
+  const int64_t v = face_slot_voxel(fs, g, q);
+  valid[q] = v >= 0 && flag[v] ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_face_slot_ids(FaceSlots fs, GridDesc g, const unsigned int* __restrict__ valid,
+                const unsigned long long* __restrict__ off, int* __restrict__ sid,
+                int* __restrict__ slist) {
+  const unsigned long long q = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= fs.off[6] || !valid[q]) return;
+  const int64_t v = face_slot_voxel(fs, g, q);
+  const int s = (int)off[q];
+  sid[v] = s;
+  slist[s] = (int)v;
+}
+
+// Flagged voxels off the box surface (none for rays that enter at the
+// surface; the host numbering handles them if any).
+__global__ void __launch_bounds__(256)
+k_interior_flags(GridDesc g, const uint8_t* __restrict__ flag, unsigned int* __restrict__ count) {
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  unsigned int c = 0;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (!flag[v]) continue;
+    const int i = (int)(v % g.X), j = (int)((v / g.X) % g.Y), k = (int)(v / ((int64_t)g.X * g.Y));
+    if (surface_face(i, j, k, g.X, g.Y, g.Z) == 6) c++;
+  }
+  if (c) atomicAdd(count, c);
+}
+
+FaceSlots face_slots(int X, int Y, int Z) {
+  FaceSlots fs{};
+  const int nu[6] = {X, X, X, X, Y, Y}, nw[6] = {Y, Y, Z, Z, Z, Z};
+  fs.off[0] = 0;
+  for (int f = 0; f < 6; f++) {
+    fs.nu[f] = nu[f];
+    fs.nw[f] = nw[f];
+    fs.off[f + 1] = fs.off[f] + (unsigned long long)((nu[f] + 7) / 8) *
+                                    (unsigned long long)((nw[f] + 7) / 8) * 64ull;
+  }
+  return fs;
+}
+
+hipError_t launch_face_slot_flags(const FaceSlots& fs, GridDesc g, const uint8_t* flag,
+                                  unsigned int* valid, unsigned int* interior, hipStream_t st) {
+  const unsigned long long n = fs.off[6];
+  hipLaunchKernelGGL(k_face_slot_flags, dim3((unsigned int)((n + 255) / 256)), dim3(256), 0, st,
+                     fs, g, flag, valid);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t V = (int64_t)g.X * g.Y * g.Z;
+  hipLaunchKernelGGL(k_interior_flags, dim3(grid_for(V, 4096)), dim3(256), 0, st, g, flag,
+                     interior);
+  return hipGetLastError();
+}
+
+hipError_t launch_face_slot_ids(const FaceSlots& fs, GridDesc g, const unsigned int* valid,
+                                const unsigned long long* off, int* sid, int* slist,
+                                hipStream_t st) {
+  const unsigned long long n = fs.off[6];
+  hipLaunchKernelGGL(k_face_slot_ids, dim3((unsigned int)((n + 255) / 256)), dim3(256), 0, st,
+                     fs, g, valid, off, sid, slist);
+  return hipGetLastError();
+}
+
 // The entry visit of a plain ray reads its own source voxel: it is counted in
 // the per-row diagonal instead of the CSR.
 struct RowCountVisitor {
