@@ -151,25 +151,27 @@ int build_mask_rows(omni3d_ctx* ctx, const double* mask, MaskRows* mr) {
   OMNI_TRY_HIP(ctx, launch_fill_u32(flag.as<void>(), (V + 3) / 4, 0u, st));
   OMNI_TRY_HIP(ctx, launch_adj_mask_starts(pl.rays.as<RayRec>(), pl.nrays, pl.dirs.as<DirConst>(),
                                            pl.g, mask, flag.as<uint8_t>(), st));
-  std::vector<uint8_t> hf((size_t)V);
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(hf.data(), flag.as<void>(), (size_t)V, hipMemcpyDeviceToHost, st));
+  // S numbered in voxel order on the device: flags -> counts -> scan -> ids
+  DevBuf fcnt, foff, ftmp;
+  OMNI_TRY_HIP(ctx, fcnt.alloc(sizeof(unsigned int) * (size_t)V));
+  OMNI_TRY_HIP(ctx, foff.alloc(sizeof(unsigned long long) * ((size_t)V + 1)));
+  OMNI_TRY_HIP(ctx, ftmp.alloc(scan_u32_tmp_bytes(V)));
+  OMNI_TRY_HIP(ctx, launch_flag_counts(flag.as<uint8_t>(), V, fcnt.as<unsigned int>(), st));
+  OMNI_TRY_HIP(ctx, launch_scan_u32(fcnt.as<unsigned int>(), V, foff.as<unsigned long long>(), 0,
+                                    ftmp.as<void>(), ftmp.size(), st));
+  unsigned long long nsrc = 0;
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nsrc, foff.as<unsigned long long>() + V, 8,
+                                   hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  std::vector<int> hsid((size_t)V, -1), hsl;
-  for (int64_t v = 0; v < V; v++)
-    if (hf[(size_t)v]) {
-      hsid[(size_t)v] = (int)hsl.size();
-      hsl.push_back((int)v);
-    }
-  const int nS = (int)hsl.size();
+  if (nsrc >= (1ull << 31)) return fail(ctx, OMNI3D_ERR_ARG, "too many run starts");
+  const int nS = (int)nsrc;
   mr->nS = nS;
   mr->tdense = false;
   if (nS == 0) return OMNI3D_OK;
   OMNI_TRY_HIP(ctx, mr->sid.alloc(sizeof(int) * V));
   OMNI_TRY_HIP(ctx, mr->slist.alloc(sizeof(int) * nS));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mr->sid.as<void>(), hsid.data(), sizeof(int) * V,
-                                   hipMemcpyHostToDevice, st));
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(mr->slist.as<void>(), hsl.data(), sizeof(int) * nS,
-                                   hipMemcpyHostToDevice, st));
+  OMNI_TRY_HIP(ctx, launch_flag_ids(flag.as<uint8_t>(), V, foff.as<unsigned long long>(),
+                                    mr->sid.as<int>(), mr->slist.as<int>(), st));
   DevBuf cnt, key;
   OMNI_TRY_HIP(ctx, cnt.alloc(sizeof(unsigned int) * (nS + 1)));
   OMNI_TRY_HIP(ctx, launch_fill_u32(cnt.as<void>(), nS + 1, 0u, st));

@@ -234,6 +234,30 @@ k_adj_tfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
   walk_geometry(rays[r], dirs, g, fv);
 }
 
+// Voxel-order compaction of 0/1 flags: the counts to scan, then sid / slist
+// from the scanned offsets (sid = -1 where unflagged).
+__global__ void __launch_bounds__(256)
+k_flag_counts(const uint8_t* __restrict__ flag, int64_t V, unsigned int* __restrict__ cnt) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x)
+    cnt[v] = flag[v] ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256)
+k_flag_ids(const uint8_t* __restrict__ flag, int64_t V, const unsigned long long* __restrict__ off,
+           int* __restrict__ sid, int* __restrict__ slist) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < V;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (flag[v]) {
+      const int s = (int)off[v];
+      sid[v] = s;
+      slist[s] = (int)v;
+    } else {
+      sid[v] = -1;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_adj_narrow(const unsigned long long* __restrict__ key, int64_t n, unsigned int* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -414,6 +438,17 @@ hipError_t launch_adj_transpose_fill(const RayRec* rays, int64_t nrays, const Di
 hipError_t launch_adj_narrow(const unsigned long long* key, int64_t n, unsigned int* out,
                              hipStream_t st) {
   hipLaunchKernelGGL(k_adj_narrow, dim3(grid_for(n)), dim3(256), 0, st, key, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_flag_counts(const uint8_t* flag, int64_t V, unsigned int* cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_flag_counts, dim3(grid_for(V)), dim3(256), 0, st, flag, V, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_flag_ids(const uint8_t* flag, int64_t V, const unsigned long long* off, int* sid,
+                           int* slist, hipStream_t st) {
+  hipLaunchKernelGGL(k_flag_ids, dim3(grid_for(V)), dim3(256), 0, st, flag, V, off, sid, slist);
   return hipGetLastError();
 }
 

@@ -136,6 +136,11 @@ hipError_t launch_adj_mask_rows(int pass, const RayRec* rays, int64_t nrays, con
                                 unsigned long long* key, hipStream_t st);
 hipError_t launch_adj_narrow(const unsigned long long* key, int64_t n, unsigned int* out,
                              hipStream_t st);
+// voxel-order compaction of 0/1 flags: cnt[v] = flag[v], then (after an
+// exclusive scan into off) sid[v] = off[v] or -1, slist[off[v]] = v
+hipError_t launch_flag_counts(const uint8_t* flag, int64_t V, unsigned int* cnt, hipStream_t st);
+hipError_t launch_flag_ids(const uint8_t* flag, int64_t V, const unsigned long long* off, int* sid,
+                           int* slist, hipStream_t st);
 hipError_t launch_adj_src_init(int nS, const int* slist, const double* t, const unsigned int* cnt,
                                double* ts, double* h, double* w, double* cs, hipStream_t st);
 hipError_t launch_adj_src_iter(int nS, const unsigned long long* tptr, const unsigned int* tent,
