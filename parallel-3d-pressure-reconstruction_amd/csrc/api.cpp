@@ -193,7 +193,17 @@ int build_staged(omni3d_ctx* ctx, int nS, int D, int64_t nnz) {
 #endif  // OMNI3D_LAYOUT_STUDY
 
 // Tile engine part of the plan: compiled step codes and per-tile segments.
-int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
+// The source CSR built by the tile walks: the count walk counts the visits
+// landing on source voxels (and the diagonal), the fill walk writes their
+// keys (pl.rowptr / pl.diag are filled; key holds the unsorted rows).
+struct SrcRowBuild {
+  int nS = 0;
+  DevBuf rowcnt;  // counts, then the fill walk's cursors
+  DevBuf key;     // uint64[nnz + 1]
+  unsigned long long nnz = 0;
+};
+
+int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays, SrcRowBuild* rb) {
   Plan& pl = *ctx->plan;
   hipStream_t st = ctx->stream;
   TileGrid tg;
@@ -213,8 +223,21 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
   OMNI_TRY_HIP(ctx, gcnt.alloc(sizeof(unsigned int) * ngroups));
   OMNI_TRY_HIP(ctx, tmp.alloc(scan_u32_tmp_bytes(std::max<int64_t>(ngroups, nslots))));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
+  const int nS = rb->nS;
+  OMNI_TRY_HIP(ctx, rb->rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, pl.diag.alloc(sizeof(unsigned int) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(rb->rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.diag.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  SrcRows cnt_rows;
+  cnt_rows.sid = pl.sid.as<int>();
+  cnt_rows.rowcnt = rb->rowcnt.as<unsigned int>();
+  cnt_rows.diag = pl.diag.as<unsigned int>();
   OMNI_TRY_HIP(ctx, launch_tile_count(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
-                                      pl.nseg.as<unsigned int>(), tcnt.as<unsigned int>(), st));
+                                      pl.nseg.as<unsigned int>(), tcnt.as<unsigned int>(),
+                                      cnt_rows, st));
+  OMNI_TRY_HIP(ctx, pl.rowptr.alloc(sizeof(unsigned long long) * (nS + 1)));
+  OMNI_TRY_HIP(ctx, launch_scan_counts(rb->rowcnt.as<unsigned int>(), nS,
+                                       pl.rowptr.as<unsigned long long>(), st));
   // value slots: groups of 64 rays, segment-ordinal major inside a group
   OMNI_TRY_HIP(ctx, launch_seg_groups(nrays, pl.nseg.as<unsigned int>(), gcnt.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, pl.val_off.alloc(sizeof(unsigned long long) * (ngroups + 1)));
@@ -230,15 +253,24 @@ int build_tiles(omni3d_ctx* ctx, const GridDesc& g, int64_t nrays) {
                                    hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nval, pl.val_off.as<unsigned long long>() + ngroups, 8,
                                    hipMemcpyDeviceToHost, st));
+  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&rb->nnz, pl.rowptr.as<unsigned long long>() + nS, 8,
+                                   hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
   if (nsegs >= (1ull << 32) || nval >= (1ull << 32))
     return fail(ctx, OMNI3D_ERR_ARG, "too many ray segments");
   OMNI_TRY_HIP(ctx, pl.segs.alloc(sizeof(SegRec) * std::max<unsigned long long>(nsegs, 1)));
   OMNI_TRY_HIP(ctx, hipMemsetAsync(tcnt.as<void>(), 0, sizeof(unsigned int) * nslots, st));
+  OMNI_TRY_HIP(ctx, rb->key.alloc(sizeof(unsigned long long) * (rb->nnz + 1)));
+  OMNI_TRY_HIP(ctx, hipMemsetAsync(rb->rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
+  SrcRows fill_rows;
+  fill_rows.sid = pl.sid.as<int>();
+  fill_rows.rowptr = pl.rowptr.as<unsigned long long>();
+  fill_rows.cursor = rb->rowcnt.as<unsigned int>();
+  fill_rows.key = rb->key.as<unsigned long long>();
   OMNI_TRY_HIP(ctx, launch_tile_fill(pl.rays.as<RayRec>(), nrays, pl.dirs.as<DirConst>(), g, tg,
                                      pl.sid.as<int>(), pl.val_off.as<unsigned long long>(),
                                      pl.tile_ptr.as<unsigned long long>(), tcnt.as<unsigned int>(),
-                                     pl.segs.as<SegRec>(), st));
+                                     pl.segs.as<SegRec>(), fill_rows, st));
   {
     // every (tile, length bucket) list in value-slot order
     int key_bits = 1;
@@ -439,9 +471,13 @@ int build_plan(omni3d_ctx* ctx) {
     OMNI_TRY_HIP(ctx, hipMemcpy(pl.slist.as<void>(), hslist.data(), sizeof(int) * nS,
                                 hipMemcpyHostToDevice));
   }
-  // the tile engine's segments, then every voxel's visit count from them
+  // the tile engine's segments (and, by the same walks, the CSR of the
+  // visits landing on source voxels), then every voxel's visit count from
+  // the segments
+  SrcRowBuild rows;
+  rows.nS = nS;
   {
-    const int rc = build_tiles(ctx, g, (int64_t)nrays);
+    const int rc = build_tiles(ctx, g, (int64_t)nrays, &rows);
     if (rc) return rc;
   }
   OMNI_TRY_HIP(ctx, launch_tile_counts(pl.ntiles, pl.tile_ptr1.as<unsigned long long>(),
@@ -450,29 +486,10 @@ int build_plan(omni3d_ctx* ctx) {
   OMNI_TRY_HIP(ctx, launch_max_u32(pl.cnt.as<unsigned int>(), V, mx.as<unsigned int>(), st));
   OMNI_TRY_HIP(ctx, hipMemcpyAsync(mxh, mx.as<unsigned int>(), 8, hipMemcpyDeviceToHost, st));
   OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  // CSR of visits landing on source voxels
-  DevBuf rowcnt;
-  OMNI_TRY_HIP(ctx, rowcnt.alloc(sizeof(unsigned int) * (nS + 1)));
-  OMNI_TRY_HIP(ctx, pl.diag.alloc(sizeof(unsigned int) * (nS + 1)));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(pl.diag.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
-  OMNI_TRY_HIP(ctx, launch_plan_rowcount(pl.rays.as<RayRec>(), (int64_t)nrays,
-                                         pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
-                                         rowcnt.as<unsigned int>(), pl.diag.as<unsigned int>(), st));
-  OMNI_TRY_HIP(ctx, pl.rowptr.alloc(sizeof(unsigned long long) * (nS + 1)));
-  OMNI_TRY_HIP(ctx, launch_scan_counts(rowcnt.as<unsigned int>(), nS,
-                                       pl.rowptr.as<unsigned long long>(), st));
-  unsigned long long nnz = 0;
-  OMNI_TRY_HIP(ctx, hipMemcpyAsync(&nnz, pl.rowptr.as<unsigned long long>() + nS, 8,
-                                   hipMemcpyDeviceToHost, st));
-  OMNI_TRY_HIP(ctx, hipStreamSynchronize(st));
-  DevBuf key;
-  OMNI_TRY_HIP(ctx, key.alloc(sizeof(unsigned long long) * (nnz + 1)));
-  OMNI_TRY_HIP(ctx, hipMemsetAsync(rowcnt.as<void>(), 0, sizeof(unsigned int) * (nS + 1), st));
-  OMNI_TRY_HIP(ctx, launch_plan_rowfill(pl.rays.as<RayRec>(), (int64_t)nrays,
-                                        pl.dirs.as<DirConst>(), g, pl.sid.as<int>(),
-                                        pl.rowptr.as<unsigned long long>(),
-                                        rowcnt.as<unsigned int>(), key.as<unsigned long long>(), st));
+  // CSR of visits landing on source voxels (counted and filled by the
+  // tile walks): each row's keys sorted
+  const unsigned long long nnz = rows.nnz;
+  DevBuf& key = rows.key;
   OMNI_TRY_HIP(ctx, launch_sort_rows(nS, pl.rowptr.as<unsigned long long>(),
                                      key.as<unsigned long long>(), st));
   // the dense layout (default iteration) unless sizes exceed its word format:

@@ -42,12 +42,6 @@ hipError_t launch_face_slot_flags(const FaceSlots& fs, GridDesc g, const uint8_t
 hipError_t launch_face_slot_ids(const FaceSlots& fs, GridDesc g, const unsigned int* valid,
                                 const unsigned long long* off, int* sid, int* slist,
                                 hipStream_t st);
-hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                                GridDesc g, const int* sid, unsigned int* rowcnt,
-                                unsigned int* diag, hipStream_t st);
-hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                               GridDesc g, const int* sid, const unsigned long long* rowptr,
-                               unsigned int* cursor, unsigned long long* key, hipStream_t st);
 // field: float4 / double4 per voxel (a_x, a_y, a_z, 0); acc: ncopies accumulator
 // copies copy_stride apart (block b adds into copy b % ncopies).
 // trilinear: the straight-ray integral of the trilinear traversal instead of

@@ -38,16 +38,22 @@ __device__ __forceinline__ int len_slot(int tile, unsigned int nvis, int lshift)
 __global__ void __launch_bounds__(256)
 k_tile_count(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
              GridDesc g, TileGrid tg, unsigned int* __restrict__ nseg,
-             unsigned int* __restrict__ tile_cnt) {
+             unsigned int* __restrict__ tile_cnt, SrcRows rows) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   Walker w;
   w.init(rays[r], dirs, g);
+  // the entry visit reads the ray's own source: the row's diagonal
+  if (rows.rowcnt) atomicAdd(&rows.diag[rows.sid[w.v]], 1u);
   int tile = tile_of(w.c0, w.c1, w.c2, tg);
   unsigned int segs = 1, nvis = 1;
   while (!w.done()) {
     int s;
     if (w.advance(&s) < 0) break;
+    if (rows.rowcnt && w.on_surface()) {
+      const int q = rows.sid[w.v];
+      if (q >= 0) atomicAdd(&rows.rowcnt[q], 1u);
+    }
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
     if (t != tile) {
       atomicAdd(&tile_cnt[len_slot(tile, nvis, tg.lshift)], 1u);
@@ -66,12 +72,14 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
             GridDesc g, TileGrid tg, const int* __restrict__ sid,
             const unsigned long long* __restrict__ val_off,
             const unsigned long long* __restrict__ tile_ptr, unsigned int* __restrict__ cursor,
-            SegRec* __restrict__ segs) {
+            SegRec* __restrict__ segs, SrcRows rows) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrays) return;
   const RayRec rr = rays[r];
   Walker w;
   w.init(rr, dirs, g);
+  const unsigned long long rkey =
+      ((unsigned long long)(rr.dir_len & 0xffffu) << 32) | (unsigned long long)(uint32_t)sid[w.v];
   int ci[3], co[3];
   unpack_voxel(rr.in, ci);
   unpack_voxel(rr.out, co);
@@ -114,6 +122,10 @@ k_tile_fill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __re
     int s;
     const int a = w.advance(&s);
     if (a < 0) break;
+    if (rows.key && w.on_surface()) {
+      const int q = rows.sid[w.v];
+      if (q >= 0) rows.key[rows.rowptr[q] + atomicAdd(&rows.cursor[q], 1u)] = rkey;
+    }
     const int t = tile_of(w.c0, w.c1, w.c2, tg);
     // the step out of a tile ends the code list of the segment it leaves
     put_code(a);
@@ -754,18 +766,18 @@ k_scan_final(const unsigned int* __restrict__ cnt, int64_t n, int words,
 
 hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
                              TileGrid tg, unsigned int* nseg, unsigned int* tile_cnt,
-                             hipStream_t st) {
+                             const SrcRows& rows, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_count, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, tg, nseg, tile_cnt);
+                     dirs, g, tg, nseg, tile_cnt, rows);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
                             TileGrid tg, const int* sid, const unsigned long long* val_off,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
-                            SegRec* segs, hipStream_t st) {
+                            SegRec* segs, const SrcRows& rows, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_fill, dim3(grid_for(nrays, 1u << 30)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, tg, sid, val_off, tile_ptr, cursor, segs);
+                     dirs, g, tg, sid, val_off, tile_ptr, cursor, segs, rows);
   return hipGetLastError();
 }
 

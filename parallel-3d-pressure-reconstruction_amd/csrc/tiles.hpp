@@ -70,13 +70,26 @@ struct SegRec {
 };
 constexpr int kSegLvBits = 12;
 
+// The source rows (the visits that land on source voxels) built by the same
+// walks: the count walk adds each ray's entry visit to diag[sid] and every
+// later visit of a surface voxel with a source id to rowcnt[sid]; the fill
+// walk puts the ray's key (direction << 32 | its source id) at rowptr[s] +
+// cursor[s]++ for those visits.  Null pointers skip them.
+struct SrcRows {
+  const int* sid = nullptr;
+  unsigned int* rowcnt = nullptr;  // count walk
+  unsigned int* diag = nullptr;    // count walk
+  const unsigned long long* rowptr = nullptr;  // fill walk
+  unsigned int* cursor = nullptr;              // fill walk
+  unsigned long long* key = nullptr;           // fill walk
+};
 hipError_t launch_tile_count(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
                              TileGrid tg, unsigned int* nseg, unsigned int* tile_cnt,
-                             hipStream_t st);
+                             const SrcRows& rows, hipStream_t st);
 hipError_t launch_tile_fill(const RayRec* rays, int64_t nrays, const DirConst* dirs, GridDesc g,
                             TileGrid tg, const int* sid, const unsigned long long* val_off,
                             const unsigned long long* tile_ptr, unsigned int* cursor,
-                            SegRec* segs, hipStream_t st);
+                            SegRec* segs, const SrcRows& rows, hipStream_t st);
 // sorts every (tile, length bucket) list of segments by value slot into out
 // (slot_ptr: the nslots + 1 list offsets; key_bits: bits of the largest slot)
 size_t sort_segments_tmp_bytes(int64_t nsegs, int nslots);

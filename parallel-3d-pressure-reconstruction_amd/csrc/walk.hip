@@ -120,8 +120,9 @@ k_enum_write(const DirConst* __restrict__ dirs, GridDesc g, int64_t npts, int64_
 }
 
 // ---------------------------------------------------------------------------
-// Plan kernels over the rays: source voxels and step totals, the CSR of the
-// visits that land on source voxels (geometry-only walks).
+// Plan kernels over the rays: source voxels and step totals, and their
+// numbering (the CSR of the visits that land on source voxels is counted
+// and filled by the tile engine's plan walks, k_tile_count / k_tile_fill).
 
 // Entry voxels (the plain method's sources) and the total staircase steps
 // (the L1 length of every ray) straight from the ray records; the visit
@@ -268,67 +269,6 @@ hipError_t launch_face_slot_ids(const FaceSlots& fs, GridDesc g, const unsigned 
   return hipGetLastError();
 }
 
-// The entry visit of a plain ray reads its own source voxel: it is counted in
-// the per-row diagonal instead of the CSR.
-struct RowCountVisitor {
-  const int* sid;
-  unsigned int* rowcnt;
-  unsigned int* diag;
-  __device__ __forceinline__ void entry(int64_t v) { atomicAdd(&diag[sid[v]], 1u); }
-  __device__ __forceinline__ void visit(int64_t v, bool surface) {
-    if (!surface) return;
-    const int s = sid[v];
-    if (s >= 0) atomicAdd(&rowcnt[s], 1u);
-  }
-};
-
-__global__ void __launch_bounds__(256)
-k_plan_rowcount(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-                GridDesc g, const int* __restrict__ sid, unsigned int* __restrict__ rowcnt,
-                unsigned int* __restrict__ diag) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    RowCountVisitor vis{sid, rowcnt, diag};
-    walk_geometry(rays[r], dirs, g, vis);
-  }
-}
-
-// Each entry is the key (direction << 32) | source id, so that sorting a row
-// groups its entries by direction (the sliced-ELL layout relies on it).
-struct RowFillVisitor {
-  const int* sid;
-  const unsigned long long* rowptr;
-  unsigned int* cursor;
-  unsigned long long* key;
-  unsigned long long src_key;
-  __device__ __forceinline__ void put(int s) {
-    const unsigned int pos = atomicAdd(&cursor[s], 1u);
-    key[rowptr[s] + pos] = src_key;
-  }
-  __device__ __forceinline__ void entry(int64_t) {}
-  __device__ __forceinline__ void visit(int64_t v, bool surface) {
-    if (!surface) return;
-    const int s = sid[v];
-    if (s >= 0) put(s);
-  }
-};
-
-__global__ void __launch_bounds__(256)
-k_plan_rowfill(const RayRec* __restrict__ rays, int64_t nrays, const DirConst* __restrict__ dirs,
-               GridDesc g, const int* __restrict__ sid, const unsigned long long* __restrict__ rowptr,
-               unsigned int* __restrict__ cursor, unsigned long long* __restrict__ key) {
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrays;
-       r += (int64_t)gridDim.x * blockDim.x) {
-    int c[3];
-    unpack_voxel(rays[r].in, c);
-    const int64_t vin = (int64_t)c[0] + (int64_t)c[1] * g.X + (int64_t)c[2] * g.X * g.Y;
-    const unsigned long long k = ((unsigned long long)(rays[r].dir_len & 0xffffu) << 32) |
-                                 (unsigned long long)(unsigned int)sid[vin];
-    RowFillVisitor vis{sid, rowptr, cursor, key, k};
-    walk_geometry(rays[r], dirs, g, vis);
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Gradient walks of a reconstruction (plain method).
 //
@@ -384,21 +324,7 @@ hipError_t launch_enum_write(const DirConst* dirs, GridDesc g, int64_t npts, int
   return hipGetLastError();
 }
 
-hipError_t launch_plan_rowcount(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                                GridDesc g, const int* sid, unsigned int* rowcnt,
-                                unsigned int* diag, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan_rowcount, dim3(grid_for(nrays)), dim3(256), 0, st, rays, nrays,
-                     dirs, g, sid, rowcnt, diag);
-  return hipGetLastError();
-}
 
-hipError_t launch_plan_rowfill(const RayRec* rays, int64_t nrays, const DirConst* dirs,
-                               GridDesc g, const int* sid, const unsigned long long* rowptr,
-                               unsigned int* cursor, unsigned long long* key, hipStream_t st) {
-  hipLaunchKernelGGL(k_plan_rowfill, dim3(grid_for(nrays)), dim3(256), 0, st, rays, nrays, dirs,
-                     g, sid, rowptr, cursor, key);
-  return hipGetLastError();
-}
 
 template <typename T>
 hipError_t launch_walk_plain(bool full, bool trilinear, const RayRec* rays, int64_t nrays,
