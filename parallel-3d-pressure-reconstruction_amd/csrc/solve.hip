@@ -559,6 +559,17 @@ __device__ __forceinline__ void iterate_grp_body(
     double* __restrict__ pnew, double* __restrict__ srow, const float* __restrict__ din,
     float* __restrict__ dnew, DevScalars* __restrict__ sc, double* __restrict__ partials,
     int max_iter, double tol) {
+  // the first group's word range and rows ([grow[g], grow[g + 1]), at most
+  // kRows, chosen so that groups hold about equal words) are loaded with the
+  // stop flag: independent loads in flight together
+  unsigned int fw0 = 0, fw1 = 0;
+  int fr0 = 0, fr1 = 0;
+  if ((int)lb < ngr) {
+    fw0 = wptr[lb];
+    fw1 = wptr[lb + 1];
+    fr0 = grow[lb];
+    fr1 = grow[lb + 1];
+  }
   // the stop rule of iteration k - 1 runs in workgroup 0 of this launch: the
   // launch computes iteration k before knowing it (the buffers keep p_S of
   // k - 1), and launch k + 1 returns at once if k - 1 stopped
@@ -594,13 +605,10 @@ __device__ __forceinline__ void iterate_grp_body(
   double d2 = 0.0, n2 = 0.0;
   unsigned long long pm = 0ull;
   for (int g = (int)lb; g < ngr; g += (int)nb) {
-    s_acc[threadIdx.x] = 0ull;
-    __syncthreads();
-    // the group's rows: [grow[g], grow[g + 1]) (at most kRows, chosen so
-    // that groups hold about equal words), or kRows consecutive rows
-    const int rbeg = grow ? grow[g] : g * kRows;
-    const int nrows = grow ? grow[g + 1] - rbeg : min(kRows, nS - rbeg);
-    const unsigned long long e0 = wptr[g], e1 = wptr[g + 1];
+    const bool first = g == (int)lb;
+    const int rbeg = first ? fr0 : grow[g];
+    const int nrows = (first ? fr1 : grow[g + 1]) - rbeg;
+    const unsigned long long e0 = first ? fw0 : wptr[g], e1 = first ? fw1 : wptr[g + 1];
     const unsigned long long q1 = (e1 + 3) >> 2;
     unsigned long long base = (e0 & ~3ull) + (unsigned long long)wave * (kU * 64);
     int4 pre[kU / 4];
@@ -609,6 +617,10 @@ __device__ __forceinline__ void iterate_grp_body(
       const unsigned long long q = (base >> 2) + (unsigned long long)(j * 64 + lane);
       pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
     }
+    // the row sums are cleared while the first words are in flight (the
+    // previous group's rows were read before the barrier that ended it)
+    s_acc[threadIdx.x] = 0ull;
+    __syncthreads();
     for (; base < e1; base += (unsigned long long)kG * (kU * 64)) {
 #pragma unroll
       for (int j = 0; j < kU / 4; j++) reinterpret_cast<int4*>(se)[j * 64 + lane] = pre[j];
@@ -1046,8 +1058,9 @@ hipError_t launch_iterate_dense(int gchunks, int variant, int k, int nS,
                                 const int* grow, int ngroups) {
   unsigned int grid = dense_grid(nS);
   if (gchunks > 1) {
-    const int rows = 64 * gchunks;
-    const int ngr = grow ? ngroups : (nS + rows - 1) / rows;
+    // groups carry their first rows (grow[ngroups + 1])
+    if (!grow || ngroups < 1) return hipErrorInvalidValue;
+    const int ngr = ngroups;
     grid = grouped_grid_n(ngr);
     const int inc = srow ? (k == 1 ? 1 : 2) : 0;
 #define OMNI_GRP(G, I)                                                                         \

@@ -128,7 +128,7 @@ struct IterDesc {
   int nS, nb, first, seed;
   const unsigned int* wptr;
   const unsigned int* dent;
-  const int* grow;  // rows of each group (nullptr: 64 * chunks consecutive rows)
+  const int* grow;  // first row of each group (ngr + 1 entries)
   int ngr;          // groups
   const unsigned int* diag;
   const int* slist;
