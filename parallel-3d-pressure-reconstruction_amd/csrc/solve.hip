@@ -574,13 +574,6 @@ __device__ __forceinline__ void iterate_grp_body(
   // launch computes iteration k before knowing it (the buffers keep p_S of
   // k - 1), and launch k + 1 returns at once if k - 1 stopped
   if (sc->converged) return;
-  const double* __restrict__ prev_part = partials + (size_t)((k - 1) & 1) * 2 * nb;
-  partials += (size_t)(k & 1) * 2 * nb;
-  if (lb == 0) {
-    if (k > 1) finish_iteration<64 * kG>(k - 1, nb, prev_part, sc, max_iter, tol);
-    // the max slot launch k + 1 fills (launch k - 1 read it)
-    if (threadIdx.x == 0) sc->itmax_bits[(k + 1) % 3] = 0ull;
-  }
   constexpr int kU = 8;
   constexpr int kRows = 64 * kG;
   static_assert(kRows == 256 || kRows == 512, "merged words hold 8- or 9-bit rows");
@@ -595,6 +588,25 @@ __device__ __forceinline__ void iterate_grp_body(
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   unsigned int* __restrict__ se = s_ent[wave];
   const int4* __restrict__ d4 = reinterpret_cast<const int4*>(dent);
+  // the first group's first words, issued before the loads of the scale
+  // (and before workgroup 0's stop rule) so that their latencies overlap
+  int4 pre[kU / 4];
+  {
+    const unsigned long long q1 = ((unsigned long long)fw1 + 3) >> 2;
+    const unsigned long long base = (fw0 & ~3ull) + (unsigned long long)wave * (kU * 64);
+#pragma unroll
+    for (int j = 0; j < kU / 4; j++) {
+      const unsigned long long q = (base >> 2) + (unsigned long long)(j * 64 + lane);
+      pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
+    }
+  }
+  const double* __restrict__ prev_part = partials + (size_t)((k - 1) & 1) * 2 * nb;
+  partials += (size_t)(k & 1) * 2 * nb;
+  if (lb == 0) {
+    if (k > 1) finish_iteration<64 * kG>(k - 1, nb, prev_part, sc, max_iter, tol);
+    // the max slot launch k + 1 fills (launch k - 1 read it)
+    if (threadIdx.x == 0) sc->itmax_bits[(k + 1) % 3] = 0ull;
+  }
   const double inv = sc->inv_scale;
   const double B = __longlong_as_double((long long)sc->itmax_bits[(k - 1) % 3]);
   const bool finite = B <= DBL_MAX;
@@ -611,11 +623,12 @@ __device__ __forceinline__ void iterate_grp_body(
     const unsigned long long e0 = first ? fw0 : wptr[g], e1 = first ? fw1 : wptr[g + 1];
     const unsigned long long q1 = (e1 + 3) >> 2;
     unsigned long long base = (e0 & ~3ull) + (unsigned long long)wave * (kU * 64);
-    int4 pre[kU / 4];
+    if (!first) {
 #pragma unroll
-    for (int j = 0; j < kU / 4; j++) {
-      const unsigned long long q = (base >> 2) + (unsigned long long)(j * 64 + lane);
-      pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
+      for (int j = 0; j < kU / 4; j++) {
+        const unsigned long long q = (base >> 2) + (unsigned long long)(j * 64 + lane);
+        pre[j] = q < q1 ? d4[q] : int4{0, 0, 0, 0};
+      }
     }
     // the row sums are cleared while the first words are in flight (the
     // previous group's rows were read before the barrier that ended it)
