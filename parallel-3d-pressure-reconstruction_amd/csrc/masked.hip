@@ -1095,7 +1095,7 @@ k_tile_entries(const unsigned long long* __restrict__ tile_ptr, const SegRec* __
 // each nonzero sum into the row's global fixed-point sum (a row's total over
 // all its entries stays below 2^63: fewer than 4096 entries per row).
 constexpr int kChunkThreads = 1024;
-constexpr int kChunkUnroll = 4;
+constexpr int kChunkUnroll = 8;
 constexpr int kTileRowsMax = 4096;
 
 __device__ __forceinline__ bool masked_scale(const DevScalars* sc, int k,
