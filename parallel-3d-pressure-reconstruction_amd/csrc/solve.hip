@@ -533,10 +533,10 @@ __device__ void finish_iteration(int k, int nparts, const double* __restrict__ p
 }
 
 // The dense iteration over groups of kG chunks: a workgroup of kG
-// wavefronts owns a group (up to kG * 64 = 256 or 512 rows, the group's
-// words sorted by source, word (row in group << (32 - b)) | (multiplicity -
-// 1) << (30 - b) | source with b = 8 or 9 row bits, the group's words from
-// wptr[g]), its wavefronts take the
+// wavefronts owns a group (rows [grow[g], grow[g + 1]), at most kG * 64 =
+// 256 or 512, the group's words sorted by source, word (row in group << (32
+// - b)) | (multiplicity - 1) << (30 - b) | source with b = 8 or 9 row bits,
+// the group's words from wptr[g]), its wavefronts take the
 // group's rounds of 512 words in turn and add into the group's LDS row sums;
 // each wavefront then finishes the 64 rows of its chunk.  Sources shared by
 // neighbouring chunks touch the same 128-byte lines in one gather.

@@ -79,8 +79,6 @@ hipError_t launch_select_ps(int nS, const double* b0, const double* b1, const do
 inline unsigned int grouped_grid_n(int ngroups) {
   return (unsigned int)(ngroups < 1 ? 1 : (ngroups > 65536 ? 65536 : ngroups));
 }
-// workgroups of the grouped iteration over uniform groups of 256 rows
-inline unsigned int grouped_grid(int nS) { return grouped_grid_n((nS + 255) / 256); }
 // Row bits of a dense word for groups of gchunks chunks: (row << (32 - bits)) | source.
 constexpr int dense_row_bits(int gchunks) { return gchunks <= 4 ? 8 : (gchunks <= 8 ? 9 : 10); }
 // gchunks: chunks per group sharing the LDS row sums (1, 4, 8 or 16); the
