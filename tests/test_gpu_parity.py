@@ -175,3 +175,19 @@ def test_word_groups_of_four_and_eight_chunks_agree_bitwise(gpu_available, ref_d
     assert r4.sources == r8.sources > 65536
     np.testing.assert_array_equal(w4, w8)
     np.testing.assert_array_equal(p4, p8)
+
+
+@pytest.mark.parametrize("shape", [(2, 9, 7), (12, 2, 5), (11, 9, 3)])
+def test_thin_grids_match_oracle(gpu_available, ref_dirs2, shape):
+    """Grids two or three voxels thick, where voxels lie on several box faces
+    at once: the source numbering (a voxel belongs to the first of its faces),
+    the tiles' partial edges and the visit counts against the oracle (counts
+    bit-exact, FP64 rel-L2 <= 1e-12)."""
+    print(f"case thin grid {shape}", flush=True)
+    f = F.taylor_green(max(shape), noise=0.05, seed=9, shape=shape)
+    p, w, rep = run_gpu(f, ref_dirs2, shape, max_iter=4)
+    po, co, it, rel = run_oracle(f, ref_dirs2, shape, max_iter=4)
+    assert rep.iterations == it == 4
+    np.testing.assert_array_equal(w, co)
+    assert w.max() > 0
+    assert F.rel_l2(p, po) < 1e-12
